@@ -1,0 +1,174 @@
+/*
+ * mdt_gpu.h -- C-ABI of libmdt_gpu.so, the MI355X (gfx950) implementation
+ * of md_tian2's force-and-propagation hot path: EMT energies/forces/density,
+ * the velocity-Verlet / Langevin+LDFA / Andersen / PILE propagators and the
+ * ring-polymer normal-mode step, batched over trajectories and beads.
+ *
+ * Conventions (chosen so a Fortran caller binds them with ISO_C_BINDING and
+ * passes its arrays unchanged):
+ *   - arrays are the reference's Fortran arrays in column-major order:
+ *     r(3,P,N) -> r[c + 3*(b + P*i)], dens(P,N) -> dens[b + P*i],
+ *     simbox(3,3) column-major, pes(ntypes,ntypes) column-major;
+ *   - type indices (idx) are 1-based as in the reference;
+ *   - logicals are passed as int32 (0 = .false.);
+ *   - every function returns 0 on success and a negative MDT_E* code on
+ *     error; mdt_last_error() returns the message of the calling thread's
+ *     last failure.
+ * A context or batch handle is used from one host thread at a time.
+ */
+#ifndef MDT_GPU_H
+#define MDT_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MDT_OK 0
+#define MDT_EINVAL -1   /* bad argument */
+#define MDT_EHIP -2     /* HIP runtime failure */
+#define MDT_ESTATE -3   /* call out of order (e.g. no EMT parameters) */
+#define MDT_ETRAJ -4    /* a trajectory flagged a failure (see mdt_batch_status) */
+#define MDT_EUNSUP -5   /* configuration outside what this library implements */
+
+/* pes ids (src/constants.f90:64-71) and propagator ids (:107-112) */
+#define MDT_PES_EMT 2003
+#define MDT_PROP_VERLET 1001
+#define MDT_PROP_LANGEVIN 1003
+#define MDT_PROP_ANDERSEN 1005
+#define MDT_PROP_PILE 1006
+
+/* energy/force flag of calc_force (src/constants.f90:114-115) */
+#define MDT_ENERGY_AND_FORCE 3001
+#define MDT_ENERGY_ONLY 3002
+
+/* precision modes */
+#define MDT_PREC_FP64 0  /* everything in FP64 */
+
+/* per-trajectory status bits (mdt_batch_status) */
+#define MDT_ST_NBR_OVERFLOW 1  /* neighbour list capacity exceeded */
+#define MDT_ST_NEG_DENSITY 2   /* ldfa saw a negative density (src/md_algo.f90:551-554) */
+#define MDT_ST_BAD_PROP 4      /* unknown propagator (src/md_algo.f90:66-67) */
+
+typedef struct mdt_context mdt_context;
+typedef struct mdt_batch mdt_batch;
+
+/* Message of the last failed call on this thread; returns its length. */
+int mdt_last_error(char *buf, int buflen);
+
+/* Library version (major*10000 + minor*100 + patch). */
+int mdt_version(void);
+
+/* Number of visible GPUs (0 when none). */
+int mdt_device_count(int *count);
+
+/* Context on GPU `device`: owns a HIP stream and the EMT parameter table. */
+int mdt_create(int device, mdt_context **out);
+int mdt_destroy(mdt_context *ctx);
+
+/*
+ * EMT parameters per type and the interaction matrix.
+ * Replaces read_emt (src/pes_emt_mod.f90:49-190), which fills the module
+ * arrays eta2, n0, e0, lambda, v0, kappa, s0 (:32-41) per type and sets
+ * pes(idx1,idx2) = pes(idx2,idx1) = 2003 (:111-112).  The number of types with
+ * pes(t,t) == 2003 selects compute_emt_1species or compute_emt_2species
+ * (src/pes_emt_mod.f90:194-225); other counts return MDT_EUNSUP.
+ */
+int mdt_emt_params_set(mdt_context *ctx, int ntypes, const double *eta2, const double *n0,
+                       const double *e0, const double *lambda, const double *v0,
+                       const double *kappa, const double *s0, const int32_t *pes);
+
+/*
+ * Drop-in for compute_emt(atoms, flag) (src/pes_emt_mod.f90:194) as called
+ * from calc_force (src/force.f90:79) on one universe:
+ *   epot(b) += E_b (src/pes_emt_mod.f90:976), f(:,b,i) += F (:1001-1011),
+ *   dens(b,i) = embedded density of every EMT atom (:855,861).
+ * idx(N) 1-based types; simbox/isimbox the universe's cell and its inverse;
+ * r(3,P,N) positions.  `flag` is accepted and ignored, as in the reference.
+ * Minimum-image distances are computed on the GPU (no distance matrix).
+ */
+int mdt_compute_emt(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx,
+                    const double *simbox, const double *isimbox, const double *r, double *epot,
+                    double *f, double *dens, int32_t flag);
+
+/*
+ * Batch of `ntraj` independent trajectories resident on the GPU.
+ * natoms[t], nbeads[t] may differ between trajectories.
+ * Replaces the trajectory loop of src/md_tian2.f90:66-115.
+ */
+int mdt_batch_create(mdt_context *ctx, int ntraj, const int32_t *natoms, const int32_t *nbeads,
+                     mdt_batch **out);
+int mdt_batch_destroy(mdt_batch *b);
+
+/*
+ * Static per-atom data, concatenated over trajectories in batch order:
+ * idx (sum N, 1-based), mass (sum N, program units eV fs^2/A^2),
+ * is_fixed (sum 3*P*N, as universe%is_fixed(3,P,N)).
+ */
+int mdt_batch_set_atoms(mdt_batch *b, const int32_t *idx, const double *mass,
+                        const int32_t *is_fixed);
+
+/* Cells: simbox and isimbox, 9 doubles each per trajectory (column-major). */
+int mdt_batch_set_cells(mdt_batch *b, const double *simbox, const double *isimbox);
+
+/*
+ * Propagation setup (src/run_config.f90 simparams and universe%algo/is_proj):
+ * algo(ntypes), is_proj(ntypes) per type; dt [fs]; Tsurf, Tproj [K] per
+ * trajectory; pile_tau and andersen_time [fs] (defaults 200 and 30 in the
+ * reference, src/run_config.f90:119-120).
+ */
+int mdt_batch_set_propagation(mdt_batch *b, int ntypes, const int32_t *algo,
+                              const int32_t *is_proj, double dt, const double *tsurf,
+                              const double *tproj, double pile_tau, double andersen_time);
+
+/*
+ * Noise: Philox4x32-10 keyed by (seed, traj_id[t]); one draw per (trajectory,
+ * step, atom, bead, component).  traj_id may be NULL (ids 0..ntraj-1).
+ */
+int mdt_batch_seed(mdt_batch *b, uint32_t seed, const uint32_t *traj_id);
+
+/* Positions and velocities, sum 3*P*N each, reference (3,P,N) layout per trajectory. */
+int mdt_batch_set_state(mdt_batch *b, const double *r, const double *v);
+
+/*
+ * Copy state back: any pointer may be NULL.  r, v, f, a: sum 3*P*N;
+ * epot: sum P; dens: sum P*N (the reference's dens module array: raw EMT
+ * density after calc_force, LDFA friction for Langevin atoms after a step).
+ */
+int mdt_batch_get_state(mdt_batch *b, double *r, double *v, double *f, double *a, double *epot,
+                        double *dens);
+
+/*
+ * calc_force(atoms, energy_and_force) on every trajectory
+ * (src/force.f90:40-98, as called at src/md_tian2.f90:69), including the
+ * ring-polymer free step when P > 1 (src/force.f90:54).
+ */
+int mdt_batch_compute_forces(mdt_batch *b);
+
+/*
+ * `nsteps` MD steps of propagate_1 / calc_force / propagate_2
+ * (src/md_tian2.f90:81-83) on every trajectory.  Step numbers continue from
+ * the previous call (first step is 1), which keys the noise.
+ */
+int mdt_batch_step(mdt_batch *b, int nsteps);
+
+/* Per-trajectory status bits (sum over all calls since the last clear). */
+int mdt_batch_status(mdt_batch *b, int32_t *status, int clear);
+
+/*
+ * Kernel timing of the last mdt_batch_step call, measured with HIP events on
+ * the batch's stream: total milliseconds and number of step launches.
+ */
+int mdt_batch_last_timing(mdt_batch *b, double *ms_total, int *nlaunch);
+
+/*
+ * Launch tuning: threads per atom (1, 2, 4 or 8; 0 = autotune) and
+ * neighbour-list capacity per atom (0 = default).  Returns the choice made.
+ */
+int mdt_batch_tune(mdt_batch *b, int tpa, int nbcap, int *tpa_chosen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDT_GPU_H */

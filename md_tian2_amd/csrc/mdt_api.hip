@@ -1,0 +1,769 @@
+// mdt_api.hip -- host side of libmdt_gpu.so: the C-ABI declared in
+// include/mdt_gpu.h, EMT constant derivation, batch buffers, launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/mdt_gpu.h"
+#include "mdt_device.h"
+
+size_t mdt_replica_smem(int N, int cap, bool nbr_lds);
+hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst &ec,
+                              const PropConst &pc, const BatchView &bv, uint32_t step, int mode,
+                              int add_fin, hipStream_t st);
+hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst &pc, const BatchView &bv,
+                           const double *cjk, const long long *cjk_off, uint32_t step_prev,
+                           uint32_t step, int flags, hipStream_t st);
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(MDT_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+template <typename T>
+int dalloc(T **p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) return fail(MDT_EHIP, "hipMalloc(%zu bytes): %s", n * sizeof(T), hipGetErrorString(e));
+    return 0;
+}
+
+const double kBeta = std::pow(16.0 * M_PI / 3.0, 1.0 / 3.0) * (1.0 / std::sqrt(2.0));
+const double kNneighs[3] = {12.0, 6.0, 24.0};
+
+}  // namespace
+
+struct mdt_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool have_params = false;
+    int ntypes = 0;
+    EmtConst ec{};
+    std::vector<int32_t> pes;
+    mdt_batch *scratch = nullptr;  // cached single-universe batch for mdt_compute_emt
+    int scratch_N = -1, scratch_P = -1;
+};
+
+struct mdt_batch {
+    mdt_context *ctx = nullptr;
+    int ntraj = 0, nrep = 0;
+    std::vector<int> N, P;
+    std::vector<long long> soff, aoff, eoff;
+    long long tot3pn = 0, totpn = 0, totn = 0, totp = 0;
+    int maxN = 0, maxP = 0;
+    bool allP1 = true;
+    // device buffers
+    int *d_rep_traj = nullptr, *d_rep_bead = nullptr, *d_N = nullptr, *d_P = nullptr;
+    long long *d_soff = nullptr, *d_aoff = nullptr, *d_eoff = nullptr, *d_cjkoff = nullptr;
+    uint8_t *d_type = nullptr, *d_fixed = nullptr;
+    double *d_mass = nullptr, *d_cell = nullptr, *d_icell = nullptr, *d_tsurf = nullptr, *d_tproj = nullptr;
+    uint32_t *d_tid = nullptr;
+    double *d_x = nullptr, *d_v = nullptr, *d_a = nullptr, *d_f = nullptr, *d_dens = nullptr, *d_epot = nullptr;
+    double *d_cjk = nullptr;
+    int *d_status = nullptr;
+    uint16_t *d_nbr = nullptr;  // global neighbour lists when they do not fit in LDS
+    long long nbr_stride = 0;
+    size_t lds_limit = 0;
+    PropConst pc{};
+    bool have_atoms = false, have_cells = false, have_prop = false, have_state = false;
+    uint32_t step = 0;
+    int tpa = 0, nbcap = 0, block = 0;
+    size_t smem = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int last_nlaunch = 0;
+
+    BatchView view() const {
+        BatchView v{};
+        v.ntraj = ntraj;
+        v.nrep = nrep;
+        v.nbcap = nbcap;
+        v.rep_traj = d_rep_traj;
+        v.rep_bead = d_rep_bead;
+        v.traj_N = d_N;
+        v.traj_P = d_P;
+        v.traj_soff = d_soff;
+        v.traj_aoff = d_aoff;
+        v.type = d_type;
+        v.mass = d_mass;
+        v.fixed = d_fixed;
+        v.cell = d_cell;
+        v.icell = d_icell;
+        v.tsurf = d_tsurf;
+        v.tproj = d_tproj;
+        v.traj_id = d_tid;
+        v.x = d_x;
+        v.v = d_v;
+        v.a = d_a;
+        v.f = d_f;
+        v.dens = d_dens;
+        v.epot = d_epot;
+        v.traj_eoff = d_eoff;
+        v.status = d_status;
+        v.nbr_global = d_nbr;
+        v.nbr_stride = nbr_stride;
+        return v;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// EMT constants: src/pes_emt_mod.f90:533-592 (2 species), :267-307 (1 species)
+// ---------------------------------------------------------------------------
+static int build_emt_const(int ntypes, const double *eta2, const double *n0, const double *e0,
+                           const double *lambda, const double *v0, const double *kappa,
+                           const double *s0, const int32_t *pes, EmtConst &ec) {
+    if (ntypes < 1 || ntypes > MDT_MAXT)
+        return fail(MDT_EUNSUP, "ntypes=%d outside 1..%d", ntypes, MDT_MAXT);
+    std::memset(&ec, 0, sizeof ec);
+    ec.ntypes = ntypes;
+    std::vector<int> emt;
+    for (int t = 0; t < ntypes; ++t)
+        if (pes[t + ntypes * t] == MDT_PES_EMT) emt.push_back(t);
+    if (emt.empty()) return fail(MDT_EINVAL, "no type has pes(t,t) == emt (pes_emt_mod.f90:218-220)");
+    if (emt.size() > 2)
+        return fail(MDT_EUNSUP, "%zu EMT species: compute_emt_nspecies is not implemented", emt.size());
+    for (int a = 0; a < ntypes; ++a)
+        for (int n = 0; n < ntypes; ++n) {
+            const bool on = pes[a + ntypes * n] == MDT_PES_EMT;
+            const bool ea = std::find(emt.begin(), emt.end(), a) != emt.end();
+            const bool en = std::find(emt.begin(), emt.end(), n) != emt.end();
+            if (on && !(ea && en))
+                return fail(MDT_EUNSUP, "EMT pair (%d,%d) involves a species without pes(t,t)=emt", a + 1,
+                            n + 1);
+            ec.pair_on[a * MDT_MAXT + n] = on ? 1 : 0;
+        }
+    const double sqrt2 = std::sqrt(2.0), sqrt3 = std::sqrt(3.0);
+    auto gammas = [&](int t, double rcut, double acut, double &ig1, double &ig2) {
+        const double bs0 = kBeta * s0[t];
+        const double kdb = kappa[t] / kBeta;
+        const double rnn[3] = {bs0, bs0 * sqrt2, bs0 * sqrt3};
+        double x[3], r3[3], s1 = 0.0, s2 = 0.0;
+        for (int k = 0; k < 3; ++k) x[k] = kNneighs[k] * MDT_TWELFTH / (1 + std::exp(acut * (rnn[k] - rcut)));
+        for (int k = 0; k < 3; ++k) r3[k] = rnn[k] - bs0;
+        for (int k = 0; k < 3; ++k) s1 += x[k] * std::exp(-eta2[t] * r3[k]);
+        for (int k = 0; k < 3; ++k) s2 += x[k] * std::exp(-kdb * r3[k]);
+        ig1 = 1.0 / s1;
+        ig2 = 1.0 / s2;
+    };
+    for (int t = 0; t < ntypes; ++t) {
+        ec.eta2[t] = eta2[t];
+        ec.betas0[t] = kBeta * s0[t];
+        ec.kappadbeta[t] = kappa[t] / kBeta;
+        ec.betaeta2[t] = kBeta * eta2[t];
+        ec.e0[t] = e0[t];
+        ec.lambda[t] = lambda[t];
+        ec.v0[t] = v0[t];
+        ec.kappa[t] = kappa[t];
+        ec.n0[t] = n0[t];
+        ec.densc[t] = 0.5 / MDT_BOHR2ANG - ec.betaeta2[t];
+        ec.owner[t] = std::find(emt.begin(), emt.end(), t) != emt.end() ? 1 : 0;
+    }
+    double rcut;
+    if (emt.size() == 1) {
+        rcut = ec.betas0[emt[0]] * sqrt3;  // :295
+    } else {
+        rcut = std::max(ec.betas0[emt[0]], ec.betas0[emt[1]]) * sqrt3;  // :574
+    }
+    const double rr = 4.0 * rcut / (sqrt3 + 2);
+    const double acut = 9.21034037197618 / (rr - rcut);
+    ec.rcut = rcut;
+    ec.acut = acut;
+    ec.cutr = 1.5 * rcut;
+    ec.cut2hi = ec.cutr * ec.cutr * (1.0 + 1e-12);
+    if (emt.size() == 1) {
+        const int e = emt[0];
+        double ig1, ig2;
+        gammas(e, rcut, acut, ig1, ig2);
+        ec.csig[e * MDT_MAXT + e] = ig1;
+        ec.etad[e * MDT_MAXT + e] = eta2[e];
+        ec.wv[e * MDT_MAXT + e] = ig2 * v0[e];
+    } else {
+        const int t1 = emt[0], t2 = emt[1];
+        double ig1_1, ig2_1, ig1_2, ig2_2;
+        gammas(t1, rcut, acut, ig1_1, ig2_1);
+        gammas(t2, rcut, acut, ig1_2, ig2_2);
+        const double chi_21 = n0[t1] / n0[t2] * std::exp(0.5 / MDT_BOHR2ANG * (s0[t2] - s0[t1]));
+        const double chi_12 = 1.0 / chi_21;
+        ec.csig[t1 * MDT_MAXT + t1] = ig1_1;
+        ec.csig[t1 * MDT_MAXT + t2] = chi_12 * ig1_1;
+        ec.csig[t2 * MDT_MAXT + t2] = ig1_2;
+        ec.csig[t2 * MDT_MAXT + t1] = chi_21 * ig1_2;
+        // the type1-type1 sigma derivative uses eta2(type2) (pes_emt_mod.f90:705)
+        ec.etad[t1 * MDT_MAXT + t1] = eta2[t2];
+        ec.etad[t1 * MDT_MAXT + t2] = eta2[t2];
+        ec.etad[t2 * MDT_MAXT + t2] = eta2[t2];
+        ec.etad[t2 * MDT_MAXT + t1] = eta2[t1];
+        ec.wv[t1 * MDT_MAXT + t1] = ig2_1 * v0[t1];
+        ec.wv[t2 * MDT_MAXT + t2] = ig2_2 * v0[t2];
+        ec.wv[t2 * MDT_MAXT + t1] = ig2_2 * v0[t2] * chi_21;
+        ec.wv[t1 * MDT_MAXT + t2] = ig2_1 * v0[t1] * chi_12;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int mdt_last_error(char *buf, int buflen) {
+    if (buf && buflen > 0) {
+        std::strncpy(buf, g_err.c_str(), (size_t)buflen - 1);
+        buf[buflen - 1] = 0;
+    }
+    return (int)g_err.size();
+}
+
+int mdt_version(void) { return 100; }
+
+int mdt_device_count(int *count) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) n = 0;
+    if (count) *count = n;
+    return 0;
+}
+
+int mdt_create(int device, mdt_context **out) {
+    if (!out) return fail(MDT_EINVAL, "mdt_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(MDT_EHIP, "mdt_create: no GPU visible (the HIP path has no CPU fallback)");
+    if (device < 0 || device >= n) return fail(MDT_EINVAL, "mdt_create: device %d of %d", device, n);
+    HIPCHK(hipSetDevice(device));
+    mdt_context *c = new mdt_context();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MDT_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return 0;
+}
+
+int mdt_destroy(mdt_context *ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->scratch) mdt_batch_destroy(ctx->scratch);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+int mdt_emt_params_set(mdt_context *ctx, int ntypes, const double *eta2, const double *n0,
+                       const double *e0, const double *lambda, const double *v0,
+                       const double *kappa, const double *s0, const int32_t *pes) {
+    if (!ctx || !eta2 || !n0 || !e0 || !lambda || !v0 || !kappa || !s0 || !pes)
+        return fail(MDT_EINVAL, "mdt_emt_params_set: NULL argument");
+    EmtConst ec;
+    int rc = build_emt_const(ntypes, eta2, n0, e0, lambda, v0, kappa, s0, pes, ec);
+    if (rc) return rc;
+    ctx->ec = ec;
+    ctx->ntypes = ntypes;
+    ctx->pes.assign(pes, pes + ntypes * ntypes);
+    ctx->have_params = true;
+    return 0;
+}
+
+int mdt_batch_destroy(mdt_batch *b) {
+    if (!b) return 0;
+    (void)hipSetDevice(b->ctx->device);
+    void *ptrs[] = {b->d_rep_traj, b->d_rep_bead, b->d_N, b->d_P, b->d_soff, b->d_aoff, b->d_eoff,
+                    b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
+                    b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
+                    b->d_cjk, b->d_status, b->d_nbr};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    delete b;
+    return 0;
+}
+
+static void build_cjk_host(int P, double *cjk) {  // rpmd.f90:36-59
+    for (int j = 1; j <= P; ++j)
+        for (int k = 0; k <= P - 1; ++k) {
+            double v;
+            if (k == 0)
+                v = std::sqrt(1.0 / P);
+            else if (1 <= k && k <= P / 2 - 1)
+                v = std::sqrt(2.0 / P) * std::cos(2.0 * M_PI * j * k / P);
+            else if (k == P / 2)
+                v = std::sqrt(1.0 / P) * ((j % 2) ? -1.0 : 1.0);
+            else
+                v = std::sqrt(2.0 / P) * std::sin(2.0 * M_PI * j * k / P);
+            cjk[(j - 1) + P * k] = v;
+        }
+}
+
+int mdt_batch_create(mdt_context *ctx, int ntraj, const int32_t *natoms, const int32_t *nbeads,
+                     mdt_batch **out) {
+    if (!ctx || !natoms || !nbeads || !out || ntraj <= 0)
+        return fail(MDT_EINVAL, "mdt_batch_create: bad arguments");
+    *out = nullptr;
+    HIPCHK(hipSetDevice(ctx->device));
+    mdt_batch *b = new mdt_batch();
+    b->ctx = ctx;
+    b->ntraj = ntraj;
+    b->N.resize(ntraj);
+    b->P.resize(ntraj);
+    b->soff.resize(ntraj);
+    b->aoff.resize(ntraj);
+    b->eoff.resize(ntraj);
+    std::vector<int> rep_t, rep_b;
+    std::vector<long long> cjkoff(ntraj);
+    std::vector<double> cjk;
+    for (int t = 0; t < ntraj; ++t) {
+        const int N = natoms[t], P = nbeads[t];
+        if (N < 1 || N > 65535 || P < 1 || P > 64) {
+            delete b;
+            return fail(MDT_EUNSUP, "trajectory %d: natoms=%d nbeads=%d (supported 1..65535, 1..64)", t, N, P);
+        }
+        b->N[t] = N;
+        b->P[t] = P;
+        b->soff[t] = b->tot3pn;
+        b->aoff[t] = b->totn;
+        b->eoff[t] = b->totp;
+        b->tot3pn += 3LL * P * N;
+        b->totpn += (long long)P * N;
+        b->totn += N;
+        b->totp += P;
+        b->maxN = std::max(b->maxN, N);
+        b->maxP = std::max(b->maxP, P);
+        if (P != 1) b->allP1 = false;
+        for (int k = 0; k < P; ++k) {
+            rep_t.push_back(t);
+            rep_b.push_back(k);
+        }
+        cjkoff[t] = (long long)cjk.size();
+        std::vector<double> c(P * P);
+        build_cjk_host(P, c.data());
+        cjk.insert(cjk.end(), c.begin(), c.end());
+    }
+    b->nrep = (int)rep_t.size();
+    int rc = 0;
+    rc |= dalloc(&b->d_rep_traj, b->nrep);
+    rc |= dalloc(&b->d_rep_bead, b->nrep);
+    rc |= dalloc(&b->d_N, ntraj);
+    rc |= dalloc(&b->d_P, ntraj);
+    rc |= dalloc(&b->d_soff, ntraj);
+    rc |= dalloc(&b->d_aoff, ntraj);
+    rc |= dalloc(&b->d_eoff, ntraj);
+    rc |= dalloc(&b->d_cjkoff, ntraj);
+    rc |= dalloc(&b->d_type, b->totn);
+    rc |= dalloc(&b->d_mass, b->totn);
+    rc |= dalloc(&b->d_fixed, b->totpn);
+    rc |= dalloc(&b->d_cell, 9 * ntraj);
+    rc |= dalloc(&b->d_icell, 9 * ntraj);
+    rc |= dalloc(&b->d_tsurf, ntraj);
+    rc |= dalloc(&b->d_tproj, ntraj);
+    rc |= dalloc(&b->d_tid, ntraj);
+    rc |= dalloc(&b->d_x, b->tot3pn);
+    rc |= dalloc(&b->d_v, b->tot3pn);
+    rc |= dalloc(&b->d_a, b->tot3pn);
+    rc |= dalloc(&b->d_f, b->tot3pn);
+    rc |= dalloc(&b->d_dens, b->totpn);
+    rc |= dalloc(&b->d_epot, b->totp);
+    rc |= dalloc(&b->d_cjk, cjk.size());
+    rc |= dalloc(&b->d_status, ntraj);
+    if (rc) {
+        std::string msg = g_err;
+        mdt_batch_destroy(b);
+        return fail(MDT_EHIP, "%s", msg.c_str());
+    }
+    hipStream_t st = ctx->stream;
+    std::vector<uint32_t> tid(ntraj);
+    for (int t = 0; t < ntraj; ++t) tid[t] = (uint32_t)t;
+    HIPCHK(hipMemcpyAsync(b->d_rep_traj, rep_t.data(), sizeof(int) * b->nrep, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_rep_bead, rep_b.data(), sizeof(int) * b->nrep, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_N, b->N.data(), sizeof(int) * ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_P, b->P.data(), sizeof(int) * ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_soff, b->soff.data(), sizeof(long long) * ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_aoff, b->aoff.data(), sizeof(long long) * ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_eoff, b->eoff.data(), sizeof(long long) * ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_cjkoff, cjkoff.data(), sizeof(long long) * ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_cjk, cjk.data(), sizeof(double) * cjk.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_tid, tid.data(), sizeof(uint32_t) * ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(int) * ntraj, st));
+    HIPCHK(hipMemsetAsync(b->d_f, 0, sizeof(double) * b->tot3pn, st));
+    HIPCHK(hipMemsetAsync(b->d_a, 0, sizeof(double) * b->tot3pn, st));
+    HIPCHK(hipMemsetAsync(b->d_v, 0, sizeof(double) * b->tot3pn, st));
+    HIPCHK(hipMemsetAsync(b->d_dens, 0, sizeof(double) * b->totpn, st));
+    HIPCHK(hipMemsetAsync(b->d_epot, 0, sizeof(double) * b->totp, st));
+    HIPCHK(hipEventCreate(&b->ev0));
+    HIPCHK(hipEventCreate(&b->ev1));
+    HIPCHK(hipStreamSynchronize(st));
+    b->pc.seed = 0;
+    *out = b;
+    return 0;
+}
+
+int mdt_batch_set_atoms(mdt_batch *b, const int32_t *idx, const double *mass, const int32_t *is_fixed) {
+    if (!b || !idx || !mass || !is_fixed) return fail(MDT_EINVAL, "mdt_batch_set_atoms: NULL argument");
+    std::vector<uint8_t> ty(b->totn), fx(b->totpn);
+    for (long long k = 0; k < b->totn; ++k) {
+        if (idx[k] < 1 || idx[k] > MDT_MAXT) return fail(MDT_EINVAL, "idx[%lld]=%d outside 1..%d", k, idx[k], MDT_MAXT);
+        if (!(mass[k] > 0.0)) return fail(MDT_EINVAL, "mass[%lld]=%g must be positive", k, mass[k]);
+        ty[k] = (uint8_t)(idx[k] - 1);
+    }
+    long long q = 0;
+    for (int t = 0; t < b->ntraj; ++t) {
+        const int N = b->N[t], P = b->P[t];
+        const long long base = b->soff[t] / 3;
+        for (int i = 0; i < N; ++i)
+            for (int bb = 0; bb < P; ++bb) {
+                uint8_t bits = 0;
+                for (int c = 0; c < 3; ++c)
+                    if (is_fixed[q + c + 3 * (bb + (long long)P * i)]) bits |= (uint8_t)(1u << c);
+                fx[base + (long long)bb * N + i] = bits;
+            }
+        q += 3LL * P * N;
+    }
+    hipStream_t st = b->ctx->stream;
+    HIPCHK(hipMemcpyAsync(b->d_type, ty.data(), b->totn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_mass, mass, sizeof(double) * b->totn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_fixed, fx.data(), b->totpn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    b->have_atoms = true;
+    return 0;
+}
+
+int mdt_batch_set_cells(mdt_batch *b, const double *simbox, const double *isimbox) {
+    if (!b || !simbox || !isimbox) return fail(MDT_EINVAL, "mdt_batch_set_cells: NULL argument");
+    hipStream_t st = b->ctx->stream;
+    HIPCHK(hipMemcpyAsync(b->d_cell, simbox, sizeof(double) * 9 * b->ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_icell, isimbox, sizeof(double) * 9 * b->ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    b->have_cells = true;
+    return 0;
+}
+
+int mdt_batch_set_propagation(mdt_batch *b, int ntypes, const int32_t *algo, const int32_t *is_proj,
+                              double dt, const double *tsurf, const double *tproj, double pile_tau,
+                              double andersen_time) {
+    if (!b || !algo || !is_proj || !tsurf || !tproj) return fail(MDT_EINVAL, "mdt_batch_set_propagation: NULL argument");
+    if (ntypes < 1 || ntypes > MDT_MAXT) return fail(MDT_EUNSUP, "ntypes=%d", ntypes);
+    PropConst &pc = b->pc;
+    const uint32_t seed = pc.seed;
+    std::memset(&pc, 0, sizeof pc);
+    pc.seed = seed;
+    pc.dt = dt;
+    pc.pile_tau = pile_tau;
+    pc.andersen_time = andersen_time;
+    for (int t = 0; t < ntypes; ++t) {
+        const int a = algo[t];
+        if (a != MDT_PROP_VERLET && a != MDT_PROP_LANGEVIN && a != MDT_PROP_ANDERSEN && a != MDT_PROP_PILE)
+            return fail(MDT_EUNSUP, "type %d: propagator %d unknown (md_algo.f90:66-67)", t + 1, a);
+        pc.algo[t] = a;
+        pc.is_proj[t] = is_proj[t] ? 1 : 0;
+        if (a == MDT_PROP_LANGEVIN) pc.any_langevin = 1;
+        if (a == MDT_PROP_PILE) pc.any_pile = 1;
+        if (a == MDT_PROP_ANDERSEN) pc.any_andersen = 1;
+    }
+    hipStream_t st = b->ctx->stream;
+    HIPCHK(hipMemcpyAsync(b->d_tsurf, tsurf, sizeof(double) * b->ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_tproj, tproj, sizeof(double) * b->ntraj, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    b->have_prop = true;
+    return 0;
+}
+
+int mdt_batch_seed(mdt_batch *b, uint32_t seed, const uint32_t *traj_id) {
+    if (!b) return fail(MDT_EINVAL, "mdt_batch_seed: NULL batch");
+    b->pc.seed = seed;
+    std::vector<uint32_t> tid(b->ntraj);
+    for (int t = 0; t < b->ntraj; ++t) tid[t] = traj_id ? traj_id[t] : (uint32_t)t;
+    HIPCHK(hipMemcpy(b->d_tid, tid.data(), sizeof(uint32_t) * b->ntraj, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// reference (3,P,N) per trajectory <-> device [b][c][i]
+static void to_device_layout(const mdt_batch *b, const double *src, std::vector<double> &dst) {
+    dst.resize(b->tot3pn);
+    for (int t = 0; t < b->ntraj; ++t) {
+        const int N = b->N[t], P = b->P[t];
+        const long long o = b->soff[t];
+        for (int i = 0; i < N; ++i)
+            for (int bb = 0; bb < P; ++bb)
+                for (int c = 0; c < 3; ++c)
+                    dst[o + ((long long)bb * 3 + c) * N + i] = src[o + c + 3 * (bb + (long long)P * i)];
+    }
+}
+
+static void from_device_layout(const mdt_batch *b, const std::vector<double> &src, double *dst) {
+    for (int t = 0; t < b->ntraj; ++t) {
+        const int N = b->N[t], P = b->P[t];
+        const long long o = b->soff[t];
+        for (int i = 0; i < N; ++i)
+            for (int bb = 0; bb < P; ++bb)
+                for (int c = 0; c < 3; ++c)
+                    dst[o + c + 3 * (bb + (long long)P * i)] = src[o + ((long long)bb * 3 + c) * N + i];
+    }
+}
+
+int mdt_batch_set_state(mdt_batch *b, const double *r, const double *v) {
+    if (!b || !r) return fail(MDT_EINVAL, "mdt_batch_set_state: NULL argument");
+    std::vector<double> tmp;
+    hipStream_t st = b->ctx->stream;
+    to_device_layout(b, r, tmp);
+    HIPCHK(hipMemcpyAsync(b->d_x, tmp.data(), sizeof(double) * b->tot3pn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (v) {
+        to_device_layout(b, v, tmp);
+        HIPCHK(hipMemcpyAsync(b->d_v, tmp.data(), sizeof(double) * b->tot3pn, hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    b->have_state = true;
+    return 0;
+}
+
+int mdt_batch_get_state(mdt_batch *b, double *r, double *v, double *f, double *a, double *epot, double *dens) {
+    if (!b) return fail(MDT_EINVAL, "mdt_batch_get_state: NULL batch");
+    hipStream_t st = b->ctx->stream;
+    std::vector<double> tmp(b->tot3pn);
+    double *srcs[4] = {b->d_x, b->d_v, b->d_f, b->d_a};
+    double *dsts[4] = {r, v, f, a};
+    for (int k = 0; k < 4; ++k) {
+        if (!dsts[k]) continue;
+        HIPCHK(hipMemcpyAsync(tmp.data(), srcs[k], sizeof(double) * b->tot3pn, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        from_device_layout(b, tmp, dsts[k]);
+    }
+    if (epot) {
+        HIPCHK(hipMemcpyAsync(epot, b->d_epot, sizeof(double) * b->totp, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    if (dens) {
+        std::vector<double> d(b->totpn);
+        HIPCHK(hipMemcpyAsync(d.data(), b->d_dens, sizeof(double) * b->totpn, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        for (int t = 0; t < b->ntraj; ++t) {
+            const int N = b->N[t], P = b->P[t];
+            const long long o = b->soff[t] / 3;
+            for (int i = 0; i < N; ++i)
+                for (int bb = 0; bb < P; ++bb) dens[o + bb + (long long)P * i] = d[o + (long long)bb * N + i];
+        }
+    }
+    return 0;
+}
+
+static int ensure_ready(mdt_batch *b, bool need_prop) {
+    if (!b->ctx->have_params) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
+    if (!b->have_atoms) return fail(MDT_ESTATE, "no atoms (call mdt_batch_set_atoms)");
+    if (!b->have_cells) return fail(MDT_ESTATE, "no cells (call mdt_batch_set_cells)");
+    if (!b->have_state) return fail(MDT_ESTATE, "no positions (call mdt_batch_set_state)");
+    if (need_prop && !b->have_prop) return fail(MDT_ESTATE, "no propagation setup (call mdt_batch_set_propagation)");
+    if (!b->have_prop) {
+        // calc_force alone needs no propagator data: neutral defaults
+        std::vector<double> z(b->ntraj, 0.0);
+        HIPCHK(hipMemcpy(b->d_tsurf, z.data(), sizeof(double) * b->ntraj, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(b->d_tproj, z.data(), sizeof(double) * b->ntraj, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+static int configure(mdt_batch *b, int tpa, int nbcap) {
+    if (b->lds_limit == 0) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, b->ctx->device));
+        b->lds_limit = (size_t)prop.sharedMemPerBlock;
+    }
+    // capacity: every other atom for small replicas, else 168 (fcc within
+    // 1.5 rcut holds 134 neighbours; the margin covers thermal disorder)
+    int cap = nbcap > 0 ? nbcap : 168;
+    if (cap > b->maxN - 1) cap = std::max(1, b->maxN - 1);
+    const size_t smem_lds = mdt_replica_smem(b->maxN, cap, true);
+    const bool in_lds = smem_lds <= b->lds_limit;
+    const size_t smem = in_lds ? smem_lds : mdt_replica_smem(b->maxN, cap, false);
+    if (smem > b->lds_limit)
+        return fail(MDT_EUNSUP, "replica of %d atoms needs %zu bytes of LDS (limit %zu)", b->maxN, smem,
+                    b->lds_limit);
+    if (b->d_nbr) {
+        HIPCHK(hipFree(b->d_nbr));
+        b->d_nbr = nullptr;
+        b->nbr_stride = 0;
+    }
+    if (!in_lds) {
+        b->nbr_stride = (long long)b->maxN * cap;
+        int rc = dalloc(&b->d_nbr, (size_t)b->nbr_stride * b->nrep);
+        if (rc) return rc;
+    }
+    const int rounded = ((b->maxN * tpa + 63) / 64) * 64;
+    b->block = std::min(1024, std::max(64, rounded));
+    b->tpa = tpa;
+    b->nbcap = cap;
+    b->smem = smem;
+    return 0;
+}
+
+static int launch_force(mdt_batch *b) {
+    mdt_context *c = b->ctx;
+    const BatchView bv = b->view();
+    if (!b->allP1) {
+        const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
+        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 4, c->stream));
+    }
+    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->ec, b->pc, bv, b->step, 0,
+                              b->allP1 ? 0 : 1, c->stream));
+    return 0;
+}
+
+static int check_status(mdt_batch *b) {
+    std::vector<int> s(b->ntraj);
+    HIPCHK(hipMemcpy(s.data(), b->d_status, sizeof(int) * b->ntraj, hipMemcpyDeviceToHost));
+    for (int t = 0; t < b->ntraj; ++t)
+        if (s[t])
+            return fail(MDT_ETRAJ, "trajectory %d flagged status %d (1: neighbour overflow, 2: negative density, 4: bad propagator)",
+                        t, s[t]);
+    return 0;
+}
+
+static int autotune_if_needed(mdt_batch *b) {
+    if (b->tpa) return 0;
+    return configure(b, b->maxN <= 256 ? 4 : 2, b->nbcap);
+}
+
+int mdt_batch_compute_forces(mdt_batch *b) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    int rc = ensure_ready(b, false);
+    if (rc) return rc;
+    if ((rc = autotune_if_needed(b))) return rc;
+    if ((rc = launch_force(b))) return rc;
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    return check_status(b);
+}
+
+int mdt_batch_step(mdt_batch *b, int nsteps) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (nsteps < 0) return fail(MDT_EINVAL, "nsteps=%d", nsteps);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    int rc = ensure_ready(b, true);
+    if (rc) return rc;
+    if ((rc = autotune_if_needed(b))) return rc;
+    mdt_context *c = b->ctx;
+    const BatchView bv = b->view();
+    int nl = 0;
+    HIPCHK(hipEventRecord(b->ev0, c->stream));
+    if (b->allP1) {
+        for (int s = 0; s < nsteps; ++s) {
+            const uint32_t st = ++b->step;
+            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->ec, b->pc, bv, st, 1, 0, c->stream));
+            ++nl;
+        }
+    } else {
+        const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
+        for (int s = 0; s < nsteps; ++s) {
+            const uint32_t prev = b->step;
+            const uint32_t st = ++b->step;
+            const int flags = (s > 0 ? 1 : 0) | 2 | 4;
+            HIPCHK(mdt_launch_traj(b->ntraj, ts, b->pc, bv, b->d_cjk, b->d_cjkoff, prev, st, flags, c->stream));
+            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->ec, b->pc, bv, st, 0, 1, c->stream));
+            ++nl;
+        }
+        if (nsteps > 0)
+            HIPCHK(mdt_launch_traj(b->ntraj, ts, b->pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 1, c->stream));
+    }
+    HIPCHK(hipEventRecord(b->ev1, c->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    b->last_nlaunch = nl;
+    return check_status(b);
+}
+
+int mdt_batch_status(mdt_batch *b, int32_t *status, int clear) {
+    if (!b || !status) return fail(MDT_EINVAL, "mdt_batch_status: NULL argument");
+    HIPCHK(hipMemcpy(status, b->d_status, sizeof(int) * b->ntraj, hipMemcpyDeviceToHost));
+    if (clear) HIPCHK(hipMemset(b->d_status, 0, sizeof(int) * b->ntraj));
+    return 0;
+}
+
+int mdt_batch_last_timing(mdt_batch *b, double *ms_total, int *nlaunch) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (ms_total) *ms_total = b->last_ms;
+    if (nlaunch) *nlaunch = b->last_nlaunch;
+    return 0;
+}
+
+int mdt_batch_tune(mdt_batch *b, int tpa, int nbcap, int *tpa_chosen) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (tpa != 0 && tpa != 1 && tpa != 2 && tpa != 4 && tpa != 8)
+        return fail(MDT_EINVAL, "tpa=%d (1, 2, 4, 8 or 0)", tpa);
+    if (nbcap < 0 || nbcap > 65535) return fail(MDT_EINVAL, "nbcap=%d", nbcap);
+    b->nbcap = nbcap;
+    int rc = configure(b, tpa ? tpa : (b->maxN <= 256 ? 4 : 2), nbcap);
+    if (rc) return rc;
+    if (tpa_chosen) *tpa_chosen = b->tpa;
+    return 0;
+}
+
+int mdt_compute_emt(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx, const double *simbox,
+                    const double *isimbox, const double *r, double *epot, double *f, double *dens,
+                    int32_t flag) {
+    (void)flag;  // compute_emt ignores it (forces are always computed)
+    if (!ctx || !idx || !simbox || !isimbox || !r || !epot || !f || !dens)
+        return fail(MDT_EINVAL, "mdt_compute_emt: NULL argument");
+    if (!ctx->have_params) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
+    int rc;
+    if (!ctx->scratch || ctx->scratch_N != natoms || ctx->scratch_P != nbeads) {
+        if (ctx->scratch) mdt_batch_destroy(ctx->scratch);
+        ctx->scratch = nullptr;
+        int32_t n = natoms, p = nbeads;
+        if ((rc = mdt_batch_create(ctx, 1, &n, &p, &ctx->scratch))) return rc;
+        ctx->scratch_N = natoms;
+        ctx->scratch_P = nbeads;
+    }
+    mdt_batch *b = ctx->scratch;
+    std::vector<double> mass(natoms, 1.0);
+    std::vector<int32_t> fixed(3LL * natoms * nbeads, 0);
+    if ((rc = mdt_batch_set_atoms(b, idx, mass.data(), fixed.data()))) return rc;
+    if ((rc = mdt_batch_set_cells(b, simbox, isimbox))) return rc;
+    if ((rc = mdt_batch_set_state(b, r, nullptr))) return rc;
+    // compute_emt only: one force launch per replica, no ring-polymer step
+    HIPCHK(hipSetDevice(ctx->device));
+    if ((rc = ensure_ready(b, false))) return rc;
+    if ((rc = autotune_if_needed(b))) return rc;
+    HIPCHK(hipMemsetAsync(b->d_f, 0, sizeof(double) * b->tot3pn, ctx->stream));
+    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, ctx->ec, b->pc, b->view(), 0, 0, 0, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if ((rc = check_status(b))) return rc;
+    const long long n3 = 3LL * natoms * nbeads;
+    std::vector<double> ff(n3), ee(nbeads), dd((size_t)natoms * nbeads);
+    if ((rc = mdt_batch_get_state(b, nullptr, nullptr, ff.data(), nullptr, ee.data(), dd.data()))) return rc;
+    for (long long k = 0; k < n3; ++k) f[k] += ff[k];
+    for (int k = 0; k < nbeads; ++k) epot[k] += ee[k];
+    for (int i = 0; i < natoms; ++i)
+        if (ctx->ec.owner[idx[i] - 1])
+            for (int k = 0; k < nbeads; ++k) dens[k + (long long)nbeads * i] = dd[k + (long long)nbeads * i];
+    return 0;
+}
+
+}  // extern "C"

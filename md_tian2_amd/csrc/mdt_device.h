@@ -1,0 +1,90 @@
+// mdt_device.h -- shared host/device definitions for the batched EMT /
+// propagator / RPMD kernels (gfx950).
+//
+// Device state layout in HBM (FP64, structure of arrays):
+//   per trajectory t with N_t atoms and P_t beads, a block of 3*P_t*N_t
+//   doubles at traj_soff[t]:  [bead b][component c][atom i]
+//   so one replica (t, b) is three contiguous N_t-arrays x, y, z.
+//   P*N arrays (dens, fixed bits) live at traj_soff[t]/3: [bead b][atom i].
+//   Per-atom arrays (type, mass) live at traj_aoff[t]: [atom i].
+#pragma once
+#include <stdint.h>
+
+#define MDT_MAXT 4
+
+// physical constants, identical to the reference's src/constants.f90:33-39,157-160
+#define MDT_PI 3.14159265358979323846
+#define MDT_KB 8.61733238496e-5
+#define MDT_HBAR 0.6582119514467406
+#define MDT_AMU2MASS 103.638239276
+#define MDT_BOHR2ANG 0.529177211
+#define MDT_TOLERANCE 1.0e-9
+#define MDT_TWELFTH (1.0 / 12.0)
+
+// propagator ids, src/constants.f90:107-112
+#define MDT_PROP_VERLET 1001
+#define MDT_PROP_LANGEVIN 1003
+#define MDT_PROP_ANDERSEN 1005
+#define MDT_PROP_PILE 1006
+
+// RNG streams (Philox counter word 2)
+#define MDT_STREAM_LANGEVIN 0u
+#define MDT_STREAM_PILE 1u
+#define MDT_STREAM_ANDERSEN_V 2u
+#define MDT_STREAM_ANDERSEN_U 3u
+
+// per-trajectory status bits
+#define MDT_ST_NBR_OVERFLOW 1
+#define MDT_ST_NEG_DENSITY 2
+#define MDT_ST_BAD_PROP 4
+
+// EMT constants derived on the host exactly as src/pes_emt_mod.f90:533-592
+// (2 species) or :267-307 (1 species), flattened into per-type and
+// per-(type a, neighbour type n) tables for a gather formulation.
+struct EmtConst {
+    int ntypes;
+    int pad_;
+    double rcut, acut, cutr;  // cutr = 1.5 * rcut (the reference's hard cut)
+    double cut2hi;            // neighbour-list bound on r^2 (cutr^2 widened by 1e-12 rel)
+    double eta2[MDT_MAXT], betas0[MDT_MAXT], kappadbeta[MDT_MAXT], betaeta2[MDT_MAXT];
+    double e0[MDT_MAXT], lambda[MDT_MAXT], v0[MDT_MAXT], kappa[MDT_MAXT], n0[MDT_MAXT];
+    double densc[MDT_MAXT];            // 0.5/bohr2ang - beta*eta2
+    double csig[MDT_MAXT * MDT_MAXT];  // sigma weight chi(ta,tn)/gamma1(ta)
+    double etad[MDT_MAXT * MDT_MAXT];  // eta in the sigma derivative (type1-type1 quirk)
+    double wv[MDT_MAXT * MDT_MAXT];    // pair energy weight V0(ta)chi(ta,tn)/gamma2(ta)
+    int owner[MDT_MAXT];               // pes(t,t) == emt
+    int pair_on[MDT_MAXT * MDT_MAXT];  // pes(ta,tn) == emt
+};
+
+struct PropConst {
+    double dt, pile_tau, andersen_time;
+    double com_factor_unused;
+    int algo[MDT_MAXT];
+    int is_proj[MDT_MAXT];
+    int any_langevin, any_pile, any_andersen, pad_;
+    uint32_t seed;
+    uint32_t pad2_;
+};
+
+struct BatchView {
+    int ntraj, nrep;
+    int nbcap;  // neighbour-list capacity per atom
+    int pad_;
+    const int *rep_traj, *rep_bead;
+    const int *traj_N, *traj_P;
+    const long long *traj_soff, *traj_aoff;
+    const uint8_t *type;    // per atom
+    const double *mass;     // per atom
+    const uint8_t *fixed;   // per atom-bead: bit c set = component c frozen
+    const double *cell;     // 9 per traj, column-major (simbox)
+    const double *icell;    // 9 per traj (isimbox)
+    const double *tsurf, *tproj;
+    const uint32_t *traj_id;
+    double *x, *v, *a, *f;  // 3PN blocks
+    double *dens;           // PN blocks
+    double *epot;           // per replica (P per traj, at traj_soff/(3N))
+    const long long *traj_eoff;  // offset of a trajectory's epot(1..P)
+    int *status;            // per traj
+    uint16_t *nbr_global;   // NULL: neighbour lists in LDS; else nrep * nbr_stride entries
+    long long nbr_stride;   // entries per replica slice (maxN * cap)
+};
