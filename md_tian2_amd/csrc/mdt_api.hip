@@ -14,12 +14,10 @@
 #include "mdt_device.h"
 
 size_t mdt_replica_smem(int N, int cap, bool nbr_lds);
-hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst &ec,
-                              const PropConst &pc, const BatchView &bv, uint32_t step, int mode,
-                              int add_fin, hipStream_t st);
-hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst &pc, const BatchView &bv,
-                           const double *cjk, const long long *cjk_off, uint32_t step_prev,
-                           uint32_t step, int flags, hipStream_t st);
+hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst *ec, const PropConst *pc,
+                              const BatchView &bv, uint32_t step, int mode, int add_fin, hipStream_t st);
+hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
+                           const long long *cjk_off, uint32_t step_prev, uint32_t step, int flags, hipStream_t st);
 
 namespace {
 
@@ -63,6 +61,7 @@ struct mdt_context {
     bool have_params = false;
     int ntypes = 0;
     EmtConst ec{};
+    EmtConst *d_ec = nullptr;  // device copy read by the kernels
     std::vector<int32_t> pes;
     mdt_batch *scratch = nullptr;  // cached single-universe batch for mdt_compute_emt
     int scratch_N = -1, scratch_P = -1;
@@ -89,9 +88,12 @@ struct mdt_batch {
     long long nbr_stride = 0;
     size_t lds_limit = 0;
     PropConst pc{};
+    PropConst *d_pc = nullptr;  // device copy read by the kernels
     bool have_atoms = false, have_cells = false, have_prop = false, have_state = false;
     uint32_t step = 0;
     int tpa = 0, nbcap = 0, block = 0;
+    int nbcap_req = 0;      // requested neighbour capacity (0 = default)
+    float tuned_ms = 0.f;   // force-kernel time of the autotuned choice
     size_t smem = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
@@ -271,6 +273,7 @@ int mdt_destroy(mdt_context *ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     if (ctx->scratch) mdt_batch_destroy(ctx->scratch);
+    if (ctx->d_ec) (void)hipFree(ctx->d_ec);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -284,6 +287,12 @@ int mdt_emt_params_set(mdt_context *ctx, int ntypes, const double *eta2, const d
     EmtConst ec;
     int rc = build_emt_const(ntypes, eta2, n0, e0, lambda, v0, kappa, s0, pes, ec);
     if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->d_ec) {
+        int rc2 = dalloc(&ctx->d_ec, 1);
+        if (rc2) return rc2;
+    }
+    HIPCHK(hipMemcpy(ctx->d_ec, &ec, sizeof ec, hipMemcpyHostToDevice));
     ctx->ec = ec;
     ctx->ntypes = ntypes;
     ctx->pes.assign(pes, pes + ntypes * ntypes);
@@ -297,7 +306,7 @@ int mdt_batch_destroy(mdt_batch *b) {
     void *ptrs[] = {b->d_rep_traj, b->d_rep_bead, b->d_N, b->d_P, b->d_soff, b->d_aoff, b->d_eoff,
                     b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
-                    b->d_cjk, b->d_status, b->d_nbr};
+                    b->d_cjk, b->d_status, b->d_nbr, b->d_pc};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
@@ -341,9 +350,9 @@ int mdt_batch_create(mdt_context *ctx, int ntraj, const int32_t *natoms, const i
     std::vector<double> cjk;
     for (int t = 0; t < ntraj; ++t) {
         const int N = natoms[t], P = nbeads[t];
-        if (N < 1 || N > 65535 || P < 1 || P > 64) {
+        if (N < 1 || N > 2048 || P < 1 || P > 64) {
             delete b;
-            return fail(MDT_EUNSUP, "trajectory %d: natoms=%d nbeads=%d (supported 1..65535, 1..64)", t, N, P);
+            return fail(MDT_EUNSUP, "trajectory %d: natoms=%d nbeads=%d (supported 1..2048, 1..64)", t, N, P);
         }
         b->N[t] = N;
         b->P[t] = P;
@@ -392,6 +401,7 @@ int mdt_batch_create(mdt_context *ctx, int ntraj, const int32_t *natoms, const i
     rc |= dalloc(&b->d_epot, b->totp);
     rc |= dalloc(&b->d_cjk, cjk.size());
     rc |= dalloc(&b->d_status, ntraj);
+    rc |= dalloc(&b->d_pc, 1);
     if (rc) {
         std::string msg = g_err;
         mdt_batch_destroy(b);
@@ -573,6 +583,11 @@ int mdt_batch_get_state(mdt_batch *b, double *r, double *v, double *f, double *a
     return 0;
 }
 
+static int push_pc(mdt_batch *b) {
+    HIPCHK(hipMemcpy(b->d_pc, &b->pc, sizeof(PropConst), hipMemcpyHostToDevice));
+    return 0;
+}
+
 static int ensure_ready(mdt_batch *b, bool need_prop) {
     if (!b->ctx->have_params) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
     if (!b->have_atoms) return fail(MDT_ESTATE, "no atoms (call mdt_batch_set_atoms)");
@@ -585,7 +600,7 @@ static int ensure_ready(mdt_batch *b, bool need_prop) {
         HIPCHK(hipMemcpy(b->d_tsurf, z.data(), sizeof(double) * b->ntraj, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b->d_tproj, z.data(), sizeof(double) * b->ntraj, hipMemcpyHostToDevice));
     }
-    return 0;
+    return push_pc(b);
 }
 
 static int configure(mdt_batch *b, int tpa, int nbcap) {
@@ -615,7 +630,7 @@ static int configure(mdt_batch *b, int tpa, int nbcap) {
         if (rc) return rc;
     }
     const int rounded = ((b->maxN * tpa + 63) / 64) * 64;
-    b->block = std::min(1024, std::max(64, rounded));
+    b->block = std::min(512, std::max(64, rounded));
     b->tpa = tpa;
     b->nbcap = cap;
     b->smem = smem;
@@ -627,9 +642,9 @@ static int launch_force(mdt_batch *b) {
     const BatchView bv = b->view();
     if (!b->allP1) {
         const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
-        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 4, c->stream));
+        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 4, c->stream));
     }
-    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->ec, b->pc, bv, b->step, 0,
+    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->step, 0,
                               b->allP1 ? 0 : 1, c->stream));
     return 0;
 }
@@ -644,9 +659,62 @@ static int check_status(mdt_batch *b) {
     return 0;
 }
 
+static int heuristic_tpa(int maxN) {
+    int tpa = 8;
+    while (tpa > 1 && maxN * tpa > 512) tpa >>= 1;
+    return tpa;
+}
+
+// Times the force kernel for every threads-per-atom choice that fits a
+// 512-thread block (3 launches each after one warm-up) and keeps the fastest.
+// f, a, epot and dens are saved and restored around the trial launches.
+static int autotune(mdt_batch *b) {
+    mdt_context *c = b->ctx;
+    hipStream_t st = c->stream;
+    double *sf = nullptr, *sa = nullptr, *se = nullptr, *sd = nullptr;
+    int rc = dalloc(&sf, b->tot3pn) | dalloc(&sa, b->tot3pn) | dalloc(&se, b->totp) | dalloc(&sd, b->totpn);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(sf, b->d_f, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(sa, b->d_a, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(se, b->d_epot, sizeof(double) * b->totp, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(sd, b->d_dens, sizeof(double) * b->totpn, hipMemcpyDeviceToDevice, st));
+    int best = heuristic_tpa(b->maxN);
+    float best_ms = 1e30f;
+    const int cands[4] = {1, 2, 4, 8};
+    for (int tpa : cands) {
+        if (b->maxN * tpa > 512 && tpa != 1) continue;
+        if ((rc = configure(b, tpa, b->nbcap_req))) return rc;
+        const BatchView bv = b->view();
+        HIPCHK(mdt_launch_replica(tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->step, 0, 0, st));
+        HIPCHK(hipEventRecord(b->ev0, st));
+        for (int k = 0; k < 3; ++k)
+            HIPCHK(mdt_launch_replica(tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->step, 0, 0, st));
+        HIPCHK(hipEventRecord(b->ev1, st));
+        HIPCHK(hipEventSynchronize(b->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = tpa;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(b->d_f, sf, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_a, sa, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_epot, se, sizeof(double) * b->totp, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(b->d_dens, sd, sizeof(double) * b->totpn, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(int) * b->ntraj, st));
+    (void)hipFree(sf);
+    (void)hipFree(sa);
+    (void)hipFree(se);
+    (void)hipFree(sd);
+    b->tuned_ms = best_ms / 3;
+    return configure(b, best, b->nbcap_req);
+}
+
 static int autotune_if_needed(mdt_batch *b) {
     if (b->tpa) return 0;
-    return configure(b, b->maxN <= 256 ? 4 : 2, b->nbcap);
+    return configure(b, heuristic_tpa(b->maxN), b->nbcap_req);
 }
 
 int mdt_batch_compute_forces(mdt_batch *b) {
@@ -674,7 +742,7 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
     if (b->allP1) {
         for (int s = 0; s < nsteps; ++s) {
             const uint32_t st = ++b->step;
-            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->ec, b->pc, bv, st, 1, 0, c->stream));
+            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, st, 1, 0, c->stream));
             ++nl;
         }
     } else {
@@ -683,12 +751,12 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
             const uint32_t prev = b->step;
             const uint32_t st = ++b->step;
             const int flags = (s > 0 ? 1 : 0) | 2 | 4;
-            HIPCHK(mdt_launch_traj(b->ntraj, ts, b->pc, bv, b->d_cjk, b->d_cjkoff, prev, st, flags, c->stream));
-            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->ec, b->pc, bv, st, 0, 1, c->stream));
+            HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, prev, st, flags, c->stream));
+            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, st, 0, 1, c->stream));
             ++nl;
         }
         if (nsteps > 0)
-            HIPCHK(mdt_launch_traj(b->ntraj, ts, b->pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 1, c->stream));
+            HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 1, c->stream));
     }
     HIPCHK(hipEventRecord(b->ev1, c->stream));
     HIPCHK(hipEventSynchronize(b->ev1));
@@ -718,8 +786,15 @@ int mdt_batch_tune(mdt_batch *b, int tpa, int nbcap, int *tpa_chosen) {
     if (tpa != 0 && tpa != 1 && tpa != 2 && tpa != 4 && tpa != 8)
         return fail(MDT_EINVAL, "tpa=%d (1, 2, 4, 8 or 0)", tpa);
     if (nbcap < 0 || nbcap > 65535) return fail(MDT_EINVAL, "nbcap=%d", nbcap);
-    b->nbcap = nbcap;
-    int rc = configure(b, tpa ? tpa : (b->maxN <= 256 ? 4 : 2), nbcap);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    b->nbcap_req = nbcap;
+    int rc;
+    if (tpa == 0) {
+        if ((rc = ensure_ready(b, false))) return rc;
+        rc = autotune(b);
+    } else {
+        rc = configure(b, tpa, nbcap);
+    }
     if (rc) return rc;
     if (tpa_chosen) *tpa_chosen = b->tpa;
     return 0;
@@ -752,7 +827,7 @@ int mdt_compute_emt(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx
     if ((rc = ensure_ready(b, false))) return rc;
     if ((rc = autotune_if_needed(b))) return rc;
     HIPCHK(hipMemsetAsync(b->d_f, 0, sizeof(double) * b->tot3pn, ctx->stream));
-    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, ctx->ec, b->pc, b->view(), 0, 0, 0, ctx->stream));
+    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, ctx->d_ec, b->d_pc, b->view(), 0, 0, 0, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if ((rc = check_status(b))) return rc;
     const long long n3 = 3LL * natoms * nbeads;

@@ -1,31 +1,46 @@
 // mdt_kernels.hip -- CDNA4 (gfx950) kernels for batched EMT forces and the
 // md_tian2 propagators.
 //
-// k_replica: one workgroup per replica (trajectory t, bead b).  The replica's
-//   positions are staged in LDS; TPA lanes per atom build a per-atom
-//   min-image neighbour list (ordered by neighbour index, compacted with a
-//   wavefront ballot), then two gather passes over it:
-//     pass A: sigma_a = sum_n csig(ta,tn) f_sigma_tn(r) and the pair (V)
-//             energy and force of atom a;
-//     per atom: s, dens, E_coh + vref, G_a = dE/dsigma_a;
-//     pass B: F_a -= sum_n (G_a c f'_tn + G_n c f'_ta) r_hat.
-//   Lane partial sums are combined with wavefront shuffles in a fixed order,
+// k_replica: one workgroup per replica (trajectory t, bead b).
+//   phase 0: stage the replica in LDS, folded into its cell, with fractional
+//            coordinates (and propagate_1 first when mode == STEP, P == 1);
+//   phase 1: TPA lanes per atom scan all atoms with the minimum-image
+//            convention and build an ordered per-atom neighbour list of
+//            (index | periodic-image code << 11), compacted with a wavefront
+//            ballot;
+//   phase 2 (pass A): sigma_a = sum_n csig(ta,tn) f_sigma_tn(r), the pair (V)
+//            energy and force of atom a, then s, dens, E_coh + vref and
+//            G_a = dE/dsigma_a per atom;
+//   phase 3 (pass B): F_a -= sum_n (G_a c f'_tn + G_n c f'_ta) r_hat,
+//            set_acceleration, and propagate_2 when mode == STEP.
+//   Lane partial sums are combined by wavefront shuffles in a fixed order,
 //   so results are bitwise reproducible (no atomics).
-//   mode MDT_MODE_FORCE: calc_force only (src/force.f90:40-98).
-//   mode MDT_MODE_STEP (P == 1): propagate_1, calc_force and propagate_2
-//   fused (src/md_tian2.f90:81-83) with state resident in LDS/registers.
 // k_traj: one workgroup per trajectory for P > 1: propagate_2 of the
 //   previous step, propagate_1, COM removal and the ring-polymer free step
 //   (src/md_algo.f90:37-106, src/rpmd.f90:309-452).
 #include <hip/hip_runtime.h>
 
 #include "mdt_device.h"
+#include "mdt_math.h"
 #include "mdt_philox.h"
 
 #define MDT_MODE_FORCE 0
 #define MDT_MODE_STEP 1
+#define MDT_NBR_IDX_BITS 11
+#define MDT_NBR_IDX_MASK 0x7FF
 
 namespace {
+
+// Tables every lane indexes by atom type: kept in LDS (divergent indices).
+struct LdsTables {
+    double eta2[MDT_MAXT], bs0[MDT_MAXT], kdb[MDT_MAXT], betaeta2[MDT_MAXT];
+    double e0[MDT_MAXT], lambda[MDT_MAXT], v0[MDT_MAXT], kappa[MDT_MAXT], n0[MDT_MAXT];
+    double densc[MDT_MAXT];
+    double csig[MDT_MAXT * MDT_MAXT], etad[MDT_MAXT * MDT_MAXT], wv[MDT_MAXT * MDT_MAXT];
+    double shift[27 * 3];  // cartesian lattice translation of each image code
+    double exptab[64];
+    int owner[MDT_MAXT], pair_on[MDT_MAXT * MDT_MAXT], algo[MDT_MAXT], is_proj[MDT_MAXT];
+};
 
 struct CellRegs {
     double h[9], ih[9];
@@ -37,19 +52,6 @@ __device__ __forceinline__ void load_cell(const BatchView &bv, int t, CellRegs &
         c.h[k] = bv.cell[9 * t + k];
         c.ih[k] = bv.icell[9 * t + k];
     }
-}
-
-// universe_mod.f90:421-426: direct coordinates, anint, back to cartesian
-__device__ __forceinline__ void minimg(const CellRegs &c, double &dx, double &dy, double &dz) {
-    double s0 = c.ih[0] * dx + c.ih[3] * dy + c.ih[6] * dz;
-    double s1 = c.ih[1] * dx + c.ih[4] * dy + c.ih[7] * dz;
-    double s2 = c.ih[2] * dx + c.ih[5] * dy + c.ih[8] * dz;
-    s0 -= round(s0);
-    s1 -= round(s1);
-    s2 -= round(s2);
-    dx = c.h[0] * s0 + c.h[3] * s1 + c.h[6] * s2;
-    dy = c.h[1] * s0 + c.h[4] * s1 + c.h[7] * s2;
-    dz = c.h[2] * s0 + c.h[5] * s1 + c.h[8] * s2;
 }
 
 template <int TPA>
@@ -89,9 +91,7 @@ __device__ __forceinline__ LanCoef lan_coef(double fric, double dt, double temp,
 }
 
 // the branch test of md_algo.f90:176 for one bead (1e-2 is a default-real literal)
-__device__ __forceinline__ bool lan_big(double fric, double dt) {
-    return fric * dt > (double)1e-2f;
-}
+__device__ __forceinline__ bool lan_big(double fric, double dt) { return fric * dt > (double)1e-2f; }
 
 // ldfa, src/md_algo.f90:506-570: friction [1/fs] from the EMT density
 __device__ __forceinline__ double ldfa(double dens, double mass, int *bad) {
@@ -117,63 +117,60 @@ __device__ __forceinline__ double ldfa(double dens, double mass, int *bad) {
     return fric * convert / MDT_HBAR / mass;
 }
 
-struct SmemLayout {
-    double *tab;  // per-type and pair tables
-    double *px, *py, *pz, *fx, *fy, *fz, *G, *E;
-    uint16_t *cnt, *nbr;
-    uint8_t *typ;
-};
-
-// table offsets inside tab
-#define TB_ETA2 0
-#define TB_BS0 (MDT_MAXT)
-#define TB_KDB (2 * MDT_MAXT)
-#define TB_CSIG (3 * MDT_MAXT)
-#define TB_ETAD (3 * MDT_MAXT + MDT_MAXT * MDT_MAXT)
-#define TB_WV (3 * MDT_MAXT + 2 * MDT_MAXT * MDT_MAXT)
-#define TB_SIZE (3 * MDT_MAXT + 3 * MDT_MAXT * MDT_MAXT)
-
-__device__ __forceinline__ SmemLayout carve(unsigned char *base, int N, int cap) {
-    SmemLayout s;
-    double *d = (double *)base;
-    s.tab = d;
-    d += TB_SIZE;
-    s.px = d; d += N;
-    s.py = d; d += N;
-    s.pz = d; d += N;
-    s.fx = d; d += N;
-    s.fy = d; d += N;
-    s.fz = d; d += N;
-    s.G = d; d += N;
-    s.E = d; d += N;
-    uint16_t *h = (uint16_t *)d;
-    s.cnt = h; h += N;
-    s.nbr = h; h += (size_t)N * cap;
-    s.typ = (uint8_t *)h;
-    return s;
+__device__ void load_tables(LdsTables &T, const EmtConst *__restrict__ ec, const PropConst *__restrict__ pc,
+                            const CellRegs &cell, int lane, int nthr) {
+    for (int k = lane; k < MDT_MAXT; k += nthr) {
+        T.eta2[k] = ec->eta2[k];
+        T.bs0[k] = ec->betas0[k];
+        T.kdb[k] = ec->kappadbeta[k];
+        T.betaeta2[k] = ec->betaeta2[k];
+        T.e0[k] = ec->e0[k];
+        T.lambda[k] = ec->lambda[k];
+        T.v0[k] = ec->v0[k];
+        T.kappa[k] = ec->kappa[k];
+        T.n0[k] = ec->n0[k];
+        T.densc[k] = ec->densc[k];
+        T.owner[k] = ec->owner[k];
+        T.algo[k] = pc->algo[k];
+        T.is_proj[k] = pc->is_proj[k];
+    }
+    for (int k = lane; k < MDT_MAXT * MDT_MAXT; k += nthr) {
+        T.csig[k] = ec->csig[k];
+        T.etad[k] = ec->etad[k];
+        T.wv[k] = ec->wv[k];
+        T.pair_on[k] = ec->pair_on[k];
+    }
+    for (int c = lane; c < 27; c += nthr) {
+        const double m0 = (double)(c % 3 - 1), m1 = (double)((c / 3) % 3 - 1), m2 = (double)(c / 9 - 1);
+        T.shift[3 * c + 0] = cell.h[0] * m0 + cell.h[3] * m1 + cell.h[6] * m2;
+        T.shift[3 * c + 1] = cell.h[1] * m0 + cell.h[4] * m1 + cell.h[7] * m2;
+        T.shift[3 * c + 2] = cell.h[2] * m0 + cell.h[5] * m1 + cell.h[8] * m2;
+    }
+    const double et[64] = MDT_EXP_TAB_INIT;
+    for (int k = lane; k < 64; k += nthr) T.exptab[k] = et[k];
 }
 
 }  // namespace
 
-// Propagate_1 for one atom-bead of a P == 1 trajectory (md_algo.f90:37-70),
-// positions and velocities in/out, noise drawn for this (traj, step, atom).
-__device__ __forceinline__ void prop1_p1(const PropConst &pc, int algo, bool isproj, double tsurf,
-                                         double tproj, double mass, uint8_t fb, double fric,
-                                         uint32_t tid, uint32_t step, int i, double r[3], double v[3],
-                                         const double acc[3]) {
-    const double dt = pc.dt;
+// propagate_1 for one atom of a P == 1 trajectory (md_algo.f90:37-70) on its
+// global state: x, v, a point at component 0 of the atom, components are N
+// apart.  Not inlined, so its registers do not add to the pair loops'.
+__device__ __noinline__ void prop1_p1(const PropConst *__restrict__ pcg, int algo, int isproj, double tsurf,
+                                      double tproj, double mass, int fb, double fric, uint32_t tid, uint32_t step,
+                                      int i, double *x, double *v, const double *a, int N) {
+    const double dt = pcg->dt;
     if (algo == MDT_PROP_LANGEVIN) {
         const double temp = MDT_KB * 1 * (isproj ? tproj : tsurf) / mass;
         const bool exact = lan_big(fric, dt) && temp > MDT_TOLERANCE;
         const LanCoef L = lan_coef(fric, dt, temp, exact);
-#pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double eta = mdt_normal(pc.seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)i, (uint32_t)c);
+            const double eta = mdt_normal(pcg->seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)i, (uint32_t)c);
+            const double r = x[c * N], vv = v[c * N], acc = a[c * N];
             if (!(fb >> c & 1)) {
-                r[c] = r[c] + L.c1 * v[c] + L.c2 * dt * acc[c] + L.sr * eta;
-                v[c] = L.c0 * v[c] + (L.c1 - L.c2) * acc[c] + L.sv * L.crv * eta;
+                x[c * N] = r + L.c1 * vv + L.c2 * dt * acc + L.sr * eta;
+                v[c * N] = L.c0 * vv + (L.c1 - L.c2) * acc + L.sv * L.crv * eta;
             } else {
-                v[c] = 0.0;
+                v[c * N] = 0.0;
             }
         }
         return;
@@ -181,50 +178,75 @@ __device__ __forceinline__ void prop1_p1(const PropConst &pc, int algo, bool isp
     if (algo == MDT_PROP_PILE) {
         // pile_thermostat with P == 1: C = [1], centroid gamma = 1/tau (md_algo.f90:440-502)
         const double betaN = 1.0 / (MDT_KB * (isproj ? tproj : tsurf) * 1);
-        const double g0 = 1 / pc.pile_tau;
+        const double g0 = 1 / pcg->pile_tau;
         const double c1 = exp(-0.5 * dt * g0);
         const double c2 = sqrt(1 - c1 * c1);
         const double sq = sqrt(mass / betaN);
-#pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double z = mdt_normal(pc.seed, tid, step, MDT_STREAM_PILE, (uint32_t)i, (uint32_t)c);
-            double np = 0.0 + (mass * v[c]) * 1.0;  // C^T p with C = sqrt(1/1)
+            const double z = mdt_normal(pcg->seed, tid, step, MDT_STREAM_PILE, (uint32_t)i, (uint32_t)c);
+            double np = 0.0 + (mass * v[c * N]) * 1.0;  // C^T p with C = sqrt(1/1)
             np = c1 * np + sq * c2 * z;
             double ap = 0.0;
             if (!(fb >> c & 1)) ap = ap + np * 1.0;
-            v[c] = ap / mass;
+            v[c * N] = ap / mass;
         }
     }
     // verlet_1 (md_algo.f90:110-128), also for andersen and after pile
-#pragma unroll
     for (int c = 0; c < 3; ++c) {
-        if (!(fb >> c & 1))
-            v[c] = v[c] + 0.5 * dt * acc[c];
-        else
-            v[c] = 0.0;
+        if (!(fb >> c & 1)) {
+            const double vn = v[c * N] + 0.5 * dt * a[c * N];
+            v[c * N] = vn;
+            x[c * N] = x[c * N] + dt * vn;
+        } else {
+            v[c * N] = 0.0;
+        }
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-        if (!(fb >> c & 1)) r[c] = r[c] + dt * v[c];
     if (algo == MDT_PROP_ANDERSEN) {
         // andersen (md_algo.f90:402-432)
         const double ibetaN = 1 * MDT_KB * (isproj ? tproj : tsurf);
         const double sigma = sqrt(ibetaN / mass);
-        const double u = mdt_uniform(pc.seed, tid, step, MDT_STREAM_ANDERSEN_U, (uint32_t)i, 0u);
-        if (u < dt / pc.andersen_time) {
-#pragma unroll
+        const double u = mdt_uniform(pcg->seed, tid, step, MDT_STREAM_ANDERSEN_U, (uint32_t)i, 0u);
+        if (u < dt / pcg->andersen_time) {
             for (int c = 0; c < 3; ++c) {
-                const double z = mdt_normal(pc.seed, tid, step, MDT_STREAM_ANDERSEN_V, (uint32_t)i, (uint32_t)c);
-                if (!(fb >> c & 1)) v[c] = z * sigma + 0.0;
+                const double z = mdt_normal(pcg->seed, tid, step, MDT_STREAM_ANDERSEN_V, (uint32_t)i, (uint32_t)c);
+                if (!(fb >> c & 1)) v[c * N] = z * sigma + 0.0;
             }
         }
     }
 }
 
-template <int TPA, bool NBR_LDS>
-__global__ void __launch_bounds__(1024) k_replica(EmtConst ec, PropConst pc, BatchView bv, uint32_t step,
-                                                  int mode, int add_fin) {
+// propagate_2 for one atom of a P == 1 trajectory (md_algo.f90:80-106) on its
+// global state; dens holds the raw EMT density on entry and the LDFA
+// friction on exit for Langevin atoms.  Returns status bits.
+__device__ __noinline__ int prop2_p1(const PropConst *__restrict__ pcg, int algo, int isproj, double tsurf,
+                                     double tproj, double mass, int fb, uint32_t tid, uint32_t step, int k,
+                                     double *v, const double *a, int N, double *dens) {
+    const double dt = pcg->dt;
+    int bad = 0;
+    if (algo == MDT_PROP_LANGEVIN) {
+        const double fric = ldfa(*dens, mass, &bad);
+        *dens = fric;
+        const double temp = MDT_KB * 1 * (isproj ? tproj : tsurf) / mass;
+        const bool exact = lan_big(fric, dt) && temp > MDT_TOLERANCE;
+        const LanCoef L = lan_coef(fric, dt, temp, exact);
+        const double w = L.sv * sqrt(1 - L.crv * L.crv);
+        for (int c = 0; c < 3; ++c) {
+            const double eta = mdt_normal(pcg->seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)k, (uint32_t)c);
+            v[c * N] = (fb >> c & 1) ? 0.0 : v[c * N] + L.c2 * a[c * N] + w * eta;
+        }
+    } else {
+        for (int c = 0; c < 3; ++c) v[c * N] = (fb >> c & 1) ? 0.0 : v[c * N] + 0.5 * dt * a[c * N];
+    }
+    return bad ? MDT_ST_NEG_DENSITY : 0;
+}
+
+template <int TPA, bool NBR_LDS, int mode>
+__global__ void __launch_bounds__(512, 4)
+    k_replica(const EmtConst *__restrict__ ecg, const PropConst *__restrict__ pcg, BatchView bv, uint32_t step,
+              int add_fin) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ LdsTables T;
+    __shared__ double red[4];
     const int rep = blockIdx.x;
     const int t = bv.rep_traj[rep];
     const int b = bv.rep_bead[rep];
@@ -236,74 +258,72 @@ __global__ void __launch_bounds__(1024) k_replica(EmtConst ec, PropConst pc, Bat
     const long long aoff = bv.traj_aoff[t];
     const uint32_t tid = bv.traj_id[t];
     const double tsurf = bv.tsurf[t], tproj = bv.tproj[t];
-    SmemLayout s = carve(smem_raw, N, NBR_LDS ? cap : 0);
-    // neighbour lists: LDS when they fit, else a per-replica slice of global scratch
-    uint16_t *nbr = NBR_LDS ? s.nbr : bv.nbr_global + (size_t)rep * bv.nbr_stride;
-    CellRegs cell;
-    load_cell(bv, t, cell);
     const int nthr = blockDim.x;
     const int lane = threadIdx.x;
+    CellRegs cell;
+    load_cell(bv, t, cell);
+    load_tables(T, ecg, pcg, cell, lane, nthr);
 
-    // tables to LDS
-    for (int k = lane; k < TB_SIZE; k += nthr) {
-        double v = 0.0;
-        if (k < TB_BS0) v = ec.eta2[k - TB_ETA2];
-        else if (k < TB_KDB) v = ec.betas0[k - TB_BS0];
-        else if (k < TB_CSIG) v = ec.kappadbeta[k - TB_KDB];
-        else if (k < TB_ETAD) v = ec.csig[k - TB_CSIG];
-        else if (k < TB_WV) v = ec.etad[k - TB_ETAD];
-        else v = ec.wv[k - TB_WV];
-        s.tab[k] = v;
-    }
+    // LDS: px py pz | (sx sy sz, later wx wy wz) | G | E | cnt | nbr | typ
+    double *px = (double *)smem_raw;
+    double *py = px + N, *pz = py + N;
+    double *sx = pz + N, *sy = sx + N, *sz = sy + N;
+    double *wx = sx, *wy = sy, *wz = sz;  // pass-A force partials reuse the scan's buffers
+    double *G = sz + N, *E = G + N;
+    uint16_t *cnt = (uint16_t *)(E + N);
+    uint16_t *nbr = NBR_LDS ? cnt + N : bv.nbr_global + (size_t)rep * bv.nbr_stride;
+    uint8_t *typ = (uint8_t *)(cnt + N + (NBR_LDS ? (size_t)N * cap : 0));
     int bad_local = 0;
+    const double dt = pcg->dt;
+    const bool any_com = pcg->any_andersen || pcg->any_pile;
+    __syncthreads();
 
-    // stage positions (after propagate_1 when fused)
+    // ---- phase 0: stage (after propagate_1 when fused) ----
     for (int i = lane; i < N; i += nthr) {
         const int ty = bv.type[aoff + i];
-        s.typ[i] = (uint8_t)ty;
-        double r[3] = {bv.x[roff + i], bv.x[roff + N + i], bv.x[roff + 2 * N + i]};
-        if (mode == MDT_MODE_STEP) {
-            double v[3] = {bv.v[roff + i], bv.v[roff + N + i], bv.v[roff + 2 * N + i]};
-            const double acc[3] = {bv.a[roff + i], bv.a[roff + N + i], bv.a[roff + 2 * N + i]};
-            const uint8_t fb = bv.fixed[doff + i];
-            prop1_p1(pc, pc.algo[ty], pc.is_proj[ty] != 0, tsurf, tproj, bv.mass[aoff + i], fb,
-                     bv.dens[doff + i], tid, step, i, r, v, acc);
-            bv.v[roff + i] = v[0];
-            bv.v[roff + N + i] = v[1];
-            bv.v[roff + 2 * N + i] = v[2];
-            bv.x[roff + i] = r[0];
-            bv.x[roff + N + i] = r[1];
-            bv.x[roff + 2 * N + i] = r[2];
-        }
-        s.px[i] = r[0];
-        s.py[i] = r[1];
-        s.pz[i] = r[2];
+        typ[i] = (uint8_t)ty;
+        if (mode == MDT_MODE_STEP)
+            prop1_p1(pcg, T.algo[ty], T.is_proj[ty], tsurf, tproj, bv.mass[aoff + i], bv.fixed[doff + i],
+                     bv.dens[doff + i], tid, step, i, bv.x + roff + i, bv.v + roff + i, bv.a + roff + i, N);
+        const double r[3] = {bv.x[roff + i], bv.x[roff + N + i], bv.x[roff + 2 * N + i]};
+        // fold into the cell: fractional in [0, 1)
+        double f0 = cell.ih[0] * r[0] + cell.ih[3] * r[1] + cell.ih[6] * r[2];
+        double f1 = cell.ih[1] * r[0] + cell.ih[4] * r[1] + cell.ih[7] * r[2];
+        double f2 = cell.ih[2] * r[0] + cell.ih[5] * r[1] + cell.ih[8] * r[2];
+        f0 -= floor(f0);
+        f1 -= floor(f1);
+        f2 -= floor(f2);
+        sx[i] = f0;
+        sy[i] = f1;
+        sz[i] = f2;
+        px[i] = cell.h[0] * f0 + cell.h[3] * f1 + cell.h[6] * f2;
+        py[i] = cell.h[1] * f0 + cell.h[4] * f1 + cell.h[7] * f2;
+        pz[i] = cell.h[2] * f0 + cell.h[5] * f1 + cell.h[8] * f2;
     }
     __syncthreads();
-    if (mode == MDT_MODE_STEP && (pc.any_andersen || pc.any_pile)) {
+    if (mode == MDT_MODE_STEP && any_com) {
         // remove_com_velocity (universe_mod.f90:222-241): 1000 subtractions of
         // S_k/(M P N) with S_k = S_0 (1-1/N)^k, summed in closed form.
-        __shared__ double red[4 * 64];
-        double sx = 0, sy = 0, sz = 0, sm = 0;
         if (lane < 64) {
+            double ax = 0, ay = 0, az = 0, am = 0;
             for (int i = lane; i < N; i += 64) {
                 const double m = bv.mass[aoff + i];
-                sx += bv.v[roff + i] * m;
-                sy += bv.v[roff + N + i] * m;
-                sz += bv.v[roff + 2 * N + i] * m;
-                sm += m;
+                ax += bv.v[roff + i] * m;
+                ay += bv.v[roff + N + i] * m;
+                az += bv.v[roff + 2 * N + i] * m;
+                am += m;
             }
             for (int o = 32; o > 0; o >>= 1) {
-                sx += __shfl_xor(sx, o, 64);
-                sy += __shfl_xor(sy, o, 64);
-                sz += __shfl_xor(sz, o, 64);
-                sm += __shfl_xor(sm, o, 64);
+                ax += __shfl_xor(ax, o, 64);
+                ay += __shfl_xor(ay, o, 64);
+                az += __shfl_xor(az, o, 64);
+                am += __shfl_xor(am, o, 64);
             }
             if (lane == 0) {
-                red[0] = sx;
-                red[1] = sy;
-                red[2] = sz;
-                red[3] = sm;
+                red[0] = ax;
+                red[1] = ay;
+                red[2] = az;
+                red[3] = am;
             }
         }
         __syncthreads();
@@ -314,72 +334,87 @@ __global__ void __launch_bounds__(1024) k_replica(EmtConst ec, PropConst pc, Bat
             bv.v[roff + N + i] -= cy;
             bv.v[roff + 2 * N + i] -= cz;
         }
-        __syncthreads();
     }
 
-    const double *eta2 = s.tab + TB_ETA2, *bs0 = s.tab + TB_BS0, *kdb = s.tab + TB_KDB;
-    const double *csig = s.tab + TB_CSIG, *etad = s.tab + TB_ETAD, *wv = s.tab + TB_WV;
-    const double acut = ec.acut, rcut = ec.rcut, cutr = ec.cutr, cut2 = ec.cut2hi;
+    const double acut = ecg->acut, rcut = ecg->rcut, cutr = ecg->cutr, cut2 = ecg->cut2hi;
     const int apb = nthr / TPA;  // atoms per chunk
     const int wl = lane & 63;
-    const unsigned long long gmask =
-        (TPA >= 64 ? ~0ull : ((1ull << TPA) - 1ull)) << ((wl / TPA) * TPA);
+    const unsigned long long gmask = (TPA >= 64 ? ~0ull : ((1ull << TPA) - 1ull)) << ((wl / TPA) * TPA);
     const unsigned long long below = (wl == 0) ? 0ull : ((~0ull) >> (64 - wl));
 
-    // ---- neighbour list + pass A ----
+    // ---- phase 1: neighbour lists (min image, ordered by index) ----
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < N;
-        const int tk = act ? s.typ[k] : 0;
-        const double xk = act ? s.px[k] : 0.0, yk = act ? s.py[k] : 0.0, zk = act ? s.pz[k] : 0.0;
-        int cnt = 0;
+        const int tk = act ? typ[k] : 0;
+        const double fxk = act ? sx[k] : 0.0, fyk = act ? sy[k] : 0.0, fzk = act ? sz[k] : 0.0;
+        int c = 0;
         for (int n0 = 0; n0 < N; n0 += TPA) {
             const int n = n0 + sub;
             bool in = false;
-            if (act && n < N && n != k) {
-                const int tn = s.typ[n];
-                if (ec.pair_on[tk * MDT_MAXT + tn]) {
-                    double dx = xk - s.px[n], dy = yk - s.py[n], dz = zk - s.pz[n];
-                    minimg(cell, dx, dy, dz);
-                    in = (dx * dx + dy * dy + dz * dz) <= cut2;
-                }
+            int code = 0;
+            if (act && n < N && n != k && T.pair_on[tk * MDT_MAXT + typ[n]]) {
+                double d0 = fxk - sx[n], d1 = fyk - sy[n], d2 = fzk - sz[n];
+                const double m0 = rint(d0), m1 = rint(d1), m2 = rint(d2);
+                d0 -= m0;
+                d1 -= m1;
+                d2 -= m2;
+                const double ex = cell.h[0] * d0 + cell.h[3] * d1 + cell.h[6] * d2;
+                const double ey = cell.h[1] * d0 + cell.h[4] * d1 + cell.h[7] * d2;
+                const double ez = cell.h[2] * d0 + cell.h[5] * d1 + cell.h[8] * d2;
+                in = (ex * ex + ey * ey + ez * ez) <= cut2;
+                code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
             }
             const unsigned long long m = __ballot(in) & gmask;
             if (in) {
-                const int pos = cnt + __popcll(m & below);
+                const int pos = c + __popcll(m & below);
                 if (pos < cap)
-                    nbr[(size_t)k * cap + pos] = (uint16_t)n;
+                    nbr[(size_t)k * cap + pos] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
                 else
                     bad_local |= MDT_ST_NBR_OVERFLOW;
             }
-            cnt += __popcll(m);
+            c += __popcll(m);
         }
-        if (cnt > cap) cnt = cap;
+        if (act && sub == 0) cnt[k] = (uint16_t)(c < cap ? c : cap);
+    }
+    __syncthreads();
+
+    // ---- phase 2: pass A ----
+    for (int k0 = 0; k0 < N; k0 += apb) {
+        const int k = k0 + lane / TPA;
+        const int sub = lane % TPA;
+        const bool act = k < N;
         double sig = 0.0, ev = 0.0, fvx = 0.0, fvy = 0.0, fvz = 0.0;
+        int tk = 0;
         if (act) {
-            for (int e = sub; e < cnt; e += TPA) {
-                const int n = nbr[(size_t)k * cap + e];
-                const int tn = s.typ[n];
-                double dx = xk - s.px[n], dy = yk - s.py[n], dz = zk - s.pz[n];
-                minimg(cell, dx, dy, dz);
+            tk = typ[k];
+            const double xk = px[k], yk = py[k], zk = pz[k];
+            const int ck = cnt[k];
+            for (int e = sub; e < ck; e += TPA) {
+                const unsigned q = nbr[(size_t)k * cap + e];
+                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+                const int tn = typ[n];
+                const double dx = (xk - px[n]) - T.shift[3 * code];
+                const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+                const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
                 const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 if (r > cutr) continue;
-                const double e1 = exp(acut * (r - rcut));
+                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
                 const double th = 1.0 / (1 + e1);
                 const double rt1 = acut * e1 * th;
-                const double fs = th * exp(-eta2[tn] * (r - bs0[tn]));
-                const double fv = th * exp(-kdb[tn] * (r - bs0[tn]));
+                const double fs = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+                const double fv = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
                 const int kn = tk * MDT_MAXT + tn;
-                sig += csig[kn] * fs;
-                ev += wv[kn] * fv;
+                sig += T.csig[kn] * fs;
+                ev += T.wv[kn] * fv;
                 double cv;
                 if (tn == tk) {
-                    cv = wv[kn] * ((kdb[tn] + rt1) * fv);
+                    cv = T.wv[kn] * ((T.kdb[tn] + rt1) * fv);
                 } else {
-                    const double fv2 = th * exp(-kdb[tk] * (r - bs0[tk]));
-                    cv = 0.5 * (wv[kn] * ((kdb[tn] + rt1) * fv) +
-                                wv[tn * MDT_MAXT + tk] * ((kdb[tk] + rt1) * fv2));
+                    const double fv2 = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
+                    cv = 0.5 * (T.wv[kn] * ((T.kdb[tn] + rt1) * fv) +
+                                T.wv[tn * MDT_MAXT + tk] * ((T.kdb[tk] + rt1) * fv2));
                 }
                 const double cr = cv / r;
                 fvx -= cr * dx;
@@ -393,61 +428,61 @@ __global__ void __launch_bounds__(1024) k_replica(EmtConst ec, PropConst pc, Bat
         fvy = group_sum<TPA>(fvy);
         fvz = group_sum<TPA>(fvz);
         if (act && sub == 0) {
-            s.cnt[k] = (uint16_t)cnt;
-            double G = 0.0, E = 0.0, dn = 0.0;
-            if (ec.owner[tk]) {
+            double Gk = 0.0, Ek = 0.0;
+            if (T.owner[tk]) {
                 const double clamp30 = (double)1e-30f;  // pes_emt_mod.f90:793-794
                 const double sc = sig > clamp30 ? sig : clamp30;
                 const double sl = sc > MDT_TOLERANCE ? sc : MDT_TOLERANCE;
-                const double sv = -log(sl * MDT_TWELFTH) / ec.betaeta2[tk];
-                dn = ec.n0[tk] * exp(ec.densc[tk] * sv);
-                const double el = exp(-ec.lambda[tk] * sv);
-                const double ek = exp(-ec.kappa[tk] * sv);
-                E = ec.e0[tk] * ((1 + ec.lambda[tk] * sv) * el - 1) + 6 * ec.v0[tk] * ek - 0.5 * ev;
-                const double D = ec.e0[tk] * ec.lambda[tk] * ec.lambda[tk] * (sv * el) +
-                                 6 * ec.v0[tk] * ec.kappa[tk] * ek;
-                G = -D / (ec.betaeta2[tk] * sc);
+                const double sv = -log(sl * MDT_TWELFTH) / T.betaeta2[tk];
+                const double dn = T.n0[tk] * exp(T.densc[tk] * sv);
+                const double el = exp(-T.lambda[tk] * sv);
+                const double ek = exp(-T.kappa[tk] * sv);
+                Ek = T.e0[tk] * ((1 + T.lambda[tk] * sv) * el - 1) + 6 * T.v0[tk] * ek - 0.5 * ev;
+                const double D = T.e0[tk] * T.lambda[tk] * T.lambda[tk] * (sv * el) + 6 * T.v0[tk] * T.kappa[tk] * ek;
+                Gk = -D / (T.betaeta2[tk] * sc);
                 bv.dens[doff + k] = dn;  // raw EMT density (pes_emt_mod.f90:855,861)
             }
-            s.G[k] = G;
-            s.E[k] = E;
-            s.fx[k] = fvx;
-            s.fy[k] = fvy;
-            s.fz[k] = fvz;
+            G[k] = Gk;
+            E[k] = Ek;
+            wx[k] = fvx;
+            wy[k] = fvy;
+            wz[k] = fvz;
         }
     }
     __syncthreads();
 
-    // ---- pass B: density-derivative forces ----
+    // ---- phase 3: pass B, set_acceleration, propagate_2 ----
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < N;
         double fx = 0.0, fy = 0.0, fz = 0.0;
         if (act) {
-            const int tk = s.typ[k];
-            const double xk = s.px[k], yk = s.py[k], zk = s.pz[k];
-            const double Gk = s.G[k];
-            const int cnt = s.cnt[k];
-            for (int e = sub; e < cnt; e += TPA) {
-                const int n = nbr[(size_t)k * cap + e];
-                const int tn = s.typ[n];
-                double dx = xk - s.px[n], dy = yk - s.py[n], dz = zk - s.pz[n];
-                minimg(cell, dx, dy, dz);
+            const int tk = typ[k];
+            const double xk = px[k], yk = py[k], zk = pz[k];
+            const double Gk = G[k];
+            const int ck = cnt[k];
+            for (int e = sub; e < ck; e += TPA) {
+                const unsigned q = nbr[(size_t)k * cap + e];
+                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+                const int tn = typ[n];
+                const double dx = (xk - px[n]) - T.shift[3 * code];
+                const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+                const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
                 const double r = sqrt(dx * dx + dy * dy + dz * dz);
                 if (r > cutr) continue;
-                const double e1 = exp(acut * (r - rcut));
+                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
                 const double th = 1.0 / (1 + e1);
                 const double rt1 = acut * e1 * th;
-                const double fsn = th * exp(-eta2[tn] * (r - bs0[tn]));
+                const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
                 const int kn = tk * MDT_MAXT + tn, nk = tn * MDT_MAXT + tk;
-                const double cA = Gk * csig[kn] * ((etad[kn] + rt1) * fsn);
+                const double cA = Gk * T.csig[kn] * ((T.etad[kn] + rt1) * fsn);
                 double cB;
                 if (tn == tk) {
-                    cB = s.G[n] * csig[nk] * ((etad[nk] + rt1) * fsn);
+                    cB = G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsn);
                 } else {
-                    const double fsk = th * exp(-eta2[tk] * (r - bs0[tk]));
-                    cB = s.G[n] * csig[nk] * ((etad[nk] + rt1) * fsk);
+                    const double fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+                    cB = G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsk);
                 }
                 const double cr = (cA + cB) / r;
                 fx -= cr * dx;
@@ -459,9 +494,9 @@ __global__ void __launch_bounds__(1024) k_replica(EmtConst ec, PropConst pc, Bat
         fy = group_sum<TPA>(fy);
         fz = group_sum<TPA>(fz);
         if (act && sub == 0) {
-            const int tk = s.typ[k];
-            double F[3] = {fx + s.fx[k], fy + s.fy[k], fz + s.fz[k]};
-            if (!ec.owner[tk]) F[0] = F[1] = F[2] = 0.0;
+            const int tk = typ[k];
+            double F[3] = {fx + wx[k], fy + wy[k], fz + wz[k]};
+            if (!T.owner[tk]) F[0] = F[1] = F[2] = 0.0;
             if (add_fin) {
                 F[0] += bv.f[roff + k];
                 F[1] += bv.f[roff + N + k];
@@ -478,40 +513,16 @@ __global__ void __launch_bounds__(1024) k_replica(EmtConst ec, PropConst pc, Bat
             bv.a[roff + k] = acc[0];
             bv.a[roff + N + k] = acc[1];
             bv.a[roff + 2 * N + k] = acc[2];
-            if (mode == MDT_MODE_STEP) {
-                // propagate_2 (md_algo.f90:80-106), P == 1
-                const int algo = pc.algo[tk];
-                double v[3] = {bv.v[roff + k], bv.v[roff + N + k], bv.v[roff + 2 * N + k]};
-                if (algo == MDT_PROP_LANGEVIN) {
-                    int bad = 0;
-                    const double fric = ldfa(bv.dens[doff + k], mass, &bad);
-                    if (bad) bad_local |= MDT_ST_NEG_DENSITY;
-                    bv.dens[doff + k] = fric;
-                    const double temp = MDT_KB * 1 * (pc.is_proj[tk] ? tproj : tsurf) / mass;
-                    const bool exact = lan_big(fric, pc.dt) && temp > MDT_TOLERANCE;
-                    const LanCoef L = lan_coef(fric, pc.dt, temp, exact);
-                    const double w = L.sv * sqrt(1 - L.crv * L.crv);
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const double eta =
-                            mdt_normal(pc.seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)k, (uint32_t)c);
-                        v[c] = (fb >> c & 1) ? 0.0 : v[c] + L.c2 * acc[c] + w * eta;
-                    }
-                } else {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) v[c] = (fb >> c & 1) ? 0.0 : v[c] + 0.5 * pc.dt * acc[c];
-                }
-                bv.v[roff + k] = v[0];
-                bv.v[roff + N + k] = v[1];
-                bv.v[roff + 2 * N + k] = v[2];
-            }
+            if (mode == MDT_MODE_STEP)
+                bad_local |= prop2_p1(pcg, T.algo[tk], T.is_proj[tk], tsurf, tproj, mass, fb, tid, step, k,
+                                      bv.v + roff + k, bv.a + roff + k, N, bv.dens + doff + k);
         }
     }
     // ---- energy of the replica: fixed-order reduction by wave 0 ----
     __syncthreads();
     if (lane < 64) {
         double e = 0.0;
-        for (int i = lane; i < N; i += 64) e += s.E[i];
+        for (int i = lane; i < N; i += 64) e += E[i];
         for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
         if (lane == 0) bv.epot[bv.traj_eoff[t] + b] = e;
     }
@@ -523,11 +534,15 @@ __global__ void __launch_bounds__(1024) k_replica(EmtConst ec, PropConst pc, Bat
 // [propagate_1 + COM removal] then [ring-polymer free step] for one
 // trajectory per workgroup.  cjk (P x P, column-major cjk(j,k)) in global.
 // flags bit0: propagate_2, bit1: propagate_1, bit2: ring-polymer step
+// (trajectories with P == 1 only get f = 0 from bit2, as calc_force zeroes f).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const double *cjk_all,
-                                              const long long *cjk_off, uint32_t step_prev,
-                                              uint32_t step, int flags) {
+__global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg, BatchView bv,
+                                              const double *cjk_all, const long long *cjk_off,
+                                              uint32_t step_prev, uint32_t step, int flags) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_algo[MDT_MAXT], s_proj[MDT_MAXT];
+    __shared__ double red[4];
+    const PropConst &pc = *pcg;
     const int t = blockIdx.x;
     const int N = bv.traj_N[t];
     const int P = bv.traj_P[t];
@@ -539,12 +554,14 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
     const double dt = pc.dt;
     const double *cjk = cjk_all + cjk_off[t];
     const int nthr = blockDim.x, lane = threadIdx.x;
-    double *C = (double *)smem_raw;        // P*P
+    double *C = (double *)smem_raw;         // P*P
     uint8_t *big = (uint8_t *)(C + P * P);  // N flags
-    __shared__ double red[8];
     for (int k = lane; k < P * P; k += nthr) C[k] = cjk[k];
+    for (int k = lane; k < MDT_MAXT; k += nthr) {
+        s_algo[k] = pc.algo[k];
+        s_proj[k] = pc.is_proj[k];
+    }
     int bad_local = 0;
-    // index helpers: component c, bead b, atom i
 #define XI(c, b, i) (soff + ((long long)(b) * 3 + (c)) * N + (i))
 #define PI_(b, i) (poff + (long long)(b) * N + (i))
     __syncthreads();
@@ -554,7 +571,7 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
         for (int q = lane; q < N * P; q += nthr) {
             const int i = q % N, b = q / N;
             const int ty = bv.type[aoff + i];
-            if (pc.algo[ty] == MDT_PROP_LANGEVIN) {
+            if (s_algo[ty] == MDT_PROP_LANGEVIN) {
                 int bad = 0;
                 bv.dens[PI_(b, i)] = ldfa(bv.dens[PI_(b, i)], bv.mass[aoff + i], &bad);
                 if (bad) bad_local |= MDT_ST_NEG_DENSITY;
@@ -573,14 +590,14 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
             const int ty = bv.type[aoff + i];
             const uint8_t fb = bv.fixed[PI_(b, i)];
             const double mass = bv.mass[aoff + i];
-            if (pc.algo[ty] == MDT_PROP_LANGEVIN) {
-                const double temp = MDT_KB * P * (pc.is_proj[ty] ? tproj : tsurf) / mass;
+            if (s_algo[ty] == MDT_PROP_LANGEVIN) {
+                const double temp = MDT_KB * P * (s_proj[ty] ? tproj : tsurf) / mass;
                 const bool exact = big[i] && temp > MDT_TOLERANCE;
                 const LanCoef L = lan_coef(bv.dens[PI_(b, i)], dt, temp, exact);
                 const double w = L.sv * sqrt(1 - L.crv * L.crv);
                 for (int c = 0; c < 3; ++c) {
-                    const double eta = mdt_normal(pc.seed, tid, step_prev, MDT_STREAM_LANGEVIN,
-                                                  (uint32_t)(i * P + b), (uint32_t)c);
+                    const double eta =
+                        mdt_normal(pc.seed, tid, step_prev, MDT_STREAM_LANGEVIN, (uint32_t)(i * P + b), (uint32_t)c);
                     const double v = bv.v[XI(c, b, i)];
                     bv.v[XI(c, b, i)] = (fb >> c & 1) ? 0.0 : v + L.c2 * bv.a[XI(c, b, i)] + w * eta;
                 }
@@ -608,9 +625,9 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
             for (int q = lane; q < N * 3; q += nthr) {
                 const int i = q / 3, c = q % 3;
                 const int ty = bv.type[aoff + i];
-                if (pc.algo[ty] != MDT_PROP_PILE) continue;
+                if (s_algo[ty] != MDT_PROP_PILE) continue;
                 const double mass = bv.mass[aoff + i];
-                const double betaN = 1.0 / (MDT_KB * (pc.is_proj[ty] ? tproj : tsurf) * P);
+                const double betaN = 1.0 / (MDT_KB * (s_proj[ty] ? tproj : tsurf) * P);
                 double np[64];
                 for (int k = 0; k < P; ++k) {
                     double acc = 0.0;
@@ -628,8 +645,7 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
                     }
                     const double c1 = exp(-0.5 * dt * g);
                     const double c2 = sqrt(1 - c1 * c1);
-                    const double z =
-                        mdt_normal(pc.seed, tid, step, MDT_STREAM_PILE, (uint32_t)(i * P + k), (uint32_t)c);
+                    const double z = mdt_normal(pc.seed, tid, step, MDT_STREAM_PILE, (uint32_t)(i * P + k), (uint32_t)c);
                     np[k] = c1 * np[k] + sqrt(mass / betaN) * c2 * z;
                 }
                 for (int b = 0; b < P; ++b) {
@@ -645,16 +661,16 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
         for (int q = lane; q < N * P; q += nthr) {
             const int i = q % N, b = q / N;
             const int ty = bv.type[aoff + i];
-            const int algo = pc.algo[ty];
+            const int algo = s_algo[ty];
             const uint8_t fb = bv.fixed[PI_(b, i)];
             const double mass = bv.mass[aoff + i];
             if (algo == MDT_PROP_LANGEVIN) {
-                const double temp = MDT_KB * P * (pc.is_proj[ty] ? tproj : tsurf) / mass;
+                const double temp = MDT_KB * P * (s_proj[ty] ? tproj : tsurf) / mass;
                 const bool exact = big[i] && temp > MDT_TOLERANCE;
                 const LanCoef L = lan_coef(bv.dens[PI_(b, i)], dt, temp, exact);
                 for (int c = 0; c < 3; ++c) {
-                    const double eta = mdt_normal(pc.seed, tid, step, MDT_STREAM_LANGEVIN,
-                                                  (uint32_t)(i * P + b), (uint32_t)c);
+                    const double eta =
+                        mdt_normal(pc.seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)(i * P + b), (uint32_t)c);
                     const double r = bv.x[XI(c, b, i)], v = bv.v[XI(c, b, i)], a = bv.a[XI(c, b, i)];
                     if (!(fb >> c & 1)) {
                         bv.x[XI(c, b, i)] = r + L.c1 * v + L.c2 * dt * a + L.sr * eta;
@@ -671,10 +687,9 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
                     if (P == 1 && !(fb >> c & 1)) bv.x[XI(c, b, i)] += dt * vn;
                 }
                 if (algo == MDT_PROP_ANDERSEN) {
-                    const double ibetaN = P * MDT_KB * (pc.is_proj[ty] ? tproj : tsurf);
+                    const double ibetaN = P * MDT_KB * (s_proj[ty] ? tproj : tsurf);
                     const double sigma = sqrt(ibetaN / mass);
-                    const double u = mdt_uniform(pc.seed, tid, step, MDT_STREAM_ANDERSEN_U,
-                                                 (uint32_t)(i * P + b), 0u);
+                    const double u = mdt_uniform(pc.seed, tid, step, MDT_STREAM_ANDERSEN_U, (uint32_t)(i * P + b), 0u);
                     if (u < dt / pc.andersen_time)
                         for (int c = 0; c < 3; ++c) {
                             const double z = mdt_normal(pc.seed, tid, step, MDT_STREAM_ANDERSEN_V,
@@ -690,7 +705,7 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
         if (pc.any_andersen || pc.any_pile) {
             // remove_com_velocity, closed form of the 1000 iterations
             if (lane < 64) {
-                double sx = 0, sy = 0, sz = 0, sm = 0;
+                double ax = 0, ay = 0, az = 0, am = 0;
                 for (int i = lane; i < N; i += 64) {
                     const double m = bv.mass[aoff + i];
                     double bx = 0, by = 0, bz = 0;
@@ -699,22 +714,22 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
                         by += bv.v[XI(1, b, i)];
                         bz += bv.v[XI(2, b, i)];
                     }
-                    sx += bx * m;
-                    sy += by * m;
-                    sz += bz * m;
-                    sm += m;
+                    ax += bx * m;
+                    ay += by * m;
+                    az += bz * m;
+                    am += m;
                 }
                 for (int o = 32; o > 0; o >>= 1) {
-                    sx += __shfl_xor(sx, o, 64);
-                    sy += __shfl_xor(sy, o, 64);
-                    sz += __shfl_xor(sz, o, 64);
-                    sm += __shfl_xor(sm, o, 64);
+                    ax += __shfl_xor(ax, o, 64);
+                    ay += __shfl_xor(ay, o, 64);
+                    az += __shfl_xor(az, o, 64);
+                    am += __shfl_xor(am, o, 64);
                 }
                 if (lane == 0) {
-                    red[0] = sx;
-                    red[1] = sy;
-                    red[2] = sz;
-                    red[3] = sm;
+                    red[0] = ax;
+                    red[1] = ay;
+                    red[2] = az;
+                    red[3] = am;
                 }
             }
             __syncthreads();
@@ -730,63 +745,67 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
     }
 
     if (flags & 4) {
-        // do_ring_polymer_step (rpmd.f90:309-452), thread per (atom, component)
-        const double kt = MDT_KB * tproj * P / MDT_HBAR;
-        const double m1 = bv.mass[aoff + 0];
-        const double prefactor = -1 * m1 * (kt * kt);
-        const double piN = MDT_PI / P;
-        for (int q = lane; q < N * 3; q += nthr) {
-            const int i = q / 3, c = q % 3;
-            const int ty = bv.type[aoff + i];
-            const double mass = bv.mass[aoff + i];
-            if (pc.algo[ty] == MDT_PROP_LANGEVIN) {
-                // reverted step; spring force with m(1) and Tproj (rpmd.f90:418-435)
-                for (int b = 0; b < P; ++b) {
-                    const int pv = b == 0 ? P - 1 : b - 1, nx = b == P - 1 ? 0 : b + 1;
-                    bv.f[XI(c, b, i)] = prefactor * (2 * bv.x[XI(c, b, i)] - bv.x[XI(c, pv, i)] -
-                                                     bv.x[XI(c, nx, i)]);
+        if (P == 1) {
+            for (int q = lane; q < 3 * N; q += nthr) bv.f[soff + q] = 0.0;
+        } else {
+            // do_ring_polymer_step (rpmd.f90:309-452), thread per (atom, component)
+            const double kt = MDT_KB * tproj * P / MDT_HBAR;
+            const double m1 = bv.mass[aoff + 0];
+            const double prefactor = -1 * m1 * (kt * kt);
+            const double piN = MDT_PI / P;
+            for (int q = lane; q < N * 3; q += nthr) {
+                const int i = q / 3, c = q % 3;
+                const int ty = bv.type[aoff + i];
+                const double mass = bv.mass[aoff + i];
+                if (s_algo[ty] == MDT_PROP_LANGEVIN) {
+                    // reverted step; spring force with m(1) and Tproj (rpmd.f90:418-435)
+                    for (int b = 0; b < P; ++b) {
+                        const int pv = b == 0 ? P - 1 : b - 1, nx = b == P - 1 ? 0 : b + 1;
+                        bv.f[XI(c, b, i)] =
+                            prefactor * (2 * bv.x[XI(c, b, i)] - bv.x[XI(c, pv, i)] - bv.x[XI(c, nx, i)]);
+                    }
+                    continue;
                 }
-                continue;
-            }
-            for (int b = 0; b < P; ++b) bv.f[XI(c, b, i)] = 0.0;
-            double pq[64], qq[64];
-            for (int k = 0; k < P; ++k) {
-                double ap = 0.0, aq = 0.0;
-                for (int j = 0; j < P; ++j) {
-                    ap = ap + (mass * bv.v[XI(c, j, i)]) * C[j + P * k];
-                    aq = aq + bv.x[XI(c, j, i)] * C[j + P * k];
-                }
-                pq[k] = ap;
-                qq[k] = aq;
-            }
-            const double betaN = 1.0 / (MDT_KB * (pc.is_proj[ty] ? tproj : tsurf) * P);
-            const double twown = 2.0 / betaN / MDT_HBAR;
-            for (int k = 0; k < P; ++k) {
-                double p1 = 1.0, p2 = 0.0, p3 = dt / mass, p4 = 1.0;
-                const int fk = (k <= P / 2) ? k : P - k;  // mirrored modes share a frequency
-                if (fk > 0) {
-                    const double wk = twown * sin(fk * piN);
-                    const double wt = wk * dt, wm = wk * mass;
-                    const double cw = cos(wt), sw = sin(wt);
-                    p1 = cw;
-                    p2 = -wm * sw;
-                    p3 = sw / wm;
-                    p4 = cw;
-                }
-                const double pn = pq[k] * p1 + qq[k] * p2;
-                qq[k] = pq[k] * p3 + qq[k] * p4;
-                pq[k] = pn;
-            }
-            for (int b = 0; b < P; ++b) {
-                const uint8_t fb = bv.fixed[PI_(b, i)];
-                if (fb >> c & 1) continue;
-                double rr = 0.0, vv = 0.0;
+                for (int b = 0; b < P; ++b) bv.f[XI(c, b, i)] = 0.0;
+                double pq[64], qq[64];
                 for (int k = 0; k < P; ++k) {
-                    rr = rr + qq[k] * C[b + P * k];
-                    vv = vv + pq[k] * C[b + P * k] / mass;
+                    double ap = 0.0, aq = 0.0;
+                    for (int j = 0; j < P; ++j) {
+                        ap = ap + (mass * bv.v[XI(c, j, i)]) * C[j + P * k];
+                        aq = aq + bv.x[XI(c, j, i)] * C[j + P * k];
+                    }
+                    pq[k] = ap;
+                    qq[k] = aq;
                 }
-                bv.x[XI(c, b, i)] = rr;
-                bv.v[XI(c, b, i)] = vv;
+                const double betaN = 1.0 / (MDT_KB * (s_proj[ty] ? tproj : tsurf) * P);
+                const double twown = 2.0 / betaN / MDT_HBAR;
+                for (int k = 0; k < P; ++k) {
+                    double p1 = 1.0, p2 = 0.0, p3 = dt / mass, p4 = 1.0;
+                    const int fk = (k <= P / 2) ? k : P - k;  // mirrored modes share a frequency
+                    if (fk > 0) {
+                        const double wk = twown * sin(fk * piN);
+                        const double wt = wk * dt, wm = wk * mass;
+                        const double cw = cos(wt), sw = sin(wt);
+                        p1 = cw;
+                        p2 = -wm * sw;
+                        p3 = sw / wm;
+                        p4 = cw;
+                    }
+                    const double pn = pq[k] * p1 + qq[k] * p2;
+                    qq[k] = pq[k] * p3 + qq[k] * p4;
+                    pq[k] = pn;
+                }
+                for (int b = 0; b < P; ++b) {
+                    const uint8_t fb = bv.fixed[PI_(b, i)];
+                    if (fb >> c & 1) continue;
+                    double rr = 0.0, vv = 0.0;
+                    for (int k = 0; k < P; ++k) {
+                        rr = rr + qq[k] * C[b + P * k];
+                        vv = vv + pq[k] * C[b + P * k] / mass;
+                    }
+                    bv.x[XI(c, b, i)] = rr;
+                    bv.v[XI(c, b, i)] = vv;
+                }
             }
         }
     }
@@ -800,37 +819,37 @@ __global__ void __launch_bounds__(256) k_traj(PropConst pc, BatchView bv, const 
 // ---------------------------------------------------------------------------
 size_t mdt_replica_smem(int N, int cap, bool nbr_lds) {
     const int c = cap < N - 1 ? cap : (N > 1 ? N - 1 : 1);
-    return sizeof(double) * (TB_SIZE + 8 * (size_t)N) +
-           sizeof(uint16_t) * ((size_t)N + (nbr_lds ? (size_t)N * c : 0)) + N + 16;
+    return sizeof(double) * (8 * (size_t)N) + sizeof(uint16_t) * ((size_t)N + (nbr_lds ? (size_t)N * c : 0)) + N +
+           16;
 }
 
-template <bool L>
-static hipError_t launch_replica_t(int tpa, dim3 g, dim3 blk, size_t smem, const EmtConst &ec,
-                                   const PropConst &pc, const BatchView &bv, uint32_t step, int mode,
-                                   int add_fin, hipStream_t st) {
+template <bool L, int M>
+static hipError_t launch_replica_t(int tpa, dim3 g, dim3 blk, size_t smem, const EmtConst *ec, const PropConst *pc,
+                                   const BatchView &bv, uint32_t step, int add_fin, hipStream_t st) {
     switch (tpa) {
-    case 1: hipLaunchKernelGGL((k_replica<1, L>), g, blk, smem, st, ec, pc, bv, step, mode, add_fin); break;
-    case 2: hipLaunchKernelGGL((k_replica<2, L>), g, blk, smem, st, ec, pc, bv, step, mode, add_fin); break;
-    case 4: hipLaunchKernelGGL((k_replica<4, L>), g, blk, smem, st, ec, pc, bv, step, mode, add_fin); break;
-    case 8: hipLaunchKernelGGL((k_replica<8, L>), g, blk, smem, st, ec, pc, bv, step, mode, add_fin); break;
+    case 1: hipLaunchKernelGGL((k_replica<1, L, M>), g, blk, smem, st, ec, pc, bv, step, add_fin); break;
+    case 2: hipLaunchKernelGGL((k_replica<2, L, M>), g, blk, smem, st, ec, pc, bv, step, add_fin); break;
+    case 4: hipLaunchKernelGGL((k_replica<4, L, M>), g, blk, smem, st, ec, pc, bv, step, add_fin); break;
+    case 8: hipLaunchKernelGGL((k_replica<8, L, M>), g, blk, smem, st, ec, pc, bv, step, add_fin); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst &ec,
-                              const PropConst &pc, const BatchView &bv, uint32_t step, int mode,
-                              int add_fin, hipStream_t st) {
+// ec and pc are device pointers; mode 0 = calc_force, 1 = fused MD step (P == 1)
+hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst *ec, const PropConst *pc,
+                              const BatchView &bv, uint32_t step, int mode, int add_fin, hipStream_t st) {
     dim3 g(nrep), blk(block);
-    if (bv.nbr_global == nullptr)
-        return launch_replica_t<true>(tpa, g, blk, smem, ec, pc, bv, step, mode, add_fin, st);
-    return launch_replica_t<false>(tpa, g, blk, smem, ec, pc, bv, step, mode, add_fin, st);
+    const bool lds = bv.nbr_global == nullptr;
+    if (mode == MDT_MODE_STEP)
+        return lds ? launch_replica_t<true, MDT_MODE_STEP>(tpa, g, blk, smem, ec, pc, bv, step, add_fin, st)
+                   : launch_replica_t<false, MDT_MODE_STEP>(tpa, g, blk, smem, ec, pc, bv, step, add_fin, st);
+    return lds ? launch_replica_t<true, MDT_MODE_FORCE>(tpa, g, blk, smem, ec, pc, bv, step, add_fin, st)
+               : launch_replica_t<false, MDT_MODE_FORCE>(tpa, g, blk, smem, ec, pc, bv, step, add_fin, st);
 }
 
-hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst &pc, const BatchView &bv,
-                           const double *cjk, const long long *cjk_off, uint32_t step_prev,
-                           uint32_t step, int flags, hipStream_t st) {
-    hipLaunchKernelGGL(k_traj, dim3(ntraj), dim3(256), smem, st, pc, bv, cjk, cjk_off, step_prev, step,
-                       flags);
+hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
+                           const long long *cjk_off, uint32_t step_prev, uint32_t step, int flags, hipStream_t st) {
+    hipLaunchKernelGGL(k_traj, dim3(ntraj), dim3(256), smem, st, pc, bv, cjk, cjk_off, step_prev, step, flags);
     return hipGetLastError();
 }

@@ -153,6 +153,30 @@ class OracleState:
         if rc:
             raise RuntimeError(f"oracle md_step failed rc={rc}")
 
+    def prop1(self, randy=None, zeta=None, and_z=None, and_u=None):
+        z = np.zeros((self.N, self.P, 3))
+        args = [np.ascontiguousarray(x if x is not None else z) for x in (randy, zeta, and_z)]
+        au = np.ascontiguousarray(and_u if and_u is not None else np.ones((self.N, self.P)))
+        rc = lib().orc_propagate_1(C.byref(self.u), C.byref(self.sp), _dp(args[0]), _dp(args[1]),
+                                   _dp(args[2]), _dp(au), _dp(self.cjk))
+        if rc:
+            raise RuntimeError(f"oracle propagate_1 failed rc={rc}")
+
+    def prop2(self, randy=None):
+        z = np.ascontiguousarray(randy if randy is not None else np.zeros((self.N, self.P, 3)))
+        rc = lib().orc_propagate_2(C.byref(self.u), C.byref(self.sp), _dp(z))
+        if rc:
+            raise RuntimeError(f"oracle propagate_2 failed rc={rc}")
+
+    def ldfa(self, i):
+        return lib().orc_ldfa(C.byref(self.u), i)
+
+    def ring_polymer_step(self):
+        lib().orc_ring_polymer_step(C.byref(self.u), C.byref(self.sp), _dp(self.cjk))
+
+    def remove_com(self):
+        lib().orc_remove_com_velocity(C.byref(self.u))
+
     def run(self, seed, traj, step0, nsteps):
         rc = lib().orc_run_trajectory(C.byref(self.u), C.byref(self.params), C.byref(self.sp),
                                       seed, traj, step0, nsteps)
