@@ -50,6 +50,7 @@ extern "C" {
 #define MDT_ST_NBR_OVERFLOW 1  /* neighbour list capacity exceeded */
 #define MDT_ST_NEG_DENSITY 2   /* ldfa saw a negative density (src/md_algo.f90:551-554) */
 #define MDT_ST_BAD_PROP 4      /* unknown propagator (src/md_algo.f90:66-67) */
+#define MDT_ST_FIXED_OVERFLOW 8 /* a pair term exceeded the fixed-point accumulator range */
 
 typedef struct mdt_context mdt_context;
 typedef struct mdt_batch mdt_batch;
@@ -161,6 +162,15 @@ int mdt_batch_status(mdt_batch *b, int32_t *status, int clear);
  * the batch's stream: total milliseconds and number of step launches.
  */
 int mdt_batch_last_timing(mdt_batch *b, double *ms_total, int *nlaunch);
+
+/*
+ * Per-launch timing of the calc_force kernel: with profiling on, every
+ * mdt_batch_step records HIP events around each force launch on the batch's
+ * stream; mdt_batch_force_timing returns their summed milliseconds for the
+ * last call (the launch count is mdt_batch_last_timing's nlaunch).
+ */
+int mdt_batch_profile(mdt_batch *b, int on);
+int mdt_batch_force_timing(mdt_batch *b, double *force_ms_total);
 
 /*
  * Launch tuning: threads per atom (1, 2, 4 or 8; 0 = autotune) and

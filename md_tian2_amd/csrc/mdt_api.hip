@@ -15,7 +15,7 @@
 
 size_t mdt_replica_smem(int N, int cap, bool nbr_lds);
 hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst *ec, const PropConst *pc,
-                              const BatchView &bv, uint32_t step, int mode, int add_fin, hipStream_t st);
+                              const BatchView &bv, int add_fin, hipStream_t st);
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
                            const long long *cjk_off, uint32_t step_prev, uint32_t step, int flags, hipStream_t st);
 
@@ -98,6 +98,9 @@ struct mdt_batch {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int last_nlaunch = 0;
+    int profile = 0;                // record events around every force launch
+    std::vector<hipEvent_t> evf;    // start/stop pairs, one per step
+    double last_force_ms = 0.0;     // summed force-kernel time of the last call
 
     BatchView view() const {
         BatchView v{};
@@ -311,6 +314,7 @@ int mdt_batch_destroy(mdt_batch *b) {
         if (p) (void)hipFree(p);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
+    for (hipEvent_t e : b->evf) (void)hipEventDestroy(e);
     delete b;
     return 0;
 }
@@ -609,9 +613,10 @@ static int configure(mdt_batch *b, int tpa, int nbcap) {
         HIPCHK(hipGetDeviceProperties(&prop, b->ctx->device));
         b->lds_limit = (size_t)prop.sharedMemPerBlock;
     }
-    // capacity: every other atom for small replicas, else 168 (fcc within
-    // 1.5 rcut holds 134 neighbours; the margin covers thermal disorder)
-    int cap = nbcap > 0 ? nbcap : 168;
+    // capacity of the owned half lists: every other atom for small replicas,
+    // else 112 (fcc within 1.5 rcut holds 134 neighbours, of which an atom
+    // owns about half; overflow is detected and reported)
+    int cap = nbcap > 0 ? nbcap : 112;
     if (cap > b->maxN - 1) cap = std::max(1, b->maxN - 1);
     const size_t smem_lds = mdt_replica_smem(b->maxN, cap, true);
     const bool in_lds = smem_lds <= b->lds_limit;
@@ -644,8 +649,8 @@ static int launch_force(mdt_batch *b) {
         const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
         HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 4, c->stream));
     }
-    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->step, 0,
-                              b->allP1 ? 0 : 1, c->stream));
+    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->allP1 ? 0 : 1,
+                              c->stream));
     return 0;
 }
 
@@ -685,10 +690,10 @@ static int autotune(mdt_batch *b) {
         if (b->maxN * tpa > 512 && tpa != 1) continue;
         if ((rc = configure(b, tpa, b->nbcap_req))) return rc;
         const BatchView bv = b->view();
-        HIPCHK(mdt_launch_replica(tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->step, 0, 0, st));
+        HIPCHK(mdt_launch_replica(tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, 0, st));
         HIPCHK(hipEventRecord(b->ev0, st));
         for (int k = 0; k < 3; ++k)
-            HIPCHK(mdt_launch_replica(tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->step, 0, 0, st));
+            HIPCHK(mdt_launch_replica(tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, 0, st));
         HIPCHK(hipEventRecord(b->ev1, st));
         HIPCHK(hipEventSynchronize(b->ev1));
         float ms = 0.f;
@@ -738,32 +743,43 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
     mdt_context *c = b->ctx;
     const BatchView bv = b->view();
     int nl = 0;
-    HIPCHK(hipEventRecord(b->ev0, c->stream));
-    if (b->allP1) {
-        for (int s = 0; s < nsteps; ++s) {
-            const uint32_t st = ++b->step;
-            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, st, 1, 0, c->stream));
-            ++nl;
-        }
-    } else {
-        const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
-        for (int s = 0; s < nsteps; ++s) {
-            const uint32_t prev = b->step;
-            const uint32_t st = ++b->step;
-            const int flags = (s > 0 ? 1 : 0) | 2 | 4;
-            HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, prev, st, flags, c->stream));
-            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, st, 0, 1, c->stream));
-            ++nl;
-        }
-        if (nsteps > 0)
-            HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 1, c->stream));
+    // one step: [propagate_2 of the previous step +] propagate_1 (+ ring-polymer
+    // step when P > 1) per trajectory, then calc_force per replica; the last
+    // propagate_2 closes the call so the host sees the reference's state.
+    const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
+    const int rp = b->allP1 ? 0 : 4;
+    if (b->profile && (int)b->evf.size() < 2 * nsteps) {
+        const size_t old = b->evf.size();
+        b->evf.resize(2 * nsteps);
+        for (size_t k = old; k < b->evf.size(); ++k) HIPCHK(hipEventCreate(&b->evf[k]));
     }
+    HIPCHK(hipEventRecord(b->ev0, c->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        const uint32_t prev = b->step;
+        const uint32_t st = ++b->step;
+        const int flags = (s > 0 ? 1 : 0) | 2 | rp;
+        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, prev, st, flags, c->stream));
+        if (b->profile) HIPCHK(hipEventRecord(b->evf[2 * s], c->stream));
+        HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->allP1 ? 0 : 1,
+                                  c->stream));
+        if (b->profile) HIPCHK(hipEventRecord(b->evf[2 * s + 1], c->stream));
+        ++nl;
+    }
+    if (nsteps > 0)
+        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 1, c->stream));
     HIPCHK(hipEventRecord(b->ev1, c->stream));
     HIPCHK(hipEventSynchronize(b->ev1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->last_ms = ms;
     b->last_nlaunch = nl;
+    b->last_force_ms = 0.0;
+    if (b->profile)
+        for (int s = 0; s < nsteps; ++s) {
+            float fm = 0.f;
+            HIPCHK(hipEventElapsedTime(&fm, b->evf[2 * s], b->evf[2 * s + 1]));
+            b->last_force_ms += fm;
+        }
     return check_status(b);
 }
 
@@ -778,6 +794,19 @@ int mdt_batch_last_timing(mdt_batch *b, double *ms_total, int *nlaunch) {
     if (!b) return fail(MDT_EINVAL, "NULL batch");
     if (ms_total) *ms_total = b->last_ms;
     if (nlaunch) *nlaunch = b->last_nlaunch;
+    return 0;
+}
+
+int mdt_batch_profile(mdt_batch *b, int on) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    b->profile = on ? 1 : 0;
+    return 0;
+}
+
+int mdt_batch_force_timing(mdt_batch *b, double *force_ms_total) {
+    if (!b || !force_ms_total) return fail(MDT_EINVAL, "mdt_batch_force_timing: NULL argument");
+    if (!b->profile) return fail(MDT_ESTATE, "profiling is off (call mdt_batch_profile)");
+    *force_ms_total = b->last_force_ms;
     return 0;
 }
 
@@ -827,7 +856,7 @@ int mdt_compute_emt(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx
     if ((rc = ensure_ready(b, false))) return rc;
     if ((rc = autotune_if_needed(b))) return rc;
     HIPCHK(hipMemsetAsync(b->d_f, 0, sizeof(double) * b->tot3pn, ctx->stream));
-    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, ctx->d_ec, b->d_pc, b->view(), 0, 0, 0, ctx->stream));
+    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, ctx->d_ec, b->d_pc, b->view(), 0, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if ((rc = check_status(b))) return rc;
     const long long n3 = 3LL * natoms * nbeads;

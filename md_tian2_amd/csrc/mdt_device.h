@@ -37,6 +37,7 @@
 #define MDT_ST_NBR_OVERFLOW 1
 #define MDT_ST_NEG_DENSITY 2
 #define MDT_ST_BAD_PROP 4
+#define MDT_ST_FIXED_OVERFLOW 8
 
 // EMT constants derived on the host exactly as src/pes_emt_mod.f90:533-592
 // (2 species) or :267-307 (1 species), flattened into per-type and

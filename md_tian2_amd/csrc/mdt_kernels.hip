@@ -1,22 +1,26 @@
 // mdt_kernels.hip -- CDNA4 (gfx950) kernels for batched EMT forces and the
 // md_tian2 propagators.
 //
-// k_replica: one workgroup per replica (trajectory t, bead b).
+// k_replica: calc_force, one workgroup per replica (trajectory t, bead b).
 //   phase 0: stage the replica in LDS, folded into its cell, with fractional
-//            coordinates (and propagate_1 first when mode == STEP, P == 1);
+//            coordinates;
 //   phase 1: TPA lanes per atom scan all atoms with the minimum-image
 //            convention and build an ordered per-atom neighbour list of
 //            (index | periodic-image code << 11), compacted with a wavefront
 //            ballot;
-//   phase 2 (pass A): sigma_a = sum_n csig(ta,tn) f_sigma_tn(r), the pair (V)
-//            energy and force of atom a, then s, dens, E_coh + vref and
-//            G_a = dE/dsigma_a per atom;
-//   phase 3 (pass B): F_a -= sum_n (G_a c f'_tn + G_n c f'_ta) r_hat,
-//            set_acceleration, and propagate_2 when mode == STEP.
-//   Lane partial sums are combined by wavefront shuffles in a fixed order,
-//   so results are bitwise reproducible (no atomics).
-// k_traj: one workgroup per trajectory for P > 1: propagate_2 of the
-//   previous step, propagate_1, COM removal and the ring-polymer free step
+//   phase 2 (pass A): every unordered pair once, by its owner atom:
+//            sigma_k += csig(tk,tn) f_sigma_tn(r) in registers and
+//            sigma_n += csig(tn,tk) f_sigma_tk(r) into a fixed-point LDS
+//            accumulator; then per atom s, dens, E_coh + vref and
+//            G_a = dE/dsigma_a;
+//   phase 3 (pass B): the pair's dE/dr (density part through G of both
+//            atoms, pair-potential part directly) gives +F on the owner and
+//            -F on the partner (Newton's third law), both accumulated in
+//            fixed point; then set_acceleration.
+//   Fixed-point integer accumulation is associative, so results are bitwise
+//   reproducible whatever order the lanes' atomics land in.
+// k_traj: one workgroup per trajectory: propagate_2 of the previous step,
+//   propagate_1, COM removal and (P > 1) the ring-polymer free step
 //   (src/md_algo.f90:37-106, src/rpmd.f90:309-452).
 #include <hip/hip_runtime.h>
 
@@ -24,8 +28,6 @@
 #include "mdt_math.h"
 #include "mdt_philox.h"
 
-#define MDT_MODE_FORCE 0
-#define MDT_MODE_STEP 1
 #define MDT_NBR_IDX_BITS 11
 #define MDT_NBR_IDX_MASK 0x7FF
 
@@ -152,98 +154,33 @@ __device__ void load_tables(LdsTables &T, const EmtConst *__restrict__ ec, const
 
 }  // namespace
 
-// propagate_1 for one atom of a P == 1 trajectory (md_algo.f90:37-70) on its
-// global state: x, v, a point at component 0 of the atom, components are N
-// apart.  Not inlined, so its registers do not add to the pair loops'.
-__device__ __noinline__ void prop1_p1(const PropConst *__restrict__ pcg, int algo, int isproj, double tsurf,
-                                      double tproj, double mass, int fb, double fric, uint32_t tid, uint32_t step,
-                                      int i, double *x, double *v, const double *a, int N) {
-    const double dt = pcg->dt;
-    if (algo == MDT_PROP_LANGEVIN) {
-        const double temp = MDT_KB * 1 * (isproj ? tproj : tsurf) / mass;
-        const bool exact = lan_big(fric, dt) && temp > MDT_TOLERANCE;
-        const LanCoef L = lan_coef(fric, dt, temp, exact);
-        for (int c = 0; c < 3; ++c) {
-            const double eta = mdt_normal(pcg->seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)i, (uint32_t)c);
-            const double r = x[c * N], vv = v[c * N], acc = a[c * N];
-            if (!(fb >> c & 1)) {
-                x[c * N] = r + L.c1 * vv + L.c2 * dt * acc + L.sr * eta;
-                v[c * N] = L.c0 * vv + (L.c1 - L.c2) * acc + L.sv * L.crv * eta;
-            } else {
-                v[c * N] = 0.0;
-            }
-        }
+// Fixed-point scales of the LDS accumulators that receive pair
+// contributions from other lanes: integer addition is associative, so the
+// totals do not depend on the order in which lanes add them.
+#define MDT_SIG_SCALE 281474976710656.0      // 2^48: sigma up to 3.2e4, step 3.6e-15
+#define MDT_SIG_ISCALE (1.0 / 281474976710656.0)
+#define MDT_F_SCALE 4398046511104.0          // 2^42: force up to 2.1e6 eV/A, step 2.3e-13
+#define MDT_F_ISCALE (1.0 / 4398046511104.0)
+
+__device__ __forceinline__ void fx_add(unsigned long long *acc, double x, double scale, int *bad) {
+    const double y = x * scale;
+    if (!(fabs(y) < 9.0e18)) {
+        *bad |= MDT_ST_FIXED_OVERFLOW;
         return;
     }
-    if (algo == MDT_PROP_PILE) {
-        // pile_thermostat with P == 1: C = [1], centroid gamma = 1/tau (md_algo.f90:440-502)
-        const double betaN = 1.0 / (MDT_KB * (isproj ? tproj : tsurf) * 1);
-        const double g0 = 1 / pcg->pile_tau;
-        const double c1 = exp(-0.5 * dt * g0);
-        const double c2 = sqrt(1 - c1 * c1);
-        const double sq = sqrt(mass / betaN);
-        for (int c = 0; c < 3; ++c) {
-            const double z = mdt_normal(pcg->seed, tid, step, MDT_STREAM_PILE, (uint32_t)i, (uint32_t)c);
-            double np = 0.0 + (mass * v[c * N]) * 1.0;  // C^T p with C = sqrt(1/1)
-            np = c1 * np + sq * c2 * z;
-            double ap = 0.0;
-            if (!(fb >> c & 1)) ap = ap + np * 1.0;
-            v[c * N] = ap / mass;
-        }
-    }
-    // verlet_1 (md_algo.f90:110-128), also for andersen and after pile
-    for (int c = 0; c < 3; ++c) {
-        if (!(fb >> c & 1)) {
-            const double vn = v[c * N] + 0.5 * dt * a[c * N];
-            v[c * N] = vn;
-            x[c * N] = x[c * N] + dt * vn;
-        } else {
-            v[c * N] = 0.0;
-        }
-    }
-    if (algo == MDT_PROP_ANDERSEN) {
-        // andersen (md_algo.f90:402-432)
-        const double ibetaN = 1 * MDT_KB * (isproj ? tproj : tsurf);
-        const double sigma = sqrt(ibetaN / mass);
-        const double u = mdt_uniform(pcg->seed, tid, step, MDT_STREAM_ANDERSEN_U, (uint32_t)i, 0u);
-        if (u < dt / pcg->andersen_time) {
-            for (int c = 0; c < 3; ++c) {
-                const double z = mdt_normal(pcg->seed, tid, step, MDT_STREAM_ANDERSEN_V, (uint32_t)i, (uint32_t)c);
-                if (!(fb >> c & 1)) v[c * N] = z * sigma + 0.0;
-            }
-        }
-    }
+    atomicAdd(acc, (unsigned long long)(long long)rint(y));
 }
 
-// propagate_2 for one atom of a P == 1 trajectory (md_algo.f90:80-106) on its
-// global state; dens holds the raw EMT density on entry and the LDFA
-// friction on exit for Langevin atoms.  Returns status bits.
-__device__ __noinline__ int prop2_p1(const PropConst *__restrict__ pcg, int algo, int isproj, double tsurf,
-                                     double tproj, double mass, int fb, uint32_t tid, uint32_t step, int k,
-                                     double *v, const double *a, int N, double *dens) {
-    const double dt = pcg->dt;
-    int bad = 0;
-    if (algo == MDT_PROP_LANGEVIN) {
-        const double fric = ldfa(*dens, mass, &bad);
-        *dens = fric;
-        const double temp = MDT_KB * 1 * (isproj ? tproj : tsurf) / mass;
-        const bool exact = lan_big(fric, dt) && temp > MDT_TOLERANCE;
-        const LanCoef L = lan_coef(fric, dt, temp, exact);
-        const double w = L.sv * sqrt(1 - L.crv * L.crv);
-        for (int c = 0; c < 3; ++c) {
-            const double eta = mdt_normal(pcg->seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)k, (uint32_t)c);
-            v[c * N] = (fb >> c & 1) ? 0.0 : v[c * N] + L.c2 * a[c * N] + w * eta;
-        }
-    } else {
-        for (int c = 0; c < 3; ++c) v[c * N] = (fb >> c & 1) ? 0.0 : v[c * N] + 0.5 * dt * a[c * N];
-    }
-    return bad ? MDT_ST_NEG_DENSITY : 0;
-}
+// Pair ownership: each unordered pair is evaluated once, by its owner; the
+// owner alternates with index parity, which gives every atom about half of
+// its neighbours.  The 2^48 fixed point of the partner sigma sums bounds the
+// force error of an isolated atom (sigma ~ 1e-11, the s clamp region) by
+// ~1e-10 eV/A, since that atom's force scales with sigma itself.
+__device__ __forceinline__ bool owns(int k, int n) { return (k < n) != (((k + n) & 1) != 0); }
 
-template <int TPA, bool NBR_LDS, int mode>
+template <int TPA, bool NBR_LDS>
 __global__ void __launch_bounds__(512, 4)
-    k_replica(const EmtConst *__restrict__ ecg, const PropConst *__restrict__ pcg, BatchView bv, uint32_t step,
-              int add_fin) {
+    k_replica(const EmtConst *__restrict__ ecg, const PropConst *__restrict__ pcg, BatchView bv, int add_fin) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ LdsTables T;
     __shared__ double red[4];
@@ -264,27 +201,24 @@ __global__ void __launch_bounds__(512, 4)
     load_cell(bv, t, cell);
     load_tables(T, ecg, pcg, cell, lane, nthr);
 
-    // LDS: px py pz | (sx sy sz, later wx wy wz) | G | E | cnt | nbr | typ
+    // LDS: px py pz | sx sy sz (scan) = sacc fxa fya (accumulators) | fza | G | E | cnt | nbr | typ
     double *px = (double *)smem_raw;
     double *py = px + N, *pz = py + N;
     double *sx = pz + N, *sy = sx + N, *sz = sy + N;
-    double *wx = sx, *wy = sy, *wz = sz;  // pass-A force partials reuse the scan's buffers
-    double *G = sz + N, *E = G + N;
+    unsigned long long *sacc = (unsigned long long *)sx;
+    unsigned long long *fxa = (unsigned long long *)sy, *fya = (unsigned long long *)sz;
+    unsigned long long *fza = (unsigned long long *)(sz + N);
+    double *G = (double *)(fza + N), *E = G + N;
     uint16_t *cnt = (uint16_t *)(E + N);
     uint16_t *nbr = NBR_LDS ? cnt + N : bv.nbr_global + (size_t)rep * bv.nbr_stride;
     uint8_t *typ = (uint8_t *)(cnt + N + (NBR_LDS ? (size_t)N * cap : 0));
     int bad_local = 0;
-    const double dt = pcg->dt;
-    const bool any_com = pcg->any_andersen || pcg->any_pile;
     __syncthreads();
 
-    // ---- phase 0: stage (after propagate_1 when fused) ----
+    // ---- phase 0: stage ----
     for (int i = lane; i < N; i += nthr) {
         const int ty = bv.type[aoff + i];
         typ[i] = (uint8_t)ty;
-        if (mode == MDT_MODE_STEP)
-            prop1_p1(pcg, T.algo[ty], T.is_proj[ty], tsurf, tproj, bv.mass[aoff + i], bv.fixed[doff + i],
-                     bv.dens[doff + i], tid, step, i, bv.x + roff + i, bv.v + roff + i, bv.a + roff + i, N);
         const double r[3] = {bv.x[roff + i], bv.x[roff + N + i], bv.x[roff + 2 * N + i]};
         // fold into the cell: fractional in [0, 1)
         double f0 = cell.ih[0] * r[0] + cell.ih[3] * r[1] + cell.ih[6] * r[2];
@@ -301,48 +235,13 @@ __global__ void __launch_bounds__(512, 4)
         pz[i] = cell.h[2] * f0 + cell.h[5] * f1 + cell.h[8] * f2;
     }
     __syncthreads();
-    if (mode == MDT_MODE_STEP && any_com) {
-        // remove_com_velocity (universe_mod.f90:222-241): 1000 subtractions of
-        // S_k/(M P N) with S_k = S_0 (1-1/N)^k, summed in closed form.
-        if (lane < 64) {
-            double ax = 0, ay = 0, az = 0, am = 0;
-            for (int i = lane; i < N; i += 64) {
-                const double m = bv.mass[aoff + i];
-                ax += bv.v[roff + i] * m;
-                ay += bv.v[roff + N + i] * m;
-                az += bv.v[roff + 2 * N + i] * m;
-                am += m;
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                ax += __shfl_xor(ax, o, 64);
-                ay += __shfl_xor(ay, o, 64);
-                az += __shfl_xor(az, o, 64);
-                am += __shfl_xor(am, o, 64);
-            }
-            if (lane == 0) {
-                red[0] = ax;
-                red[1] = ay;
-                red[2] = az;
-                red[3] = am;
-            }
-        }
-        __syncthreads();
-        const double fac = (1.0 - pow(1.0 - 1.0 / N, 1000.0)) / red[3] / 1;
-        const double cx = red[0] * fac, cy = red[1] * fac, cz = red[2] * fac;
-        for (int i = lane; i < N; i += nthr) {
-            bv.v[roff + i] -= cx;
-            bv.v[roff + N + i] -= cy;
-            bv.v[roff + 2 * N + i] -= cz;
-        }
-    }
-
     const double acut = ecg->acut, rcut = ecg->rcut, cutr = ecg->cutr, cut2 = ecg->cut2hi;
     const int apb = nthr / TPA;  // atoms per chunk
     const int wl = lane & 63;
     const unsigned long long gmask = (TPA >= 64 ? ~0ull : ((1ull << TPA) - 1ull)) << ((wl / TPA) * TPA);
     const unsigned long long below = (wl == 0) ? 0ull : ((~0ull) >> (64 - wl));
 
-    // ---- phase 1: neighbour lists (min image, ordered by index) ----
+    // ---- phase 1: owned half neighbour lists (min image, ordered by index) ----
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
@@ -354,17 +253,20 @@ __global__ void __launch_bounds__(512, 4)
             const int n = n0 + sub;
             bool in = false;
             int code = 0;
-            if (act && n < N && n != k && T.pair_on[tk * MDT_MAXT + typ[n]]) {
-                double d0 = fxk - sx[n], d1 = fyk - sy[n], d2 = fzk - sz[n];
-                const double m0 = rint(d0), m1 = rint(d1), m2 = rint(d2);
-                d0 -= m0;
-                d1 -= m1;
-                d2 -= m2;
-                const double ex = cell.h[0] * d0 + cell.h[3] * d1 + cell.h[6] * d2;
-                const double ey = cell.h[1] * d0 + cell.h[4] * d1 + cell.h[7] * d2;
-                const double ez = cell.h[2] * d0 + cell.h[5] * d1 + cell.h[8] * d2;
-                in = (ex * ex + ey * ey + ez * ez) <= cut2;
-                code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
+            if (act && n < N && n != k) {
+                const int tn = typ[n];
+                if (T.pair_on[tk * MDT_MAXT + tn] && owns(k, n)) {
+                    double d0 = fxk - sx[n], d1 = fyk - sy[n], d2 = fzk - sz[n];
+                    const double m0 = rint(d0), m1 = rint(d1), m2 = rint(d2);
+                    d0 -= m0;
+                    d1 -= m1;
+                    d2 -= m2;
+                    const double ex = cell.h[0] * d0 + cell.h[3] * d1 + cell.h[6] * d2;
+                    const double ey = cell.h[1] * d0 + cell.h[4] * d1 + cell.h[7] * d2;
+                    const double ez = cell.h[2] * d0 + cell.h[5] * d1 + cell.h[8] * d2;
+                    in = (ex * ex + ey * ey + ez * ez) <= cut2;
+                    code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
+                }
             }
             const unsigned long long m = __ballot(in) & gmask;
             if (in) {
@@ -379,16 +281,22 @@ __global__ void __launch_bounds__(512, 4)
         if (act && sub == 0) cnt[k] = (uint16_t)(c < cap ? c : cap);
     }
     __syncthreads();
+    for (int i = lane; i < N; i += nthr) {
+        sacc[i] = 0ull;
+        fxa[i] = 0ull;
+        fya[i] = 0ull;
+        fza[i] = 0ull;
+    }
+    __syncthreads();
 
-    // ---- phase 2: pass A ----
+    // ---- phase 2: pass A, sigma ----
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < N;
-        double sig = 0.0, ev = 0.0, fvx = 0.0, fvy = 0.0, fvz = 0.0;
-        int tk = 0;
+        double sig = 0.0;
         if (act) {
-            tk = typ[k];
+            const int tk = typ[k];
             const double xk = px[k], yk = py[k], zk = pz[k];
             const int ck = cnt[k];
             for (int e = sub; e < ck; e += TPA) {
@@ -398,65 +306,51 @@ __global__ void __launch_bounds__(512, 4)
                 const double dx = (xk - px[n]) - T.shift[3 * code];
                 const double dy = (yk - py[n]) - T.shift[3 * code + 1];
                 const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
-                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double r = r2 * mdt_rsqrt(r2);
                 if (r > cutr) continue;
                 const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
-                const double th = 1.0 / (1 + e1);
-                const double rt1 = acut * e1 * th;
-                const double fs = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
-                const double fv = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
-                const int kn = tk * MDT_MAXT + tn;
-                sig += T.csig[kn] * fs;
-                ev += T.wv[kn] * fv;
-                double cv;
-                if (tn == tk) {
-                    cv = T.wv[kn] * ((T.kdb[tn] + rt1) * fv);
-                } else {
-                    const double fv2 = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
-                    cv = 0.5 * (T.wv[kn] * ((T.kdb[tn] + rt1) * fv) +
-                                T.wv[tn * MDT_MAXT + tk] * ((T.kdb[tk] + rt1) * fv2));
-                }
-                const double cr = cv / r;
-                fvx -= cr * dx;
-                fvy -= cr * dy;
-                fvz -= cr * dz;
+                const double th = mdt_rcp(1 + e1);
+                const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+                const double fsk = (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+                sig += T.csig[tk * MDT_MAXT + tn] * fsn;
+                fx_add(&sacc[n], T.csig[tn * MDT_MAXT + tk] * fsk, MDT_SIG_SCALE, &bad_local);
             }
         }
         sig = group_sum<TPA>(sig);
-        ev = group_sum<TPA>(ev);
-        fvx = group_sum<TPA>(fvx);
-        fvy = group_sum<TPA>(fvy);
-        fvz = group_sum<TPA>(fvz);
-        if (act && sub == 0) {
-            double Gk = 0.0, Ek = 0.0;
-            if (T.owner[tk]) {
-                const double clamp30 = (double)1e-30f;  // pes_emt_mod.f90:793-794
-                const double sc = sig > clamp30 ? sig : clamp30;
-                const double sl = sc > MDT_TOLERANCE ? sc : MDT_TOLERANCE;
-                const double sv = -log(sl * MDT_TWELFTH) / T.betaeta2[tk];
-                const double dn = T.n0[tk] * exp(T.densc[tk] * sv);
-                const double el = exp(-T.lambda[tk] * sv);
-                const double ek = exp(-T.kappa[tk] * sv);
-                Ek = T.e0[tk] * ((1 + T.lambda[tk] * sv) * el - 1) + 6 * T.v0[tk] * ek - 0.5 * ev;
-                const double D = T.e0[tk] * T.lambda[tk] * T.lambda[tk] * (sv * el) + 6 * T.v0[tk] * T.kappa[tk] * ek;
-                Gk = -D / (T.betaeta2[tk] * sc);
-                bv.dens[doff + k] = dn;  // raw EMT density (pes_emt_mod.f90:855,861)
-            }
-            G[k] = Gk;
-            E[k] = Ek;
-            wx[k] = fvx;
-            wy[k] = fvy;
-            wz[k] = fvz;
-        }
+        if (act && sub == 0) G[k] = sig;  // own part; the partners' part is in sacc
     }
     __syncthreads();
 
-    // ---- phase 3: pass B, set_acceleration, propagate_2 ----
+    // ---- per atom: s, dens, E_coh + vref, G = dE/dsigma ----
+    for (int k = lane; k < N; k += nthr) {
+        const int tk = typ[k];
+        double Gk = 0.0, Ek = 0.0;
+        if (T.owner[tk]) {
+            const double sig = G[k] + (double)(long long)sacc[k] * MDT_SIG_ISCALE;
+            const double clamp30 = (double)1e-30f;  // pes_emt_mod.f90:793-794
+            const double sc = sig > clamp30 ? sig : clamp30;
+            const double sl = sc > MDT_TOLERANCE ? sc : MDT_TOLERANCE;
+            const double sv = -log(sl * MDT_TWELFTH) / T.betaeta2[tk];
+            const double dn = T.n0[tk] * exp(T.densc[tk] * sv);
+            const double el = exp(-T.lambda[tk] * sv);
+            const double ek = exp(-T.kappa[tk] * sv);
+            Ek = T.e0[tk] * ((1 + T.lambda[tk] * sv) * el - 1) + 6 * T.v0[tk] * ek;
+            const double D = T.e0[tk] * T.lambda[tk] * T.lambda[tk] * (sv * el) + 6 * T.v0[tk] * T.kappa[tk] * ek;
+            Gk = -D / (T.betaeta2[tk] * sc);
+            bv.dens[doff + k] = dn;  // raw EMT density (pes_emt_mod.f90:855,861)
+        }
+        G[k] = Gk;
+        E[k] = Ek;
+    }
+    __syncthreads();
+
+    // ---- phase 3: pass B, pair forces and pair energies ----
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < N;
-        double fx = 0.0, fy = 0.0, fz = 0.0;
+        double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
         if (act) {
             const int tk = typ[k];
             const double xk = px[k], yk = py[k], zk = pz[k];
@@ -469,57 +363,72 @@ __global__ void __launch_bounds__(512, 4)
                 const double dx = (xk - px[n]) - T.shift[3 * code];
                 const double dy = (yk - py[n]) - T.shift[3 * code + 1];
                 const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
-                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double ri = mdt_rsqrt(r2);
+                const double r = r2 * ri;
                 if (r > cutr) continue;
                 const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
-                const double th = 1.0 / (1 + e1);
+                const double th = mdt_rcp(1 + e1);
                 const double rt1 = acut * e1 * th;
                 const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+                const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
                 const int kn = tk * MDT_MAXT + tn, nk = tn * MDT_MAXT + tk;
-                const double cA = Gk * T.csig[kn] * ((T.etad[kn] + rt1) * fsn);
-                double cB;
-                if (tn == tk) {
-                    cB = G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsn);
-                } else {
-                    const double fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
-                    cB = G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsk);
+                double fsk = fsn, fvk = fvn;
+                if (tn != tk) {
+                    fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+                    fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
                 }
-                const double cr = (cA + cB) / r;
+                // dE/dr of this pair: density part through G, pair (V) part directly
+                const double c = Gk * T.csig[kn] * ((T.etad[kn] + rt1) * fsn) +
+                                 G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsk) +
+                                 0.5 * (T.wv[kn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[nk] * ((T.kdb[tk] + rt1) * fvk));
+                ep += T.wv[kn] * fvn + T.wv[nk] * fvk;
+                const double cr = c * ri;
                 fx -= cr * dx;
                 fy -= cr * dy;
                 fz -= cr * dz;
+                fx_add(&fxa[n], cr * dx, MDT_F_SCALE, &bad_local);
+                fx_add(&fya[n], cr * dy, MDT_F_SCALE, &bad_local);
+                fx_add(&fza[n], cr * dz, MDT_F_SCALE, &bad_local);
             }
         }
         fx = group_sum<TPA>(fx);
         fy = group_sum<TPA>(fy);
         fz = group_sum<TPA>(fz);
+        ep = group_sum<TPA>(ep);
         if (act && sub == 0) {
-            const int tk = typ[k];
-            double F[3] = {fx + wx[k], fy + wy[k], fz + wz[k]};
-            if (!T.owner[tk]) F[0] = F[1] = F[2] = 0.0;
-            if (add_fin) {
-                F[0] += bv.f[roff + k];
-                F[1] += bv.f[roff + N + k];
-                F[2] += bv.f[roff + 2 * N + k];
-            }
-            const uint8_t fb = bv.fixed[doff + k];
-            const double mass = bv.mass[aoff + k];
-            double acc[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) acc[c] = (fb >> c & 1) ? 0.0 : F[c] / mass;
-            bv.f[roff + k] = F[0];
-            bv.f[roff + N + k] = F[1];
-            bv.f[roff + 2 * N + k] = F[2];
-            bv.a[roff + k] = acc[0];
-            bv.a[roff + N + k] = acc[1];
-            bv.a[roff + 2 * N + k] = acc[2];
-            if (mode == MDT_MODE_STEP)
-                bad_local |= prop2_p1(pcg, T.algo[tk], T.is_proj[tk], tsurf, tproj, mass, fb, tid, step, k,
-                                      bv.v + roff + k, bv.a + roff + k, N, bv.dens + doff + k);
+            fx_add(&fxa[k], fx, MDT_F_SCALE, &bad_local);
+            fx_add(&fya[k], fy, MDT_F_SCALE, &bad_local);
+            fx_add(&fza[k], fz, MDT_F_SCALE, &bad_local);
+            E[k] -= 0.5 * ep;
         }
     }
-    // ---- energy of the replica: fixed-order reduction by wave 0 ----
     __syncthreads();
+
+    // ---- phase 4: set_acceleration (force.f90:89-98), outputs ----
+    for (int k = lane; k < N; k += nthr) {
+        const int tk = typ[k];
+        double F[3] = {(double)(long long)fxa[k] * MDT_F_ISCALE, (double)(long long)fya[k] * MDT_F_ISCALE,
+                       (double)(long long)fza[k] * MDT_F_ISCALE};
+        if (!T.owner[tk]) F[0] = F[1] = F[2] = 0.0;
+        if (add_fin) {
+            F[0] += bv.f[roff + k];
+            F[1] += bv.f[roff + N + k];
+            F[2] += bv.f[roff + 2 * N + k];
+        }
+        const uint8_t fb = bv.fixed[doff + k];
+        const double mass = bv.mass[aoff + k];
+        const double a0 = (fb & 1) ? 0.0 : F[0] / mass;
+        const double a1 = (fb & 2) ? 0.0 : F[1] / mass;
+        const double a2 = (fb & 4) ? 0.0 : F[2] / mass;
+        bv.f[roff + k] = F[0];
+        bv.f[roff + N + k] = F[1];
+        bv.f[roff + 2 * N + k] = F[2];
+        bv.a[roff + k] = a0;
+        bv.a[roff + N + k] = a1;
+        bv.a[roff + 2 * N + k] = a2;
+    }
+    // ---- energy of the replica: fixed-order reduction by wave 0 ----
     if (lane < 64) {
         double e = 0.0;
         for (int i = lane; i < N; i += 64) e += E[i];
@@ -819,33 +728,30 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
 // ---------------------------------------------------------------------------
 size_t mdt_replica_smem(int N, int cap, bool nbr_lds) {
     const int c = cap < N - 1 ? cap : (N > 1 ? N - 1 : 1);
-    return sizeof(double) * (8 * (size_t)N) + sizeof(uint16_t) * ((size_t)N + (nbr_lds ? (size_t)N * c : 0)) + N +
+    // px py pz | sx sy sz (= three accumulators) | fza | G | E: 9 N doubles
+    return sizeof(double) * (9 * (size_t)N) + sizeof(uint16_t) * ((size_t)N + (nbr_lds ? (size_t)N * c : 0)) + N +
            16;
 }
 
-template <bool L, int M>
+template <bool L>
 static hipError_t launch_replica_t(int tpa, dim3 g, dim3 blk, size_t smem, const EmtConst *ec, const PropConst *pc,
-                                   const BatchView &bv, uint32_t step, int add_fin, hipStream_t st) {
+                                   const BatchView &bv, int add_fin, hipStream_t st) {
     switch (tpa) {
-    case 1: hipLaunchKernelGGL((k_replica<1, L, M>), g, blk, smem, st, ec, pc, bv, step, add_fin); break;
-    case 2: hipLaunchKernelGGL((k_replica<2, L, M>), g, blk, smem, st, ec, pc, bv, step, add_fin); break;
-    case 4: hipLaunchKernelGGL((k_replica<4, L, M>), g, blk, smem, st, ec, pc, bv, step, add_fin); break;
-    case 8: hipLaunchKernelGGL((k_replica<8, L, M>), g, blk, smem, st, ec, pc, bv, step, add_fin); break;
+    case 1: hipLaunchKernelGGL((k_replica<1, L>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 2: hipLaunchKernelGGL((k_replica<2, L>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 4: hipLaunchKernelGGL((k_replica<4, L>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 8: hipLaunchKernelGGL((k_replica<8, L>), g, blk, smem, st, ec, pc, bv, add_fin); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-// ec and pc are device pointers; mode 0 = calc_force, 1 = fused MD step (P == 1)
+// calc_force on every replica; ec and pc are device pointers
 hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst *ec, const PropConst *pc,
-                              const BatchView &bv, uint32_t step, int mode, int add_fin, hipStream_t st) {
+                              const BatchView &bv, int add_fin, hipStream_t st) {
     dim3 g(nrep), blk(block);
-    const bool lds = bv.nbr_global == nullptr;
-    if (mode == MDT_MODE_STEP)
-        return lds ? launch_replica_t<true, MDT_MODE_STEP>(tpa, g, blk, smem, ec, pc, bv, step, add_fin, st)
-                   : launch_replica_t<false, MDT_MODE_STEP>(tpa, g, blk, smem, ec, pc, bv, step, add_fin, st);
-    return lds ? launch_replica_t<true, MDT_MODE_FORCE>(tpa, g, blk, smem, ec, pc, bv, step, add_fin, st)
-               : launch_replica_t<false, MDT_MODE_FORCE>(tpa, g, blk, smem, ec, pc, bv, step, add_fin, st);
+    if (bv.nbr_global == nullptr) return launch_replica_t<true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+    return launch_replica_t<false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
 }
 
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,

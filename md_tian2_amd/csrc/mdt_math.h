@@ -35,6 +35,35 @@
      1.8340080864093424, 1.8539791250833855, 1.8741676341103, 1.8945759815869656,            \
      1.9152065613971474, 1.9360617934922943, 1.9571441241754002, 1.978456026387951}
 
+#ifdef __HIPCC__
+#ifndef __HIP_DEVICE_COMPILE__
+// host pass of hipcc: declarations only, device code is not generated here
+__device__ double mdt_rcp(double x);
+__device__ double mdt_rsqrt(double x);
+#else
+// 1/x for finite x >= 1e-300: hardware reciprocal seed and two Newton steps
+// (the refinement LLVM itself uses inside its f64 division expansion).
+__device__ __forceinline__ double mdt_rcp(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
+
+// 1/sqrt(x) for finite x > 0: hardware seed and two Newton steps.
+__device__ __forceinline__ double mdt_rsqrt(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double h = x * y;
+    double e = fma(-h, y, 1.0);  // 1 - x y^2
+    y = fma(y * 0.5, e, y);
+    h = x * y;
+    e = fma(-h, y, 1.0);
+    return fma(y * 0.5, e, y);
+}
+#endif
+#endif
+
 // exp(x) for |x| < 700 with `tab` the 64 entries of MDT_EXP_TAB_INIT
 // (held in LDS on the device).
 MDT_MHD double mdt_exp(double x, const double *tab) {

@@ -27,7 +27,8 @@ SYMBOLS = (
     "mdt_emt_params_set", "mdt_compute_emt", "mdt_batch_create", "mdt_batch_destroy",
     "mdt_batch_set_atoms", "mdt_batch_set_cells", "mdt_batch_set_propagation", "mdt_batch_seed",
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
-    "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune",
+    "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
+    "mdt_batch_force_timing",
 )
 
 
@@ -62,6 +63,8 @@ def lib():
         L.mdt_batch_status.argtypes = [vp, ip, C.c_int]
         L.mdt_batch_last_timing.argtypes = [vp, dp, C.POINTER(C.c_int)]
         L.mdt_batch_tune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.mdt_batch_profile.argtypes = [vp, C.c_int]
+        L.mdt_batch_force_timing.argtypes = [vp, dp]
         _LIB = L
     return _LIB
 
@@ -193,6 +196,14 @@ class Batch:
         n = C.c_int(0)
         _check(lib().mdt_batch_last_timing(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    def profile(self, on=True):
+        _check(lib().mdt_batch_profile(self.h, 1 if on else 0))
+
+    def force_timing(self):
+        ms = C.c_double(0)
+        _check(lib().mdt_batch_force_timing(self.h, C.byref(ms)))
+        return ms.value
 
     def status(self, clear=False):
         st = np.zeros(len(self.systems), dtype=np.int32)
