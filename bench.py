@@ -102,7 +102,7 @@ def cpu_baseline(seconds=12.0, procs=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--traj", type=int, default=4096)
     ap.add_argument("--tpa", type=int, default=0)

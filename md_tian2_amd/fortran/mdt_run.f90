@@ -1,0 +1,326 @@
+! mdt_run.f90 -- batched MD driver on the GPU path (Fortran host).
+!
+! Replaces the trajectory loop of md_tian2 (src/md_tian2.f90:64-115) for
+! H-on-metal scattering: it reads an md_tian.inp-style deck, builds the
+! trajectories' initial conditions on the host, and runs all of them at
+! once through libmdt_gpu.so (calc_force at t = 0, then nsteps of
+! propagate_1 / calc_force / propagate_2).
+!
+! Keywords (md_tian.inp syntax, src/run_config.f90:224-670, where they
+! exist): ntrajs, start, nsteps, step, Einc, polar, azimuth (number or r),
+! pip (x y z; r = uniform over the cell), Tsurf, Tproj, pile_tau,
+! andersen_time, force_beads, projectile <n> <name> <mass> <algo>,
+! lattice <n> <name> <mass> <algo>, pes <file>.  Added for the batch:
+! slab <a> <nx> <ny> <nlayers> <nfixed> (fcc(111) slab generated here, as
+! the reference ships no metal slabs), rng_seed <n> (key of the noise
+! stream) and dump <file> (binary initial and final states).
+program mdt_run
+
+    use, intrinsic :: iso_c_binding
+    use mdt_gpu
+    use universe_mod
+    use pes_emt_gpu_mod, only : emt_gpu_init, emt_gpu_context, emt_params_from_file
+
+    implicit none
+
+    integer, parameter :: dp = c_double
+    real(dp), parameter :: pi = acos(-1.0_dp), kB = 8.61733238496e-5_dp, amu2mass = 103.638239276_dp
+
+    ! deck
+    integer :: ntrajs = 1, start = 1, nsteps = 0, nbeads = 1, nx = 3, ny = 3, nlayers = 4, nfixed = 2
+    integer :: rng_seed = 12345
+    real(dp) :: dt = 0.1_dp, einc = 1.0_dp, polar = 0.0_dp, tsurf = 0.0_dp, tproj = 0.0_dp
+    real(dp) :: pile_tau = 200.0_dp, andersen_time = 30.0_dp, alat = 4.078_dp, pip_z = 6.0_dp
+    real(dp) :: mproj = 1.00794_dp, mlatt = 196.966569_dp
+    character(len=8) :: algo_proj = "ver", algo_latt = "lan", azimuth = "0"
+    character(len=8) :: pip_x = "r", pip_y = "r"
+    character(len=3) :: name_proj = "H", name_latt = "Au"
+    character(len=512) :: pes_file = "", dump_file = "", deck = ""
+
+    type(universe) :: tmpl
+    type(c_ptr) :: ctx, batch
+    integer :: natoms, ntot, t, i, nl
+    integer(c_int32_t), allocatable :: nat(:), nbd(:), idx(:), fixed(:), tid(:), algo(:), isproj(:)
+    real(dp), allocatable :: mass(:), cell(:), icell(:), r(:), v(:), f(:), a(:), epot(:), dens(:)
+    real(dp), allocatable :: ts(:), tp(:), r0(:), v0(:)
+    real(dp), dimension(2) :: eta2, n0, e0, lambda, v0p, kappa, s0
+    real(dp) :: ms
+    integer(c_int) :: tpa
+    integer :: u
+
+    if (command_argument_count() < 1) then
+        print '(a)', "usage: mdt_run <deck>"
+        stop 2
+    end if
+    call get_command_argument(1, deck)
+    call read_deck(trim(deck))
+
+    natoms = 1 + nx * ny * nlayers
+    tmpl = universe_alloc(nbeads, natoms, 2)
+    call build_template(tmpl)
+    call emt_gpu_init(0)
+    ctx = emt_gpu_context()
+    call emt_params_from_file(tmpl, trim(pes_file), eta2, n0, e0, lambda, v0p, kappa, s0)
+    call mdt_check(mdt_emt_params_set(ctx, 2_c_int, eta2, n0, e0, lambda, v0p, kappa, s0, &
+                                      int(tmpl%pes, c_int32_t)), "mdt_emt_params_set")
+
+    ntot = 3 * nbeads * natoms
+    allocate(nat(ntrajs), nbd(ntrajs), tid(ntrajs), ts(ntrajs), tp(ntrajs))
+    allocate(idx(natoms * ntrajs), mass(natoms * ntrajs), fixed(ntot * ntrajs))
+    allocate(cell(9 * ntrajs), icell(9 * ntrajs), r(ntot * ntrajs), v(ntot * ntrajs))
+    allocate(f(ntot * ntrajs), a(ntot * ntrajs), epot(nbeads * ntrajs), dens(nbeads * natoms * ntrajs))
+    nat = natoms
+    nbd = nbeads
+    ts = tsurf
+    tp = tproj
+    do t = 1, ntrajs
+        tid(t) = int(start + t - 1, c_int32_t)
+        call initial_conditions(tmpl, start + t - 1)
+        idx((t - 1) * natoms + 1 : t * natoms) = int(tmpl%idx, c_int32_t)
+        mass((t - 1) * natoms + 1 : t * natoms) = tmpl%m
+        fixed((t - 1) * ntot + 1 : t * ntot) = reshape(merge(1_c_int32_t, 0_c_int32_t, tmpl%is_fixed), [ntot])
+        cell((t - 1) * 9 + 1 : t * 9) = reshape(tmpl%simbox, [9])
+        icell((t - 1) * 9 + 1 : t * 9) = reshape(tmpl%isimbox, [9])
+        r((t - 1) * ntot + 1 : t * ntot) = reshape(tmpl%r, [ntot])
+        v((t - 1) * ntot + 1 : t * ntot) = reshape(tmpl%v, [ntot])
+    end do
+    r0 = r
+    v0 = v
+    allocate(algo(2), isproj(2))
+    algo = [algo_id(algo_proj), algo_id(algo_latt)]
+    isproj = [1_c_int32_t, 0_c_int32_t]
+
+    call mdt_check(mdt_batch_create(ctx, int(ntrajs, c_int), nat, nbd, batch), "mdt_batch_create")
+    call mdt_check(mdt_batch_set_atoms(batch, idx, mass, fixed), "mdt_batch_set_atoms")
+    call mdt_check(mdt_batch_set_cells(batch, cell, icell), "mdt_batch_set_cells")
+    call mdt_check(mdt_batch_set_propagation(batch, 2_c_int, algo, isproj, dt, ts, tp, pile_tau, andersen_time), &
+                   "mdt_batch_set_propagation")
+    call mdt_check(mdt_batch_seed(batch, int(rng_seed, c_int32_t), tid), "mdt_batch_seed")
+    call mdt_check(mdt_batch_set_state(batch, r, v), "mdt_batch_set_state")
+    call mdt_check(mdt_batch_tune(batch, 0_c_int, 0_c_int, tpa), "mdt_batch_tune")
+    call mdt_check(mdt_batch_compute_forces(batch), "mdt_batch_compute_forces")
+    call mdt_check(mdt_batch_step(batch, int(nsteps, c_int)), "mdt_batch_step")
+    call mdt_check(mdt_batch_last_timing(batch, ms, nl), "mdt_batch_last_timing")
+    call mdt_check(mdt_batch_get_state(batch, r, v, f, a, epot, dens), "mdt_batch_get_state")
+
+    print '(a,i0,a,i0,a,i0,a,f10.3,a,es12.4,a)', "mdt_run: ", ntrajs, " trajectories x ", nsteps, &
+        " steps (tpa ", tpa, ") in ", ms, " ms: ", real(ntrajs, dp) * nsteps / max(ms, 1e-9_dp) * 1e3_dp, &
+        " trajectory-steps/s"
+    do t = 1, min(ntrajs, 4)
+        print '(a,i0,a,3f12.6,a,f14.8)', "  traj ", tid(t), "  r_proj ", r((t - 1) * ntot + 1 : (t - 1) * ntot + 3), &
+            "  epot ", epot((t - 1) * nbeads + 1)
+    end do
+
+    if (len_trim(dump_file) > 0) then
+        open(newunit=u, file=trim(dump_file), access="stream", form="unformatted", status="replace")
+        write(u) int(ntrajs, c_int32_t), int(natoms, c_int32_t), int(nbeads, c_int32_t), int(nsteps, c_int32_t)
+        write(u) tid, r0, v0, r, v, epot, cell
+        close(u)
+    end if
+    call mdt_check(mdt_batch_destroy(batch), "mdt_batch_destroy")
+
+contains
+
+    integer(c_int32_t) function algo_id(s)
+        character(len=*), intent(in) :: s
+        select case (trim(s))
+        case ("ver"); algo_id = MDT_PROP_VERLET
+        case ("lan"); algo_id = MDT_PROP_LANGEVIN
+        case ("and"); algo_id = MDT_PROP_ANDERSEN
+        case ("pil"); algo_id = MDT_PROP_PILE
+        case default
+            print '(a,a)', "Error: unknown propagation algorithm ", trim(s)
+            stop 1
+        end select
+    end function algo_id
+
+    subroutine read_deck(path)
+        character(len=*), intent(in) :: path
+        character(len=512) :: line, key
+        character(len=512) :: w(8)
+        integer :: unit, ios, nw, n
+        open(newunit=unit, file=path, status="old", action="read", iostat=ios)
+        if (ios /= 0) then
+            print '(a,a)', "Error: cannot open deck ", path
+            stop 1
+        end if
+        do
+            read(unit, '(a)', iostat=ios) line
+            if (ios /= 0) exit
+            n = index(line, "!")
+            if (n > 0) line = line(1:n - 1)
+            call words(line, w, nw)
+            if (nw < 2) cycle
+            key = w(1)
+            select case (trim(key))
+            case ("ntrajs"); read(w(2), *) ntrajs
+            case ("start"); read(w(2), *) start
+            case ("nsteps"); read(w(2), *) nsteps
+            case ("step"); read(w(2), *) dt
+            case ("Einc"); read(w(2), *) einc
+            case ("polar"); read(w(2), *) polar; polar = polar * pi / 180.0_dp
+            case ("azimuth"); azimuth = w(2)
+            case ("pip"); pip_x = w(2); pip_y = w(3); read(w(4), *) pip_z
+            case ("Tsurf"); read(w(2), *) tsurf
+            case ("Tproj"); read(w(2), *) tproj
+            case ("pile_tau"); read(w(2), *) pile_tau
+            case ("andersen_time"); read(w(2), *) andersen_time
+            case ("force_beads"); read(w(2), *) nbeads
+            case ("projectile"); name_proj = w(3); read(w(4), *) mproj; algo_proj = w(5)
+            case ("lattice"); name_latt = w(3); read(w(4), *) mlatt; algo_latt = w(5)
+            case ("pes"); pes_file = unquote(w(2))
+            case ("slab"); read(w(2), *) alat; read(w(3), *) nx; read(w(4), *) ny
+                read(w(5), *) nlayers; read(w(6), *) nfixed
+            case ("rng_seed"); read(w(2), *) rng_seed
+            case ("dump"); dump_file = unquote(w(2))
+            case default
+                continue  ! unknown keywords are skipped, as src/run_config.f90:667-669
+            end select
+        end do
+        close(unit)
+    end subroutine read_deck
+
+    function unquote(s) result(t)
+        character(len=*), intent(in) :: s
+        character(len=len(s)) :: t
+        t = s
+        if (len_trim(t) >= 2) then
+            if (t(1:1) == "'" .or. t(1:1) == '"') t = t(2:len_trim(t) - 1)
+        end if
+    end function unquote
+
+    subroutine words(line, w, nw)
+        character(len=*), intent(in) :: line
+        character(len=512), intent(out) :: w(:)
+        integer, intent(out) :: nw
+        integer :: i, st
+        logical :: inw
+        nw = 0
+        inw = .false.
+        st = 1
+        do i = 1, len_trim(line) + 1
+            if (i <= len_trim(line)) then
+                if (line(i:i) /= " " .and. iachar(line(i:i)) /= 9) then
+                    if (.not. inw) then
+                        inw = .true.
+                        st = i
+                    end if
+                    cycle
+                end if
+            end if
+            if (inw) then
+                inw = .false.
+                if (nw < size(w)) then
+                    nw = nw + 1
+                    w(nw) = line(st:i - 1)
+                end if
+            end if
+        end do
+    end subroutine words
+
+    ! fcc(111) slab: ABC stacking, layers a/sqrt(3) apart from z = 0 down,
+    ! in-plane cell nx a/sqrt2 (1,0,0) and ny a/sqrt2 (1/2, sqrt3/2, 0),
+    ! c = 40 A; projectile first, bottom nfixed layers frozen.
+    subroutine build_template(u)
+        type(universe), intent(inout) :: u
+        real(dp) :: d, dz, u1(3), u2(3), sh(3), p(3), fr(3)
+        integer :: l, i, j, k
+        d = alat / sqrt(2.0_dp)
+        dz = alat / sqrt(3.0_dp)
+        u1 = [d, 0.0_dp, 0.0_dp]
+        u2 = [0.5_dp * d, 0.5_dp * sqrt(3.0_dp) * d, 0.0_dp]
+        sh = (u1 + u2) / 3.0_dp
+        u%simbox = 0.0_dp
+        u%simbox(:, 1) = nx * u1
+        u%simbox(:, 2) = ny * u2
+        u%simbox(3, 3) = 40.0_dp
+        u%isimbox = invert3(u%simbox)
+        u%name = [name_proj, name_latt]
+        u%is_proj = [.true., .false.]
+        u%pes = 0
+        u%idx(1) = 1
+        u%m(1) = mproj * amu2mass
+        k = 1
+        do l = 0, nlayers - 1
+            do j = 0, ny - 1
+                do i = 0, nx - 1
+                    k = k + 1
+                    p = i * u1 + j * u2 + mod(l, 3) * sh
+                    p(3) = -l * dz
+                    fr = matmul(u%isimbox, p)
+                    fr(1:2) = fr(1:2) - floor(fr(1:2))
+                    u%r(:, 1, k) = matmul(u%simbox, fr)
+                    u%idx(k) = 2
+                    u%m(k) = mlatt * amu2mass
+                    u%is_fixed(:, :, k) = (l >= nlayers - nfixed)
+                end do
+            end do
+        end do
+    end subroutine build_template
+
+    function invert3(m) result(inv)
+        real(dp), intent(in) :: m(3,3)
+        real(dp) :: inv(3,3), det
+        det = m(1,1) * (m(2,2) * m(3,3) - m(2,3) * m(3,2)) - m(1,2) * (m(2,1) * m(3,3) - m(2,3) * m(3,1)) &
+            + m(1,3) * (m(2,1) * m(3,2) - m(2,2) * m(3,1))
+        inv(1,1) = (m(2,2) * m(3,3) - m(2,3) * m(3,2)) / det
+        inv(1,2) = (m(1,3) * m(3,2) - m(1,2) * m(3,3)) / det
+        inv(1,3) = (m(1,2) * m(2,3) - m(1,3) * m(2,2)) / det
+        inv(2,1) = (m(2,3) * m(3,1) - m(2,1) * m(3,3)) / det
+        inv(2,2) = (m(1,1) * m(3,3) - m(1,3) * m(3,1)) / det
+        inv(2,3) = (m(1,3) * m(2,1) - m(1,1) * m(2,3)) / det
+        inv(3,1) = (m(2,1) * m(3,2) - m(2,2) * m(3,1)) / det
+        inv(3,2) = (m(1,2) * m(3,1) - m(1,1) * m(3,2)) / det
+        inv(3,3) = (m(1,1) * m(2,2) - m(1,2) * m(2,1)) / det
+    end function invert3
+
+    ! Trajectory initial conditions: projectile at (pip x, pip y, z) with
+    ! v = sqrt(2 E/m)(sin t cos p, sin t sin p, -cos t) (src/md_init.f90:829-877),
+    ! mobile slab atoms Maxwell-Boltzmann at Tsurf; host RNG seeded per
+    ! trajectory id.
+    subroutine initial_conditions(u, traj)
+        type(universe), intent(inout) :: u
+        integer, intent(in) :: traj
+        integer :: nseed, k, c
+        integer, allocatable :: seedv(:)
+        real(dp) :: fx, fy, phi, vinc, sig, u1, u2
+        call build_template(u)
+        call random_seed(size=nseed)
+        allocate(seedv(nseed))
+        seedv = traj * 7919 + 17
+        call random_seed(put=seedv)
+        call random_number(fx)
+        call random_number(fy)
+        if (trim(pip_x) /= "r") read(pip_x, *) fx
+        if (trim(pip_y) /= "r") read(pip_y, *) fy
+        u%r(:, :, 1) = spread(u%simbox(:, 1) * fx + u%simbox(:, 2) * fy, 2, nbeads)
+        u%r(3, :, 1) = pip_z
+        if (trim(azimuth) == "r") then
+            call random_number(phi)
+            phi = 2.0_dp * pi * phi
+        else
+            read(azimuth, *) phi
+            phi = phi * pi / 180.0_dp
+        end if
+        vinc = sqrt(2.0_dp * einc / u%m(1))
+        u%v = 0.0_dp
+        u%v(1, :, 1) = vinc * sin(polar) * cos(phi)
+        u%v(2, :, 1) = vinc * sin(polar) * sin(phi)
+        u%v(3, :, 1) = -vinc * cos(polar)
+        if (tsurf > 0.0_dp) then
+            do k = 2, u%natoms
+                sig = sqrt(kB * tsurf / u%m(k))
+                do c = 1, 3
+                    call random_number(u1)
+                    call random_number(u2)
+                    if (.not. u%is_fixed(c, 1, k)) u%v(c, :, k) = sig * sqrt(-2.0_dp * log(1.0_dp - u1)) &
+                                                                  * cos(2.0_dp * pi * u2)
+                end do
+            end do
+        end if
+        do k = 2, u%natoms
+            u%r(:, :, k) = spread(u%r(:, 1, k), 2, nbeads)
+        end do
+    end subroutine initial_conditions
+
+end program mdt_run
