@@ -68,6 +68,24 @@ def emt_cutoff(pes):
     return 1.5 * beta * s0 * math.sqrt(3.0)
 
 
+def read_traffic(ntraj):
+    """HBM bytes per k_replica launch from the committed rocprofv3 --pmc
+    summary of this workload (None when absent or for another batch size)."""
+    path = os.path.join(ROOT, "profiles", "traffic_k_replica_h_au_6x6x4_4096traj.txt")
+    if ntraj != 4096 or not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 2 and not line.startswith("#"):
+            kv[parts[0]] = parts[1]
+    try:
+        kib = float(kv["fetch_kib"]) / float(kv["fetch_factor"]) + float(kv["write_kib"]) / float(kv["write_factor"])
+    except (KeyError, ValueError):
+        return None
+    return kib * 1024.0
+
+
 def _cpu_worker(args):
     n_steps, seed = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -175,7 +193,7 @@ def main():
         "roofline": {
             "bound": "mfma", "pipe": "fp64 vector ALU (MI355X FP64 vector peak = FP64 matrix peak)",
             "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+            "frac": achieved / PEAK_FP64_TFLOPS, "traffic": read_traffic(len(systems)),
             "kernel": "k_replica (calc_force: min-image lists + EMT density + forces)",
             "flop_per_traj_step": flop_traj, "avg_launch_ms": launch_s * 1e3,
         },

@@ -614,9 +614,9 @@ static int configure(mdt_batch *b, int tpa, int nbcap) {
         b->lds_limit = (size_t)prop.sharedMemPerBlock;
     }
     // capacity of the owned half lists: every other atom for small replicas,
-    // else 112 (fcc within 1.5 rcut holds 134 neighbours, of which an atom
+    // else 96 (fcc within 1.5 rcut holds 134 neighbours, of which an atom
     // owns about half; overflow is detected and reported)
-    int cap = nbcap > 0 ? nbcap : 112;
+    int cap = nbcap > 0 ? nbcap : 96;
     if (cap > b->maxN - 1) cap = std::max(1, b->maxN - 1);
     const size_t smem_lds = mdt_replica_smem(b->maxN, cap, true);
     const bool in_lds = smem_lds <= b->lds_limit;

@@ -157,10 +157,37 @@ __device__ void load_tables(LdsTables &T, const EmtConst *__restrict__ ec, const
 // Fixed-point scales of the LDS accumulators that receive pair
 // contributions from other lanes: integer addition is associative, so the
 // totals do not depend on the order in which lanes add them.
-#define MDT_SIG_SCALE 281474976710656.0      // 2^48: sigma up to 3.2e4, step 3.6e-15
-#define MDT_SIG_ISCALE (1.0 / 281474976710656.0)
+#define MDT_W48 281474976710656.0            // 2^48
+#define MDT_IW48 (1.0 / 281474976710656.0)
 #define MDT_F_SCALE 4398046511104.0          // 2^42: force up to 2.1e6 eV/A, step 2.3e-13
 #define MDT_F_ISCALE (1.0 / 4398046511104.0)
+
+// sigma contributions span ~1e3 (close contact) down to ~1e-35 (pairs at the
+// cut-off) and an isolated atom's sigma sits near the reference's 1e-30
+// clamp, where G = dE/dsigma ~ 1/sigma: a sum with relative precision at
+// any magnitude is needed.  Three int64 words at 2^-48, 2^-96 and 2^-144
+// hold each contribution exactly down to 4.5e-44 (each conversion step is an
+// exact power-of-two scaling and an exact subtraction of the rounded part).
+__device__ __forceinline__ void sig_add(unsigned long long *hi, unsigned long long *mid, unsigned long long *lo,
+                                        double x, int *bad) {
+    const double y = x * MDT_W48;
+    if (!(fabs(y) < 9.0e18)) {
+        *bad |= MDT_ST_FIXED_OVERFLOW;
+        return;
+    }
+    const double h = rint(y);
+    const double y1 = (y - h) * MDT_W48;
+    const double m = rint(y1);
+    const double l = rint((y1 - m) * MDT_W48);
+    atomicAdd(hi, (unsigned long long)(long long)h);
+    if (m != 0.0) atomicAdd(mid, (unsigned long long)(long long)m);
+    if (l != 0.0) atomicAdd(lo, (unsigned long long)(long long)l);
+}
+
+__device__ __forceinline__ double sig_value(unsigned long long hi, unsigned long long mid, unsigned long long lo) {
+    return (((double)(long long)lo * MDT_IW48 + (double)(long long)mid) * MDT_IW48 + (double)(long long)hi) *
+           MDT_IW48;
+}
 
 __device__ __forceinline__ void fx_add(unsigned long long *acc, double x, double scale, int *bad) {
     const double y = x * scale;
@@ -201,14 +228,17 @@ __global__ void __launch_bounds__(512, 4)
     load_cell(bv, t, cell);
     load_tables(T, ecg, pcg, cell, lane, nthr);
 
-    // LDS: px py pz | sx sy sz (scan) = sacc fxa fya (accumulators) | fza | G | E | cnt | nbr | typ
+    // LDS: px py pz | three words A0 A1 A2 | G | E | cnt | nbr | typ, where the
+    // words hold the fractional coordinates in the scan, the sigma fixed
+    // point (hi, mid, lo) in pass A and the force fixed point (x, y, z) in
+    // pass B.
     double *px = (double *)smem_raw;
     double *py = px + N, *pz = py + N;
     double *sx = pz + N, *sy = sx + N, *sz = sy + N;
-    unsigned long long *sacc = (unsigned long long *)sx;
-    unsigned long long *fxa = (unsigned long long *)sy, *fya = (unsigned long long *)sz;
-    unsigned long long *fza = (unsigned long long *)(sz + N);
-    double *G = (double *)(fza + N), *E = G + N;
+    unsigned long long *shi = (unsigned long long *)sx, *smid = (unsigned long long *)sy;
+    unsigned long long *slo = (unsigned long long *)sz;
+    unsigned long long *fxa = shi, *fya = smid, *fza = slo;
+    double *G = sz + N, *E = G + N;
     uint16_t *cnt = (uint16_t *)(E + N);
     uint16_t *nbr = NBR_LDS ? cnt + N : bv.nbr_global + (size_t)rep * bv.nbr_stride;
     uint8_t *typ = (uint8_t *)(cnt + N + (NBR_LDS ? (size_t)N * cap : 0));
@@ -282,10 +312,9 @@ __global__ void __launch_bounds__(512, 4)
     }
     __syncthreads();
     for (int i = lane; i < N; i += nthr) {
-        sacc[i] = 0ull;
-        fxa[i] = 0ull;
-        fya[i] = 0ull;
-        fza[i] = 0ull;
+        shi[i] = 0ull;
+        smid[i] = 0ull;
+        slo[i] = 0ull;
     }
     __syncthreads();
 
@@ -314,11 +343,11 @@ __global__ void __launch_bounds__(512, 4)
                 const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
                 const double fsk = (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
                 sig += T.csig[tk * MDT_MAXT + tn] * fsn;
-                fx_add(&sacc[n], T.csig[tn * MDT_MAXT + tk] * fsk, MDT_SIG_SCALE, &bad_local);
+                sig_add(&shi[n], &smid[n], &slo[n], T.csig[tn * MDT_MAXT + tk] * fsk, &bad_local);
             }
         }
         sig = group_sum<TPA>(sig);
-        if (act && sub == 0) G[k] = sig;  // own part; the partners' part is in sacc
+        if (act && sub == 0) G[k] = sig;  // own part; the partners' part is in shi/smid/slo
     }
     __syncthreads();
 
@@ -327,7 +356,7 @@ __global__ void __launch_bounds__(512, 4)
         const int tk = typ[k];
         double Gk = 0.0, Ek = 0.0;
         if (T.owner[tk]) {
-            const double sig = G[k] + (double)(long long)sacc[k] * MDT_SIG_ISCALE;
+            const double sig = G[k] + sig_value(shi[k], smid[k], slo[k]);
             const double clamp30 = (double)1e-30f;  // pes_emt_mod.f90:793-794
             const double sc = sig > clamp30 ? sig : clamp30;
             const double sl = sc > MDT_TOLERANCE ? sc : MDT_TOLERANCE;
@@ -342,6 +371,12 @@ __global__ void __launch_bounds__(512, 4)
         }
         G[k] = Gk;
         E[k] = Ek;
+    }
+    __syncthreads();
+    for (int i = lane; i < N; i += nthr) {
+        fxa[i] = 0ull;
+        fya[i] = 0ull;
+        fza[i] = 0ull;
     }
     __syncthreads();
 
@@ -728,8 +763,8 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
 // ---------------------------------------------------------------------------
 size_t mdt_replica_smem(int N, int cap, bool nbr_lds) {
     const int c = cap < N - 1 ? cap : (N > 1 ? N - 1 : 1);
-    // px py pz | sx sy sz (= three accumulators) | fza | G | E: 9 N doubles
-    return sizeof(double) * (9 * (size_t)N) + sizeof(uint16_t) * ((size_t)N + (nbr_lds ? (size_t)N * c : 0)) + N +
+    // px py pz | A0 A1 A2 | G | E: 8 N doubles
+    return sizeof(double) * (8 * (size_t)N) + sizeof(uint16_t) * ((size_t)N + (nbr_lds ? (size_t)N * c : 0)) + N +
            16;
 }
 
