@@ -117,8 +117,30 @@ def cpu_baseline(seconds=12.0, procs=None):
     return steps / wall, procs, per, single
 
 
+def copy_calibration():
+    """Known-size device copies for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE:
+    a 1 GiB FP64 source read by a blit copy and by an elementwise kernel,
+    each after a 1 GiB fill of another buffer has evicted the source from
+    L2 and the 256 MB infinity cache."""
+    import torch
+
+    n = (1 << 30) // 8
+    src = torch.empty(n, dtype=torch.float64, device="cuda:0").uniform_()
+    dst = torch.empty_like(src)
+    evict = torch.empty_like(src)
+    for _ in range(3):
+        evict.fill_(1.0)
+        dst.copy_(src)
+        evict.fill_(2.0)
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes_read": n * 8, "bytes_written": n * 8}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only the known-size copies used to scale PMC byte counters")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
@@ -131,6 +153,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "described in DESIGN.md and not launched"}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     from md_tian2_amd import mdt, pesfile
 
@@ -139,23 +164,36 @@ def main():
     cols, pesm = pesfile.emt_tables(pes, systems[0].type_names)
     ctx = mdt.Context(0)
     ctx.set_emt(cols, pesm)
-    b = mdt.Batch(ctx, systems, dt=0.1, tsurf=300.0, tproj=0.0, seed=12345)
-    tpa = b.tune(tpa=args.tpa)
-    b.profile(True)  # HIP events around every force launch on the batch's stream
-    b.compute_forces()
-    if args.warmup:
-        b.step(args.warmup)
-    print(f"[bench] warmup done ({args.warmup} steps, tpa={tpa})", file=sys.stderr, flush=True)
-    t0 = time.perf_counter()
-    b.step(args.steps)  # synchronises on its end event
-    wall = time.perf_counter() - t0
-    ms_total, nlaunch = b.timing()
-    force_ms = b.force_timing()
-    st = b.status()
-    assert not st.any(), "trajectory failure flags set"
-    state = b.get_state()
-    print(f"[bench] timed {args.steps} steps: wall {wall:.3f}s events {ms_total:.3f}ms", file=sys.stderr,
-          flush=True)
+
+    def run(precision, graph, profile):
+        """W warmup steps, then exactly K timed steps (HIP events on the
+        batch's stream bracket them; the call synchronises on its end event)."""
+        b = mdt.Batch(ctx, systems, dt=0.1, tsurf=300.0, tproj=0.0, seed=12345)
+        b.set_precision(precision)
+        tpa_ = b.tune(tpa=args.tpa)
+        b.profile(profile)
+        b.set_graph(graph)
+        b.compute_forces()
+        if args.warmup:
+            b.step(args.warmup)
+        t0 = time.perf_counter()
+        b.step(args.steps)
+        wall_ = time.perf_counter() - t0
+        ms_, nl_ = b.timing()
+        fms = b.force_timing() if profile else None
+        assert not b.status().any(), "trajectory failure flags set"
+        st_ = b.get_state() if profile else None
+        b.close()
+        print(f"[bench] precision={precision} graph={graph} tpa={tpa_}: {args.steps} steps, "
+              f"events {ms_:.3f} ms, wall {wall_:.3f} s", file=sys.stderr, flush=True)
+        return dict(ms=ms_, nlaunch=nl_, force_ms=fms, state=st_, tpa=tpa_)
+
+    # FP64 with an event pair around every force launch: value and roofline
+    main_run = run(0, 0, True)
+    graph_run = run(0, 50, False)
+    f32_run = run(1, 50, False)
+    f32_prof = run(1, 0, True)
+    ms_total, nlaunch, force_ms, state, tpa = (main_run[k] for k in ("ms", "nlaunch", "force_ms", "state", "tpa"))
 
     # algorithmic work from the kernel's own geometry at the end of timing
     cut = emt_cutoff(pes)
@@ -170,6 +208,7 @@ def main():
     launch_s = force_ms / 1e3 / max(1, nlaunch)
     achieved = flop_traj * len(systems) / launch_s / 1e12
     value = len(systems) * args.steps / (ms_total / 1e3)
+    f32_launch_s = f32_prof["force_ms"] / 1e3 / max(1, f32_prof["nlaunch"])
 
     out = {
         "metric": METRIC,
@@ -197,6 +236,20 @@ def main():
             "kernel": "k_replica (calc_force: min-image lists + EMT density + forces)",
             "flop_per_traj_step": flop_traj, "avg_launch_ms": launch_s * 1e3,
         },
+        "variants": {
+            "fp64_hipgraph": {
+                "value": len(systems) * args.steps / (graph_run["ms"] / 1e3),
+                "ms_per_step": graph_run["ms"] / args.steps, "graph_chunk": 50,
+            },
+            "fp32_pair_hipgraph": {
+                "value": len(systems) * args.steps / (f32_run["ms"] / 1e3),
+                "ms_per_step": f32_run["ms"] / args.steps, "graph_chunk": 50,
+                "precision": "FP32 pair terms, FP64 accumulation and integration",
+                "avg_force_launch_ms": f32_launch_s * 1e3,
+                "achieved_tflops": flop_traj * len(systems) / f32_launch_s / 1e12,
+                "error_bound": "<= 1e-6 eV/atom, <= 1e-5 eV/A vs the FP64 oracle (tests/test_gpu_precision.py)",
+            },
+        },
     }
     if not args.no_cpu:
         v, procs, per, single = cpu_baseline(args.cpu_seconds)
@@ -207,7 +260,6 @@ def main():
             "single_core": single,
         }
         out["gpu_over_cpu"] = value / v
-    b.close()
     ctx.close()
     print(json.dumps(out), flush=True)
 

@@ -178,6 +178,29 @@ int mdt_batch_force_timing(mdt_batch *b, double *force_ms_total);
  */
 int mdt_batch_tune(mdt_batch *b, int tpa, int nbcap, int *tpa_chosen);
 
+/*
+ * Arithmetic of the EMT pair terms in compute_emt_2species
+ * (src/pes_emt_mod.f90:646-763, 793-842, 871-972), which the reference
+ * evaluates in FP64:
+ *   mode 0: FP64 throughout (the reference's arithmetic; default);
+ *   mode 1: the cut-off function, the sigma / pair-potential exponentials
+ *           and their products in FP32, with r, the exponent arguments, the
+ *           per-atom embedding terms and all accumulations (sigma, energy,
+ *           forces) in FP64 or exact fixed point.  Integration, LDFA and the
+ *           normal-mode transform are FP64 in both modes.
+ */
+int mdt_batch_set_precision(mdt_batch *b, int mode);
+
+/*
+ * Step-loop launches through a hipGraph: with chunk > 0, mdt_batch_step
+ * replays a graph of `chunk` steps (captured on first use, recaptured after
+ * mdt_batch_tune or mdt_batch_set_precision) and launches the remaining
+ * nsteps % chunk steps directly; results are identical to chunk = 0 (direct
+ * launches, the default).  Profiling (mdt_batch_profile) forces direct
+ * launches.  Replaces the per-step calls of src/md_tian2.f90:78-98.
+ */
+int mdt_batch_set_graph(mdt_batch *b, int chunk);
+
 #ifdef __cplusplus
 }
 #endif

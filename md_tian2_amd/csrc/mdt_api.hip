@@ -17,7 +17,9 @@ size_t mdt_replica_smem(int N, int cap, bool nbr_lds);
 hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst *ec, const PropConst *pc,
                               const BatchView &bv, int add_fin, hipStream_t st);
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
-                           const long long *cjk_off, uint32_t step_prev, uint32_t step, int flags, hipStream_t st);
+                           const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
+                           int flags, hipStream_t st);
+hipError_t mdt_launch_advance(uint32_t *step_base, uint32_t n, int set, hipStream_t st);
 
 namespace {
 
@@ -101,12 +103,21 @@ struct mdt_batch {
     int profile = 0;                // record events around every force launch
     std::vector<hipEvent_t> evf;    // start/stop pairs, one per step
     double last_force_ms = 0.0;     // summed force-kernel time of the last call
+    int precision = 0;              // 0: FP64; 1: FP32 pair terms, FP64 accumulation
+    // hipGraph replay of the step loop: a chunk of graph_chunk steps is
+    // captured once per launch configuration; the step counter lives on the
+    // device (d_step) and the chunk's last node advances it
+    int graph_chunk = 0;
+    hipGraphExec_t gexec = nullptr;
+    uint32_t *d_step = nullptr;
+    int graph_replays = 0;          // replays of the last call
 
     BatchView view() const {
         BatchView v{};
         v.ntraj = ntraj;
         v.nrep = nrep;
         v.nbcap = nbcap;
+        v.precision = precision;
         v.rep_traj = d_rep_traj;
         v.rep_bead = d_rep_bead;
         v.traj_N = d_N;
@@ -309,9 +320,10 @@ int mdt_batch_destroy(mdt_batch *b) {
     void *ptrs[] = {b->d_rep_traj, b->d_rep_bead, b->d_N, b->d_P, b->d_soff, b->d_aoff, b->d_eoff,
                     b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
-                    b->d_cjk, b->d_status, b->d_nbr, b->d_pc};
+                    b->d_cjk, b->d_status, b->d_nbr, b->d_pc, b->d_step};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     for (hipEvent_t e : b->evf) (void)hipEventDestroy(e);
@@ -607,7 +619,13 @@ static int ensure_ready(mdt_batch *b, bool need_prop) {
     return push_pc(b);
 }
 
+static void drop_graph(mdt_batch *b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+}
+
 static int configure(mdt_batch *b, int tpa, int nbcap) {
+    drop_graph(b);  // launch shapes and the neighbour buffer may change
     if (b->lds_limit == 0) {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, b->ctx->device));
@@ -647,7 +665,8 @@ static int launch_force(mdt_batch *b) {
     const BatchView bv = b->view();
     if (!b->allP1) {
         const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
-        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 4, c->stream));
+        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, nullptr, b->step, b->step, 4,
+                               c->stream));
     }
     HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->allP1 ? 0 : 1,
                               c->stream));
@@ -733,6 +752,62 @@ int mdt_batch_compute_forces(mdt_batch *b) {
     return check_status(b);
 }
 
+// Launches of `nsteps` steps on the batch's stream.  One step: [propagate_2
+// of the previous step +] propagate_1 (+ ring-polymer step when P > 1) per
+// trajectory, then calc_force per replica; a closing propagate_2 ends the
+// sequence so that the host sees the reference's state.  step_base != NULL
+// makes the step numbers offsets from the device counter (graph capture).
+static int enqueue_steps(mdt_batch *b, int nsteps, const uint32_t *step_base, uint32_t first, bool events) {
+    mdt_context *c = b->ctx;
+    const BatchView bv = b->view();
+    const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
+    const int rp = b->allP1 ? 0 : 4;
+    for (int s = 0; s < nsteps; ++s) {
+        const uint32_t prev = first + s, st = first + s + 1;
+        const int flags = (s > 0 ? 1 : 0) | 2 | rp;
+        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, step_base, prev, st, flags,
+                               c->stream));
+        if (events) HIPCHK(hipEventRecord(b->evf[2 * s], c->stream));
+        HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->allP1 ? 0 : 1,
+                                  c->stream));
+        if (events) HIPCHK(hipEventRecord(b->evf[2 * s + 1], c->stream));
+    }
+    if (nsteps > 0)
+        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, step_base, first + nsteps,
+                               first + nsteps, 1, c->stream));
+    return 0;
+}
+
+// Captures one chunk of graph_chunk steps (step numbers relative to d_step,
+// which the chunk's last node advances) and instantiates it.
+static int capture_chunk(mdt_batch *b) {
+    mdt_context *c = b->ctx;
+    if (!b->d_step) {
+        int rc = dalloc(&b->d_step, 1);
+        if (rc) return rc;
+    }
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_steps(b, b->graph_chunk, b->d_step, 0u, false);
+    hipError_t e = rc ? hipSuccess : mdt_launch_advance(b->d_step, (uint32_t)b->graph_chunk, 0, c->stream);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(MDT_EHIP, "graph capture failed: %s / %s", hipGetErrorString(e), hipGetErrorString(e2));
+    }
+    e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->gexec = nullptr;
+        return fail(MDT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return 0;
+}
+
 int mdt_batch_step(mdt_batch *b, int nsteps) {
     if (!b) return fail(MDT_EINVAL, "NULL batch");
     if (nsteps < 0) return fail(MDT_EINVAL, "nsteps=%d", nsteps);
@@ -741,38 +816,31 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
     if (rc) return rc;
     if ((rc = autotune_if_needed(b))) return rc;
     mdt_context *c = b->ctx;
-    const BatchView bv = b->view();
-    int nl = 0;
-    // one step: [propagate_2 of the previous step +] propagate_1 (+ ring-polymer
-    // step when P > 1) per trajectory, then calc_force per replica; the last
-    // propagate_2 closes the call so the host sees the reference's state.
-    const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
-    const int rp = b->allP1 ? 0 : 4;
     if (b->profile && (int)b->evf.size() < 2 * nsteps) {
         const size_t old = b->evf.size();
         b->evf.resize(2 * nsteps);
         for (size_t k = old; k < b->evf.size(); ++k) HIPCHK(hipEventCreate(&b->evf[k]));
     }
     HIPCHK(hipEventRecord(b->ev0, c->stream));
-    for (int s = 0; s < nsteps; ++s) {
-        const uint32_t prev = b->step;
-        const uint32_t st = ++b->step;
-        const int flags = (s > 0 ? 1 : 0) | 2 | rp;
-        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, prev, st, flags, c->stream));
-        if (b->profile) HIPCHK(hipEventRecord(b->evf[2 * s], c->stream));
-        HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->allP1 ? 0 : 1,
-                                  c->stream));
-        if (b->profile) HIPCHK(hipEventRecord(b->evf[2 * s + 1], c->stream));
-        ++nl;
+    const int S = b->profile ? 0 : b->graph_chunk;  // profiling times every launch: eager
+    int replays = 0;
+    if (S > 0 && nsteps >= S) {
+        if (!b->gexec && (rc = capture_chunk(b))) return rc;
+        HIPCHK(mdt_launch_advance(b->d_step, b->step, 1, c->stream));
+        replays = nsteps / S;
+        for (int k = 0; k < replays; ++k) HIPCHK(hipGraphLaunch(b->gexec, c->stream));
+        b->step += (uint32_t)(replays * S);
     }
-    if (nsteps > 0)
-        HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, b->step, b->step, 1, c->stream));
+    const int rest = nsteps - replays * S;
+    if ((rc = enqueue_steps(b, rest, nullptr, b->step, b->profile != 0))) return rc;
+    b->step += (uint32_t)rest;
     HIPCHK(hipEventRecord(b->ev1, c->stream));
     HIPCHK(hipEventSynchronize(b->ev1));
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
     b->last_ms = ms;
-    b->last_nlaunch = nl;
+    b->last_nlaunch = nsteps;
+    b->graph_replays = replays;
     b->last_force_ms = 0.0;
     if (b->profile)
         for (int s = 0; s < nsteps; ++s) {
@@ -781,6 +849,24 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
             b->last_force_ms += fm;
         }
     return check_status(b);
+}
+
+int mdt_batch_set_graph(mdt_batch *b, int chunk) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (chunk < 0 || chunk > 4096) return fail(MDT_EINVAL, "graph chunk %d outside 0..4096", chunk);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    if (chunk != b->graph_chunk) drop_graph(b);
+    b->graph_chunk = chunk;
+    return 0;
+}
+
+int mdt_batch_set_precision(mdt_batch *b, int mode) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (mode != 0 && mode != 1) return fail(MDT_EINVAL, "precision mode %d (0: FP64, 1: FP32 pair terms)", mode);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    if (mode != b->precision) drop_graph(b);
+    b->precision = mode;
+    return 0;
 }
 
 int mdt_batch_status(mdt_batch *b, int32_t *status, int clear) {

@@ -69,8 +69,8 @@ struct PropConst {
 
 struct BatchView {
     int ntraj, nrep;
-    int nbcap;  // neighbour-list capacity per atom
-    int pad_;
+    int nbcap;      // neighbour-list capacity per atom
+    int precision;  // 0: FP64 pair terms; 1: FP32 pair terms, FP64 accumulation
     const int *rep_traj, *rep_bead;
     const int *traj_N, *traj_P;
     const long long *traj_soff, *traj_aoff;

@@ -42,7 +42,13 @@ struct LdsTables {
     double shift[27 * 3];  // cartesian lattice translation of each image code
     double exptab[64];
     int owner[MDT_MAXT], pair_on[MDT_MAXT * MDT_MAXT], algo[MDT_MAXT], is_proj[MDT_MAXT];
+    // FP32 copies for the single-precision pair terms; *_l2 carry a factor
+    // log2(e) so that exp(-a x) = 2^(-a_l2 x)
+    float f_eta2l2[MDT_MAXT], f_kdbl2[MDT_MAXT], f_kdb[MDT_MAXT];
+    float f_csig[MDT_MAXT * MDT_MAXT], f_etad[MDT_MAXT * MDT_MAXT], f_wv[MDT_MAXT * MDT_MAXT];
 };
+
+#define MDT_LOG2E 1.4426950408889634
 
 struct CellRegs {
     double h[9], ih[9];
@@ -135,12 +141,18 @@ __device__ void load_tables(LdsTables &T, const EmtConst *__restrict__ ec, const
         T.owner[k] = ec->owner[k];
         T.algo[k] = pc->algo[k];
         T.is_proj[k] = pc->is_proj[k];
+        T.f_eta2l2[k] = (float)(ec->eta2[k] * MDT_LOG2E);
+        T.f_kdbl2[k] = (float)(ec->kappadbeta[k] * MDT_LOG2E);
+        T.f_kdb[k] = (float)ec->kappadbeta[k];
     }
     for (int k = lane; k < MDT_MAXT * MDT_MAXT; k += nthr) {
         T.csig[k] = ec->csig[k];
         T.etad[k] = ec->etad[k];
         T.wv[k] = ec->wv[k];
         T.pair_on[k] = ec->pair_on[k];
+        T.f_csig[k] = (float)ec->csig[k];
+        T.f_etad[k] = (float)ec->etad[k];
+        T.f_wv[k] = (float)ec->wv[k];
     }
     for (int c = lane; c < 27; c += nthr) {
         const double m0 = (double)(c % 3 - 1), m1 = (double)((c / 3) % 3 - 1), m2 = (double)(c / 9 - 1);
@@ -205,7 +217,12 @@ __device__ __forceinline__ void fx_add(unsigned long long *acc, double x, double
 // ~1e-10 eV/A, since that atom's force scales with sigma itself.
 __device__ __forceinline__ bool owns(int k, int n) { return (k < n) != (((k + n) & 1) != 0); }
 
-template <int TPA, bool NBR_LDS>
+// F32 (precision mode 1): the pair terms -- cut-off function, the sigma and
+// pair-potential exponentials and their products -- are evaluated in FP32;
+// the geometry (differences, r, the exponent arguments r - rcut and
+// r - beta s0, which carry factors up to acut ~ 26/A), the per-atom
+// embedding terms, G and every accumulation stay FP64 / fixed point.
+template <int TPA, bool NBR_LDS, bool F32>
 __global__ void __launch_bounds__(512, 4)
     k_replica(const EmtConst *__restrict__ ecg, const PropConst *__restrict__ pcg, BatchView bv, int add_fin) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -266,6 +283,7 @@ __global__ void __launch_bounds__(512, 4)
     }
     __syncthreads();
     const double acut = ecg->acut, rcut = ecg->rcut, cutr = ecg->cutr, cut2 = ecg->cut2hi;
+    const float acutf = (float)acut, acutl2f = (float)(acut * MDT_LOG2E);
     const int apb = nthr / TPA;  // atoms per chunk
     const int wl = lane & 63;
     const unsigned long long gmask = (TPA >= 64 ? ~0ull : ((1ull << TPA) - 1ull)) << ((wl / TPA) * TPA);
@@ -338,12 +356,26 @@ __global__ void __launch_bounds__(512, 4)
                 const double r2 = dx * dx + dy * dy + dz * dz;
                 const double r = r2 * mdt_rsqrt(r2);
                 if (r > cutr) continue;
-                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
-                const double th = mdt_rcp(1 + e1);
-                const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
-                const double fsk = (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
-                sig += T.csig[tk * MDT_MAXT + tn] * fsn;
-                sig_add(&shi[n], &smid[n], &slo[n], T.csig[tn * MDT_MAXT + tk] * fsk, &bad_local);
+                double own, other;
+                if constexpr (F32) {
+                    const float e1 = mdt_exp2f(acutl2f * (float)(r - rcut));
+                    const float th = mdt_rcpf(1.0f + e1);
+                    const float fsn = th * mdt_exp2f(-T.f_eta2l2[tn] * (float)(r - T.bs0[tn]));
+                    const float fsk =
+                        (tn == tk) ? fsn : th * mdt_exp2f(-T.f_eta2l2[tk] * (float)(r - T.bs0[tk]));
+                    own = (double)(T.f_csig[tk * MDT_MAXT + tn] * fsn);
+                    other = (double)(T.f_csig[tn * MDT_MAXT + tk] * fsk);
+                } else {
+                    const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
+                    const double th = mdt_rcp(1 + e1);
+                    const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+                    const double fsk =
+                        (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+                    own = T.csig[tk * MDT_MAXT + tn] * fsn;
+                    other = T.csig[tn * MDT_MAXT + tk] * fsk;
+                }
+                sig += own;
+                sig_add(&shi[n], &smid[n], &slo[n], other, &bad_local);
             }
         }
         sig = group_sum<TPA>(sig);
@@ -402,22 +434,43 @@ __global__ void __launch_bounds__(512, 4)
                 const double ri = mdt_rsqrt(r2);
                 const double r = r2 * ri;
                 if (r > cutr) continue;
-                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
-                const double th = mdt_rcp(1 + e1);
-                const double rt1 = acut * e1 * th;
-                const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
-                const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
                 const int kn = tk * MDT_MAXT + tn, nk = tn * MDT_MAXT + tk;
-                double fsk = fsn, fvk = fvn;
-                if (tn != tk) {
-                    fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
-                    fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
+                double c;
+                if constexpr (F32) {
+                    const float e1 = mdt_exp2f(acutl2f * (float)(r - rcut));
+                    const float th = mdt_rcpf(1.0f + e1);
+                    const float rt1 = acutf * e1 * th;
+                    const float drn = (float)(r - T.bs0[tn]);
+                    const float fsn = th * mdt_exp2f(-T.f_eta2l2[tn] * drn);
+                    const float fvn = th * mdt_exp2f(-T.f_kdbl2[tn] * drn);
+                    float fsk = fsn, fvk = fvn;
+                    if (tn != tk) {
+                        const float drk = (float)(r - T.bs0[tk]);
+                        fsk = th * mdt_exp2f(-T.f_eta2l2[tk] * drk);
+                        fvk = th * mdt_exp2f(-T.f_kdbl2[tk] * drk);
+                    }
+                    const float ta = T.f_csig[kn] * ((T.f_etad[kn] + rt1) * fsn);
+                    const float tb = T.f_csig[nk] * ((T.f_etad[nk] + rt1) * fsk);
+                    const float tv =
+                        0.5f * (T.f_wv[kn] * ((T.f_kdb[tn] + rt1) * fvn) + T.f_wv[nk] * ((T.f_kdb[tk] + rt1) * fvk));
+                    c = Gk * (double)ta + G[n] * (double)tb + (double)tv;
+                    ep += (double)(T.f_wv[kn] * fvn + T.f_wv[nk] * fvk);
+                } else {
+                    const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
+                    const double th = mdt_rcp(1 + e1);
+                    const double rt1 = acut * e1 * th;
+                    const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+                    const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
+                    double fsk = fsn, fvk = fvn;
+                    if (tn != tk) {
+                        fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+                        fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
+                    }
+                    // dE/dr of this pair: density part through G, pair (V) part directly
+                    c = Gk * T.csig[kn] * ((T.etad[kn] + rt1) * fsn) + G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsk) +
+                        0.5 * (T.wv[kn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[nk] * ((T.kdb[tk] + rt1) * fvk));
+                    ep += T.wv[kn] * fvn + T.wv[nk] * fvk;
                 }
-                // dE/dr of this pair: density part through G, pair (V) part directly
-                const double c = Gk * T.csig[kn] * ((T.etad[kn] + rt1) * fsn) +
-                                 G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsk) +
-                                 0.5 * (T.wv[kn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[nk] * ((T.kdb[tk] + rt1) * fvk));
-                ep += T.wv[kn] * fvn + T.wv[nk] * fvk;
                 const double cr = c * ri;
                 fx -= cr * dx;
                 fy -= cr * dy;
@@ -479,10 +532,13 @@ __global__ void __launch_bounds__(512, 4)
 // trajectory per workgroup.  cjk (P x P, column-major cjk(j,k)) in global.
 // flags bit0: propagate_2, bit1: propagate_1, bit2: ring-polymer step
 // (trajectories with P == 1 only get f = 0 from bit2, as calc_force zeroes f).
+// step_base (graph replays): when non-null, *step_base is added to both step
+// numbers, which are then offsets within the captured chunk.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg, BatchView bv,
                                               const double *cjk_all, const long long *cjk_off,
-                                              uint32_t step_prev, uint32_t step, int flags) {
+                                              const uint32_t *step_base, uint32_t step_prev, uint32_t step,
+                                              int flags) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_algo[MDT_MAXT], s_proj[MDT_MAXT];
     __shared__ double red[4];
@@ -498,6 +554,11 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
     const double dt = pc.dt;
     const double *cjk = cjk_all + cjk_off[t];
     const int nthr = blockDim.x, lane = threadIdx.x;
+    if (step_base) {
+        const uint32_t s0 = *step_base;
+        step_prev += s0;
+        step += s0;
+    }
     double *C = (double *)smem_raw;         // P*P
     uint8_t *big = (uint8_t *)(C + P * P);  // N flags
     for (int k = lane; k < P * P; k += nthr) C[k] = cjk[k];
@@ -768,14 +829,14 @@ size_t mdt_replica_smem(int N, int cap, bool nbr_lds) {
            16;
 }
 
-template <bool L>
+template <bool L, bool F>
 static hipError_t launch_replica_t(int tpa, dim3 g, dim3 blk, size_t smem, const EmtConst *ec, const PropConst *pc,
                                    const BatchView &bv, int add_fin, hipStream_t st) {
     switch (tpa) {
-    case 1: hipLaunchKernelGGL((k_replica<1, L>), g, blk, smem, st, ec, pc, bv, add_fin); break;
-    case 2: hipLaunchKernelGGL((k_replica<2, L>), g, blk, smem, st, ec, pc, bv, add_fin); break;
-    case 4: hipLaunchKernelGGL((k_replica<4, L>), g, blk, smem, st, ec, pc, bv, add_fin); break;
-    case 8: hipLaunchKernelGGL((k_replica<8, L>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 1: hipLaunchKernelGGL((k_replica<1, L, F>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 2: hipLaunchKernelGGL((k_replica<2, L, F>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 4: hipLaunchKernelGGL((k_replica<4, L, F>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 8: hipLaunchKernelGGL((k_replica<8, L, F>), g, blk, smem, st, ec, pc, bv, add_fin); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -785,12 +846,29 @@ static hipError_t launch_replica_t(int tpa, dim3 g, dim3 blk, size_t smem, const
 hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst *ec, const PropConst *pc,
                               const BatchView &bv, int add_fin, hipStream_t st) {
     dim3 g(nrep), blk(block);
-    if (bv.nbr_global == nullptr) return launch_replica_t<true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
-    return launch_replica_t<false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+    const bool lds = bv.nbr_global == nullptr;
+    if (bv.precision == 1)
+        return lds ? launch_replica_t<true, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
+                   : launch_replica_t<false, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+    return lds ? launch_replica_t<true, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
+               : launch_replica_t<false, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
 }
 
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
-                           const long long *cjk_off, uint32_t step_prev, uint32_t step, int flags, hipStream_t st) {
-    hipLaunchKernelGGL(k_traj, dim3(ntraj), dim3(256), smem, st, pc, bv, cjk, cjk_off, step_prev, step, flags);
+                           const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
+                           int flags, hipStream_t st) {
+    hipLaunchKernelGGL(k_traj, dim3(ntraj), dim3(256), smem, st, pc, bv, cjk, cjk_off, step_base, step_prev, step,
+                       flags);
+    return hipGetLastError();
+}
+
+// the step counter of graph replays: one lane sets it (set != 0) or adds
+// the chunk length
+__global__ void k_advance(uint32_t *step_base, uint32_t n, int set) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *step_base = set ? n : *step_base + n;
+}
+
+hipError_t mdt_launch_advance(uint32_t *step_base, uint32_t n, int set, hipStream_t st) {
+    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, step_base, n, set);
     return hipGetLastError();
 }

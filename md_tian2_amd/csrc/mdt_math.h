@@ -40,7 +40,14 @@
 // host pass of hipcc: declarations only, device code is not generated here
 __device__ double mdt_rcp(double x);
 __device__ double mdt_rsqrt(double x);
+__device__ float mdt_rcpf(float x);
+__device__ float mdt_exp2f(float x);
 #else
+// FP32 pair terms: single hardware instructions, each within 1 ulp
+// (v_rcp_f32; v_exp_f32 = 2^x, results below 2^-126 flush to zero).
+__device__ __forceinline__ float mdt_rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float mdt_exp2f(float x) { return __builtin_amdgcn_exp2f(x); }
+
 // 1/x for finite x >= 1e-300: hardware reciprocal seed and two Newton steps
 // (the refinement LLVM itself uses inside its f64 division expansion).
 __device__ __forceinline__ double mdt_rcp(double x) {

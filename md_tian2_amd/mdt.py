@@ -28,7 +28,7 @@ SYMBOLS = (
     "mdt_batch_set_atoms", "mdt_batch_set_cells", "mdt_batch_set_propagation", "mdt_batch_seed",
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
-    "mdt_batch_force_timing",
+    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph",
 )
 
 
@@ -65,6 +65,8 @@ def lib():
         L.mdt_batch_tune.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
         L.mdt_batch_profile.argtypes = [vp, C.c_int]
         L.mdt_batch_force_timing.argtypes = [vp, dp]
+        L.mdt_batch_set_precision.argtypes = [vp, C.c_int]
+        L.mdt_batch_set_graph.argtypes = [vp, C.c_int]
         _LIB = L
     return _LIB
 
@@ -187,6 +189,14 @@ class Batch:
 
     def compute_forces(self):
         _check(lib().mdt_batch_compute_forces(self.h))
+
+    def set_precision(self, mode):
+        """0: FP64 pair terms (reference arithmetic); 1: FP32 pair terms with FP64 accumulation."""
+        _check(lib().mdt_batch_set_precision(self.h, int(mode)))
+
+    def set_graph(self, chunk):
+        """Replay the step loop as hipGraph chunks of `chunk` steps (0: direct launches)."""
+        _check(lib().mdt_batch_set_graph(self.h, int(chunk)))
 
     def step(self, nsteps=1):
         _check(lib().mdt_batch_step(self.h, nsteps))
