@@ -44,13 +44,15 @@ extern "C" {
 #define MDT_ENERGY_ONLY 3002
 
 /* precision modes */
-#define MDT_PREC_FP64 0  /* everything in FP64 */
+#define MDT_PREC_FP64 0      /* everything in FP64 */
+#define MDT_PREC_FP32_PAIR 1 /* FP32 pair terms, FP64 accumulation (mdt_batch_set_precision) */
 
 /* per-trajectory status bits (mdt_batch_status) */
 #define MDT_ST_NBR_OVERFLOW 1  /* neighbour list capacity exceeded */
 #define MDT_ST_NEG_DENSITY 2   /* ldfa saw a negative density (src/md_algo.f90:551-554) */
 #define MDT_ST_BAD_PROP 4      /* unknown propagator (src/md_algo.f90:66-67) */
 #define MDT_ST_FIXED_OVERFLOW 8 /* a pair term exceeded the fixed-point accumulator range */
+#define MDT_ST_NN_EXTRAPOLATION 16 /* a network input left its scaling range (EMT fallback applied) */
 
 typedef struct mdt_context mdt_context;
 typedef struct mdt_batch mdt_batch;
@@ -200,6 +202,67 @@ int mdt_batch_set_precision(mdt_batch *b, int mode);
  * launches.  Replaces the per-step calls of src/md_tian2.f90:78-98.
  */
 int mdt_batch_set_graph(mdt_batch *b, int chunk);
+
+/*
+ * NENE surrogate: the short-range atomic network of read_nene / compute_nene
+ * (src/pes_nene_mod.f90:123-5759, :7145-7368), RuNNer nn_type_short 1 with
+ * cut-off type 1.  Arrays are the contents of input.nn, scaling.data and
+ * weights.XXX.data, per network element (elements in ascending atomic
+ * number), symmetry functions in the reference's sorted order (radial type
+ * 2 first, then angular type 3):
+ *   elem_of_type[ntypes]  network element of each atom type (-1: none)
+ *   nsf[nelem]            functions per element; the per-function arrays
+ *                         below hold sum(nsf) entries, element-major
+ *   sf_type, sf_e1, sf_e2 type 2/3 and neighbour elements (element indices)
+ *   eta, rs, lambda, zeta, rc   parameters (Bohr units; one cut-off)
+ *   gmin, gmax, gavg      scaling.data columns
+ *   scmin, scmax, scale, center  scale_min/max_short_atomic,
+ *                         scale_symmetry_functions, center_symmetry_functions
+ *   nh1, nh2              nodes of the two hidden layers (<= 32)
+ *   activations           three letters (t, s or l) of global_activation_short
+ *   weights               per element in weights.XXX.data order, concatenated
+ * Limits: 4 elements, 64 functions (32 radial, 32 angular), zeta in
+ * {1, 2, 4, 8, 16}.
+ */
+int mdt_nn_params_set(mdt_context *ctx, int nelem, const int32_t *elem_of_type, int ntypes, const int32_t *nsf,
+                      const int32_t *sf_type, const int32_t *sf_e1, const int32_t *sf_e2, const double *eta,
+                      const double *rs, const double *lambda, const double *zeta, const double *rc,
+                      const double *gmin, const double *gmax, const double *gavg, double scmin, double scmax,
+                      int32_t scale, int32_t center, int32_t nh1, int32_t nh2, const char *activations,
+                      const double *weights);
+
+/*
+ * compute_nene(atoms, flag) on one universe (src/pes_nene_mod.f90:7145-7368):
+ * epot(b) += E_b [eV], f(:,b,i) += F [eV/A]; r, f column-major (3,P,N) in
+ * Angstrom as in compute_emt.  nextrap (may be NULL) is set to 1 when a
+ * symmetry function lies outside the scaling range by more than 1e-4 (the
+ * reference counts these as extrapolation warnings and goes on).
+ */
+int mdt_compute_nene(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx, const double *simbox,
+                     const double *isimbox, const double *r, double *epot, double *f, int32_t *nextrap,
+                     int32_t flag);
+
+/*
+ * Potential of a batch's calc_force (the dispatch of src/force.f90:77-82):
+ * pes 0 = EMT, 1 = the network, with EMT (when its parameters are set) on
+ * every replica whose symmetry functions leave the scaling range, and with
+ * EMT as the judge of the network on a rotating 1 % of the replicas every
+ * judge_interval steps (0: no judge).  Has no counterpart in the reference,
+ * which runs one potential per trajectory.
+ */
+int mdt_batch_set_pes(mdt_batch *b, int pes, int judge_interval);
+
+/* Judge statistics accumulated since the network first ran on the batch:
+ * sum of squared force differences [(eV/A)^2], number of force components
+ * judged, largest difference [eV/A]; replicas on the EMT fallback in the
+ * last force evaluation. */
+int mdt_batch_nn_stats(mdt_batch *b, double *judge_sq, double *judge_n, double *judge_max, int32_t *nfallback);
+
+/* Raw symmetry functions of the last network evaluation, float, nrep x maxN x
+ * 64 (replicas in batch order, trajectory-major; unused entries
+ * unspecified): the training set's descriptors and scaling statistics.
+ * Has no counterpart in the reference (RuNNer writes them in mode 1). */
+int mdt_batch_nn_descriptors(mdt_batch *b, float *G);
 
 #ifdef __cplusplus
 }

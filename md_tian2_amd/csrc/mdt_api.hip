@@ -20,6 +20,10 @@ hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const Ba
                            const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
                            int flags, hipStream_t st);
 hipError_t mdt_launch_advance(uint32_t *step_base, uint32_t n, int set, hipStream_t st);
+size_t mdt_nene_smem(int N);
+hipError_t mdt_nene_attributes(hipFuncAttributes *fa);
+hipError_t mdt_launch_nene(int nrep, size_t smem, const NnConst *M, const BatchView &bv, float *G, float *C,
+                           int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, hipStream_t st);
 
 namespace {
 
@@ -65,8 +69,11 @@ struct mdt_context {
     EmtConst ec{};
     EmtConst *d_ec = nullptr;  // device copy read by the kernels
     std::vector<int32_t> pes;
-    mdt_batch *scratch = nullptr;  // cached single-universe batch for mdt_compute_emt
+    mdt_batch *scratch = nullptr;  // cached single-universe batch for mdt_compute_emt / mdt_compute_nene
     int scratch_N = -1, scratch_P = -1;
+    NnConst *nn = nullptr;    // NENE surrogate (host copy)
+    NnConst *d_nn = nullptr;  // device copy read by the kernels
+    bool have_nn = false;
 };
 
 struct mdt_batch {
@@ -111,6 +118,13 @@ struct mdt_batch {
     hipGraphExec_t gexec = nullptr;
     uint32_t *d_step = nullptr;
     int graph_replays = 0;          // replays of the last call
+    int pes = 0;                    // 0: EMT; 1: NENE network with EMT fallback on extrapolation
+    float *d_nnG = nullptr, *d_nnC = nullptr;  // per replica maxN x NN_K1 symmetry functions, dE/dG
+    int *d_nnflag = nullptr;        // per replica: network extrapolated -> EMT fallback
+    double *d_nnF = nullptr;        // the network's forces alone (3PN blocks)
+    double *d_judge = nullptr;      // per replica: sum of squared force differences, count, max
+    int judge_interval = 0;         // EMT judge every judge_interval steps (0: off)
+    uint32_t judge_events = 0;
 
     BatchView view() const {
         BatchView v{};
@@ -288,6 +302,8 @@ int mdt_destroy(mdt_context *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->scratch) mdt_batch_destroy(ctx->scratch);
     if (ctx->d_ec) (void)hipFree(ctx->d_ec);
+    if (ctx->d_nn) (void)hipFree(ctx->d_nn);
+    delete ctx->nn;
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -320,7 +336,8 @@ int mdt_batch_destroy(mdt_batch *b) {
     void *ptrs[] = {b->d_rep_traj, b->d_rep_bead, b->d_N, b->d_P, b->d_soff, b->d_aoff, b->d_eoff,
                     b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
-                    b->d_cjk, b->d_status, b->d_nbr, b->d_pc, b->d_step};
+                    b->d_cjk, b->d_status, b->d_nbr, b->d_pc, b->d_step, b->d_nnG, b->d_nnC,
+                    b->d_nnflag, b->d_nnF, b->d_judge};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -605,7 +622,8 @@ static int push_pc(mdt_batch *b) {
 }
 
 static int ensure_ready(mdt_batch *b, bool need_prop) {
-    if (!b->ctx->have_params) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
+    if (b->pes == 1 && !b->ctx->have_nn) return fail(MDT_ESTATE, "no network (call mdt_nn_params_set)");
+    if (b->pes == 0 && !b->ctx->have_params) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
     if (!b->have_atoms) return fail(MDT_ESTATE, "no atoms (call mdt_batch_set_atoms)");
     if (!b->have_cells) return fail(MDT_ESTATE, "no cells (call mdt_batch_set_cells)");
     if (!b->have_state) return fail(MDT_ESTATE, "no positions (call mdt_batch_set_state)");
@@ -660,6 +678,60 @@ static int configure(mdt_batch *b, int tpa, int nbcap) {
     return 0;
 }
 
+static int ensure_nn(mdt_batch *b) {
+    if (b->d_nnG) return 0;
+    if (b->lds_limit == 0) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, b->ctx->device));
+        b->lds_limit = (size_t)prop.sharedMemPerBlock;
+    }
+    hipFuncAttributes fa;
+    HIPCHK(mdt_nene_attributes(&fa));
+    if (fa.sharedSizeBytes + mdt_nene_smem(b->maxN) > b->lds_limit)
+        return fail(MDT_EUNSUP, "network on %d atoms needs %zu bytes of LDS (limit %zu)", b->maxN,
+                    fa.sharedSizeBytes + mdt_nene_smem(b->maxN), b->lds_limit);
+    const size_t rows = (size_t)b->nrep * b->maxN * NN_K1;
+    int rc = dalloc(&b->d_nnG, rows) | dalloc(&b->d_nnC, rows) | dalloc(&b->d_nnflag, (size_t)b->nrep) |
+             dalloc(&b->d_nnF, (size_t)b->tot3pn) | dalloc(&b->d_judge, 3 * (size_t)b->nrep);
+    if (rc) return rc;
+    HIPCHK(hipMemset(b->d_nnflag, 0, sizeof(int) * b->nrep));
+    HIPCHK(hipMemset(b->d_judge, 0, sizeof(double) * 3 * b->nrep));
+    return 0;
+}
+
+// calc_force's PES dispatch (src/force.f90:77-82) for every replica: EMT, or
+// the network followed by EMT on the replicas whose symmetry functions left
+// the training range (when EMT parameters are set).
+static int launch_pes(mdt_batch *b, const BatchView &bv, int add_fin, hipStream_t st) {
+    mdt_context *c = b->ctx;
+    if (b->pes == 1) {
+        HIPCHK(mdt_launch_nene(b->nrep, mdt_nene_smem(b->maxN), c->d_nn, bv, b->d_nnG, b->d_nnC, b->d_nnflag,
+                               b->d_nnF, b->maxN, add_fin, 1, st));
+        if (c->have_params) {
+            BatchView fb = bv;
+            fb.only = b->d_nnflag;
+            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, fb, add_fin, st));
+        }
+        return 0;
+    }
+    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, add_fin, st));
+    return 0;
+}
+
+// EMT as the judge of the network on a rotating 1 % of the replicas
+static int launch_judge(mdt_batch *b, hipStream_t st) {
+    mdt_context *c = b->ctx;
+    BatchView jb = b->view();
+    jb.judge_f = b->d_nnF;
+    jb.skip = b->d_nnflag;  // replicas on the fallback have no network forces
+    jb.judge_acc = b->d_judge;
+    jb.judge_mod = 100;
+    jb.judge_phase = (int)(b->judge_events % 100u);
+    ++b->judge_events;
+    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, jb, 0, st));
+    return 0;
+}
+
 static int launch_force(mdt_batch *b) {
     mdt_context *c = b->ctx;
     const BatchView bv = b->view();
@@ -668,17 +740,18 @@ static int launch_force(mdt_batch *b) {
         HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, nullptr, b->step, b->step, 4,
                                c->stream));
     }
-    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->allP1 ? 0 : 1,
-                              c->stream));
-    return 0;
+    return launch_pes(b, bv, b->allP1 ? 0 : 1, c->stream);
 }
 
 static int check_status(mdt_batch *b) {
     std::vector<int> s(b->ntraj);
     HIPCHK(hipMemcpy(s.data(), b->d_status, sizeof(int) * b->ntraj, hipMemcpyDeviceToHost));
+    // network extrapolation is handled by the EMT fallback when EMT is set
+    const int ignore = (b->pes == 1 && b->ctx->have_params) ? MDT_ST_NN_EXTRAPOLATION : 0;
     for (int t = 0; t < b->ntraj; ++t)
-        if (s[t])
-            return fail(MDT_ETRAJ, "trajectory %d flagged status %d (1: neighbour overflow, 2: negative density, 4: bad propagator)",
+        if (s[t] & ~ignore)
+            return fail(MDT_ETRAJ, "trajectory %d flagged status %d (1: neighbour overflow, 2: negative density, 4: bad propagator "
+                        "or type, 8: fixed-point overflow, 16: network extrapolation without EMT fallback)",
                         t, s[t]);
     return 0;
 }
@@ -747,6 +820,7 @@ int mdt_batch_compute_forces(mdt_batch *b) {
     int rc = ensure_ready(b, false);
     if (rc) return rc;
     if ((rc = autotune_if_needed(b))) return rc;
+    if (b->pes == 1 && (rc = ensure_nn(b))) return rc;
     if ((rc = launch_force(b))) return rc;
     HIPCHK(hipStreamSynchronize(b->ctx->stream));
     return check_status(b);
@@ -768,9 +842,12 @@ static int enqueue_steps(mdt_batch *b, int nsteps, const uint32_t *step_base, ui
         HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, step_base, prev, st, flags,
                                c->stream));
         if (events) HIPCHK(hipEventRecord(b->evf[2 * s], c->stream));
-        HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, b->allP1 ? 0 : 1,
-                                  c->stream));
+        int rc = launch_pes(b, bv, b->allP1 ? 0 : 1, c->stream);
+        if (rc) return rc;
         if (events) HIPCHK(hipEventRecord(b->evf[2 * s + 1], c->stream));
+        if (b->pes == 1 && b->judge_interval > 0 && c->have_params && (st % (uint32_t)b->judge_interval) == 0 &&
+            !step_base && (rc = launch_judge(b, c->stream)))
+            return rc;
     }
     if (nsteps > 0)
         HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, step_base, first + nsteps,
@@ -815,6 +892,7 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
     int rc = ensure_ready(b, true);
     if (rc) return rc;
     if ((rc = autotune_if_needed(b))) return rc;
+    if (b->pes == 1 && (rc = ensure_nn(b))) return rc;
     mdt_context *c = b->ctx;
     if (b->profile && (int)b->evf.size() < 2 * nsteps) {
         const size_t old = b->evf.size();
@@ -822,7 +900,9 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
         for (size_t k = old; k < b->evf.size(); ++k) HIPCHK(hipEventCreate(&b->evf[k]));
     }
     HIPCHK(hipEventRecord(b->ev0, c->stream));
-    const int S = b->profile ? 0 : b->graph_chunk;  // profiling times every launch: eager
+    // profiling times every launch and the network's judge is keyed by the
+    // host's step count: both launch directly
+    const int S = (b->profile || (b->pes == 1 && b->judge_interval > 0)) ? 0 : b->graph_chunk;
     int replays = 0;
     if (S > 0 && nsteps >= S) {
         if (!b->gexec && (rc = capture_chunk(b))) return rc;
@@ -953,6 +1033,226 @@ int mdt_compute_emt(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx
     for (int i = 0; i < natoms; ++i)
         if (ctx->ec.owner[idx[i] - 1])
             for (int k = 0; k < nbeads; ++k) dens[k + (long long)nbeads * i] = dd[k + (long long)nbeads * i];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// NENE surrogate (src/pes_nene_mod.f90: read_nene :123-5759, compute_nene
+// :7145-7368)
+// ---------------------------------------------------------------------------
+int mdt_nn_params_set(mdt_context *ctx, int nelem, const int32_t *elem_of_type, int ntypes, const int32_t *nsf,
+                      const int32_t *sf_type, const int32_t *sf_e1, const int32_t *sf_e2, const double *eta,
+                      const double *rs, const double *lambda, const double *zeta, const double *rc,
+                      const double *gmin, const double *gmax, const double *gavg, double scmin, double scmax,
+                      int32_t scale, int32_t center, int32_t nh1, int32_t nh2, const char *activations,
+                      const double *weights) {
+    if (!ctx || !elem_of_type || !nsf || !sf_type || !sf_e1 || !sf_e2 || !eta || !rs || !lambda || !zeta || !rc ||
+        !gmin || !gmax || !gavg || !activations || !weights)
+        return fail(MDT_EINVAL, "mdt_nn_params_set: NULL argument");
+    if (nelem < 1 || nelem > NN_MAXE) return fail(MDT_EUNSUP, "nelem=%d outside 1..%d", nelem, NN_MAXE);
+    if (ntypes < 1 || ntypes > MDT_MAXT) return fail(MDT_EUNSUP, "ntypes=%d outside 1..%d", ntypes, MDT_MAXT);
+    if (nh1 < 1 || nh1 > NN_H || nh2 < 1 || nh2 > NN_H)
+        return fail(MDT_EUNSUP, "hidden layers %d, %d (1..%d nodes each)", nh1, nh2, NN_H);
+    NnConst *M = new NnConst();
+    std::memset(M, 0, sizeof(NnConst));
+    auto bail = [&](int code, const char *msg, int a) {
+        delete M;
+        return fail(code, msg, a);
+    };
+    M->nelem = nelem;
+    int acts[3];
+    for (int l = 0; l < 3; ++l) {
+        const char a = activations[l];
+        acts[l] = a == 't' ? 1 : a == 's' ? 2 : a == 'l' ? 0 : -1;
+        if (acts[l] < 0) return bail(MDT_EUNSUP, "activation %d not in t, s, l", l);
+    }
+    M->act1 = acts[0];
+    M->act2 = acts[1];
+    M->act3 = acts[2];
+    for (int t = 0; t < MDT_MAXT; ++t) M->elem_of_type[t] = -1;
+    for (int t = 0; t < ntypes; ++t) {
+        if (elem_of_type[t] < -1 || elem_of_type[t] >= nelem) return bail(MDT_EINVAL, "elem_of_type[%d]", t);
+        M->elem_of_type[t] = elem_of_type[t];
+    }
+    const double ang2bohr = 1.0 / 0.529177211;  // src/constants.f90:161
+    double rc0 = -1.0;
+    long o = 0, wo = 0;
+    for (int e = 0; e < nelem; ++e) {
+        const int n = nsf[e];
+        if (n < 1 || n > NN_K1) return bail(MDT_EUNSUP, "element %d: number of symmetry functions outside 1..64", e);
+        int nr = 0, na = 0;
+        for (int k = 0; k < n; ++k) {
+            const long q = o + k;
+            if (rc0 < 0) rc0 = rc[q];
+            if (std::fabs(rc[q] - rc0) > 1e-12 * rc0) return bail(MDT_EUNSUP, "element %d: more than one cut-off", e);
+            if (sf_type[q] == 2) {
+                if (na > 0) return bail(MDT_EINVAL, "element %d: radial functions must precede angular ones", e);
+                if (nr >= NN_MAXR) return bail(MDT_EUNSUP, "element %d: too many radial functions", e);
+                M->r_el[e][nr] = sf_e1[q];
+                M->r_eta[e][nr] = (float)eta[q];
+                M->r_rs[e][nr] = (float)rs[q];
+                ++nr;
+            } else if (sf_type[q] == 3) {
+                if (na >= NN_MAXA) return bail(MDT_EUNSUP, "element %d: too many angular functions", e);
+                int zp = -1;
+                for (int z = 0; z <= 4; ++z)
+                    if (zeta[q] == (double)(1 << z)) zp = z;
+                if (zp < 0) return bail(MDT_EUNSUP, "element %d: zeta must be 1, 2, 4, 8 or 16", e);
+                const int a1 = std::min(sf_e1[q], sf_e2[q]), a2 = std::max(sf_e1[q], sf_e2[q]);
+                M->a_e1[e][na] = a1;
+                M->a_e2[e][na] = a2;
+                M->a_etal2[e][na] = (float)(eta[q] * 1.4426950408889634);
+                M->a_lam[e][na] = (float)lambda[q];
+                M->a_norm[e][na] = (float)std::pow(2.0, 1.0 - zeta[q]);
+                M->a_zp[e][na] = zp;
+                ++na;
+            } else {
+                return bail(MDT_EUNSUP, "symmetry function type %d (2 and 3 are supported)", sf_type[q]);
+            }
+            const double mn = gmin[q], mx = gmax[q], av = gavg[q];
+            if (scale && !(mx > mn)) return bail(MDT_EINVAL, "element %d: scaling range max <= min", e);
+            double mul = 1.0, add = 0.0, dfac = 1.0;
+            if (scale && !center) {
+                mul = (scmax - scmin) / (mx - mn);
+                add = scmin - mn * mul;
+            } else if (center && !scale) {
+                add = -av;
+            } else if (scale && center) {
+                mul = 1.0 / (mx - mn);
+                add = -av * mul;
+            }
+            if (scale) dfac = (scmax - scmin) / (mx - mn);  // getshortatomic.f90:177-189 (whenever scaling is on)
+            M->smul[e][k] = (float)mul;
+            M->sadd[e][k] = (float)add;
+            M->dfac[e][k] = (float)dfac;
+            M->glo[e][k] = (float)(mn - 1e-4);
+            M->ghi[e][k] = (float)(mx + 1e-4);
+        }
+        M->nr[e] = nr;
+        M->na[e] = na;
+        // weights, RuNNer order: W1 (n x nh1) input-major, b1, W2 (nh1 x nh2), b2, W3 (nh2 x 1), b3
+        const double *w = weights + wo;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < nh1; ++j) M->W1[e][i][j] = (float)w[i * nh1 + j];
+        w += n * nh1;
+        for (int j = 0; j < nh1; ++j) M->b1[e][j] = (float)w[j];
+        w += nh1;
+        for (int i = 0; i < nh1; ++i)
+            for (int j = 0; j < nh2; ++j) M->W2[e][i][j] = (float)w[i * nh2 + j];
+        w += nh1 * nh2;
+        for (int j = 0; j < nh2; ++j) M->b2[e][j] = (float)w[j];
+        w += nh2;
+        for (int j = 0; j < nh2; ++j) M->w3[e][j] = (float)w[j];
+        w += nh2;
+        M->b3[e] = (float)w[0];
+        wo += (long)n * nh1 + nh1 + (long)nh1 * nh2 + nh2 + nh2 + 1;
+        o += n;
+    }
+    M->rc = (float)rc0;
+    M->rc2 = (float)(rc0 * rc0);
+    M->pi_rc = (float)(M_PI / rc0);
+    M->ang2bohr = (float)ang2bohr;
+    HIPCHK(hipSetDevice(ctx->device));
+    if (!ctx->d_nn) {
+        int rc2 = dalloc(&ctx->d_nn, 1);
+        if (rc2) {
+            delete M;
+            return rc2;
+        }
+    }
+    HIPCHK(hipMemcpy(ctx->d_nn, M, sizeof(NnConst), hipMemcpyHostToDevice));
+    delete ctx->nn;
+    ctx->nn = M;
+    ctx->have_nn = true;
+    return 0;
+}
+
+int mdt_batch_set_pes(mdt_batch *b, int pes, int judge_interval) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (pes != 0 && pes != 1) return fail(MDT_EINVAL, "pes %d (0: EMT, 1: network)", pes);
+    if (judge_interval < 0) return fail(MDT_EINVAL, "judge_interval %d", judge_interval);
+    if (pes == 1 && !b->ctx->have_nn) return fail(MDT_ESTATE, "no network (call mdt_nn_params_set)");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    if (pes != b->pes) drop_graph(b);
+    b->pes = pes;
+    b->judge_interval = judge_interval;
+    if (pes == 1) return ensure_nn(b);
+    return 0;
+}
+
+int mdt_batch_nn_stats(mdt_batch *b, double *judge_sq, double *judge_n, double *judge_max, int32_t *nfallback) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (!b->d_nnG) return fail(MDT_ESTATE, "the network has not run on this batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    std::vector<double> j(3 * (size_t)b->nrep);
+    std::vector<int> f(b->nrep);
+    HIPCHK(hipMemcpy(j.data(), b->d_judge, sizeof(double) * j.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(f.data(), b->d_nnflag, sizeof(int) * f.size(), hipMemcpyDeviceToHost));
+    double sq = 0, n = 0, mx = 0;
+    int nf = 0;
+    for (int r = 0; r < b->nrep; ++r) {
+        sq += j[3 * r];
+        n += j[3 * r + 1];
+        mx = std::max(mx, j[3 * r + 2]);
+        nf += f[r] != 0;
+    }
+    if (judge_sq) *judge_sq = sq;
+    if (judge_n) *judge_n = n;
+    if (judge_max) *judge_max = mx;
+    if (nfallback) *nfallback = nf;
+    return 0;
+}
+
+int mdt_batch_nn_descriptors(mdt_batch *b, float *G) {
+    if (!b || !G) return fail(MDT_EINVAL, "mdt_batch_nn_descriptors: NULL argument");
+    if (!b->d_nnG) return fail(MDT_ESTATE, "the network has not run on this batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipMemcpy(G, b->d_nnG, sizeof(float) * (size_t)b->nrep * b->maxN * NN_K1, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mdt_compute_nene(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx, const double *simbox,
+                     const double *isimbox, const double *r, double *epot, double *f, int32_t *nextrap,
+                     int32_t flag) {
+    (void)flag;  // forces are always computed
+    if (!ctx || !idx || !simbox || !isimbox || !r || !epot || !f)
+        return fail(MDT_EINVAL, "mdt_compute_nene: NULL argument");
+    if (!ctx->have_nn) return fail(MDT_ESTATE, "no network (call mdt_nn_params_set)");
+    int rc;
+    if (!ctx->scratch || ctx->scratch_N != natoms || ctx->scratch_P != nbeads) {
+        if (ctx->scratch) mdt_batch_destroy(ctx->scratch);
+        ctx->scratch = nullptr;
+        int32_t n = natoms, p = nbeads;
+        if ((rc = mdt_batch_create(ctx, 1, &n, &p, &ctx->scratch))) return rc;
+        ctx->scratch_N = natoms;
+        ctx->scratch_P = nbeads;
+    }
+    mdt_batch *b = ctx->scratch;
+    std::vector<double> mass(natoms, 1.0);
+    std::vector<int32_t> fixed(3LL * natoms * nbeads, 0);
+    if ((rc = mdt_batch_set_atoms(b, idx, mass.data(), fixed.data()))) return rc;
+    if ((rc = mdt_batch_set_cells(b, simbox, isimbox))) return rc;
+    if ((rc = mdt_batch_set_state(b, r, nullptr))) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    b->pes = 1;
+    if ((rc = ensure_nn(b))) return rc;
+    if ((rc = ensure_ready(b, false))) return rc;
+    HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(int) * b->ntraj, ctx->stream));
+    HIPCHK(mdt_launch_nene(b->nrep, mdt_nene_smem(b->maxN), ctx->d_nn, b->view(), b->d_nnG, b->d_nnC, b->d_nnflag,
+                           b->d_nnF, b->maxN, 0, 0, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<int> st(1);
+    HIPCHK(hipMemcpy(st.data(), b->d_status, sizeof(int), hipMemcpyDeviceToHost));
+    b->pes = 0;
+    if (st[0] & ~MDT_ST_NN_EXTRAPOLATION)
+        return fail(MDT_ETRAJ, "compute_nene: status %d (1: neighbour overflow, 4: atom type not in the network, "
+                    "8: fixed-point overflow)", st[0]);
+    if (nextrap) *nextrap = (st[0] & MDT_ST_NN_EXTRAPOLATION) ? 1 : 0;
+    const long long n3 = 3LL * natoms * nbeads;
+    std::vector<double> ff(n3), ee(nbeads);
+    if ((rc = mdt_batch_get_state(b, nullptr, nullptr, ff.data(), nullptr, ee.data(), nullptr))) return rc;
+    for (long long k = 0; k < n3; ++k) f[k] += ff[k];
+    for (int k = 0; k < nbeads; ++k) epot[k] += ee[k];
     return 0;
 }
 

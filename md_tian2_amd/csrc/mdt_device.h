@@ -88,4 +88,53 @@ struct BatchView {
     int *status;            // per traj
     uint16_t *nbr_global;   // NULL: neighbour lists in LDS; else nrep * nbr_stride entries
     long long nbr_stride;   // entries per replica slice (maxN * cap)
+    const int *only;        // non-NULL: calc_force only on replicas with only[rep] != 0
+    const int *skip;        // non-NULL: no calc_force on replicas with skip[rep] != 0
+    // judge mode (judge_acc non-NULL): replicas with rep % judge_mod ==
+    // judge_phase compute EMT forces and accumulate, per replica, the sum of
+    // squared differences to judge_f, the component count and the largest
+    // difference into judge_acc[3 rep ..]; nothing else is written
+    const double *judge_f;
+    double *judge_acc;
+    int judge_mod, judge_phase;
 };
+
+// ---------------------------------------------------------------------------
+// NENE (Behler-Parrinello atomic network) surrogate: the model in the form
+// compute_nene evaluates (src/pes_nene_mod.f90:7145-7368) -- radial (type 2)
+// and angular (type 3) symmetry functions with one cosine cut-off, min/max
+// scaling, per element a network of inputs -> 2 hidden layers -> 1.
+// Lengths in Bohr, energies in Hartree (the network's units).  Inputs and
+// hidden layers are zero-padded to NN_K1 and NN_H (exact: padded weights
+// are zero).
+// ---------------------------------------------------------------------------
+#define NN_MAXE 4
+#define NN_MAXR 32
+#define NN_MAXA 32
+#define NN_K1 64
+#define NN_H 32
+
+struct NnConst {
+    int nelem;
+    int act1, act2, act3;            // activation codes: 0 linear, 1 tanh, 2 sigmoid
+    int elem_of_type[MDT_MAXT];      // batch atom type -> network element (-1: not in the network)
+    float rc, rc2, pi_rc;            // single cut-off radius (Bohr), rc^2, pi/rc
+    float ang2bohr;
+    int nr[NN_MAXE], na[NN_MAXE];    // radial / angular functions per element (inputs 0..nr-1, nr..nr+na-1)
+    int r_el[NN_MAXE][NN_MAXR];      // neighbour element of radial function
+    float r_eta[NN_MAXE][NN_MAXR], r_rs[NN_MAXE][NN_MAXR];
+    int a_e1[NN_MAXE][NN_MAXA], a_e2[NN_MAXE][NN_MAXA];
+    float a_etal2[NN_MAXE][NN_MAXA];  // eta log2(e)
+    float a_lam[NN_MAXE][NN_MAXA];
+    float a_norm[NN_MAXE][NN_MAXA];   // 2^(1 - zeta)
+    int a_zp[NN_MAXE][NN_MAXA];       // zeta = 2^a_zp (1, 2, 4, 8 or 16)
+    float smul[NN_MAXE][NN_K1], sadd[NN_MAXE][NN_K1];  // network input x = G smul + sadd
+    float dfac[NN_MAXE][NN_K1];                         // dE/dG = dE/dx dfac
+    float glo[NN_MAXE][NN_K1], ghi[NN_MAXE][NN_K1];     // extrapolation bounds min - 1e-4, max + 1e-4
+    float W1[NN_MAXE][NN_K1][NN_H], b1[NN_MAXE][NN_H];
+    float W2[NN_MAXE][NN_H][NN_H], b2[NN_MAXE][NN_H];
+    float w3[NN_MAXE][NN_H], b3[NN_MAXE];
+};
+
+// per-replica NN status bit (in the trajectory status word)
+#define MDT_ST_NN_EXTRAPOLATION 16

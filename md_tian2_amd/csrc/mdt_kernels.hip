@@ -229,6 +229,9 @@ __global__ void __launch_bounds__(512, 4)
     __shared__ LdsTables T;
     __shared__ double red[4];
     const int rep = blockIdx.x;
+    if (bv.only && !bv.only[rep]) return;                                          // whole block: uniform
+    if (bv.skip && bv.skip[rep]) return;                                           // whole block: uniform
+    if (bv.judge_acc && (rep % bv.judge_mod) != bv.judge_phase) return;           // whole block: uniform
     const int t = bv.rep_traj[rep];
     const int b = bv.rep_bead[rep];
     const int N = bv.traj_N[t];
@@ -399,7 +402,7 @@ __global__ void __launch_bounds__(512, 4)
             Ek = T.e0[tk] * ((1 + T.lambda[tk] * sv) * el - 1) + 6 * T.v0[tk] * ek;
             const double D = T.e0[tk] * T.lambda[tk] * T.lambda[tk] * (sv * el) + 6 * T.v0[tk] * T.kappa[tk] * ek;
             Gk = -D / (T.betaeta2[tk] * sc);
-            bv.dens[doff + k] = dn;  // raw EMT density (pes_emt_mod.f90:855,861)
+            if (!bv.judge_acc) bv.dens[doff + k] = dn;  // raw EMT density (pes_emt_mod.f90:855,861)
         }
         G[k] = Gk;
         E[k] = Ek;
@@ -492,6 +495,45 @@ __global__ void __launch_bounds__(512, 4)
         }
     }
     __syncthreads();
+
+    // ---- judge mode: compare with the given forces, write nothing else ----
+    if (bv.judge_acc) {
+        double sq = 0.0, mx = 0.0;
+        for (int k = lane; k < N; k += nthr) {
+            const int tk = typ[k];
+            for (int cc = 0; cc < 3; ++cc) {
+                unsigned long long *acc = cc == 0 ? fxa : (cc == 1 ? fya : fza);
+                const double F = T.owner[tk] ? (double)(long long)acc[k] * MDT_F_ISCALE : 0.0;
+                const double d = F - bv.judge_f[roff + (long long)cc * N + k];
+                sq += d * d;
+                mx = fabs(d) > mx ? fabs(d) : mx;
+            }
+        }
+        // fixed-order block reduction through red[] (wave partials, then wave 0)
+        for (int o = 32; o > 0; o >>= 1) {
+            sq += __shfl_xor(sq, o, 64);
+            const double m2 = __shfl_xor(mx, o, 64);
+            mx = m2 > mx ? m2 : mx;
+        }
+        __shared__ double jr[16];
+        if ((lane & 63) == 0) {
+            jr[2 * (lane >> 6)] = sq;
+            jr[2 * (lane >> 6) + 1] = mx;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            double s2 = 0.0, m2 = 0.0;
+            for (int w = 0; w < (nthr + 63) / 64; ++w) {
+                s2 += jr[2 * w];
+                m2 = jr[2 * w + 1] > m2 ? jr[2 * w + 1] : m2;
+            }
+            bv.judge_acc[3 * rep] += s2;
+            bv.judge_acc[3 * rep + 1] += 3.0 * N;
+            bv.judge_acc[3 * rep + 2] = m2 > bv.judge_acc[3 * rep + 2] ? m2 : bv.judge_acc[3 * rep + 2];
+        }
+        if (bad_local) atomicOr(&bv.status[t], bad_local);
+        return;
+    }
 
     // ---- phase 4: set_acceleration (force.f90:89-98), outputs ----
     for (int k = lane; k < N; k += nthr) {

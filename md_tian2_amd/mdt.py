@@ -29,6 +29,8 @@ SYMBOLS = (
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
     "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph",
+    "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats",
+    "mdt_batch_nn_descriptors",
 )
 
 
@@ -67,6 +69,13 @@ def lib():
         L.mdt_batch_force_timing.argtypes = [vp, dp]
         L.mdt_batch_set_precision.argtypes = [vp, C.c_int]
         L.mdt_batch_set_graph.argtypes = [vp, C.c_int]
+        L.mdt_nn_params_set.argtypes = [vp, C.c_int, ip, C.c_int, ip, ip, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp,
+                                        C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                        C.c_char_p, dp]
+        L.mdt_compute_nene.argtypes = [vp, C.c_int, C.c_int, ip, dp, dp, dp, dp, dp, ip, C.c_int32]
+        L.mdt_batch_set_pes.argtypes = [vp, C.c_int, C.c_int]
+        L.mdt_batch_nn_stats.argtypes = [vp, dp, dp, dp, ip]
+        L.mdt_batch_nn_descriptors.argtypes = [vp, C.POINTER(C.c_float)]
         _LIB = L
     return _LIB
 
@@ -117,6 +126,47 @@ class Context:
         pesf = np.ascontiguousarray(np.array(pesm, dtype=np.int32).T)  # column-major
         _check(lib().mdt_emt_params_set(self.h, nt, *[_dp(a) for a in arr], _ip(pesf)))
         self._keep = (arr, pesf)
+
+    def set_nn(self, model, type_names):
+        """Upload a nene.NeneModel; type_names as in geometry.System (symbol, is_proj)."""
+        from md_tian2_amd import nene
+        el = {z: i for i, z in enumerate(model.elements)}
+        eot = np.ascontiguousarray([el.get(nene.ATOMIC_NUMBER.get(t[0], -1), -1) for t in type_names], np.int32)
+        nsf = np.ascontiguousarray([model.nsf(z) for z in model.elements], np.int32)
+        sfs = [(z, f) for z in model.elements for f in model.sf[z]]
+        i32 = lambda v: np.ascontiguousarray(v, dtype=np.int32)
+        f64 = lambda v: np.ascontiguousarray(v, dtype=np.float64)
+        typ = i32([f.type for _, f in sfs])
+        e1 = i32([el[f.e1] for _, f in sfs])
+        e2 = i32([el[f.e2] if f.type == 3 else 0 for _, f in sfs])
+        eta, rs = f64([f.eta for _, f in sfs]), f64([f.rs for _, f in sfs])
+        lam, zeta, rc = f64([f.lam for _, f in sfs]), f64([f.zeta for _, f in sfs]), f64([f.rc for _, f in sfs])
+        gmin = f64(np.concatenate([model.gmin[z] for z in model.elements]))
+        gmax = f64(np.concatenate([model.gmax[z] for z in model.elements]))
+        gavg = f64(np.concatenate([model.gavg[z] for z in model.elements]))
+        w = f64(np.concatenate([model.weights[z] for z in model.elements]))
+        if len(model.hidden) != 2:
+            raise MdtError("the GPU network has exactly two hidden layers")
+        _check(lib().mdt_nn_params_set(self.h, len(model.elements), _ip(eot), len(type_names), _ip(nsf), _ip(typ),
+                                       _ip(e1), _ip(e2), _dp(eta), _dp(rs), _dp(lam), _dp(zeta), _dp(rc),
+                                       _dp(gmin), _dp(gmax), _dp(gavg), model.scmin, model.scmax,
+                                       int(model.scale), int(model.center), model.hidden[0], model.hidden[1],
+                                       model.activations.encode(), _dp(w)))
+        self._keep_nn = (eot, nsf, typ, e1, e2, eta, rs, lam, zeta, rc, gmin, gmax, gavg, w)
+
+    def compute_nene(self, sysm, epot=None, f=None):
+        """Drop-in compute_nene on one universe; returns (epot [eV], f [eV/A], extrapolated)."""
+        N, P = sysm.natoms, sysm.nbeads
+        epot = np.zeros(P) if epot is None else epot
+        f = np.zeros((N, P, 3)) if f is None else f
+        idx = np.ascontiguousarray(sysm.idx + 1, dtype=np.int32)
+        sb = np.ascontiguousarray(np.asarray(sysm.simbox, dtype=np.float64).T)
+        isb = np.ascontiguousarray(np.linalg.inv(sysm.simbox).T)
+        r = np.ascontiguousarray(sysm.r, dtype=np.float64)
+        nex = C.c_int32(0)
+        _check(lib().mdt_compute_nene(self.h, N, P, _ip(idx), _dp(sb), _dp(isb), _dp(r), _dp(epot), _dp(f),
+                                      C.byref(nex), ENERGY_AND_FORCE))
+        return epot, f, bool(nex.value)
 
     def compute_emt(self, sysm, epot=None, f=None, dens=None):
         """Drop-in compute_emt on one universe; returns (epot, f, dens) accumulated."""
@@ -193,6 +243,29 @@ class Batch:
     def set_precision(self, mode):
         """0: FP64 pair terms (reference arithmetic); 1: FP32 pair terms with FP64 accumulation."""
         _check(lib().mdt_batch_set_precision(self.h, int(mode)))
+
+    def set_pes(self, pes, judge_interval=0):
+        """0: EMT; 1: the network (EMT fallback on extrapolation, EMT judge every judge_interval steps)."""
+        _check(lib().mdt_batch_set_pes(self.h, int(pes), int(judge_interval)))
+
+    def nn_stats(self):
+        """(judge force RMSE [eV/A] or None, components judged, max |dF|, replicas on the fallback)."""
+        sq, n, mx = C.c_double(0), C.c_double(0), C.c_double(0)
+        nf = C.c_int32(0)
+        _check(lib().mdt_batch_nn_stats(self.h, C.byref(sq), C.byref(n), C.byref(mx), C.byref(nf)))
+        return (np.sqrt(sq.value / n.value) if n.value else None), n.value, mx.value, nf.value
+
+    def nn_descriptors(self):
+        """Raw symmetry functions of the last network evaluation: per trajectory (P, N, 64) float32."""
+        maxn = int(self.N.max())
+        nrep = int(self.P.sum())
+        g = np.zeros((nrep, maxn, 64), dtype=np.float32)
+        _check(lib().mdt_batch_nn_descriptors(self.h, g.ctypes.data_as(C.POINTER(C.c_float))))
+        out, r = [], 0
+        for s in self.systems:
+            out.append(g[r:r + s.nbeads, :s.natoms].copy())
+            r += s.nbeads
+        return out
 
     def set_graph(self, chunk):
         """Replay the step loop as hipGraph chunks of `chunk` steps (0: direct launches)."""
