@@ -264,6 +264,41 @@ int mdt_batch_nn_stats(mdt_batch *b, double *judge_sq, double *judge_n, double *
  * Has no counterpart in the reference (RuNNer writes them in mode 1). */
 int mdt_batch_nn_descriptors(mdt_batch *b, float *G);
 
+/*
+ * Training of the surrogate on the GPU: the project's own forward, backward
+ * and force-backward kernels (BF16 MFMA, FP32 accumulation) with Adam.  No
+ * counterpart in the reference, which fits with RuNNer's Kalman filter
+ * (src/fit.f90).  Loss over a minibatch of Nf frames with Ntot atoms, in eV
+ * and eV/A:  L = wE/Nf sum_f ((E_f - Eref_f)/N_f)^2 + wF/(3 Ntot) sum |F - Fref|^2.
+ *   mdt_nn_train_create: frames (natoms[f] atoms each; idx 1-based types,
+ *     simbox 9 per frame column-major [A], r 3 per atom [A], Eref [eV],
+ *     Fref 3 per atom [eV/A]) uploaded once, descriptors computed once on
+ *     the GPU; fixed minibatches of batch_frames frames over a seeded
+ *     permutation, the last val_fraction of them held out;
+ *   mdt_nn_train_descriptor_stats: per element (nelem x 64) min, max, mean
+ *     of the symmetry functions over all frames (scaling.data values; pass
+ *     them to mdt_nn_params_set);
+ *   mdt_nn_train_epoch: one pass over the training minibatches in a seeded
+ *     random order, one Adam step each; loss = mean minibatch loss;
+ *   mdt_nn_train_eval: energy RMSE per atom [eV/atom] and force-component
+ *     RMSE [eV/A] on the training (set 0) or validation (set 1) frames;
+ *   mdt_nn_train_loss_grad: loss and its gradient (weights.XXX.data layout,
+ *     elements concatenated) over the given frames, network unchanged;
+ *   mdt_nn_get_weights: the current weights in that layout.
+ * The network must have a linear output layer.
+ */
+typedef struct mdt_nntrain mdt_nntrain;
+int mdt_nn_train_create(mdt_context *ctx, int nframes, const int32_t *natoms, const int32_t *idx,
+                        const double *simbox, const double *r, const double *Eref, const double *Fref,
+                        int batch_frames, double val_fraction, uint32_t seed, mdt_nntrain **out);
+int mdt_nn_train_destroy(mdt_nntrain *tr);
+int mdt_nn_train_descriptor_stats(mdt_nntrain *tr, double *gmin, double *gmax, double *gavg);
+int mdt_nn_train_epoch(mdt_nntrain *tr, double lr, double wE, double wF, double *loss);
+int mdt_nn_train_eval(mdt_nntrain *tr, int set, double *rmse_e, double *rmse_f);
+int mdt_nn_train_loss_grad(mdt_nntrain *tr, int nsel, const int32_t *frames, double wE, double wF, double *loss,
+                           double *grad);
+int mdt_nn_get_weights(mdt_context *ctx, double *weights);
+
 #ifdef __cplusplus
 }
 #endif

@@ -22,6 +22,15 @@ hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const Ba
 hipError_t mdt_launch_advance(uint32_t *step_base, uint32_t n, int set, hipStream_t st);
 size_t mdt_nene_smem(int N);
 hipError_t mdt_nene_attributes(hipFuncAttributes *fa);
+size_t mdt_nn_frame_smem(int N);
+hipError_t mdt_launch_nn_frame(int nblocks, size_t smem, const NnConst *M, const NnTrainView &tv, const int *frames,
+                               int mode, double cEw, double cFw, hipStream_t st);
+hipError_t mdt_launch_nn_mlp(int ntiles, const NnConst *M, const NnTrainView &tv, const int4 *tiles,
+                             const int *rows, hipStream_t st);
+hipError_t mdt_launch_nn_grad(int nblocks, const NnConst *M, const NnTrainView &tv, const int4 *tiles, int ntiles,
+                              const int *rows, float *partials, hipStream_t st);
+hipError_t mdt_launch_nn_adam(NnConst *M, const float *partials, int W, float *grad, float *am, float *av, float lr,
+                              float beta1, float beta2, float eps, int step, int mode, hipStream_t st);
 hipError_t mdt_launch_nene(int nrep, size_t smem, const NnConst *M, const BatchView &bv, float *G, float *C,
                            int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, hipStream_t st);
 
@@ -1148,6 +1157,8 @@ int mdt_nn_params_set(mdt_context *ctx, int nelem, const int32_t *elem_of_type, 
         wo += (long)n * nh1 + nh1 + (long)nh1 * nh2 + nh2 + nh2 + 1;
         o += n;
     }
+    M->nh1 = nh1;
+    M->nh2 = nh2;
     M->rc = (float)rc0;
     M->rc2 = (float)(rc0 * rc0);
     M->pi_rc = (float)(M_PI / rc0);
@@ -1253,6 +1264,424 @@ int mdt_compute_nene(mdt_context *ctx, int natoms, int nbeads, const int32_t *id
     if ((rc = mdt_batch_get_state(b, nullptr, nullptr, ff.data(), nullptr, ee.data(), nullptr))) return rc;
     for (long long k = 0; k < n3; ++k) f[k] += ff[k];
     for (int k = 0; k < nbeads; ++k) epot[k] += ee[k];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Training of the surrogate on the GPU (no reference counterpart: md_tian2
+// fits with RuNNer's Kalman filter in fit.f90)
+// ---------------------------------------------------------------------------
+struct mdt_nntrain {
+    mdt_context *ctx = nullptr;
+    int nframes = 0, maxN = 0;
+    long long natoms = 0;
+    int *d_natoms = nullptr, *d_atom_frame = nullptr, *d_frames = nullptr, *d_rows = nullptr;
+    long long *d_aoff = nullptr;
+    double *d_cell = nullptr, *d_icell = nullptr, *d_pos = nullptr, *d_Fref = nullptr, *d_Eref = nullptr;
+    uint8_t *d_type = nullptr;
+    float *d_G = nullptr, *d_C = nullptr, *d_U = nullptr;
+    double *d_Eatom = nullptr, *d_cE = nullptr, *d_loss = nullptr;
+    int4 *d_tiles = nullptr;
+    float *d_partials = nullptr, *d_grad = nullptr, *d_am = nullptr, *d_av = nullptr;
+    int nblocks_grad = 256;
+    int adam_step = 0;
+    // minibatches: frames d_frames[fr_off[m] .. fr_off[m+1]), tiles d_tiles[tl_off[m] ..], rows d_rows[..]
+    std::vector<int> fr_off, tl_off, natoms_mb;
+    std::vector<int> h_natoms;
+    std::vector<long long> h_aoff;
+    std::vector<uint8_t> h_elem;  // network element per atom
+    int ntrain_mb = 0;            // minibatches 0 .. ntrain_mb-1 train, the rest validate
+    uint64_t rng = 0;
+
+    NnTrainView view() const {
+        NnTrainView v{};
+        v.nframes = nframes;
+        v.fr_natoms = d_natoms;
+        v.fr_aoff = d_aoff;
+        v.fr_cell = d_cell;
+        v.fr_icell = d_icell;
+        v.type = d_type;
+        v.pos = d_pos;
+        v.Fref = d_Fref;
+        v.Eref = d_Eref;
+        v.atom_frame = d_atom_frame;
+        v.G = d_G;
+        v.C = d_C;
+        v.U = d_U;
+        v.Eatom = d_Eatom;
+        v.cE = d_cE;
+        v.loss = d_loss;
+        return v;
+    }
+};
+
+static uint64_t splitmix(uint64_t &x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// tiles (element, first row, count) of 16 same-element atoms over a frame list
+static void build_tiles(const mdt_nntrain *tr, const int *frames, int nf, int nelem, std::vector<int4> &tiles,
+                        std::vector<int> &rows, int row_base) {
+    for (int e = 0; e < nelem; ++e) {
+        std::vector<int> at;
+        for (int q = 0; q < nf; ++q) {
+            const int f = frames[q];
+            for (int i = 0; i < tr->h_natoms[f]; ++i)
+                if (tr->h_elem[tr->h_aoff[f] + i] == e) at.push_back((int)(tr->h_aoff[f] + i));
+        }
+        for (size_t k = 0; k < at.size(); k += 16) {
+            const int cnt = (int)std::min<size_t>(16, at.size() - k);
+            tiles.push_back(make_int4(e, row_base + (int)rows.size(), cnt, 0));
+            for (int j = 0; j < cnt; ++j) rows.push_back(at[k + j]);
+        }
+    }
+}
+
+int mdt_nn_train_destroy(mdt_nntrain *tr) {
+    if (!tr) return 0;
+    (void)hipSetDevice(tr->ctx->device);
+    void *ptrs[] = {tr->d_natoms, tr->d_atom_frame, tr->d_frames, tr->d_rows, tr->d_aoff, tr->d_cell, tr->d_icell,
+                    tr->d_pos, tr->d_Fref, tr->d_Eref, tr->d_type, tr->d_G, tr->d_C, tr->d_U, tr->d_Eatom,
+                    tr->d_cE, tr->d_loss, tr->d_tiles, tr->d_partials, tr->d_grad, tr->d_am, tr->d_av};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    delete tr;
+    return 0;
+}
+
+int mdt_nn_train_create(mdt_context *ctx, int nframes, const int32_t *natoms, const int32_t *idx,
+                        const double *simbox, const double *r, const double *Eref, const double *Fref,
+                        int batch_frames, double val_fraction, uint32_t seed, mdt_nntrain **out) {
+    if (!ctx || !natoms || !idx || !simbox || !r || !Eref || !Fref || !out)
+        return fail(MDT_EINVAL, "mdt_nn_train_create: NULL argument");
+    *out = nullptr;
+    if (!ctx->have_nn) return fail(MDT_ESTATE, "no network (call mdt_nn_params_set)");
+    if (ctx->nn->act3 != 0) return fail(MDT_EUNSUP, "training needs a linear output layer");
+    if (nframes < 1 || batch_frames < 1) return fail(MDT_EINVAL, "nframes=%d batch_frames=%d", nframes, batch_frames);
+    if (!(val_fraction >= 0.0 && val_fraction < 1.0)) return fail(MDT_EINVAL, "val_fraction %g", val_fraction);
+    HIPCHK(hipSetDevice(ctx->device));
+    mdt_nntrain *tr = new mdt_nntrain();
+    tr->ctx = ctx;
+    tr->nframes = nframes;
+    tr->rng = seed;
+    tr->h_natoms.assign(natoms, natoms + nframes);
+    tr->h_aoff.resize(nframes);
+    long long na = 0;
+    for (int f = 0; f < nframes; ++f) {
+        if (natoms[f] < 1 || natoms[f] > 4096) {
+            delete tr;
+            return fail(MDT_EINVAL, "frame %d: natoms=%d", f, natoms[f]);
+        }
+        tr->h_aoff[f] = na;
+        na += natoms[f];
+        tr->maxN = std::max(tr->maxN, natoms[f]);
+    }
+    tr->natoms = na;
+    const NnConst &M = *ctx->nn;
+    std::vector<uint8_t> type(na);
+    std::vector<int> afr(na);
+    tr->h_elem.resize(na);
+    for (int f = 0; f < nframes; ++f)
+        for (int i = 0; i < natoms[f]; ++i) {
+            const long long a = tr->h_aoff[f] + i;
+            const int t = idx[a] - 1;
+            if (t < 0 || t >= MDT_MAXT || M.elem_of_type[t] < 0) {
+                mdt_nn_train_destroy(tr);
+                return fail(MDT_EINVAL, "frame %d atom %d: type %d is not in the network", f, i, idx[a]);
+            }
+            type[a] = (uint8_t)t;
+            afr[a] = f;
+            tr->h_elem[a] = (uint8_t)M.elem_of_type[t];
+        }
+    std::vector<double> icell(9 * (size_t)nframes);
+    for (int f = 0; f < nframes; ++f) {
+        const double *a = simbox + 9 * f;  // column-major: a[i + 3 j]
+        double *ai = icell.data() + 9 * f;
+        const double det = a[0] * (a[4] * a[8] - a[7] * a[5]) - a[3] * (a[1] * a[8] - a[7] * a[2]) +
+                           a[6] * (a[1] * a[5] - a[4] * a[2]);
+        if (std::fabs(det) < 1e-12) {
+            mdt_nn_train_destroy(tr);
+            return fail(MDT_EINVAL, "frame %d: singular cell", f);
+        }
+        const double id = 1.0 / det;
+        ai[0] = (a[4] * a[8] - a[7] * a[5]) * id;
+        ai[3] = (a[6] * a[5] - a[3] * a[8]) * id;
+        ai[6] = (a[3] * a[7] - a[6] * a[4]) * id;
+        ai[1] = (a[7] * a[2] - a[1] * a[8]) * id;
+        ai[4] = (a[0] * a[8] - a[6] * a[2]) * id;
+        ai[7] = (a[6] * a[1] - a[0] * a[7]) * id;
+        ai[2] = (a[1] * a[5] - a[4] * a[2]) * id;
+        ai[5] = (a[3] * a[2] - a[0] * a[5]) * id;
+        ai[8] = (a[0] * a[4] - a[3] * a[1]) * id;
+    }
+    // minibatches over a seeded permutation; the last val_fraction of the
+    // permutation (whole minibatches) is held out for validation
+    std::vector<int> perm(nframes);
+    for (int f = 0; f < nframes; ++f) perm[f] = f;
+    for (int f = nframes - 1; f > 0; --f) std::swap(perm[f], perm[splitmix(tr->rng) % (uint64_t)(f + 1)]);
+    const int nmb = (nframes + batch_frames - 1) / batch_frames;
+    int nval = (int)std::floor(val_fraction * nmb + 0.5);
+    if (val_fraction > 0 && nval == 0 && nmb > 1) nval = 1;
+    if (nval >= nmb) nval = nmb - 1;
+    tr->ntrain_mb = nmb - nval;
+    std::vector<int4> tiles;
+    std::vector<int> rows;
+    tr->fr_off.push_back(0);
+    tr->tl_off.push_back(0);
+    for (int m = 0; m < nmb; ++m) {
+        const int f0 = m * batch_frames, f1 = std::min(nframes, f0 + batch_frames);
+        build_tiles(tr, perm.data() + f0, f1 - f0, M.nelem, tiles, rows, 0);
+        int nm = 0;
+        for (int q = f0; q < f1; ++q) nm += natoms[perm[q]];
+        tr->natoms_mb.push_back(nm);
+        tr->fr_off.push_back(f1);
+        tr->tl_off.push_back((int)tiles.size());
+    }
+    const size_t rowsK = (size_t)na * NN_K1;
+    const int W = tr->nblocks_grad * 4;
+    int rc = dalloc(&tr->d_natoms, nframes) | dalloc(&tr->d_atom_frame, na) | dalloc(&tr->d_frames, nframes) |
+             dalloc(&tr->d_rows, rows.size()) | dalloc(&tr->d_aoff, nframes) | dalloc(&tr->d_cell, 9 * (size_t)nframes) |
+             dalloc(&tr->d_icell, 9 * (size_t)nframes) | dalloc(&tr->d_pos, 3 * (size_t)na) |
+             dalloc(&tr->d_Fref, 3 * (size_t)na) | dalloc(&tr->d_Eref, nframes) | dalloc(&tr->d_type, na) |
+             dalloc(&tr->d_G, rowsK) | dalloc(&tr->d_C, rowsK) | dalloc(&tr->d_U, rowsK) |
+             dalloc(&tr->d_Eatom, na) | dalloc(&tr->d_cE, nframes) | dalloc(&tr->d_loss, 2 * (size_t)nframes) |
+             dalloc(&tr->d_tiles, tiles.size()) | dalloc(&tr->d_partials, (size_t)W * NN_MAXE * NN_NPAR) |
+             dalloc(&tr->d_grad, (size_t)NN_MAXE * NN_NPAR) | dalloc(&tr->d_am, (size_t)NN_MAXE * NN_NPAR) |
+             dalloc(&tr->d_av, (size_t)NN_MAXE * NN_NPAR);
+    if (rc) {
+        mdt_nn_train_destroy(tr);
+        return rc;
+    }
+    auto up = [&](void *d, const void *h, size_t n) { return hipMemcpy(d, h, n, hipMemcpyHostToDevice); };
+    HIPCHK(up(tr->d_natoms, natoms, sizeof(int) * nframes));
+    HIPCHK(up(tr->d_atom_frame, afr.data(), sizeof(int) * na));
+    HIPCHK(up(tr->d_frames, perm.data(), sizeof(int) * nframes));
+    HIPCHK(up(tr->d_rows, rows.data(), sizeof(int) * rows.size()));
+    HIPCHK(up(tr->d_aoff, tr->h_aoff.data(), sizeof(long long) * nframes));
+    HIPCHK(up(tr->d_cell, simbox, sizeof(double) * 9 * nframes));
+    HIPCHK(up(tr->d_icell, icell.data(), sizeof(double) * 9 * nframes));
+    HIPCHK(up(tr->d_pos, r, sizeof(double) * 3 * na));
+    HIPCHK(up(tr->d_Fref, Fref, sizeof(double) * 3 * na));
+    HIPCHK(up(tr->d_Eref, Eref, sizeof(double) * nframes));
+    HIPCHK(up(tr->d_type, type.data(), na));
+    HIPCHK(up(tr->d_tiles, tiles.data(), sizeof(int4) * tiles.size()));
+    HIPCHK(hipMemset(tr->d_am, 0, sizeof(float) * NN_MAXE * NN_NPAR));
+    HIPCHK(hipMemset(tr->d_av, 0, sizeof(float) * NN_MAXE * NN_NPAR));
+    HIPCHK(hipMemset(tr->d_G, 0, sizeof(float) * rowsK));
+    // descriptors of every frame (they depend on the positions only)
+    hipStream_t st = ctx->stream;
+    HIPCHK(mdt_launch_nn_frame(nframes, mdt_nn_frame_smem(tr->maxN), ctx->d_nn, tr->view(), tr->d_frames, 0, 0.0,
+                               0.0, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *out = tr;
+    return 0;
+}
+
+// min / max / mean of every symmetry function over the training frames'
+// atoms, per element (nelem x 64 each): the scaling.data statistics
+int mdt_nn_train_descriptor_stats(mdt_nntrain *tr, double *gmin, double *gmax, double *gavg) {
+    if (!tr || !gmin || !gmax || !gavg) return fail(MDT_EINVAL, "mdt_nn_train_descriptor_stats: NULL argument");
+    HIPCHK(hipSetDevice(tr->ctx->device));
+    std::vector<float> G((size_t)tr->natoms * NN_K1);
+    HIPCHK(hipMemcpy(G.data(), tr->d_G, sizeof(float) * G.size(), hipMemcpyDeviceToHost));
+    const NnConst &M = *tr->ctx->nn;
+    for (int e = 0; e < M.nelem; ++e) {
+        const int n = M.nr[e] + M.na[e];
+        std::vector<double> mn(n, 1e300), mx(n, -1e300), sm(n, 0.0);
+        long long cnt = 0;
+        for (long long a = 0; a < tr->natoms; ++a) {
+            if (tr->h_elem[a] != e) continue;
+            ++cnt;
+            for (int k = 0; k < n; ++k) {
+                const double g = G[(size_t)a * NN_K1 + k];
+                mn[k] = std::min(mn[k], g);
+                mx[k] = std::max(mx[k], g);
+                sm[k] += g;
+            }
+        }
+        for (int k = 0; k < NN_K1; ++k) {
+            gmin[e * NN_K1 + k] = k < n && cnt ? mn[k] : 0.0;
+            gmax[e * NN_K1 + k] = k < n && cnt ? mx[k] : 0.0;
+            gavg[e * NN_K1 + k] = k < n && cnt ? sm[k] / cnt : 0.0;
+        }
+    }
+    return 0;
+}
+
+// one minibatch: forward/backward, forces, residuals, JVP, gradient partials;
+// then mode 0: reduce into d_grad; mode 1: reduce + Adam
+static int run_minibatch(mdt_nntrain *tr, int m, double wE, double wF, int mode, float lr) {
+    mdt_context *c = tr->ctx;
+    hipStream_t st = c->stream;
+    const NnTrainView tv = tr->view();
+    const int nf = tr->fr_off[m + 1] - tr->fr_off[m];
+    const int t0 = tr->tl_off[m], nt = tr->tl_off[m + 1] - t0;
+    HIPCHK(mdt_launch_nn_mlp(nt, c->d_nn, tv, tr->d_tiles + t0, tr->d_rows, st));
+    HIPCHK(mdt_launch_nn_frame(nf, mdt_nn_frame_smem(tr->maxN), c->d_nn, tv, tr->d_frames + tr->fr_off[m], 1,
+                               wE / nf, wF / (3.0 * tr->natoms_mb[m]), st));
+    if (mode < 0) return 0;  // evaluation only
+    const int W = tr->nblocks_grad * 4;
+    HIPCHK(hipMemsetAsync(tr->d_partials, 0, sizeof(float) * (size_t)W * NN_MAXE * NN_NPAR, st));
+    HIPCHK(mdt_launch_nn_grad(tr->nblocks_grad, c->d_nn, tv, tr->d_tiles + t0, nt, tr->d_rows, tr->d_partials, st));
+    if (mode == 1) ++tr->adam_step;
+    HIPCHK(mdt_launch_nn_adam(c->d_nn, tr->d_partials, W, tr->d_grad, tr->d_am, tr->d_av, lr, 0.9f, 0.999f, 1e-8f,
+                              tr->adam_step > 0 ? tr->adam_step : 1, mode, st));
+    return 0;
+}
+
+static int sum_loss(mdt_nntrain *tr, int m0, int m1, double *lE, double *lF) {
+    std::vector<double> loss(2 * (size_t)tr->nframes);
+    std::vector<int> perm(tr->nframes);
+    HIPCHK(hipMemcpy(loss.data(), tr->d_loss, sizeof(double) * loss.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(perm.data(), tr->d_frames, sizeof(int) * perm.size(), hipMemcpyDeviceToHost));
+    double se = 0.0, sf = 0.0;
+    for (int q = tr->fr_off[m0]; q < tr->fr_off[m1]; ++q) {
+        se += loss[2 * perm[q]];
+        sf += loss[2 * perm[q] + 1];
+    }
+    *lE = se;
+    *lF = sf;
+    return 0;
+}
+
+int mdt_nn_train_epoch(mdt_nntrain *tr, double lr, double wE, double wF, double *loss) {
+    if (!tr) return fail(MDT_EINVAL, "NULL trainer");
+    HIPCHK(hipSetDevice(tr->ctx->device));
+    std::vector<int> order(tr->ntrain_mb);
+    for (int m = 0; m < tr->ntrain_mb; ++m) order[m] = m;
+    for (int m = tr->ntrain_mb - 1; m > 0; --m) std::swap(order[m], order[splitmix(tr->rng) % (uint64_t)(m + 1)]);
+    int rc;
+    for (int m : order)
+        if ((rc = run_minibatch(tr, m, wE, wF, 1, (float)lr))) return rc;
+    HIPCHK(hipStreamSynchronize(tr->ctx->stream));
+    HIPCHK(hipMemcpy(tr->ctx->nn, tr->ctx->d_nn, sizeof(NnConst), hipMemcpyDeviceToHost));
+    if (loss) {
+        double lE, lF;
+        if ((rc = sum_loss(tr, 0, tr->ntrain_mb, &lE, &lF))) return rc;
+        *loss = (lE + lF) / std::max(1, tr->ntrain_mb);
+    }
+    return 0;
+}
+
+// energy RMSE per atom [eV/atom] and force-component RMSE [eV/A] of the
+// current network over the training (set 0) or validation (set 1) frames
+int mdt_nn_train_eval(mdt_nntrain *tr, int set, double *rmse_e, double *rmse_f) {
+    if (!tr || !rmse_e || !rmse_f) return fail(MDT_EINVAL, "mdt_nn_train_eval: NULL argument");
+    HIPCHK(hipSetDevice(tr->ctx->device));
+    const int nmb = (int)tr->fr_off.size() - 1;
+    const int m0 = set == 0 ? 0 : tr->ntrain_mb, m1 = set == 0 ? tr->ntrain_mb : nmb;
+    if (m1 <= m0) return fail(MDT_ESTATE, "set %d is empty", set);
+    int rc;
+    long long nat = 0;
+    for (int m = m0; m < m1; ++m) {
+        const int nf = tr->fr_off[m + 1] - tr->fr_off[m];
+        // cEw = nf, cFw = 3 natoms_mb: loss parts become dE^2 per atom^2 and |dF|^2
+        if ((rc = run_minibatch(tr, m, (double)nf, 3.0 * tr->natoms_mb[m], -1, 0.f))) return rc;
+        nat += tr->natoms_mb[m];
+    }
+    HIPCHK(hipStreamSynchronize(tr->ctx->stream));
+    double lE, lF;
+    if ((rc = sum_loss(tr, m0, m1, &lE, &lF))) return rc;
+    *rmse_e = std::sqrt(lE / (tr->fr_off[m1] - tr->fr_off[m0]));
+    *rmse_f = std::sqrt(lF / (3.0 * nat));
+    return 0;
+}
+
+// padded-layout index of RuNNer weight q of element e (nin x n1 x n2 x 1)
+static int runner_to_padded(int q, int nin, int n1, int n2) {
+    if (q < nin * n1) return NN_PW1 + (q / n1) * NN_H + q % n1;
+    q -= nin * n1;
+    if (q < n1) return NN_PB1 + q;
+    q -= n1;
+    if (q < n1 * n2) return NN_PW2 + (q / n2) * NN_H + q % n2;
+    q -= n1 * n2;
+    if (q < n2) return NN_PB2 + q;
+    q -= n2;
+    if (q < n2) return NN_PW3 + q;
+    return NN_PB3;
+}
+
+// Loss and gradient (RuNNer weight layout, elements concatenated) over the
+// given frames, without updating the network
+int mdt_nn_train_loss_grad(mdt_nntrain *tr, int nsel, const int32_t *frames, double wE, double wF, double *loss,
+                           double *grad) {
+    if (!tr || !frames || !loss || !grad) return fail(MDT_EINVAL, "mdt_nn_train_loss_grad: NULL argument");
+    if (nsel < 1) return fail(MDT_EINVAL, "nsel=%d", nsel);
+    HIPCHK(hipSetDevice(tr->ctx->device));
+    mdt_context *c = tr->ctx;
+    const NnConst &M = *c->nn;
+    std::vector<int4> tiles;
+    std::vector<int> rows;
+    int nat = 0;
+    for (int q = 0; q < nsel; ++q) {
+        if (frames[q] < 0 || frames[q] >= tr->nframes) return fail(MDT_EINVAL, "frame %d", frames[q]);
+        nat += tr->h_natoms[frames[q]];
+    }
+    build_tiles(tr, frames, nsel, M.nelem, tiles, rows, 0);
+    int4 *dt = nullptr;
+    int *dr = nullptr, *df = nullptr;
+    int rc = dalloc(&dt, tiles.size()) | dalloc(&dr, rows.size()) | dalloc(&df, nsel);
+    if (rc) return rc;
+    hipStream_t st = c->stream;
+    const NnTrainView tv = tr->view();
+    const int W = tr->nblocks_grad * 4;
+    hipError_t e1 = hipMemcpy(dt, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice);
+    if (e1 == hipSuccess) e1 = hipMemcpy(dr, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice);
+    if (e1 == hipSuccess) e1 = hipMemcpy(df, frames, sizeof(int) * nsel, hipMemcpyHostToDevice);
+    if (e1 == hipSuccess) e1 = mdt_launch_nn_mlp((int)tiles.size(), c->d_nn, tv, dt, dr, st);
+    if (e1 == hipSuccess)
+        e1 = mdt_launch_nn_frame(nsel, mdt_nn_frame_smem(tr->maxN), c->d_nn, tv, df, 1, wE / nsel, wF / (3.0 * nat), st);
+    if (e1 == hipSuccess)
+        e1 = hipMemsetAsync(tr->d_partials, 0, sizeof(float) * (size_t)W * NN_MAXE * NN_NPAR, st);
+    if (e1 == hipSuccess)
+        e1 = mdt_launch_nn_grad(tr->nblocks_grad, c->d_nn, tv, dt, (int)tiles.size(), dr, tr->d_partials, st);
+    if (e1 == hipSuccess)
+        e1 = mdt_launch_nn_adam(c->d_nn, tr->d_partials, W, tr->d_grad, tr->d_am, tr->d_av, 0.f, 0.9f, 0.999f,
+                                1e-8f, 1, 0, st);
+    if (e1 == hipSuccess) e1 = hipStreamSynchronize(st);
+    (void)hipFree(dt);
+    (void)hipFree(dr);
+    (void)hipFree(df);
+    if (e1 != hipSuccess) return fail(MDT_EHIP, "training gradient: %s", hipGetErrorString(e1));
+    std::vector<float> g((size_t)NN_MAXE * NN_NPAR);
+    std::vector<double> ls(2 * (size_t)tr->nframes);
+    HIPCHK(hipMemcpy(g.data(), tr->d_grad, sizeof(float) * g.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(ls.data(), tr->d_loss, sizeof(double) * ls.size(), hipMemcpyDeviceToHost));
+    double L = 0.0;
+    for (int q = 0; q < nsel; ++q) L += ls[2 * frames[q]] + ls[2 * frames[q] + 1];
+    *loss = L;
+    long o = 0;
+    for (int e = 0; e < M.nelem; ++e) {
+        const int nin = M.nr[e] + M.na[e], n1 = M.nh1, n2 = M.nh2;
+        const int nw = nin * n1 + n1 + n1 * n2 + n2 + n2 + 1;
+        for (int q = 0; q < nw; ++q) grad[o + q] = g[(size_t)e * NN_NPAR + runner_to_padded(q, nin, n1, n2)];
+        o += nw;
+    }
+    return 0;
+}
+
+// current weights in the weights.XXX.data layout, elements concatenated
+int mdt_nn_get_weights(mdt_context *ctx, double *weights) {
+    if (!ctx || !weights) return fail(MDT_EINVAL, "mdt_nn_get_weights: NULL argument");
+    if (!ctx->have_nn) return fail(MDT_ESTATE, "no network");
+    HIPCHK(hipSetDevice(ctx->device));
+    HIPCHK(hipMemcpy(ctx->nn, ctx->d_nn, sizeof(NnConst), hipMemcpyDeviceToHost));
+    const NnConst &M = *ctx->nn;
+    long o = 0;
+    for (int e = 0; e < M.nelem; ++e) {
+        const int nin = M.nr[e] + M.na[e], n1 = M.nh1, n2 = M.nh2;
+        for (int i = 0; i < nin; ++i)
+            for (int j = 0; j < n1; ++j) weights[o++] = M.W1[e][i][j];
+        for (int j = 0; j < n1; ++j) weights[o++] = M.b1[e][j];
+        for (int i = 0; i < n1; ++i)
+            for (int j = 0; j < n2; ++j) weights[o++] = M.W2[e][i][j];
+        for (int j = 0; j < n2; ++j) weights[o++] = M.b2[e][j];
+        for (int j = 0; j < n2; ++j) weights[o++] = M.w3[e][j];
+        weights[o++] = M.b3[e];
+    }
     return 0;
 }
 

@@ -134,6 +134,33 @@ struct NnConst {
     float W1[NN_MAXE][NN_K1][NN_H], b1[NN_MAXE][NN_H];
     float W2[NN_MAXE][NN_H][NN_H], b2[NN_MAXE][NN_H];
     float w3[NN_MAXE][NN_H], b3[NN_MAXE];
+    int nh1, nh2;                                       // real hidden widths (<= NN_H)
+};
+
+// padded gradient / Adam layout of one element's parameters
+#define NN_PW1 0
+#define NN_PB1 (NN_K1 * NN_H)
+#define NN_PW2 (NN_PB1 + NN_H)
+#define NN_PB2 (NN_PW2 + NN_H * NN_H)
+#define NN_PW3 (NN_PB2 + NN_H)
+#define NN_PB3 (NN_PW3 + NN_H)
+#define NN_NPAR (NN_PB3 + 1)
+
+// training set resident on the GPU (frames of atoms, reference energies and forces)
+struct NnTrainView {
+    int nframes;
+    const int *fr_natoms;        // per frame
+    const long long *fr_aoff;    // first atom of the frame
+    const double *fr_cell, *fr_icell;  // 9 per frame, Angstrom, column-major
+    const uint8_t *type;         // per atom, 0-based type
+    const double *pos;           // per atom x, y, z (Angstrom)
+    const double *Fref;          // per atom (eV/A)
+    const double *Eref;          // per frame (eV)
+    const int *atom_frame;       // per atom
+    float *G, *C, *U;            // per atom NN_K1: raw symmetry functions, dE/dG, dLoss/d(dE/dx)
+    double *Eatom;               // per atom (Hartree)
+    double *cE;                  // per frame: dLoss/dE (per Hartree)
+    double *loss;                // per frame: energy part, force part
 };
 
 // per-replica NN status bit (in the trajectory status word)

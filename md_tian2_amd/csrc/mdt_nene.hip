@@ -333,6 +333,356 @@ __device__ void mlp_tile(const NnConst &M, int e, const int *rows, const float *
     }
 }
 
+// Symmetry functions of the chunk's centre atoms (16 lanes each) into
+// Grep rows; *extrap set when a value leaves [min - 1e-4, max + 1e-4].
+__device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem, int N, int k0, int lane,
+                        float rc2, float pi_rc, float *Grep, int *extrapp) {
+    const int c = lane / NN_TPA, sub = lane % NN_TPA;
+    int extrap = 0;
+    struct {
+        float rc2, pi_rc;
+    } M{rc2, pi_rc};
+    const int i = k0 + c;
+    const int cnt = L.cnt[c];
+    const int e = (i < N) ? elem[i] : 0;
+    const int nr = (i < N) ? S.nr[e] : 0, na = (i < N) ? S.na[e] : 0;
+    const int base = c * NN_CAP;
+    float g[NN_MAXR > NN_MAXA ? NN_MAXR : NN_MAXA];
+    // radial (atomsymfunction2: exp(-eta (r - rs)^2) fc(r))
+#pragma unroll
+    for (int k = 0; k < NN_MAXR; ++k) g[k] = 0.f;
+    for (int p = sub; p < cnt; p += NN_TPA) {
+        const float r = L.r[base + p], fcv = L.fc[base + p];
+        const int ep = L.el[base + p];
+#pragma unroll
+        for (int k = 0; k < NN_MAXR; ++k)
+            if (k < nr && ep == S.r_el[e][k]) {
+                const float dr = r - S.r_rs[e][k];
+                g[k] += __builtin_amdgcn_exp2f(-S.r_eta[e][k] * NN_L2E * dr * dr) * fcv;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < NN_MAXR; ++k) {
+        const float v = sum_group<NN_TPA>(g[k]);
+        if (i < N && k < nr && sub == (k & (NN_TPA - 1))) {
+            Grep[(size_t)i * NN_K1 + k] = v;
+            if (v < S.glo[e][k] || v > S.ghi[e][k]) extrap = 1;
+        }
+    }
+    // angular (atomsymfunction3Andi): pairs p < q of the neighbour list
+#pragma unroll
+    for (int k = 0; k < NN_MAXA; ++k) g[k] = 0.f;
+    for (int p = sub; p < cnt; p += NN_TPA) {
+        const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
+        const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p];
+        const int ep = L.el[base + p];
+        for (int q = p + 1; q < cnt; ++q) {
+            const float bx = L.dx[base + q], by = L.dy[base + q], bz = L.dz[base + q];
+            const float cx = bx - ax_, cy = by - ay_, cz = bz - az_;
+            const float rc2v = cx * cx + cy * cy + cz * cz;
+            if (rc2v > M.rc2) continue;
+            const float rb = L.r[base + q];
+            const int eq = L.el[base + q];
+            const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
+            const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * L.ir[base + q];
+            float fcc, dfcc;
+            cutoff(M.pi_rc, sqrtf(rc2v), &fcc, &dfcc);
+            const float Cf = fa * L.fc[base + q] * fcc;
+            const float Sd = ra * ra + rb * rb + rc2v;
+            float B = 0.f, etaprev = -1.f;
+#pragma unroll
+            for (int k = 0; k < NN_MAXA; ++k)
+                if (k < na && cls == S.a_cls[e][k]) {
+                    const float x = 1.0f + S.a_lam[e][k] * cth;
+                    if (x > 0.f) {
+                        float pw = x;
+                        for (int z = 0; z < S.a_zp[e][k]; ++z) pw *= pw;
+                        if (S.a_etal2[e][k] != etaprev) {
+                            etaprev = S.a_etal2[e][k];
+                            B = __builtin_amdgcn_exp2f(-etaprev * Sd);
+                        }
+                        g[k] += S.a_norm[e][k] * pw * B * Cf;
+                    }
+                }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NN_MAXA; ++k) {
+        const float v = sum_group<NN_TPA>(g[k]);
+        if (i < N && k < na && sub == (k & (NN_TPA - 1))) {
+            Grep[(size_t)i * NN_K1 + nr + k] = v;
+            if (v < S.glo[e][nr + k] || v > S.ghi[e][nr + k]) extrap = 1;
+        }
+    }
+    if (extrap) *extrapp = 1;
+}
+
+// dE/dr of the chunk's centre atoms' symmetry functions with coefficients
+// cb = dE/dG (VJP): centre force in registers, neighbours' shares into the
+// fixed-point accumulators ax, ay, az (Hartree/Bohr).
+__device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*cb)[NN_K1], const uint8_t *elem,
+                            int N, int k0, int lane, float rc2, float pi_rc, unsigned long long *ax,
+                            unsigned long long *ay, unsigned long long *az, int *badp) {
+    const int c = lane / NN_TPA, sub = lane % NN_TPA;
+    int bad = 0;
+    struct {
+        float rc2, pi_rc;
+    } M{rc2, pi_rc};
+    const int i = k0 + c;
+    const int cnt = L.cnt[c];
+    const int e = (i < N) ? elem[i] : 0;
+    const int nr = (i < N) ? S.nr[e] : 0, na = (i < N) ? S.na[e] : 0;
+    const int base = c * NN_CAP;
+    float Fi0 = 0.f, Fi1 = 0.f, Fi2 = 0.f;  // force on the centre (Hartree/Bohr)
+    // radial: dG/dr = (-2 eta (r - rs) fc + dfc) exp(-eta (r - rs)^2)
+    for (int p = sub; p < cnt; p += NN_TPA) {
+        const float r = L.r[base + p], fcv = L.fc[base + p], dfcv = L.dfc[base + p];
+        const int ep = L.el[base + p];
+        float s = 0.f;
+#pragma unroll
+        for (int k = 0; k < NN_MAXR; ++k)
+            if (k < nr && ep == S.r_el[e][k]) {
+                const float dr = r - S.r_rs[e][k];
+                const float ex = __builtin_amdgcn_exp2f(-S.r_eta[e][k] * NN_L2E * dr * dr);
+                s += cb[c][k] * ex * (dfcv - 2.f * S.r_eta[e][k] * dr * fcv);
+            }
+        const float ir = L.ir[base + p];
+        const float v0 = s * L.dx[base + p] * ir, v1 = s * L.dy[base + p] * ir, v2 = s * L.dz[base + p] * ir;
+        Fi0 += v0;
+        Fi1 += v1;
+        Fi2 += v2;
+        const int j = L.j[base + p];
+        fxadd(&ax[j], -(double)v0, &bad);
+        fxadd(&ay[j], -(double)v1, &bad);
+        fxadd(&az[j], -(double)v2, &bad);
+    }
+    // angular
+    for (int p = sub; p < cnt; p += NN_TPA) {
+        const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
+        const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p], dfa = L.dfc[base + p];
+        const int ep = L.el[base + p];
+        float Fj0 = 0.f, Fj1 = 0.f, Fj2 = 0.f;
+        for (int q = p + 1; q < cnt; ++q) {
+            const float bx = L.dx[base + q], by = L.dy[base + q], bz = L.dz[base + q];
+            const float cx = bx - ax_, cy = by - ay_, cz = bz - az_;
+            const float rc2v = cx * cx + cy * cy + cz * cz;
+            if (rc2v > M.rc2) continue;
+            const float rb = L.r[base + q], ib = L.ir[base + q], fb = L.fc[base + q], dfb = L.dfc[base + q];
+            const int eq = L.el[base + q];
+            const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
+            const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * ib;
+            const float rcc = sqrtf(rc2v);
+            float fcc, dfcc;
+            cutoff(M.pi_rc, rcc, &fcc, &dfcc);
+            const float Cf = fa * fb * fcc;
+            const float Sd = ra * ra + rb * rb + rc2v;
+            float Sg = 0.f, Sc = 0.f, Ss = 0.f, B = 0.f, etaprev = -1.f;
+#pragma unroll
+            for (int k = 0; k < NN_MAXA; ++k)
+                if (k < na && cls == S.a_cls[e][k]) {
+                    const float lam = S.a_lam[e][k];
+                    const float x = 1.0f + lam * cth;
+                    if (x > 0.f) {
+                        float pw = x, pm = 1.f;  // x^zeta, x^(zeta-1)
+                        for (int z = 0; z < S.a_zp[e][k]; ++z) {
+                            pm *= pw;
+                            pw *= pw;
+                        }
+                        if (S.a_etal2[e][k] != etaprev) {
+                            etaprev = S.a_etal2[e][k];
+                            B = __builtin_amdgcn_exp2f(-etaprev * Sd);
+                        }
+                        const float ck = cb[c][nr + k] * S.a_norm[e][k] * B;
+                        const float zeta = (float)(1 << S.a_zp[e][k]);
+                        Sg += ck * pw;
+                        Sc += ck * zeta * lam * pm;
+                        Ss += ck * S.a_eta[e][k] * pw;
+                    }
+                }
+            if (Sg == 0.f && Sc == 0.f) continue;
+            const float iab = ia * ib;
+            const float tc = Cf * Sc, ts = -2.f * Cf * Ss;
+            const float ua = Sg * dfa * fb * fcc * ia, ub = Sg * fa * dfb * fcc * ib;
+            const float uc = Sg * fa * fb * dfcc / rcc;
+            const float ga0 = tc * (bx * iab - cth * ax_ * ia * ia) + ts * ax_ + ua * ax_;
+            const float ga1 = tc * (by * iab - cth * ay_ * ia * ia) + ts * ay_ + ua * ay_;
+            const float ga2 = tc * (bz * iab - cth * az_ * ia * ia) + ts * az_ + ua * az_;
+            const float gb0 = tc * (ax_ * iab - cth * bx * ib * ib) + ts * bx + ub * bx;
+            const float gb1 = tc * (ay_ * iab - cth * by * ib * ib) + ts * by + ub * by;
+            const float gb2 = tc * (az_ * iab - cth * bz * ib * ib) + ts * bz + ub * bz;
+            const float gc0 = (ts + uc) * cx, gc1 = (ts + uc) * cy, gc2 = (ts + uc) * cz;
+            Fi0 += ga0 + gb0;
+            Fi1 += ga1 + gb1;
+            Fi2 += ga2 + gb2;
+            Fj0 -= ga0 - gc0;
+            Fj1 -= ga1 - gc1;
+            Fj2 -= ga2 - gc2;
+            const int kq = L.j[base + q];
+            fxadd(&ax[kq], -(double)(gb0 + gc0), &bad);
+            fxadd(&ay[kq], -(double)(gb1 + gc1), &bad);
+            fxadd(&az[kq], -(double)(gb2 + gc2), &bad);
+        }
+        const int j = L.j[base + p];
+        fxadd(&ax[j], (double)Fj0, &bad);
+        fxadd(&ay[j], (double)Fj1, &bad);
+        fxadd(&az[j], (double)Fj2, &bad);
+    }
+    const float S0 = sum_group<NN_TPA>(Fi0), S1 = sum_group<NN_TPA>(Fi1), S2 = sum_group<NN_TPA>(Fi2);
+    if (i < N && sub == 0) {
+        fxadd(&ax[i], (double)S0, &bad);
+        fxadd(&ay[i], (double)S1, &bad);
+        fxadd(&az[i], (double)S2, &bad);
+    }
+    if (bad) *badp |= bad;
+}
+
+// u_ik = sum_a dG_ik/dr_a . g_a for the chunk's centre atoms i (the
+// Jacobian-vector product of the training gradient; g per atom in LDS).
+// Per neighbour pair: g_j - g_i projected on the pair direction; per
+// neighbour triple: the three difference vectors dotted with the angular
+// function's partial vectors, shared over all functions of the triple.
+// Result scaled by -dfac (u = dLoss/d(dE/dx)) into Urep rows.
+__device__ void jvp_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem, int N, int k0, int lane,
+                          float rc2, float pi_rc, const float *gx, const float *gy, const float *gz,
+                          const float (*dfac)[NN_K1], float *Urep) {
+    const int c = lane / NN_TPA, sub = lane % NN_TPA;
+    const int i = k0 + c;
+    const int cnt = L.cnt[c];
+    const int e = (i < N) ? elem[i] : 0;
+    const int nr = (i < N) ? S.nr[e] : 0, na = (i < N) ? S.na[e] : 0;
+    const int base = c * NN_CAP;
+    const float gix = (i < N) ? gx[i] : 0.f, giy = (i < N) ? gy[i] : 0.f, giz = (i < N) ? gz[i] : 0.f;
+    float P[NN_MAXR > NN_MAXA ? NN_MAXR : NN_MAXA];
+#pragma unroll
+    for (int k = 0; k < NN_MAXR; ++k) P[k] = 0.f;
+    for (int p = sub; p < cnt; p += NN_TPA) {
+        const float r = L.r[base + p], fcv = L.fc[base + p], dfcv = L.dfc[base + p], ir = L.ir[base + p];
+        const int j = L.j[base + p];
+        const float proj = (L.dx[base + p] * (gx[j] - gix) + L.dy[base + p] * (gy[j] - giy) +
+                            L.dz[base + p] * (gz[j] - giz)) * ir;
+        const int ep = L.el[base + p];
+#pragma unroll
+        for (int k = 0; k < NN_MAXR; ++k)
+            if (k < nr && ep == S.r_el[e][k]) {
+                const float dr = r - S.r_rs[e][k];
+                const float ex = __builtin_amdgcn_exp2f(-S.r_eta[e][k] * NN_L2E * dr * dr);
+                P[k] += ex * (dfcv - 2.f * S.r_eta[e][k] * dr * fcv) * proj;
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < NN_MAXR; ++k) {
+        const float v = sum_group<NN_TPA>(P[k]);
+        if (i < N && k < nr && sub == (k & (NN_TPA - 1))) Urep[(size_t)i * NN_K1 + k] = -dfac[e][k] * v;
+    }
+#pragma unroll
+    for (int k = 0; k < NN_MAXA; ++k) P[k] = 0.f;
+    for (int p = sub; p < cnt; p += NN_TPA) {
+        const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
+        const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p], dfa = L.dfc[base + p];
+        const int ep = L.el[base + p];
+        const int jp = L.j[base + p];
+        const float Dja0 = gx[jp] - gix, Dja1 = gy[jp] - giy, Dja2 = gz[jp] - giz;
+        for (int q = p + 1; q < cnt; ++q) {
+            const float bx = L.dx[base + q], by = L.dy[base + q], bz = L.dz[base + q];
+            const float cx = bx - ax_, cy = by - ay_, cz = bz - az_;
+            const float rc2v = cx * cx + cy * cy + cz * cz;
+            if (rc2v > rc2) continue;
+            const float rb = L.r[base + q], ib = L.ir[base + q], fb = L.fc[base + q], dfb = L.dfc[base + q];
+            const int eq = L.el[base + q];
+            const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
+            const int kq = L.j[base + q];
+            const float Dka0 = gx[kq] - gix, Dka1 = gy[kq] - giy, Dka2 = gz[kq] - giz;
+            const float Dkj0 = Dka0 - Dja0, Dkj1 = Dka1 - Dja1, Dkj2 = Dka2 - Dja2;
+            const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * ib;
+            const float rcc = sqrtf(rc2v);
+            float fcc, dfcc;
+            cutoff(pi_rc, rcc, &fcc, &dfcc);
+            const float Cf = fa * fb * fcc;
+            const float Sd = ra * ra + rb * rb + rc2v;
+            const float iab = ia * ib;
+            const float aD = ax_ * Dja0 + ay_ * Dja1 + az_ * Dja2;   // a . Dja
+            const float bD = bx * Dka0 + by * Dka1 + bz * Dka2;      // b . Dka
+            const float cD = cx * Dkj0 + cy * Dkj1 + cz * Dkj2;      // c . Dkj
+            const float bDa = bx * Dja0 + by * Dja1 + bz * Dja2;     // b . Dja
+            const float aDb = ax_ * Dka0 + ay_ * Dka1 + az_ * Dka2;  // a . Dka
+            const float dcosD = (bDa * iab - cth * aD * ia * ia) + (aDb * iab - cth * bD * ib * ib);
+            const float SD = aD + bD + cD;
+            const float FD = dfa * fb * fcc * aD * ia + fa * dfb * fcc * bD * ib + fa * fb * dfcc * cD / rcc;
+            float B = 0.f, etaprev = -1.f;
+#pragma unroll
+            for (int k = 0; k < NN_MAXA; ++k)
+                if (k < na && cls == S.a_cls[e][k]) {
+                    const float lam = S.a_lam[e][k];
+                    const float x = 1.0f + lam * cth;
+                    if (x > 0.f) {
+                        float pw = x, pm = 1.f;
+                        for (int z = 0; z < S.a_zp[e][k]; ++z) {
+                            pm *= pw;
+                            pw *= pw;
+                        }
+                        if (S.a_etal2[e][k] != etaprev) {
+                            etaprev = S.a_etal2[e][k];
+                            B = __builtin_amdgcn_exp2f(-etaprev * Sd);
+                        }
+                        const float zeta = (float)(1 << S.a_zp[e][k]);
+                        P[k] += S.a_norm[e][k] * B *
+                                (zeta * lam * pm * Cf * dcosD - 2.f * S.a_eta[e][k] * pw * Cf * SD + pw * FD);
+                    }
+                }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NN_MAXA; ++k) {
+        const float v = sum_group<NN_TPA>(P[k]);
+        if (i < N && k < na && sub == (k & (NN_TPA - 1))) Urep[(size_t)i * NN_K1 + nr + k] = -dfac[e][nr + k] * v;
+    }
+}
+
+// LDS copy of the symmetry-function tables and extrapolation bounds
+__device__ void load_sftab(NnSfTab &S, const NnConst &M, int lane) {
+    for (int k = lane; k < NN_MAXE; k += NN_BLOCK) {
+        S.nr[k] = M.nr[k];
+        S.na[k] = M.na[k];
+    }
+    for (int k = lane; k < NN_MAXE * NN_MAXR; k += NN_BLOCK) {
+        const int e = k / NN_MAXR, q = k % NN_MAXR;
+        S.r_el[e][q] = M.r_el[e][q];
+        S.r_eta[e][q] = M.r_eta[e][q];
+        S.r_rs[e][q] = M.r_rs[e][q];
+    }
+    for (int k = lane; k < NN_MAXE * NN_MAXA; k += NN_BLOCK) {
+        const int e = k / NN_MAXA, q = k % NN_MAXA;
+        S.a_cls[e][q] = M.a_e1[e][q] * NN_MAXE + M.a_e2[e][q];
+        S.a_etal2[e][q] = M.a_etal2[e][q];
+        S.a_eta[e][q] = M.a_etal2[e][q] * (1.0f / NN_L2E);
+        S.a_lam[e][q] = M.a_lam[e][q];
+        S.a_norm[e][q] = M.a_norm[e][q];
+        S.a_zp[e][q] = M.a_zp[e][q];
+    }
+    for (int k = lane; k < NN_MAXE * NN_K1; k += NN_BLOCK) {
+        const int e = k / NN_K1, q = k % NN_K1;
+        S.glo[e][q] = M.glo[e][q];
+        S.ghi[e][q] = M.ghi[e][q];
+    }
+}
+
+// cell in Bohr (columns = lattice vectors), its inverse in 1/Angstrom, and
+// per axis the fractional half-width of the cut-off sphere rc |row k of H^-1|
+// and the largest image shift to try
+__device__ void cell_setup(const NnConst &M, const double *cellA, const double *icellA, double h[9], double ih[9],
+                           float w[3], int imax[3]) {
+    for (int k = 0; k < 9; ++k) {
+        h[k] = cellA[k] * (double)M.ang2bohr;
+        ih[k] = icellA[k];
+    }
+    const double rcb = (double)M.rc;
+    for (int k = 0; k < 3; ++k) {
+        const double bk = sqrt(ih[k] * ih[k] + ih[k + 3] * ih[k + 3] + ih[k + 6] * ih[k + 6]) / (double)M.ang2bohr;
+        w[k] = (float)(rcb * bk);
+        imax[k] = (int)ceil(rcb * bk) + 1;
+    }
+}
+
 __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
                                                    int add_fin, int skip_extrap) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -365,45 +715,11 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
 
     // ---- tables ----
     const int ne_model = M.nelem;
-    for (int k = lane; k < NN_MAXE; k += NN_BLOCK) {
-        S.nr[k] = M.nr[k];
-        S.na[k] = M.na[k];
-    }
-    for (int k = lane; k < NN_MAXE * NN_MAXR; k += NN_BLOCK) {
-        const int e = k / NN_MAXR, q = k % NN_MAXR;
-        S.r_el[e][q] = M.r_el[e][q];
-        S.r_eta[e][q] = M.r_eta[e][q];
-        S.r_rs[e][q] = M.r_rs[e][q];
-    }
-    for (int k = lane; k < NN_MAXE * NN_MAXA; k += NN_BLOCK) {
-        const int e = k / NN_MAXA, q = k % NN_MAXA;
-        S.a_cls[e][q] = M.a_e1[e][q] * NN_MAXE + M.a_e2[e][q];
-        S.a_etal2[e][q] = M.a_etal2[e][q];
-        S.a_eta[e][q] = M.a_etal2[e][q] * (1.0f / NN_L2E);
-        S.a_lam[e][q] = M.a_lam[e][q];
-        S.a_norm[e][q] = M.a_norm[e][q];
-        S.a_zp[e][q] = M.a_zp[e][q];
-    }
-    for (int k = lane; k < NN_MAXE * NN_K1; k += NN_BLOCK) {
-        const int e = k / NN_K1, q = k % NN_K1;
-        S.glo[e][q] = M.glo[e][q];
-        S.ghi[e][q] = M.ghi[e][q];
-    }
-    // cell in Bohr (columns = lattice vectors) and the fractional half-widths
-    // of the cut-off sphere, rc |row k of H^-1|
+    load_sftab(S, M, lane);
     double h[9], ih[9];
-    for (int k = 0; k < 9; ++k) {
-        h[k] = bv.cell[9 * t + k] * (double)M.ang2bohr;
-        ih[k] = bv.icell[9 * t + k];
-    }
-    const double rcb = (double)M.rc;
     float w[3];
     int imax[3];
-    for (int k = 0; k < 3; ++k) {
-        const double bk = sqrt(ih[k] * ih[k] + ih[k + 3] * ih[k + 3] + ih[k + 6] * ih[k + 6]) / (double)M.ang2bohr;
-        w[k] = (float)(rcb * bk);
-        imax[k] = (int)ceil(rcb * bk) + 1;
-    }
+    cell_setup(M, bv.cell + 9 * t, bv.icell + 9 * t, h, ih, w, imax);
 
     // ---- stage ----
     for (int i = lane; i < N; i += NN_BLOCK) {
@@ -444,78 +760,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
     for (int k0 = 0; k0 < N; k0 += NN_CA) {
         build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
         __syncthreads();
-        const int i = k0 + c;
-        const int cnt = L.cnt[c];
-        const int e = (i < N) ? elem[i] : 0;
-        const int nr = (i < N) ? S.nr[e] : 0, na = (i < N) ? S.na[e] : 0;
-        const int base = c * NN_CAP;
-        float g[NN_MAXR > NN_MAXA ? NN_MAXR : NN_MAXA];
-        // radial (atomsymfunction2: exp(-eta (r - rs)^2) fc(r))
-#pragma unroll
-        for (int k = 0; k < NN_MAXR; ++k) g[k] = 0.f;
-        for (int p = sub; p < cnt; p += NN_TPA) {
-            const float r = L.r[base + p], fcv = L.fc[base + p];
-            const int ep = L.el[base + p];
-#pragma unroll
-            for (int k = 0; k < NN_MAXR; ++k)
-                if (k < nr && ep == S.r_el[e][k]) {
-                    const float dr = r - S.r_rs[e][k];
-                    g[k] += __builtin_amdgcn_exp2f(-S.r_eta[e][k] * NN_L2E * dr * dr) * fcv;
-                }
-        }
-#pragma unroll
-        for (int k = 0; k < NN_MAXR; ++k) {
-            const float v = sum_group<NN_TPA>(g[k]);
-            if (i < N && k < nr && sub == (k & (NN_TPA - 1))) {
-                Grep[(size_t)i * NN_K1 + k] = v;
-                if (v < S.glo[e][k] || v > S.ghi[e][k]) extrap = 1;
-            }
-        }
-        // angular (atomsymfunction3Andi): pairs p < q of the neighbour list
-#pragma unroll
-        for (int k = 0; k < NN_MAXA; ++k) g[k] = 0.f;
-        for (int p = sub; p < cnt; p += NN_TPA) {
-            const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
-            const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p];
-            const int ep = L.el[base + p];
-            for (int q = p + 1; q < cnt; ++q) {
-                const float bx = L.dx[base + q], by = L.dy[base + q], bz = L.dz[base + q];
-                const float cx = bx - ax_, cy = by - ay_, cz = bz - az_;
-                const float rc2v = cx * cx + cy * cy + cz * cz;
-                if (rc2v > M.rc2) continue;
-                const float rb = L.r[base + q];
-                const int eq = L.el[base + q];
-                const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
-                const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * L.ir[base + q];
-                float fcc, dfcc;
-                cutoff(M.pi_rc, sqrtf(rc2v), &fcc, &dfcc);
-                const float Cf = fa * L.fc[base + q] * fcc;
-                const float Sd = ra * ra + rb * rb + rc2v;
-                float B = 0.f, etaprev = -1.f;
-#pragma unroll
-                for (int k = 0; k < NN_MAXA; ++k)
-                    if (k < na && cls == S.a_cls[e][k]) {
-                        const float x = 1.0f + S.a_lam[e][k] * cth;
-                        if (x > 0.f) {
-                            float pw = x;
-                            for (int z = 0; z < S.a_zp[e][k]; ++z) pw *= pw;
-                            if (S.a_etal2[e][k] != etaprev) {
-                                etaprev = S.a_etal2[e][k];
-                                B = __builtin_amdgcn_exp2f(-etaprev * Sd);
-                            }
-                            g[k] += S.a_norm[e][k] * pw * B * Cf;
-                        }
-                    }
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < NN_MAXA; ++k) {
-            const float v = sum_group<NN_TPA>(g[k]);
-            if (i < N && k < na && sub == (k & (NN_TPA - 1))) {
-                Grep[(size_t)i * NN_K1 + nr + k] = v;
-                if (v < S.glo[e][nr + k] || v > S.ghi[e][nr + k]) extrap = 1;
-            }
-        }
+        g_chunk(S, L, elem, N, k0, lane, M.rc2, M.pi_rc, Grep, &extrap);
         __syncthreads();
     }
 
@@ -570,111 +815,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
             cb[ci][k] = (i < N && k < S.nr[elem[i]] + S.na[elem[i]]) ? Crep[(size_t)i * NN_K1 + k] : 0.f;
         }
         __syncthreads();
-        const int i = k0 + c;
-        const int cnt = L.cnt[c];
-        const int e = (i < N) ? elem[i] : 0;
-        const int nr = (i < N) ? S.nr[e] : 0, na = (i < N) ? S.na[e] : 0;
-        const int base = c * NN_CAP;
-        float Fi0 = 0.f, Fi1 = 0.f, Fi2 = 0.f;  // force on the centre (Hartree/Bohr)
-        // radial: dG/dr = (-2 eta (r - rs) fc + dfc) exp(-eta (r - rs)^2)
-        for (int p = sub; p < cnt; p += NN_TPA) {
-            const float r = L.r[base + p], fcv = L.fc[base + p], dfcv = L.dfc[base + p];
-            const int ep = L.el[base + p];
-            float s = 0.f;
-#pragma unroll
-            for (int k = 0; k < NN_MAXR; ++k)
-                if (k < nr && ep == S.r_el[e][k]) {
-                    const float dr = r - S.r_rs[e][k];
-                    const float ex = __builtin_amdgcn_exp2f(-S.r_eta[e][k] * NN_L2E * dr * dr);
-                    s += cb[c][k] * ex * (dfcv - 2.f * S.r_eta[e][k] * dr * fcv);
-                }
-            const float ir = L.ir[base + p];
-            const float v0 = s * L.dx[base + p] * ir, v1 = s * L.dy[base + p] * ir, v2 = s * L.dz[base + p] * ir;
-            Fi0 += v0;
-            Fi1 += v1;
-            Fi2 += v2;
-            const int j = L.j[base + p];
-            fxadd(&ax[j], -(double)v0, &bad);
-            fxadd(&ay[j], -(double)v1, &bad);
-            fxadd(&az[j], -(double)v2, &bad);
-        }
-        // angular
-        for (int p = sub; p < cnt; p += NN_TPA) {
-            const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
-            const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p], dfa = L.dfc[base + p];
-            const int ep = L.el[base + p];
-            float Fj0 = 0.f, Fj1 = 0.f, Fj2 = 0.f;
-            for (int q = p + 1; q < cnt; ++q) {
-                const float bx = L.dx[base + q], by = L.dy[base + q], bz = L.dz[base + q];
-                const float cx = bx - ax_, cy = by - ay_, cz = bz - az_;
-                const float rc2v = cx * cx + cy * cy + cz * cz;
-                if (rc2v > M.rc2) continue;
-                const float rb = L.r[base + q], ib = L.ir[base + q], fb = L.fc[base + q], dfb = L.dfc[base + q];
-                const int eq = L.el[base + q];
-                const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
-                const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * ib;
-                const float rcc = sqrtf(rc2v);
-                float fcc, dfcc;
-                cutoff(M.pi_rc, rcc, &fcc, &dfcc);
-                const float Cf = fa * fb * fcc;
-                const float Sd = ra * ra + rb * rb + rc2v;
-                float Sg = 0.f, Sc = 0.f, Ss = 0.f, B = 0.f, etaprev = -1.f;
-#pragma unroll
-                for (int k = 0; k < NN_MAXA; ++k)
-                    if (k < na && cls == S.a_cls[e][k]) {
-                        const float lam = S.a_lam[e][k];
-                        const float x = 1.0f + lam * cth;
-                        if (x > 0.f) {
-                            float pw = x, pm = 1.f;  // x^zeta, x^(zeta-1)
-                            for (int z = 0; z < S.a_zp[e][k]; ++z) {
-                                pm *= pw;
-                                pw *= pw;
-                            }
-                            if (S.a_etal2[e][k] != etaprev) {
-                                etaprev = S.a_etal2[e][k];
-                                B = __builtin_amdgcn_exp2f(-etaprev * Sd);
-                            }
-                            const float ck = cb[c][nr + k] * S.a_norm[e][k] * B;
-                            const float zeta = (float)(1 << S.a_zp[e][k]);
-                            Sg += ck * pw;
-                            Sc += ck * zeta * lam * pm;
-                            Ss += ck * S.a_eta[e][k] * pw;
-                        }
-                    }
-                if (Sg == 0.f && Sc == 0.f) continue;
-                const float iab = ia * ib;
-                const float tc = Cf * Sc, ts = -2.f * Cf * Ss;
-                const float ua = Sg * dfa * fb * fcc * ia, ub = Sg * fa * dfb * fcc * ib;
-                const float uc = Sg * fa * fb * dfcc / rcc;
-                const float ga0 = tc * (bx * iab - cth * ax_ * ia * ia) + ts * ax_ + ua * ax_;
-                const float ga1 = tc * (by * iab - cth * ay_ * ia * ia) + ts * ay_ + ua * ay_;
-                const float ga2 = tc * (bz * iab - cth * az_ * ia * ia) + ts * az_ + ua * az_;
-                const float gb0 = tc * (ax_ * iab - cth * bx * ib * ib) + ts * bx + ub * bx;
-                const float gb1 = tc * (ay_ * iab - cth * by * ib * ib) + ts * by + ub * by;
-                const float gb2 = tc * (az_ * iab - cth * bz * ib * ib) + ts * bz + ub * bz;
-                const float gc0 = (ts + uc) * cx, gc1 = (ts + uc) * cy, gc2 = (ts + uc) * cz;
-                Fi0 += ga0 + gb0;
-                Fi1 += ga1 + gb1;
-                Fi2 += ga2 + gb2;
-                Fj0 -= ga0 - gc0;
-                Fj1 -= ga1 - gc1;
-                Fj2 -= ga2 - gc2;
-                const int kq = L.j[base + q];
-                fxadd(&ax[kq], -(double)(gb0 + gc0), &bad);
-                fxadd(&ay[kq], -(double)(gb1 + gc1), &bad);
-                fxadd(&az[kq], -(double)(gb2 + gc2), &bad);
-            }
-            const int j = L.j[base + p];
-            fxadd(&ax[j], (double)Fj0, &bad);
-            fxadd(&ay[j], (double)Fj1, &bad);
-            fxadd(&az[j], (double)Fj2, &bad);
-        }
-        const float S0 = sum_group<NN_TPA>(Fi0), S1 = sum_group<NN_TPA>(Fi1), S2 = sum_group<NN_TPA>(Fi2);
-        if (i < N && sub == 0) {
-            fxadd(&ax[i], (double)S0, &bad);
-            fxadd(&ay[i], (double)S1, &bad);
-            fxadd(&az[i], (double)S2, &bad);
-        }
+        force_chunk(S, L, cb, elem, N, k0, lane, M.rc2, M.pi_rc, ax, ay, az, &bad);
         __syncthreads();
     }
 
@@ -713,6 +854,484 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
         if (extrap) bad |= MDT_ST_NN_EXTRAPOLATION;
     }
     if (bad) atomicOr(&bv.status[t], bad);
+}
+
+
+// ===========================================================================
+// Training (no reference counterpart: md_tian2 fits with RuNNer's Kalman
+// filter, fit.f90).  Loss over a minibatch of Nf frames with Ntot atoms:
+//   L = wE/Nf sum_f ((E_f - Eref_f)/N_f)^2 + wF/(3 Ntot) sum |F - Fref|^2
+// in eV and eV/A.  One iteration: k_nn_mlp (network forward/backward on
+// 16-atom tiles), k_nn_frame mode 1 (forces, residuals, the Jacobian-vector
+// product u = dL/d(dE/dx)), k_nn_grad (tangent and second-order passes,
+// weight gradients on MFMA, per-wave partials), k_nn_adam (fixed-order
+// reduction of the partials, Adam on the FP32 master weights in NnConst).
+// ===========================================================================
+
+// k_nn_frame: mode 0 writes the raw symmetry functions of every atom of the
+// frame; mode 1 computes the frame's forces from C, its energy from Eatom,
+// the loss parts, cE = dL/dE and U rows.
+__global__ void __launch_bounds__(NN_BLOCK) k_nn_frame(const NnConst *__restrict__ Mg, NnTrainView tv,
+                                                       const int *frames, int mode, double cEw, double cFw) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ NnSfTab S;
+    __shared__ ChunkNbr L;
+    __shared__ float cb[NN_CA][NN_K1];
+    __shared__ float sdfac[NN_MAXE][NN_K1];
+    __shared__ double red[NN_BLOCK / 64];
+    const NnConst &M = *Mg;
+    const int f = frames[blockIdx.x];
+    const int N = tv.fr_natoms[f];
+    const long long ao = tv.fr_aoff[f];
+    const int lane = threadIdx.x;
+    int bad = 0, extrap = 0;
+    // dynamic LDS: fx fy fz (double N) | ax ay az (u64 N) | gx gy gz (float N) | elem (u8 N)
+    double *fx = (double *)smem_raw, *fy = fx + N, *fz = fy + N;
+    unsigned long long *ax = (unsigned long long *)(fz + N), *ay = ax + N, *az = ay + N;
+    float *gx = (float *)(az + N), *gy = gx + N, *gz = gy + N;
+    uint8_t *elem = (uint8_t *)(gz + N);
+    load_sftab(S, M, lane);
+    for (int k = lane; k < NN_MAXE * NN_K1; k += NN_BLOCK) sdfac[k / NN_K1][k % NN_K1] = M.dfac[k / NN_K1][k % NN_K1];
+    double h[9], ih[9];
+    float w[3];
+    int imax[3];
+    cell_setup(M, tv.fr_cell + 9 * f, tv.fr_icell + 9 * f, h, ih, w, imax);
+    for (int i = lane; i < N; i += NN_BLOCK) {
+        const int e = M.elem_of_type[tv.type[ao + i]];
+        elem[i] = (uint8_t)(e < 0 ? 0 : e);
+        const double *r = tv.pos + 3 * (ao + i);
+        const double f0 = ih[0] * r[0] + ih[3] * r[1] + ih[6] * r[2];
+        const double f1 = ih[1] * r[0] + ih[4] * r[1] + ih[7] * r[2];
+        const double f2 = ih[2] * r[0] + ih[5] * r[1] + ih[8] * r[2];
+        fx[i] = f0 - floor(f0);
+        fy[i] = f1 - floor(f1);
+        fz[i] = f2 - floor(f2);
+        ax[i] = ay[i] = az[i] = 0ull;
+    }
+    __syncthreads();
+    float *Grow = tv.G + (size_t)ao * NN_K1;
+    if (mode == 0) {
+        for (int k0 = 0; k0 < N; k0 += NN_CA) {
+            build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
+            __syncthreads();
+            g_chunk(S, L, elem, N, k0, lane, M.rc2, M.pi_rc, Grow, &extrap);
+            __syncthreads();
+        }
+        return;
+    }
+    // forces with the current network (C rows from k_nn_mlp)
+    const float *Crow = tv.C + (size_t)ao * NN_K1;
+    for (int k0 = 0; k0 < N; k0 += NN_CA) {
+        build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
+        for (int q = lane; q < NN_CA * NN_K1; q += NN_BLOCK) {
+            const int ci = q / NN_K1, k = q % NN_K1;
+            const int i = k0 + ci;
+            cb[ci][k] = (i < N && k < S.nr[elem[i]] + S.na[elem[i]]) ? Crow[(size_t)i * NN_K1 + k] : 0.f;
+        }
+        __syncthreads();
+        force_chunk(S, L, cb, elem, N, k0, lane, M.rc2, M.pi_rc, ax, ay, az, &bad);
+        __syncthreads();
+    }
+    // residuals: g = dL/dF (per Hartree/Bohr), force loss part
+    const double fconv = NN_HA2EV / NN_BOHR2ANG;
+    double lf = 0.0;
+    for (int i = lane; i < N; i += NN_BLOCK) {
+        const double F[3] = {(double)(long long)ax[i] * NN_FISCALE * fconv,
+                             (double)(long long)ay[i] * NN_FISCALE * fconv,
+                             (double)(long long)az[i] * NN_FISCALE * fconv};
+        const double *Fr = tv.Fref + 3 * (ao + i);
+        const double d0 = F[0] - Fr[0], d1 = F[1] - Fr[1], d2 = F[2] - Fr[2];
+        lf += cFw * (d0 * d0 + d1 * d1 + d2 * d2);
+        gx[i] = (float)(2.0 * cFw * d0 * fconv);
+        gy[i] = (float)(2.0 * cFw * d1 * fconv);
+        gz[i] = (float)(2.0 * cFw * d2 * fconv);
+    }
+    for (int o = 32; o > 0; o >>= 1) lf += __shfl_xor(lf, o, 64);
+    if ((lane & 63) == 0) red[lane >> 6] = lf;
+    __syncthreads();
+    if (lane == 0) {
+        double sF = 0.0, E = 0.0;
+        for (int q = 0; q < NN_BLOCK / 64; ++q) sF += red[q];
+        for (int i = 0; i < N; ++i) E += tv.Eatom[ao + i];
+        const double dE = (NN_HA2EV * E - tv.Eref[f]) / N;
+        tv.loss[2 * f] = cEw * dE * dE;
+        tv.loss[2 * f + 1] = sF;
+        tv.cE[f] = 2.0 * cEw * dE / N * NN_HA2EV;
+    }
+    // Jacobian-vector product
+    float *Urow = tv.U + (size_t)ao * NN_K1;
+    for (int k0 = 0; k0 < N; k0 += NN_CA) {
+        build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
+        __syncthreads();
+        jvp_chunk(S, L, elem, N, k0, lane, M.rc2, M.pi_rc, gx, gy, gz, sdfac, Urow);
+        __syncthreads();
+    }
+}
+
+// network forward/backward on the minibatch's tiles (rows: global atom
+// indices; tiles: element, first row, row count)
+__global__ void __launch_bounds__(NN_BLOCK) k_nn_mlp(const NnConst *__restrict__ Mg, NnTrainView tv,
+                                                     const int4 *tiles, int ntiles, const int *rows) {
+    __shared__ float T[NN_BLOCK / 64][16 * 33];
+    __shared__ int rowsb[NN_BLOCK / 64][16];
+    const int lane = threadIdx.x, wave = lane >> 6;
+    const int tile = blockIdx.x * (NN_BLOCK / 64) + wave;
+    int e = -1;
+    if (tile < ntiles) e = tiles[tile].x;
+    if ((lane & 63) < 16) {
+        const int rr = lane & 15;
+        rowsb[wave][rr] = (e >= 0 && rr < tiles[tile].z) ? rows[tiles[tile].y + rr] : -1;
+    }
+    __syncthreads();
+    mlp_tile(*Mg, e, rowsb[wave], tv.G, tv.C, tv.Eatom, T[wave], lane);
+}
+
+namespace {
+// acc += P^T Q over 16 rows: P[16][ldp] columns p0..p0+15, Q[16][ldq]
+// columns q0..q0+15 (K = 32 with rows 16..31 zero)
+__device__ __forceinline__ f32x4 outer16(const float *P, int ldp, int p0, const float *Q, int ldq, int q0, f32x4 acc,
+                                         int l) {
+    float av[8], bv[8];
+    const int m = l & 15, kb = (l >> 4) * 8;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int r = kb + j;
+        av[j] = r < 16 ? P[r * ldp + p0 + m] : 0.f;
+        bv[j] = r < 16 ? Q[r * ldq + q0 + m] : 0.f;
+    }
+    bf16x8 ah, al, bh, bl;
+    split8(av, ah, al);
+    split8(bv, bh, bl);
+    return mfma3(ah, al, bh, bl, acc);
+}
+
+__device__ __forceinline__ float act2nd(int code, float h, float d1) {
+    if (code == 1) return -2.f * h * d1;
+    if (code == 2) return d1 * (1.f - 2.f * h);
+    return 0.f;
+}
+
+struct GradAcc {
+    f32x4 w1[4][2], w2[2][2];
+    float b1[2], b2[2], w3[2], b3;
+};
+
+__device__ void acc_zero(GradAcc &a) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n) a.w1[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n) a.w2[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+    a.b1[0] = a.b1[1] = a.b2[0] = a.b2[1] = a.w3[0] = a.w3[1] = a.b3 = 0.f;
+}
+
+// per-wave partial gradient of element e (padded layout, NN_NPAR floats)
+__device__ void acc_flush(const GradAcc &a, float *out, int l) {
+    const int nB = l & 15, rC = (l >> 4) * 4;
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) out[NN_PW1 + (m * 16 + rC + i) * NN_H + n * 16 + nB] = a.w1[m][n][i];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) out[NN_PW2 + (m * 16 + rC + i) * NN_H + n * 16 + nB] = a.w2[m][n][i];
+    // bias / output sums: reduce the lane partials over the 4 row groups
+    float v[5] = {a.b1[0], a.b1[1], a.b2[0], a.b2[1], 0.f};
+    float w3v[2] = {a.w3[0], a.w3[1]};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        v[q] += __shfl_xor(v[q], 16, 64);
+        v[q] += __shfl_xor(v[q], 32, 64);
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        w3v[q] += __shfl_xor(w3v[q], 16, 64);
+        w3v[q] += __shfl_xor(w3v[q], 32, 64);
+    }
+    float b3 = a.b3;
+    for (int o = 1; o < 64; o <<= 1) b3 += __shfl_xor(b3, o, 64);
+    if (l < 16) {
+        out[NN_PB1 + nB] = v[0];
+        out[NN_PB1 + 16 + nB] = v[1];
+        out[NN_PB2 + nB] = v[2];
+        out[NN_PB2 + 16 + nB] = v[3];
+        out[NN_PW3 + nB] = w3v[0];
+        out[NN_PW3 + 16 + nB] = w3v[1];
+    }
+    if (l == 0) out[NN_PB3] = b3;
+}
+}  // namespace
+
+// Weight gradient of one 16-row tile of element e (e < 0: padding tile; every
+// lane of the block calls this the same number of times).  Stage: wave's
+// LDS area of 2 x 16 x 65 + 4 x 16 x 33 floats.  Linear output layer.
+__device__ void grad_tile(const NnConst &M, int e, const int *rows, const NnTrainView &tv, float *stage, GradAcc &acc,
+                          int lane) {
+    const int l = lane & 63;
+    const int rA = l & 15, kg = (l >> 4) * 8, nB = l & 15, rC = (l >> 4) * 4;
+    const bool act = e >= 0;
+    const int ee = act ? e : 0;
+    const int nin = act ? M.nr[ee] + M.na[ee] : 0;
+    float *Xs = stage, *Us = Xs + 16 * 65, *A1 = Us + 16 * 65, *A2 = A1 + 16 * 33, *A3 = A2 + 16 * 33,
+          *A4 = A3 + 16 * 33;
+    const int atomA = act ? rows[rA] : -1;
+    // cE of this lane's C rows
+    float cEr[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int a = act ? rows[rC + i] : -1;
+        cEr[i] = a >= 0 ? (float)tv.cE[tv.atom_frame[a]] : 0.f;
+    }
+    // ---- layer 1 primal and tangent: z1 = x W1 + b1, zd1 = u W1 ----
+    f32x4 z1[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}}, zd1[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks) {
+        float xv[8], uv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = ks * 32 + kg + j;
+            const bool ok = atomA >= 0 && k < nin;
+            xv[j] = ok ? tv.G[(size_t)atomA * NN_K1 + k] * M.smul[ee][k] + M.sadd[ee][k] : 0.f;
+            uv[j] = ok ? tv.U[(size_t)atomA * NN_K1 + k] : 0.f;
+            Xs[rA * 65 + k] = xv[j];
+            Us[rA * 65 + k] = uv[j];
+        }
+        bf16x8 xh, xl, uh, ul;
+        split8(xv, xh, xl);
+        split8(uv, uh, ul);
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            float bv[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bv[j] = M.W1[ee][ks * 32 + kg + j][nt * 16 + nB];
+            bf16x8 bh, bl;
+            split8(bv, bh, bl);
+            z1[nt] = mfma3(xh, xl, bh, bl, z1[nt]);
+            zd1[nt] = mfma3(uh, ul, bh, bl, zd1[nt]);
+        }
+    }
+    float h1[2][4], d1[2][4], s1[2][4], hd1[2][4];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            h1[nt][i] = act_f(M.act1, z1[nt][i] + M.b1[ee][nt * 16 + nB], &d1[nt][i]);
+            s1[nt][i] = act2nd(M.act1, h1[nt][i], d1[nt][i]);
+            hd1[nt][i] = d1[nt][i] * zd1[nt][i];
+            A1[(rC + i) * 33 + nt * 16 + nB] = h1[nt][i];
+            A2[(rC + i) * 33 + nt * 16 + nB] = hd1[nt][i];
+        }
+    __syncthreads();
+    // ---- layer 2 primal and tangent ----
+    float hv[8], hdv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        hv[j] = A1[rA * 33 + kg + j];
+        hdv[j] = A2[rA * 33 + kg + j];
+    }
+    bf16x8 hh, hl, dh, dl;
+    split8(hv, hh, hl);
+    split8(hdv, dh, dl);
+    f32x4 z2[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}}, zd2[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        float bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bv[j] = M.W2[ee][kg + j][nt * 16 + nB];
+        bf16x8 bh, bl;
+        split8(bv, bh, bl);
+        z2[nt] = mfma3(hh, hl, bh, bl, z2[nt]);
+        zd2[nt] = mfma3(dh, dl, bh, bl, zd2[nt]);
+    }
+    // rho2 = d2 w3, pi2 = a2'' zd2 w3, Y2 = cE rho2 + pi2
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float d2;
+            const float h2 = act_f(M.act2, z2[nt][i] + M.b2[ee][nt * 16 + nB], &d2);
+            const float s2 = act2nd(M.act2, h2, d2);
+            const float w3 = M.w3[ee][nt * 16 + nB];
+            const float rho2 = d2 * w3, pi2 = s2 * zd2[nt][i] * w3;
+            const float y2 = cEr[i] * rho2 + pi2;
+            A3[(rC + i) * 33 + nt * 16 + nB] = y2;
+            A4[(rC + i) * 33 + nt * 16 + nB] = rho2;
+            acc.b2[nt] += y2;
+            acc.w3[nt] += cEr[i] * h2 + d2 * zd2[nt][i];
+        }
+    if (nB == 0)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc.b3 += cEr[i];
+    __syncthreads();
+    // gW2 += H1^T Y2 + Hd1^T rho2
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            acc.w2[m][n] = outer16(A1, 33, m * 16, A3, 33, n * 16, acc.w2[m][n], l);
+            acc.w2[m][n] = outer16(A2, 33, m * 16, A4, 33, n * 16, acc.w2[m][n], l);
+        }
+    // q = rho2 W2^T, p = pi2 W2^T  (pi2 = Y2 - cE rho2 in A layout)
+    float rv[8], pv[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int a = act ? rows[rA] : -1;
+        const float cE = a >= 0 ? (float)tv.cE[tv.atom_frame[a]] : 0.f;
+        rv[j] = A4[rA * 33 + kg + j];
+        pv[j] = A3[rA * 33 + kg + j] - cE * rv[j];
+    }
+    bf16x8 rh, rl, ph, pl;
+    split8(rv, rh, rl);
+    split8(pv, ph, pl);
+    f32x4 q[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}}, pq[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        float bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bv[j] = M.W2[ee][nt * 16 + nB][kg + j];
+        bf16x8 bh, bl;
+        split8(bv, bh, bl);
+        q[nt] = mfma3(rh, rl, bh, bl, q[nt]);
+        pq[nt] = mfma3(ph, pl, bh, bl, pq[nt]);
+    }
+    __syncthreads();
+    // rho1 = d1 q, pi1 = d1 p + a1'' zd1 q, Y1 = cE rho1 + pi1
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float rho1 = d1[nt][i] * q[nt][i];
+            const float pi1 = d1[nt][i] * pq[nt][i] + s1[nt][i] * zd1[nt][i] * q[nt][i];
+            const float y1 = cEr[i] * rho1 + pi1;
+            A1[(rC + i) * 33 + nt * 16 + nB] = y1;
+            A2[(rC + i) * 33 + nt * 16 + nB] = rho1;
+            acc.b1[nt] += y1;
+        }
+    __syncthreads();
+    // gW1 += X^T Y1 + U^T rho1
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            acc.w1[m][n] = outer16(Xs, 65, m * 16, A1, 33, n * 16, acc.w1[m][n], l);
+            acc.w1[m][n] = outer16(Us, 65, m * 16, A2, 33, n * 16, acc.w1[m][n], l);
+        }
+    __syncthreads();
+}
+
+// per-wave partial gradients over the minibatch's tiles: wave gw takes tiles
+// gw, gw + W, ... (element-grouped, so a wave meets each element in one run);
+// partials[gw][e] (NN_NPAR floats each, zeroed by the host)
+__global__ void __launch_bounds__(NN_BLOCK) k_nn_grad(const NnConst *__restrict__ Mg, NnTrainView tv,
+                                                      const int4 *tiles, int ntiles, const int *rows,
+                                                      float *partials) {
+    __shared__ float stage[NN_BLOCK / 64][2 * 16 * 65 + 4 * 16 * 33];
+    __shared__ int rowsb[NN_BLOCK / 64][16];
+    const NnConst &M = *Mg;
+    const int lane = threadIdx.x, wave = lane >> 6, l = lane & 63;
+    const int W = gridDim.x * (NN_BLOCK / 64);
+    const int gw = blockIdx.x * (NN_BLOCK / 64) + wave;
+    const int rounds = (ntiles + W - 1) / W;
+    GradAcc acc;
+    acc_zero(acc);
+    int cur = -1;
+    for (int r = 0; r < rounds; ++r) {
+        const int tile = r * W + gw;
+        const int e = tile < ntiles ? tiles[tile].x : -1;
+        if (e >= 0 && e != cur) {
+            if (cur >= 0) acc_flush(acc, partials + ((size_t)gw * NN_MAXE + cur) * NN_NPAR, l);
+            acc_zero(acc);
+            cur = e;
+        }
+        if (l < 16) rowsb[wave][l] = (e >= 0 && l < tiles[tile].z) ? rows[tiles[tile].y + l] : -1;
+        __syncthreads();
+        grad_tile(M, e, rowsb[wave], tv, stage[wave], acc, lane);
+    }
+    if (cur >= 0) acc_flush(acc, partials + ((size_t)gw * NN_MAXE + cur) * NN_NPAR, l);
+}
+
+// fixed-order reduction of the partials; mode 0 writes the gradient to grad
+// (padded layout, NN_MAXE x NN_NPAR), mode 1 also applies Adam to NnConst
+__global__ void k_nn_adam(NnConst *Mg, const float *partials, int W, float *grad, float *am, float *av, float lr,
+                          float beta1, float beta2, float eps, int step, int mode) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= NN_MAXE * NN_NPAR) return;
+    const int e = idx / NN_NPAR, p = idx % NN_NPAR;
+    if (e >= Mg->nelem) return;
+    double g = 0.0;
+    for (int w = 0; w < W; ++w) g += (double)partials[((size_t)w * NN_MAXE + e) * NN_NPAR + p];
+    const int nin = Mg->nr[e] + Mg->na[e], n1 = Mg->nh1, n2 = Mg->nh2;
+    float *par;
+    bool real;
+    if (p < NN_PB1) {
+        const int k = p / NN_H, n = p % NN_H;
+        par = &Mg->W1[e][k][n];
+        real = k < nin && n < n1;
+    } else if (p < NN_PW2) {
+        par = &Mg->b1[e][p - NN_PB1];
+        real = p - NN_PB1 < n1;
+    } else if (p < NN_PB2) {
+        const int k = (p - NN_PW2) / NN_H, n = (p - NN_PW2) % NN_H;
+        par = &Mg->W2[e][k][n];
+        real = k < n1 && n < n2;
+    } else if (p < NN_PW3) {
+        par = &Mg->b2[e][p - NN_PB2];
+        real = p - NN_PB2 < n2;
+    } else if (p < NN_PB3) {
+        par = &Mg->w3[e][p - NN_PW3];
+        real = p - NN_PW3 < n2;
+    } else {
+        par = &Mg->b3[e];
+        real = true;
+    }
+    if (!real) g = 0.0;
+    grad[idx] = (float)g;
+    if (mode != 1 || !real) return;
+    const float gf = (float)g;
+    const float m = beta1 * am[idx] + (1.f - beta1) * gf;
+    const float v = beta2 * av[idx] + (1.f - beta2) * gf * gf;
+    am[idx] = m;
+    av[idx] = v;
+    const float mh = m / (1.f - powf(beta1, (float)step));
+    const float vh = v / (1.f - powf(beta2, (float)step));
+    *par -= lr * mh / (sqrtf(vh) + eps);
+}
+
+size_t mdt_nn_frame_smem(int N) { return (size_t)N * (3 * 8 + 3 * 8 + 3 * 4 + 1) + 16; }
+
+hipError_t mdt_launch_nn_frame(int nblocks, size_t smem, const NnConst *M, const NnTrainView &tv, const int *frames,
+                               int mode, double cEw, double cFw, hipStream_t st) {
+    hipLaunchKernelGGL(k_nn_frame, dim3(nblocks), dim3(NN_BLOCK), smem, st, M, tv, frames, mode, cEw, cFw);
+    return hipGetLastError();
+}
+
+hipError_t mdt_launch_nn_mlp(int ntiles, const NnConst *M, const NnTrainView &tv, const int4 *tiles,
+                             const int *rows, hipStream_t st) {
+    const int nb = (ntiles + NN_BLOCK / 64 - 1) / (NN_BLOCK / 64);
+    hipLaunchKernelGGL(k_nn_mlp, dim3(nb > 0 ? nb : 1), dim3(NN_BLOCK), 0, st, M, tv, tiles, ntiles, rows);
+    return hipGetLastError();
+}
+
+hipError_t mdt_launch_nn_grad(int nblocks, const NnConst *M, const NnTrainView &tv, const int4 *tiles, int ntiles,
+                              const int *rows, float *partials, hipStream_t st) {
+    hipLaunchKernelGGL(k_nn_grad, dim3(nblocks), dim3(NN_BLOCK), 0, st, M, tv, tiles, ntiles, rows, partials);
+    return hipGetLastError();
+}
+
+hipError_t mdt_launch_nn_adam(NnConst *M, const float *partials, int W, float *grad, float *am, float *av, float lr,
+                              float beta1, float beta2, float eps, int step, int mode, hipStream_t st) {
+    const int n = NN_MAXE * NN_NPAR;
+    hipLaunchKernelGGL(k_nn_adam, dim3((n + 255) / 256), dim3(256), 0, st, M, partials, W, grad, am, av, lr, beta1,
+                       beta2, eps, step, mode);
+    return hipGetLastError();
 }
 
 size_t mdt_nene_smem(int N) { return (size_t)N * (4 * 8 + 3 * 8 + 2 + 1) + 16; }

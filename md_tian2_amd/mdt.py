@@ -30,7 +30,8 @@ SYMBOLS = (
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
     "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph",
     "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats",
-    "mdt_batch_nn_descriptors",
+    "mdt_batch_nn_descriptors", "mdt_nn_train_create", "mdt_nn_train_destroy", "mdt_nn_train_descriptor_stats",
+    "mdt_nn_train_epoch", "mdt_nn_train_eval", "mdt_nn_train_loss_grad", "mdt_nn_get_weights",
 )
 
 
@@ -76,6 +77,14 @@ def lib():
         L.mdt_batch_set_pes.argtypes = [vp, C.c_int, C.c_int]
         L.mdt_batch_nn_stats.argtypes = [vp, dp, dp, dp, ip]
         L.mdt_batch_nn_descriptors.argtypes = [vp, C.POINTER(C.c_float)]
+        L.mdt_nn_train_create.argtypes = [vp, C.c_int, ip, ip, dp, dp, dp, dp, C.c_int, C.c_double, C.c_uint32,
+                                          C.POINTER(vp)]
+        L.mdt_nn_train_destroy.argtypes = [vp]
+        L.mdt_nn_train_descriptor_stats.argtypes = [vp, dp, dp, dp]
+        L.mdt_nn_train_epoch.argtypes = [vp, C.c_double, C.c_double, C.c_double, dp]
+        L.mdt_nn_train_eval.argtypes = [vp, C.c_int, dp, dp]
+        L.mdt_nn_train_loss_grad.argtypes = [vp, C.c_int, ip, C.c_double, C.c_double, dp, dp]
+        L.mdt_nn_get_weights.argtypes = [vp, dp]
         _LIB = L
     return _LIB
 
@@ -311,3 +320,68 @@ class Batch:
             op += P
             opn += N * P
         return out
+
+
+class Trainer:
+    """GPU training set and optimiser for the surrogate held by a Context.
+
+    frames: list of geometry.System (one bead) with reference energies
+    E [eV] and forces F [eV/A] (N, 3)."""
+
+    def __init__(self, ctx, frames, E, F, batch_frames=64, val_fraction=0.05, seed=1):
+        self.ctx = ctx
+        nat = np.ascontiguousarray([s.natoms for s in frames], dtype=np.int32)
+        idx = np.ascontiguousarray(np.concatenate([s.idx + 1 for s in frames]).astype(np.int32))
+        sb = np.ascontiguousarray(np.concatenate([np.asarray(s.simbox).T.reshape(-1) for s in frames]))
+        r = np.ascontiguousarray(np.concatenate([s.r[:, 0, :].reshape(-1) for s in frames]))
+        e = np.ascontiguousarray(E, dtype=np.float64)
+        f = np.ascontiguousarray(np.concatenate([np.asarray(x).reshape(-1) for x in F]), dtype=np.float64)
+        self.h = C.c_void_p()
+        _check(lib().mdt_nn_train_create(ctx.h, len(frames), _ip(nat), _ip(idx), _dp(sb), _dp(r), _dp(e), _dp(f),
+                                         batch_frames, val_fraction, seed, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().mdt_nn_train_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def descriptor_stats(self, nelem):
+        mn, mx, av = (np.zeros(nelem * 64) for _ in range(3))
+        _check(lib().mdt_nn_train_descriptor_stats(self.h, _dp(mn), _dp(mx), _dp(av)))
+        return mn.reshape(nelem, 64), mx.reshape(nelem, 64), av.reshape(nelem, 64)
+
+    def epoch(self, lr=1e-3, wE=1.0, wF=1.0):
+        loss = C.c_double(0)
+        _check(lib().mdt_nn_train_epoch(self.h, lr, wE, wF, C.byref(loss)))
+        return loss.value
+
+    def eval(self, validation=False):
+        """(energy RMSE [eV/atom], force RMSE [eV/A])."""
+        e, f = C.c_double(0), C.c_double(0)
+        _check(lib().mdt_nn_train_eval(self.h, 1 if validation else 0, C.byref(e), C.byref(f)))
+        return e.value, f.value
+
+    def loss_grad(self, frames, nweights, wE=1.0, wF=1.0):
+        fr = np.ascontiguousarray(frames, dtype=np.int32)
+        g = np.zeros(nweights)
+        L = C.c_double(0)
+        _check(lib().mdt_nn_train_loss_grad(self.h, len(fr), _ip(fr), wE, wF, C.byref(L), _dp(g)))
+        return L.value, g
+
+
+def nn_weights(ctx, model):
+    """Current network weights split per element (weights.XXX.data layout)."""
+    n = sum(model.nweights(z) for z in model.elements)
+    w = np.zeros(n)
+    _check(lib().mdt_nn_get_weights(ctx.h, _dp(w)))
+    out, o = {}, 0
+    for z in model.elements:
+        out[z] = w[o:o + model.nweights(z)].copy()
+        o += model.nweights(z)
+    return out
