@@ -137,6 +137,74 @@ def copy_calibration():
     print(json.dumps({"calibration": "copy", "bytes_read": n * 8, "bytes_written": n * 8}), flush=True)
 
 
+def nn_variant(ctx, pes, args):
+    """The NENE surrogate on its workload: 6x6x4 Au + H trajectories, slab
+    Langevin at 300 K, H Verlet, dt 0.1 fs; EMT fallback on extrapolation
+    and the EMT judge on a rotating 1 % of the replicas every 10 steps."""
+    from md_tian2_amd import mdt, surrogate
+
+    model = surrogate.load_model()
+    if model is None:
+        return {"value": None, "reason": "no trained model in md_tian2_amd/nn/hau (python -m md_tian2_amd.surrogate train)"}
+    systems = make_workload(args.nn_traj, seed=4242)
+    ctx.set_nn(model, systems[0].type_names)
+    b = mdt.Batch(ctx, systems, dt=0.1, tsurf=300.0, tproj=0.0, seed=777)
+    b.set_pes(1, judge_interval=10)
+    b.compute_forces()
+    if args.warmup:
+        b.step(min(args.warmup, 10))
+    b.step(args.nn_steps)
+    ms, _ = b.timing()
+    rmse, ncomp, mx, nfb = b.nn_stats()
+    st = b.status()
+    b.close()
+    print(f"[bench] nn: {args.nn_steps} steps of {len(systems)} trajectories, events {ms:.3f} ms, "
+          f"judge RMSE {rmse} eV/A over {ncomp:.0f} components", file=sys.stderr, flush=True)
+    return {
+        "value": len(systems) * args.nn_steps / (ms / 1e3), "unit": "trajectory-steps/s",
+        "ms_per_step": ms / args.nn_steps, "steps": args.nn_steps, "trajectories": len(systems),
+        "workload": "NENE surrogate H/Au(111) 6x6x4 + H (24 G2 + 16 G4 per element, 6.5 A, 40-32-32-1 tanh), "
+                    "slab Langevin 300 K, H Verlet, dt 0.1 fs, EMT fallback on extrapolation, EMT judge on a "
+                    "rotating 1 % of replicas every 10 steps",
+        "judge_force_rmse_eV_per_A": rmse, "judge_components": ncomp, "judge_max_abs_eV_per_A": mx,
+        "fallback_replicas_last_step": nfb, "trajectories_flagged": int((st != 0).sum()),
+        "precision": "FP32 symmetry functions, BF16 hi/lo-split MFMA network (FP32 accumulate), FP64 integration",
+    }
+
+
+def rpmd_variant(ctx, args):
+    """RPMD H/Pt(111): 8x8x6 Pt (384, bottom 2 layers fixed, Langevin+LDFA at
+    100 K), H under PILE (tau 200 fs, 100 K), 32 beads on every atom (the
+    reference's force_beads replication of the 1-bead slab), dt 0.1 fs;
+    FP64 and FP32-pair EMT, FP64 normal-mode transform."""
+    from md_tian2_amd import geometry, mdt, pesfile
+
+    pes = pesfile.load_pes(os.path.join(ROOT, "md_tian2_amd", "pes", "EMT-HPt.pes"))
+    g = np.random.default_rng(31337)
+    systems = [geometry.h_on_slab("Pt", n=8, nlayers=6, nfixed=2, rng=g, nbeads=32, einc=g.uniform(0.5, 5.0),
+                                  polar_deg=g.uniform(0, 60), azimuth_deg=g.uniform(0, 360), z0=6.0, tsurf=100.0,
+                                  proj_algo=geometry.PROP_PILE) for _ in range(args.rpmd_traj)]
+    cols, pesm = pesfile.emt_tables(pes, systems[0].type_names)
+    ctx.set_emt(cols, pesm)
+    out = {"trajectories": len(systems), "beads": 32, "natoms": systems[0].natoms, "steps": args.rpmd_steps,
+           "workload": "RPMD H/Pt(111) 8x8x6 (384 Pt) + H, 32 beads on every atom (force_beads), H PILE tau 200 fs, "
+                       "slab Langevin+LDFA, 100 K, dt 0.1 fs"}
+    for prec, key in ((0, "fp64"), (1, "fp32_pair")):
+        b = mdt.Batch(ctx, systems, dt=0.1, tsurf=100.0, tproj=100.0, pile_tau=200.0, seed=99)
+        b.set_precision(prec)
+        b.tune(tpa=0)
+        b.compute_forces()
+        b.step(2)
+        b.step(args.rpmd_steps)
+        ms, _ = b.timing()
+        assert not b.status().any(), "trajectory failure flags set"
+        b.close()
+        out[key] = {"value": len(systems) * args.rpmd_steps / (ms / 1e3), "unit": "trajectory-steps/s",
+                    "ms_per_step": ms / args.rpmd_steps}
+        print(f"[bench] rpmd {key}: {ms / args.rpmd_steps:.3f} ms/step", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-calibration", action="store_true",
@@ -147,6 +215,12 @@ def main():
     ap.add_argument("--traj", type=int, default=4096)
     ap.add_argument("--tpa", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--nn-traj", type=int, default=8192)
+    ap.add_argument("--nn-steps", type=int, default=200)
+    ap.add_argument("--no-nn", action="store_true")
+    ap.add_argument("--rpmd-traj", type=int, default=1024)
+    ap.add_argument("--rpmd-steps", type=int, default=20)
+    ap.add_argument("--no-rpmd", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -209,6 +283,8 @@ def main():
     achieved = flop_traj * len(systems) / launch_s / 1e12
     value = len(systems) * args.steps / (ms_total / 1e3)
     f32_launch_s = f32_prof["force_ms"] / 1e3 / max(1, f32_prof["nlaunch"])
+    nn = None if args.no_nn else nn_variant(ctx, pes, args)
+    rpmd = None if args.no_rpmd else rpmd_variant(ctx, args)
 
     out = {
         "metric": METRIC,
@@ -249,8 +325,12 @@ def main():
                 "achieved_tflops": flop_traj * len(systems) / f32_launch_s / 1e12,
                 "error_bound": "<= 1e-6 eV/atom, <= 1e-5 eV/A vs the FP64 oracle (tests/test_gpu_precision.py)",
             },
+            "nn_surrogate": nn,
+            "rpmd_h_pt": rpmd,
         },
     }
+    if nn and nn.get("value"):
+        out["variants"]["nn_surrogate"]["vs_fp64_emt_per_traj_step"] = nn["value"] / value
     if not args.no_cpu:
         v, procs, per, single = cpu_baseline(args.cpu_seconds)
         out["cpu_baseline"] = {

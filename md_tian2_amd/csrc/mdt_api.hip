@@ -85,8 +85,24 @@ struct mdt_context {
     bool have_nn = false;
 };
 
+// Replicas whose atom count lies in (previous bound, bound] of 64, 128, ...,
+// 2048 form a size class with its own launch shape (threads per atom, block,
+// LDS, neighbour buffer), so that small slabs in a mixed batch do not run
+// with the largest slab's shape.
+struct SizeClass {
+    int nmax = 0;              // largest replica of the class
+    int nrep = 0;
+    int *d_reps = nullptr;     // replica ids; NULL when the class holds every replica in order
+    int tpa = 0, nbcap = 0, block = 0;
+    size_t smem = 0;
+    uint16_t *d_nbr = nullptr; // neighbour lists in HBM when they do not fit in LDS
+    long long nbr_stride = 0;
+    float tuned_ms = 0.f;
+};
+
 struct mdt_batch {
     mdt_context *ctx = nullptr;
+    std::vector<SizeClass> cls;
     int ntraj = 0, nrep = 0;
     std::vector<int> N, P;
     std::vector<long long> soff, aoff, eoff;
@@ -102,8 +118,6 @@ struct mdt_batch {
     double *d_x = nullptr, *d_v = nullptr, *d_a = nullptr, *d_f = nullptr, *d_dens = nullptr, *d_epot = nullptr;
     double *d_cjk = nullptr;
     int *d_status = nullptr;
-    uint16_t *d_nbr = nullptr;  // global neighbour lists when they do not fit in LDS
-    long long nbr_stride = 0;
     size_t lds_limit = 0;
     PropConst pc{};
     PropConst *d_pc = nullptr;  // device copy read by the kernels
@@ -163,8 +177,8 @@ struct mdt_batch {
         v.epot = d_epot;
         v.traj_eoff = d_eoff;
         v.status = d_status;
-        v.nbr_global = d_nbr;
-        v.nbr_stride = nbr_stride;
+        v.nbr_global = nullptr;  // per size class (launch_replicas)
+        v.nbr_stride = 0;
         return v;
     }
 };
@@ -345,7 +359,7 @@ int mdt_batch_destroy(mdt_batch *b) {
     void *ptrs[] = {b->d_rep_traj, b->d_rep_bead, b->d_N, b->d_P, b->d_soff, b->d_aoff, b->d_eoff,
                     b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
-                    b->d_cjk, b->d_status, b->d_nbr, b->d_pc, b->d_step, b->d_nnG, b->d_nnC,
+                    b->d_cjk, b->d_status, b->d_pc, b->d_step, b->d_nnG, b->d_nnC,
                     b->d_nnflag, b->d_nnF, b->d_judge};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -353,6 +367,10 @@ int mdt_batch_destroy(mdt_batch *b) {
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     for (hipEvent_t e : b->evf) (void)hipEventDestroy(e);
+    for (SizeClass &c : b->cls) {
+        if (c.d_reps) (void)hipFree(c.d_reps);
+        if (c.d_nbr) (void)hipFree(c.d_nbr);
+    }
     delete b;
     return 0;
 }
@@ -418,6 +436,36 @@ int mdt_batch_create(mdt_context *ctx, int ntraj, const int32_t *natoms, const i
         cjk.insert(cjk.end(), c.begin(), c.end());
     }
     b->nrep = (int)rep_t.size();
+    {
+        const int bounds[6] = {64, 128, 256, 512, 1024, 2048};
+        std::vector<std::vector<int>> mem(6);
+        std::vector<int> nmax(6, 0);
+        for (int r = 0; r < b->nrep; ++r) {
+            const int n = b->N[rep_t[r]];
+            int k = 0;
+            while (n > bounds[k]) ++k;
+            mem[k].push_back(r);
+            nmax[k] = std::max(nmax[k], n);
+        }
+        int nonempty = 0;
+        for (int k = 0; k < 6; ++k) nonempty += !mem[k].empty();
+        for (int k = 0; k < 6; ++k) {
+            if (mem[k].empty()) continue;
+            SizeClass c;
+            c.nmax = nmax[k];
+            c.nrep = (int)mem[k].size();
+            if (nonempty > 1) {
+                if (dalloc(&c.d_reps, mem[k].size()) ||
+                    hipMemcpy(c.d_reps, mem[k].data(), sizeof(int) * mem[k].size(), hipMemcpyHostToDevice) !=
+                        hipSuccess) {
+                    b->cls.push_back(c);
+                    mdt_batch_destroy(b);
+                    return fail(MDT_EHIP, "size-class replica list");
+                }
+            }
+            b->cls.push_back(c);
+        }
+    }
     int rc = 0;
     rc |= dalloc(&b->d_rep_traj, b->nrep);
     rc |= dalloc(&b->d_rep_bead, b->nrep);
@@ -651,39 +699,85 @@ static void drop_graph(mdt_batch *b) {
     b->gexec = nullptr;
 }
 
-static int configure(mdt_batch *b, int tpa, int nbcap) {
-    drop_graph(b);  // launch shapes and the neighbour buffer may change
+static int heuristic_tpa(int maxN) {
+    int tpa = 8;
+    while (tpa > 1 && maxN * tpa > 512) tpa >>= 1;
+    return tpa;
+}
+
+static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap) {
+    // capacity of the owned half lists: every other atom for small replicas,
+    // else 96 (fcc within 1.5 rcut holds 134 neighbours, of which an atom
+    // owns about half; overflow is detected and reported)
+    int cap = nbcap > 0 ? nbcap : 96;
+    if (cap > c.nmax - 1) cap = std::max(1, c.nmax - 1);
+    while (tpa > 1 && c.nmax * tpa > 512) tpa >>= 1;
+    const size_t smem_lds = mdt_replica_smem(c.nmax, cap, true);
+    const bool in_lds = smem_lds <= b->lds_limit;
+    const size_t smem = in_lds ? smem_lds : mdt_replica_smem(c.nmax, cap, false);
+    if (smem > b->lds_limit)
+        return fail(MDT_EUNSUP, "replica of %d atoms needs %zu bytes of LDS (limit %zu)", c.nmax, smem,
+                    b->lds_limit);
+    if (c.d_nbr) {
+        HIPCHK(hipFree(c.d_nbr));
+        c.d_nbr = nullptr;
+        c.nbr_stride = 0;
+    }
+    if (!in_lds) {
+        c.nbr_stride = (long long)c.nmax * cap;
+        int rc = dalloc(&c.d_nbr, (size_t)c.nbr_stride * c.nrep);
+        if (rc) return rc;
+    }
+    const int rounded = ((c.nmax * tpa + 63) / 64) * 64;
+    c.block = std::min(512, std::max(64, rounded));
+    c.tpa = tpa;
+    c.nbcap = cap;
+    c.smem = smem;
+    return 0;
+}
+
+static int lds_limit(mdt_batch *b) {
     if (b->lds_limit == 0) {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, b->ctx->device));
         b->lds_limit = (size_t)prop.sharedMemPerBlock;
     }
-    // capacity of the owned half lists: every other atom for small replicas,
-    // else 96 (fcc within 1.5 rcut holds 134 neighbours, of which an atom
-    // owns about half; overflow is detected and reported)
-    int cap = nbcap > 0 ? nbcap : 96;
-    if (cap > b->maxN - 1) cap = std::max(1, b->maxN - 1);
-    const size_t smem_lds = mdt_replica_smem(b->maxN, cap, true);
-    const bool in_lds = smem_lds <= b->lds_limit;
-    const size_t smem = in_lds ? smem_lds : mdt_replica_smem(b->maxN, cap, false);
-    if (smem > b->lds_limit)
-        return fail(MDT_EUNSUP, "replica of %d atoms needs %zu bytes of LDS (limit %zu)", b->maxN, smem,
-                    b->lds_limit);
-    if (b->d_nbr) {
-        HIPCHK(hipFree(b->d_nbr));
-        b->d_nbr = nullptr;
-        b->nbr_stride = 0;
+    return 0;
+}
+
+// the reported choice: the threads per atom of the class with most replicas
+static void note_tpa(mdt_batch *b) {
+    int best = 0;
+    for (size_t k = 1; k < b->cls.size(); ++k)
+        if (b->cls[k].nrep > b->cls[best].nrep) best = (int)k;
+    b->tpa = b->cls[best].tpa;
+    b->block = b->cls[best].block;
+    b->smem = b->cls[best].smem;
+    b->nbcap = b->cls[best].nbcap;
+}
+
+// tpa > 0: that many threads per atom in every class (halved where a class's
+// replicas would exceed 512 threads); tpa == 0: per-class heuristic
+static int configure(mdt_batch *b, int tpa, int nbcap) {
+    drop_graph(b);  // launch shapes and the neighbour buffers may change
+    int rc = lds_limit(b);
+    if (rc) return rc;
+    for (SizeClass &c : b->cls)
+        if ((rc = configure_class(b, c, tpa > 0 ? tpa : heuristic_tpa(c.nmax), nbcap))) return rc;
+    note_tpa(b);
+    return 0;
+}
+
+// calc_force on every replica, one launch per size class
+static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hipStream_t st) {
+    mdt_context *c = b->ctx;
+    for (const SizeClass &k : b->cls) {
+        BatchView v = base;
+        v.rep_list = k.d_reps;
+        v.nbr_global = k.d_nbr;
+        v.nbr_stride = k.nbr_stride;
+        HIPCHK(mdt_launch_replica(k.tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, add_fin, st));
     }
-    if (!in_lds) {
-        b->nbr_stride = (long long)b->maxN * cap;
-        int rc = dalloc(&b->d_nbr, (size_t)b->nbr_stride * b->nrep);
-        if (rc) return rc;
-    }
-    const int rounded = ((b->maxN * tpa + 63) / 64) * 64;
-    b->block = std::min(512, std::max(64, rounded));
-    b->tpa = tpa;
-    b->nbcap = cap;
-    b->smem = smem;
     return 0;
 }
 
@@ -719,17 +813,15 @@ static int launch_pes(mdt_batch *b, const BatchView &bv, int add_fin, hipStream_
         if (c->have_params) {
             BatchView fb = bv;
             fb.only = b->d_nnflag;
-            HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, fb, add_fin, st));
+            return launch_replicas(b, fb, add_fin, st);
         }
         return 0;
     }
-    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, add_fin, st));
-    return 0;
+    return launch_replicas(b, bv, add_fin, st);
 }
 
 // EMT as the judge of the network on a rotating 1 % of the replicas
 static int launch_judge(mdt_batch *b, hipStream_t st) {
-    mdt_context *c = b->ctx;
     BatchView jb = b->view();
     jb.judge_f = b->d_nnF;
     jb.skip = b->d_nnflag;  // replicas on the fallback have no network forces
@@ -737,8 +829,7 @@ static int launch_judge(mdt_batch *b, hipStream_t st) {
     jb.judge_mod = 100;
     jb.judge_phase = (int)(b->judge_events % 100u);
     ++b->judge_events;
-    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, jb, 0, st));
-    return 0;
+    return launch_replicas(b, jb, 0, st);
 }
 
 static int launch_force(mdt_batch *b) {
@@ -765,44 +856,50 @@ static int check_status(mdt_batch *b) {
     return 0;
 }
 
-static int heuristic_tpa(int maxN) {
-    int tpa = 8;
-    while (tpa > 1 && maxN * tpa > 512) tpa >>= 1;
-    return tpa;
-}
-
-// Times the force kernel for every threads-per-atom choice that fits a
-// 512-thread block (3 launches each after one warm-up) and keeps the fastest.
-// f, a, epot and dens are saved and restored around the trial launches.
+// Times the force kernel of every size class for every threads-per-atom
+// choice that fits a 512-thread block (3 launches each after one warm-up)
+// and keeps each class's fastest.  f, a, epot and dens are saved and
+// restored around the trial launches.
 static int autotune(mdt_batch *b) {
     mdt_context *c = b->ctx;
     hipStream_t st = c->stream;
+    int rc = configure(b, 0, b->nbcap_req);
+    if (rc) return rc;
     double *sf = nullptr, *sa = nullptr, *se = nullptr, *sd = nullptr;
-    int rc = dalloc(&sf, b->tot3pn) | dalloc(&sa, b->tot3pn) | dalloc(&se, b->totp) | dalloc(&sd, b->totpn);
+    rc = dalloc(&sf, b->tot3pn) | dalloc(&sa, b->tot3pn) | dalloc(&se, b->totp) | dalloc(&sd, b->totpn);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(sf, b->d_f, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(sa, b->d_a, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(se, b->d_epot, sizeof(double) * b->totp, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(sd, b->d_dens, sizeof(double) * b->totpn, hipMemcpyDeviceToDevice, st));
-    int best = heuristic_tpa(b->maxN);
-    float best_ms = 1e30f;
     const int cands[4] = {1, 2, 4, 8};
-    for (int tpa : cands) {
-        if (b->maxN * tpa > 512 && tpa != 1) continue;
-        if ((rc = configure(b, tpa, b->nbcap_req))) return rc;
-        const BatchView bv = b->view();
-        HIPCHK(mdt_launch_replica(tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, 0, st));
-        HIPCHK(hipEventRecord(b->ev0, st));
-        for (int k = 0; k < 3; ++k)
-            HIPCHK(mdt_launch_replica(tpa, b->nrep, b->block, b->smem, c->d_ec, b->d_pc, bv, 0, st));
-        HIPCHK(hipEventRecord(b->ev1, st));
-        HIPCHK(hipEventSynchronize(b->ev1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-        if (ms < best_ms) {
-            best_ms = ms;
-            best = tpa;
+    float total = 0.f;
+    for (SizeClass &k : b->cls) {
+        int best = k.tpa;
+        float best_ms = 1e30f;
+        for (int tpa : cands) {
+            if (k.nmax * tpa > 512 && tpa != 1) continue;
+            if ((rc = configure_class(b, k, tpa, b->nbcap_req))) return rc;
+            BatchView v = b->view();
+            v.rep_list = k.d_reps;
+            v.nbr_global = k.d_nbr;
+            v.nbr_stride = k.nbr_stride;
+            HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
+            HIPCHK(hipEventRecord(b->ev0, st));
+            for (int q = 0; q < 3; ++q)
+                HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
+            HIPCHK(hipEventRecord(b->ev1, st));
+            HIPCHK(hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = tpa;
+            }
         }
+        if ((rc = configure_class(b, k, best, b->nbcap_req))) return rc;
+        k.tuned_ms = best_ms / 3;
+        total += k.tuned_ms;
     }
     HIPCHK(hipMemcpyAsync(b->d_f, sf, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(b->d_a, sa, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
@@ -814,13 +911,15 @@ static int autotune(mdt_batch *b) {
     (void)hipFree(sa);
     (void)hipFree(se);
     (void)hipFree(sd);
-    b->tuned_ms = best_ms / 3;
-    return configure(b, best, b->nbcap_req);
+    b->tuned_ms = total;
+    drop_graph(b);
+    note_tpa(b);
+    return 0;
 }
 
 static int autotune_if_needed(mdt_batch *b) {
     if (b->tpa) return 0;
-    return configure(b, heuristic_tpa(b->maxN), b->nbcap_req);
+    return configure(b, 0, b->nbcap_req);
 }
 
 int mdt_batch_compute_forces(mdt_batch *b) {
@@ -1031,7 +1130,7 @@ int mdt_compute_emt(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx
     if ((rc = ensure_ready(b, false))) return rc;
     if ((rc = autotune_if_needed(b))) return rc;
     HIPCHK(hipMemsetAsync(b->d_f, 0, sizeof(double) * b->tot3pn, ctx->stream));
-    HIPCHK(mdt_launch_replica(b->tpa, b->nrep, b->block, b->smem, ctx->d_ec, b->d_pc, b->view(), 0, ctx->stream));
+    if ((rc = launch_replicas(b, b->view(), 0, ctx->stream))) return rc;
     HIPCHK(hipStreamSynchronize(ctx->stream));
     if ((rc = check_status(b))) return rc;
     const long long n3 = 3LL * natoms * nbeads;

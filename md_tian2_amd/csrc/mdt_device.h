@@ -88,6 +88,7 @@ struct BatchView {
     int *status;            // per traj
     uint16_t *nbr_global;   // NULL: neighbour lists in LDS; else nrep * nbr_stride entries
     long long nbr_stride;   // entries per replica slice (maxN * cap)
+    const int *rep_list;    // non-NULL: block k works on replica rep_list[k] (size-class launches)
     const int *only;        // non-NULL: calc_force only on replicas with only[rep] != 0
     const int *skip;        // non-NULL: no calc_force on replicas with skip[rep] != 0
     // judge mode (judge_acc non-NULL): replicas with rep % judge_mod ==

@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(512, 4)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ LdsTables T;
     __shared__ double red[4];
-    const int rep = blockIdx.x;
+    const int rep = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;
     if (bv.only && !bv.only[rep]) return;                                          // whole block: uniform
     if (bv.skip && bv.skip[rep]) return;                                           // whole block: uniform
     if (bv.judge_acc && (rep % bv.judge_mod) != bv.judge_phase) return;           // whole block: uniform
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(512, 4)
     unsigned long long *fxa = shi, *fya = smid, *fza = slo;
     double *G = sz + N, *E = G + N;
     uint16_t *cnt = (uint16_t *)(E + N);
-    uint16_t *nbr = NBR_LDS ? cnt + N : bv.nbr_global + (size_t)rep * bv.nbr_stride;
+    uint16_t *nbr = NBR_LDS ? cnt + N : bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
     uint8_t *typ = (uint8_t *)(cnt + N + (NBR_LDS ? (size_t)N * cap : 0));
     int bad_local = 0;
     __syncthreads();

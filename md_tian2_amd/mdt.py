@@ -329,13 +329,29 @@ class Trainer:
     E [eV] and forces F [eV/A] (N, 3)."""
 
     def __init__(self, ctx, frames, E, F, batch_frames=64, val_fraction=0.05, seed=1):
+        nat = np.array([s.natoms for s in frames], dtype=np.int32)
+        idx = np.concatenate([s.idx + 1 for s in frames])
+        sb = np.concatenate([np.asarray(s.simbox).T.reshape(-1) for s in frames])
+        r = np.concatenate([s.r[:, 0, :].reshape(-1) for s in frames])
+        f = np.concatenate([np.asarray(x).reshape(-1) for x in F])
+        self._create(ctx, nat, idx, sb, r, E, f, batch_frames, val_fraction, seed)
+
+    @classmethod
+    def from_arrays(cls, ctx, natoms, idx, simbox_colmajor, r, E, F, batch_frames=64, val_fraction=0.05, seed=1):
+        """natoms (nf,), idx (sum N, 0-based types), simbox (nf, 9) column-major [A],
+        r and F (sum N, 3) [A, eV/A], E (nf,) [eV]."""
+        self = cls.__new__(cls)
+        self._create(ctx, natoms, np.asarray(idx) + 1, simbox_colmajor, r, E, F, batch_frames, val_fraction, seed)
+        return self
+
+    def _create(self, ctx, nat, idx, sb, r, E, f, batch_frames, val_fraction, seed):
         self.ctx = ctx
-        nat = np.ascontiguousarray([s.natoms for s in frames], dtype=np.int32)
-        idx = np.ascontiguousarray(np.concatenate([s.idx + 1 for s in frames]).astype(np.int32))
-        sb = np.ascontiguousarray(np.concatenate([np.asarray(s.simbox).T.reshape(-1) for s in frames]))
-        r = np.ascontiguousarray(np.concatenate([s.r[:, 0, :].reshape(-1) for s in frames]))
+        nat = np.ascontiguousarray(nat, dtype=np.int32)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        sb = np.ascontiguousarray(sb, dtype=np.float64).reshape(-1)
+        r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1)
         e = np.ascontiguousarray(E, dtype=np.float64)
-        f = np.ascontiguousarray(np.concatenate([np.asarray(x).reshape(-1) for x in F]), dtype=np.float64)
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
         self.h = C.c_void_p()
         _check(lib().mdt_nn_train_create(ctx.h, len(frames), _ip(nat), _ip(idx), _dp(sb), _dp(r), _dp(e), _dp(f),
                                          batch_frames, val_fraction, seed, C.byref(self.h)))
