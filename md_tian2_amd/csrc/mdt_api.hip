@@ -153,7 +153,7 @@ struct mdt_batch {
         BatchView v{};
         v.ntraj = ntraj;
         v.nrep = nrep;
-        v.nbcap = nbcap;
+        v.nbcap = 0;  // per size class (launch_replicas)
         v.precision = precision;
         v.rep_traj = d_rep_traj;
         v.rep_bead = d_rep_bead;
@@ -576,6 +576,11 @@ int mdt_batch_set_propagation(mdt_batch *b, int ntypes, const int32_t *algo, con
     pc.dt = dt;
     pc.pile_tau = pile_tau;
     pc.andersen_time = andersen_time;
+    // the free ring-polymer step's frequencies scale with kB Tproj P / hbar
+    // (src/rpmd.f90:354-393): P > 1 with Tproj <= 0 has none
+    for (int t = 0; t < b->ntraj; ++t)
+        if (b->P[t] > 1 && !(tproj[t] > 0.0))
+            return fail(MDT_EINVAL, "trajectory %d: %d beads need Tproj > 0 (got %g)", t, b->P[t], tproj[t]);
     for (int t = 0; t < ntypes; ++t) {
         const int a = algo[t];
         if (a != MDT_PROP_VERLET && a != MDT_PROP_LANGEVIN && a != MDT_PROP_ANDERSEN && a != MDT_PROP_PILE)
@@ -776,6 +781,7 @@ static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hip
         v.rep_list = k.d_reps;
         v.nbr_global = k.d_nbr;
         v.nbr_stride = k.nbr_stride;
+        v.nbcap = k.nbcap;
         HIPCHK(mdt_launch_replica(k.tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, add_fin, st));
     }
     return 0;
@@ -884,6 +890,7 @@ static int autotune(mdt_batch *b) {
             v.rep_list = k.d_reps;
             v.nbr_global = k.d_nbr;
             v.nbr_stride = k.nbr_stride;
+            v.nbcap = k.nbcap;
             HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
             HIPCHK(hipEventRecord(b->ev0, st));
             for (int q = 0; q < 3; ++q)
@@ -925,6 +932,8 @@ static int autotune_if_needed(mdt_batch *b) {
 int mdt_batch_compute_forces(mdt_batch *b) {
     if (!b) return fail(MDT_EINVAL, "NULL batch");
     HIPCHK(hipSetDevice(b->ctx->device));
+    if (!b->allP1 && !b->have_prop)
+        return fail(MDT_ESTATE, "beads > 1: calc_force's ring-polymer step needs Tproj (call mdt_batch_set_propagation)");
     int rc = ensure_ready(b, false);
     if (rc) return rc;
     if ((rc = autotune_if_needed(b))) return rc;
@@ -1610,6 +1619,12 @@ int mdt_nn_train_descriptor_stats(mdt_nntrain *tr, double *gmin, double *gmax, d
     return 0;
 }
 
+// gradient blocks for ntiles tiles: about two tiles per wavefront, at most
+// the partial buffer's nblocks_grad
+static int grad_blocks(const mdt_nntrain *tr, int ntiles) {
+    return std::max(1, std::min(tr->nblocks_grad, (ntiles + 7) / 8));
+}
+
 // one minibatch: forward/backward, forces, residuals, JVP, gradient partials;
 // then mode 0: reduce into d_grad; mode 1: reduce + Adam
 static int run_minibatch(mdt_nntrain *tr, int m, double wE, double wF, int mode, float lr) {
@@ -1622,9 +1637,9 @@ static int run_minibatch(mdt_nntrain *tr, int m, double wE, double wF, int mode,
     HIPCHK(mdt_launch_nn_frame(nf, mdt_nn_frame_smem(tr->maxN), c->d_nn, tv, tr->d_frames + tr->fr_off[m], 1,
                                wE / nf, wF / (3.0 * tr->natoms_mb[m]), st));
     if (mode < 0) return 0;  // evaluation only
-    const int W = tr->nblocks_grad * 4;
+    const int nb = grad_blocks(tr, nt), W = nb * 4;
     HIPCHK(hipMemsetAsync(tr->d_partials, 0, sizeof(float) * (size_t)W * NN_MAXE * NN_NPAR, st));
-    HIPCHK(mdt_launch_nn_grad(tr->nblocks_grad, c->d_nn, tv, tr->d_tiles + t0, nt, tr->d_rows, tr->d_partials, st));
+    HIPCHK(mdt_launch_nn_grad(nb, c->d_nn, tv, tr->d_tiles + t0, nt, tr->d_rows, tr->d_partials, st));
     if (mode == 1) ++tr->adam_step;
     HIPCHK(mdt_launch_nn_adam(c->d_nn, tr->d_partials, W, tr->d_grad, tr->d_am, tr->d_av, lr, 0.9f, 0.999f, 1e-8f,
                               tr->adam_step > 0 ? tr->adam_step : 1, mode, st));
@@ -1726,7 +1741,7 @@ int mdt_nn_train_loss_grad(mdt_nntrain *tr, int nsel, const int32_t *frames, dou
     if (rc) return rc;
     hipStream_t st = c->stream;
     const NnTrainView tv = tr->view();
-    const int W = tr->nblocks_grad * 4;
+    const int nbg = grad_blocks(tr, (int)tiles.size()), W = nbg * 4;
     hipError_t e1 = hipMemcpy(dt, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice);
     if (e1 == hipSuccess) e1 = hipMemcpy(dr, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice);
     if (e1 == hipSuccess) e1 = hipMemcpy(df, frames, sizeof(int) * nsel, hipMemcpyHostToDevice);
@@ -1736,7 +1751,7 @@ int mdt_nn_train_loss_grad(mdt_nntrain *tr, int nsel, const int32_t *frames, dou
     if (e1 == hipSuccess)
         e1 = hipMemsetAsync(tr->d_partials, 0, sizeof(float) * (size_t)W * NN_MAXE * NN_NPAR, st);
     if (e1 == hipSuccess)
-        e1 = mdt_launch_nn_grad(tr->nblocks_grad, c->d_nn, tv, dt, (int)tiles.size(), dr, tr->d_partials, st);
+        e1 = mdt_launch_nn_grad(nbg, c->d_nn, tv, dt, (int)tiles.size(), dr, tr->d_partials, st);
     if (e1 == hipSuccess)
         e1 = mdt_launch_nn_adam(c->d_nn, tr->d_partials, W, tr->d_grad, tr->d_am, tr->d_av, 0.f, 0.9f, 0.999f,
                                 1e-8f, 1, 0, st);

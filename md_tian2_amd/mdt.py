@@ -353,7 +353,7 @@ class Trainer:
         e = np.ascontiguousarray(E, dtype=np.float64)
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
         self.h = C.c_void_p()
-        _check(lib().mdt_nn_train_create(ctx.h, len(frames), _ip(nat), _ip(idx), _dp(sb), _dp(r), _dp(e), _dp(f),
+        _check(lib().mdt_nn_train_create(ctx.h, len(nat), _ip(nat), _ip(idx), _dp(sb), _dp(r), _dp(e), _dp(f),
                                          batch_frames, val_fraction, seed, C.byref(self.h)))
 
     def close(self):

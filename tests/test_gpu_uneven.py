@@ -27,10 +27,15 @@ def test_uneven_batch_forces(gpu_ctx, pes_hau):
     systems = _mixed(rng)
     cols, pesm = pesfile.emt_tables(pes_hau, systems[0].type_names)
     gpu_ctx.set_emt(cols, pesm)
-    b = mdt.Batch(gpu_ctx, systems)
+    # ring-polymer frequencies scale with kB Tproj P / hbar: P > 1 needs Tproj > 0
+    b = mdt.Batch(gpu_ctx, systems, dt=0.1, tsurf=300.0, tproj=300.0)
     b.tune(tpa=0)  # per-class autotune
     b.compute_forces()
     for s, o in zip(systems, b.get_state()):
+        # calc_force runs the free ring-polymer step first when P > 1
+        # (src/force.f90:54): the forces belong to the returned positions
+        s = s.copy()
+        s.r = o["r"].copy()
         e_r, f_r, d_r = oracle.emt_energy_forces(s, cols, pesm)
         assert np.abs(o["epot"] - e_r).max() <= 1e-9 * max(1.0, np.abs(e_r).max())
         assert np.abs(o["f"] - f_r).max() <= 1e-9
