@@ -32,14 +32,13 @@ def test_uneven_batch_forces(gpu_ctx, pes_hau):
     b.tune(tpa=0)  # per-class autotune
     b.compute_forces()
     for s, o in zip(systems, b.get_state()):
-        # calc_force runs the free ring-polymer step first when P > 1
-        # (src/force.f90:54): the forces belong to the returned positions
-        s = s.copy()
-        s.r = o["r"].copy()
-        e_r, f_r, d_r = oracle.emt_energy_forces(s, cols, pesm)
-        assert np.abs(o["epot"] - e_r).max() <= 1e-9 * max(1.0, np.abs(e_r).max())
-        assert np.abs(o["f"] - f_r).max() <= 1e-9
-        assert np.allclose(o["dens"], d_r, rtol=1e-12, atol=1e-15)
+        # the oracle's calc_force: the free ring-polymer step first when P > 1
+        # (src/force.f90:54), spring forces on Langevin beads (src/rpmd.f90:418-435)
+        st = oracle.OracleState(s, cols, pesm, step=0.1, tsurf=300.0, tproj=300.0)
+        st.calc_force()
+        assert np.abs(o["r"] - st.r).max() <= 1e-12
+        assert np.abs(o["epot"] - st.epot).max() <= 1e-9 * max(1.0, np.abs(st.epot).max())
+        assert np.abs(o["f"] - st.f).max() <= 1e-9
     b.close()
 
 
