@@ -222,7 +222,9 @@ __device__ __forceinline__ bool owns(int k, int n) { return (k < n) != (((k + n)
 // the geometry (differences, r, the exponent arguments r - rcut and
 // r - beta s0, which carry factors up to acut ~ 26/A), the per-atom
 // embedding terms, G and every accumulation stay FP64 / fixed point.
-template <int TPA, bool NBR_LDS, bool F32>
+// JUDGE: FP64 EMT forces of the selected replicas compared with
+// bv.judge_f (the network's judge); nothing else is written.
+template <int TPA, bool NBR_LDS, bool F32, bool JUDGE>
 __global__ void __launch_bounds__(512, 4)
     k_replica(const EmtConst *__restrict__ ecg, const PropConst *__restrict__ pcg, BatchView bv, int add_fin) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -231,7 +233,7 @@ __global__ void __launch_bounds__(512, 4)
     const int rep = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;
     if (bv.only && !bv.only[rep]) return;                                          // whole block: uniform
     if (bv.skip && bv.skip[rep]) return;                                           // whole block: uniform
-    if (bv.judge_acc && (rep % bv.judge_mod) != bv.judge_phase) return;           // whole block: uniform
+    if (JUDGE && (rep % bv.judge_mod) != bv.judge_phase) return;                  // whole block: uniform
     const int t = bv.rep_traj[rep];
     const int b = bv.rep_bead[rep];
     const int N = bv.traj_N[t];
@@ -293,12 +295,28 @@ __global__ void __launch_bounds__(512, 4)
     const unsigned long long below = (wl == 0) ? 0ull : ((~0ull) >> (64 - wl));
 
     // ---- phase 1: owned half neighbour lists (min image, ordered by index) ----
+    // The scan only decides membership: it runs in FP32 on float copies of
+    // the fractional coordinates (in the G/E words, unused until pass A) with
+    // r^2 bound widened by 1e-4 relative; both passes re-check r <= cutr in
+    // FP64.  A fractional difference within 1e-4 of +-1/2 takes its image
+    // from the FP64 rounding, so the chosen image is the FP64 minimum image.
+    float *fx32 = (float *)G, *fy32 = fx32 + N, *fz32 = fy32 + N;
+    for (int i = lane; i < N; i += nthr) {
+        fx32[i] = (float)sx[i];
+        fy32[i] = (float)sy[i];
+        fz32[i] = (float)sz[i];
+    }
+    __syncthreads();
+    float hf[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) hf[q] = (float)cell.h[q];
+    const float cut2f = (float)(cut2 * (1.0 + 1e-4));
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < N;
         const int tk = act ? typ[k] : 0;
-        const double fxk = act ? sx[k] : 0.0, fyk = act ? sy[k] : 0.0, fzk = act ? sz[k] : 0.0;
+        const float fxk = act ? fx32[k] : 0.f, fyk = act ? fy32[k] : 0.f, fzk = act ? fz32[k] : 0.f;
         int c = 0;
         for (int n0 = 0; n0 < N; n0 += TPA) {
             const int n = n0 + sub;
@@ -307,15 +325,18 @@ __global__ void __launch_bounds__(512, 4)
             if (act && n < N && n != k) {
                 const int tn = typ[n];
                 if (T.pair_on[tk * MDT_MAXT + tn] && owns(k, n)) {
-                    double d0 = fxk - sx[n], d1 = fyk - sy[n], d2 = fzk - sz[n];
-                    const double m0 = rint(d0), m1 = rint(d1), m2 = rint(d2);
+                    float d0 = fxk - fx32[n], d1 = fyk - fy32[n], d2 = fzk - fz32[n];
+                    float m0 = rintf(d0), m1 = rintf(d1), m2 = rintf(d2);
+                    if (fabsf(fabsf(d0) - 0.5f) < 1e-4f) m0 = (float)rint(sx[k] - sx[n]);
+                    if (fabsf(fabsf(d1) - 0.5f) < 1e-4f) m1 = (float)rint(sy[k] - sy[n]);
+                    if (fabsf(fabsf(d2) - 0.5f) < 1e-4f) m2 = (float)rint(sz[k] - sz[n]);
                     d0 -= m0;
                     d1 -= m1;
                     d2 -= m2;
-                    const double ex = cell.h[0] * d0 + cell.h[3] * d1 + cell.h[6] * d2;
-                    const double ey = cell.h[1] * d0 + cell.h[4] * d1 + cell.h[7] * d2;
-                    const double ez = cell.h[2] * d0 + cell.h[5] * d1 + cell.h[8] * d2;
-                    in = (ex * ex + ey * ey + ez * ez) <= cut2;
+                    const float ex = hf[0] * d0 + hf[3] * d1 + hf[6] * d2;
+                    const float ey = hf[1] * d0 + hf[4] * d1 + hf[7] * d2;
+                    const float ez = hf[2] * d0 + hf[5] * d1 + hf[8] * d2;
+                    in = (ex * ex + ey * ey + ez * ez) <= cut2f;
                     code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
                 }
             }
@@ -402,7 +423,7 @@ __global__ void __launch_bounds__(512, 4)
             Ek = T.e0[tk] * ((1 + T.lambda[tk] * sv) * el - 1) + 6 * T.v0[tk] * ek;
             const double D = T.e0[tk] * T.lambda[tk] * T.lambda[tk] * (sv * el) + 6 * T.v0[tk] * T.kappa[tk] * ek;
             Gk = -D / (T.betaeta2[tk] * sc);
-            if (!bv.judge_acc) bv.dens[doff + k] = dn;  // raw EMT density (pes_emt_mod.f90:855,861)
+            if (!JUDGE) bv.dens[doff + k] = dn;  // raw EMT density (pes_emt_mod.f90:855,861)
         }
         G[k] = Gk;
         E[k] = Ek;
@@ -497,7 +518,7 @@ __global__ void __launch_bounds__(512, 4)
     __syncthreads();
 
     // ---- judge mode: compare with the given forces, write nothing else ----
-    if (bv.judge_acc) {
+    if constexpr (JUDGE) {
         double sq = 0.0, mx = 0.0;
         for (int k = lane; k < N; k += nthr) {
             const int tk = typ[k];
@@ -871,29 +892,33 @@ size_t mdt_replica_smem(int N, int cap, bool nbr_lds) {
            16;
 }
 
-template <bool L, bool F>
+template <bool L, bool F, bool J>
 static hipError_t launch_replica_t(int tpa, dim3 g, dim3 blk, size_t smem, const EmtConst *ec, const PropConst *pc,
                                    const BatchView &bv, int add_fin, hipStream_t st) {
     switch (tpa) {
-    case 1: hipLaunchKernelGGL((k_replica<1, L, F>), g, blk, smem, st, ec, pc, bv, add_fin); break;
-    case 2: hipLaunchKernelGGL((k_replica<2, L, F>), g, blk, smem, st, ec, pc, bv, add_fin); break;
-    case 4: hipLaunchKernelGGL((k_replica<4, L, F>), g, blk, smem, st, ec, pc, bv, add_fin); break;
-    case 8: hipLaunchKernelGGL((k_replica<8, L, F>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 1: hipLaunchKernelGGL((k_replica<1, L, F, J>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 2: hipLaunchKernelGGL((k_replica<2, L, F, J>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 4: hipLaunchKernelGGL((k_replica<4, L, F, J>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 8: hipLaunchKernelGGL((k_replica<8, L, F, J>), g, blk, smem, st, ec, pc, bv, add_fin); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-// calc_force on every replica; ec and pc are device pointers
+// calc_force on every replica; ec and pc are device pointers.  The judge
+// (bv.judge_acc set) always runs the FP64 arithmetic.
 hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst *ec, const PropConst *pc,
                               const BatchView &bv, int add_fin, hipStream_t st) {
     dim3 g(nrep), blk(block);
     const bool lds = bv.nbr_global == nullptr;
+    if (bv.judge_acc)
+        return lds ? launch_replica_t<true, false, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
+                   : launch_replica_t<false, false, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
     if (bv.precision == 1)
-        return lds ? launch_replica_t<true, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
-                   : launch_replica_t<false, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
-    return lds ? launch_replica_t<true, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
-               : launch_replica_t<false, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+        return lds ? launch_replica_t<true, true, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
+                   : launch_replica_t<false, true, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+    return lds ? launch_replica_t<true, false, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
+               : launch_replica_t<false, false, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
 }
 
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
