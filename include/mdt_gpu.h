@@ -232,6 +232,15 @@ int mdt_nn_params_set(mdt_context *ctx, int nelem, const int32_t *elem_of_type, 
                       const double *weights);
 
 /*
+ * read_nene's file part (src/pes_nene_mod.f90:123-5759): reads input.nn,
+ * scaling.data and weights.XXX.data from `directory` (the reference's
+ * inp_path) and uploads the network as mdt_nn_params_set does.
+ * type_symbols: ntypes element symbols of 2 characters each (space padded),
+ * the universe's type names in idx order.
+ */
+int mdt_nn_load(mdt_context *ctx, const char *directory, int ntypes, const char *type_symbols);
+
+/*
  * compute_nene(atoms, flag) on one universe (src/pes_nene_mod.f90:7145-7368):
  * epot(b) += E_b [eV], f(:,b,i) += F [eV/A]; r, f column-major (3,P,N) in
  * Angstrom as in compute_emt.  nextrap (may be NULL) is set to 1 when a

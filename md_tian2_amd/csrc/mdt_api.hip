@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
+#include <cstdlib>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -1797,6 +1799,201 @@ int mdt_nn_get_weights(mdt_context *ctx, double *weights) {
         weights[o++] = M.b3[e];
     }
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// RuNNer model files (read_nene, src/pes_nene_mod.f90:123-5759): input.nn
+// keywords, scaling.data, weights.XXX.data; symmetry functions sorted as
+// sortsymfunctions does (type, cut-off, eta, zeta, lambda, rshift, element 1,
+// element 2; stable), elements by atomic number.
+// ---------------------------------------------------------------------------
+namespace {
+struct SfDef {
+    int type, e1, e2;  // atomic numbers
+    double eta, rs, lam, zeta, rc;
+};
+
+int atomic_number(const std::string &sym) {
+    static const char *tab[] = {"H",  "He", "Li", "Be", "B",  "C",  "N",  "O",  "F",  "Ne", "Na", "Mg", "Al", "Si",
+                                "P",  "S",  "Cl", "Ar", "K",  "Ca", "Sc", "Ti", "V",  "Cr", "Mn", "Fe", "Co", "Ni",
+                                "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr", "Rb", "Sr", "Y",  "Zr", "Nb", "Mo",
+                                "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn", "Sb", "Te", "I",  "Xe", "Cs", "Ba",
+                                "La", "Ce", "Pr", "Nd", "Pm", "Sm", "Eu", "Gd", "Tb", "Dy", "Ho", "Er", "Tm", "Yb",
+                                "Lu", "Hf", "Ta", "W",  "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl", "Pb", "Bi"};
+    for (int z = 0; z < (int)(sizeof(tab) / sizeof(tab[0])); ++z)
+        if (sym == tab[z]) return z + 1;
+    return -1;
+}
+
+std::vector<std::string> words_of(const std::string &line) {
+    std::string l = line.substr(0, line.find('#'));
+    std::vector<std::string> w;
+    size_t i = 0;
+    while (i < l.size()) {
+        while (i < l.size() && std::isspace((unsigned char)l[i])) ++i;
+        size_t j = i;
+        while (j < l.size() && !std::isspace((unsigned char)l[j])) ++j;
+        if (j > i) w.push_back(l.substr(i, j - i));
+        i = j;
+    }
+    return w;
+}
+}  // namespace
+
+int mdt_nn_load(mdt_context *ctx, const char *directory, int ntypes, const char *type_symbols) {
+    if (!ctx || !directory || !type_symbols) return fail(MDT_EINVAL, "mdt_nn_load: NULL argument");
+    if (ntypes < 1 || ntypes > MDT_MAXT) return fail(MDT_EINVAL, "ntypes=%d", ntypes);
+    const std::string dir = std::string(directory) + "/";
+    FILE *fp = std::fopen((dir + "input.nn").c_str(), "r");
+    if (!fp) return fail(MDT_EINVAL, "cannot open %sinput.nn", dir.c_str());
+    std::vector<int> elems;
+    std::vector<int> hidden;
+    std::string act;
+    int nh = -1, scale = 0, center = 0, ctype = 1;
+    double scmin = 0.0, scmax = 1.0, calpha = 0.0;
+    std::vector<std::pair<int, SfDef>> sfs;  // (centre Z, function)
+    char buf[4096];
+    while (std::fgets(buf, sizeof buf, fp)) {
+        const std::vector<std::string> w = words_of(buf);
+        if (w.empty()) continue;
+        const std::string &k = w[0];
+        auto num = [&](size_t i) { return i < w.size() ? std::atof(w[i].c_str()) : 0.0; };
+        if (k == "elements") {
+            for (size_t i = 1; i < w.size(); ++i) elems.push_back(atomic_number(w[i]));
+        } else if (k == "global_hidden_layers_short") {
+            nh = (int)num(1);
+        } else if (k == "global_nodes_short") {
+            for (size_t i = 1; i < w.size(); ++i) hidden.push_back(std::atoi(w[i].c_str()));
+        } else if (k == "global_activation_short") {
+            for (size_t i = 1; i < w.size(); ++i) act += w[i];
+        } else if (k == "scale_symmetry_functions") {
+            scale = 1;
+        } else if (k == "center_symmetry_functions") {
+            center = 1;
+        } else if (k == "scale_min_short_atomic") {
+            scmin = num(1);
+        } else if (k == "scale_max_short_atomic") {
+            scmax = num(1);
+        } else if (k == "cutoff_type") {
+            ctype = (int)num(1);
+        } else if (k == "cutoff_alpha") {
+            calpha = num(1);
+        } else if (k == "symfunction_short" && w.size() >= 4) {
+            SfDef f{};
+            const int zc = atomic_number(w[1]);
+            f.type = std::atoi(w[2].c_str());
+            if (f.type == 2 && w.size() >= 7) {
+                f.e1 = atomic_number(w[3]);
+                f.eta = num(4);
+                f.rs = num(5);
+                f.rc = num(6);
+            } else if (f.type == 3 && w.size() >= 9) {
+                const int a = atomic_number(w[3]), b2 = atomic_number(w[4]);
+                f.e1 = std::min(a, b2);
+                f.e2 = std::max(a, b2);
+                f.eta = num(5);
+                f.lam = num(6);
+                f.zeta = num(7);
+                f.rc = num(8);
+            } else {
+                std::fclose(fp);
+                return fail(MDT_EUNSUP, "input.nn: symmetry function type %d (2 and 3 are supported)", f.type);
+            }
+            sfs.push_back({zc, f});
+        }
+    }
+    std::fclose(fp);
+    std::sort(elems.begin(), elems.end());
+    if (nh >= 0 && (int)hidden.size() > nh) hidden.resize(nh);
+    if (hidden.size() != 2 || act.size() != 3)
+        return fail(MDT_EUNSUP, "network must have two hidden layers and three activations");
+    if (ctype != 1 || calpha != 0.0) return fail(MDT_EUNSUP, "only cutoff_type 1 with cutoff_alpha 0");
+    const int ne = (int)elems.size();
+    if (ne < 1 || ne > NN_MAXE) return fail(MDT_EUNSUP, "%d elements", ne);
+    auto eidx = [&](int z) {
+        for (int i = 0; i < ne; ++i)
+            if (elems[i] == z) return i;
+        return -1;
+    };
+    std::vector<std::vector<SfDef>> per(ne);
+    for (auto &p : sfs) {
+        const int e = eidx(p.first);
+        if (e < 0) return fail(MDT_EINVAL, "input.nn: symmetry function of an element not in `elements`");
+        per[e].push_back(p.second);
+    }
+    auto key_less = [](const SfDef &a, const SfDef &b) {
+        if (a.type != b.type) return a.type < b.type;
+        if (a.rc != b.rc) return a.rc < b.rc;
+        if (a.eta != b.eta) return a.eta < b.eta;
+        if (a.zeta != b.zeta) return a.zeta < b.zeta;
+        if (a.lam != b.lam) return a.lam < b.lam;
+        if (a.rs != b.rs) return a.rs < b.rs;
+        if (a.e1 != b.e1) return a.e1 < b.e1;
+        return a.e2 < b.e2;
+    };
+    std::vector<int32_t> nsf(ne), typ, e1, e2;
+    std::vector<double> eta, rs, lam, zeta, rc;
+    for (int e = 0; e < ne; ++e) {
+        std::stable_sort(per[e].begin(), per[e].end(), key_less);
+        nsf[e] = (int)per[e].size();
+        for (const SfDef &f : per[e]) {
+            typ.push_back(f.type);
+            e1.push_back(eidx(f.e1));
+            e2.push_back(f.type == 3 ? eidx(f.e2) : 0);
+            if (e1.back() < 0 || (f.type == 3 && e2.back() < 0))
+                return fail(MDT_EINVAL, "input.nn: neighbour element not in `elements`");
+            eta.push_back(f.eta);
+            rs.push_back(f.rs);
+            lam.push_back(f.lam);
+            zeta.push_back(f.zeta);
+            rc.push_back(f.rc);
+        }
+    }
+    // scaling.data
+    std::vector<double> gmin, gmax, gavg;
+    fp = std::fopen((dir + "scaling.data").c_str(), "r");
+    if (!fp) return fail(MDT_EINVAL, "cannot open %sscaling.data", dir.c_str());
+    for (int e = 0; e < ne; ++e)
+        for (int q = 0; q < nsf[e]; ++q) {
+            int ie, iq;
+            double a, b2, c;
+            if (std::fscanf(fp, "%d %d %lf %lf %lf", &ie, &iq, &a, &b2, &c) != 5 || ie != e + 1 || iq != q + 1) {
+                std::fclose(fp);
+                return fail(MDT_EINVAL, "scaling.data: bad line for element %d function %d", e + 1, q + 1);
+            }
+            gmin.push_back(a);
+            gmax.push_back(b2);
+            gavg.push_back(c);
+        }
+    std::fclose(fp);
+    // weights.XXX.data
+    std::vector<double> wts;
+    for (int e = 0; e < ne; ++e) {
+        char name[32];
+        std::snprintf(name, sizeof name, "weights.%03d.data", elems[e]);
+        fp = std::fopen((dir + name).c_str(), "r");
+        if (!fp) return fail(MDT_EINVAL, "cannot open %s%s", dir.c_str(), name);
+        const long nw = (long)nsf[e] * hidden[0] + hidden[0] + (long)hidden[0] * hidden[1] + hidden[1] + hidden[1] + 1;
+        for (long q = 0; q < nw; ++q) {
+            double x;
+            if (std::fscanf(fp, "%lf", &x) != 1) {
+                std::fclose(fp);
+                return fail(MDT_EINVAL, "%s: %ld weights, the network needs %ld", name, q, nw);
+            }
+            wts.push_back(x);
+        }
+        std::fclose(fp);
+    }
+    // atom types -> elements (type_symbols: ntypes entries of 2 characters)
+    std::vector<int32_t> eot(ntypes);
+    for (int t = 0; t < ntypes; ++t) {
+        std::string sym(type_symbols + 2 * t, 2);
+        while (!sym.empty() && (sym.back() == ' ' || sym.back() == '\0')) sym.pop_back();
+        eot[t] = eidx(atomic_number(sym));
+    }
+    return mdt_nn_params_set(ctx, ne, eot.data(), ntypes, nsf.data(), typ.data(), e1.data(), e2.data(), eta.data(),
+                             rs.data(), lam.data(), zeta.data(), rc.data(), gmin.data(), gmax.data(), gavg.data(),
+                             scmin, scmax, scale, center, hidden[0], hidden[1], act.c_str(), wts.data());
 }
 
 }  // extern "C"

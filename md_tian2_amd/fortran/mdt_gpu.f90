@@ -172,6 +172,58 @@ module mdt_gpu
             integer(c_int) :: rc
         end function
 
+        function mdt_batch_set_precision(batch, mode) bind(C, name="mdt_batch_set_precision") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: mode
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_set_graph(batch, chunk) bind(C, name="mdt_batch_set_graph") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: chunk
+            integer(c_int) :: rc
+        end function
+
+        function mdt_nn_load(ctx, directory, ntypes, type_symbols) bind(C, name="mdt_nn_load") result(rc)
+            import :: c_int, c_ptr, c_char
+            type(c_ptr), value :: ctx
+            character(kind=c_char), intent(in) :: directory(*)
+            integer(c_int), value :: ntypes
+            character(kind=c_char), intent(in) :: type_symbols(*)
+            integer(c_int) :: rc
+        end function
+
+        function mdt_compute_nene(ctx, natoms, nbeads, idx, simbox, isimbox, r, epot, f, nextrap, flag) &
+                bind(C, name="mdt_compute_nene") result(rc)
+            import :: c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: natoms, nbeads
+            integer(c_int32_t), intent(in) :: idx(*)
+            real(c_double), intent(in) :: simbox(*), isimbox(*), r(*)
+            real(c_double), intent(inout) :: epot(*), f(*)
+            integer(c_int32_t), intent(out) :: nextrap
+            integer(c_int32_t), value :: flag
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_set_pes(batch, pes, judge_interval) bind(C, name="mdt_batch_set_pes") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: pes, judge_interval
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_nn_stats(batch, judge_sq, judge_n, judge_max, nfallback) &
+                bind(C, name="mdt_batch_nn_stats") result(rc)
+            import :: c_int, c_ptr, c_double, c_int32_t
+            type(c_ptr), value :: batch
+            real(c_double), intent(out) :: judge_sq, judge_n, judge_max
+            integer(c_int32_t), intent(out) :: nfallback
+            integer(c_int) :: rc
+        end function
+
     end interface
 
 contains

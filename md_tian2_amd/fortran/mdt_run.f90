@@ -13,7 +13,12 @@
 ! lattice <n> <name> <mass> <algo>, pes <file>.  Added for the batch:
 ! slab <a> <nx> <ny> <nlayers> <nfixed> (fcc(111) slab generated here, as
 ! the reference ships no metal slabs), rng_seed <n> (key of the noise
-! stream) and dump <file> (binary initial and final states).
+! stream), dump <file> (binary initial and final states), precision
+! <fp64|fp32_pair> (EMT pair-term arithmetic), graph <chunk> (hipGraph
+! replay of the step loop), nene <dir> (the NENE network of dir's input.nn,
+! scaling.data, weights.XXX.data as the potential, EMT of `pes` as its
+! fallback on extrapolation) and judge <n> (EMT judge of the network on a
+! rotating 1 % of the trajectories every n steps).
 program mdt_run
 
     use, intrinsic :: iso_c_binding
@@ -35,7 +40,10 @@ program mdt_run
     character(len=8) :: algo_proj = "ver", algo_latt = "lan", azimuth = "0"
     character(len=8) :: pip_x = "r", pip_y = "r"
     character(len=3) :: name_proj = "H", name_latt = "Au"
-    character(len=512) :: pes_file = "", dump_file = "", deck = ""
+    character(len=512) :: pes_file = "", dump_file = "", deck = "", nene_dir = ""
+    integer :: prec = 0, graph_chunk = 0, judge_interval = 0
+    real(dp) :: jsq, jn, jmax
+    integer(c_int32_t) :: nfb
 
     type(universe) :: tmpl
     type(c_ptr) :: ctx, batch
@@ -97,11 +105,24 @@ program mdt_run
                    "mdt_batch_set_propagation")
     call mdt_check(mdt_batch_seed(batch, int(rng_seed, c_int32_t), tid), "mdt_batch_seed")
     call mdt_check(mdt_batch_set_state(batch, r, v), "mdt_batch_set_state")
+    call mdt_check(mdt_batch_set_precision(batch, int(prec, c_int)), "mdt_batch_set_precision")
+    call mdt_check(mdt_batch_set_graph(batch, int(graph_chunk, c_int)), "mdt_batch_set_graph")
+    if (len_trim(nene_dir) > 0) then
+        call mdt_check(mdt_nn_load(ctx, cstr(nene_dir), 2_c_int, cstr(name2(tmpl%name(1)) // name2(tmpl%name(2)))), &
+                       "mdt_nn_load")
+        call mdt_check(mdt_batch_set_pes(batch, 1_c_int, int(judge_interval, c_int)), "mdt_batch_set_pes")
+    end if
     call mdt_check(mdt_batch_tune(batch, 0_c_int, 0_c_int, tpa), "mdt_batch_tune")
     call mdt_check(mdt_batch_compute_forces(batch), "mdt_batch_compute_forces")
     call mdt_check(mdt_batch_step(batch, int(nsteps, c_int)), "mdt_batch_step")
     call mdt_check(mdt_batch_last_timing(batch, ms, nl), "mdt_batch_last_timing")
     call mdt_check(mdt_batch_get_state(batch, r, v, f, a, epot, dens), "mdt_batch_get_state")
+    if (len_trim(nene_dir) > 0) then
+        call mdt_check(mdt_batch_nn_stats(batch, jsq, jn, jmax, nfb), "mdt_batch_nn_stats")
+        if (jn > 0) print '(a,es12.4,a,i0,a)', "network judge: force RMSE ", sqrt(jsq / jn), " eV/A over ", &
+            int(jn), " components"
+        print '(a,i0)', "trajectories on the EMT fallback at the last step: ", nfb
+    end if
 
     print '(a,i0,a,i0,a,i0,a,f10.3,a,es12.4,a)', "mdt_run: ", ntrajs, " trajectories x ", nsteps, &
         " steps (tpa ", tpa, ") in ", ms, " ms: ", real(ntrajs, dp) * nsteps / max(ms, 1e-9_dp) * 1e3_dp, &
@@ -173,12 +194,42 @@ contains
                 read(w(5), *) nlayers; read(w(6), *) nfixed
             case ("rng_seed"); read(w(2), *) rng_seed
             case ("dump"); dump_file = unquote(w(2))
+            case ("nene"); nene_dir = unquote(w(2))
+            case ("judge"); read(w(2), *) judge_interval
+            case ("graph"); read(w(2), *) graph_chunk
+            case ("precision")
+                if (trim(w(2)) == "fp32_pair") then
+                    prec = 1
+                else if (trim(w(2)) == "fp64") then
+                    prec = 0
+                else
+                    print '(a,a)', "Error: precision must be fp64 or fp32_pair, got ", trim(w(2))
+                    stop 1
+                end if
             case default
                 continue  ! unknown keywords are skipped, as src/run_config.f90:667-669
             end select
         end do
         close(unit)
     end subroutine read_deck
+
+    ! NUL-terminated copy of a Fortran string for the C-ABI
+    function cstr(s) result(c)
+        character(len=*), intent(in) :: s
+        character(kind=c_char) :: c(len_trim(s) + 1)
+        integer :: i
+        do i = 1, len_trim(s)
+            c(i) = s(i:i)
+        end do
+        c(len_trim(s) + 1) = c_null_char
+    end function cstr
+
+    ! element symbol padded to the 2 characters per type of mdt_nn_load
+    function name2(s) result(t)
+        character(len=*), intent(in) :: s
+        character(len=2) :: t
+        t = adjustl(s)
+    end function name2
 
     function unquote(s) result(t)
         character(len=*), intent(in) :: s

@@ -31,7 +31,7 @@ SYMBOLS = (
     "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph",
     "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats",
     "mdt_batch_nn_descriptors", "mdt_nn_train_create", "mdt_nn_train_destroy", "mdt_nn_train_descriptor_stats",
-    "mdt_nn_train_epoch", "mdt_nn_train_eval", "mdt_nn_train_loss_grad", "mdt_nn_get_weights",
+    "mdt_nn_train_epoch", "mdt_nn_train_eval", "mdt_nn_train_loss_grad", "mdt_nn_get_weights", "mdt_nn_load",
 )
 
 
@@ -85,6 +85,7 @@ def lib():
         L.mdt_nn_train_eval.argtypes = [vp, C.c_int, dp, dp]
         L.mdt_nn_train_loss_grad.argtypes = [vp, C.c_int, ip, C.c_double, C.c_double, dp, dp]
         L.mdt_nn_get_weights.argtypes = [vp, dp]
+        L.mdt_nn_load.argtypes = [vp, C.c_char_p, C.c_int, C.c_char_p]
         _LIB = L
     return _LIB
 
@@ -162,6 +163,11 @@ class Context:
                                        int(model.scale), int(model.center), model.hidden[0], model.hidden[1],
                                        model.activations.encode(), _dp(w)))
         self._keep_nn = (eot, nsf, typ, e1, e2, eta, rs, lam, zeta, rc, gmin, gmax, gavg, w)
+
+    def load_nn(self, directory, type_names):
+        """The library's own reader of input.nn / scaling.data / weights.XXX.data."""
+        syms = "".join(f"{t[0]:<2s}"[:2] for t in type_names).encode()
+        _check(lib().mdt_nn_load(self.h, str(directory).encode(), len(type_names), syms))
 
     def compute_nene(self, sysm, epot=None, f=None):
         """Drop-in compute_nene on one universe; returns (epot [eV], f [eV/A], extrapolated)."""

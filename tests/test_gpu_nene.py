@@ -150,3 +150,19 @@ def test_judge_statistics(gpu_ctx, pes_hau):
     assert n == cnt
     assert abs(rmse - np.sqrt(sq / cnt)) <= 1e-6 * np.sqrt(sq / cnt)
     b.close()
+
+
+def test_library_reader_matches_python_reader(gpu_ctx):
+    """mdt_nn_load (the library's RuNNer reader, used by the Fortran host)
+    uploads the same network as the Python reader: bitwise equal results."""
+    structs = nene.read_input_data(os.path.join(FIX, "input.data"), limit=2)
+    m = nene.load(FIX)
+    out = []
+    for loader in ("python", "library"):
+        if loader == "python":
+            gpu_ctx.set_nn(m, [("C", False)])
+        else:
+            gpu_ctx.load_nn(FIX, [("C", False)])
+        out.append([gpu_ctx.compute_nene(_system_from_runner(s))[:2] for s in structs])
+    for (e1, f1), (e2, f2) in zip(*out):
+        assert np.array_equal(e1, e2) and np.array_equal(f1, f2)
