@@ -176,7 +176,8 @@ def rpmd_variant(ctx, args):
     """RPMD H/Pt(111): 8x8x6 Pt (384, bottom 2 layers fixed, Langevin+LDFA at
     100 K), H under PILE (tau 200 fs, 100 K), 32 beads on every atom (the
     reference's force_beads replication of the 1-bead slab), dt 0.1 fs;
-    FP64 and FP32-pair EMT, FP64 normal-mode transform."""
+    FP64 and FP32-pair EMT, FP64 normal-mode transform; then the mixed bead
+    model of the same trajectories (H 32 beads, classical slab), FP64."""
     from md_tian2_amd import geometry, mdt, pesfile
 
     pes = pesfile.load_pes(os.path.join(ROOT, "md_tian2_amd", "pes", "EMT-HPt.pes"))
@@ -202,6 +203,20 @@ def rpmd_variant(ctx, args):
         out[key] = {"value": len(systems) * args.rpmd_steps / (ms / 1e3), "unit": "trajectory-steps/s",
                     "ms_per_step": ms / args.rpmd_steps}
         print(f"[bench] rpmd {key}: {ms / args.rpmd_steps:.3f} ms/step", file=sys.stderr, flush=True)
+    # the mixed bead model of the same workload: H 32 beads, slab classical
+    b = mdt.Batch(ctx, systems, dt=0.1, tsurf=100.0, tproj=100.0, pile_tau=200.0, seed=99)
+    b.set_bead_mode(1)
+    b.tune(tpa=0)
+    b.compute_forces()
+    b.step(2)
+    b.step(args.rpmd_mixed_steps)
+    ms, _ = b.timing()
+    assert not b.status().any(), "trajectory failure flags set"
+    b.close()
+    out["mixed_fp64"] = {"value": len(systems) * args.rpmd_mixed_steps / (ms / 1e3), "unit": "trajectory-steps/s",
+                         "ms_per_step": ms / args.rpmd_mixed_steps, "steps": args.rpmd_mixed_steps,
+                         "model": "H 32 beads, slab classical (1 bead)"}
+    print(f"[bench] rpmd mixed_fp64: {ms / args.rpmd_mixed_steps:.3f} ms/step", file=sys.stderr, flush=True)
     return out
 
 
@@ -220,6 +235,7 @@ def main():
     ap.add_argument("--no-nn", action="store_true")
     ap.add_argument("--rpmd-traj", type=int, default=1024)
     ap.add_argument("--rpmd-steps", type=int, default=20)
+    ap.add_argument("--rpmd-mixed-steps", type=int, default=200)
     ap.add_argument("--no-rpmd", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

@@ -47,6 +47,10 @@ extern "C" {
 #define MDT_PREC_FP64 0      /* everything in FP64 */
 #define MDT_PREC_FP32_PAIR 1 /* FP32 pair terms, FP64 accumulation (mdt_batch_set_precision) */
 
+/* bead modes (mdt_batch_set_bead_mode) */
+#define MDT_BEADS_ALL 0   /* every atom carries the trajectory's beads (the reference's model) */
+#define MDT_BEADS_MIXED 1 /* the projectile carries the beads, the other atoms are classical */
+
 /* per-trajectory status bits (mdt_batch_status) */
 #define MDT_ST_NBR_OVERFLOW 1  /* neighbour list capacity exceeded */
 #define MDT_ST_NEG_DENSITY 2   /* ldfa saw a negative density (src/md_algo.f90:551-554) */
@@ -192,6 +196,29 @@ int mdt_batch_tune(mdt_batch *b, int tpa, int nbcap, int *tpa_chosen);
  *           normal-mode transform are FP64 in both modes.
  */
 int mdt_batch_set_precision(mdt_batch *b, int mode);
+
+/*
+ * Bead model of the trajectories with nbeads > 1.  The reference gives every
+ * atom the same number of beads (universe%r(3,nbeads,natoms),
+ * src/universe_mod.f90:38-67; calc_force evaluates the PES per bead,
+ * src/force.f90:40-98).  MDT_BEADS_MIXED is an extension without a
+ * counterpart there: atom 1 (a projectile type) keeps its beads and the
+ * other atoms (non-projectile types only) are classical, with their state
+ * in bead 1's slots (positions and velocities given for other beads are not
+ * read; the state getters report bead 1 in every bead).  The potential of
+ * bead b is the EMT energy of the classical atoms with the projectile's
+ * bead b; the projectile's bead b feels -dV_b/dr_b and a classical atom the
+ * bead average of -dV_b/dR, so that the total energy
+ *   sum_b (m v_b^2 / 2 + V_b) + ring-polymer springs + P sum_classical M V^2 / 2
+ * is conserved by the Verlet propagator.  epot holds V_b per bead; dens of a
+ * classical atom is its bead-averaged density.  Classical atoms propagate as
+ * one-bead atoms (kB T / m thermostat temperature, PILE on the single mode,
+ * no ring-polymer step).  EMT only, FP64 (the precision mode applies to the
+ * one-bead trajectories).  Trajectories with nbeads == 1 are unaffected.
+ * Checked at the next launch: atom 1 of a projectile type, no other
+ * projectile atom.
+ */
+int mdt_batch_set_bead_mode(mdt_batch *b, int mode);
 
 /*
  * Step-loop launches through a hipGraph: with chunk > 0, mdt_batch_step

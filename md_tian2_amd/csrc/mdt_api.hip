@@ -22,6 +22,9 @@ hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const Ba
                            const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
                            int flags, hipStream_t st);
 hipError_t mdt_launch_advance(uint32_t *step_base, uint32_t n, int set, hipStream_t st);
+size_t mdt_mixed_smem(int nmax, int pmax);
+hipError_t mdt_launch_mixed(int tpa, int ntraj, size_t smem, const EmtConst *ec, const PropConst *pc,
+                            const BatchView &bv, int add_fin, hipStream_t st);
 size_t mdt_nene_smem(int N);
 hipError_t mdt_nene_attributes(hipFuncAttributes *fa);
 size_t mdt_nn_frame_smem(int N);
@@ -107,6 +110,7 @@ struct mdt_batch {
     std::vector<SizeClass> cls;
     int ntraj = 0, nrep = 0;
     std::vector<int> N, P;
+    std::vector<int> rep_t;         // trajectory of each replica
     std::vector<long long> soff, aoff, eoff;
     long long tot3pn = 0, totpn = 0, totn = 0, totp = 0;
     int maxN = 0, maxP = 0;
@@ -150,6 +154,18 @@ struct mdt_batch {
     double *d_judge = nullptr;      // per replica: sum of squared force differences, count, max
     int judge_interval = 0;         // EMT judge every judge_interval steps (0: off)
     uint32_t judge_events = 0;
+    // bead mode 1 (mixed): in trajectories with P > 1 the projectile (atom 0)
+    // carries the beads and the other atoms are classical, kept in bead 0;
+    // those trajectories run k_mixed (one workgroup each), the rest k_replica
+    int bead_mode = 0;
+    std::vector<uint8_t> h_type;    // atom types (0-based), host copy for validation
+    std::vector<int> mix_traj;      // trajectories on k_mixed
+    int *d_mix = nullptr;
+    uint16_t *d_mix_nbr = nullptr;  // classical atoms' owned half lists, one block per listed trajectory
+    long long mix_stride = 0;
+    int mix_tpa = 0, mix_nbcap = 0, mix_nmax = 0, mix_pmax = 0;
+    size_t mix_smem = 0;
+    float mix_tuned_ms = 0.f;
 
     BatchView view() const {
         BatchView v{};
@@ -355,6 +371,8 @@ int mdt_emt_params_set(mdt_context *ctx, int ntypes, const double *eta2, const d
     return 0;
 }
 
+static void free_classes(mdt_batch *b);
+
 int mdt_batch_destroy(mdt_batch *b) {
     if (!b) return 0;
     (void)hipSetDevice(b->ctx->device);
@@ -362,17 +380,14 @@ int mdt_batch_destroy(mdt_batch *b) {
                     b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
                     b->d_cjk, b->d_status, b->d_pc, b->d_step, b->d_nnG, b->d_nnC,
-                    b->d_nnflag, b->d_nnF, b->d_judge};
+                    b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     for (hipEvent_t e : b->evf) (void)hipEventDestroy(e);
-    for (SizeClass &c : b->cls) {
-        if (c.d_reps) (void)hipFree(c.d_reps);
-        if (c.d_nbr) (void)hipFree(c.d_nbr);
-    }
+    free_classes(b);
     delete b;
     return 0;
 }
@@ -391,6 +406,55 @@ static void build_cjk_host(int P, double *cjk) {  // rpmd.f90:36-59
                 v = std::sqrt(2.0 / P) * std::sin(2.0 * M_PI * j * k / P);
             cjk[(j - 1) + P * k] = v;
         }
+}
+
+static void free_classes(mdt_batch *b) {
+    for (SizeClass &c : b->cls) {
+        if (c.d_reps) (void)hipFree(c.d_reps);
+        if (c.d_nbr) (void)hipFree(c.d_nbr);
+    }
+    b->cls.clear();
+}
+
+// Size classes of the replicas that k_replica computes: every replica, or in
+// mixed bead mode those of the trajectories with one bead.
+static int build_classes(mdt_batch *b) {
+    free_classes(b);
+    const int bounds[6] = {64, 128, 256, 512, 1024, 2048};
+    std::vector<std::vector<int>> mem(6);
+    std::vector<int> nmax(6, 0);
+    int taken = 0;
+    for (int r = 0; r < b->nrep; ++r) {
+        const int t = b->rep_t[r];
+        if (b->bead_mode == 1 && b->P[t] > 1) continue;
+        const int n = b->N[t];
+        int k = 0;
+        while (n > bounds[k]) ++k;
+        mem[k].push_back(r);
+        nmax[k] = std::max(nmax[k], n);
+        ++taken;
+    }
+    int nonempty = 0;
+    for (int k = 0; k < 6; ++k) nonempty += !mem[k].empty();
+    // a list is needed unless one class holds every replica in order
+    const bool lists = nonempty > 1 || taken < b->nrep;
+    for (int k = 0; k < 6; ++k) {
+        if (mem[k].empty()) continue;
+        SizeClass c;
+        c.nmax = nmax[k];
+        c.nrep = (int)mem[k].size();
+        if (lists) {
+            int rc = dalloc(&c.d_reps, mem[k].size());
+            if (!rc && hipMemcpy(c.d_reps, mem[k].data(), sizeof(int) * mem[k].size(), hipMemcpyHostToDevice) !=
+                           hipSuccess)
+                rc = fail(MDT_EHIP, "size-class replica list");
+            b->cls.push_back(c);
+            if (rc) return rc;
+        } else {
+            b->cls.push_back(c);
+        }
+    }
+    return 0;
 }
 
 int mdt_batch_create(mdt_context *ctx, int ntraj, const int32_t *natoms, const int32_t *nbeads,
@@ -438,35 +502,11 @@ int mdt_batch_create(mdt_context *ctx, int ntraj, const int32_t *natoms, const i
         cjk.insert(cjk.end(), c.begin(), c.end());
     }
     b->nrep = (int)rep_t.size();
-    {
-        const int bounds[6] = {64, 128, 256, 512, 1024, 2048};
-        std::vector<std::vector<int>> mem(6);
-        std::vector<int> nmax(6, 0);
-        for (int r = 0; r < b->nrep; ++r) {
-            const int n = b->N[rep_t[r]];
-            int k = 0;
-            while (n > bounds[k]) ++k;
-            mem[k].push_back(r);
-            nmax[k] = std::max(nmax[k], n);
-        }
-        int nonempty = 0;
-        for (int k = 0; k < 6; ++k) nonempty += !mem[k].empty();
-        for (int k = 0; k < 6; ++k) {
-            if (mem[k].empty()) continue;
-            SizeClass c;
-            c.nmax = nmax[k];
-            c.nrep = (int)mem[k].size();
-            if (nonempty > 1) {
-                if (dalloc(&c.d_reps, mem[k].size()) ||
-                    hipMemcpy(c.d_reps, mem[k].data(), sizeof(int) * mem[k].size(), hipMemcpyHostToDevice) !=
-                        hipSuccess) {
-                    b->cls.push_back(c);
-                    mdt_batch_destroy(b);
-                    return fail(MDT_EHIP, "size-class replica list");
-                }
-            }
-            b->cls.push_back(c);
-        }
+    b->rep_t = rep_t;
+    if (build_classes(b)) {
+        std::string msg = g_err;
+        mdt_batch_destroy(b);
+        return fail(MDT_EHIP, "%s", msg.c_str());
     }
     int rc = 0;
     rc |= dalloc(&b->d_rep_traj, b->nrep);
@@ -548,6 +588,7 @@ int mdt_batch_set_atoms(mdt_batch *b, const int32_t *idx, const double *mass, co
         q += 3LL * P * N;
     }
     hipStream_t st = b->ctx->stream;
+    b->h_type = ty;
     HIPCHK(hipMemcpyAsync(b->d_type, ty.data(), b->totn, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(b->d_mass, mass, sizeof(double) * b->totn, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(b->d_fixed, fx.data(), b->totpn, hipMemcpyHostToDevice, st));
@@ -623,14 +664,23 @@ static void to_device_layout(const mdt_batch *b, const double *src, std::vector<
     }
 }
 
+// bead slot holding atom i's bead bb on the device: in mixed bead mode a
+// classical atom (i > 0 of a trajectory with beads) lives in bead 0 and is
+// reported in every bead
+static inline int dev_bead(const mdt_batch *b, int t, int i, int bb) {
+    return (b->bead_mode == 1 && b->P[t] > 1 && i > 0) ? 0 : bb;
+}
+
 static void from_device_layout(const mdt_batch *b, const std::vector<double> &src, double *dst) {
     for (int t = 0; t < b->ntraj; ++t) {
         const int N = b->N[t], P = b->P[t];
         const long long o = b->soff[t];
         for (int i = 0; i < N; ++i)
-            for (int bb = 0; bb < P; ++bb)
+            for (int bb = 0; bb < P; ++bb) {
+                const int sb = dev_bead(b, t, i, bb);
                 for (int c = 0; c < 3; ++c)
-                    dst[o + c + 3 * (bb + (long long)P * i)] = src[o + ((long long)bb * 3 + c) * N + i];
+                    dst[o + c + 3 * (bb + (long long)P * i)] = src[o + ((long long)sb * 3 + c) * N + i];
+            }
     }
 }
 
@@ -674,7 +724,8 @@ int mdt_batch_get_state(mdt_batch *b, double *r, double *v, double *f, double *a
             const int N = b->N[t], P = b->P[t];
             const long long o = b->soff[t] / 3;
             for (int i = 0; i < N; ++i)
-                for (int bb = 0; bb < P; ++bb) dens[o + bb + (long long)P * i] = d[o + (long long)bb * N + i];
+                for (int bb = 0; bb < P; ++bb)
+                    dens[o + bb + (long long)P * i] = d[o + (long long)dev_bead(b, t, i, bb) * N + i];
         }
     }
     return 0;
@@ -682,6 +733,25 @@ int mdt_batch_get_state(mdt_batch *b, double *r, double *v, double *f, double *a
 
 static int push_pc(mdt_batch *b) {
     HIPCHK(hipMemcpy(b->d_pc, &b->pc, sizeof(PropConst), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// mixed bead mode: every trajectory with beads has one projectile, atom 0;
+// its other atoms are of non-projectile types
+static int check_mixed(mdt_batch *b) {
+    if (b->pes == 1) return fail(MDT_EUNSUP, "mixed bead mode runs with the EMT potential only");
+    if (!b->have_prop) return fail(MDT_ESTATE, "mixed bead mode: projectile types unknown (call mdt_batch_set_propagation)");
+    for (int t = 0; t < b->ntraj; ++t) {
+        if (b->P[t] == 1) continue;
+        const int N = b->N[t];
+        if (N < 2 || N > 2048) return fail(MDT_EUNSUP, "mixed bead mode: trajectory %d has %d atoms", t, N);
+        const uint8_t *ty = b->h_type.data() + b->aoff[t];
+        if (!b->pc.is_proj[ty[0]])
+            return fail(MDT_EINVAL, "mixed bead mode: atom 1 of trajectory %d is not of a projectile type", t);
+        for (int i = 1; i < N; ++i)
+            if (b->pc.is_proj[ty[i]])
+                return fail(MDT_EUNSUP, "mixed bead mode: trajectory %d has a second projectile atom (%d)", t, i + 1);
+    }
     return 0;
 }
 
@@ -698,6 +768,11 @@ static int ensure_ready(mdt_batch *b, bool need_prop) {
         HIPCHK(hipMemcpy(b->d_tsurf, z.data(), sizeof(double) * b->ntraj, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(b->d_tproj, z.data(), sizeof(double) * b->ntraj, hipMemcpyHostToDevice));
     }
+    if (b->bead_mode == 1 && b->have_atoms) {
+        int rc = check_mixed(b);
+        if (rc) return rc;
+    }
+    b->pc.mixed = b->bead_mode == 1 ? 1 : 0;
     return push_pc(b);
 }
 
@@ -754,6 +829,13 @@ static int lds_limit(mdt_batch *b) {
 
 // the reported choice: the threads per atom of the class with most replicas
 static void note_tpa(mdt_batch *b) {
+    if (b->cls.empty()) {  // mixed bead mode, every trajectory on k_mixed
+        b->tpa = b->mix_tpa;
+        b->block = 512;
+        b->smem = b->mix_smem;
+        b->nbcap = b->mix_nbcap;
+        return;
+    }
     int best = 0;
     for (size_t k = 1; k < b->cls.size(); ++k)
         if (b->cls[k].nrep > b->cls[best].nrep) best = (int)k;
@@ -765,13 +847,59 @@ static void note_tpa(mdt_batch *b) {
 
 // tpa > 0: that many threads per atom in every class (halved where a class's
 // replicas would exceed 512 threads); tpa == 0: per-class heuristic
+// k_mixed's launch: trajectory list, the classical atoms' list buffer and
+// LDS (tpa: lanes per classical atom in the pair passes, 2, 4 or 8)
+static int configure_mixed(mdt_batch *b, int tpa, int nbcap) {
+    if (b->d_mix) (void)hipFree(b->d_mix);
+    if (b->d_mix_nbr) (void)hipFree(b->d_mix_nbr);
+    b->d_mix = nullptr;
+    b->d_mix_nbr = nullptr;
+    b->mix_traj.clear();
+    b->mix_nmax = b->mix_pmax = 0;
+    if (b->bead_mode != 1) return 0;
+    for (int t = 0; t < b->ntraj; ++t)
+        if (b->P[t] > 1) {
+            b->mix_traj.push_back(t);
+            b->mix_nmax = std::max(b->mix_nmax, b->N[t]);
+            b->mix_pmax = std::max(b->mix_pmax, b->P[t]);
+        }
+    if (b->mix_traj.empty()) return 0;
+    const int ns = b->mix_nmax - 1;
+    int cap = nbcap > 0 ? nbcap : 96;
+    if (cap > ns - 1) cap = std::max(1, ns - 1);
+    const size_t smem = mdt_mixed_smem(b->mix_nmax, b->mix_pmax);
+    if (smem + 4096 > b->lds_limit)  // 4 KiB: the kernel's static tables
+        return fail(MDT_EUNSUP, "mixed bead mode: %d atoms and %d beads need %zu bytes of LDS (limit %zu)",
+                    b->mix_nmax, b->mix_pmax, smem + 4096, b->lds_limit);
+    b->mix_stride = (long long)ns * cap;
+    const size_t nm = b->mix_traj.size();
+    int rc = dalloc(&b->d_mix, nm) | dalloc(&b->d_mix_nbr, (size_t)b->mix_stride * nm);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(b->d_mix, b->mix_traj.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
+    b->mix_tpa = (tpa == 2 || tpa == 4 || tpa == 8) ? tpa : (tpa == 1 ? 2 : 4);
+    b->mix_nbcap = cap;
+    b->mix_smem = smem;
+    return 0;
+}
+
 static int configure(mdt_batch *b, int tpa, int nbcap) {
     drop_graph(b);  // launch shapes and the neighbour buffers may change
     int rc = lds_limit(b);
     if (rc) return rc;
     for (SizeClass &c : b->cls)
         if ((rc = configure_class(b, c, tpa > 0 ? tpa : heuristic_tpa(c.nmax), nbcap))) return rc;
+    if ((rc = configure_mixed(b, tpa, nbcap))) return rc;
     note_tpa(b);
+    return 0;
+}
+
+static int launch_mixed(mdt_batch *b, const BatchView &base, int tpa, int add_fin, hipStream_t st) {
+    BatchView v = base;
+    v.rep_list = b->d_mix;
+    v.nbr_global = b->d_mix_nbr;
+    v.nbr_stride = b->mix_stride;
+    v.nbcap = b->mix_nbcap;
+    HIPCHK(mdt_launch_mixed(tpa, (int)b->mix_traj.size(), b->mix_smem, b->ctx->d_ec, b->d_pc, v, add_fin, st));
     return 0;
 }
 
@@ -786,6 +914,7 @@ static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hip
         v.nbcap = k.nbcap;
         HIPCHK(mdt_launch_replica(k.tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, add_fin, st));
     }
+    if (!b->mix_traj.empty()) return launch_mixed(b, base, b->mix_tpa, add_fin, st);
     return 0;
 }
 
@@ -909,6 +1038,27 @@ static int autotune(mdt_batch *b) {
         if ((rc = configure_class(b, k, best, b->nbcap_req))) return rc;
         k.tuned_ms = best_ms / 3;
         total += k.tuned_ms;
+    }
+    if (!b->mix_traj.empty()) {
+        int best = b->mix_tpa;
+        float best_ms = 1e30f;
+        for (int tpa : {2, 4, 8}) {
+            if ((rc = launch_mixed(b, b->view(), tpa, 0, st))) return rc;
+            HIPCHK(hipEventRecord(b->ev0, st));
+            for (int q = 0; q < 3; ++q)
+                if ((rc = launch_mixed(b, b->view(), tpa, 0, st))) return rc;
+            HIPCHK(hipEventRecord(b->ev1, st));
+            HIPCHK(hipEventSynchronize(b->ev1));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = tpa;
+            }
+        }
+        b->mix_tpa = best;
+        b->mix_tuned_ms = best_ms / 3;
+        total += b->mix_tuned_ms;
     }
     HIPCHK(hipMemcpyAsync(b->d_f, sf, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(b->d_a, sa, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
@@ -1066,6 +1216,25 @@ int mdt_batch_set_precision(mdt_batch *b, int mode) {
     if (mode != b->precision) drop_graph(b);
     b->precision = mode;
     return 0;
+}
+
+int mdt_batch_set_bead_mode(mdt_batch *b, int mode) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (mode != MDT_BEADS_ALL && mode != MDT_BEADS_MIXED)
+        return fail(MDT_EINVAL, "bead mode %d (0: every atom carries the beads, 1: projectile beads only)", mode);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    if (mode == b->bead_mode) return 0;
+    drop_graph(b);
+    b->bead_mode = mode;
+    int rc = build_classes(b);
+    if (rc) return rc;
+    if (b->have_atoms && b->have_prop && mode == MDT_BEADS_MIXED && (rc = check_mixed(b))) {
+        b->bead_mode = MDT_BEADS_ALL;
+        build_classes(b);
+        b->tpa = 0;
+        return rc;
+    }
+    return configure(b, 0, b->nbcap_req);
 }
 
 int mdt_batch_status(mdt_batch *b, int32_t *status, int clear) {

@@ -62,7 +62,8 @@ struct PropConst {
     double com_factor_unused;
     int algo[MDT_MAXT];
     int is_proj[MDT_MAXT];
-    int any_langevin, any_pile, any_andersen, pad_;
+    int any_langevin, any_pile, any_andersen;
+    int mixed;  // 1: atoms of non-projectile types are classical (one bead, kept in bead 0)
     uint32_t seed;
     uint32_t pad2_;
 };

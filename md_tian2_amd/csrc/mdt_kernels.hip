@@ -210,6 +210,38 @@ __device__ __forceinline__ void fx_add(unsigned long long *acc, double x, double
     atomicAdd(acc, (unsigned long long)(long long)rint(y));
 }
 
+// s, embedded density, E_coh + vref and G = dE/dsigma of an atom of type tk
+// with total sigma (src/pes_emt_mod.f90:793-869 and :871-972)
+__device__ __forceinline__ void emt_atom(const LdsTables &T, int tk, double sig, double *Gk, double *Ek, double *dn) {
+    const double clamp30 = (double)1e-30f;  // pes_emt_mod.f90:793-794
+    const double sc = sig > clamp30 ? sig : clamp30;
+    const double sl = sc > MDT_TOLERANCE ? sc : MDT_TOLERANCE;
+    const double sv = -log(sl * MDT_TWELFTH) / T.betaeta2[tk];
+    *dn = T.n0[tk] * exp(T.densc[tk] * sv);
+    const double el = exp(-T.lambda[tk] * sv);
+    const double ek = exp(-T.kappa[tk] * sv);
+    *Ek = T.e0[tk] * ((1 + T.lambda[tk] * sv) * el - 1) + 6 * T.v0[tk] * ek;
+    const double D = T.e0[tk] * T.lambda[tk] * T.lambda[tk] * (sv * el) + 6 * T.v0[tk] * T.kappa[tk] * ek;
+    *Gk = -D / (T.betaeta2[tk] * sc);
+}
+
+// Image along one axis of a pair whose folded fractional difference x lies
+// within 1e-4 of +-1/2 -- in a cell of even width, atoms of perfect lattice
+// layers sit exactly half a cell apart.  The reference's minimum image
+// (src/universe_mod.f90:411-428) rounds s = isimbox (r_j - r_i) of the
+// unfolded positions half away from zero (anint), so the tie goes to the
+// image that expression, evaluated in its order without fused
+// multiply-adds, picks; returns m with x - m equal to its reduced
+// coordinate (s - anint(s) is odd in r_j - r_i, so the pair's order does
+// not matter).  (ia, ib, ic) = isimbox(axis, 1..3); ra, rb the x, y, z of
+// the two atoms.
+__device__ __forceinline__ double tie_image(double ia, double ib, double ic, double ra0, double ra1, double ra2, double rb0,
+                                         double rb1, double rb2, double x) {
+#pragma clang fp contract(off)
+    const double s = ia * (ra0 - rb0) + ib * (ra1 - rb1) + ic * (ra2 - rb2);
+    return rint(x - (s - round(s)));
+}
+
 // Pair ownership: each unordered pair is evaluated once, by its owner; the
 // owner alternates with index parity, which gives every atom about half of
 // its neighbours.  The 2^48 fixed point of the partner sigma sums bounds the
@@ -327,9 +359,16 @@ __global__ void __launch_bounds__(512, 4)
                 if (T.pair_on[tk * MDT_MAXT + tn] && owns(k, n)) {
                     float d0 = fxk - fx32[n], d1 = fyk - fy32[n], d2 = fzk - fz32[n];
                     float m0 = rintf(d0), m1 = rintf(d1), m2 = rintf(d2);
-                    if (fabsf(fabsf(d0) - 0.5f) < 1e-4f) m0 = (float)rint(sx[k] - sx[n]);
-                    if (fabsf(fabsf(d1) - 0.5f) < 1e-4f) m1 = (float)rint(sy[k] - sy[n]);
-                    if (fabsf(fabsf(d2) - 0.5f) < 1e-4f) m2 = (float)rint(sz[k] - sz[n]);
+                    const bool t0 = fabsf(fabsf(d0) - 0.5f) < 1e-4f, t1 = fabsf(fabsf(d1) - 0.5f) < 1e-4f,
+                               t2 = fabsf(fabsf(d2) - 0.5f) < 1e-4f;
+                    if (t0 | t1 | t2) {  // half-cell tie: the reference's image, from the unfolded positions
+                        const double *xr = bv.x + roff;
+                        const double a0 = xr[k], a1 = xr[N + k], a2 = xr[2 * N + k];
+                        const double b0 = xr[n], b1 = xr[N + n], b2 = xr[2 * N + n];
+                        if (t0) m0 = (float)tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, sx[k] - sx[n]);
+                        if (t1) m1 = (float)tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, sy[k] - sy[n]);
+                        if (t2) m2 = (float)tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, sz[k] - sz[n]);
+                    }
                     d0 -= m0;
                     d1 -= m1;
                     d2 -= m2;
@@ -412,17 +451,8 @@ __global__ void __launch_bounds__(512, 4)
         const int tk = typ[k];
         double Gk = 0.0, Ek = 0.0;
         if (T.owner[tk]) {
-            const double sig = G[k] + sig_value(shi[k], smid[k], slo[k]);
-            const double clamp30 = (double)1e-30f;  // pes_emt_mod.f90:793-794
-            const double sc = sig > clamp30 ? sig : clamp30;
-            const double sl = sc > MDT_TOLERANCE ? sc : MDT_TOLERANCE;
-            const double sv = -log(sl * MDT_TWELFTH) / T.betaeta2[tk];
-            const double dn = T.n0[tk] * exp(T.densc[tk] * sv);
-            const double el = exp(-T.lambda[tk] * sv);
-            const double ek = exp(-T.kappa[tk] * sv);
-            Ek = T.e0[tk] * ((1 + T.lambda[tk] * sv) * el - 1) + 6 * T.v0[tk] * ek;
-            const double D = T.e0[tk] * T.lambda[tk] * T.lambda[tk] * (sv * el) + 6 * T.v0[tk] * T.kappa[tk] * ek;
-            Gk = -D / (T.betaeta2[tk] * sc);
+            double dn;
+            emt_atom(T, tk, G[k] + sig_value(shi[k], smid[k], slo[k]), &Gk, &Ek, &dn);
             if (!JUDGE) bv.dens[doff + k] = dn;  // raw EMT density (pes_emt_mod.f90:855,861)
         }
         G[k] = Gk;
@@ -597,6 +627,11 @@ __global__ void __launch_bounds__(512, 4)
 // (trajectories with P == 1 only get f = 0 from bit2, as calc_force zeroes f).
 // step_base (graph replays): when non-null, *step_base is added to both step
 // numbers, which are then offsets within the captured chunk.
+// Mixed bead mode (pc.mixed): atoms of non-projectile types are classical --
+// one bead, kept in bead 0 (the other bead slots are not read): their
+// Langevin / Andersen temperature is kB T / m, PILE acts on their single
+// mode, positions advance in propagate_1 and the ring-polymer step skips
+// them; in the COM sum a classical atom counts as P identical beads.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg, BatchView bv,
                                               const double *cjk_all, const long long *cjk_off,
@@ -633,12 +668,15 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
 #define XI(c, b, i) (soff + ((long long)(b) * 3 + (c)) * N + (i))
 #define PI_(b, i) (poff + (long long)(b) * N + (i))
     __syncthreads();
+    // beads of atom type ty: P, or 1 for a classical atom in mixed mode
+#define PT(ty) ((pc.mixed && !s_proj[ty]) ? 1 : P)
 
     if (flags & 1) {
         // ldfa for Langevin atoms, per bead (md_algo.f90:97)
         for (int q = lane; q < N * P; q += nthr) {
             const int i = q % N, b = q / N;
             const int ty = bv.type[aoff + i];
+            if (b >= PT(ty)) continue;
             if (s_algo[ty] == MDT_PROP_LANGEVIN) {
                 int bad = 0;
                 bv.dens[PI_(b, i)] = ldfa(bv.dens[PI_(b, i)], bv.mass[aoff + i], &bad);
@@ -648,7 +686,8 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
         __syncthreads();
         for (int i = lane; i < N; i += nthr) {
             int all = 1;
-            for (int b = 0; b < P; ++b)
+            const int pb = PT(bv.type[aoff + i]);
+            for (int b = 0; b < pb; ++b)
                 if (!lan_big(bv.dens[PI_(b, i)], dt)) all = 0;
             big[i] = (uint8_t)all;
         }
@@ -656,10 +695,11 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
         for (int q = lane; q < N * P; q += nthr) {
             const int i = q % N, b = q / N;
             const int ty = bv.type[aoff + i];
+            if (b >= PT(ty)) continue;
             const uint8_t fb = bv.fixed[PI_(b, i)];
             const double mass = bv.mass[aoff + i];
             if (s_algo[ty] == MDT_PROP_LANGEVIN) {
-                const double temp = MDT_KB * P * (s_proj[ty] ? tproj : tsurf) / mass;
+                const double temp = MDT_KB * PT(ty) * (s_proj[ty] ? tproj : tsurf) / mass;
                 const bool exact = big[i] && temp > MDT_TOLERANCE;
                 const LanCoef L = lan_coef(bv.dens[PI_(b, i)], dt, temp, exact);
                 const double w = L.sv * sqrt(1 - L.crv * L.crv);
@@ -683,7 +723,8 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
         // the branch test of langevin_1 over all beads (md_algo.f90:176)
         for (int i = lane; i < N; i += nthr) {
             int all = 1;
-            for (int b = 0; b < P; ++b)
+            const int pb = PT(bv.type[aoff + i]);
+            for (int b = 0; b < pb; ++b)
                 if (!lan_big(bv.dens[PI_(b, i)], dt)) all = 0;
             big[i] = (uint8_t)all;
         }
@@ -695,14 +736,18 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
                 const int ty = bv.type[aoff + i];
                 if (s_algo[ty] != MDT_PROP_PILE) continue;
                 const double mass = bv.mass[aoff + i];
-                const double betaN = 1.0 / (MDT_KB * (s_proj[ty] ? tproj : tsurf) * P);
+                const int pb = PT(ty);
+                const double betaN = 1.0 / (MDT_KB * (s_proj[ty] ? tproj : tsurf) * pb);
                 double np[64];
-                for (int k = 0; k < P; ++k) {
+                for (int k = 0; k < pb; ++k) {
                     double acc = 0.0;
-                    for (int j = 0; j < P; ++j) acc = acc + (mass * bv.v[XI(c, j, i)]) * C[j + P * k];
+                    if (pb == 1)
+                        acc = mass * bv.v[XI(c, 0, i)];
+                    else
+                        for (int j = 0; j < P; ++j) acc = acc + (mass * bv.v[XI(c, j, i)]) * C[j + P * k];
                     np[k] = acc;
                 }
-                for (int k = 0; k < P; ++k) {
+                for (int k = 0; k < pb; ++k) {
                     double g;
                     if (k == 0) {
                         g = 1 / pc.pile_tau;
@@ -716,11 +761,15 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
                     const double z = mdt_normal(pc.seed, tid, step, MDT_STREAM_PILE, (uint32_t)(i * P + k), (uint32_t)c);
                     np[k] = c1 * np[k] + sqrt(mass / betaN) * c2 * z;
                 }
-                for (int b = 0; b < P; ++b) {
+                for (int b = 0; b < pb; ++b) {
                     const uint8_t fb = bv.fixed[PI_(b, i)];
                     double ap = 0.0;
-                    if (!(fb >> c & 1))
-                        for (int k = 0; k < P; ++k) ap = ap + np[k] * C[b + P * k];
+                    if (!(fb >> c & 1)) {
+                        if (pb == 1)
+                            ap = np[0];
+                        else
+                            for (int k = 0; k < P; ++k) ap = ap + np[k] * C[b + P * k];
+                    }
                     bv.v[XI(c, b, i)] = ap / mass;
                 }
             }
@@ -729,11 +778,13 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
         for (int q = lane; q < N * P; q += nthr) {
             const int i = q % N, b = q / N;
             const int ty = bv.type[aoff + i];
+            const int pb = PT(ty);
+            if (b >= pb) continue;
             const int algo = s_algo[ty];
             const uint8_t fb = bv.fixed[PI_(b, i)];
             const double mass = bv.mass[aoff + i];
             if (algo == MDT_PROP_LANGEVIN) {
-                const double temp = MDT_KB * P * (s_proj[ty] ? tproj : tsurf) / mass;
+                const double temp = MDT_KB * pb * (s_proj[ty] ? tproj : tsurf) / mass;
                 const bool exact = big[i] && temp > MDT_TOLERANCE;
                 const LanCoef L = lan_coef(bv.dens[PI_(b, i)], dt, temp, exact);
                 for (int c = 0; c < 3; ++c) {
@@ -752,10 +803,10 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
                     const double v = bv.v[XI(c, b, i)];
                     const double vn = (fb >> c & 1) ? 0.0 : v + 0.5 * dt * bv.a[XI(c, b, i)];
                     bv.v[XI(c, b, i)] = vn;
-                    if (P == 1 && !(fb >> c & 1)) bv.x[XI(c, b, i)] += dt * vn;
+                    if (pb == 1 && !(fb >> c & 1)) bv.x[XI(c, b, i)] += dt * vn;
                 }
                 if (algo == MDT_PROP_ANDERSEN) {
-                    const double ibetaN = P * MDT_KB * (s_proj[ty] ? tproj : tsurf);
+                    const double ibetaN = pb * MDT_KB * (s_proj[ty] ? tproj : tsurf);
                     const double sigma = sqrt(ibetaN / mass);
                     const double u = mdt_uniform(pc.seed, tid, step, MDT_STREAM_ANDERSEN_U, (uint32_t)(i * P + b), 0u);
                     if (u < dt / pc.andersen_time)
@@ -777,10 +828,16 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
                 for (int i = lane; i < N; i += 64) {
                     const double m = bv.mass[aoff + i];
                     double bx = 0, by = 0, bz = 0;
-                    for (int b = 0; b < P; ++b) {
+                    const int pb = PT(bv.type[aoff + i]);
+                    for (int b = 0; b < pb; ++b) {
                         bx += bv.v[XI(0, b, i)];
                         by += bv.v[XI(1, b, i)];
                         bz += bv.v[XI(2, b, i)];
+                    }
+                    if (pb == 1 && P > 1) {  // a classical atom stands for P identical beads
+                        bx *= P;
+                        by *= P;
+                        bz *= P;
                     }
                     ax += bx * m;
                     ay += by * m;
@@ -804,6 +861,7 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
             const double fac = (1.0 - pow(1.0 - 1.0 / N, 1000.0)) / red[3] / P;
             for (int q = lane; q < N * P; q += nthr) {
                 const int i = q % N, b = q / N;
+                if (b >= PT(bv.type[aoff + i])) continue;
                 bv.v[XI(0, b, i)] -= red[0] * fac;
                 bv.v[XI(1, b, i)] -= red[1] * fac;
                 bv.v[XI(2, b, i)] -= red[2] * fac;
@@ -825,6 +883,10 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
                 const int i = q / 3, c = q % 3;
                 const int ty = bv.type[aoff + i];
                 const double mass = bv.mass[aoff + i];
+                if (PT(ty) == 1) {  // classical atom (mixed mode): no ring polymer, f = 0
+                    bv.f[XI(c, 0, i)] = 0.0;
+                    continue;
+                }
                 if (s_algo[ty] == MDT_PROP_LANGEVIN) {
                     // reverted step; spring force with m(1) and Tproj (rpmd.f90:418-435)
                     for (int b = 0; b < P; ++b) {
@@ -879,6 +941,463 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
     }
 #undef XI
 #undef PI_
+#undef PT
+    if (bad_local) atomicOr(&bv.status[t], bad_local);
+}
+
+// ---------------------------------------------------------------------------
+// k_mixed: calc_force of a mixed-bead trajectory (one workgroup, 512 lanes):
+// atom 0 (the projectile) has P beads, atoms 1..N-1 are classical and kept
+// in bead 0.  The potential of bead b is the EMT energy of the classical
+// atoms with projectile bead b; the classical atoms feel the bead average
+// of their forces (the force on a rigid P-bead copy divided by P).  Because
+// the classical-classical pair geometry does not depend on the bead, the
+// averaged force needs the pair passes once, with G replaced by its bead
+// average, plus the projectile pairs of every bead:
+//   1. owned half lists of the classical atoms (HBM), sigma^0 from their
+//      pairs (pass A), per atom G^0, E^0, dens^0;
+//   2. per bead b, 16 lanes: the classical atoms within the cut-off of the
+//      projectile's bead b; its sigma, G_H(b), E_H(b); then per listed atom
+//      a: sigma_a(b) = sigma_a^0 + its projectile term, G_a(b), E_a(b), the
+//      pair force (+F to bead b, -F/P to atom a) and pair energy, and the
+//      bead averages of G_a and dens_a (fixed point);
+//   3. the classical pairs (pass B) with the averaged G.
+// epot(b) = sum_a (E_a^0 - V pairs / 2) + sum_{a near b} (E_a(b) - E_a^0)
+//           + E_H(b) - V_H(b) / 2.
+// ---------------------------------------------------------------------------
+#define MIX_CAPQ 192  // classical atoms within the cut-off of one projectile bead
+#define MIX_G 16      // lanes per bead
+#define MIX_BLOCK 512
+
+template <int TPA>
+__global__ void __launch_bounds__(512, 2)
+    k_mixed(const EmtConst *__restrict__ ecg, const PropConst *__restrict__ pcg, BatchView bv, int add_fin) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ LdsTables T;
+    __shared__ double red[1];
+    const int t = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;  // trajectory list
+    const int N = bv.traj_N[t], P = bv.traj_P[t];
+    const int Ns = N - 1;
+    const int cap = bv.nbcap < Ns - 1 ? bv.nbcap : (Ns > 1 ? Ns - 1 : 1);
+    const long long soff = bv.traj_soff[t];
+    const long long poff = soff / 3;
+    const long long aoff = bv.traj_aoff[t];
+    const int nthr = blockDim.x, lane = threadIdx.x;
+    CellRegs cell;
+    load_cell(bv, t, cell);
+    load_tables(T, ecg, pcg, cell, lane, nthr);
+
+    // LDS: px py pz | A0 A1 A2 | G0 E0 D0 S0 | Gacc Dacc (classical atoms,
+    // local index a = atom - 1) | per bead hx hy hz Eb | per
+    // bead group GH EH DH | cnt | group hcnt, hlist | typ (all atoms).  The
+    // words A0-A2 hold the fractional coordinates in the scan, the sigma
+    // fixed point in pass A and the force fixed point afterwards.
+    const int ngrp = MIX_BLOCK / MIX_G;
+    double *px = (double *)smem_raw, *py = px + Ns, *pz = py + Ns;
+    double *sx = pz + Ns, *sy = sx + Ns, *sz = sy + Ns;
+    unsigned long long *shi = (unsigned long long *)sx, *smid = (unsigned long long *)sy,
+                       *slo = (unsigned long long *)sz;
+    unsigned long long *fxa = shi, *fya = smid, *fza = slo;
+    double *G0 = sz + Ns, *E0 = G0 + Ns, *D0 = E0 + Ns, *S0 = D0 + Ns;
+    unsigned long long *Gacc = (unsigned long long *)(S0 + Ns), *Dacc = Gacc + Ns;
+    double *hx = (double *)(Dacc + Ns), *hy = hx + P, *hz = hy + P, *Eb = hz + P;
+    double *GH = Eb + P, *EH = GH + ngrp, *DH = EH + ngrp;
+    uint16_t *cnt = (uint16_t *)(DH + ngrp);
+    uint16_t *hcnt = cnt + Ns;
+    uint16_t *hlist = hcnt + ngrp;
+    uint8_t *typ = (uint8_t *)(hlist + (size_t)ngrp * MIX_CAPQ);
+    uint16_t *nbr = bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
+    int bad_local = 0;
+    __syncthreads();
+
+    // ---- stage: classical atoms (bead 0) and the projectile's beads, folded ----
+    for (int i = lane; i < N; i += nthr) typ[i] = (uint8_t)bv.type[aoff + i];
+    for (int a = lane; a < Ns; a += nthr) {
+        const int i = a + 1;
+        const double r[3] = {bv.x[soff + i], bv.x[soff + N + i], bv.x[soff + 2 * N + i]};
+        double f0 = cell.ih[0] * r[0] + cell.ih[3] * r[1] + cell.ih[6] * r[2];
+        double f1 = cell.ih[1] * r[0] + cell.ih[4] * r[1] + cell.ih[7] * r[2];
+        double f2 = cell.ih[2] * r[0] + cell.ih[5] * r[1] + cell.ih[8] * r[2];
+        f0 -= floor(f0);
+        f1 -= floor(f1);
+        f2 -= floor(f2);
+        sx[a] = f0;
+        sy[a] = f1;
+        sz[a] = f2;
+        px[a] = cell.h[0] * f0 + cell.h[3] * f1 + cell.h[6] * f2;
+        py[a] = cell.h[1] * f0 + cell.h[4] * f1 + cell.h[7] * f2;
+        pz[a] = cell.h[2] * f0 + cell.h[5] * f1 + cell.h[8] * f2;
+    }
+    for (int b = lane; b < P; b += nthr) {
+        const long long rb = soff + (long long)b * 3 * N;
+        const double r[3] = {bv.x[rb], bv.x[rb + N], bv.x[rb + 2 * N]};
+        double f0 = cell.ih[0] * r[0] + cell.ih[3] * r[1] + cell.ih[6] * r[2];
+        double f1 = cell.ih[1] * r[0] + cell.ih[4] * r[1] + cell.ih[7] * r[2];
+        double f2 = cell.ih[2] * r[0] + cell.ih[5] * r[1] + cell.ih[8] * r[2];
+        f0 -= floor(f0);
+        f1 -= floor(f1);
+        f2 -= floor(f2);
+        hx[b] = cell.h[0] * f0 + cell.h[3] * f1 + cell.h[6] * f2;
+        hy[b] = cell.h[1] * f0 + cell.h[4] * f1 + cell.h[7] * f2;
+        hz[b] = cell.h[2] * f0 + cell.h[5] * f1 + cell.h[8] * f2;
+    }
+    __syncthreads();
+    const double acut = ecg->acut, rcut = ecg->rcut, cutr = ecg->cutr, cut2 = ecg->cut2hi;
+    const int apb = nthr / TPA;
+    const int wl = lane & 63;
+    const unsigned long long below = (wl == 0) ? 0ull : ((~0ull) >> (64 - wl));
+    const unsigned long long gmask = (TPA >= 64 ? ~0ull : ((1ull << TPA) - 1ull)) << ((wl / TPA) * TPA);
+    const int tq = typ[0];
+
+    // ---- 1a: owned half lists of the classical atoms (FP64 scan) ----
+    for (int k0 = 0; k0 < Ns; k0 += apb) {
+        const int k = k0 + lane / TPA;
+        const int sub = lane % TPA;
+        const bool act = k < Ns;
+        const int tk = act ? typ[k + 1] : 0;
+        const double fxk = act ? sx[k] : 0.0, fyk = act ? sy[k] : 0.0, fzk = act ? sz[k] : 0.0;
+        int c = 0;
+        for (int n0 = 0; n0 < Ns; n0 += TPA) {
+            const int n = n0 + sub;
+            bool in = false;
+            int code = 0;
+            if (act && n < Ns && n != k) {
+                const int tn = typ[n + 1];
+                if (T.pair_on[tk * MDT_MAXT + tn] && owns(k, n)) {
+                    double d0 = fxk - sx[n], d1 = fyk - sy[n], d2 = fzk - sz[n];
+                    double m0 = rint(d0), m1 = rint(d1), m2 = rint(d2);
+                    const bool t0 = fabs(fabs(d0) - 0.5) < 1e-4, t1 = fabs(fabs(d1) - 0.5) < 1e-4,
+                               t2 = fabs(fabs(d2) - 0.5) < 1e-4;
+                    if (t0 | t1 | t2) {  // half-cell tie: the reference's image
+                        const double *xr = bv.x + soff;
+                        const double a0 = xr[k + 1], a1 = xr[N + k + 1], a2 = xr[2 * N + k + 1];
+                        const double b0 = xr[n + 1], b1 = xr[N + n + 1], b2 = xr[2 * N + n + 1];
+                        if (t0) m0 = tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, d0);
+                        if (t1) m1 = tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, d1);
+                        if (t2) m2 = tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, d2);
+                    }
+                    d0 -= m0;
+                    d1 -= m1;
+                    d2 -= m2;
+                    const double ex = cell.h[0] * d0 + cell.h[3] * d1 + cell.h[6] * d2;
+                    const double ey = cell.h[1] * d0 + cell.h[4] * d1 + cell.h[7] * d2;
+                    const double ez = cell.h[2] * d0 + cell.h[5] * d1 + cell.h[8] * d2;
+                    in = (ex * ex + ey * ey + ez * ez) <= cut2;
+                    code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
+                }
+            }
+            const unsigned long long m = __ballot(in) & gmask;
+            if (in) {
+                const int pos = c + __popcll(m & below);
+                if (pos < cap)
+                    nbr[(size_t)k * cap + pos] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
+                else
+                    bad_local |= MDT_ST_NBR_OVERFLOW;
+            }
+            c += __popcll(m);
+        }
+        if (act && sub == 0) cnt[k] = (uint16_t)(c < cap ? c : cap);
+    }
+    __syncthreads();
+    for (int a = lane; a < Ns; a += nthr) shi[a] = smid[a] = slo[a] = 0ull;
+    __syncthreads();
+
+    // ---- 1b: sigma^0 of the classical atoms (pass A over their pairs) ----
+    for (int k0 = 0; k0 < Ns; k0 += apb) {
+        const int k = k0 + lane / TPA;
+        const int sub = lane % TPA;
+        const bool act = k < Ns;
+        double sig = 0.0;
+        if (act) {
+            const int tk = typ[k + 1];
+            const double xk = px[k], yk = py[k], zk = pz[k];
+            const int ck = cnt[k];
+            for (int e = sub; e < ck; e += TPA) {
+                const unsigned q = nbr[(size_t)k * cap + e];
+                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+                const int tn = typ[n + 1];
+                const double dx = (xk - px[n]) - T.shift[3 * code];
+                const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+                const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double r = r2 * mdt_rsqrt(r2);
+                if (r > cutr) continue;
+                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
+                const double th = mdt_rcp(1 + e1);
+                const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+                const double fsk = (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+                sig += T.csig[tk * MDT_MAXT + tn] * fsn;
+                sig_add(&shi[n], &smid[n], &slo[n], T.csig[tn * MDT_MAXT + tk] * fsk, &bad_local);
+            }
+        }
+        sig = group_sum<TPA>(sig);
+        if (act && sub == 0) G0[k] = sig;
+    }
+    __syncthreads();
+    for (int a = lane; a < Ns; a += nthr) {
+        const int ta = typ[a + 1];
+        double g = 0.0, e = 0.0, dn = 0.0;
+        const double s0 = G0[a] + sig_value(shi[a], smid[a], slo[a]);
+        if (T.owner[ta]) emt_atom(T, ta, s0, &g, &e, &dn);
+        S0[a] = s0;
+        G0[a] = g;
+        E0[a] = e;
+        D0[a] = dn;
+        Gacc[a] = Dacc[a] = 0ull;
+    }
+    __syncthreads();
+    for (int a = lane; a < Ns; a += nthr) fxa[a] = fya[a] = fza[a] = 0ull;  // force words from here on
+    __syncthreads();
+
+    // ---- 2: the projectile's beads, MIX_G lanes each ----
+    const int grp = lane / MIX_G, gsub = lane % MIX_G;
+    const unsigned long long qmask = ((1ull << MIX_G) - 1ull) << ((wl / MIX_G) * MIX_G);
+    const double invP = 1.0 / P;
+    const int kn = tq * MDT_MAXT, kq = tq;
+    for (int b0 = 0; b0 < P; b0 += ngrp) {
+        const int b = b0 + grp;
+        const bool bact = b < P;
+        // 2a: classical atoms within the cut-off of bead b
+        {
+            int c = 0;
+            const double xq = bact ? hx[b] : 0.0, yq = bact ? hy[b] : 0.0, zq = bact ? hz[b] : 0.0;
+            for (int n0 = 0; n0 < Ns; n0 += MIX_G) {
+                const int n = n0 + gsub;
+                bool in = false;
+                int code = 0;
+                if (bact && n < Ns && T.pair_on[kn + typ[n + 1]]) {
+                    // fractional difference from the folded cartesian positions (the
+                    // fractional words hold sigma / force sums by now)
+                    const double cx = xq - px[n], cy = yq - py[n], cz = zq - pz[n];
+                    double d0 = cell.ih[0] * cx + cell.ih[3] * cy + cell.ih[6] * cz;
+                    double d1 = cell.ih[1] * cx + cell.ih[4] * cy + cell.ih[7] * cz;
+                    double d2 = cell.ih[2] * cx + cell.ih[5] * cy + cell.ih[8] * cz;
+                    double m0 = rint(d0), m1 = rint(d1), m2 = rint(d2);
+                    const bool t0 = fabs(fabs(d0) - 0.5) < 1e-4, t1 = fabs(fabs(d1) - 0.5) < 1e-4,
+                               t2 = fabs(fabs(d2) - 0.5) < 1e-4;
+                    if (t0 | t1 | t2) {  // half-cell tie: the reference's image (slab in bead 0)
+                        const double *xh = bv.x + soff + (long long)b * 3 * N, *xs = bv.x + soff;
+                        const double a0 = xh[0], a1 = xh[N], a2 = xh[2 * N];
+                        const double b0 = xs[n + 1], b1 = xs[N + n + 1], b2 = xs[2 * N + n + 1];
+                        if (t0) m0 = tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, d0);
+                        if (t1) m1 = tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, d1);
+                        if (t2) m2 = tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, d2);
+                    }
+                    d0 -= m0;
+                    d1 -= m1;
+                    d2 -= m2;
+                    const double ex = cell.h[0] * d0 + cell.h[3] * d1 + cell.h[6] * d2;
+                    const double ey = cell.h[1] * d0 + cell.h[4] * d1 + cell.h[7] * d2;
+                    const double ez = cell.h[2] * d0 + cell.h[5] * d1 + cell.h[8] * d2;
+                    in = (ex * ex + ey * ey + ez * ez) <= cut2;
+                    code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
+                }
+                const unsigned long long m = __ballot(in) & qmask;
+                if (in) {
+                    const int pos = c + __popcll(m & below);
+                    if (pos < MIX_CAPQ)
+                        hlist[(size_t)grp * MIX_CAPQ + pos] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
+                    else
+                        bad_local |= MDT_ST_NBR_OVERFLOW;
+                }
+                c += __popcll(m);
+            }
+            if (gsub == 0) hcnt[grp] = (uint16_t)(c < MIX_CAPQ ? c : MIX_CAPQ);
+        }
+        __syncthreads();
+        // 2b: sigma of the projectile's bead b
+        {
+            double sig = 0.0;
+            const int ch = bact ? hcnt[grp] : 0;
+            for (int e = gsub; e < ch; e += MIX_G) {
+                const unsigned q = hlist[(size_t)grp * MIX_CAPQ + e];
+                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+                const int tn = typ[n + 1];
+                const double dx = (hx[b] - px[n]) - T.shift[3 * code];
+                const double dy = (hy[b] - py[n]) - T.shift[3 * code + 1];
+                const double dz = (hz[b] - pz[n]) - T.shift[3 * code + 2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double r = r2 * mdt_rsqrt(r2);
+                if (r > cutr) continue;
+                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
+                const double th = mdt_rcp(1 + e1);
+                sig += T.csig[kn + tn] * (th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab));
+            }
+            sig = group_sum<MIX_G>(sig);
+            if (bact && gsub == 0) {
+                double g = 0.0, e = 0.0, dn = 0.0;
+                if (T.owner[tq]) emt_atom(T, tq, sig, &g, &e, &dn);
+                GH[grp] = g;
+                EH[grp] = e;
+                DH[grp] = dn;
+            }
+        }
+        __syncthreads();
+        // 2c: the listed atoms' sigma(b), G(b), E(b); pair forces and energies
+        {
+            double fx = 0.0, fy = 0.0, fz = 0.0, dE = 0.0, vs = 0.0;
+            const int ch = bact ? hcnt[grp] : 0;
+            const double Gq = bact ? GH[grp] : 0.0;
+            for (int e = gsub; e < ch; e += MIX_G) {
+                const unsigned q = hlist[(size_t)grp * MIX_CAPQ + e];
+                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+                const int tn = typ[n + 1];
+                const double dx = (hx[b] - px[n]) - T.shift[3 * code];
+                const double dy = (hy[b] - py[n]) - T.shift[3 * code + 1];
+                const double dz = (hz[b] - pz[n]) - T.shift[3 * code + 2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double ri = mdt_rsqrt(r2);
+                const double r = r2 * ri;
+                if (r > cutr) continue;
+                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
+                const double th = mdt_rcp(1 + e1);
+                const double rt1 = acut * e1 * th;
+                const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+                const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
+                double fsk = fsn, fvk = fvn;
+                if (tn != tq) {
+                    fsk = th * mdt_exp(-T.eta2[kq] * (r - T.bs0[kq]), T.exptab);
+                    fvk = th * mdt_exp(-T.kdb[kq] * (r - T.bs0[kq]), T.exptab);
+                }
+                const int nk = tn * MDT_MAXT + tq;
+                // the classical atom with this bead's projectile term
+                double Gn = 0.0, En = 0.0, Dn = 0.0;
+                if (T.owner[tn]) emt_atom(T, tn, S0[n] + T.csig[nk] * fsk, &Gn, &En, &Dn);
+                dE += En - E0[n];
+                fx_add(&Gacc[n], (Gn - G0[n]) * invP, MDT_F_SCALE, &bad_local);
+                fx_add(&Dacc[n], (Dn - D0[n]) * invP, MDT_F_SCALE, &bad_local);
+                const double c = Gq * T.csig[kn + tn] * ((T.etad[kn + tn] + rt1) * fsn) +
+                                 Gn * T.csig[nk] * ((T.etad[nk] + rt1) * fsk) +
+                                 0.5 * (T.wv[kn + tn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[nk] * ((T.kdb[kq] + rt1) * fvk));
+                vs += T.wv[kn + tn] * fvn + T.wv[nk] * fvk;
+                const double cr = c * ri;
+                fx -= cr * dx;
+                fy -= cr * dy;
+                fz -= cr * dz;
+                fx_add(&fxa[n], cr * dx * invP, MDT_F_SCALE, &bad_local);
+                fx_add(&fya[n], cr * dy * invP, MDT_F_SCALE, &bad_local);
+                fx_add(&fza[n], cr * dz * invP, MDT_F_SCALE, &bad_local);
+            }
+            fx = group_sum<MIX_G>(fx);
+            fy = group_sum<MIX_G>(fy);
+            fz = group_sum<MIX_G>(fz);
+            dE = group_sum<MIX_G>(dE);
+            vs = group_sum<MIX_G>(vs);
+            if (bact && gsub == 0) {
+                const long long rb = soff + (long long)b * 3 * N;
+                double F[3] = {T.owner[tq] ? fx : 0.0, T.owner[tq] ? fy : 0.0, T.owner[tq] ? fz : 0.0};
+                if (add_fin) {
+                    F[0] += bv.f[rb];
+                    F[1] += bv.f[rb + N];
+                    F[2] += bv.f[rb + 2 * N];
+                }
+                const uint8_t fb = bv.fixed[poff + (long long)b * N];
+                const double mass = bv.mass[aoff];
+                bv.f[rb] = F[0];
+                bv.f[rb + N] = F[1];
+                bv.f[rb + 2 * N] = F[2];
+                bv.a[rb] = (fb & 1) ? 0.0 : F[0] / mass;
+                bv.a[rb + N] = (fb & 2) ? 0.0 : F[1] / mass;
+                bv.a[rb + 2 * N] = (fb & 4) ? 0.0 : F[2] / mass;
+                bv.dens[poff + (long long)b * N] = DH[grp];
+                Eb[b] = dE + EH[grp] - 0.5 * vs;  // bead part of the energy
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- 3: classical pairs with the bead-averaged G (pass B) ----
+    for (int a = lane; a < Ns; a += nthr)
+        G0[a] += (double)(long long)Gacc[a] * MDT_F_ISCALE;  // G0 now holds the bead average
+    __syncthreads();
+    for (int k0 = 0; k0 < Ns; k0 += apb) {
+        const int k = k0 + lane / TPA;
+        const int sub = lane % TPA;
+        const bool act = k < Ns;
+        double fx = 0.0, fy = 0.0, fz = 0.0, ep = 0.0;
+        if (act) {
+            const int tk = typ[k + 1];
+            const double xk = px[k], yk = py[k], zk = pz[k];
+            const double Gk = G0[k];
+            const int ck = cnt[k];
+            for (int e = sub; e < ck; e += TPA) {
+                const unsigned q = nbr[(size_t)k * cap + e];
+                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+                const int tn = typ[n + 1];
+                const double dx = (xk - px[n]) - T.shift[3 * code];
+                const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+                const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
+                const double r2 = dx * dx + dy * dy + dz * dz;
+                const double ri = mdt_rsqrt(r2);
+                const double r = r2 * ri;
+                if (r > cutr) continue;
+                const int ikn = tk * MDT_MAXT + tn, ink = tn * MDT_MAXT + tk;
+                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
+                const double th = mdt_rcp(1 + e1);
+                const double rt1 = acut * e1 * th;
+                const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+                const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
+                double fsk = fsn, fvk = fvn;
+                if (tn != tk) {
+                    fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+                    fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
+                }
+                const double c = Gk * T.csig[ikn] * ((T.etad[ikn] + rt1) * fsn) +
+                                 G0[n] * T.csig[ink] * ((T.etad[ink] + rt1) * fsk) +
+                                 0.5 * (T.wv[ikn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[ink] * ((T.kdb[tk] + rt1) * fvk));
+                ep += T.wv[ikn] * fvn + T.wv[ink] * fvk;
+                const double cr = c * ri;
+                fx -= cr * dx;
+                fy -= cr * dy;
+                fz -= cr * dz;
+                fx_add(&fxa[n], cr * dx, MDT_F_SCALE, &bad_local);
+                fx_add(&fya[n], cr * dy, MDT_F_SCALE, &bad_local);
+                fx_add(&fza[n], cr * dz, MDT_F_SCALE, &bad_local);
+            }
+        }
+        fx = group_sum<TPA>(fx);
+        fy = group_sum<TPA>(fy);
+        fz = group_sum<TPA>(fz);
+        ep = group_sum<TPA>(ep);
+        if (act && sub == 0) {
+            fx_add(&fxa[k], fx, MDT_F_SCALE, &bad_local);
+            fx_add(&fya[k], fy, MDT_F_SCALE, &bad_local);
+            fx_add(&fza[k], fz, MDT_F_SCALE, &bad_local);
+            E0[k] -= 0.5 * ep;
+        }
+    }
+    __syncthreads();
+
+    // ---- outputs: classical atoms (bead 0), bead energies ----
+    for (int a = lane; a < Ns; a += nthr) {
+        const int i = a + 1;
+        const int ta = typ[i];
+        double F[3] = {(double)(long long)fxa[a] * MDT_F_ISCALE, (double)(long long)fya[a] * MDT_F_ISCALE,
+                       (double)(long long)fza[a] * MDT_F_ISCALE};
+        if (!T.owner[ta]) F[0] = F[1] = F[2] = 0.0;
+        if (add_fin) {
+            F[0] += bv.f[soff + i];
+            F[1] += bv.f[soff + N + i];
+            F[2] += bv.f[soff + 2 * N + i];
+        }
+        const uint8_t fb = bv.fixed[poff + i];
+        const double mass = bv.mass[aoff + i];
+        bv.f[soff + i] = F[0];
+        bv.f[soff + N + i] = F[1];
+        bv.f[soff + 2 * N + i] = F[2];
+        bv.a[soff + i] = (fb & 1) ? 0.0 : F[0] / mass;
+        bv.a[soff + N + i] = (fb & 2) ? 0.0 : F[1] / mass;
+        bv.a[soff + 2 * N + i] = (fb & 4) ? 0.0 : F[2] / mass;
+        if (T.owner[ta]) bv.dens[poff + i] = D0[a] + (double)(long long)Dacc[a] * MDT_F_ISCALE;
+    }
+    if (lane < 64) {
+        double e = 0.0;
+        for (int a = lane; a < Ns; a += 64) e += E0[a];
+        for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+        if (lane == 0) red[0] = e;
+    }
+    __syncthreads();
+    for (int b = lane; b < P; b += nthr) bv.epot[bv.traj_eoff[t] + b] = red[0] + Eb[b];
     if (bad_local) atomicOr(&bv.status[t], bad_local);
 }
 
@@ -919,6 +1438,31 @@ hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const E
                    : launch_replica_t<false, true, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
     return lds ? launch_replica_t<true, false, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
                : launch_replica_t<false, false, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+}
+
+// LDS of k_mixed for trajectories of up to nmax atoms (one projectile) and
+// pmax beads
+size_t mdt_mixed_smem(int nmax, int pmax) {
+    const size_t ns = (size_t)(nmax - 1), ngrp = MIX_BLOCK / MIX_G;
+    return sizeof(double) * (12 * ns + 4 * (size_t)pmax + 3 * ngrp) +
+           sizeof(uint16_t) * (ns + ngrp + ngrp * MIX_CAPQ) + (size_t)nmax + 16;
+}
+
+int mdt_mixed_block(void) { return MIX_BLOCK; }
+
+// calc_force of the mixed-bead trajectories listed in bv.rep_list (ntraj of
+// them); the classical atoms' owned half lists go to bv.nbr_global +
+// list position x bv.nbr_stride
+hipError_t mdt_launch_mixed(int tpa, int ntraj, size_t smem, const EmtConst *ec, const PropConst *pc,
+                            const BatchView &bv, int add_fin, hipStream_t st) {
+    dim3 g(ntraj), blk(MIX_BLOCK);
+    switch (tpa) {
+    case 2: hipLaunchKernelGGL((k_mixed<2>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 4: hipLaunchKernelGGL((k_mixed<4>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    case 8: hipLaunchKernelGGL((k_mixed<8>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,

@@ -28,7 +28,7 @@ SYMBOLS = (
     "mdt_batch_set_atoms", "mdt_batch_set_cells", "mdt_batch_set_propagation", "mdt_batch_seed",
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
-    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph",
+    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph", "mdt_batch_set_bead_mode",
     "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats",
     "mdt_batch_nn_descriptors", "mdt_nn_train_create", "mdt_nn_train_destroy", "mdt_nn_train_descriptor_stats",
     "mdt_nn_train_epoch", "mdt_nn_train_eval", "mdt_nn_train_loss_grad", "mdt_nn_get_weights", "mdt_nn_load",
@@ -69,6 +69,7 @@ def lib():
         L.mdt_batch_profile.argtypes = [vp, C.c_int]
         L.mdt_batch_force_timing.argtypes = [vp, dp]
         L.mdt_batch_set_precision.argtypes = [vp, C.c_int]
+        L.mdt_batch_set_bead_mode.argtypes = [vp, C.c_int]
         L.mdt_batch_set_graph.argtypes = [vp, C.c_int]
         L.mdt_nn_params_set.argtypes = [vp, C.c_int, ip, C.c_int, ip, ip, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp,
                                         C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -258,6 +259,11 @@ class Batch:
     def set_precision(self, mode):
         """0: FP64 pair terms (reference arithmetic); 1: FP32 pair terms with FP64 accumulation."""
         _check(lib().mdt_batch_set_precision(self.h, int(mode)))
+
+    def set_bead_mode(self, mode):
+        """0: every atom carries the beads; 1 (mixed): the projectile (atom 0)
+        carries them and the other atoms are classical, kept in bead 0."""
+        _check(lib().mdt_batch_set_bead_mode(self.h, int(mode)))
 
     def set_pes(self, pes, judge_interval=0):
         """0: EMT; 1: the network (EMT fallback on extrapolation, EMT judge every judge_interval steps)."""

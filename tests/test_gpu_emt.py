@@ -41,6 +41,33 @@ def test_hau_6x6x4_thermal(gpu_ctx, pes_hau):
     _cmp(gpu_ctx, s, pes_hau)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("metal,n", [("Pt", 4), ("Au", 4), ("Pt", 2)])
+def test_even_cell_half_cell_ties(gpu_ctx, pes_hau, pes_hpt, metal, n, precision):
+    """Perfect fixed layers in a cell of even width put atom pairs exactly
+    half a cell apart; within the cut-off (4 x 4 and 2 x 2 cells) the pair's
+    image is the one the reference's anint picks (universe_mod.f90:411-428)."""
+    pes = pes_hpt if metal == "Pt" else pes_hau
+    rng = np.random.default_rng(40 + n)
+    s = geometry.h_on_slab(metal, n=n, nlayers=4, rng=rng, z0=1.7)
+    mob = ~s.fixed[:, 0, :]
+    s.r[:, 0, :] += rng.normal(0, 0.05, s.r[:, 0, :].shape) * mob
+    cols, pesm = pesfile.emt_tables(pes, s.type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    b = mdt.Batch(gpu_ctx, [s], dt=0.1)
+    b.set_precision(precision)
+    b.compute_forces()
+    o = b.get_state()[0]
+    b.close()
+    e_r, f_r, _ = oracle.emt_energy_forces(s, cols, pesm)
+    if precision == 0:
+        assert np.abs(o["epot"] - e_r).max() <= E_TOL * max(1.0, np.abs(e_r).max())
+        assert np.abs(o["f"] - f_r).max() <= F_TOL, np.abs(o["f"] - f_r).max()
+    else:  # FP32 pair terms: the H/Pt bounds of the precision tests (a wrong image is ~1e-2)
+        assert np.abs(o["epot"] - e_r).max() / s.natoms <= 2e-5
+        assert np.abs(o["f"] - f_r).max() <= 2e-3, np.abs(o["f"] - f_r).max()
+
+
 def test_hpt_8x8x6(gpu_ctx, pes_hpt):
     s = geometry.h_on_slab("Pt", n=8, nlayers=6, rng=np.random.default_rng(8))
     s.r[0, 0, 2] = 1.1
