@@ -969,11 +969,17 @@ static int launch_judge(mdt_batch *b, hipStream_t st) {
     return launch_replicas(b, jb, 0, st);
 }
 
+// LDS of k_traj: the P x P normal-mode matrix, three atom lists (uint16)
+// and one flag byte per atom
+static size_t traj_smem(const mdt_batch *b) {
+    return sizeof(double) * b->maxP * b->maxP + 7 * (size_t)b->maxN + 16;
+}
+
 static int launch_force(mdt_batch *b) {
     mdt_context *c = b->ctx;
     const BatchView bv = b->view();
     if (!b->allP1) {
-        const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
+        const size_t ts = traj_smem(b);
         HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, nullptr, b->step, b->step, 4,
                                c->stream));
     }
@@ -1103,7 +1109,7 @@ int mdt_batch_compute_forces(mdt_batch *b) {
 static int enqueue_steps(mdt_batch *b, int nsteps, const uint32_t *step_base, uint32_t first, bool events) {
     mdt_context *c = b->ctx;
     const BatchView bv = b->view();
-    const size_t ts = sizeof(double) * b->maxP * b->maxP + b->maxN + 16;
+    const size_t ts = traj_smem(b);
     const int rp = b->allP1 ? 0 : 4;
     for (int s = 0; s < nsteps; ++s) {
         const uint32_t prev = first + s, st = first + s + 1;

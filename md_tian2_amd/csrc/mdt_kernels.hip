@@ -657,26 +657,58 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
         step_prev += s0;
         step += s0;
     }
-    double *C = (double *)smem_raw;         // P*P
-    uint8_t *big = (uint8_t *)(C + P * P);  // N flags
+    // LDS: C (P x P) | ring-polymer atom lists: PILE, transformed (not
+    // Langevin), Langevin (spring force) | big flags
+    double *C = (double *)smem_raw;
+    uint16_t *l_pile = (uint16_t *)(C + P * P), *l_tr = l_pile + N, *l_lan = l_tr + N;
+    uint8_t *big = (uint8_t *)(l_lan + N);
+    __shared__ int s_cnt[3];
     for (int k = lane; k < P * P; k += nthr) C[k] = cjk[k];
     for (int k = lane; k < MDT_MAXT; k += nthr) {
         s_algo[k] = pc.algo[k];
         s_proj[k] = pc.is_proj[k];
     }
+    if (lane < 3) s_cnt[lane] = 0;
     int bad_local = 0;
 #define XI(c, b, i) (soff + ((long long)(b) * 3 + (c)) * N + (i))
 #define PI_(b, i) (poff + (long long)(b) * N + (i))
     __syncthreads();
     // beads of atom type ty: P, or 1 for a classical atom in mixed mode
 #define PT(ty) ((pc.mixed && !s_proj[ty]) ? 1 : P)
+    // atom-bead slots in use: every atom's P beads, or in the mixed model
+    // (atom 0 the only ring polymer) P beads of atom 0 and bead 0 of the rest
+    const bool mix = pc.mixed && P > 1;
+    const int nab = mix ? P + N - 1 : N * P;
+#define AB(q, i, b)                       \
+    if (mix) {                            \
+        i = (q) < P ? 0 : (q) - P + 1;    \
+        b = (q) < P ? (q) : 0;            \
+    } else {                              \
+        i = (q) % N;                      \
+        b = (q) / N;                      \
+    }
+    // ring-polymer atoms (P > 1 beads) by propagator; list order is free
+    // (every task below is independent of the others)
+    if (P > 1)
+        for (int i = lane; i < N; i += nthr) {
+            const int ty = bv.type[aoff + i];
+            if (PT(ty) == 1) continue;
+            const int a = s_algo[ty];
+            if (a == MDT_PROP_PILE) l_pile[atomicAdd(&s_cnt[0], 1)] = (uint16_t)i;
+            if (a == MDT_PROP_LANGEVIN)
+                l_lan[atomicAdd(&s_cnt[2], 1)] = (uint16_t)i;
+            else
+                l_tr[atomicAdd(&s_cnt[1], 1)] = (uint16_t)i;
+        }
+    __syncthreads();
+    const int n_pile = s_cnt[0], n_tr = s_cnt[1], n_lan = s_cnt[2];
 
     if (flags & 1) {
         // ldfa for Langevin atoms, per bead (md_algo.f90:97)
-        for (int q = lane; q < N * P; q += nthr) {
-            const int i = q % N, b = q / N;
+        for (int q = lane; q < nab; q += nthr) {
+            int i, b;
+            AB(q, i, b);
             const int ty = bv.type[aoff + i];
-            if (b >= PT(ty)) continue;
             if (s_algo[ty] == MDT_PROP_LANGEVIN) {
                 int bad = 0;
                 bv.dens[PI_(b, i)] = ldfa(bv.dens[PI_(b, i)], bv.mass[aoff + i], &bad);
@@ -692,10 +724,10 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
             big[i] = (uint8_t)all;
         }
         __syncthreads();
-        for (int q = lane; q < N * P; q += nthr) {
-            const int i = q % N, b = q / N;
+        for (int q = lane; q < nab; q += nthr) {
+            int i, b;
+            AB(q, i, b);
             const int ty = bv.type[aoff + i];
-            if (b >= PT(ty)) continue;
             const uint8_t fb = bv.fixed[PI_(b, i)];
             const double mass = bv.mass[aoff + i];
             if (s_algo[ty] == MDT_PROP_LANGEVIN) {
@@ -729,57 +761,62 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
             big[i] = (uint8_t)all;
         }
         __syncthreads();
-        // pile_thermostat per atom (md_algo.f90:440-502): thread per (atom, component)
+        // pile_thermostat per atom (md_algo.f90:440-502)
         if (pc.any_pile) {
-            for (int q = lane; q < N * 3; q += nthr) {
-                const int i = q / 3, c = q % 3;
-                const int ty = bv.type[aoff + i];
-                if (s_algo[ty] != MDT_PROP_PILE) continue;
-                const double mass = bv.mass[aoff + i];
-                const int pb = PT(ty);
-                const double betaN = 1.0 / (MDT_KB * (s_proj[ty] ? tproj : tsurf) * pb);
-                double np[64];
-                for (int k = 0; k < pb; ++k) {
-                    double acc = 0.0;
-                    if (pb == 1)
-                        acc = mass * bv.v[XI(c, 0, i)];
-                    else
-                        for (int j = 0; j < P; ++j) acc = acc + (mass * bv.v[XI(c, j, i)]) * C[j + P * k];
-                    np[k] = acc;
-                }
-                for (int k = 0; k < pb; ++k) {
-                    double g;
-                    if (k == 0) {
-                        g = 1 / pc.pile_tau;
-                    } else {
-                        const double wn = 1 / betaN / MDT_HBAR;
-                        const double wk = 2 * wn * sin(k * MDT_PI / P);
-                        g = 2 * wk;
-                    }
-                    const double c1 = exp(-0.5 * dt * g);
+            // one-bead atoms: thread per (atom, component)
+            if (P == 1 || mix)
+                for (int q = lane; q < N * 3; q += nthr) {
+                    const int i = q / 3, c = q % 3;
+                    const int ty = bv.type[aoff + i];
+                    if (s_algo[ty] != MDT_PROP_PILE || PT(ty) != 1) continue;
+                    const double mass = bv.mass[aoff + i];
+                    const double betaN = 1.0 / (MDT_KB * (s_proj[ty] ? tproj : tsurf));
+                    const double c1 = exp(-0.5 * dt * (1 / pc.pile_tau));
                     const double c2 = sqrt(1 - c1 * c1);
-                    const double z = mdt_normal(pc.seed, tid, step, MDT_STREAM_PILE, (uint32_t)(i * P + k), (uint32_t)c);
-                    np[k] = c1 * np[k] + sqrt(mass / betaN) * c2 * z;
+                    const double z = mdt_normal(pc.seed, tid, step, MDT_STREAM_PILE, (uint32_t)(i * P), (uint32_t)c);
+                    const double np = c1 * (mass * bv.v[XI(c, 0, i)]) + sqrt(mass / betaN) * c2 * z;
+                    const uint8_t fb = bv.fixed[PI_(0, i)];
+                    bv.v[XI(c, 0, i)] = (fb >> c & 1) ? 0.0 : np / mass;
                 }
-                for (int b = 0; b < pb; ++b) {
-                    const uint8_t fb = bv.fixed[PI_(b, i)];
-                    double ap = 0.0;
-                    if (!(fb >> c & 1)) {
-                        if (pb == 1)
-                            ap = np[0];
-                        else
-                            for (int k = 0; k < P; ++k) ap = ap + np[k] * C[b + P * k];
-                    }
-                    bv.v[XI(c, b, i)] = ap / mass;
+            // ring polymers: thread per (atom, component, mode); the thermostatted
+            // normal-mode momenta go through f (rewritten by calc_force after this)
+            for (int q = lane; q < n_pile * 3 * P; q += nthr) {
+                const int k = q % P, c = (q / P) % 3, i = l_pile[q / (3 * P)];
+                const int ty = bv.type[aoff + i];
+                const double mass = bv.mass[aoff + i];
+                const double betaN = 1.0 / (MDT_KB * (s_proj[ty] ? tproj : tsurf) * P);
+                double acc = 0.0;
+                for (int j = 0; j < P; ++j) acc = acc + (mass * bv.v[XI(c, j, i)]) * C[j + P * k];
+                double g;
+                if (k == 0) {
+                    g = 1 / pc.pile_tau;
+                } else {
+                    const double wn = 1 / betaN / MDT_HBAR;
+                    const double wk = 2 * wn * sin(k * MDT_PI / P);
+                    g = 2 * wk;
                 }
+                const double c1 = exp(-0.5 * dt * g);
+                const double c2 = sqrt(1 - c1 * c1);
+                const double z = mdt_normal(pc.seed, tid, step, MDT_STREAM_PILE, (uint32_t)(i * P + k), (uint32_t)c);
+                bv.f[XI(c, k, i)] = c1 * acc + sqrt(mass / betaN) * c2 * z;
+            }
+            __syncthreads();
+            for (int q = lane; q < n_pile * 3 * P; q += nthr) {
+                const int b = q % P, c = (q / P) % 3, i = l_pile[q / (3 * P)];
+                const double mass = bv.mass[aoff + i];
+                const uint8_t fb = bv.fixed[PI_(b, i)];
+                double ap = 0.0;
+                if (!(fb >> c & 1))
+                    for (int k = 0; k < P; ++k) ap = ap + bv.f[XI(c, k, i)] * C[b + P * k];
+                bv.v[XI(c, b, i)] = ap / mass;
             }
             __syncthreads();
         }
-        for (int q = lane; q < N * P; q += nthr) {
-            const int i = q % N, b = q / N;
+        for (int q = lane; q < nab; q += nthr) {
+            int i, b;
+            AB(q, i, b);
             const int ty = bv.type[aoff + i];
             const int pb = PT(ty);
-            if (b >= pb) continue;
             const int algo = s_algo[ty];
             const uint8_t fb = bv.fixed[PI_(b, i)];
             const double mass = bv.mass[aoff + i];
@@ -859,9 +896,9 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
             }
             __syncthreads();
             const double fac = (1.0 - pow(1.0 - 1.0 / N, 1000.0)) / red[3] / P;
-            for (int q = lane; q < N * P; q += nthr) {
-                const int i = q % N, b = q / N;
-                if (b >= PT(bv.type[aoff + i])) continue;
+            for (int q = lane; q < nab; q += nthr) {
+                int i, b;
+                AB(q, i, b);
                 bv.v[XI(0, b, i)] -= red[0] * fac;
                 bv.v[XI(1, b, i)] -= red[1] * fac;
                 bv.v[XI(2, b, i)] -= red[2] * fac;
@@ -874,74 +911,75 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
         if (P == 1) {
             for (int q = lane; q < 3 * N; q += nthr) bv.f[soff + q] = 0.0;
         } else {
-            // do_ring_polymer_step (rpmd.f90:309-452), thread per (atom, component)
+            // do_ring_polymer_step (rpmd.f90:309-452)
             const double kt = MDT_KB * tproj * P / MDT_HBAR;
             const double m1 = bv.mass[aoff + 0];
             const double prefactor = -1 * m1 * (kt * kt);
             const double piN = MDT_PI / P;
-            for (int q = lane; q < N * 3; q += nthr) {
-                const int i = q / 3, c = q % 3;
+            if (mix)  // classical atoms: no ring polymer, f = 0
+                for (int q = lane; q < (N - 1) * 3; q += nthr) bv.f[XI(q % 3, 0, 1 + q / 3)] = 0.0;
+            // Langevin atoms: the step is reverted; spring force with m(1) and
+            // Tproj (rpmd.f90:418-435)
+            for (int q = lane; q < n_lan * 3 * P; q += nthr) {
+                const int b = q % P, c = (q / P) % 3, i = l_lan[q / (3 * P)];
+                const int pv = b == 0 ? P - 1 : b - 1, nx = b == P - 1 ? 0 : b + 1;
+                bv.f[XI(c, b, i)] = prefactor * (2 * bv.x[XI(c, b, i)] - bv.x[XI(c, pv, i)] - bv.x[XI(c, nx, i)]);
+            }
+            // the other ring polymers: thread per (atom, component, mode) for the
+            // forward transform and the exact harmonic rotation, whose results
+            // go through a (momenta) and f (positions) -- calc_force zeroes
+            // both (force.f90:49-50) -- then per (atom, component, bead) back
+            for (int q = lane; q < n_tr * 3 * P; q += nthr) {
+                const int k = q % P, c = (q / P) % 3, i = l_tr[q / (3 * P)];
                 const int ty = bv.type[aoff + i];
                 const double mass = bv.mass[aoff + i];
-                if (PT(ty) == 1) {  // classical atom (mixed mode): no ring polymer, f = 0
-                    bv.f[XI(c, 0, i)] = 0.0;
-                    continue;
-                }
-                if (s_algo[ty] == MDT_PROP_LANGEVIN) {
-                    // reverted step; spring force with m(1) and Tproj (rpmd.f90:418-435)
-                    for (int b = 0; b < P; ++b) {
-                        const int pv = b == 0 ? P - 1 : b - 1, nx = b == P - 1 ? 0 : b + 1;
-                        bv.f[XI(c, b, i)] =
-                            prefactor * (2 * bv.x[XI(c, b, i)] - bv.x[XI(c, pv, i)] - bv.x[XI(c, nx, i)]);
-                    }
-                    continue;
-                }
-                for (int b = 0; b < P; ++b) bv.f[XI(c, b, i)] = 0.0;
-                double pq[64], qq[64];
-                for (int k = 0; k < P; ++k) {
-                    double ap = 0.0, aq = 0.0;
-                    for (int j = 0; j < P; ++j) {
-                        ap = ap + (mass * bv.v[XI(c, j, i)]) * C[j + P * k];
-                        aq = aq + bv.x[XI(c, j, i)] * C[j + P * k];
-                    }
-                    pq[k] = ap;
-                    qq[k] = aq;
+                double ap = 0.0, aq = 0.0;
+                for (int j = 0; j < P; ++j) {
+                    ap = ap + (mass * bv.v[XI(c, j, i)]) * C[j + P * k];
+                    aq = aq + bv.x[XI(c, j, i)] * C[j + P * k];
                 }
                 const double betaN = 1.0 / (MDT_KB * (s_proj[ty] ? tproj : tsurf) * P);
                 const double twown = 2.0 / betaN / MDT_HBAR;
+                double p1 = 1.0, p2 = 0.0, p3 = dt / mass, p4 = 1.0;
+                const int fk = (k <= P / 2) ? k : P - k;  // mirrored modes share a frequency
+                if (fk > 0) {
+                    const double wk = twown * sin(fk * piN);
+                    const double wt = wk * dt, wm = wk * mass;
+                    const double cw = cos(wt), sw = sin(wt);
+                    p1 = cw;
+                    p2 = -wm * sw;
+                    p3 = sw / wm;
+                    p4 = cw;
+                }
+                bv.a[XI(c, k, i)] = ap * p1 + aq * p2;
+                bv.f[XI(c, k, i)] = ap * p3 + aq * p4;
+            }
+            __syncthreads();
+            for (int q = lane; q < n_tr * 3 * P; q += nthr) {
+                const int b = q % P, c = (q / P) % 3, i = l_tr[q / (3 * P)];
+                const uint8_t fb = bv.fixed[PI_(b, i)];
+                if (fb >> c & 1) continue;
+                const double mass = bv.mass[aoff + i];
+                double rr = 0.0, vv = 0.0;
                 for (int k = 0; k < P; ++k) {
-                    double p1 = 1.0, p2 = 0.0, p3 = dt / mass, p4 = 1.0;
-                    const int fk = (k <= P / 2) ? k : P - k;  // mirrored modes share a frequency
-                    if (fk > 0) {
-                        const double wk = twown * sin(fk * piN);
-                        const double wt = wk * dt, wm = wk * mass;
-                        const double cw = cos(wt), sw = sin(wt);
-                        p1 = cw;
-                        p2 = -wm * sw;
-                        p3 = sw / wm;
-                        p4 = cw;
-                    }
-                    const double pn = pq[k] * p1 + qq[k] * p2;
-                    qq[k] = pq[k] * p3 + qq[k] * p4;
-                    pq[k] = pn;
+                    rr = rr + bv.f[XI(c, k, i)] * C[b + P * k];
+                    vv = vv + bv.a[XI(c, k, i)] * C[b + P * k] / mass;
                 }
-                for (int b = 0; b < P; ++b) {
-                    const uint8_t fb = bv.fixed[PI_(b, i)];
-                    if (fb >> c & 1) continue;
-                    double rr = 0.0, vv = 0.0;
-                    for (int k = 0; k < P; ++k) {
-                        rr = rr + qq[k] * C[b + P * k];
-                        vv = vv + pq[k] * C[b + P * k] / mass;
-                    }
-                    bv.x[XI(c, b, i)] = rr;
-                    bv.v[XI(c, b, i)] = vv;
-                }
+                bv.x[XI(c, b, i)] = rr;
+                bv.v[XI(c, b, i)] = vv;
+            }
+            __syncthreads();
+            for (int q = lane; q < n_tr * 3 * P; q += nthr) {
+                const int b = q % P, c = (q / P) % 3, i = l_tr[q / (3 * P)];
+                bv.f[XI(c, b, i)] = 0.0;
+                bv.a[XI(c, b, i)] = 0.0;
             }
         }
     }
 #undef XI
 #undef PI_
 #undef PT
+#undef AB
     if (bad_local) atomicOr(&bv.status[t], bad_local);
 }
 

@@ -179,6 +179,13 @@ module mdt_gpu
             integer(c_int) :: rc
         end function
 
+        function mdt_batch_set_bead_mode(batch, mode) bind(C, name="mdt_batch_set_bead_mode") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: mode
+            integer(c_int) :: rc
+        end function
+
         function mdt_batch_set_graph(batch, chunk) bind(C, name="mdt_batch_set_graph") result(rc)
             import :: c_int, c_ptr
             type(c_ptr), value :: batch

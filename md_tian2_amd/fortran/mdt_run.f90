@@ -18,7 +18,9 @@
 ! replay of the step loop), nene <dir> (the NENE network of dir's input.nn,
 ! scaling.data, weights.XXX.data as the potential, EMT of `pes` as its
 ! fallback on extrapolation) and judge <n> (EMT judge of the network on a
-! rotating 1 % of the trajectories every n steps).
+! rotating 1 % of the trajectories every n steps), beads <all|mixed> (with
+! force_beads > 1: every atom carries the beads, as the reference, or only
+! the projectile, the slab staying classical).
 program mdt_run
 
     use, intrinsic :: iso_c_binding
@@ -41,7 +43,7 @@ program mdt_run
     character(len=8) :: pip_x = "r", pip_y = "r"
     character(len=3) :: name_proj = "H", name_latt = "Au"
     character(len=512) :: pes_file = "", dump_file = "", deck = "", nene_dir = ""
-    integer :: prec = 0, graph_chunk = 0, judge_interval = 0
+    integer :: prec = 0, graph_chunk = 0, judge_interval = 0, bead_mode = 0
     real(dp) :: jsq, jn, jmax
     integer(c_int32_t) :: nfb
 
@@ -107,6 +109,7 @@ program mdt_run
     call mdt_check(mdt_batch_set_state(batch, r, v), "mdt_batch_set_state")
     call mdt_check(mdt_batch_set_precision(batch, int(prec, c_int)), "mdt_batch_set_precision")
     call mdt_check(mdt_batch_set_graph(batch, int(graph_chunk, c_int)), "mdt_batch_set_graph")
+    call mdt_check(mdt_batch_set_bead_mode(batch, int(bead_mode, c_int)), "mdt_batch_set_bead_mode")
     if (len_trim(nene_dir) > 0) then
         call mdt_check(mdt_nn_load(ctx, cstr(nene_dir), 2_c_int, cstr(name2(tmpl%name(1)) // name2(tmpl%name(2)))), &
                        "mdt_nn_load")
@@ -197,6 +200,15 @@ contains
             case ("nene"); nene_dir = unquote(w(2))
             case ("judge"); read(w(2), *) judge_interval
             case ("graph"); read(w(2), *) graph_chunk
+            case ("beads")
+                if (trim(w(2)) == "mixed") then
+                    bead_mode = 1
+                else if (trim(w(2)) == "all") then
+                    bead_mode = 0
+                else
+                    print '(a,a)', "Error: beads must be all or mixed, got ", trim(w(2))
+                    stop 1
+                end if
             case ("precision")
                 if (trim(w(2)) == "fp32_pair") then
                     prec = 1

@@ -99,3 +99,51 @@ def test_fortran_driver_network_with_judge(tmp_path):
     assert res.returncode == 0, res.stdout + res.stderr
     rmse = [float(line.split()[4]) for line in res.stdout.splitlines() if line.startswith("network judge")]
     assert rmse and rmse[0] < 0.02
+
+
+def test_fortran_driver_mixed_beads_matches_library(tmp_path):
+    """`beads mixed` with force_beads 8 and a PILE projectile: the driver's
+    final state equals the Python binding's mixed-model run from the driver's
+    dumped initial state with the same seed and trajectory ids."""
+    from md_tian2_amd import mdt
+
+    deck = open(os.path.join(ROOT, "tests", "decks", "h_au_3x3x4.inp")).read()
+    dump = tmp_path / "mixed.bin"
+    deck = deck.replace("dump 'gpurun_out/mdt_run_h_au.bin'", f"dump '{dump}'")
+    deck = deck.replace("projectile 1 H 1.00794 ver", "projectile 1 H 1.00794 pil")
+    deck = deck.replace("Tproj 0", "Tproj 100\npile_tau 200")
+    deck += "\nforce_beads 8\nbeads mixed\n"
+    p = tmp_path / "mixed.inp"
+    p.write_text(deck)
+    exe = os.path.join(ROOT, "md_tian2_amd", "fortran", "mdt_run")
+    res = subprocess.run([exe, str(p)], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr
+    d = read_dump(str(dump))
+    assert d["nbeads"] == 8
+    tmpl = geometry.h_on_slab("Au", n=3, nlayers=4, nfixed=2, thermal=False, proj_algo=geometry.PROP_PILE,
+                              nbeads=8)
+    systems = []
+    for t in range(d["ntraj"]):
+        s = tmpl.copy()
+        s.r = d["r0"][t].copy()
+        s.v = d["v0"][t].copy()
+        systems.append(s)
+    pes = pesfile.load_pes(os.path.join(ROOT, "md_tian2_amd", "pes", "EMT-HAu.pes"))
+    cols, pesm = pesfile.emt_tables(pes, tmpl.type_names)
+    ctx = mdt.Context(0)
+    ctx.set_emt(cols, pesm)
+    b = mdt.Batch(ctx, systems, dt=0.1, tsurf=300.0, tproj=100.0, pile_tau=200.0, seed=777,
+                  traj_ids=[int(x) for x in d["tid"]])
+    b.set_bead_mode(1)
+    b.compute_forces()
+    b.step(int(d["nsteps"]))
+    out = b.get_state()
+    b.close()
+    ctx.close()
+    for t, o in enumerate(out):
+        assert np.abs(d["r1"][t] - o["r"]).max() < 1e-9
+        assert np.abs(d["v1"][t] - o["v"]).max() < 1e-9
+        assert np.abs(d["epot"][t] - o["epot"]).max() < 1e-8
+        # the slab is classical: every bead reports the same positions
+        assert np.array_equal(d["r1"][t][1:], np.repeat(d["r1"][t][1:, :1], 8, axis=1))
