@@ -1087,13 +1087,28 @@ __global__ void __launch_bounds__(512, 2)
     const unsigned long long gmask = (TPA >= 64 ? ~0ull : ((1ull << TPA) - 1ull)) << ((wl / TPA) * TPA);
     const int tq = typ[0];
 
-    // ---- 1a: owned half lists of the classical atoms (FP64 scan) ----
+    // ---- 1a: owned half lists of the classical atoms ----
+    // FP32 membership scan on float copies of the fractional coordinates (in
+    // the G0/E0/D0 words, unused until pass A), r^2 bound widened by 1e-4;
+    // both passes re-check r <= cutr in FP64, and half-cell ties take the
+    // reference's image (tie_image)
+    float *fx32 = (float *)G0, *fy32 = (float *)E0, *fz32 = (float *)D0;
+    for (int a = lane; a < Ns; a += nthr) {
+        fx32[a] = (float)sx[a];
+        fy32[a] = (float)sy[a];
+        fz32[a] = (float)sz[a];
+    }
+    __syncthreads();
+    float hf[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) hf[q] = (float)cell.h[q];
+    const float cut2f = (float)(cut2 * (1.0 + 1e-4));
     for (int k0 = 0; k0 < Ns; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < Ns;
         const int tk = act ? typ[k + 1] : 0;
-        const double fxk = act ? sx[k] : 0.0, fyk = act ? sy[k] : 0.0, fzk = act ? sz[k] : 0.0;
+        const float fxk = act ? fx32[k] : 0.f, fyk = act ? fy32[k] : 0.f, fzk = act ? fz32[k] : 0.f;
         int c = 0;
         for (int n0 = 0; n0 < Ns; n0 += TPA) {
             const int n = n0 + sub;
@@ -1102,25 +1117,25 @@ __global__ void __launch_bounds__(512, 2)
             if (act && n < Ns && n != k) {
                 const int tn = typ[n + 1];
                 if (T.pair_on[tk * MDT_MAXT + tn] && owns(k, n)) {
-                    double d0 = fxk - sx[n], d1 = fyk - sy[n], d2 = fzk - sz[n];
-                    double m0 = rint(d0), m1 = rint(d1), m2 = rint(d2);
-                    const bool t0 = fabs(fabs(d0) - 0.5) < 1e-4, t1 = fabs(fabs(d1) - 0.5) < 1e-4,
-                               t2 = fabs(fabs(d2) - 0.5) < 1e-4;
+                    float d0 = fxk - fx32[n], d1 = fyk - fy32[n], d2 = fzk - fz32[n];
+                    float m0 = rintf(d0), m1 = rintf(d1), m2 = rintf(d2);
+                    const bool t0 = fabsf(fabsf(d0) - 0.5f) < 1e-4f, t1 = fabsf(fabsf(d1) - 0.5f) < 1e-4f,
+                               t2 = fabsf(fabsf(d2) - 0.5f) < 1e-4f;
                     if (t0 | t1 | t2) {  // half-cell tie: the reference's image
                         const double *xr = bv.x + soff;
                         const double a0 = xr[k + 1], a1 = xr[N + k + 1], a2 = xr[2 * N + k + 1];
                         const double b0 = xr[n + 1], b1 = xr[N + n + 1], b2 = xr[2 * N + n + 1];
-                        if (t0) m0 = tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, d0);
-                        if (t1) m1 = tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, d1);
-                        if (t2) m2 = tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, d2);
+                        if (t0) m0 = (float)tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, sx[k] - sx[n]);
+                        if (t1) m1 = (float)tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, sy[k] - sy[n]);
+                        if (t2) m2 = (float)tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, sz[k] - sz[n]);
                     }
                     d0 -= m0;
                     d1 -= m1;
                     d2 -= m2;
-                    const double ex = cell.h[0] * d0 + cell.h[3] * d1 + cell.h[6] * d2;
-                    const double ey = cell.h[1] * d0 + cell.h[4] * d1 + cell.h[7] * d2;
-                    const double ez = cell.h[2] * d0 + cell.h[5] * d1 + cell.h[8] * d2;
-                    in = (ex * ex + ey * ey + ez * ez) <= cut2;
+                    const float ex = hf[0] * d0 + hf[3] * d1 + hf[6] * d2;
+                    const float ey = hf[1] * d0 + hf[4] * d1 + hf[7] * d2;
+                    const float ez = hf[2] * d0 + hf[5] * d1 + hf[8] * d2;
+                    in = (ex * ex + ey * ey + ez * ez) <= cut2f;
                     code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
                 }
             }
