@@ -61,7 +61,19 @@ struct NnSfTab {
     float a_eta[NN_MAXE][NN_MAXA], a_etal2[NN_MAXE][NN_MAXA], a_lam[NN_MAXE][NN_MAXA], a_norm[NN_MAXE][NN_MAXA];
     int a_zp[NN_MAXE][NN_MAXA];
     float glo[NN_MAXE][NN_K1], ghi[NN_MAXE][NN_K1];
+    // factored angular sums: an element whose angular functions share one
+    // neighbour-pair class, take at most NN_FB distinct eta and NN_FB
+    // distinct (lambda, zeta), and map one-to-one onto (eta, (lambda,
+    // zeta)) slots accumulates per triplet the outer product of
+    // B_n = exp(-eta_n (rij^2 + rik^2 + rjk^2)) and u_m = (1 + lambda_m
+    // cos)^zeta_m; function k is norm_k * slot (n_k, m_k)
+    int f_ok[NN_MAXE], f_cls[NN_MAXE], f_nb[NN_MAXE], f_nu[NN_MAXE];
+    float f_etal2[NN_MAXE][4], f_eta[NN_MAXE][4], f_lam[NN_MAXE][4];
+    int f_zp[NN_MAXE][4];
+    int f_slot[NN_MAXE][NN_MAXA];  // n * NN_FB + m
 };
+#define NN_FB 4
+static_assert(NN_FB * NN_FB == NN_TPA, "one slot of the factored angular sums per lane of a centre's group");
 
 __device__ __forceinline__ float act_f(int code, float z, float *d1) {
     if (code == 1) {
@@ -370,6 +382,74 @@ __device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem
         }
     }
     // angular (atomsymfunction3Andi): pairs p < q of the neighbour list
+    if (i < N && S.f_ok[e]) {
+        // factored form: per triplet NN_FB exponentials and NN_FB powers,
+        // their outer product accumulated per slot
+        const int fcls = S.f_cls[e], nb = S.f_nb[e], nu = S.f_nu[e];
+        float etl[NN_FB], lam[NN_FB];
+        int zp[NN_FB];
+#pragma unroll
+        for (int q = 0; q < NN_FB; ++q) {
+            etl[q] = S.f_etal2[e][q];
+            lam[q] = S.f_lam[e][q];
+            zp[q] = S.f_zp[e][q];
+        }
+        float acc[NN_FB * NN_FB];
+#pragma unroll
+        for (int q = 0; q < NN_FB * NN_FB; ++q) acc[q] = 0.f;
+        for (int p = sub; p < cnt; p += NN_TPA) {
+            const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
+            const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p];
+            const int ep = L.el[base + p];
+            for (int q = p + 1; q < cnt; ++q) {
+                const float bx = L.dx[base + q], by = L.dy[base + q], bz = L.dz[base + q];
+                const float cx = bx - ax_, cy = by - ay_, cz = bz - az_;
+                const float rc2v = cx * cx + cy * cy + cz * cz;
+                if (rc2v > M.rc2) continue;
+                const int eq = L.el[base + q];
+                const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
+                if (cls != fcls) continue;
+                const float rb = L.r[base + q];
+                const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * L.ir[base + q];
+                float fcc, dfcc;
+                cutoff(M.pi_rc, sqrtf(rc2v), &fcc, &dfcc);
+                const float Cf = fa * L.fc[base + q] * fcc;
+                const float Sd = ra * ra + rb * rb + rc2v;
+                float u[NN_FB];
+#pragma unroll
+                for (int m = 0; m < NN_FB; ++m) {
+                    const float x = 1.0f + lam[m] * cth;
+                    float pw = x;
+#pragma unroll
+                    for (int z = 0; z < 4; ++z) pw = z < zp[m] ? pw * pw : pw;  // zeta <= 16
+                    u[m] = (m < nu && x > 0.f) ? pw * Cf : 0.f;
+                }
+#pragma unroll
+                for (int n = 0; n < NN_FB; ++n) {
+                    const float B = n < nb ? __builtin_amdgcn_exp2f(-etl[n] * Sd) : 0.f;
+#pragma unroll
+                    for (int m = 0; m < NN_FB; ++m) acc[n * NN_FB + m] += B * u[m];
+                }
+            }
+        }
+        // slot q's sum to lane q of the group; function k reads its slot's lane
+        float mine = 0.f;
+#pragma unroll
+        for (int q = 0; q < NN_FB * NN_FB; ++q) {
+            const float v = sum_group<NN_TPA>(acc[q]);
+            mine = (sub == q) ? v : mine;
+        }
+#pragma unroll 1
+        for (int k = 0; k < NN_MAXA; ++k) {
+            const float v = __shfl(mine, (lane & ~(NN_TPA - 1)) + S.f_slot[e][k < na ? k : 0], 64);
+            if (k < na && sub == (k & (NN_TPA - 1))) {
+                const float g4 = v * S.a_norm[e][k];
+                Grep[(size_t)i * NN_K1 + nr + k] = g4;
+                if (g4 < S.glo[e][nr + k] || g4 > S.ghi[e][nr + k]) extrap = 1;
+            }
+        }
+    } else {
+        // generic form: every function per triplet
 #pragma unroll
     for (int k = 0; k < NN_MAXA; ++k) g[k] = 0.f;
     for (int p = sub; p < cnt; p += NN_TPA) {
@@ -414,15 +494,16 @@ __device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem
             if (v < S.glo[e][nr + k] || v > S.ghi[e][nr + k]) extrap = 1;
         }
     }
+    }
     if (extrap) *extrapp = 1;
 }
 
 // dE/dr of the chunk's centre atoms' symmetry functions with coefficients
 // cb = dE/dG (VJP): centre force in registers, neighbours' shares into the
 // fixed-point accumulators ax, ay, az (Hartree/Bohr).
-__device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*cb)[NN_K1], const uint8_t *elem,
-                            int N, int k0, int lane, float rc2, float pi_rc, unsigned long long *ax,
-                            unsigned long long *ay, unsigned long long *az, int *badp) {
+__device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*cb)[NN_K1], float (*Wsh)[NN_FB * NN_FB],
+                            const uint8_t *elem, int N, int k0, int lane, float rc2, float pi_rc,
+                            unsigned long long *ax, unsigned long long *ay, unsigned long long *az, int *badp) {
     const int c = lane / NN_TPA, sub = lane % NN_TPA;
     int bad = 0;
     struct {
@@ -457,6 +538,35 @@ __device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*c
         fxadd(&az[j], -(double)v2, &bad);
     }
     // angular
+    const bool fast = i < N && S.f_ok[e];
+    float W[NN_FB * NN_FB], etl[NN_FB], eta[NN_FB], lam[NN_FB];
+    int zp[NN_FB];
+    int fcls = -1, nb = 0, nu = 0;
+    {
+        // W(n, m) = sum over the functions on slot (n, m) of dE/dG_k norm_k,
+        // lane `sub` of each centre's group computing slot sub (every lane
+        // of the block passes the barrier)
+        float wq = 0.f;
+        if (fast)
+            for (int k = 0; k < na; ++k)
+                if (S.f_slot[e][k] == sub) wq += cb[c][nr + k] * S.a_norm[e][k];
+        Wsh[c][sub] = wq;
+        __syncthreads();
+    }
+    if (fast) {
+        fcls = S.f_cls[e];
+        nb = S.f_nb[e];
+        nu = S.f_nu[e];
+#pragma unroll
+        for (int q = 0; q < NN_FB; ++q) {
+            etl[q] = S.f_etal2[e][q];
+            eta[q] = S.f_eta[e][q];
+            lam[q] = S.f_lam[e][q];
+            zp[q] = S.f_zp[e][q];
+        }
+#pragma unroll
+        for (int q = 0; q < NN_FB * NN_FB; ++q) W[q] = Wsh[c][q];
+    }
     for (int p = sub; p < cnt; p += NN_TPA) {
         const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
         const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p], dfa = L.dfc[base + p];
@@ -467,38 +577,73 @@ __device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*c
             const float cx = bx - ax_, cy = by - ay_, cz = bz - az_;
             const float rc2v = cx * cx + cy * cy + cz * cz;
             if (rc2v > M.rc2) continue;
-            const float rb = L.r[base + q], ib = L.ir[base + q], fb = L.fc[base + q], dfb = L.dfc[base + q];
             const int eq = L.el[base + q];
             const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
+            if (fast && cls != fcls) continue;
+            const float rb = L.r[base + q], ib = L.ir[base + q], fb = L.fc[base + q], dfb = L.dfc[base + q];
             const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * ib;
             const float rcc = sqrtf(rc2v);
             float fcc, dfcc;
             cutoff(M.pi_rc, rcc, &fcc, &dfcc);
             const float Cf = fa * fb * fcc;
             const float Sd = ra * ra + rb * rb + rc2v;
-            float Sg = 0.f, Sc = 0.f, Ss = 0.f, B = 0.f, etaprev = -1.f;
+            float Sg = 0.f, Sc = 0.f, Ss = 0.f;
+            if (fast) {
+                // Sg = sum B_n W(n,m) u_m, Sc = sum B_n W(n,m) du_m/dcos,
+                // Ss = sum eta_n B_n W(n,m) u_m
+                float u[NN_FB], du[NN_FB];
 #pragma unroll
-            for (int k = 0; k < NN_MAXA; ++k)
-                if (k < na && cls == S.a_cls[e][k]) {
-                    const float lam = S.a_lam[e][k];
-                    const float x = 1.0f + lam * cth;
-                    if (x > 0.f) {
-                        float pw = x, pm = 1.f;  // x^zeta, x^(zeta-1)
-                        for (int z = 0; z < S.a_zp[e][k]; ++z) {
-                            pm *= pw;
-                            pw *= pw;
-                        }
-                        if (S.a_etal2[e][k] != etaprev) {
-                            etaprev = S.a_etal2[e][k];
-                            B = __builtin_amdgcn_exp2f(-etaprev * Sd);
-                        }
-                        const float ck = cb[c][nr + k] * S.a_norm[e][k] * B;
-                        const float zeta = (float)(1 << S.a_zp[e][k]);
-                        Sg += ck * pw;
-                        Sc += ck * zeta * lam * pm;
-                        Ss += ck * S.a_eta[e][k] * pw;
+                for (int m = 0; m < NN_FB; ++m) {
+                    const float x = 1.0f + lam[m] * cth;
+                    float pw = x, pm = 1.f;  // x^zeta, x^(zeta-1); zeta <= 16
+#pragma unroll
+                    for (int z = 0; z < 4; ++z) {
+                        const bool go = z < zp[m];
+                        pm = go ? pm * pw : pm;
+                        pw = go ? pw * pw : pw;
                     }
+                    const bool on = m < nu && x > 0.f;
+                    u[m] = on ? pw : 0.f;
+                    du[m] = on ? (float)(1 << zp[m]) * lam[m] * pm : 0.f;
                 }
+#pragma unroll
+                for (int n = 0; n < NN_FB; ++n) {
+                    const float B = n < nb ? __builtin_amdgcn_exp2f(-etl[n] * Sd) : 0.f;
+                    float t = 0.f, td = 0.f;
+#pragma unroll
+                    for (int m = 0; m < NN_FB; ++m) {
+                        t += W[n * NN_FB + m] * u[m];
+                        td += W[n * NN_FB + m] * du[m];
+                    }
+                    Sg += B * t;
+                    Sc += B * td;
+                    Ss += eta[n] * B * t;
+                }
+            } else {
+                float B = 0.f, etaprev = -1.f;
+#pragma unroll
+                for (int k = 0; k < NN_MAXA; ++k)
+                    if (k < na && cls == S.a_cls[e][k]) {
+                        const float lamk = S.a_lam[e][k];
+                        const float x = 1.0f + lamk * cth;
+                        if (x > 0.f) {
+                            float pw = x, pm = 1.f;  // x^zeta, x^(zeta-1)
+                            for (int z = 0; z < S.a_zp[e][k]; ++z) {
+                                pm *= pw;
+                                pw *= pw;
+                            }
+                            if (S.a_etal2[e][k] != etaprev) {
+                                etaprev = S.a_etal2[e][k];
+                                B = __builtin_amdgcn_exp2f(-etaprev * Sd);
+                            }
+                            const float ck = cb[c][nr + k] * S.a_norm[e][k] * B;
+                            const float zeta = (float)(1 << S.a_zp[e][k]);
+                            Sg += ck * pw;
+                            Sc += ck * zeta * lamk * pm;
+                            Ss += ck * S.a_eta[e][k] * pw;
+                        }
+                    }
+            }
             if (Sg == 0.f && Sc == 0.f) continue;
             const float iab = ia * ib;
             const float tc = Cf * Sc, ts = -2.f * Cf * Ss;
@@ -664,6 +809,50 @@ __device__ void load_sftab(NnSfTab &S, const NnConst &M, int lane) {
         S.glo[e][q] = M.glo[e][q];
         S.ghi[e][q] = M.ghi[e][q];
     }
+    // factored angular tables, one lane per element (from M: no dependence
+    // on the copies above)
+    if (lane < NN_MAXE) {
+        const int e = lane, na = M.na[e];
+        int ok = na > 0, nb = 0, nu = 0, cls = na > 0 ? M.a_e1[e][0] * NN_MAXE + M.a_e2[e][0] : -1;
+        int used = 0;  // bit per slot
+        for (int k = 0; k < na && ok; ++k) {
+            if (M.a_e1[e][k] * NN_MAXE + M.a_e2[e][k] != cls) ok = 0;
+            int n = 0, m = 0;
+            while (n < nb && S.f_etal2[e][n] != M.a_etal2[e][k]) ++n;
+            if (n == nb) {
+                if (nb == NN_FB) {
+                    ok = 0;
+                    break;
+                }
+                S.f_etal2[e][nb] = M.a_etal2[e][k];
+                S.f_eta[e][nb] = M.a_etal2[e][k] * (1.0f / NN_L2E);
+                ++nb;
+            }
+            while (m < nu && !(S.f_lam[e][m] == M.a_lam[e][k] && S.f_zp[e][m] == M.a_zp[e][k])) ++m;
+            if (m == nu) {
+                if (nu == NN_FB) {
+                    ok = 0;
+                    break;
+                }
+                S.f_lam[e][nu] = M.a_lam[e][k];
+                S.f_zp[e][nu] = M.a_zp[e][k];
+                ++nu;
+            }
+            const int slot = n * NN_FB + m;
+            if (used >> slot & 1) ok = 0;  // two functions on one slot
+            used |= 1 << slot;
+            S.f_slot[e][k] = slot;
+        }
+        for (int n = nb; n < NN_FB; ++n) S.f_etal2[e][n] = S.f_eta[e][n] = 0.f;
+        for (int m = nu; m < NN_FB; ++m) {
+            S.f_lam[e][m] = 0.f;
+            S.f_zp[e][m] = 0;
+        }
+        S.f_ok[e] = ok;
+        S.f_cls[e] = cls;
+        S.f_nb[e] = nb;
+        S.f_nu[e] = nu;
+    }
 }
 
 // cell in Bohr (columns = lattice vectors), its inverse in 1/Angstrom, and
@@ -683,13 +872,15 @@ __device__ void cell_setup(const NnConst &M, const double *cellA, const double *
     }
 }
 
-__global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
-                                                   int add_fin, int skip_extrap) {
+// Pass G and the network: raw G rows, c = dE/dG rows, the replica's energy
+// and extrapolation flag.  A replica outside the training range leaves
+// with its flag set when skip_extrap (k_nene_f skips it too).
+__global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
+                                                     int skip_extrap) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ NnSfTab S;
     __shared__ ChunkNbr L;
     __shared__ float T[NN_BLOCK / 64][16 * 33];
-    __shared__ float cb[NN_CA][NN_K1];
     __shared__ int rowsb[NN_BLOCK / 64][16];
     __shared__ int s_ne[NN_MAXE];
     const NnConst &M = *Mg;
@@ -699,7 +890,6 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
     const int N = bv.traj_N[t];
     const long long soff = bv.traj_soff[t];
     const long long roff = soff + (long long)b * 3 * N;
-    const long long doff = soff / 3 + (long long)b * N;
     const long long aoff = bv.traj_aoff[t];
     const int lane = threadIdx.x;
     const int wave = lane >> 6;
@@ -707,10 +897,9 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
     float *Crep = nv.C + (size_t)rep * nv.maxN * NN_K1;
     int bad = 0, extrap = 0;
 
-    // dynamic LDS: fx fy fz Ea (double N each) | ax ay az (u64 N) | elist (u16 N) | elem (u8 N)
+    // dynamic LDS: fx fy fz Ea (double N each) | elist (u16 N) | elem (u8 N)
     double *fx = (double *)smem_raw, *fy = fx + N, *fz = fy + N, *Ea = fz + N;
-    unsigned long long *ax = (unsigned long long *)(Ea + N), *ay = ax + N, *az = ay + N;
-    uint16_t *elist = (uint16_t *)(az + N);
+    uint16_t *elist = (uint16_t *)(Ea + N);
     uint8_t *elem = (uint8_t *)(elist + N);
 
     // ---- tables ----
@@ -735,7 +924,6 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
         fy[i] = f1 - floor(f1);
         fz[i] = f2 - floor(f2);
         Ea[i] = 0.0;
-        ax[i] = ay[i] = az[i] = 0ull;
     }
     __syncthreads();
     // atoms grouped by element, ascending index within each group (wave 0)
@@ -755,7 +943,6 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
     }
     __syncthreads();
 
-    const int c = lane / NN_TPA, sub = lane % NN_TPA;
     // ---- pass G: symmetry functions ----
     for (int k0 = 0; k0 < N; k0 += NN_CA) {
         build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
@@ -766,13 +953,12 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
 
     // replicas outside the training range go to the EMT fallback untouched
     extrap = __syncthreads_or(extrap);
+    if (lane == 0) nv.rep_flag[rep] = extrap;
     if (extrap && skip_extrap) {
-        if (lane == 0) {
-            nv.rep_flag[rep] = 1;
+        if (lane == 0)
             atomicOr(&bv.status[t], bad | MDT_ST_NN_EXTRAPOLATION);
-        } else if (bad) {
+        else if (bad)
             atomicOr(&bv.status[t], bad);
-        }
         return;
     }
 
@@ -806,6 +992,65 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
         __syncthreads();
     }
 
+    // ---- energy [eV] (fixed-order sum) ----
+    if (lane < 64) {
+        double e = 0.0;
+        for (int i = lane; i < N; i += 64) e += Ea[i];
+        for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+        if (lane == 0) bv.epot[bv.traj_eoff[t] + b] = e * NN_HA2EV;
+    }
+    if (extrap) bad |= MDT_ST_NN_EXTRAPOLATION;
+    if (bad) atomicOr(&bv.status[t], bad);
+}
+
+// Pass F and the outputs: dE/dr from the c rows of k_nene_g, f (+ the
+// forces already in f when add_fin), a = f/m, dens = 0.
+__global__ void __launch_bounds__(NN_BLOCK) k_nene_f(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
+                                                     int add_fin, int skip_extrap) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ NnSfTab S;
+    __shared__ ChunkNbr L;
+    __shared__ float cb[NN_CA][NN_K1];
+    __shared__ float Wsh[NN_CA][NN_FB * NN_FB];
+    const NnConst &M = *Mg;
+    const int rep = blockIdx.x;
+    if (skip_extrap && nv.rep_flag[rep]) return;  // whole block: uniform
+    const int t = bv.rep_traj[rep];
+    const int b = bv.rep_bead[rep];
+    const int N = bv.traj_N[t];
+    const long long soff = bv.traj_soff[t];
+    const long long roff = soff + (long long)b * 3 * N;
+    const long long doff = soff / 3 + (long long)b * N;
+    const long long aoff = bv.traj_aoff[t];
+    const int lane = threadIdx.x;
+    const float *Crep = nv.C + (size_t)rep * nv.maxN * NN_K1;
+    int bad = 0;
+
+    // dynamic LDS: fx fy fz (double N each) | ax ay az (u64 N) | elem (u8 N)
+    double *fx = (double *)smem_raw, *fy = fx + N, *fz = fy + N;
+    unsigned long long *ax = (unsigned long long *)(fz + N), *ay = ax + N, *az = ay + N;
+    uint8_t *elem = (uint8_t *)(az + N);
+
+    const int ne_model = M.nelem;
+    load_sftab(S, M, lane);
+    double h[9], ih[9];
+    float w[3];
+    int imax[3];
+    cell_setup(M, bv.cell + 9 * t, bv.icell + 9 * t, h, ih, w, imax);
+    for (int i = lane; i < N; i += NN_BLOCK) {
+        const int e = M.elem_of_type[bv.type[aoff + i]];
+        elem[i] = (uint8_t)(e < 0 || e >= ne_model ? 0 : e);
+        const double r0 = bv.x[roff + i], r1 = bv.x[roff + N + i], r2 = bv.x[roff + 2 * N + i];
+        double f0 = ih[0] * r0 + ih[3] * r1 + ih[6] * r2;
+        double f1 = ih[1] * r0 + ih[4] * r1 + ih[7] * r2;
+        double f2 = ih[2] * r0 + ih[5] * r1 + ih[8] * r2;
+        fx[i] = f0 - floor(f0);
+        fy[i] = f1 - floor(f1);
+        fz[i] = f2 - floor(f2);
+        ax[i] = ay[i] = az[i] = 0ull;
+    }
+    __syncthreads();
+
     // ---- pass F: forces ----
     for (int k0 = 0; k0 < N; k0 += NN_CA) {
         build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
@@ -815,11 +1060,11 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
             cb[ci][k] = (i < N && k < S.nr[elem[i]] + S.na[elem[i]]) ? Crep[(size_t)i * NN_K1 + k] : 0.f;
         }
         __syncthreads();
-        force_chunk(S, L, cb, elem, N, k0, lane, M.rc2, M.pi_rc, ax, ay, az, &bad);
+        force_chunk(S, L, cb, Wsh, elem, N, k0, lane, M.rc2, M.pi_rc, ax, ay, az, &bad);
         __syncthreads();
     }
 
-    // ---- outputs: f [eV/A], a, dens (no EMT density: 0), epot [eV] ----
+    // ---- outputs: f [eV/A], a, dens (no EMT density: 0) ----
     const double fconv = NN_HA2EV / NN_BOHR2ANG;
     for (int i = lane; i < N; i += NN_BLOCK) {
         double F[3] = {(double)(long long)ax[i] * NN_FISCALE * fconv, (double)(long long)ay[i] * NN_FISCALE * fconv,
@@ -841,17 +1086,6 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene(const NnConst *__restrict__ M
         bv.a[roff + N + i] = (fb & 2) ? 0.0 : F[1] / mass;
         bv.a[roff + 2 * N + i] = (fb & 4) ? 0.0 : F[2] / mass;
         bv.dens[doff + i] = 0.0;
-    }
-    __syncthreads();
-    if (lane < 64) {
-        double e = 0.0;
-        for (int i = lane; i < N; i += 64) e += Ea[i];
-        for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
-        if (lane == 0) bv.epot[bv.traj_eoff[t] + b] = e * NN_HA2EV;
-    }
-    if (lane == 0) {
-        nv.rep_flag[rep] = extrap;
-        if (extrap) bad |= MDT_ST_NN_EXTRAPOLATION;
     }
     if (bad) atomicOr(&bv.status[t], bad);
 }
@@ -877,6 +1111,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nn_frame(const NnConst *__restrict
     __shared__ NnSfTab S;
     __shared__ ChunkNbr L;
     __shared__ float cb[NN_CA][NN_K1];
+    __shared__ float Wsh[NN_CA][NN_FB * NN_FB];
     __shared__ float sdfac[NN_MAXE][NN_K1];
     __shared__ double red[NN_BLOCK / 64];
     const NnConst &M = *Mg;
@@ -929,7 +1164,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nn_frame(const NnConst *__restrict
             cb[ci][k] = (i < N && k < S.nr[elem[i]] + S.na[elem[i]]) ? Crow[(size_t)i * NN_K1 + k] : 0.f;
         }
         __syncthreads();
-        force_chunk(S, L, cb, elem, N, k0, lane, M.rc2, M.pi_rc, ax, ay, az, &bad);
+        force_chunk(S, L, cb, Wsh, elem, N, k0, lane, M.rc2, M.pi_rc, ax, ay, az, &bad);
         __syncthreads();
     }
     // residuals: g = dL/dF (per Hartree/Bohr), force loss part
@@ -1334,13 +1569,27 @@ hipError_t mdt_launch_nn_adam(NnConst *M, const float *partials, int W, float *g
     return hipGetLastError();
 }
 
-size_t mdt_nene_smem(int N) { return (size_t)N * (4 * 8 + 3 * 8 + 2 + 1) + 16; }
+// dynamic LDS of the larger of k_nene_g (4 N doubles, u16 + u8 per atom)
+// and k_nene_f (3 N doubles, 3 N u64, u8 per atom)
+size_t mdt_nene_smem(int N) { return (size_t)N * (6 * 8 + 1) + 16; }
 
-hipError_t mdt_nene_attributes(hipFuncAttributes *fa) { return hipFuncGetAttributes(fa, (const void *)k_nene); }
+// static LDS: the larger of the two kernels'
+hipError_t mdt_nene_attributes(hipFuncAttributes *fa) {
+    hipFuncAttributes g, f;
+    hipError_t e = hipFuncGetAttributes(&g, (const void *)k_nene_g);
+    if (e != hipSuccess) return e;
+    e = hipFuncGetAttributes(&f, (const void *)k_nene_f);
+    if (e != hipSuccess) return e;
+    *fa = g.sharedSizeBytes >= f.sharedSizeBytes ? g : f;
+    return hipSuccess;
+}
 
 hipError_t mdt_launch_nene(int nrep, size_t smem, const NnConst *M, const BatchView &bv, float *G, float *C,
                            int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, hipStream_t st) {
     NnView nv{G, C, rep_flag, fpes, maxN};
-    hipLaunchKernelGGL(k_nene, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, add_fin, skip_extrap);
+    hipLaunchKernelGGL(k_nene_g, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, skip_extrap);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_nene_f, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, add_fin, skip_extrap);
     return hipGetLastError();
 }
