@@ -55,12 +55,11 @@ struct NnView {
 // symmetry-function tables every lane indexes by element (LDS copy)
 struct NnSfTab {
     int nr[NN_MAXE], na[NN_MAXE];
-    int r_el[NN_MAXE][NN_MAXR];
+    int8_t r_el[NN_MAXE][NN_MAXR];
     float r_eta[NN_MAXE][NN_MAXR], r_rs[NN_MAXE][NN_MAXR];
-    int a_cls[NN_MAXE][NN_MAXA];  // neighbour-pair class e1 * NN_MAXE + e2 (e1 <= e2)
+    int8_t a_cls[NN_MAXE][NN_MAXA];  // neighbour-pair class e1 * NN_MAXE + e2 (e1 <= e2)
     float a_eta[NN_MAXE][NN_MAXA], a_etal2[NN_MAXE][NN_MAXA], a_lam[NN_MAXE][NN_MAXA], a_norm[NN_MAXE][NN_MAXA];
-    int a_zp[NN_MAXE][NN_MAXA];
-    float glo[NN_MAXE][NN_K1], ghi[NN_MAXE][NN_K1];
+    int8_t a_zp[NN_MAXE][NN_MAXA];
     // factored angular sums: an element whose angular functions share one
     // neighbour-pair class, take at most NN_FB distinct eta and NN_FB
     // distinct (lambda, zeta), and map one-to-one onto (eta, (lambda,
@@ -70,7 +69,7 @@ struct NnSfTab {
     int f_ok[NN_MAXE], f_cls[NN_MAXE], f_nb[NN_MAXE], f_nu[NN_MAXE];
     float f_etal2[NN_MAXE][4], f_eta[NN_MAXE][4], f_lam[NN_MAXE][4];
     int f_zp[NN_MAXE][4];
-    int f_slot[NN_MAXE][NN_MAXA];  // n * NN_FB + m
+    int8_t f_slot[NN_MAXE][NN_MAXA];  // n * NN_FB + m
 };
 #define NN_FB 4
 static_assert(NN_FB * NN_FB == NN_TPA, "one slot of the factored angular sums per lane of a centre's group");
@@ -91,9 +90,10 @@ __device__ __forceinline__ float act_f(int code, float z, float *d1) {
 }
 
 // cut-off type 1 of getcutoff (inner cut-off 0): fc = 0.5 (cos(pi r/rc) + 1)
+// (hardware sine / cosine of x / 2 pi revolutions; x = pi r / rc in [0, pi])
 __device__ __forceinline__ void cutoff(float pi_rc, float r, float *fc, float *dfc) {
-    const float x = r * pi_rc;
-    const float s = sinf(x), c = cosf(x);
+    const float rev = r * pi_rc * 0.15915494309189535f;
+    const float s = __builtin_amdgcn_sinf(rev), c = __builtin_amdgcn_cosf(rev);
     *fc = 0.5f * (c + 1.0f);
     *dfc = -0.5f * pi_rc * s;
 }
@@ -128,11 +128,19 @@ __device__ __forceinline__ void fxadd(unsigned long long *acc, double x, int *ba
 // per-chunk neighbour data (structure of arrays)
 struct ChunkNbr {
     float dx[NN_CA * NN_CAP], dy[NN_CA * NN_CAP], dz[NN_CA * NN_CAP];
-    float ir[NN_CA * NN_CAP], r[NN_CA * NN_CAP], fc[NN_CA * NN_CAP], dfc[NN_CA * NN_CAP];
+    float fc[NN_CA * NN_CAP], dfc[NN_CA * NN_CAP];
     uint16_t j[NN_CA * NN_CAP];
     uint8_t el[NN_CA * NN_CAP];
     int cnt[NN_CA];
 };
+
+// r and 1/r of a neighbour entry (from its vector: one reciprocal square root)
+__device__ __forceinline__ void nbr_rir(const ChunkNbr &L, int e, float *r, float *ir) {
+    const float x = L.dx[e], y = L.dy[e], z = L.dz[e];
+    const float r2 = x * x + y * y + z * z;
+    *ir = __builtin_amdgcn_rsqf(r2);
+    *r = r2 * *ir;
+}
 
 }  // namespace
 
@@ -190,8 +198,6 @@ __device__ void build_chunk(float rc2, float pi_rc, ChunkNbr &L, const double *f
                                     L.dx[e] = (float)ex;
                                     L.dy[e] = (float)ey;
                                     L.dz[e] = (float)ez;
-                                    L.r[e] = rr;
-                                    L.ir[e] = 1.0f / rr;
                                     L.fc[e] = fcv;
                                     L.dfc[e] = dfcv;
                                     L.j[e] = (uint16_t)n;
@@ -347,7 +353,8 @@ __device__ void mlp_tile(const NnConst &M, int e, const int *rows, const float *
 
 // Symmetry functions of the chunk's centre atoms (16 lanes each) into
 // Grep rows; *extrap set when a value leaves [min - 1e-4, max + 1e-4].
-__device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem, int N, int k0, int lane,
+__device__ void g_chunk(const NnConst &Mc, const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem, int N, int k0,
+                        int lane,
                         float rc2, float pi_rc, float *Grep, int *extrapp) {
     const int c = lane / NN_TPA, sub = lane % NN_TPA;
     int extrap = 0;
@@ -364,7 +371,9 @@ __device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem
 #pragma unroll
     for (int k = 0; k < NN_MAXR; ++k) g[k] = 0.f;
     for (int p = sub; p < cnt; p += NN_TPA) {
-        const float r = L.r[base + p], fcv = L.fc[base + p];
+        float r, irp;
+        nbr_rir(L, base + p, &r, &irp);
+        const float fcv = L.fc[base + p];
         const int ep = L.el[base + p];
 #pragma unroll
         for (int k = 0; k < NN_MAXR; ++k)
@@ -378,7 +387,7 @@ __device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem
         const float v = sum_group<NN_TPA>(g[k]);
         if (i < N && k < nr && sub == (k & (NN_TPA - 1))) {
             Grep[(size_t)i * NN_K1 + k] = v;
-            if (v < S.glo[e][k] || v > S.ghi[e][k]) extrap = 1;
+            if (v < Mc.glo[e][k] || v > Mc.ghi[e][k]) extrap = 1;
         }
     }
     // angular (atomsymfunction3Andi): pairs p < q of the neighbour list
@@ -399,7 +408,9 @@ __device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem
         for (int q = 0; q < NN_FB * NN_FB; ++q) acc[q] = 0.f;
         for (int p = sub; p < cnt; p += NN_TPA) {
             const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
-            const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p];
+            float ra, ia;
+            nbr_rir(L, base + p, &ra, &ia);
+            const float fa = L.fc[base + p];
             const int ep = L.el[base + p];
             for (int q = p + 1; q < cnt; ++q) {
                 const float bx = L.dx[base + q], by = L.dy[base + q], bz = L.dz[base + q];
@@ -409,8 +420,9 @@ __device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem
                 const int eq = L.el[base + q];
                 const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
                 if (cls != fcls) continue;
-                const float rb = L.r[base + q];
-                const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * L.ir[base + q];
+                const float rb2 = bx * bx + by * by + bz * bz;
+                const float ib = __builtin_amdgcn_rsqf(rb2), rb = rb2 * ib;
+                const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * ib;
                 float fcc, dfcc;
                 cutoff(M.pi_rc, sqrtf(rc2v), &fcc, &dfcc);
                 const float Cf = fa * L.fc[base + q] * fcc;
@@ -445,7 +457,7 @@ __device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem
             if (k < na && sub == (k & (NN_TPA - 1))) {
                 const float g4 = v * S.a_norm[e][k];
                 Grep[(size_t)i * NN_K1 + nr + k] = g4;
-                if (g4 < S.glo[e][nr + k] || g4 > S.ghi[e][nr + k]) extrap = 1;
+                if (g4 < Mc.glo[e][nr + k] || g4 > Mc.ghi[e][nr + k]) extrap = 1;
             }
         }
     } else {
@@ -454,17 +466,20 @@ __device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem
     for (int k = 0; k < NN_MAXA; ++k) g[k] = 0.f;
     for (int p = sub; p < cnt; p += NN_TPA) {
         const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
-        const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p];
+        float ra, ia;
+        nbr_rir(L, base + p, &ra, &ia);
+        const float fa = L.fc[base + p];
         const int ep = L.el[base + p];
         for (int q = p + 1; q < cnt; ++q) {
             const float bx = L.dx[base + q], by = L.dy[base + q], bz = L.dz[base + q];
             const float cx = bx - ax_, cy = by - ay_, cz = bz - az_;
             const float rc2v = cx * cx + cy * cy + cz * cz;
             if (rc2v > M.rc2) continue;
-            const float rb = L.r[base + q];
+            const float rb2 = bx * bx + by * by + bz * bz;
+            const float ib = __builtin_amdgcn_rsqf(rb2), rb = rb2 * ib;
             const int eq = L.el[base + q];
             const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
-            const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * L.ir[base + q];
+            const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * ib;
             float fcc, dfcc;
             cutoff(M.pi_rc, sqrtf(rc2v), &fcc, &dfcc);
             const float Cf = fa * L.fc[base + q] * fcc;
@@ -491,7 +506,7 @@ __device__ void g_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *elem
         const float v = sum_group<NN_TPA>(g[k]);
         if (i < N && k < na && sub == (k & (NN_TPA - 1))) {
             Grep[(size_t)i * NN_K1 + nr + k] = v;
-            if (v < S.glo[e][nr + k] || v > S.ghi[e][nr + k]) extrap = 1;
+            if (v < Mc.glo[e][nr + k] || v > Mc.ghi[e][nr + k]) extrap = 1;
         }
     }
     }
@@ -517,7 +532,9 @@ __device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*c
     float Fi0 = 0.f, Fi1 = 0.f, Fi2 = 0.f;  // force on the centre (Hartree/Bohr)
     // radial: dG/dr = (-2 eta (r - rs) fc + dfc) exp(-eta (r - rs)^2)
     for (int p = sub; p < cnt; p += NN_TPA) {
-        const float r = L.r[base + p], fcv = L.fc[base + p], dfcv = L.dfc[base + p];
+        float r, ir;
+        nbr_rir(L, base + p, &r, &ir);
+        const float fcv = L.fc[base + p], dfcv = L.dfc[base + p];
         const int ep = L.el[base + p];
         float s = 0.f;
 #pragma unroll
@@ -527,7 +544,6 @@ __device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*c
                 const float ex = __builtin_amdgcn_exp2f(-S.r_eta[e][k] * NN_L2E * dr * dr);
                 s += cb[c][k] * ex * (dfcv - 2.f * S.r_eta[e][k] * dr * fcv);
             }
-        const float ir = L.ir[base + p];
         const float v0 = s * L.dx[base + p] * ir, v1 = s * L.dy[base + p] * ir, v2 = s * L.dz[base + p] * ir;
         Fi0 += v0;
         Fi1 += v1;
@@ -569,7 +585,9 @@ __device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*c
     }
     for (int p = sub; p < cnt; p += NN_TPA) {
         const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
-        const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p], dfa = L.dfc[base + p];
+        float ra, ia;
+        nbr_rir(L, base + p, &ra, &ia);
+        const float fa = L.fc[base + p], dfa = L.dfc[base + p];
         const int ep = L.el[base + p];
         float Fj0 = 0.f, Fj1 = 0.f, Fj2 = 0.f;
         for (int q = p + 1; q < cnt; ++q) {
@@ -580,7 +598,9 @@ __device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*c
             const int eq = L.el[base + q];
             const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
             if (fast && cls != fcls) continue;
-            const float rb = L.r[base + q], ib = L.ir[base + q], fb = L.fc[base + q], dfb = L.dfc[base + q];
+            const float rb2 = bx * bx + by * by + bz * bz;
+            const float ib = __builtin_amdgcn_rsqf(rb2), rb = rb2 * ib;
+            const float fb = L.fc[base + q], dfb = L.dfc[base + q];
             const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * ib;
             const float rcc = sqrtf(rc2v);
             float fcc, dfcc;
@@ -701,7 +721,9 @@ __device__ void jvp_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *el
 #pragma unroll
     for (int k = 0; k < NN_MAXR; ++k) P[k] = 0.f;
     for (int p = sub; p < cnt; p += NN_TPA) {
-        const float r = L.r[base + p], fcv = L.fc[base + p], dfcv = L.dfc[base + p], ir = L.ir[base + p];
+        float r, ir;
+        nbr_rir(L, base + p, &r, &ir);
+        const float fcv = L.fc[base + p], dfcv = L.dfc[base + p];
         const int j = L.j[base + p];
         const float proj = (L.dx[base + p] * (gx[j] - gix) + L.dy[base + p] * (gy[j] - giy) +
                             L.dz[base + p] * (gz[j] - giz)) * ir;
@@ -723,7 +745,9 @@ __device__ void jvp_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *el
     for (int k = 0; k < NN_MAXA; ++k) P[k] = 0.f;
     for (int p = sub; p < cnt; p += NN_TPA) {
         const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
-        const float ra = L.r[base + p], ia = L.ir[base + p], fa = L.fc[base + p], dfa = L.dfc[base + p];
+        float ra, ia;
+        nbr_rir(L, base + p, &ra, &ia);
+        const float fa = L.fc[base + p], dfa = L.dfc[base + p];
         const int ep = L.el[base + p];
         const int jp = L.j[base + p];
         const float Dja0 = gx[jp] - gix, Dja1 = gy[jp] - giy, Dja2 = gz[jp] - giz;
@@ -732,7 +756,9 @@ __device__ void jvp_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *el
             const float cx = bx - ax_, cy = by - ay_, cz = bz - az_;
             const float rc2v = cx * cx + cy * cy + cz * cz;
             if (rc2v > rc2) continue;
-            const float rb = L.r[base + q], ib = L.ir[base + q], fb = L.fc[base + q], dfb = L.dfc[base + q];
+            const float rb2 = bx * bx + by * by + bz * bz;
+            const float ib = __builtin_amdgcn_rsqf(rb2), rb = rb2 * ib;
+            const float fb = L.fc[base + q], dfb = L.dfc[base + q];
             const int eq = L.el[base + q];
             const int cls = ep < eq ? ep * NN_MAXE + eq : eq * NN_MAXE + ep;
             const int kq = L.j[base + q];
@@ -783,7 +809,8 @@ __device__ void jvp_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *el
     }
 }
 
-// LDS copy of the symmetry-function tables and extrapolation bounds
+// LDS copy of the symmetry-function tables (the extrapolation bounds are read
+// from the constants, once per atom)
 __device__ void load_sftab(NnSfTab &S, const NnConst &M, int lane) {
     for (int k = lane; k < NN_MAXE; k += NN_BLOCK) {
         S.nr[k] = M.nr[k];
@@ -803,11 +830,6 @@ __device__ void load_sftab(NnSfTab &S, const NnConst &M, int lane) {
         S.a_lam[e][q] = M.a_lam[e][q];
         S.a_norm[e][q] = M.a_norm[e][q];
         S.a_zp[e][q] = M.a_zp[e][q];
-    }
-    for (int k = lane; k < NN_MAXE * NN_K1; k += NN_BLOCK) {
-        const int e = k / NN_K1, q = k % NN_K1;
-        S.glo[e][q] = M.glo[e][q];
-        S.ghi[e][q] = M.ghi[e][q];
     }
     // factored angular tables, one lane per element (from M: no dependence
     // on the copies above)
@@ -879,8 +901,13 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__
                                                      int skip_extrap) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ NnSfTab S;
-    __shared__ ChunkNbr L;
-    __shared__ float T[NN_BLOCK / 64][16 * 33];
+    // the neighbour cache of pass G and the network's transposes share LDS
+    __shared__ union {
+        ChunkNbr L;
+        float T[NN_BLOCK / 64][16 * 33];
+    } U;
+    ChunkNbr &L = U.L;
+    float(*T)[16 * 33] = U.T;
     __shared__ int rowsb[NN_BLOCK / 64][16];
     __shared__ int s_ne[NN_MAXE];
     const NnConst &M = *Mg;
@@ -947,7 +974,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__
     for (int k0 = 0; k0 < N; k0 += NN_CA) {
         build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
         __syncthreads();
-        g_chunk(S, L, elem, N, k0, lane, M.rc2, M.pi_rc, Grep, &extrap);
+        g_chunk(M, S, L, elem, N, k0, lane, M.rc2, M.pi_rc, Grep, &extrap);
         __syncthreads();
     }
 
@@ -1149,7 +1176,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nn_frame(const NnConst *__restrict
         for (int k0 = 0; k0 < N; k0 += NN_CA) {
             build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
             __syncthreads();
-            g_chunk(S, L, elem, N, k0, lane, M.rc2, M.pi_rc, Grow, &extrap);
+            g_chunk(M, S, L, elem, N, k0, lane, M.rc2, M.pi_rc, Grow, &extrap);
             __syncthreads();
         }
         return;
