@@ -602,7 +602,7 @@ __device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*c
             const float ib = __builtin_amdgcn_rsqf(rb2), rb = rb2 * ib;
             const float fb = L.fc[base + q], dfb = L.dfc[base + q];
             const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * ib;
-            const float rcc = sqrtf(rc2v);
+            const float ircc = __builtin_amdgcn_rsqf(rc2v), rcc = rc2v * ircc;
             float fcc, dfcc;
             cutoff(M.pi_rc, rcc, &fcc, &dfcc);
             const float Cf = fa * fb * fcc;
@@ -668,7 +668,7 @@ __device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*c
             const float iab = ia * ib;
             const float tc = Cf * Sc, ts = -2.f * Cf * Ss;
             const float ua = Sg * dfa * fb * fcc * ia, ub = Sg * fa * dfb * fcc * ib;
-            const float uc = Sg * fa * fb * dfcc / rcc;
+            const float uc = Sg * fa * fb * dfcc * ircc;
             const float ga0 = tc * (bx * iab - cth * ax_ * ia * ia) + ts * ax_ + ua * ax_;
             const float ga1 = tc * (by * iab - cth * ay_ * ia * ia) + ts * ay_ + ua * ay_;
             const float ga2 = tc * (bz * iab - cth * az_ * ia * ia) + ts * az_ + ua * az_;
@@ -765,7 +765,7 @@ __device__ void jvp_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *el
             const float Dka0 = gx[kq] - gix, Dka1 = gy[kq] - giy, Dka2 = gz[kq] - giz;
             const float Dkj0 = Dka0 - Dja0, Dkj1 = Dka1 - Dja1, Dkj2 = Dka2 - Dja2;
             const float cth = (ax_ * bx + ay_ * by + az_ * bz) * ia * ib;
-            const float rcc = sqrtf(rc2v);
+            const float ircc = __builtin_amdgcn_rsqf(rc2v), rcc = rc2v * ircc;
             float fcc, dfcc;
             cutoff(pi_rc, rcc, &fcc, &dfcc);
             const float Cf = fa * fb * fcc;
@@ -778,7 +778,7 @@ __device__ void jvp_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *el
             const float aDb = ax_ * Dka0 + ay_ * Dka1 + az_ * Dka2;  // a . Dka
             const float dcosD = (bDa * iab - cth * aD * ia * ia) + (aDb * iab - cth * bD * ib * ib);
             const float SD = aD + bD + cD;
-            const float FD = dfa * fb * fcc * aD * ia + fa * dfb * fcc * bD * ib + fa * fb * dfcc * cD / rcc;
+            const float FD = dfa * fb * fcc * aD * ia + fa * dfb * fcc * bD * ib + fa * fb * dfcc * cD * ircc;
             float B = 0.f, etaprev = -1.f;
 #pragma unroll
             for (int k = 0; k < NN_MAXA; ++k)
