@@ -294,6 +294,14 @@ int mdt_batch_set_pes(mdt_batch *b, int pes, int judge_interval);
  * last force evaluation. */
 int mdt_batch_nn_stats(mdt_batch *b, double *judge_sq, double *judge_n, double *judge_max, int32_t *nfallback);
 
+/*
+ * Per replica (trajectories in order, beads within), 1 when the last force
+ * evaluation found a symmetry function outside the scaling range and the
+ * replica took the EMT fallback: the per-replica form of the extrapolation
+ * warnings compute_nene prints (src/pes_nene_mod.f90:7260-7261).
+ */
+int mdt_batch_nn_fallback(mdt_batch *b, int32_t *flags);
+
 /* Raw symmetry functions of the last network evaluation, float, nrep x maxN x
  * 64 (replicas in batch order, trajectory-major; unused entries
  * unspecified): the training set's descriptors and scaling statistics.

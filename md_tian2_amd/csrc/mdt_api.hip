@@ -1499,6 +1499,16 @@ int mdt_batch_nn_stats(mdt_batch *b, double *judge_sq, double *judge_n, double *
     return 0;
 }
 
+int mdt_batch_nn_fallback(mdt_batch *b, int32_t *flags) {
+    if (!b || !flags) return fail(MDT_EINVAL, "mdt_batch_nn_fallback: NULL argument");
+    if (!b->d_nnflag) return fail(MDT_ESTATE, "the network has not run on this batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    std::vector<int> f(b->nrep);
+    HIPCHK(hipMemcpy(f.data(), b->d_nnflag, sizeof(int) * f.size(), hipMemcpyDeviceToHost));
+    for (int r = 0; r < b->nrep; ++r) flags[r] = f[r] != 0;
+    return 0;
+}
+
 int mdt_batch_nn_descriptors(mdt_batch *b, float *G) {
     if (!b || !G) return fail(MDT_EINVAL, "mdt_batch_nn_descriptors: NULL argument");
     if (!b->d_nnG) return fail(MDT_ESTATE, "the network has not run on this batch");

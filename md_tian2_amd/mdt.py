@@ -29,7 +29,7 @@ SYMBOLS = (
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
     "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph", "mdt_batch_set_bead_mode",
-    "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats",
+    "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats", "mdt_batch_nn_fallback",
     "mdt_batch_nn_descriptors", "mdt_nn_train_create", "mdt_nn_train_destroy", "mdt_nn_train_descriptor_stats",
     "mdt_nn_train_epoch", "mdt_nn_train_eval", "mdt_nn_train_loss_grad", "mdt_nn_get_weights", "mdt_nn_load",
 )
@@ -77,6 +77,7 @@ def lib():
         L.mdt_compute_nene.argtypes = [vp, C.c_int, C.c_int, ip, dp, dp, dp, dp, dp, ip, C.c_int32]
         L.mdt_batch_set_pes.argtypes = [vp, C.c_int, C.c_int]
         L.mdt_batch_nn_stats.argtypes = [vp, dp, dp, dp, ip]
+        L.mdt_batch_nn_fallback.argtypes = [vp, ip]
         L.mdt_batch_nn_descriptors.argtypes = [vp, C.POINTER(C.c_float)]
         L.mdt_nn_train_create.argtypes = [vp, C.c_int, ip, ip, dp, dp, dp, dp, C.c_int, C.c_double, C.c_uint32,
                                           C.POINTER(vp)]
@@ -275,6 +276,16 @@ class Batch:
         nf = C.c_int32(0)
         _check(lib().mdt_batch_nn_stats(self.h, C.byref(sq), C.byref(n), C.byref(mx), C.byref(nf)))
         return (np.sqrt(sq.value / n.value) if n.value else None), n.value, mx.value, nf.value
+
+    def nn_fallback(self):
+        """Per trajectory (P,) bool: replicas that took the EMT fallback in the last force evaluation."""
+        f = np.zeros(int(self.P.sum()), dtype=np.int32)
+        _check(lib().mdt_batch_nn_fallback(self.h, _ip(f)))
+        out, o = [], 0
+        for p in self.P:
+            out.append(f[o:o + p].astype(bool))
+            o += p
+        return out
 
     def nn_descriptors(self):
         """Raw symmetry functions of the last network evaluation: per trajectory (P, N, 64) float32."""
