@@ -20,13 +20,18 @@
 ! fallback on extrapolation) and judge <n> (EMT judge of the network on a
 ! rotating 1 % of the trajectories every n steps), beads <all|mixed> (with
 ! force_beads > 1: every atom carries the beads, as the reference, or only
-! the projectile, the slab staying classical).
+! the projectile, the slab staying classical).  conf mxt <file> | mxt
+! <folder> <n> | poscar <file> | merge <proj folder> <n> <lattice folder>
+! <m> reads the trajectories' universes instead of `slab` (conf_io).
+! `mdt_run --check-input <deck>` writes every trajectory's initial universe
+! as <dump>_NNNNNNNN.mxt and stops before any GPU call.
 program mdt_run
 
     use, intrinsic :: iso_c_binding
     use mdt_gpu
     use universe_mod
     use pes_emt_gpu_mod, only : emt_gpu_init, emt_gpu_context, emt_params_from_file
+    use conf_io, only : read_mxt, write_mxt, read_poscar, merge_conf, mxt_file, invert_cell
 
     implicit none
 
@@ -44,6 +49,12 @@ program mdt_run
     character(len=3) :: name_proj = "H", name_latt = "Au"
     character(len=512) :: pes_file = "", dump_file = "", deck = "", nene_dir = ""
     integer :: prec = 0, graph_chunk = 0, judge_interval = 0, bead_mode = 0
+    ! conf <mxt|poscar|merge> (src/run_config.f90:440-500)
+    character(len=8) :: conf_kind = ""
+    character(len=512) :: conf_path = "", proj_path = ""
+    integer :: conf_n = 0, proj_n = 0
+    logical :: einc_set = .false., pip_set = .false., check_input = .false.
+    character(len=512) :: arg
     real(dp) :: jsq, jn, jmax
     integer(c_int32_t) :: nfb
 
@@ -59,15 +70,42 @@ program mdt_run
     integer :: u
 
     if (command_argument_count() < 1) then
-        print '(a)', "usage: mdt_run <deck>"
+        print '(a)', "usage: mdt_run [--check-input] <deck>"
         stop 2
     end if
-    call get_command_argument(1, deck)
+    do i = 1, command_argument_count()
+        call get_command_argument(i, arg)
+        if (trim(arg) == "--check-input") then
+            check_input = .true.
+        else
+            deck = arg
+        end if
+    end do
     call read_deck(trim(deck))
 
-    natoms = 1 + nx * ny * nlayers
-    tmpl = universe_alloc(nbeads, natoms, 2)
-    call build_template(tmpl)
+    if (len_trim(conf_kind) > 0) then
+        call conf_universe(tmpl, start)
+        natoms = tmpl%natoms
+        nbeads = tmpl%nbeads
+    else
+        natoms = 1 + nx * ny * nlayers
+        tmpl = universe_alloc(nbeads, natoms, 2)
+        call build_template(tmpl)
+    end if
+    if (check_input) then
+        ! the initial universe of every trajectory as <dump>_NNNNNNNN.mxt; no GPU
+        do t = 1, ntrajs
+            call initial_conditions(tmpl, start + t - 1)
+            write(arg, '(a, a, i8.8, a)') trim(dump_file), "_", start + t - 1, ".mxt"
+            call write_mxt(tmpl, trim(arg), i)
+            if (i /= 0) then
+                print '(a,a)', "Error: cannot write ", trim(arg)
+                stop 1
+            end if
+        end do
+        print '(a,i0,a)', "mdt_run: initial universes of ", ntrajs, " trajectories written"
+        stop
+    end if
     call emt_gpu_init(0)
     ctx = emt_gpu_context()
     call emt_params_from_file(tmpl, trim(pes_file), eta2, n0, e0, lambda, v0p, kappa, s0)
@@ -181,10 +219,10 @@ contains
             case ("start"); read(w(2), *) start
             case ("nsteps"); read(w(2), *) nsteps
             case ("step"); read(w(2), *) dt
-            case ("Einc"); read(w(2), *) einc
+            case ("Einc"); read(w(2), *) einc; einc_set = .true.
             case ("polar"); read(w(2), *) polar; polar = polar * pi / 180.0_dp
             case ("azimuth"); azimuth = w(2)
-            case ("pip"); pip_x = w(2); pip_y = w(3); read(w(4), *) pip_z
+            case ("pip"); pip_x = w(2); pip_y = w(3); read(w(4), *) pip_z; pip_set = .true.
             case ("Tsurf"); read(w(2), *) tsurf
             case ("Tproj"); read(w(2), *) tproj
             case ("pile_tau"); read(w(2), *) pile_tau
@@ -209,6 +247,25 @@ contains
                     print '(a,a)', "Error: beads must be all or mixed, got ", trim(w(2))
                     stop 1
                 end if
+            case ("conf")
+                conf_kind = w(2)
+                select case (trim(conf_kind))
+                case ("mxt", "poscar")
+                    conf_path = unquote(w(3))
+                    if (nw >= 4) read(w(4), *) conf_n   ! a folder of mxt_NNNNNNNN.dat files
+                case ("merge")
+                    if (nw /= 6) then
+                        print '(a)', "Error: conf merge needs projectile folder, count, lattice folder, count"
+                        stop 1
+                    end if
+                    proj_path = unquote(w(3))
+                    read(w(4), *) proj_n
+                    conf_path = unquote(w(5))
+                    read(w(6), *) conf_n
+                case default
+                    print '(a,a)', "Error: conf must be mxt, poscar or merge, got ", trim(conf_kind)
+                    stop 1
+                end select
             case ("precision")
                 if (trim(w(2)) == "fp32_pair") then
                     prec = 1
@@ -347,6 +404,10 @@ contains
         integer :: nseed, k, c
         integer, allocatable :: seedv(:)
         real(dp) :: fx, fy, phi, vinc, sig, u1, u2
+        if (len_trim(conf_kind) > 0) then
+            call conf_universe(u, traj)
+            return
+        end if
         call build_template(u)
         call random_seed(size=nseed)
         allocate(seedv(nseed))
@@ -385,5 +446,124 @@ contains
             u%r(:, :, k) = spread(u%r(:, 1, k), 2, nbeads)
         end do
     end subroutine initial_conditions
+
+    ! The universe of trajectory `traj` from the conf files (read_geometry,
+    ! src/md_init.f90:160-192, and post_process :630-671): the file, or file
+    ! k of the folder with k uniform in 1..n (src/run_config.f90:466-496;
+    ! drawn from this driver's per-trajectory host stream, not the
+    ! reference's), the projectile merged first for `conf merge` and put at
+    ! `pip` (x, y: "r" uniform over the cell, else cartesian; z cartesian),
+    ! its bead-centroid velocity replaced by the incidence velocity when
+    ! Einc is given (set_proj_incidence, :829-877), `force_beads` copies of
+    ! bead 1, masses and propagators from the projectile / lattice lines.
+    ! Velocities of the lattice are the file's.
+    subroutine conf_universe(u, traj)
+        type(universe), intent(inout) :: u
+        integer, intent(in) :: traj
+        type(universe) :: latt, proj, one
+        integer :: nseed, ios, b, i
+        integer, allocatable :: seedv(:)
+        real(dp) :: x, fx, fy, phi, vinc, cen(3), target(3), vc(3)
+        call random_seed(size=nseed)
+        allocate(seedv(nseed))
+        seedv = traj * 7919 + 29
+        call random_seed(put=seedv)
+        select case (trim(conf_kind))
+        case ("poscar")
+            call read_poscar(one, trim(conf_path), 1, ios)
+            call conf_check(ios, conf_path)
+        case ("mxt")
+            if (conf_n > 0) then
+                call random_number(x)
+                call read_mxt(one, mxt_file(trim(conf_path), int(x * conf_n) + 1), ios)
+            else
+                call read_mxt(one, trim(conf_path), ios)
+            end if
+            call conf_check(ios, conf_path)
+        case ("merge")
+            call random_number(x)
+            call read_mxt(proj, mxt_file(trim(proj_path), int(x * proj_n) + 1), ios)
+            call conf_check(ios, proj_path)
+            call random_number(x)
+            call read_mxt(latt, mxt_file(trim(conf_path), int(x * conf_n) + 1), ios)
+            call conf_check(ios, conf_path)
+            if (pip_set) then  ! the projectile's centroid to the pip target
+                call random_number(fx)
+                call random_number(fy)
+                target = 0.0_dp
+                if (trim(pip_x) == "r") then
+                    target = target + fx * latt%simbox(:, 1)
+                else
+                    read(pip_x, *) target(1)
+                end if
+                if (trim(pip_y) == "r") then
+                    target = target + fy * latt%simbox(:, 2)
+                else
+                    read(pip_y, *) target(2)
+                end if
+                target(3) = pip_z
+                cen = sum(proj%r(:, :, 1), dim=2) / proj%nbeads
+                do b = 1, proj%nbeads
+                    proj%r(:, b, 1) = proj%r(:, b, 1) - cen + target
+                end do
+            end if
+            call merge_conf(latt, proj, one, ios)
+            if (ios /= 0) then
+                print '(a,i0)', "Error: conf merge: incompatible files, code ", ios
+                stop 1
+            end if
+        end select
+        if (one%ntypes /= 2 .or. .not. one%is_proj(1) .or. one%is_proj(2) .or. count(one%idx == 1) /= 1) then
+            print '(a)', "Error: the driver runs one projectile atom (type 1) over one lattice type (type 2)"
+            stop 1
+        end if
+        if (nbeads > 1 .and. one%nbeads == 1) then  ! force_beads
+            if (trim(conf_kind) == "mxt") then
+                print '(a)', "Error: cannot force beads on mxt files"   ! as src/md_init.f90:428
+                stop 1
+            end if
+            u = universe_alloc(nbeads, one%natoms, one%ntypes)
+            do b = 1, nbeads
+                u%r(:, b, :) = one%r(:, 1, :)
+                u%v(:, b, :) = one%v(:, 1, :)
+                u%is_fixed(:, b, :) = one%is_fixed(:, 1, :)
+            end do
+            u%idx = one%idx
+            u%name = one%name
+            u%is_proj = one%is_proj
+            u%simbox = one%simbox
+            u%isimbox = one%isimbox
+            u%is_cart = .true.
+        else
+            u = one
+        end if
+        u%pes = 0
+        do i = 1, u%natoms
+            u%m(i) = merge(mproj, mlatt, u%idx(i) == 1) * amu2mass
+        end do
+        if (einc_set) then
+            if (trim(azimuth) == "r") then
+                call random_number(phi)
+                phi = 2.0_dp * pi * phi
+            else
+                read(azimuth, *) phi
+                phi = phi * pi / 180.0_dp
+            end if
+            vinc = sqrt(2.0_dp * einc / u%m(1))
+            vc = sum(u%v(:, :, 1), dim=2) / u%nbeads
+            do b = 1, u%nbeads
+                u%v(:, b, 1) = u%v(:, b, 1) - vc + vinc * [sin(polar) * cos(phi), sin(polar) * sin(phi), -cos(polar)]
+            end do
+        end if
+    end subroutine conf_universe
+
+    subroutine conf_check(ios, path)
+        integer, intent(in) :: ios
+        character(len=*), intent(in) :: path
+        if (ios /= 0) then
+            print '(a,a,a,i0)', "Error: cannot read configuration ", trim(path), ", code ", ios
+            stop 1
+        end if
+    end subroutine conf_check
 
 end program mdt_run

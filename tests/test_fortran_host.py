@@ -29,3 +29,58 @@ def test_driver_fails_loudly_without_gpu():
                          capture_output=True, text=True, timeout=60)
     assert res.returncode != 0
     assert "no GPU visible" in res.stdout + res.stderr
+
+
+def _check_input(tmp_path, conf_line, extra=""):
+    deck = tmp_path / "deck.inp"
+    deck.write_text("\n".join([
+        "run md", "start 3", "ntrajs 2", "nsteps 10", "step 0.1",
+        "projectile 1 H 1.00794 pil", "lattice 1 C 12.0 ver", "Tsurf 300", "Tproj 300",
+        conf_line, f"dump '{tmp_path}/init'", extra]) + "\n")
+    res = subprocess.run([EXE, "--check-input", str(deck)], cwd=ROOT, capture_output=True, text=True,
+                         timeout=60)
+    assert res.returncode == 0, res.stdout + res.stderr
+    from md_tian2_amd import conf
+    return [conf.read_mxt(tmp_path / ("init_%08d.mxt" % t)) for t in (3, 4)]
+
+
+def test_driver_conf_merge_matches_python_reader(tmp_path):
+    """conf merge of the reference's H and graphene files with pip, Einc,
+    polar, azimuth and force_beads: the driver's initial universes (written
+    by its own mxt writer, no GPU) against the Python reader and merge."""
+    import numpy as np
+    from md_tian2_amd import conf, geometry
+    fix = os.path.join(ROOT, "tests", "golden", "conf")
+    out = _check_input(tmp_path, f"conf merge '{fix}/proj' 1 '{fix}/latt' 1",
+                       "force_beads 4\nEinc 1.92\npolar 50\nazimuth 30\npip r r 6.0")
+    ref = conf.merge(conf.read_mxt(os.path.join(fix, "latt", "mxt_00000001.dat")),
+                     conf.read_mxt(os.path.join(fix, "proj", "mxt_00000001.dat")))
+    m_h = geometry.MASS_AMU["H"] * geometry.AMU2MASS
+    vinc = np.sqrt(2 * 1.92 / m_h)
+    th, ph = np.radians(50.0), np.radians(30.0)
+    vexp = vinc * np.array([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), -np.cos(th)])
+    for c in out:
+        assert (c.natoms, c.nbeads) == (25, 4) and c.names == ["H", "C"] and c.is_proj == [True, False]
+        assert list(c.idx) == list(ref.idx)
+        # lattice: the file's positions folded as merge_universes does, in every bead
+        assert np.abs(c.r[1:] - ref.r[1:, :1]).max() < 1e-10
+        assert np.array_equal(c.v[1:], np.repeat(ref.v[1:, :1], 4, axis=1))
+        assert np.array_equal(c.fixed[1:], np.repeat(ref.fixed[1:, :1], 4, axis=1))
+        # projectile: z at pip, x y inside the cell, incidence velocity
+        assert np.allclose(c.r[0, :, 2], 6.0)
+        f = np.linalg.solve(c.simbox, c.r[0, 0])
+        assert 0 <= f[0] < 1 and 0 <= f[1] < 1
+        assert np.allclose(c.v[0], vexp, rtol=1e-12)
+    assert not np.array_equal(out[0].r[0], out[1].r[0])  # x, y drawn per trajectory
+
+
+def test_driver_conf_poscar_matches_python_reader(tmp_path):
+    import numpy as np
+    from md_tian2_amd import conf
+    fix = os.path.join(ROOT, "tests", "golden", "conf")
+    out = _check_input(tmp_path, f"conf poscar '{fix}/example.POSCAR'")
+    ref = conf.read_poscar(os.path.join(fix, "example.POSCAR"), nprojectiles=1)
+    for c in out:
+        assert (c.natoms, c.nbeads) == (25, 1) and c.names == ["H", "C"]
+        assert np.array_equal(c.r, ref.cartesian()) and np.array_equal(c.fixed, ref.fixed)
+        assert np.allclose(c.simbox, ref.simbox, rtol=1e-15)

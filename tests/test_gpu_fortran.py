@@ -147,3 +147,34 @@ def test_fortran_driver_mixed_beads_matches_library(tmp_path):
         assert np.abs(d["epot"][t] - o["epot"]).max() < 1e-8
         # the slab is classical: every bead reports the same positions
         assert np.array_equal(d["r1"][t][1:], np.repeat(d["r1"][t][1:, :1], 8, axis=1))
+
+
+def test_fortran_driver_conf_merge_matches_oracle(tmp_path, pes_hau):
+    """The driver started from `conf merge` of mxt files (H over a 3x3x4 Au
+    lattice with its own velocities, tests/golden/conf_hau): the final
+    states against the oracle run from the dumped initial state."""
+    deck = open(os.path.join(ROOT, "tests", "decks", "h_au_3x3x4.inp")).read()
+    dump = tmp_path / "conf.bin"
+    deck = deck.replace("dump 'gpurun_out/mdt_run_h_au.bin'", f"dump '{dump}'")
+    deck = deck.replace("slab 4.078 3 3 4 2", "conf merge 'tests/golden/conf_hau/proj' 1 'tests/golden/conf_hau/latt' 1")
+    p = tmp_path / "conf.inp"
+    p.write_text(deck)
+    exe = os.path.join(ROOT, "md_tian2_amd", "fortran", "mdt_run")
+    res = subprocess.run([exe, str(p)], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr
+    d = read_dump(str(dump))
+    tmpl = geometry.h_on_slab("Au", n=3, nlayers=4, nfixed=2, thermal=False)
+    cols, pesm = pesfile.emt_tables(pes_hau, tmpl.type_names)
+    from md_tian2_amd import conf
+    latt = conf.read_mxt(os.path.join(ROOT, "tests", "golden", "conf_hau", "latt", "mxt_00000001.dat"))
+    for t in range(d["ntraj"]):
+        assert np.array_equal(d["v0"][t][1:], latt.v)  # the lattice keeps the file's velocities
+        s = tmpl.copy()
+        s.r = d["r0"][t].copy()
+        s.v = d["v0"][t].copy()
+        st = oracle.OracleState(s, cols, pesm, step=0.1, tsurf=300.0, tproj=0.0)
+        st.calc_force()
+        st.run(777, int(d["tid"][t]), 1, int(d["nsteps"]))
+        assert np.abs(d["r1"][t] - st.r).max() < 1e-9
+        assert np.abs(d["v1"][t] - st.v).max() < 1e-9
