@@ -221,6 +221,69 @@ int mdt_batch_set_precision(mdt_batch *b, int mode);
 int mdt_batch_set_bead_mode(mdt_batch *b, int mode);
 
 /*
+ * Trajectory observation: collect_trajectory_characteristics
+ * (src/trajectory_info.f90:47-263) after each step's propagate_2, and the
+ * early end of a trajectory (src/md_tian2.f90:88-92).  The call resets the
+ * observables of every trajectory; from the next step on, per trajectory:
+ * the closest centroid distance of the projectile (atom 1) to any other
+ * atom (minimum image) and the time [fs] it was reached, the projectile
+ * centroid at its lowest z, the time with a nonzero force on the
+ * projectile, the sign changes of its centroid z velocity, and the
+ * adsorption status (set below ads_start, cleared above ads_end).  With
+ * exit_at_limits, a trajectory whose projectile beads are all above
+ * proj_ul (`pul`) or all below proj_ll (`pll`) after a step is frozen: no
+ * further propagation or forces; its state stays that of the exit step.
+ * Defaults of the reference: proj_ul = DBL_MAX, proj_ll = -DBL_MAX,
+ * ads_start = ads_end = DBL_MAX.
+ */
+int mdt_batch_set_observe(mdt_batch *b, double proj_ul, double proj_ll, double ads_start, double ads_end,
+                          int exit_at_limits);
+
+/* Observables of one trajectory; istep counts the steps from the
+ * mdt_batch_set_observe call, 1-based as the reference's istep. */
+typedef struct mdt_traj_obs {
+    double cl_appr;          /* closest_approach [A] */
+    double cl_last;          /* the closest distance at the last observed step [A] */
+    double r_min[3];         /* lowest_z: projectile centroid at its lowest z [A] */
+    double t_int;            /* interaction_time [fs] */
+    int32_t turning_points;  /* nturning_points */
+    int32_t moving_down;     /* was_moving_down */
+    int32_t adsorbed;        /* is_adsorbed */
+    int32_t adsorbed_prev;   /* is_adsorbed before the last observed step */
+    int32_t exit_step;       /* istep after which the trajectory was frozen, 0 while it runs */
+    int32_t cl_appr_step;    /* istep of the closest approach (closest_approach_time / step) */
+    int32_t last_step;       /* last observed istep */
+    int32_t pad_;
+} mdt_traj_obs;
+
+/* Observables of every trajectory into out[ntraj]. */
+int mdt_batch_get_observables(mdt_batch *b, mdt_traj_obs *out);
+
+/*
+ * Estimators of the current state, MDT_NEST doubles per trajectory into
+ * out[ntraj][MDT_NEST], in the order of columns 2-21 of the reference's
+ * energy file (output_nrg, src/output_mod.f90:170-229):
+ *   0 atom temperature [K] (calc_atom_temperature, universe_mod.f90:314-327)
+ *   1 bead temperature [K] (calc_bead_temperature, rpmd.f90:63-77)
+ *   2,3 squared radius of gyration, projectile / lattice [A^2]
+ *       (radius_of_gyration, rpmd.f90:511-545)
+ *   4,5 centroid kinetic energy, projectile / lattice (atom_ekin,
+ *       universe_mod.f90:433-458)
+ *   6,7 bead kinetic energy, projectile / lattice (bead_ekin, rpmd.f90:547-572)
+ *   8,9 virial quantum kinetic energy, projectile / lattice
+ *       (virial_quantum_ekin, rpmd.f90:273-301)
+ *   10 potential energy averaged over beads, 11 ring-polymer spring energy
+ *       (calc_bead_epot, rpmd.f90:236-264), 12 total energy 6+7+10+11,
+ *   13 sum of all forces, 14-16 projectile centroid [A], 17-19 its
+ *       velocity [A/fs].
+ * Energies in eV.  In mixed bead mode a classical atom counts as P copies
+ * of its bead (the state mdt_batch_get_state returns).  The temperatures
+ * use dof = 3 N P - (number of fixed components over all beads).
+ */
+#define MDT_NEST 20
+int mdt_batch_estimators(mdt_batch *b, double *out);
+
+/*
  * Step-loop launches through a hipGraph: with chunk > 0, mdt_batch_step
  * replays a graph of `chunk` steps (captured on first use, recaptured after
  * mdt_batch_tune or mdt_batch_set_precision) and launches the remaining

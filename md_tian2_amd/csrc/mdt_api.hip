@@ -3,7 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cctype>
+#include <cstddef>
 #include <cstdlib>
 #include <cmath>
 #include <cstdarg>
@@ -18,6 +20,7 @@
 size_t mdt_replica_smem(int N, int cap, bool nbr_lds);
 hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst *ec, const PropConst *pc,
                               const BatchView &bv, int add_fin, hipStream_t st);
+hipError_t mdt_launch_estimators(int ntraj, const PropConst *pc, const BatchView &bv, double *out, hipStream_t st);
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
                            const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
                            int flags, hipStream_t st);
@@ -166,6 +169,13 @@ struct mdt_batch {
     int mix_tpa = 0, mix_nbcap = 0, mix_nmax = 0, mix_pmax = 0;
     size_t mix_smem = 0;
     float mix_tuned_ms = 0.f;
+    // trajectory observation and the exit at the projectile limits
+    TrajObs *d_obs = nullptr;
+    int *d_done = nullptr;
+    double obs_ul = 0, obs_ll = 0, ads_start = 0, ads_end = 0;
+    int obs_exit = 0;
+    uint32_t obs_step0 = 0;
+    double *d_est = nullptr;  // MDT_NEST per trajectory
 
     BatchView view() const {
         BatchView v{};
@@ -197,6 +207,14 @@ struct mdt_batch {
         v.status = d_status;
         v.nbr_global = nullptr;  // per size class (launch_replicas)
         v.nbr_stride = 0;
+        v.obs = d_obs;
+        v.done = d_obs ? d_done : nullptr;
+        v.obs_ul = obs_ul;
+        v.obs_ll = obs_ll;
+        v.ads_start = ads_start;
+        v.ads_end = ads_end;
+        v.obs_exit = obs_exit;
+        v.obs_step0 = obs_step0;
         return v;
     }
 };
@@ -380,7 +398,7 @@ int mdt_batch_destroy(mdt_batch *b) {
                     b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
                     b->d_cjk, b->d_status, b->d_pc, b->d_step, b->d_nnG, b->d_nnC,
-                    b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr};
+                    b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr, b->d_obs, b->d_done, b->d_est};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -1241,6 +1259,69 @@ int mdt_batch_set_bead_mode(mdt_batch *b, int mode) {
         return rc;
     }
     return configure(b, 0, b->nbcap_req);
+}
+
+int mdt_batch_set_observe(mdt_batch *b, double proj_ul, double proj_ll, double ads_start, double ads_end,
+                          int exit_at_limits) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    drop_graph(b);  // the captured launches carry the observation arguments
+    if (!b->d_obs) {
+        int rc = dalloc(&b->d_obs, b->ntraj) | dalloc(&b->d_done, b->ntraj);
+        if (rc) return rc;
+    }
+    // the reference's starting values (src/trajectory_info.f90:53-67)
+    std::vector<TrajObs> o(b->ntraj);
+    for (TrajObs &x : o) {
+        x.cl_appr = DBL_MAX;
+        x.cl_last = DBL_MAX;
+        x.r_min[0] = x.r_min[1] = x.r_min[2] = DBL_MAX;
+        x.t_int = 0.0;
+        x.turning_points = 0;
+        x.moving_down = 1;
+        x.adsorbed = 0;
+        x.adsorbed_prev = 0;
+        x.exit_step = 0;
+        x.cl_appr_step = 0;
+        x.last_step = 0;
+        x.pad_ = 0;
+    }
+    HIPCHK(hipMemcpy(b->d_obs, o.data(), sizeof(TrajObs) * b->ntraj, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(b->d_done, 0, sizeof(int) * b->ntraj));
+    b->obs_ul = proj_ul;
+    b->obs_ll = proj_ll;
+    b->ads_start = ads_start;
+    b->ads_end = ads_end;
+    b->obs_exit = exit_at_limits ? 1 : 0;
+    b->obs_step0 = b->step;
+    return 0;
+}
+
+static_assert(sizeof(TrajObs) == sizeof(mdt_traj_obs) && offsetof(TrajObs, turning_points) ==
+                  offsetof(mdt_traj_obs, turning_points) && offsetof(TrajObs, last_step) ==
+                  offsetof(mdt_traj_obs, last_step),
+              "TrajObs and mdt_traj_obs differ");
+
+int mdt_batch_get_observables(mdt_batch *b, mdt_traj_obs *out) {
+    if (!b || !out) return fail(MDT_EINVAL, "NULL argument");
+    if (!b->d_obs) return fail(MDT_ESTATE, "observation is off (call mdt_batch_set_observe)");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    HIPCHK(hipMemcpy(out, b->d_obs, sizeof(TrajObs) * b->ntraj, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int mdt_batch_estimators(mdt_batch *b, double *out) {
+    if (!b || !out) return fail(MDT_EINVAL, "NULL argument");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    int rc = ensure_ready(b, false);
+    if (rc) return rc;
+    if (!b->d_est && (rc = dalloc(&b->d_est, (size_t)MDT_NEST * b->ntraj))) return rc;
+    const BatchView bv = b->view();
+    HIPCHK(mdt_launch_estimators(b->ntraj, b->d_pc, bv, b->d_est, b->ctx->stream));
+    HIPCHK(hipMemcpyAsync(out, b->d_est, sizeof(double) * MDT_NEST * b->ntraj, hipMemcpyDeviceToHost,
+                          b->ctx->stream));
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    return 0;
 }
 
 int mdt_batch_status(mdt_batch *b, int32_t *status, int clear) {

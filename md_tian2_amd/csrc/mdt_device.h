@@ -99,6 +99,38 @@ struct BatchView {
     const double *judge_f;
     double *judge_acc;
     int judge_mod, judge_phase;
+    // trajectory observation (collect_trajectory_characteristics,
+    // src/trajectory_info.f90:47-263; the exit test of src/md_tian2.f90:88-92)
+    // when obs != NULL: after each step's propagate_2 k_traj updates
+    // obs[t], and a trajectory whose projectile beads are all above
+    // obs_ul or all below obs_ll is frozen (done[t] set: k_traj and the
+    // force kernels skip it from then on)
+    struct TrajObs *obs;
+    int *done;
+    double obs_ul, obs_ll, ads_start, ads_end;
+    int obs_exit;        // 1: the upper / lower limit freezes the trajectory
+    uint32_t obs_step0;  // device step count when observation started: istep = step - obs_step0
+};
+
+// estimators per trajectory written by k_estimators (the columns 2-21 of
+// the reference's energy file)
+#define MDT_NEST 20
+
+// Per trajectory, after the step istep (1-based from the start of
+// observation); the layout of mdt_traj_obs in the C interface.
+struct TrajObs {
+    double cl_appr;        // closest centroid distance projectile - other atom [A]
+    double cl_last;        // that distance at the last observed step [A]
+    double r_min[3];       // projectile centroid at its lowest z
+    double t_int;          // time with a nonzero force on the projectile [fs]
+    int turning_points;    // sign changes of the centroid's z velocity
+    int moving_down;       // the centroid's z velocity was < 0 at the last step
+    int adsorbed;          // within ads_start, until farther than ads_end
+    int adsorbed_prev;     // adsorbed before the last observed step
+    int exit_step;         // istep after which the trajectory was frozen (0: running)
+    int cl_appr_step;      // istep of the closest approach
+    int last_step;         // last observed istep
+    int pad_;
 };
 
 // ---------------------------------------------------------------------------

@@ -267,6 +267,7 @@ __global__ void __launch_bounds__(512, 4)
     if (bv.skip && bv.skip[rep]) return;                                           // whole block: uniform
     if (JUDGE && (rep % bv.judge_mod) != bv.judge_phase) return;                  // whole block: uniform
     const int t = bv.rep_traj[rep];
+    if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
     const int b = bv.rep_bead[rep];
     const int N = bv.traj_N[t];
     const int cap = bv.nbcap < N - 1 ? bv.nbcap : (N > 1 ? N - 1 : 1);
@@ -642,6 +643,7 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
     __shared__ double red[4];
     const PropConst &pc = *pcg;
     const int t = blockIdx.x;
+    if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
     const int N = bv.traj_N[t];
     const int P = bv.traj_P[t];
     const long long soff = bv.traj_soff[t];
@@ -749,6 +751,98 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
             }
         }
         __syncthreads();
+    }
+
+    if ((flags & 1) && bv.obs) {
+        // collect_trajectory_characteristics after propagate_2 of step
+        // step_prev (src/trajectory_info.f90:47-263): centroids, the
+        // projectile (atom 0) against every other atom (min image)
+        double cx = 0.0, cy = 0.0, cz = 0.0, vz = 0.0;
+        int fnz = 0, above = 1, below = 1;
+        for (int b = 0; b < P; ++b) {
+            cx += bv.x[XI(0, b, 0)];
+            cy += bv.x[XI(1, b, 0)];
+            cz += bv.x[XI(2, b, 0)];
+            vz += bv.v[XI(2, b, 0)];
+            if (bv.f[XI(0, b, 0)] != 0.0 || bv.f[XI(1, b, 0)] != 0.0 || bv.f[XI(2, b, 0)] != 0.0) fnz = 1;
+            if (!(bv.x[XI(2, b, 0)] > bv.obs_ul)) above = 0;
+            if (!(bv.x[XI(2, b, 0)] < bv.obs_ll)) below = 0;
+        }
+        cx /= P;
+        cy /= P;
+        cz /= P;
+        vz /= P;
+        double h[9], ih[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            h[k] = bv.cell[9 * t + k];
+            ih[k] = bv.icell[9 * t + k];
+        }
+        double rmin = 1.0e300;
+        for (int i = 1 + lane; i < N; i += nthr) {
+            const int pb = PT(bv.type[aoff + i]);
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            for (int b = 0; b < pb; ++b) {
+                ax += bv.x[XI(0, b, i)];
+                ay += bv.x[XI(1, b, i)];
+                az += bv.x[XI(2, b, i)];
+            }
+            const double d0 = cx - ax / pb, d1 = cy - ay / pb, d2 = cz - az / pb;
+            double s0 = ih[0] * d0 + ih[3] * d1 + ih[6] * d2;
+            double s1 = ih[1] * d0 + ih[4] * d1 + ih[7] * d2;
+            double s2 = ih[2] * d0 + ih[5] * d1 + ih[8] * d2;
+            s0 -= round(s0);
+            s1 -= round(s1);
+            s2 -= round(s2);
+            const double e0 = h[0] * s0 + h[3] * s1 + h[6] * s2;
+            const double e1 = h[1] * s0 + h[4] * s1 + h[7] * s2;
+            const double e2 = h[2] * s0 + h[5] * s1 + h[8] * s2;
+            const double r = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            rmin = r < rmin ? r : rmin;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double y = __shfl_xor(rmin, o, 64);
+            rmin = y < rmin ? y : rmin;
+        }
+        __shared__ double s_rmin[4];
+        __shared__ int s_done;
+        if ((lane & 63) == 0) s_rmin[lane >> 6] = rmin;
+        __syncthreads();
+        if (lane == 0) {
+            for (int w = 1; w < (nthr + 63) / 64; ++w) rmin = s_rmin[w] < rmin ? s_rmin[w] : rmin;
+            TrajObs &o = bv.obs[t];
+            const int istep = (int)(step_prev - bv.obs_step0);
+            if (rmin < o.cl_appr) {
+                o.cl_appr = rmin;
+                o.cl_appr_step = istep;
+            }
+            o.cl_last = rmin;
+            o.last_step = istep;
+            o.adsorbed_prev = o.adsorbed;
+            if (!o.adsorbed && rmin < bv.ads_start) o.adsorbed = 1;
+            if (o.adsorbed && rmin > bv.ads_end) o.adsorbed = 0;
+            if (cz < o.r_min[2]) {
+                o.r_min[0] = cx;
+                o.r_min[1] = cy;
+                o.r_min[2] = cz;
+            }
+            if (fnz) o.t_int += dt;
+            const int down = vz < 0.0;
+            if (down != o.moving_down) {
+                ++o.turning_points;
+                o.moving_down = down;
+            }
+            s_done = bv.obs_exit && (above || below);
+            if (s_done) {
+                o.exit_step = istep;
+                bv.done[t] = 1;
+            }
+        }
+        __syncthreads();
+        if (s_done) {  // frozen after this step: no propagate_1 / ring-polymer step
+            if (bad_local) atomicOr(&bv.status[t], bad_local);
+            return;
+        }
     }
 
     if (flags & 2) {
@@ -1014,6 +1108,7 @@ __global__ void __launch_bounds__(512, 2)
     __shared__ LdsTables T;
     __shared__ double red[1];
     const int t = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;  // trajectory list
+    if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
     const int N = bv.traj_N[t], P = bv.traj_P[t];
     const int Ns = N - 1;
     const int cap = bv.nbcap < Ns - 1 ? bv.nbcap : (Ns > 1 ? Ns - 1 : 1);

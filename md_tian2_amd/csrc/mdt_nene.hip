@@ -913,6 +913,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__
     const NnConst &M = *Mg;
     const int rep = blockIdx.x;
     const int t = bv.rep_traj[rep];
+    if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
     const int b = bv.rep_bead[rep];
     const int N = bv.traj_N[t];
     const long long soff = bv.traj_soff[t];
@@ -1043,6 +1044,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_f(const NnConst *__restrict__
     const int rep = blockIdx.x;
     if (skip_extrap && nv.rep_flag[rep]) return;  // whole block: uniform
     const int t = bv.rep_traj[rep];
+    if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
     const int b = bv.rep_bead[rep];
     const int N = bv.traj_N[t];
     const long long soff = bv.traj_soff[t];

@@ -29,10 +29,25 @@ SYMBOLS = (
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
     "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph", "mdt_batch_set_bead_mode",
+    "mdt_batch_set_observe", "mdt_batch_get_observables", "mdt_batch_estimators",
     "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats", "mdt_batch_nn_fallback",
     "mdt_batch_nn_descriptors", "mdt_nn_train_create", "mdt_nn_train_destroy", "mdt_nn_train_descriptor_stats",
     "mdt_nn_train_epoch", "mdt_nn_train_eval", "mdt_nn_train_loss_grad", "mdt_nn_get_weights", "mdt_nn_load",
 )
+
+
+NEST = 20
+ESTIMATORS = ("atom_T", "bead_T", "rgyr_sq_p", "rgyr_sq_l", "atom_ekin_p", "atom_ekin_l", "bead_ekin_p",
+              "bead_ekin_l", "qntm_ekin_p", "qntm_ekin_l", "atom_epot", "bead_epot", "e_total", "f_total",
+              "r_x", "r_y", "r_z", "v_x", "v_y", "v_z")
+
+
+class TrajObs(C.Structure):
+    """mdt_traj_obs of the C interface."""
+    _fields_ = [("cl_appr", C.c_double), ("cl_last", C.c_double), ("r_min", C.c_double * 3),
+                ("t_int", C.c_double), ("turning_points", C.c_int32), ("moving_down", C.c_int32),
+                ("adsorbed", C.c_int32), ("adsorbed_prev", C.c_int32), ("exit_step", C.c_int32),
+                ("cl_appr_step", C.c_int32), ("last_step", C.c_int32), ("pad_", C.c_int32)]
 
 
 class MdtError(RuntimeError):
@@ -70,6 +85,9 @@ def lib():
         L.mdt_batch_force_timing.argtypes = [vp, dp]
         L.mdt_batch_set_precision.argtypes = [vp, C.c_int]
         L.mdt_batch_set_bead_mode.argtypes = [vp, C.c_int]
+        L.mdt_batch_set_observe.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
+        L.mdt_batch_get_observables.argtypes = [vp, C.c_void_p]
+        L.mdt_batch_estimators.argtypes = [vp, dp]
         L.mdt_batch_set_graph.argtypes = [vp, C.c_int]
         L.mdt_nn_params_set.argtypes = [vp, C.c_int, ip, C.c_int, ip, ip, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp,
                                         C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -207,6 +225,7 @@ class Batch:
                  seed=12345, traj_ids=None):
         self.ctx = ctx
         self.systems = systems
+        self.dt = dt
         B = len(systems)
         self.N = np.array([s.natoms for s in systems], dtype=np.int32)
         self.P = np.array([s.nbeads for s in systems], dtype=np.int32)
@@ -265,6 +284,34 @@ class Batch:
         """0: every atom carries the beads; 1 (mixed): the projectile (atom 0)
         carries them and the other atoms are classical, kept in bead 0."""
         _check(lib().mdt_batch_set_bead_mode(self.h, int(mode)))
+
+    def set_observe(self, proj_ul=None, proj_ll=None, adsorption=None, exit_at_limits=True):
+        """Trajectory observation and the exit at the projectile limits
+        (`pul`, `pll`, `adsorption_distance start end`); None = the
+        reference's default (no limit; adsorption thresholds at +inf)."""
+        big = np.finfo(np.float64).max
+        ul = big if proj_ul is None else float(proj_ul)
+        ll = -big if proj_ll is None else float(proj_ll)
+        a0, a1 = (big, big) if adsorption is None else (float(adsorption[0]), float(adsorption[1]))
+        _check(lib().mdt_batch_set_observe(self.h, ul, ll, a0, a1, 1 if exit_at_limits else 0))
+
+    def observables(self):
+        """Per trajectory dict of mdt_traj_obs: cl_appr [A], cl_last [A],
+        r_min (3,) [A], t_int [fs], turning_points, moving_down, adsorbed,
+        exit_step (0 while running), cl_appr_step, last_step."""
+        out = (TrajObs * len(self.systems))()
+        _check(lib().mdt_batch_get_observables(self.h, out))
+        return [dict(cl_appr=o.cl_appr, cl_last=o.cl_last, r_min=np.array(o.r_min[:]), t_int=o.t_int,
+                     turning_points=o.turning_points, moving_down=bool(o.moving_down), adsorbed=bool(o.adsorbed),
+                     adsorbed_prev=bool(o.adsorbed_prev), exit_step=o.exit_step, cl_appr_step=o.cl_appr_step,
+                     last_step=o.last_step) for o in out]
+
+    def estimators(self):
+        """(ntraj, 20) estimators of the current state: the columns 2-21 of
+        the reference's energy file (ESTIMATORS names them)."""
+        out = np.zeros((len(self.systems), NEST))
+        _check(lib().mdt_batch_estimators(self.h, _dp(out)))
+        return out
 
     def set_pes(self, pes, judge_interval=0):
         """0: EMT; 1: the network (EMT fallback on extrapolation, EMT judge every judge_interval steps)."""

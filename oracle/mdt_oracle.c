@@ -1143,6 +1143,21 @@ void orc_philox_fill_normals(uint64_t seed, uint32_t traj, uint32_t step,
 
 /* md_tian2.f90:78-98 without output: nsteps of propagate_1 / calc_force /
  * propagate_2 with Philox noise */
+/* the step's noise for orc_md_step (counter scheme of the GPU path) */
+static void fill_step_noise(const orc_universe *u, uint64_t seed, uint32_t traj, uint32_t st, int lan, int pil,
+                            int and_, double *randy, double *zeta, double *and_z, double *and_u) {
+    const int P = u->nbeads, N = u->natoms;
+    if (lan) orc_philox_fill_normals(seed, traj, st, ORC_STREAM_LANGEVIN, N, P, randy);
+    if (pil) orc_philox_fill_normals(seed, traj, st, ORC_STREAM_PILE, N, P, zeta);
+    if (and_) {
+        orc_philox_fill_normals(seed, traj, st, ORC_STREAM_ANDERSEN_V, N, P, and_z);
+        for (int i = 0; i < N; ++i)
+            for (int b = 0; b < P; ++b)
+                and_u[IPN(b, i, P)] = orc_philox_uniform(seed, traj, st, ORC_STREAM_ANDERSEN_U,
+                                                         (uint32_t)(i * P + b), 0u);
+    }
+}
+
 int orc_run_trajectory(orc_universe *u, const orc_emt_params *p,
                        const orc_simparams *sp, uint64_t seed, uint32_t traj,
                        int step0, int nsteps) {
@@ -1155,18 +1170,291 @@ int orc_run_trajectory(orc_universe *u, const orc_emt_params *p,
               and_ = any_algo(u, ORC_PROP_ANDERSEN);
     int rc = 0;
     for (int s = 0; s < nsteps && !rc; ++s) {
-        const uint32_t st = (uint32_t)(step0 + s);
-        if (lan) orc_philox_fill_normals(seed, traj, st, ORC_STREAM_LANGEVIN, N, P, randy);
-        if (pil) orc_philox_fill_normals(seed, traj, st, ORC_STREAM_PILE, N, P, zeta);
-        if (and_) {
-            orc_philox_fill_normals(seed, traj, st, ORC_STREAM_ANDERSEN_V, N, P, and_z);
-            for (int i = 0; i < N; ++i)
-                for (int b = 0; b < P; ++b)
-                    and_u[IPN(b, i, P)] = orc_philox_uniform(seed, traj, st, ORC_STREAM_ANDERSEN_U,
-                                                             (uint32_t)(i * P + b), 0u);
-        }
+        fill_step_noise(u, seed, traj, (uint32_t)(step0 + s), lan, pil, and_, randy, zeta, and_z, and_u);
         rc = orc_md_step(u, p, sp, cjk, randy, zeta, and_z, and_u);
     }
     free(randy); free(zeta); free(and_z); free(and_u); free(cjk);
     return rc;
+}
+
+/* calc_centroid_positions (rpmd.f90:81-103): bead sum / nbeads, bead order */
+static void centroid(const orc_universe *u, int i, double c[3]) {
+    const int P = u->nbeads;
+    for (int k = 0; k < 3; ++k) {
+        double a = 0.0;
+        for (int b = 0; b < P; ++b) a += u->r[I3PN(k, b, i, P)];
+        c[k] = P > 1 ? a / P : a;
+    }
+}
+
+/* collect_trajectory_characteristics (trajectory_info.f90:47-76) with
+ * check_for_lowest_position (:127-137), check_for_closest_approach
+ * (:81-122), check_for_proj_surf_interaction (:142-150) and the turning
+ * points of check_for_bounce (:248-255).  The bounce statistics are not
+ * restated: output_scatter does not write them (output_mod.f90:699). */
+void orc_collect_characteristics(const orc_universe *u, const orc_simparams *sp, double ads_start,
+                                 double ads_end, int istep, orc_traj_obs *o) {
+    const int P = u->nbeads, N = u->natoms;
+    if (istep == 1) {
+        o->closest_approach = ORC_DEFAULT_REAL;
+        o->closest_approach_time = 0.0;
+        for (int k = 0; k < 3; ++k) o->lowest_z[k] = ORC_DEFAULT_REAL;
+        o->interaction_time = 0.0;
+        o->nturning_points = 0;
+        o->was_moving_down = 1;
+    }
+    double c1[3], ci[3], pp[3];
+    centroid(u, 0, c1);
+    /* check_for_lowest_position: sum(r(:,:,1), dim=2)/nbeads */
+    for (int k = 0; k < 3; ++k) {
+        double a = 0.0;
+        for (int b = 0; b < P; ++b) a += u->r[I3PN(k, b, 0, P)];
+        pp[k] = a / P;
+    }
+    if (pp[2] < o->lowest_z[2])
+        for (int k = 0; k < 3; ++k) o->lowest_z[k] = pp[k];
+    /* check_for_closest_approach */
+    double here = ORC_DEFAULT_REAL;
+    for (int i = 1; i < N; ++i) {
+        centroid(u, i, ci);
+        double d[3], s[3], e[3];
+        for (int k = 0; k < 3; ++k) d[k] = c1[k] - ci[k];
+        for (int k = 0; k < 3; ++k)
+            s[k] = u->isimbox[k + 0] * d[0] + u->isimbox[k + 3] * d[1] + u->isimbox[k + 6] * d[2];
+        for (int k = 0; k < 3; ++k) s[k] = s[k] - round(s[k]);
+        for (int k = 0; k < 3; ++k)
+            e[k] = u->simbox[k + 0] * s[0] + u->simbox[k + 3] * s[1] + u->simbox[k + 6] * s[2];
+        const double r = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        if (r < o->closest_approach) {
+            o->closest_approach = r;
+            o->closest_approach_time = (double)istep;
+        }
+        if (r < here) here = r;
+    }
+    o->closest_approach_time = o->closest_approach_time * sp->step; /* every call (:111) */
+    if (!o->is_adsorbed && here < ads_start) o->is_adsorbed = 1;
+    if (o->is_adsorbed && here > ads_end) o->is_adsorbed = 0;
+    /* check_for_proj_surf_interaction: any(f(:,:,1) /= 0) */
+    int fnz = 0;
+    for (int q = 0; q < 3 * P; ++q)
+        if (u->f[q] != 0.0) fnz = 1;
+    if (fnz) o->interaction_time = o->interaction_time + sp->step;
+    /* turning points: sum(v(3,:,1))/nbeads < 0 */
+    double vz = 0.0;
+    for (int b = 0; b < P; ++b) vz += u->v[I3PN(2, b, 0, P)];
+    const int down = vz / P < 0.0;
+    if (down != o->was_moving_down) {
+        o->nturning_points += 1;
+        o->was_moving_down = down;
+    }
+}
+
+/* One trajectory of the md loop with its exits (md_tian2.f90:75-101):
+ * propagate_1, calc_force, propagate_2; with scatter output, leave when
+ * every projectile bead is above proj_ul or below proj_ll; collect the
+ * characteristics; after the loop collect once more with the final istep
+ * (the exit step, or nsteps + 1).  *istep_end receives that istep. */
+int orc_run_scatter(orc_universe *u, const orc_emt_params *p, const orc_simparams *sp, uint64_t seed,
+                    uint32_t traj, int nsteps, double proj_ul, double proj_ll, int scatter, double ads_start,
+                    double ads_end, orc_traj_obs *o, int *istep_end) {
+    const int P = u->nbeads, N = u->natoms;
+    const size_t n = (size_t)3 * P * N;
+    double *randy = ALLOC0(n), *zeta = ALLOC0(n), *and_z = ALLOC0(n), *and_u = ALLOC0(P * N);
+    double *cjk = ALLOC0(P * P);
+    orc_build_cjk(P, cjk);
+    const int lan = any_algo(u, ORC_PROP_LANGEVIN), pil = any_algo(u, ORC_PROP_PILE),
+              and_ = any_algo(u, ORC_PROP_ANDERSEN);
+    const int proj1 = u->is_proj[u->idx[0]];
+    int rc = 0, istep = 1;
+    for (; istep <= nsteps; ++istep) {
+        fill_step_noise(u, seed, traj, (uint32_t)istep, lan, pil, and_, randy, zeta, and_z, and_u);
+        rc = orc_md_step(u, p, sp, cjk, randy, zeta, and_z, and_u);
+        if (rc) break;
+        int above = 1, below = 1;
+        for (int b = 0; b < P; ++b) {
+            if (!(u->r[I3PN(2, b, 0, P)] > proj_ul)) above = 0;
+            if (!(u->r[I3PN(2, b, 0, P)] < proj_ll)) below = 0;
+        }
+        if (scatter && proj1 && (above || below)) break;
+        orc_collect_characteristics(u, sp, ads_start, ads_end, istep, o);
+    }
+    if (!rc) orc_collect_characteristics(u, sp, ads_start, ads_end, istep, o);
+    *istep_end = istep;
+    free(randy); free(zeta); free(and_z); free(and_u); free(cjk);
+    return rc;
+}
+
+/* ---- estimators of output_nrg (output_mod.f90:170-229) ---- */
+
+/* set_atomic_dofs (md_init.f90:882-887) */
+static int dof_count(const orc_universe *u) {
+    const int n = 3 * u->nbeads * u->natoms;
+    int c = 0;
+    for (int q = 0; q < n; ++q) c += u->is_fixed[q] != 0;
+    return n - c;
+}
+
+/* calc_atom_temperature (universe_mod.f90:314-327) */
+double orc_atom_temperature(const orc_universe *u) {
+    const int P = u->nbeads;
+    double temp = 0.0;
+    for (int i = 0; i < u->natoms; ++i) {
+        double s2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            double a = 0.0;
+            for (int b = 0; b < P; ++b) a += u->v[I3PN(k, b, i, P)];
+            s2 += a * a;
+        }
+        temp = temp + u->m[i] * s2;
+    }
+    return temp / KB / dof_count(u) / P;
+}
+
+/* calc_bead_temperature (rpmd.f90:63-77) */
+double orc_bead_temperature(const orc_universe *u) {
+    const int P = u->nbeads;
+    double temp = 0.0;
+    for (int i = 0; i < u->natoms; ++i) {
+        double s2 = 0.0;
+        for (int q = 0; q < 3 * P; ++q) s2 += u->v[3 * P * i + q] * u->v[3 * P * i + q];
+        temp = temp + u->m[i] * s2;
+    }
+    return temp / KB / dof_count(u);
+}
+
+/* radius_of_gyration (rpmd.f90:511-545) */
+void orc_radius_of_gyration(const orc_universe *u, double *rgyr_p, double *rgyr_l) {
+    const int P = u->nbeads;
+    *rgyr_p = 0.0;
+    *rgyr_l = 0.0;
+    if (P == 1) return;
+    int nprojs = 0;
+    for (int i = 0; i < u->natoms; ++i) {
+        double c[3];
+        centroid(u, i, c);
+        double acc = 0.0;
+        for (int b = 0; b < P; ++b)
+            for (int k = 0; k < 3; ++k) {
+                const double d = u->r[I3PN(k, b, i, P)] - c[k];
+                acc += d * d;
+            }
+        if (u->is_proj[u->idx[i]]) {
+            nprojs += 1;
+            *rgyr_p += acc;
+        } else {
+            *rgyr_l += acc;
+        }
+    }
+    *rgyr_p = *rgyr_p / P / (nprojs > 1 ? nprojs : 1);
+    const int nl = u->natoms - nprojs;
+    *rgyr_l = *rgyr_l / P / (nl > 1 ? nl : 1);
+}
+
+/* atom_ekin (universe_mod.f90:433-458) */
+void orc_atom_ekin(const orc_universe *u, double *ekin_p, double *ekin_l) {
+    const int P = u->nbeads;
+    *ekin_p = 0.0;
+    *ekin_l = 0.0;
+    for (int i = 0; i < u->natoms; ++i) {
+        double s2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            double a = 0.0;
+            for (int b = 0; b < P; ++b) a += u->v[I3PN(k, b, i, P)];
+            s2 += a * a;
+        }
+        const double nrg = u->m[i] * s2;
+        if (u->is_proj[u->idx[i]]) *ekin_p += nrg;
+        else *ekin_l += nrg;
+    }
+    *ekin_p = 0.5 * *ekin_p / P / P;
+    *ekin_l = 0.5 * *ekin_l / P / P;
+}
+
+/* bead_ekin (rpmd.f90:547-572) */
+void orc_bead_ekin(const orc_universe *u, double *ekin_p, double *ekin_l) {
+    const int P = u->nbeads;
+    *ekin_p = 0.0;
+    *ekin_l = 0.0;
+    for (int i = 0; i < u->natoms; ++i) {
+        double s2 = 0.0;
+        for (int q = 0; q < 3 * P; ++q) s2 += u->v[3 * P * i + q] * u->v[3 * P * i + q];
+        const double nrg = u->m[i] * s2;
+        if (u->is_proj[u->idx[i]]) *ekin_p += nrg;
+        else *ekin_l += nrg;
+    }
+    *ekin_p = 0.5 * *ekin_p / P;
+    *ekin_l = 0.5 * *ekin_l / P;
+}
+
+/* virial_quantum_ekin (rpmd.f90:273-301) */
+void orc_virial_quantum_ekin(const orc_universe *u, double *ekin_p, double *ekin_l) {
+    const int P = u->nbeads;
+    *ekin_p = 0.0;
+    *ekin_l = 0.0;
+    if (P == 1) return;
+    for (int i = 0; i < u->natoms; ++i) {
+        double c[3];
+        centroid(u, i, c);
+        for (int b = 0; b < P; ++b) {
+            double d = 0.0;
+            for (int k = 0; k < 3; ++k) d += (c[k] - u->r[I3PN(k, b, i, P)]) * u->f[I3PN(k, b, i, P)];
+            if (u->is_proj[u->idx[i]]) *ekin_p += d;
+            else *ekin_l += d;
+        }
+    }
+    *ekin_p = 0.5 * *ekin_p / P;
+    *ekin_l = 0.5 * *ekin_l / P;
+}
+
+/* calc_bead_epot (rpmd.f90:236-264) over calc_inter_bead_distances
+ * (:212-233): plain differences r(b) - r(b+1), no minimum image */
+double orc_bead_epot(const orc_universe *u, const orc_simparams *sp) {
+    const int P = u->nbeads;
+    double epot = 0.0;
+    for (int i = 0; i < u->natoms; ++i) {
+        double dx2 = 0.0;
+        if (P > 1)
+            for (int b = 0; b < P; ++b) {
+                const int k = (b + 1) % P;
+                double d2 = 0.0;
+                for (int c = 0; c < 3; ++c) {
+                    const double d = u->r[I3PN(c, b, i, P)] - u->r[I3PN(c, k, i, P)];
+                    d2 += d * d;
+                }
+                const double dist = sqrt(d2);
+                dx2 += dist * dist;
+            }
+        const double T = u->is_proj[u->idx[i]] ? sp->Tproj : sp->Tsurf;
+        const double w = KB * T * P / HBAR;
+        epot = epot + u->m[i] * dx2 * (w * w);
+    }
+    return 0.5 * epot / P;
+}
+
+/* the row of output_nrg after the time column */
+void orc_energy_row(const orc_universe *u, const orc_simparams *sp, double out[20]) {
+    const int P = u->nbeads;
+    out[0] = orc_atom_temperature(u);
+    out[1] = orc_bead_temperature(u);
+    orc_radius_of_gyration(u, &out[2], &out[3]);
+    orc_atom_ekin(u, &out[4], &out[5]);
+    orc_bead_ekin(u, &out[6], &out[7]);
+    orc_virial_quantum_ekin(u, &out[8], &out[9]);
+    double ep = 0.0;
+    for (int b = 0; b < P; ++b) ep += u->epot[b];
+    out[10] = ep / P;
+    out[11] = orc_bead_epot(u, sp);
+    out[12] = out[6] + out[7] + out[10] + out[11];
+    double fs = 0.0;
+    for (int q = 0; q < 3 * P * u->natoms; ++q) fs += u->f[q];
+    out[13] = fs;
+    for (int k = 0; k < 3; ++k) {
+        double xa = 0.0, va = 0.0;
+        for (int b = 0; b < P; ++b) {
+            xa += u->r[I3PN(k, b, 0, P)];
+            va += u->v[I3PN(k, b, 0, P)];
+        }
+        out[14 + k] = xa / P;
+        out[17 + k] = va / P;
+    }
 }

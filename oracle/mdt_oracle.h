@@ -129,6 +129,35 @@ int orc_run_trajectory(orc_universe *u, const orc_emt_params *p,
                        const orc_simparams *sp, uint64_t seed, uint32_t traj,
                        int step0, int nsteps);
 
+/* ---- trajectory characteristics (trajectory_info.f90:25-43, 47-263):
+ * the module variables of one trajectory.  is_adsorbed is not reset at
+ * istep 1 in the reference (it carries over from the previous trajectory);
+ * the caller sets its starting value. ---- */
+#define ORC_DEFAULT_REAL 1.7976931348623157e308 /* huge(0_dp), constants.f90:101 */
+typedef struct {
+    double closest_approach, closest_approach_time, lowest_z[3], interaction_time;
+    int nturning_points, was_moving_down, is_adsorbed;
+} orc_traj_obs;
+
+void orc_collect_characteristics(const orc_universe *u, const orc_simparams *sp, double ads_start,
+                                 double ads_end, int istep, orc_traj_obs *o);
+/* noise of step istep from orc_philox_* with step index istep (1-based) */
+int orc_run_scatter(orc_universe *u, const orc_emt_params *p, const orc_simparams *sp, uint64_t seed,
+                    uint32_t traj, int nsteps, double proj_ul, double proj_ll, int scatter, double ads_start,
+                    double ads_end, orc_traj_obs *o, int *istep_end);
+
+/* ---- estimators (rpmd.f90:63-77, 236-301, 511-572; universe_mod.f90:
+ * 314-327, 433-458) and the energy-file row of output_nrg
+ * (output_mod.f90:170-229, columns 2-21) ---- */
+double orc_atom_temperature(const orc_universe *u);
+double orc_bead_temperature(const orc_universe *u);
+void orc_radius_of_gyration(const orc_universe *u, double *rgyr_p, double *rgyr_l);
+void orc_atom_ekin(const orc_universe *u, double *ekin_p, double *ekin_l);
+void orc_bead_ekin(const orc_universe *u, double *ekin_p, double *ekin_l);
+void orc_virial_quantum_ekin(const orc_universe *u, double *ekin_p, double *ekin_l);
+double orc_bead_epot(const orc_universe *u, const orc_simparams *sp);
+void orc_energy_row(const orc_universe *u, const orc_simparams *sp, double out[20]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -42,6 +42,13 @@ class SimParams(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("step", "Tsurf", "Tproj", "pile_tau", "andersen_time")]
 
 
+class TrajObs(C.Structure):
+    """orc_traj_obs: the module variables of trajectory_info.f90:25-43."""
+    _fields_ = [("closest_approach", C.c_double), ("closest_approach_time", C.c_double),
+                ("lowest_z", C.c_double * 3), ("interaction_time", C.c_double), ("nturning_points", C.c_int),
+                ("was_moving_down", C.c_int), ("is_adsorbed", C.c_int)]
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -76,6 +83,13 @@ def lib():
         L.orc_run_trajectory.argtypes = [C.POINTER(Universe), C.POINTER(EmtParams), C.POINTER(SimParams),
                                          C.c_uint64, C.c_uint32, C.c_int, C.c_int]
         L.orc_run_trajectory.restype = C.c_int
+        L.orc_collect_characteristics.argtypes = [C.POINTER(Universe), C.POINTER(SimParams), C.c_double,
+                                                  C.c_double, C.c_int, C.POINTER(TrajObs)]
+        L.orc_run_scatter.argtypes = [C.POINTER(Universe), C.POINTER(EmtParams), C.POINTER(SimParams), C.c_uint64,
+                                      C.c_uint32, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double,
+                                      C.c_double, C.POINTER(TrajObs), C.POINTER(C.c_int)]
+        L.orc_run_scatter.restype = C.c_int
+        L.orc_energy_row.argtypes = [C.POINTER(Universe), C.POINTER(SimParams), dp]
         _LIB = L
     return _LIB
 
@@ -182,6 +196,34 @@ class OracleState:
                                       seed, traj, step0, nsteps)
         if rc:
             raise RuntimeError(f"oracle run failed rc={rc}")
+
+    def run_scatter(self, seed, traj, nsteps, proj_ul=None, proj_ll=None, scatter=True, adsorption=None,
+                    adsorbed=False):
+        """The md loop of one trajectory with its exits and characteristics
+        (md_tian2.f90:75-101); returns (TrajObs, final istep)."""
+        big = np.finfo(np.float64).max
+        ul = big if proj_ul is None else proj_ul
+        ll = -big if proj_ll is None else proj_ll
+        a0, a1 = (big, big) if adsorption is None else adsorption
+        o = TrajObs()
+        o.is_adsorbed = 1 if adsorbed else 0
+        iend = C.c_int(0)
+        rc = lib().orc_run_scatter(C.byref(self.u), C.byref(self.params), C.byref(self.sp), seed, traj, nsteps,
+                                   ul, ll, 1 if scatter else 0, a0, a1, C.byref(o), C.byref(iend))
+        if rc:
+            raise RuntimeError(f"oracle run failed rc={rc}")
+        return o, iend.value
+
+    def collect(self, istep, obs, adsorption=None):
+        big = np.finfo(np.float64).max
+        a0, a1 = (big, big) if adsorption is None else adsorption
+        lib().orc_collect_characteristics(C.byref(self.u), C.byref(self.sp), a0, a1, istep, C.byref(obs))
+
+    def energy_row(self):
+        """Columns 2-21 of output_nrg for the current state."""
+        out = np.zeros(20)
+        lib().orc_energy_row(C.byref(self.u), C.byref(self.sp), _dp(out))
+        return out
 
     def minimg(self, i, j):
         d = np.zeros(self.P)
