@@ -17,6 +17,15 @@ module mdt_gpu
     integer(c_int), parameter :: MDT_PROP_VERLET = 1001, MDT_PROP_LANGEVIN = 1003
     integer(c_int), parameter :: MDT_PROP_ANDERSEN = 1005, MDT_PROP_PILE = 1006
 
+    ! mdt_traj_obs: the characteristics of one trajectory (trajectory_info.f90:25-43)
+    type, bind(C) :: mdt_traj_obs
+        real(c_double) :: cl_appr, cl_last, r_min(3), t_int
+        integer(c_int32_t) :: turning_points, moving_down, adsorbed, adsorbed_prev
+        integer(c_int32_t) :: exit_step, cl_appr_step, last_step, pad_
+    end type mdt_traj_obs
+
+    integer(c_int), parameter :: MDT_NEST = 20
+
     interface
 
         function mdt_last_error(buf, buflen) bind(C, name="mdt_last_error") result(n)
@@ -183,6 +192,29 @@ module mdt_gpu
             import :: c_int, c_ptr
             type(c_ptr), value :: batch
             integer(c_int), value :: mode
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_set_observe(batch, proj_ul, proj_ll, ads_start, ads_end, exit_at_limits) &
+                bind(C, name="mdt_batch_set_observe") result(rc)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: batch
+            real(c_double), value :: proj_ul, proj_ll, ads_start, ads_end
+            integer(c_int), value :: exit_at_limits
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_get_observables(batch, obs) bind(C, name="mdt_batch_get_observables") result(rc)
+            import :: c_int, c_ptr, mdt_traj_obs
+            type(c_ptr), value :: batch
+            type(mdt_traj_obs), intent(out) :: obs(*)
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_estimators(batch, est) bind(C, name="mdt_batch_estimators") result(rc)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: batch
+            real(c_double), intent(out) :: est(*)
             integer(c_int) :: rc
         end function
 

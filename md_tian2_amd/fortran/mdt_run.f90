@@ -25,6 +25,11 @@
 ! <m> reads the trajectories' universes instead of `slab` (conf_io).
 ! `mdt_run --check-input <deck>` writes every trajectory's initial universe
 ! as <dump>_NNNNNNNN.mxt and stops before any GPU call.
+! output scatter <n> energy <n> adsorption_status <n> (other formats are
+! ignored with a warning), pul <z>, pll <z> and adsorption_distance <start>
+! <end> turn on the device-side trajectory observation: trajectories leave
+! through pul / pll when scatter output is on, and traj/mxt_fin%08d.dat,
+! energy/traj_%08d.dat and conf/ads_%08d.dat are written (scatter_out).
 program mdt_run
 
     use, intrinsic :: iso_c_binding
@@ -32,6 +37,7 @@ program mdt_run
     use universe_mod
     use pes_emt_gpu_mod, only : emt_gpu_init, emt_gpu_context, emt_params_from_file
     use conf_io, only : read_mxt, write_mxt, read_poscar, merge_conf, mxt_file, invert_cell
+    use scatter_out
 
     implicit none
 
@@ -54,6 +60,11 @@ program mdt_run
     character(len=512) :: conf_path = "", proj_path = ""
     integer :: conf_n = 0, proj_n = 0
     logical :: einc_set = .false., pip_set = .false., check_input = .false.
+    ! output and trajectory limits (src/run_config.f90:508-556, 595-650)
+    logical :: out_scatter = .false.
+    integer :: out_energy = 0, out_ads = 0
+    real(dp) :: proj_ul = huge(0.0_dp), proj_ll = -huge(0.0_dp)
+    real(dp) :: ads_start = huge(0.0_dp), ads_end = huge(0.0_dp)
     character(len=512) :: arg
     real(dp) :: jsq, jn, jmax
     integer(c_int32_t) :: nfb
@@ -65,9 +76,11 @@ program mdt_run
     real(dp), allocatable :: mass(:), cell(:), icell(:), r(:), v(:), f(:), a(:), epot(:), dens(:)
     real(dp), allocatable :: ts(:), tp(:), r0(:), v0(:)
     real(dp), dimension(2) :: eta2, n0, e0, lambda, v0p, kappa, s0
-    real(dp) :: ms
+    real(dp) :: ms, ms_step
     integer(c_int) :: tpa
     integer :: u
+    type(mdt_traj_obs), allocatable :: obs(:)
+    real(dp), allocatable :: est(:)
 
     if (command_argument_count() < 1) then
         print '(a)', "usage: mdt_run [--check-input] <deck>"
@@ -155,9 +168,14 @@ program mdt_run
     end if
     call mdt_check(mdt_batch_tune(batch, 0_c_int, 0_c_int, tpa), "mdt_batch_tune")
     call mdt_check(mdt_batch_compute_forces(batch), "mdt_batch_compute_forces")
-    call mdt_check(mdt_batch_step(batch, int(nsteps, c_int)), "mdt_batch_step")
-    call mdt_check(mdt_batch_last_timing(batch, ms, nl), "mdt_batch_last_timing")
+    if (out_scatter .or. out_energy > 0 .or. out_ads > 0) then
+        call run_observed()
+    else
+        call mdt_check(mdt_batch_step(batch, int(nsteps, c_int)), "mdt_batch_step")
+        call mdt_check(mdt_batch_last_timing(batch, ms, nl), "mdt_batch_last_timing")
+    end if
     call mdt_check(mdt_batch_get_state(batch, r, v, f, a, epot, dens), "mdt_batch_get_state")
+    if (out_scatter) call scatter_final()
     if (len_trim(nene_dir) > 0) then
         call mdt_check(mdt_batch_nn_stats(batch, jsq, jn, jmax, nfb), "mdt_batch_nn_stats")
         if (jn > 0) print '(a,es12.4,a,i0,a)', "network judge: force RMSE ", sqrt(jsq / jn), " eV/A over ", &
@@ -183,6 +201,83 @@ program mdt_run
 
 contains
 
+    ! md_tian2's trajectory loop with its outputs (src/md_tian2.f90:68-118)
+    ! for all trajectories at once: the scatter start block, then steps in
+    ! chunks that end where an energy or adsorption record is due; a
+    ! trajectory frozen at its exit writes no records after its exit step.
+    subroutine run_observed()
+        integer :: cur, nxt, tt
+        logical :: due_e, due_a, e_started, a_started
+        character(len=64) :: path
+        allocate(est(MDT_NEST * ntrajs), obs(ntrajs))
+        if (out_scatter) call make_dir("traj")
+        if (out_energy > 0) call make_dir("energy")
+        if (out_ads > 0) call make_dir("conf")
+        call mdt_check(mdt_batch_estimators(batch, est), "mdt_batch_estimators")
+        if (out_scatter) then
+            do tt = 1, ntrajs
+                write(path, '(a12, i8.8, a4)') 'traj/mxt_fin', tid(tt), '.dat'
+                call write_scatter_initial(trim(path), est((tt - 1) * MDT_NEST + 1 : tt * MDT_NEST))
+            end do
+        end if
+        call mdt_check(mdt_batch_set_observe(batch, proj_ul, proj_ll, ads_start, ads_end, &
+                       merge(1_c_int, 0_c_int, out_scatter .and. tmpl%is_proj(tmpl%idx(1)))), &
+                       "mdt_batch_set_observe")
+        cur = 0
+        ms = 0.0_dp
+        e_started = .false.
+        a_started = .false.
+        do while (cur < nsteps)
+            nxt = nsteps
+            if (out_energy > 0) nxt = min(nxt, (cur / out_energy + 1) * out_energy)
+            if (out_ads > 0) nxt = min(nxt, (cur / out_ads + 1) * out_ads)
+            call mdt_check(mdt_batch_step(batch, int(nxt - cur, c_int)), "mdt_batch_step")
+            call mdt_check(mdt_batch_last_timing(batch, ms_step, nl), "mdt_batch_last_timing")
+            ms = ms + ms_step
+            cur = nxt
+            call mdt_check(mdt_batch_get_observables(batch, obs), "mdt_batch_get_observables")
+            due_e = out_energy > 0
+            if (due_e) due_e = mod(cur, out_energy) == 0
+            due_a = out_ads > 0
+            if (due_a) due_a = mod(cur, out_ads) == 0
+            if (due_e) call mdt_check(mdt_batch_estimators(batch, est), "mdt_batch_estimators")
+            do tt = 1, ntrajs
+                if (obs(tt)%exit_step /= 0 .and. obs(tt)%exit_step < cur) cycle
+                if (due_e) then
+                    write(path, '(a12, i8.8, a4)') 'energy/traj_', tid(tt), '.dat'
+                    call energy_record(trim(path), tid(tt) == start .and. .not. e_started, cur, dt, &
+                                       est((tt - 1) * MDT_NEST + 1 : tt * MDT_NEST))
+                    if (tid(tt) == start) e_started = .true.
+                end if
+                if (due_a) then
+                    write(path, '(a9, i8.8, a4)') 'conf/ads_', tid(tt), '.dat'
+                    call adsorption_record(trim(path), tid(tt) == start .and. .not. a_started, cur, dt, &
+                                           obs(tt)%adsorbed_prev /= 0)
+                    if (tid(tt) == start) a_started = .true.
+                end if
+            end do
+            if (all(obs%exit_step /= 0)) exit
+        end do
+    end subroutine run_observed
+
+    ! the final scatter block of every trajectory (the state is in r, v, f)
+    subroutine scatter_final()
+        integer :: tt, istep
+        real(dp) :: t_int, cat
+        logical :: ads, fnz
+        character(len=64) :: path
+        call mdt_check(mdt_batch_get_observables(batch, obs), "mdt_batch_get_observables")
+        call mdt_check(mdt_batch_estimators(batch, est), "mdt_batch_estimators")
+        do tt = 1, ntrajs
+            fnz = any(f((tt - 1) * ntot + 1 : (tt - 1) * ntot + 3 * nbeads) /= 0.0_dp)
+            call finalize_characteristics(obs(tt), fnz, dt, nsteps, ads_start, ads_end, istep, t_int, ads, cat)
+            write(path, '(a12, i8.8, a4)') 'traj/mxt_fin', tid(tt), '.dat'
+            call write_scatter_final(trim(path), est((tt - 1) * MDT_NEST + 1 : tt * MDT_NEST), &
+                                     (istep - 1) * dt, int(obs(tt)%turning_points), obs(tt)%cl_appr, cat, &
+                                     obs(tt)%r_min)
+        end do
+    end subroutine scatter_final
+
     integer(c_int32_t) function algo_id(s)
         character(len=*), intent(in) :: s
         select case (trim(s))
@@ -199,7 +294,7 @@ contains
     subroutine read_deck(path)
         character(len=*), intent(in) :: path
         character(len=512) :: line, key
-        character(len=512) :: w(8)
+        character(len=512) :: w(16)
         integer :: unit, ios, nw, n
         open(newunit=unit, file=path, status="old", action="read", iostat=ios)
         if (ios /= 0) then
@@ -266,6 +361,29 @@ contains
                     print '(a,a)', "Error: conf must be mxt, poscar or merge, got ", trim(conf_kind)
                     stop 1
                 end select
+            case ("output")
+                if (mod(nw, 2) /= 1) then
+                    print '(a)', "Error: number of output arguments must be even"
+                    stop 1
+                end if
+                do n = 2, nw - 1, 2
+                    select case (trim(w(n)))
+                    case ("scatter"); out_scatter = .true.
+                    case ("energy"); read(w(n + 1), *) out_energy
+                    case ("adsorption_status"); read(w(n + 1), *) out_ads
+                    case default
+                        print '(a,a,a)', "Warning: output format ", trim(w(n)), " is not written by mdt_run"
+                    end select
+                end do
+            case ("pul"); read(w(2), *) proj_ul
+            case ("pll"); read(w(2), *) proj_ll
+            case ("adsorption_distance")
+                if (nw /= 3) then
+                    print '(a)', "Error: adsorption_distance key needs 2 arguments"
+                    stop 1
+                end if
+                read(w(2), *) ads_start
+                read(w(3), *) ads_end
             case ("precision")
                 if (trim(w(2)) == "fp32_pair") then
                     prec = 1

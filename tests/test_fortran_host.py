@@ -84,3 +84,22 @@ def test_driver_conf_poscar_matches_python_reader(tmp_path):
         assert (c.natoms, c.nbeads) == (25, 1) and c.names == ["H", "C"]
         assert np.array_equal(c.r, ref.cartesian()) and np.array_equal(c.fixed, ref.fixed)
         assert np.allclose(c.simbox, ref.simbox, rtol=1e-15)
+
+
+def test_driver_reads_output_keywords(tmp_path):
+    """output / pul / pll / adsorption_distance parse (a format the driver
+    does not write is reported and skipped); malformed ones stop the run."""
+    fix = os.path.join(ROOT, "tests", "golden", "conf")
+    deck = tmp_path / "deck.inp"
+    base = ["run md", "start 1", "ntrajs 1", "nsteps 10", "step 0.1", "projectile 1 H 1.00794 ver",
+            "lattice 1 C 12.0 ver", f"conf merge '{fix}/proj' 1 '{fix}/latt' 1", f"dump '{tmp_path}/init'"]
+    deck.write_text("\n".join(base + ["output scatter 1 energy 10 xyz 5 adsorption_status 2", "pul 6.5",
+                                      "pll -3", "adsorption_distance 1.5 2.5"]) + "\n")
+    res = subprocess.run([EXE, "--check-input", str(deck)], cwd=ROOT, capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "output format xyz is not written" in res.stdout
+    for bad in ("output scatter", "adsorption_distance 1.5"):
+        deck.write_text("\n".join(base + [bad]) + "\n")
+        res = subprocess.run([EXE, "--check-input", str(deck)], cwd=ROOT, capture_output=True, text=True,
+                             timeout=60)
+        assert res.returncode != 0, bad

@@ -178,3 +178,58 @@ def test_fortran_driver_conf_merge_matches_oracle(tmp_path, pes_hau):
         st.run(777, int(d["tid"][t]), 1, int(d["nsteps"]))
         assert np.abs(d["r1"][t] - st.r).max() < 1e-9
         assert np.abs(d["v1"][t] - st.v).max() < 1e-9
+
+
+def test_fortran_driver_scatter_output(tmp_path, pes_hau):
+    """`output scatter 1 energy 50 adsorption_status 25` with pul and
+    adsorption_distance: the driver's traj/mxt_fin files carry the oracle's
+    characteristics (md_tian2's loop with exits, started from the dumped
+    initial states) and the initial / final energies of the oracle's state;
+    the energy and adsorption files hold one record per interval up to the
+    exit step, the header only in the first trajectory's energy file."""
+    deck = open(os.path.join(ROOT, "tests", "decks", "h_au_3x3x4.inp")).read()
+    dump = tmp_path / "scatter.bin"
+    deck = deck.replace("dump 'gpurun_out/mdt_run_h_au.bin'", f"dump '{dump}'")
+    deck = deck.replace("'md_tian2_amd/pes/EMT-HAu.pes'", "'%s'" % os.path.join(ROOT, "md_tian2_amd", "pes",
+                                                                                   "EMT-HAu.pes"))
+    deck = deck.replace("nsteps 30", "nsteps 500").replace("Einc 1.92", "Einc 4.0").replace("pip r r 3.0",
+                                                                                          "pip r r 4.0")
+    deck += "\npul 4.3\nadsorption_distance 1.9 2.6\noutput scatter 1 energy 50 adsorption_status 25\n"
+    p = tmp_path / "scatter.inp"
+    p.write_text(deck)
+    exe = os.path.join(ROOT, "md_tian2_amd", "fortran", "mdt_run")
+    res = subprocess.run([exe, str(p)], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    print(res.stdout)
+    assert res.returncode == 0, res.stdout + res.stderr
+    d = read_dump(str(dump))
+    tmpl = geometry.h_on_slab("Au", n=3, nlayers=4, nfixed=2, thermal=False)
+    cols, pesm = pesfile.emt_tables(pes_hau, tmpl.type_names)
+    n_exit = 0
+    for t in range(d["ntraj"]):
+        tid = int(d["tid"][t])
+        s = tmpl.copy()
+        s.r = d["r0"][t].copy()
+        s.v = d["v0"][t].copy()
+        st = oracle.OracleState(s, cols, pesm, step=0.1, tsurf=300.0, tproj=0.0)
+        st.calc_force()
+        row0 = st.energy_row()
+        o, iend = st.run_scatter(777, tid, 500, proj_ul=4.3, adsorption=(1.9, 2.6))
+        row1 = st.energy_row()
+        n_exit += iend <= 500
+        lines = open(tmp_path / "traj" / ("mxt_fin%08d.dat" % tid)).read().splitlines()
+        assert len(lines) == 23
+        kv = {ln.split("=")[0].strip(): [float(x) for x in ln.split("=")[1].split()] for ln in lines if "=" in ln}
+        assert abs(kv["ekin_p_i"][0] - row0[4]) <= 1e-7 and abs(kv["epot_i"][0] - row0[10]) <= 1e-9 * 400
+        assert abs(kv["etotal_f"][0] - row1[12]) <= 1e-6 and np.abs(np.array(kv["r_f"]) - row1[14:17]).max() <= 1e-6
+        assert kv["time"][0] == pytest.approx((iend - 1) * 0.1, abs=1e-7)
+        assert int(kv["turn_pnts"][0]) == o.nturning_points
+        assert abs(kv["cl_appr"][0] - o.closest_approach) <= 1e-6
+        assert abs(kv["cl_appr_time"][0] - o.closest_approach_time) <= 1e-7
+        assert np.abs(np.array(kv["r_min_p"]) - np.array(o.lowest_z[:])).max() <= 1e-6
+        en = open(tmp_path / "energy" / ("traj_%08d.dat" % tid)).read().splitlines()
+        assert en[0].startswith("#") == (tid == 1)
+        en = [ln for ln in en if not ln.split()[0].startswith("#")]
+        assert len(en) == min(500, iend) // 50
+        ads = open(tmp_path / "conf" / ("ads_%08d.dat" % tid)).read().splitlines()
+        assert len(ads) == min(500, iend) // 25
+    print(f"driver scatter output: {n_exit} of {d['ntraj']} trajectories left through pul")
