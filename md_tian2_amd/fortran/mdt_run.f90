@@ -62,7 +62,7 @@ program mdt_run
     logical :: einc_set = .false., pip_set = .false., check_input = .false.
     ! output and trajectory limits (src/run_config.f90:508-556, 595-650)
     logical :: out_scatter = .false.
-    integer :: out_energy = 0, out_ads = 0
+    integer :: out_energy = 0, out_ads = 0, out_runner = 0
     real(dp) :: proj_ul = huge(0.0_dp), proj_ll = -huge(0.0_dp)
     real(dp) :: ads_start = huge(0.0_dp), ads_end = huge(0.0_dp)
     character(len=512) :: arg
@@ -168,7 +168,7 @@ program mdt_run
     end if
     call mdt_check(mdt_batch_tune(batch, 0_c_int, 0_c_int, tpa), "mdt_batch_tune")
     call mdt_check(mdt_batch_compute_forces(batch), "mdt_batch_compute_forces")
-    if (out_scatter .or. out_energy > 0 .or. out_ads > 0) then
+    if (out_scatter .or. out_energy > 0 .or. out_ads > 0 .or. out_runner > 0) then
         call run_observed()
     else
         call mdt_check(mdt_batch_step(batch, int(nsteps, c_int)), "mdt_batch_step")
@@ -206,13 +206,19 @@ contains
     ! chunks that end where an energy or adsorption record is due; a
     ! trajectory frozen at its exit writes no records after its exit step.
     subroutine run_observed()
-        integer :: cur, nxt, tt
-        logical :: due_e, due_a, e_started, a_started
+        integer :: cur, nxt, tt, j
+        logical :: due_e, due_a, due_r, e_started, a_started
         character(len=64) :: path
+        character(len=3), allocatable :: names(:)
         allocate(est(MDT_NEST * ntrajs), obs(ntrajs))
         if (out_scatter) call make_dir("traj")
         if (out_energy > 0) call make_dir("energy")
         if (out_ads > 0) call make_dir("conf")
+        if (out_runner > 0) then
+            call make_dir("runner")
+            allocate(names(natoms))
+            names = [(tmpl%name(tmpl%idx(j)), j = 1, natoms)]
+        end if
         call mdt_check(mdt_batch_estimators(batch, est), "mdt_batch_estimators")
         if (out_scatter) then
             do tt = 1, ntrajs
@@ -231,6 +237,7 @@ contains
             nxt = nsteps
             if (out_energy > 0) nxt = min(nxt, (cur / out_energy + 1) * out_energy)
             if (out_ads > 0) nxt = min(nxt, (cur / out_ads + 1) * out_ads)
+            if (out_runner > 0) nxt = min(nxt, (cur / out_runner + 1) * out_runner)
             call mdt_check(mdt_batch_step(batch, int(nxt - cur, c_int)), "mdt_batch_step")
             call mdt_check(mdt_batch_last_timing(batch, ms_step, nl), "mdt_batch_last_timing")
             ms = ms + ms_step
@@ -240,7 +247,10 @@ contains
             if (due_e) due_e = mod(cur, out_energy) == 0
             due_a = out_ads > 0
             if (due_a) due_a = mod(cur, out_ads) == 0
+            due_r = out_runner > 0
+            if (due_r) due_r = mod(cur, out_runner) == 0
             if (due_e) call mdt_check(mdt_batch_estimators(batch, est), "mdt_batch_estimators")
+            if (due_r) call mdt_check(mdt_batch_get_state(batch, r, v, f, a, epot, dens), "mdt_batch_get_state")
             do tt = 1, ntrajs
                 if (obs(tt)%exit_step /= 0 .and. obs(tt)%exit_step < cur) cycle
                 if (due_e) then
@@ -255,9 +265,16 @@ contains
                                            obs(tt)%adsorbed_prev /= 0)
                     if (tid(tt) == start) a_started = .true.
                 end if
+                if (due_r) then
+                    write(path, '(a14, i8.8, a4)') 'runner/frames_', tid(tt), '.tmp'
+                    call runner_frame(trim(path), tmpl%simbox, names, natoms, nbeads, &
+                                      r((tt - 1) * ntot + 1 : tt * ntot), f((tt - 1) * ntot + 1 : tt * ntot), &
+                                      epot((tt - 1) * nbeads + 1 : tt * nbeads))
+                end if
             end do
             if (all(obs%exit_step /= 0)) exit
         end do
+        if (out_runner > 0) call runner_join(ntrajs, tid)
     end subroutine run_observed
 
     ! the final scatter block of every trajectory (the state is in r, v, f)
@@ -371,6 +388,7 @@ contains
                     case ("scatter"); out_scatter = .true.
                     case ("energy"); read(w(n + 1), *) out_energy
                     case ("adsorption_status"); read(w(n + 1), *) out_ads
+                    case ("runner"); read(w(n + 1), *) out_runner
                     case default
                         print '(a,a,a)', "Warning: output format ", trim(w(n)), " is not written by mdt_run"
                     end select

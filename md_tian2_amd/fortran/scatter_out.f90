@@ -6,7 +6,11 @@
 !                         final block after the loop;
 !   energy/traj_%08d.dat  output_nrg (:170-229), one record every
 !                         `output energy <n>` steps;
-!   conf/ads_%08d.dat     output_adsorption_status (:792-818).
+!   conf/ads_%08d.dat     output_adsorption_status (:792-818);
+!   runner/input.data     output_runner (:550-609): RuNNer frames of bead
+!                         centroids and centroid forces, gathered per
+!                         trajectory in runner/frames_%08d.tmp and joined in
+!                         trajectory order at the end (runner_join).
 ! The records use the reference's edit descriptors; the reference's bare
 ! `i` (ifort: width 12) and `L` (width 2) are written as i12 and L2.
 !
@@ -25,10 +29,14 @@ module scatter_out
 
     private
     public :: make_dir, write_scatter_initial, write_scatter_final, energy_record, adsorption_record, &
-              finalize_characteristics
+              finalize_characteristics, runner_frame, runner_join
 
     integer, parameter :: dp = c_double
     real(dp), parameter :: rad2deg = 180.0_dp / acos(-1.0_dp)
+    ! constants.f90:160-168
+    real(dp), parameter :: bohr2ang = 0.529177211_dp, ang2bohr = 1.0_dp / bohr2ang
+    real(dp), parameter :: ha2ev = 27.21138602_dp, ev2ha = 1.0_dp / ha2ev
+    real(dp), parameter :: evang2habohr = 1.0_dp / (ha2ev * ang2bohr)
 
     interface
         function c_mkdir(path, mode) bind(C, name="mkdir") result(rc)
@@ -136,6 +144,65 @@ contains
         write(u, '(e13.6e2, L2)') (istep - 1) * step, adsorbed
         close(u)
     end subroutine adsorption_record
+
+    ! one RuNNer structure appended to path: simbox(3,3) columns = cell
+    ! vectors [A], names(natoms), r and f (3, nbeads, natoms) [A, eV/A],
+    ! epot(nbeads) [eV]
+    subroutine runner_frame(path, simbox, names, natoms, nbeads, r, f, epot)
+        character(len=*), intent(in) :: path
+        integer, intent(in) :: natoms, nbeads
+        real(dp), intent(in) :: simbox(3, 3), r(3, nbeads, natoms), f(3, nbeads, natoms), epot(nbeads)
+        character(len=3), intent(in) :: names(natoms)
+        character(len=*), parameter :: lattice_format = '(A7, X, F11.8, X, F11.8, X, F11.8)'
+        character(len=*), parameter :: atom_format = &
+            '(A4, X, F16.9, X, F16.9, X, F16.9, X, A3, X, F11.8, X, F11.8, X, F11.8, X, F11.8, X, F11.8)'
+        character(len=*), parameter :: ce_format = '(A6 ,X , F11.8)'
+        real(dp) :: cr(3), cf(3)
+        integer :: u, k, j
+        open(newunit=u, file=path, status="unknown", position="append", action="write")
+        write(u, '(A5)') 'begin'
+        do k = 1, 3
+            write(u, lattice_format) 'lattice', simbox(:, k) * ang2bohr
+        end do
+        do j = 1, natoms
+            cr = sum(r(:, :, j), dim=2) / nbeads
+            cf = sum(f(:, :, j), dim=2) / nbeads
+            write(u, atom_format) 'atom', cr * ang2bohr, names(j), 0.0_dp, 0.0_dp, cf * evang2habohr
+        end do
+        write(u, ce_format) 'charge', 0.0_dp
+        write(u, ce_format) 'energy', sum(epot) / nbeads * ev2ha
+        write(u, '(A3)') 'end'
+        close(u)
+    end subroutine runner_frame
+
+    ! runner/input.data: output_runner's header, then every trajectory's
+    ! frames in trajectory order (the reference runs them one after another)
+    subroutine runner_join(ntraj, tids)
+        integer, intent(in) :: ntraj
+        integer(c_int32_t), intent(in) :: tids(ntraj)
+        integer :: u, v, t, ios, tv(8)
+        character(len=512) :: line
+        character(len=64) :: path
+        logical :: ex
+        open(newunit=u, file="runner/input.data", status="replace", action="write")
+        write(u, '(a)') '# This file was generated by MDT2.'
+        call date_and_time(values=tv)
+        write(u, '(a,i4, a, i2.2, a, i2.2, a, i2.2, a, i2.2)') "# ", tv(1), "-", tv(2), "-", tv(3), " - ", tv(5), &
+            ".", tv(6)
+        do t = 1, ntraj
+            write(path, '(a14, i8.8, a4)') 'runner/frames_', tids(t), '.tmp'
+            inquire(file=trim(path), exist=ex)
+            if (.not. ex) cycle
+            open(newunit=v, file=trim(path), status="old", action="read")
+            do
+                read(v, '(a)', iostat=ios) line
+                if (ios /= 0) exit
+                write(u, '(a)') trim(line)
+            end do
+            close(v, status="delete")
+        end do
+        close(u)
+    end subroutine runner_join
 
     ! o: device observables; fnz: any projectile force component /= 0 in
     ! the final state; nsteps: steps run since mdt_batch_set_observe

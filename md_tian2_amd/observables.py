@@ -11,7 +11,10 @@ trajectory whose projectile leaves through `pul` / `pll`
   initial block after the first calc_force, final block after the loop;
 * energy/traj_%08d.dat  -- output_nrg (src/output_mod.f90:170-229) every
   `interval` steps;
-* conf/ads_%08d.dat     -- output_adsorption_status (:792-818).
+* conf/ads_%08d.dat     -- output_adsorption_status (:792-818);
+* runner/input.data     -- output_runner (:550-609): RuNNer training
+  frames (bead centroids in Bohr, centroid forces in Ha/Bohr, the
+  bead-averaged energy in Ha), one file for the run in trajectory order.
 
 Numbers are written with the reference's Fortran edit descriptors (Fw.d,
 Ew.d, Ew.dEe, I, L with ifort's default widths for I and L: 12 and 2).
@@ -40,6 +43,11 @@ import os
 import numpy as np
 
 RAD2DEG = 180.0 / math.acos(-1.0)  # constants.f90:36, 159
+BOHR2ANG = 0.529177211               # constants.f90:160-168
+ANG2BOHR = 1.0 / BOHR2ANG
+HA2EV = 27.21138602
+EV2HA = 1.0 / HA2EV
+EVANG2HABOHR = 1.0 / (HA2EV * ANG2BOHR)
 BIG = np.finfo(np.float64).max      # huge(0_dp)
 
 ENERGY_LABELS = ("#time/fs", "atom T/K", "bead T/K", "r_gyr_sq_p/A²", "r_gyr_sq_l/A²", "atom ekin proj",
@@ -186,21 +194,53 @@ def adsorption_line(istep, dt, adsorbed):
     return efmt((istep - 1) * dt, 13, 6, 2) + lfmt(adsorbed) + "\n"
 
 
+def runner_header(now=None):
+    """The two comment lines output_runner starts input.data with."""
+    import datetime
+    t = now or datetime.datetime.now()
+    return "# This file was generated by MDT2.\n" + "# %4d-%02d-%02d - %02d.%02d\n" % (t.year, t.month, t.day,
+                                                                                    t.hour, t.minute)
+
+
+def runner_frame_text(simbox, symbols, r, f, epot):
+    """One RuNNer structure: simbox (3, 3) with the cell vectors as columns
+    [A], symbols per atom, r and f (N, P, 3) [A, eV/A], epot (P,) [eV]."""
+    lines = ["begin"]
+    for k in range(3):
+        lines.append("lattice" + "".join(" " + ffmt(x * ANG2BOHR, 11, 8) for x in np.asarray(simbox)[:, k]))
+    P = r.shape[1]
+    cr = r.sum(axis=1) / P if P > 1 else r[:, 0, :]
+    cf = f.sum(axis=1) / P if P > 1 else f[:, 0, :]
+    for j in range(r.shape[0]):
+        lines.append("atom" + "".join(" " + ffmt(x * ANG2BOHR, 16, 9) for x in cr[j]) + " " + symbols[j][:3].ljust(3)
+                     + " " + ffmt(0.0, 11, 8) + " " + ffmt(0.0, 11, 8)
+                     + "".join(" " + ffmt(x * EVANG2HABOHR, 11, 8) for x in cf[j]))
+    lines.append("charge " + ffmt(0.0, 11, 8))
+    lines.append("energy " + ffmt(float(np.sum(epot)) / len(epot) * EV2HA, 11, 8))
+    lines.append("end")
+    return "\n".join(lines) + "\n"
+
+
 class Output:
     """The reference's output folders for trajectories traj_ids (md_tian.inp
     `output scatter 1 energy 10 adsorption_status 1`; intervals 0 = off)."""
 
-    def __init__(self, folder, traj_ids, scatter=True, energy=0, adsorption=0):
+    def __init__(self, folder, traj_ids, scatter=True, energy=0, adsorption=0, runner=0):
         self.folder = folder
         self.traj_ids = list(traj_ids)
         self.scatter, self.energy, self.adsorption = bool(scatter), int(energy), int(adsorption)
+        self.runner = int(runner)
         self.first = min(self.traj_ids) if self.traj_ids else 0
-        for sub, on in (("traj", self.scatter), ("energy", self.energy), ("conf", self.adsorption)):
+        self.frames = {t: [] for t in self.traj_ids}   # runner frames, written in trajectory order
+        for sub, on in (("traj", self.scatter), ("energy", self.energy), ("conf", self.adsorption),
+                        ("runner", self.runner)):
             if on:
                 os.makedirs(os.path.join(folder, sub), exist_ok=True)
         self._opened = set()
 
     def path(self, kind, tid):
+        if kind == "runner":
+            return os.path.join(self.folder, "runner", "input.data")
         name = {"scatter": "traj/mxt_fin%08d.dat", "energy": "energy/traj_%08d.dat",
                 "ads": "conf/ads_%08d.dat"}[kind] % tid
         return os.path.join(self.folder, name)
@@ -233,7 +273,7 @@ def run(batch, nsteps, out, proj_ul=None, proj_ll=None, adsorption=None):
                 f.write(scatter_initial_text(est[t]))
     proj_first = all(s.type_names[s.idx[0]][1] for s in batch.systems)
     batch.set_observe(proj_ul, proj_ll, adsorption, exit_at_limits=out.scatter and proj_first)
-    intervals = [i for i in (out.energy, out.adsorption) if i > 0]
+    intervals = [i for i in (out.energy, out.adsorption, out.runner) if i > 0]
     cur = 0
     while cur < nsteps:
         nxt = nsteps
@@ -243,9 +283,11 @@ def run(batch, nsteps, out, proj_ul=None, proj_ll=None, adsorption=None):
         cur = nxt
         due_e = out.energy and cur % out.energy == 0
         due_a = out.adsorption and cur % out.adsorption == 0
+        due_r = out.runner and cur % out.runner == 0
         obs = batch.observables()
-        if due_e or due_a:
+        if due_e or due_a or due_r:
             est = batch.estimators() if due_e else None
+            state = batch.get_state() if due_r else None
             for t in range(B):
                 ex = obs[t]["exit_step"]
                 if ex and ex < cur:
@@ -255,6 +297,10 @@ def run(batch, nsteps, out, proj_ul=None, proj_ll=None, adsorption=None):
                     out._append("energy", tid, energy_line(cur, dt, est[t]), energy_header())
                 if due_a:
                     out._append("ads", tid, adsorption_line(cur, dt, obs[t]["adsorbed_prev"]))
+                if due_r:
+                    s = batch.systems[t]
+                    out.frames[tid].append(runner_frame_text(s.simbox, [s.type_names[k][0] for k in s.idx],
+                                                             state[t]["r"], state[t]["f"], state[t]["epot"]))
         if all(o["exit_step"] for o in obs):
             break
     obs = batch.observables()
@@ -267,4 +313,9 @@ def run(batch, nsteps, out, proj_ul=None, proj_ll=None, adsorption=None):
         if out.scatter:
             with open(out.path("scatter", out.traj_ids[t]), "a") as f:
                 f.write(scatter_final_text(est[t], fin))
+    if out.runner:
+        with open(out.path("runner", 0), "w") as f:
+            f.write(runner_header())
+            for tid in sorted(out.traj_ids):
+                f.writelines(out.frames[tid])
     return fins

@@ -61,6 +61,18 @@ def generate_frames(ctx, ntraj=2000, nsteps=1000, every=10, seed=7, n=3, nlayers
     return systems[0], np.concatenate(R), np.concatenate(E), np.concatenate(F)
 
 
+def write_input_data(path, template, R, E, F):
+    """Frames of generate_frames as a RuNNer input.data file (the reference's
+    output_runner format, src/output_mod.f90:550-609): Bohr, Hartree."""
+    from md_tian2_amd import observables
+    symbols = [template.type_names[k][0] for k in template.idx]
+    with open(path, "w") as f:
+        f.write(observables.runner_header())
+        for k in range(len(E)):
+            f.write(observables.runner_frame_text(template.simbox, symbols, R[k][:, None, :], F[k][:, None, :],
+                                                  np.array([E[k]])))
+
+
 def train(ctx, sets, epochs=60, batch_frames=64, lr=2e-3, wE=1.0, wF=10.0, seed=11, log=None):
     """Designs, scales and trains the H/Au network on the GPU from frame
     sets [(template System, R, E, F), ...] (generate_frames's output, one
@@ -120,6 +132,8 @@ def main(argv=None):
     t.add_argument("--layers", default="4", help="slab layer counts of the frame sets (3x3 cells)")
     t.add_argument("--tsurf", default="300", help="Tsurf [K], or a range lo,hi drawn per trajectory")
     t.add_argument("--out", default=MODEL_DIR)
+    t.add_argument("--input-data", default=None,
+                   help="also write the EMT frames as RuNNer input.data files (<path>.<layers>)")
     a = ap.parse_args(argv)
     ctx = mdt.Context(0)
     pes = pesfile.load_pes(PES_HAU)
@@ -135,6 +149,9 @@ def main(argv=None):
                             tsurf=trange, log=sys.stderr) for k, nl in enumerate(layers)]
     print(f"[surrogate] {sum(len(x[2]) for x in sets)} EMT frames ({', '.join(f'{nl} layers' for nl in layers)}) "
           f"in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    if a.input_data:
+        for nl, st in zip(layers, sets):
+            write_input_data(f"{a.input_data}.{nl}", *st)
     m, hist = train(ctx, sets, epochs=a.epochs, batch_frames=a.batch, lr=a.lr, log=sys.stderr)
     nene.write(m, a.out)
     for h in hist:

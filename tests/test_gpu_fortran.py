@@ -194,7 +194,7 @@ def test_fortran_driver_scatter_output(tmp_path, pes_hau):
                                                                                    "EMT-HAu.pes"))
     deck = deck.replace("nsteps 30", "nsteps 500").replace("Einc 1.92", "Einc 4.0").replace("pip r r 3.0",
                                                                                           "pip r r 4.0")
-    deck += "\npul 4.3\nadsorption_distance 1.9 2.6\noutput scatter 1 energy 50 adsorption_status 25\n"
+    deck += "\npul 4.3\nadsorption_distance 1.9 2.6\noutput scatter 1 energy 50 adsorption_status 25 runner 100\n"
     p = tmp_path / "scatter.inp"
     p.write_text(deck)
     exe = os.path.join(ROOT, "md_tian2_amd", "fortran", "mdt_run")
@@ -204,7 +204,7 @@ def test_fortran_driver_scatter_output(tmp_path, pes_hau):
     d = read_dump(str(dump))
     tmpl = geometry.h_on_slab("Au", n=3, nlayers=4, nfixed=2, thermal=False)
     cols, pesm = pesfile.emt_tables(pes_hau, tmpl.type_names)
-    n_exit = 0
+    n_exit = nframes = 0
     for t in range(d["ntraj"]):
         tid = int(d["tid"][t])
         s = tmpl.copy()
@@ -232,4 +232,8 @@ def test_fortran_driver_scatter_output(tmp_path, pes_hau):
         assert len(en) == min(500, iend) // 50
         ads = open(tmp_path / "conf" / ("ads_%08d.dat" % tid)).read().splitlines()
         assert len(ads) == min(500, iend) // 25
+        nframes += min(500, iend) // 100
     print(f"driver scatter output: {n_exit} of {d['ntraj']} trajectories left through pul")
+    from md_tian2_amd import nene
+    frames = nene.read_input_data(str(tmp_path / "runner" / "input.data"))
+    assert len(frames) == nframes and not list((tmp_path / "runner").glob("*.tmp"))

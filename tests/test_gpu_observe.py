@@ -160,7 +160,8 @@ def test_output_files(gpu_ctx, pes_hau, tmp_path):
     nsteps = 600
     refs = _oracle_runs(systems, cols, pesm, nsteps, 300.0, 0.0)
     b = mdt.Batch(gpu_ctx, systems, dt=DT, tsurf=300.0, tproj=0.0, seed=SEED, traj_ids=list(range(4)))
-    out = observables.Output(str(tmp_path), [101, 102, 103, 104], scatter=True, energy=50, adsorption=25)
+    out = observables.Output(str(tmp_path), [101, 102, 103, 104], scatter=True, energy=50, adsorption=25,
+                             runner=100)
     fins = observables.run(b, nsteps, out, proj_ul=PUL, adsorption=ADS)
     b.close()
     for t, (st, o, iend) in enumerate(refs):
@@ -178,6 +179,17 @@ def test_output_files(gpu_ctx, pes_hau, tmp_path):
         assert len(en) == min(nsteps, iend) // 50
         ads = open(out.path("ads", 101 + t)).read().splitlines()
         assert len(ads) == min(nsteps, iend) // 25
+    from md_tian2_amd import nene
+    frames = nene.read_input_data(out.path("runner", 0))
+    counts = [min(nsteps, iend) // 100 for (_, _, iend) in refs]
+    assert len(frames) == sum(counts)
+    k = 0
+    for t, (st, o, iend) in enumerate(refs):
+        k += counts[t]
+        if iend > nsteps:   # the trajectory's last frame is its final state
+            fr = frames[k - 1]
+            assert np.abs(fr["xyz"] * observables.BOHR2ANG - st.r[:, 0, :]).max() <= 2e-7
+            assert abs(fr["energy"] * observables.HA2EV - st.epot[0]) <= 1e-6
     head = open(out.path("energy", 101)).read().splitlines()[:3]
     assert head[0] == "# All energies are given in eV." and head[2].split()[0] == "#time/fs"
     assert not open(out.path("energy", 102)).read().startswith("#")

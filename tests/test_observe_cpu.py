@@ -179,7 +179,7 @@ FMT_VALUES = [0.0, 1.0, -1.0, 0.5, 123.456789012345, -392.20940943, 1.0e-5, -2.5
                     reason="no Fortran compiler")
 def test_fortran_edit_descriptors(tmp_path):
     fc = shutil.which("flang") or "/opt/rocm/lib/llvm/bin/flang"
-    descs = ["f14.7", "e21.10", "e17.8e2", "e13.6e2"]
+    descs = ["f14.7", "e21.10", "e17.8e2", "e13.6e2", "f11.8", "f16.9"]
     vals = ", ".join("%.17e_8" % v for v in FMT_VALUES)
     src = ["program fmt", "  implicit none", "  real(8) :: x(%d) = [%s]" % (len(FMT_VALUES), vals),
            "  integer :: i", "  do i = 1, size(x)"]
@@ -192,7 +192,39 @@ def test_fortran_edit_descriptors(tmp_path):
     py = []
     for v in FMT_VALUES:
         py += [observables.ffmt(v, 14, 7), observables.efmt(v, 21, 10), observables.efmt(v, 17, 8, 2),
-               observables.efmt(v, 13, 6, 2)]
+               observables.efmt(v, 13, 6, 2), observables.ffmt(v, 11, 8), observables.ffmt(v, 16, 9)]
     py.append(observables.ifmt(42))
     got = [line.strip()[1:-1] for line in out]
     assert got == py
+
+
+def test_runner_frames_read_back(tmp_path):
+    """input.data frames (output_runner layout) read back by the RuNNer
+    reader: cell, symbols, bead-centroid positions in Bohr, centroid forces
+    in Ha/Bohr and the bead-averaged energy in Ha, to the printed digits."""
+    from md_tian2_amd import nene, surrogate
+    rng = np.random.default_rng(5)
+    s = geometry.h_on_slab("Au", n=3, nlayers=2, rng=rng, nbeads=4)
+    r = s.r + rng.normal(0, 0.05, s.r.shape)
+    f = rng.normal(0, 0.3, s.r.shape)
+    ep = np.array([-0.31, -0.29, -0.33, -0.30])
+    path = tmp_path / "input.data"
+    path.write_text(observables.runner_header() + observables.runner_frame_text(
+        s.simbox, [s.type_names[k][0] for k in s.idx], r, f, ep))
+    (st,) = nene.read_input_data(str(path))
+    b2a = observables.BOHR2ANG
+    assert np.abs(st["lattice"] - s.simbox.T / b2a).max() <= 5e-9
+    assert st["symbols"] == ["H"] + ["Au"] * 18
+    assert np.abs(st["xyz"] - r.mean(axis=1) / b2a).max() <= 5e-10
+    assert np.abs(st["forces"] - f.mean(axis=1) / observables.HA2EV * b2a).max() <= 5e-9
+    assert abs(st["energy"] - ep.mean() / observables.HA2EV) <= 5e-9
+    lines = path.read_text().splitlines()
+    assert lines[0] == "# This file was generated by MDT2." and lines[2] == "begin" and lines[-1] == "end"
+    # the surrogate's frame export is the same format, one frame per row of R
+    R = np.stack([r[:, 0, :], r[:, 1, :]])
+    F = np.stack([f[:, 0, :], f[:, 1, :]])
+    s1 = s.copy()
+    surrogate.write_input_data(str(tmp_path / "frames.data"), s1, R, np.array([-1.0, -2.0]), F)
+    fr = nene.read_input_data(str(tmp_path / "frames.data"))
+    assert len(fr) == 2 and abs(fr[1]["energy"] + 2.0 / observables.HA2EV) <= 5e-9
+    assert np.abs(fr[1]["xyz"] - R[1] / b2a).max() <= 5e-10
