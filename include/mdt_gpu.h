@@ -221,6 +221,18 @@ int mdt_batch_set_precision(mdt_batch *b, int mode);
 int mdt_batch_set_bead_mode(mdt_batch *b, int mode);
 
 /*
+ * COM velocity removal after propagate_1 (remove_com_velocity,
+ * src/universe_mod.f90:222-241, called from src/md_algo.f90:71-74 whenever an
+ * Andersen or PILE atom is present): on (1, the default) or off (0).  Off is
+ * an extension for thermostatting a bare slab with fixed layers (thermal
+ * slab libraries): the removal runs over fixed and mobile atoms alike, damps
+ * the mobile atoms' collective motion every step and lowers their
+ * temperature (by about 11 % for a 3x3x4 Au(111) slab with 2 fixed layers
+ * under Andersen at 300 K).
+ */
+int mdt_batch_set_com_removal(mdt_batch *b, int on);
+
+/*
  * Trajectory observation: collect_trajectory_characteristics
  * (src/trajectory_info.f90:47-263) after each step's propagate_2, and the
  * early end of a trajectory (src/md_tian2.f90:88-92).  The call resets the
@@ -405,6 +417,71 @@ int mdt_nn_train_eval(mdt_nntrain *tr, int set, double *rmse_e, double *rmse_f);
 int mdt_nn_train_loss_grad(mdt_nntrain *tr, int nsel, const int32_t *frames, double wE, double wF, double *loss,
                            double *grad);
 int mdt_nn_get_weights(mdt_context *ctx, double *weights);
+
+/* ------------------------------------------------------------------
+ * Initial conditions of a trajectory (host code, no GPU needed).  Every
+ * draw is a uniform of the Philox4x32-10 stream keyed by (seed, trajectory
+ * id) at its own slot, so a trajectory's initial conditions depend only on
+ * its id and the seed (the reference draws them in sequence from Fortran's
+ * random_number, whose stream is not reproducible across compilers).
+ * Arrays are the reference's: r, v (3, nbeads, natoms) column-major, idx
+ * 1-based types, is_proj per type, simbox (3,3) column-major (columns are
+ * the cell vectors).
+ * ------------------------------------------------------------------ */
+#define MDT_SLOT_PROJ_FILE 0u
+#define MDT_SLOT_LATT_FILE 1u
+#define MDT_SLOT_PIP 2u       /* 2, 3, 4: x, y, z */
+#define MDT_SLOT_EINC 5u      /* 5, 6 */
+#define MDT_SLOT_AZIMUTH 7u   /* 7 (random) or 7, 8 (normal) */
+#define MDT_SLOT_MAXWELL 16u  /* 16 + 2 (3 i + c) and the next */
+
+/* pip x y z (src/run_config.f90: pip key): random[k] for "r" */
+typedef struct mdt_pip {
+    int32_t random[3];
+    double value[3];
+} mdt_pip;
+
+/* Einc <e> [sigma], polar <deg>, azimuth <deg|r> [sigma]
+ * (src/run_config.f90:361-400) */
+typedef struct mdt_incidence {
+    double einc, sigma_einc;
+    double polar_deg;
+    int32_t azimuth_random;
+    int32_t pad_;
+    double azimuth_deg, sigma_azimuth;
+} mdt_incidence;
+
+/* uniform in [0, 1) at a slot of the trajectory's sampling stream */
+double mdt_sample_uniform(uint32_t seed, uint32_t traj_id, uint32_t slot);
+
+/* the file of a conf folder: int(rnd * n) + 1 (prepare_next_traj,
+ * src/md_init.f90:904-926); which = 0 projectile folder, 1 lattice folder */
+int mdt_sample_conf_index(uint32_t seed, uint32_t traj_id, int which, int n);
+
+/* apply_pip (src/md_init.f90:498-554) on the projectile universe (r
+ * in-out, cartesian), with that universe's cell: each fixed component is
+ * added to every bead; for each random component k the whole target vector
+ * is scaled by the column sum of simbox(:,k); then every atom's beads are
+ * moved so that the centroid sits at simbox . target (plus the fixed
+ * components). */
+int mdt_apply_pip(const mdt_pip *pip, uint32_t seed, uint32_t traj_id, int natoms, int nbeads,
+                  const double *simbox, double *r);
+
+/* set_proj_incidence (src/md_init.f90:829-877): Einc drawn from a normal of
+ * width sigma_einc, azimuth uniform (random) or normal of width
+ * sigma_azimuth, vinc = sqrt(2 Einc / projectile mass); each leading
+ * projectile atom loses its bead-centroid velocity and gains
+ * vinc (sin(polar) cos(az), sin(polar) sin(az), -cos(polar)). */
+int mdt_set_proj_incidence(const mdt_incidence *inc, uint32_t seed, uint32_t traj_id, int natoms, int nbeads,
+                           const int32_t *idx, const int32_t *is_proj, const double *mass, double *v);
+
+/* Maxwell-Boltzmann velocities at `temperature` for the non-projectile
+ * atoms (the same in every bead; fixed components 0).  No counterpart in
+ * the reference, which reads thermalised slabs from mxt files; for
+ * generated slabs. */
+int mdt_sample_maxwell(double temperature, uint32_t seed, uint32_t traj_id, int natoms, int nbeads,
+                       const int32_t *idx, const int32_t *is_proj, const double *mass, const int32_t *is_fixed,
+                       double *v);
 
 #ifdef __cplusplus
 }

@@ -175,6 +175,7 @@ struct mdt_batch {
     double obs_ul = 0, obs_ll = 0, ads_start = 0, ads_end = 0;
     int obs_exit = 0;
     uint32_t obs_step0 = 0;
+    int no_com = 0;  // mdt_batch_set_com_removal(b, 0)
     double *d_est = nullptr;  // MDT_NEST per trajectory
 
     BatchView view() const {
@@ -791,6 +792,7 @@ static int ensure_ready(mdt_batch *b, bool need_prop) {
         if (rc) return rc;
     }
     b->pc.mixed = b->bead_mode == 1 ? 1 : 0;
+    b->pc.no_com = b->no_com;
     return push_pc(b);
 }
 
@@ -1239,6 +1241,15 @@ int mdt_batch_set_precision(mdt_batch *b, int mode) {
     HIPCHK(hipSetDevice(b->ctx->device));
     if (mode != b->precision) drop_graph(b);
     b->precision = mode;
+    return 0;
+}
+
+int mdt_batch_set_com_removal(mdt_batch *b, int on) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    HIPCHK(hipSetDevice(b->ctx->device));
+    const int off = on ? 0 : 1;
+    if (off != b->no_com) drop_graph(b);
+    b->no_com = off;
     return 0;
 }
 

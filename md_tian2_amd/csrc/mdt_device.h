@@ -65,7 +65,7 @@ struct PropConst {
     int any_langevin, any_pile, any_andersen;
     int mixed;  // 1: atoms of non-projectile types are classical (one bead, kept in bead 0)
     uint32_t seed;
-    uint32_t pad2_;
+    int no_com;  // 1: no remove_com_velocity after propagate_1 (an extension; see mdt_batch_set_com_removal)
 };
 
 struct BatchView {

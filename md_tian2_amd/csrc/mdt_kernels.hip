@@ -952,7 +952,7 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
             }
         }
         __syncthreads();
-        if (pc.any_andersen || pc.any_pile) {
+        if ((pc.any_andersen || pc.any_pile) && !pc.no_com) {
             // remove_com_velocity, closed form of the 1000 iterations
             if (lane < 64) {
                 double ax = 0, ay = 0, az = 0, am = 0;

@@ -26,6 +26,18 @@ module mdt_gpu
 
     integer(c_int), parameter :: MDT_NEST = 20
 
+    ! initial conditions (mdt_sample.hip)
+    type, bind(C) :: mdt_pip
+        integer(c_int32_t) :: random(3)
+        real(c_double) :: value(3)
+    end type mdt_pip
+
+    type, bind(C) :: mdt_incidence
+        real(c_double) :: einc, sigma_einc, polar_deg
+        integer(c_int32_t) :: azimuth_random, pad_
+        real(c_double) :: azimuth_deg, sigma_azimuth
+    end type mdt_incidence
+
     interface
 
         function mdt_last_error(buf, buflen) bind(C, name="mdt_last_error") result(n)
@@ -215,6 +227,53 @@ module mdt_gpu
             import :: c_int, c_ptr, c_double
             type(c_ptr), value :: batch
             real(c_double), intent(out) :: est(*)
+            integer(c_int) :: rc
+        end function
+
+        function mdt_sample_uniform(seed, traj_id, slot) bind(C, name="mdt_sample_uniform") result(x)
+            import :: c_int32_t, c_double
+            integer(c_int32_t), value :: seed, traj_id, slot
+            real(c_double) :: x
+        end function
+
+        function mdt_sample_conf_index(seed, traj_id, which, n) bind(C, name="mdt_sample_conf_index") result(k)
+            import :: c_int32_t, c_int
+            integer(c_int32_t), value :: seed, traj_id
+            integer(c_int), value :: which, n
+            integer(c_int) :: k
+        end function
+
+        function mdt_apply_pip(pip, seed, traj_id, natoms, nbeads, simbox, r) bind(C, name="mdt_apply_pip") result(rc)
+            import :: c_int32_t, c_int, c_double, mdt_pip
+            type(mdt_pip), intent(in) :: pip
+            integer(c_int32_t), value :: seed, traj_id
+            integer(c_int), value :: natoms, nbeads
+            real(c_double), intent(in) :: simbox(*)
+            real(c_double), intent(inout) :: r(*)
+            integer(c_int) :: rc
+        end function
+
+        function mdt_set_proj_incidence(inc, seed, traj_id, natoms, nbeads, idx, is_proj, mass, v) &
+                bind(C, name="mdt_set_proj_incidence") result(rc)
+            import :: c_int32_t, c_int, c_double, mdt_incidence
+            type(mdt_incidence), intent(in) :: inc
+            integer(c_int32_t), value :: seed, traj_id
+            integer(c_int), value :: natoms, nbeads
+            integer(c_int32_t), intent(in) :: idx(*), is_proj(*)
+            real(c_double), intent(in) :: mass(*)
+            real(c_double), intent(inout) :: v(*)
+            integer(c_int) :: rc
+        end function
+
+        function mdt_sample_maxwell(temperature, seed, traj_id, natoms, nbeads, idx, is_proj, mass, is_fixed, v) &
+                bind(C, name="mdt_sample_maxwell") result(rc)
+            import :: c_int32_t, c_int, c_double
+            real(c_double), value :: temperature
+            integer(c_int32_t), value :: seed, traj_id
+            integer(c_int), value :: natoms, nbeads
+            integer(c_int32_t), intent(in) :: idx(*), is_proj(*), is_fixed(*)
+            real(c_double), intent(in) :: mass(*)
+            real(c_double), intent(inout) :: v(*)
             integer(c_int) :: rc
         end function
 

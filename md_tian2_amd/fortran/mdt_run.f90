@@ -46,12 +46,13 @@ program mdt_run
 
     ! deck
     integer :: ntrajs = 1, start = 1, nsteps = 0, nbeads = 1, nx = 3, ny = 3, nlayers = 4, nfixed = 2
-    integer :: rng_seed = 12345
+    integer :: rng_seed = 2   ! key of the noise and sampling streams; 2 = traj_id, the reference's default
     real(dp) :: dt = 0.1_dp, einc = 1.0_dp, polar = 0.0_dp, tsurf = 0.0_dp, tproj = 0.0_dp
-    real(dp) :: pile_tau = 200.0_dp, andersen_time = 30.0_dp, alat = 4.078_dp, pip_z = 6.0_dp
+    real(dp) :: pile_tau = 200.0_dp, andersen_time = 30.0_dp, alat = 4.078_dp
+    real(dp) :: sigma_einc = 0.0_dp, sigma_azimuth = 0.0_dp
     real(dp) :: mproj = 1.00794_dp, mlatt = 196.966569_dp
     character(len=8) :: algo_proj = "ver", algo_latt = "lan", azimuth = "0"
-    character(len=8) :: pip_x = "r", pip_y = "r"
+    character(len=32) :: pip_w(3) = ["r  ", "r  ", "6.0"]
     character(len=3) :: name_proj = "H", name_latt = "Au"
     character(len=512) :: pes_file = "", dump_file = "", deck = "", nene_dir = ""
     integer :: prec = 0, graph_chunk = 0, judge_interval = 0, bead_mode = 0
@@ -332,9 +333,17 @@ contains
             case ("nsteps"); read(w(2), *) nsteps
             case ("step"); read(w(2), *) dt
             case ("Einc"); read(w(2), *) einc; einc_set = .true.
-            case ("polar"); read(w(2), *) polar; polar = polar * pi / 180.0_dp
+                if (nw >= 3) read(w(3), *) sigma_einc
+            case ("polar"); read(w(2), *) polar
             case ("azimuth"); azimuth = w(2)
-            case ("pip"); pip_x = w(2); pip_y = w(3); read(w(4), *) pip_z; pip_set = .true.
+                if (nw >= 3) read(w(3), *) sigma_azimuth
+            case ("pip")
+                if (nw /= 4) then
+                    print '(a)', "Error: pip key needs 3 arguments"
+                    stop 1
+                end if
+                pip_w = w(2:4)
+                pip_set = .true.
             case ("Tsurf"); read(w(2), *) tsurf
             case ("Tproj"); read(w(2), *) tproj
             case ("pile_tau"); read(w(2), *) pile_tau
@@ -345,7 +354,12 @@ contains
             case ("pes"); pes_file = unquote(w(2))
             case ("slab"); read(w(2), *) alat; read(w(3), *) nx; read(w(4), *) ny
                 read(w(5), *) nlayers; read(w(6), *) nfixed
-            case ("rng_seed"); read(w(2), *) rng_seed
+            case ("rng_seed")  ! global / traj_id (src/run_config.f90:190-203) or a key
+                select case (trim(w(2)))
+                case ("global"); rng_seed = 1
+                case ("traj_id"); rng_seed = 2
+                case default; read(w(2), *) rng_seed
+                end select
             case ("dump"); dump_file = unquote(w(2))
             case ("nene"); nene_dir = unquote(w(2))
             case ("judge"); read(w(2), *) judge_interval
@@ -530,58 +544,72 @@ contains
         inv(3,3) = (m(1,1) * m(2,2) - m(1,2) * m(2,1)) / det
     end function invert3
 
-    ! Trajectory initial conditions: projectile at (pip x, pip y, z) with
-    ! v = sqrt(2 E/m)(sin t cos p, sin t sin p, -cos t) (src/md_init.f90:829-877),
-    ! mobile slab atoms Maxwell-Boltzmann at Tsurf; host RNG seeded per
-    ! trajectory id.
+    ! Trajectory initial conditions on the generated slab: the projectile
+    ! placed by apply_pip (pip, with the slab's cell) and given its incidence
+    ! velocity by set_proj_incidence (src/md_init.f90:498-554, 829-877),
+    ! mobile slab atoms Maxwell-Boltzmann at Tsurf; every draw from the
+    ! library's sampling stream of (rng_seed, trajectory id).
     subroutine initial_conditions(u, traj)
         type(universe), intent(inout) :: u
         integer, intent(in) :: traj
-        integer :: nseed, k, c
-        integer, allocatable :: seedv(:)
-        real(dp) :: fx, fy, phi, vinc, sig, u1, u2
+        type(universe) :: p1
+        integer(c_int32_t), allocatable :: fx(:)
+        integer :: k_
         if (len_trim(conf_kind) > 0) then
             call conf_universe(u, traj)
             return
         end if
         call build_template(u)
-        call random_seed(size=nseed)
-        allocate(seedv(nseed))
-        seedv = traj * 7919 + 17
-        call random_seed(put=seedv)
-        call random_number(fx)
-        call random_number(fy)
-        if (trim(pip_x) /= "r") read(pip_x, *) fx
-        if (trim(pip_y) /= "r") read(pip_y, *) fy
-        u%r(:, :, 1) = spread(u%simbox(:, 1) * fx + u%simbox(:, 2) * fy, 2, nbeads)
-        u%r(3, :, 1) = pip_z
-        if (trim(azimuth) == "r") then
-            call random_number(phi)
-            phi = 2.0_dp * pi * phi
-        else
-            read(azimuth, *) phi
-            phi = phi * pi / 180.0_dp
-        end if
-        vinc = sqrt(2.0_dp * einc / u%m(1))
+        p1 = universe_alloc(nbeads, 1, 1)
+        p1%r(:, :, 1) = u%r(:, :, 1)
+        call mdt_check(mdt_apply_pip(pip_struct(), int(rng_seed, c_int32_t), int(traj, c_int32_t), 1_c_int, &
+                                     int(nbeads, c_int), u%simbox, p1%r), "mdt_apply_pip")
+        u%r(:, :, 1) = p1%r(:, :, 1)
         u%v = 0.0_dp
-        u%v(1, :, 1) = vinc * sin(polar) * cos(phi)
-        u%v(2, :, 1) = vinc * sin(polar) * sin(phi)
-        u%v(3, :, 1) = -vinc * cos(polar)
-        if (tsurf > 0.0_dp) then
-            do k = 2, u%natoms
-                sig = sqrt(kB * tsurf / u%m(k))
-                do c = 1, 3
-                    call random_number(u1)
-                    call random_number(u2)
-                    if (.not. u%is_fixed(c, 1, k)) u%v(c, :, k) = sig * sqrt(-2.0_dp * log(1.0_dp - u1)) &
-                                                                  * cos(2.0_dp * pi * u2)
-                end do
-            end do
-        end if
-        do k = 2, u%natoms
-            u%r(:, :, k) = spread(u%r(:, 1, k), 2, nbeads)
+        call incidence(u, traj)
+        allocate(fx(3 * u%nbeads * u%natoms))
+        fx = reshape(merge(1_c_int32_t, 0_c_int32_t, u%is_fixed), [size(fx)])
+        call mdt_check(mdt_sample_maxwell(tsurf, int(rng_seed, c_int32_t), int(traj, c_int32_t), &
+                                          int(u%natoms, c_int), int(u%nbeads, c_int), int(u%idx, c_int32_t), &
+                                          int(merge(1, 0, u%is_proj), c_int32_t), u%m, fx, u%v), "mdt_sample_maxwell")
+        do k_ = 2, u%natoms
+            u%r(:, :, k_) = spread(u%r(:, 1, k_), 2, nbeads)
         end do
     end subroutine initial_conditions
+
+    function pip_struct() result(p)
+        type(mdt_pip) :: p
+        integer :: k
+        p%random = 0
+        p%value = 0.0_dp
+        do k = 1, 3
+            if (trim(pip_w(k)) == "r") then
+                p%random(k) = 1
+            else
+                read(pip_w(k), *) p%value(k)
+            end if
+        end do
+    end function pip_struct
+
+    ! set_proj_incidence through the library's sampler (Einc given)
+    subroutine incidence(u, traj)
+        type(universe), intent(inout) :: u
+        integer, intent(in) :: traj
+        type(mdt_incidence) :: inc
+        if (.not. einc_set) return
+        inc%einc = einc
+        inc%sigma_einc = sigma_einc
+        inc%polar_deg = polar
+        inc%azimuth_random = merge(1, 0, trim(azimuth) == "r")
+        inc%pad_ = 0
+        inc%azimuth_deg = 0.0_dp
+        if (inc%azimuth_random == 0) read(azimuth, *) inc%azimuth_deg
+        inc%sigma_azimuth = sigma_azimuth
+        call mdt_check(mdt_set_proj_incidence(inc, int(rng_seed, c_int32_t), int(traj, c_int32_t), &
+                                              int(u%natoms, c_int), int(u%nbeads, c_int), int(u%idx, c_int32_t), &
+                                              int(merge(1, 0, u%is_proj), c_int32_t), u%m, u%v), &
+                       "mdt_set_proj_incidence")
+    end subroutine incidence
 
     ! The universe of trajectory `traj` from the conf files (read_geometry,
     ! src/md_init.f90:160-192, and post_process :630-671): the file, or file
@@ -597,52 +625,31 @@ contains
         type(universe), intent(inout) :: u
         integer, intent(in) :: traj
         type(universe) :: latt, proj, one
-        integer :: nseed, ios, b, i
-        integer, allocatable :: seedv(:)
-        real(dp) :: x, fx, fy, phi, vinc, cen(3), target(3), vc(3)
-        call random_seed(size=nseed)
-        allocate(seedv(nseed))
-        seedv = traj * 7919 + 29
-        call random_seed(put=seedv)
+        integer :: ios, b, i
+        integer(c_int32_t) :: key, tr
+        key = int(rng_seed, c_int32_t)
+        tr = int(traj, c_int32_t)
         select case (trim(conf_kind))
         case ("poscar")
             call read_poscar(one, trim(conf_path), 1, ios)
             call conf_check(ios, conf_path)
         case ("mxt")
             if (conf_n > 0) then
-                call random_number(x)
-                call read_mxt(one, mxt_file(trim(conf_path), int(x * conf_n) + 1), ios)
+                call read_mxt(one, mxt_file(trim(conf_path), int(mdt_sample_conf_index(key, tr, 1_c_int, &
+                              int(conf_n, c_int)))), ios)
             else
                 call read_mxt(one, trim(conf_path), ios)
             end if
             call conf_check(ios, conf_path)
-        case ("merge")
-            call random_number(x)
-            call read_mxt(proj, mxt_file(trim(proj_path), int(x * proj_n) + 1), ios)
+        case ("merge")  ! prepare_next_traj (src/md_init.f90:904-941)
+            call read_mxt(proj, mxt_file(trim(proj_path), int(mdt_sample_conf_index(key, tr, 0_c_int, &
+                          int(proj_n, c_int)))), ios)
             call conf_check(ios, proj_path)
-            call random_number(x)
-            call read_mxt(latt, mxt_file(trim(conf_path), int(x * conf_n) + 1), ios)
+            call read_mxt(latt, mxt_file(trim(conf_path), int(mdt_sample_conf_index(key, tr, 1_c_int, &
+                          int(conf_n, c_int)))), ios)
             call conf_check(ios, conf_path)
-            if (pip_set) then  ! the projectile's centroid to the pip target
-                call random_number(fx)
-                call random_number(fy)
-                target = 0.0_dp
-                if (trim(pip_x) == "r") then
-                    target = target + fx * latt%simbox(:, 1)
-                else
-                    read(pip_x, *) target(1)
-                end if
-                if (trim(pip_y) == "r") then
-                    target = target + fy * latt%simbox(:, 2)
-                else
-                    read(pip_y, *) target(2)
-                end if
-                target(3) = pip_z
-                cen = sum(proj%r(:, :, 1), dim=2) / proj%nbeads
-                do b = 1, proj%nbeads
-                    proj%r(:, b, 1) = proj%r(:, b, 1) - cen + target
-                end do
-            end if
+            if (pip_set) call mdt_check(mdt_apply_pip(pip_struct(), key, tr, int(proj%natoms, c_int), &
+                                                      int(proj%nbeads, c_int), proj%simbox, proj%r), "mdt_apply_pip")
             call merge_conf(latt, proj, one, ios)
             if (ios /= 0) then
                 print '(a,i0)', "Error: conf merge: incompatible files, code ", ios
@@ -677,20 +684,7 @@ contains
         do i = 1, u%natoms
             u%m(i) = merge(mproj, mlatt, u%idx(i) == 1) * amu2mass
         end do
-        if (einc_set) then
-            if (trim(azimuth) == "r") then
-                call random_number(phi)
-                phi = 2.0_dp * pi * phi
-            else
-                read(azimuth, *) phi
-                phi = phi * pi / 180.0_dp
-            end if
-            vinc = sqrt(2.0_dp * einc / u%m(1))
-            vc = sum(u%v(:, :, 1), dim=2) / u%nbeads
-            do b = 1, u%nbeads
-                u%v(:, b, 1) = u%v(:, b, 1) - vc + vinc * [sin(polar) * cos(phi), sin(polar) * sin(phi), -cos(polar)]
-            end do
-        end if
+        call incidence(u, traj)
     end subroutine conf_universe
 
     subroutine conf_check(ios, path)

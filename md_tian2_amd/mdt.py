@@ -28,8 +28,9 @@ SYMBOLS = (
     "mdt_batch_set_atoms", "mdt_batch_set_cells", "mdt_batch_set_propagation", "mdt_batch_seed",
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
-    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph", "mdt_batch_set_bead_mode",
+    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
     "mdt_batch_set_observe", "mdt_batch_get_observables", "mdt_batch_estimators",
+    "mdt_sample_uniform", "mdt_sample_conf_index", "mdt_apply_pip", "mdt_set_proj_incidence", "mdt_sample_maxwell",
     "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats", "mdt_batch_nn_fallback",
     "mdt_batch_nn_descriptors", "mdt_nn_train_create", "mdt_nn_train_destroy", "mdt_nn_train_descriptor_stats",
     "mdt_nn_train_epoch", "mdt_nn_train_eval", "mdt_nn_train_loss_grad", "mdt_nn_get_weights", "mdt_nn_load",
@@ -85,9 +86,17 @@ def lib():
         L.mdt_batch_force_timing.argtypes = [vp, dp]
         L.mdt_batch_set_precision.argtypes = [vp, C.c_int]
         L.mdt_batch_set_bead_mode.argtypes = [vp, C.c_int]
+        L.mdt_batch_set_com_removal.argtypes = [vp, C.c_int]
         L.mdt_batch_set_observe.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
         L.mdt_batch_get_observables.argtypes = [vp, C.c_void_p]
         L.mdt_batch_estimators.argtypes = [vp, dp]
+        u32 = C.c_uint32
+        L.mdt_sample_uniform.argtypes = [u32, u32, u32]
+        L.mdt_sample_uniform.restype = C.c_double
+        L.mdt_sample_conf_index.argtypes = [u32, u32, C.c_int, C.c_int]
+        L.mdt_apply_pip.argtypes = [C.c_void_p, u32, u32, C.c_int, C.c_int, dp, dp]
+        L.mdt_set_proj_incidence.argtypes = [C.c_void_p, u32, u32, C.c_int, C.c_int, ip, ip, dp, dp]
+        L.mdt_sample_maxwell.argtypes = [C.c_double, u32, u32, C.c_int, C.c_int, ip, ip, dp, ip, dp]
         L.mdt_batch_set_graph.argtypes = [vp, C.c_int]
         L.mdt_nn_params_set.argtypes = [vp, C.c_int, ip, C.c_int, ip, ip, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp,
                                         C.c_double, C.c_double, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
@@ -284,6 +293,11 @@ class Batch:
         """0: every atom carries the beads; 1 (mixed): the projectile (atom 0)
         carries them and the other atoms are classical, kept in bead 0."""
         _check(lib().mdt_batch_set_bead_mode(self.h, int(mode)))
+
+    def set_com_removal(self, on):
+        """remove_com_velocity after propagate_1 with Andersen / PILE atoms:
+        on (the reference) or off (thermostatted bare slabs)."""
+        _check(lib().mdt_batch_set_com_removal(self.h, 1 if on else 0))
 
     def set_observe(self, proj_ul=None, proj_ll=None, adsorption=None, exit_at_limits=True):
         """Trajectory observation and the exit at the projectile limits

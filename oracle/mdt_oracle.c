@@ -1458,3 +1458,70 @@ void orc_energy_row(const orc_universe *u, const orc_simparams *sp, double out[2
         out[17 + k] = va / P;
     }
 }
+
+/* ---- initial conditions with explicit uniforms ---- */
+
+/* apply_pip (md_init.f90:498-554) on a projectile universe; U[k] is the
+ * random_number drawn for a random component k */
+void orc_apply_pip(orc_universe *u, const int random[3], const double value[3], const double U[3]) {
+    const int P = u->nbeads, N = u->natoms;
+    double *cents = (double *)malloc(sizeof(double) * 3 * N);
+    for (int i = 0; i < N; ++i) centroid(u, i, cents + 3 * i);
+    double target[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 3; ++k) {
+        if (random[k]) {
+            target[k] = U[k];
+            const double cs = u->simbox[0 + 3 * k] + u->simbox[1 + 3 * k] + u->simbox[2 + 3 * k];
+            for (int j = 0; j < 3; ++j) target[j] = target[j] * cs;  /* the whole vector */
+        } else {
+            for (int b = 0; b < P; ++b)
+                for (int i = 0; i < N; ++i) u->r[I3PN(k, b, i, P)] += value[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        const double fp = u->simbox[k + 0] * target[0] + u->simbox[k + 3] * target[1] +
+                          u->simbox[k + 6] * target[2];
+        for (int b = 0; b < P; ++b)
+            for (int i = 0; i < N; ++i) u->r[I3PN(k, b, i, P)] = u->r[I3PN(k, b, i, P)] - cents[k + 3 * i] + fp;
+    }
+    free(cents);
+}
+
+/* set_proj_incidence (md_init.f90:829-877) with normal_deviate
+ * (useful_things.f90:121-131): rnd[0], rnd[1] for Einc; rnd[2] for a random
+ * azimuth, or rnd[2], rnd[3] for a normal one */
+void orc_set_proj_incidence(orc_universe *u, double einc, double sigma_einc, double polar_deg, int az_random,
+                            double az_deg, double sigma_az, const double rnd[4]) {
+    const int P = u->nbeads, N = u->natoms;
+    const double deg2rad = ORC_PI / 180.0;
+    const double drawn = sqrt(-2 * log(rnd[0])) * cos(2 * ORC_PI * rnd[1]) * sigma_einc + einc;
+    double pm = 0.0;
+    for (int i = 0; i < N; ++i)
+        if (u->is_proj[u->idx[i]]) pm = pm + u->m[i];
+    double az;
+    if (az_random) {
+        az = 2 * ORC_PI * rnd[2];
+    } else {
+        az = sqrt(-2 * log(rnd[2])) * cos(2 * ORC_PI * rnd[3]) * sigma_az + az_deg;
+        az = az * deg2rad;
+    }
+    const double polar = polar_deg * deg2rad;
+    const double vinc = sqrt(2 * drawn / pm);
+    const double vx = vinc * sin(polar) * cos(az), vy = vinc * sin(polar) * sin(az), vz = -vinc * cos(polar);
+    for (int i = 0; i < N; ++i) {
+        if (!u->is_proj[u->idx[i]]) return;
+        double vcm[3];
+        for (int c = 0; c < 3; ++c) {
+            double acc = 0.0;
+            for (int b = 0; b < P; ++b) acc += u->v[I3PN(c, b, i, P)];
+            vcm[c] = acc / P;
+        }
+        for (int b = 0; b < P; ++b)
+            for (int c = 0; c < 3; ++c) u->v[I3PN(c, b, i, P)] = u->v[I3PN(c, b, i, P)] - vcm[c];
+        for (int b = 0; b < P; ++b) {
+            u->v[I3PN(0, b, i, P)] = u->v[I3PN(0, b, i, P)] + vx;
+            u->v[I3PN(1, b, i, P)] = u->v[I3PN(1, b, i, P)] + vy;
+            u->v[I3PN(2, b, i, P)] = u->v[I3PN(2, b, i, P)] + vz;
+        }
+    }
+}

@@ -158,6 +158,12 @@ void orc_virial_quantum_ekin(const orc_universe *u, double *ekin_p, double *ekin
 double orc_bead_epot(const orc_universe *u, const orc_simparams *sp);
 void orc_energy_row(const orc_universe *u, const orc_simparams *sp, double out[20]);
 
+/* ---- initial conditions (md_init.f90:498-554, 829-877) with the
+ * uniforms given explicitly ---- */
+void orc_apply_pip(orc_universe *u, const int random[3], const double value[3], const double U[3]);
+void orc_set_proj_incidence(orc_universe *u, double einc, double sigma_einc, double polar_deg, int az_random,
+                            double az_deg, double sigma_az, const double rnd[4]);
+
 #ifdef __cplusplus
 }
 #endif

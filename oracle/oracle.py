@@ -90,6 +90,9 @@ def lib():
                                       C.c_double, C.POINTER(TrajObs), C.POINTER(C.c_int)]
         L.orc_run_scatter.restype = C.c_int
         L.orc_energy_row.argtypes = [C.POINTER(Universe), C.POINTER(SimParams), dp]
+        L.orc_apply_pip.argtypes = [C.POINTER(Universe), C.POINTER(C.c_int), dp, dp]
+        L.orc_set_proj_incidence.argtypes = [C.POINTER(Universe), C.c_double, C.c_double, C.c_double, C.c_int,
+                                             C.c_double, C.c_double, dp]
         _LIB = L
     return _LIB
 
@@ -224,6 +227,15 @@ class OracleState:
         out = np.zeros(20)
         lib().orc_energy_row(C.byref(self.u), C.byref(self.sp), _dp(out))
         return out
+
+    def apply_pip(self, random, value, U):
+        rnd = np.ascontiguousarray(random, dtype=np.int32)
+        lib().orc_apply_pip(C.byref(self.u), _ip(rnd), _dp(np.ascontiguousarray(value, dtype=np.float64)),
+                            _dp(np.ascontiguousarray(U, dtype=np.float64)))
+
+    def set_proj_incidence(self, einc, sigma_einc, polar_deg, az_random, az_deg, sigma_az, rnd):
+        lib().orc_set_proj_incidence(C.byref(self.u), einc, sigma_einc, polar_deg, 1 if az_random else 0, az_deg,
+                                     sigma_az, _dp(np.ascontiguousarray(rnd, dtype=np.float64)))
 
     def minimg(self, i, j):
         d = np.zeros(self.P)

@@ -13,7 +13,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     txt = open(os.path.join(ROOT, "include", "mdt_gpu.h")).read()
-    return sorted(set(re.findall(r"^int\s+(mdt_\w+)\s*\(", txt, flags=re.M)))
+    return sorted(set(re.findall(r"^(?:int|double)\s+(mdt_\w+)\s*\(", txt, flags=re.M)))
 
 
 def test_header_declares_the_binding_symbols():

@@ -103,3 +103,28 @@ def test_driver_reads_output_keywords(tmp_path):
         res = subprocess.run([EXE, "--check-input", str(deck)], cwd=ROOT, capture_output=True, text=True,
                              timeout=60)
         assert res.returncode != 0, bad
+
+
+def test_driver_initial_conditions_equal_the_python_sampler(tmp_path):
+    """The driver's conf merge universes (file choice, apply_pip,
+    force_beads, set_proj_incidence with Einc and azimuth widths) equal the
+    Python path through the same library sampler, trajectory by trajectory."""
+    import numpy as np
+    from md_tian2_amd import conf, geometry, sampling
+    fix = os.path.join(ROOT, "tests", "golden", "conf")
+    out = _check_input(tmp_path, f"conf merge '{fix}/proj' 1 '{fix}/latt' 1",
+                       "force_beads 4\nEinc 1.92 0.2\npolar 50\nazimuth 30 10\npip r r 6.0\nrng_seed 777")
+    for tid, c in zip((3, 4), out):
+        kp, kl = sampling.conf_index(777, tid, 0, 1), sampling.conf_index(777, tid, 1, 1)
+        proj = conf.read_mxt(conf.mxt_name(os.path.join(fix, "proj"), kp))
+        latt = conf.read_mxt(conf.mxt_name(os.path.join(fix, "latt"), kl))
+        proj.r, proj.is_cart = proj.cartesian(), True
+        sampling.apply_pip(proj.r, proj.simbox, sampling.Pip.parse(["r", "r", "6.0"]), 777, tid)
+        m = conf.merge(latt, proj)
+        r = np.ascontiguousarray(np.repeat(m.r, 4, axis=1))
+        v = np.ascontiguousarray(np.repeat(m.v, 4, axis=1))
+        mass = np.array([1.00794 if i == 1 else 12.0 for i in m.idx]) * geometry.AMU2MASS
+        inc = sampling.Incidence.make(1.92, 50.0, "30", sigma_einc=0.2, sigma_azimuth=10.0)
+        sampling.set_proj_incidence(v, m.idx - 1, m.is_proj, mass, inc, 777, tid)
+        assert np.abs(c.r - r).max() <= 1e-12
+        assert np.abs(c.v - v).max() <= 1e-15
