@@ -344,20 +344,27 @@ __global__ void __launch_bounds__(512, 4)
 #pragma unroll
     for (int q = 0; q < 9; ++q) hf[q] = (float)cell.h[q];
     const float cut2f = (float)(cut2 * (1.0 + 1e-4));
+    // only the owned candidates are visited, in increasing index: below k
+    // those of the other parity (a = (k + 1) / 2 of them), above k those of
+    // the same parity -- owns(k, n) of every n, without the half of the
+    // lanes that would test and drop the others
+    const int mmax = N / 2 + 1;
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < N;
         const int tk = act ? typ[k] : 0;
         const float fxk = act ? fx32[k] : 0.f, fyk = act ? fy32[k] : 0.f, fzk = act ? fz32[k] : 0.f;
+        const int a_lo = (k + 1) >> 1;
         int c = 0;
-        for (int n0 = 0; n0 < N; n0 += TPA) {
-            const int n = n0 + sub;
+        for (int m0 = 0; m0 < mmax; m0 += TPA) {
+            const int mi = m0 + sub;
+            const int n = mi < a_lo ? ((k + 1) & 1) + 2 * mi : k + 2 + 2 * (mi - a_lo);
             bool in = false;
             int code = 0;
-            if (act && n < N && n != k) {
+            if (act && n < N) {
                 const int tn = typ[n];
-                if (T.pair_on[tk * MDT_MAXT + tn] && owns(k, n)) {
+                if (T.pair_on[tk * MDT_MAXT + tn]) {
                     float d0 = fxk - fx32[n], d1 = fyk - fy32[n], d2 = fzk - fz32[n];
                     float m0 = rintf(d0), m1 = rintf(d1), m2 = rintf(d2);
                     const bool t0 = fabsf(fabsf(d0) - 0.5f) < 1e-4f, t1 = fabsf(fabsf(d1) - 0.5f) < 1e-4f,
@@ -1198,20 +1205,23 @@ __global__ void __launch_bounds__(512, 2)
 #pragma unroll
     for (int q = 0; q < 9; ++q) hf[q] = (float)cell.h[q];
     const float cut2f = (float)(cut2 * (1.0 + 1e-4));
+    const int mmax = Ns / 2 + 1;  // owned candidates only, in increasing index (as k_replica)
     for (int k0 = 0; k0 < Ns; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < Ns;
         const int tk = act ? typ[k + 1] : 0;
         const float fxk = act ? fx32[k] : 0.f, fyk = act ? fy32[k] : 0.f, fzk = act ? fz32[k] : 0.f;
+        const int a_lo = (k + 1) >> 1;
         int c = 0;
-        for (int n0 = 0; n0 < Ns; n0 += TPA) {
-            const int n = n0 + sub;
+        for (int m0 = 0; m0 < mmax; m0 += TPA) {
+            const int mi = m0 + sub;
+            const int n = mi < a_lo ? ((k + 1) & 1) + 2 * mi : k + 2 + 2 * (mi - a_lo);
             bool in = false;
             int code = 0;
-            if (act && n < Ns && n != k) {
+            if (act && n < Ns) {
                 const int tn = typ[n + 1];
-                if (T.pair_on[tk * MDT_MAXT + tn] && owns(k, n)) {
+                if (T.pair_on[tk * MDT_MAXT + tn]) {
                     float d0 = fxk - fx32[n], d1 = fyk - fy32[n], d2 = fzk - fz32[n];
                     float m0 = rintf(d0), m1 = rintf(d1), m2 = rintf(d2);
                     const bool t0 = fabsf(fabsf(d0) - 0.5f) < 1e-4f, t1 = fabsf(fabsf(d1) - 0.5f) < 1e-4f,
