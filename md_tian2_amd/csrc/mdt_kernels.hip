@@ -333,16 +333,25 @@ __global__ void __launch_bounds__(512, 4)
     // r^2 bound widened by 1e-4 relative; both passes re-check r <= cutr in
     // FP64.  A fractional difference within 1e-4 of +-1/2 takes its image
     // from the FP64 rounding, so the chosen image is the FP64 minimum image.
-    float *fx32 = (float *)G, *fy32 = fx32 + N, *fz32 = fy32 + N;
-    for (int i = lane; i < N; i += nthr) {
-        fx32[i] = (float)sx[i];
-        fy32[i] = (float)sy[i];
-        fz32[i] = (float)sz[i];
-    }
+    // fractional coordinates and type per atom as one float4 (16 B: the G
+    // and E words, 2N doubles); squared distances from the metric H^T H
+    float4 *f4 = (float4 *)G;
+    for (int i = lane; i < N; i += nthr)
+        f4[i] = make_float4((float)sx[i], (float)sy[i], (float)sz[i], __int_as_float((int)typ[i]));
+    int all_on = 1;
+    for (int q = 0; q < ecg->ntypes * ecg->ntypes; ++q)
+        if (!T.pair_on[(q / ecg->ntypes) * MDT_MAXT + q % ecg->ntypes]) all_on = 0;
     __syncthreads();
-    float hf[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) hf[q] = (float)cell.h[q];
+    float g00, g11, g22, g01, g02, g12;  // H^T H, cell vectors = columns of H
+    {
+        const double *h = cell.h;
+        g00 = (float)(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+        g11 = (float)(h[3] * h[3] + h[4] * h[4] + h[5] * h[5]);
+        g22 = (float)(h[6] * h[6] + h[7] * h[7] + h[8] * h[8]);
+        g01 = (float)(2 * (h[0] * h[3] + h[1] * h[4] + h[2] * h[5]));
+        g02 = (float)(2 * (h[0] * h[6] + h[1] * h[7] + h[2] * h[8]));
+        g12 = (float)(2 * (h[3] * h[6] + h[4] * h[7] + h[5] * h[8]));
+    }
     const float cut2f = (float)(cut2 * (1.0 + 1e-4));
     // only the owned candidates are visited, in increasing index: below k
     // those of the other parity (a = (k + 1) / 2 of them), above k those of
@@ -354,7 +363,7 @@ __global__ void __launch_bounds__(512, 4)
         const int sub = lane % TPA;
         const bool act = k < N;
         const int tk = act ? typ[k] : 0;
-        const float fxk = act ? fx32[k] : 0.f, fyk = act ? fy32[k] : 0.f, fzk = act ? fz32[k] : 0.f;
+        const float4 pk = act ? f4[k] : make_float4(0.f, 0.f, 0.f, 0.f);
         const int a_lo = (k + 1) >> 1;
         int c = 0;
         for (int m0 = 0; m0 < mmax; m0 += TPA) {
@@ -363,28 +372,27 @@ __global__ void __launch_bounds__(512, 4)
             bool in = false;
             int code = 0;
             if (act && n < N) {
-                const int tn = typ[n];
-                if (T.pair_on[tk * MDT_MAXT + tn]) {
-                    float d0 = fxk - fx32[n], d1 = fyk - fy32[n], d2 = fzk - fz32[n];
-                    float m0 = rintf(d0), m1 = rintf(d1), m2 = rintf(d2);
-                    const bool t0 = fabsf(fabsf(d0) - 0.5f) < 1e-4f, t1 = fabsf(fabsf(d1) - 0.5f) < 1e-4f,
-                               t2 = fabsf(fabsf(d2) - 0.5f) < 1e-4f;
-                    if (t0 | t1 | t2) {  // half-cell tie: the reference's image, from the unfolded positions
+                const float4 pn = f4[n];
+                if (all_on || T.pair_on[tk * MDT_MAXT + __float_as_int(pn.w)]) {
+                    const float e0 = pk.x - pn.x, e1 = pk.y - pn.y, e2 = pk.z - pn.z;
+                    float m0f = rintf(e0), m1f = rintf(e1), m2f = rintf(e2);
+                    float d0 = e0 - m0f, d1 = e1 - m1f, d2 = e2 - m2f;
+                    // within 1e-4 of a half-cell tie: the reference's image, from the unfolded positions
+                    const bool t0 = fabsf(d0) > 0.4999f, t1 = fabsf(d1) > 0.4999f, t2 = fabsf(d2) > 0.4999f;
+                    if (t0 | t1 | t2) {
                         const double *xr = bv.x + roff;
                         const double a0 = xr[k], a1 = xr[N + k], a2 = xr[2 * N + k];
                         const double b0 = xr[n], b1 = xr[N + n], b2 = xr[2 * N + n];
-                        if (t0) m0 = (float)tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, sx[k] - sx[n]);
-                        if (t1) m1 = (float)tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, sy[k] - sy[n]);
-                        if (t2) m2 = (float)tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, sz[k] - sz[n]);
+                        if (t0) m0f = (float)tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, sx[k] - sx[n]);
+                        if (t1) m1f = (float)tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, sy[k] - sy[n]);
+                        if (t2) m2f = (float)tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, sz[k] - sz[n]);
+                        d0 = e0 - m0f;
+                        d1 = e1 - m1f;
+                        d2 = e2 - m2f;
                     }
-                    d0 -= m0;
-                    d1 -= m1;
-                    d2 -= m2;
-                    const float ex = hf[0] * d0 + hf[3] * d1 + hf[6] * d2;
-                    const float ey = hf[1] * d0 + hf[4] * d1 + hf[7] * d2;
-                    const float ez = hf[2] * d0 + hf[5] * d1 + hf[8] * d2;
-                    in = (ex * ex + ey * ey + ez * ez) <= cut2f;
-                    code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
+                    const float r2 = d0 * (g00 * d0 + g01 * d1 + g02 * d2) + d1 * (g11 * d1 + g12 * d2) + g22 * d2 * d2;
+                    in = r2 <= cut2f;
+                    code = ((int)m0f + 1) + 3 * ((int)m1f + 1) + 9 * ((int)m2f + 1);
                 }
             }
             const unsigned long long m = __ballot(in) & gmask;
