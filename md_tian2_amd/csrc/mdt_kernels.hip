@@ -1633,8 +1633,8 @@ hipError_t mdt_launch_mixed(int tpa, int ntraj, size_t smem, const EmtConst *ec,
 
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
                            const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
-                           int flags, hipStream_t st) {
-    hipLaunchKernelGGL(k_traj, dim3(ntraj), dim3(256), smem, st, pc, bv, cjk, cjk_off, step_base, step_prev, step,
+                           int flags, int block, hipStream_t st) {
+    hipLaunchKernelGGL(k_traj, dim3(ntraj), dim3(block), smem, st, pc, bv, cjk, cjk_off, step_base, step_prev, step,
                        flags);
     return hipGetLastError();
 }
