@@ -166,3 +166,31 @@ def test_projectile_at_cutoff_edge(gpu_ctx, pes_hau, z):
     # resolution of the 2^42 fixed-point force sums (2.3e-13 eV/A per term)
     assert abs(f_r[0, 0, 2]) > 1e-12
     assert abs(f_g[0, 0, 2] - f_r[0, 0, 2]) <= 1e-12
+
+
+def test_largest_supported_trajectory(gpu_ctx, pes_hau):
+    """2026 atoms (H over a 15x15x9 Au slab, lists in global memory): the
+    largest size class against the oracle."""
+    s = geometry.h_on_slab("Au", n=15, nlayers=9, rng=np.random.default_rng(21))
+    assert s.natoms == 2026
+    s.r[:, 0, :] += np.random.default_rng(22).normal(0, 0.02, s.r[:, 0, :].shape)
+    s.r[0, 0, 2] = 1.2
+    _cmp(gpu_ctx, s, pes_hau)
+
+
+def test_most_beads(gpu_ctx, pes_hau):
+    """64 beads (the largest supported count) on every atom."""
+    s = geometry.h_on_slab("Au", n=3, nlayers=4, nbeads=64, rng=np.random.default_rng(23))
+    s.r += np.random.default_rng(24).normal(0, 0.03, s.r.shape)
+    s.r[0, :, 2] = np.linspace(0.9, 3.0, 64)
+    _cmp(gpu_ctx, s, pes_hau)
+
+
+def test_sizes_beyond_the_limits_are_refused(gpu_ctx, pes_hau):
+    """2049 atoms or 65 beads: a clear error before any launch."""
+    for kw in (dict(n=16, nlayers=8), dict(n=3, nlayers=4, nbeads=65)):
+        s = geometry.h_on_slab("Au", rng=np.random.default_rng(25), **kw)
+        cols, pesm = pesfile.emt_tables(pes_hau, s.type_names)
+        gpu_ctx.set_emt(cols, pesm)
+        with pytest.raises(mdt.MdtError, match="supported|natoms|beads"):
+            mdt.Batch(gpu_ctx, [s], dt=0.1).compute_forces()
