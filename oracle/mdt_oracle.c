@@ -18,7 +18,14 @@
  * Langevin against the exact Ornstein-Uhlenbeck update, Langevin(T=0,xi=0)
  * == Verlet, LDFA against numpy polyval, C^T C = I, the free ring polymer
  * against scipy expm, PILE coefficients, COM removal closed form, and the
- * Philox4x32-10 known-answer vectors.
+ * Philox4x32-10 known-answer vectors.  Trajectory characteristics, the
+ * energy-file estimators and the initial-condition routines
+ * (tests/test_observe_cpu.py, tests/test_sampling.py): a hand-built state
+ * sequence with known closest approach (through a periodic image), lowest
+ * point, turning points, interaction time and adsorption; a free
+ * projectile's exit step from straight flight; ring-polymer closed forms
+ * (radius of gyration, spring energy, virial kinetic energy); hand-computed
+ * apply_pip targets and incidence energies.
  */
 #include "mdt_oracle.h"
 
