@@ -193,3 +193,39 @@ def test_output_files(gpu_ctx, pes_hau, tmp_path):
     head = open(out.path("energy", 101)).read().splitlines()[:3]
     assert head[0] == "# All energies are given in eV." and head[2].split()[0] == "#time/fs"
     assert not open(out.path("energy", 102)).read().startswith("#")
+
+
+def test_exit_on_the_network_potential(gpu_ctx, pes_hau):
+    """With the NENE surrogate as the potential, a projectile flying up out
+    of the interaction range leaves through pul at the step straight flight
+    predicts and stays frozen; a trajectory at the surface keeps running."""
+    from md_tian2_amd import surrogate
+    model = surrogate.load_model(surrogate.MODEL_DIR)
+    assert model is not None
+    rng = np.random.default_rng(15)
+    systems = _systems(rng, 2)
+    fly = systems[0]
+    fly.r[0, 0, 2] = 12.0                 # beyond the network's 6.5 A cut-off
+    fly.v[0, 0] = [0.0, 0.0, 0.0123]
+    stay = systems[1]
+    stay.r[0, 0, 2] = 1.0                 # bound at the surface, at rest
+    stay.v[0, 0] = 0.0
+    cols, pesm = pesfile.emt_tables(pes_hau, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    gpu_ctx.set_nn(model, systems[0].type_names)
+    b = mdt.Batch(gpu_ctx, systems, dt=DT, tsurf=300.0, tproj=0.0, seed=SEED)
+    b.set_pes(1)
+    b.compute_forces()
+    b.set_observe(proj_ul=13.0)
+    b.step(900)
+    obs = b.observables()
+    want = int(np.floor(1.0 / (0.0123 * DT))) + 1
+    assert obs[0]["exit_step"] == want, obs[0]
+    assert obs[1]["exit_step"] == 0
+    before = b.get_state()
+    b.step(20)
+    after = b.get_state()
+    assert np.array_equal(before[0]["r"], after[0]["r"]) and np.array_equal(before[0]["f"], after[0]["f"])
+    assert not np.array_equal(before[1]["r"], after[1]["r"])
+    assert not (b.status() & ~16).any()
+    b.close()
