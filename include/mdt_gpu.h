@@ -246,7 +246,9 @@ int mdt_batch_set_com_removal(mdt_batch *b, int on);
  * proj_ul (`pul`) or all below proj_ll (`pll`) after a step is frozen: no
  * further propagation or forces; its state stays that of the exit step.
  * Defaults of the reference: proj_ul = DBL_MAX, proj_ll = -DBL_MAX,
- * ads_start = ads_end = DBL_MAX.
+ * ads_start = ads_end = DBL_MAX.  The call also unfreezes every trajectory;
+ * mdt_batch_set_state does not, so a batch reused for new trajectories
+ * calls this again after setting their state.
  */
 int mdt_batch_set_observe(mdt_batch *b, double proj_ul, double proj_ll, double ads_start, double ads_end,
                           int exit_at_limits);
