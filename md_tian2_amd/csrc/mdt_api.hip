@@ -125,6 +125,7 @@ struct mdt_batch {
     double *d_mass = nullptr, *d_cell = nullptr, *d_icell = nullptr, *d_tsurf = nullptr, *d_tproj = nullptr;
     uint32_t *d_tid = nullptr;
     double *d_x = nullptr, *d_v = nullptr, *d_a = nullptr, *d_f = nullptr, *d_dens = nullptr, *d_epot = nullptr;
+    double *d_eta = nullptr;  // langevin_1's deviates for langevin_2 (3PN)
     double *d_cjk = nullptr;
     int *d_status = nullptr;
     size_t lds_limit = 0;
@@ -203,6 +204,7 @@ struct mdt_batch {
         v.a = d_a;
         v.f = d_f;
         v.dens = d_dens;
+        v.eta = d_eta;
         v.epot = d_epot;
         v.traj_eoff = d_eoff;
         v.status = d_status;
@@ -399,7 +401,7 @@ int mdt_batch_destroy(mdt_batch *b) {
                     b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
                     b->d_cjk, b->d_status, b->d_pc, b->d_step, b->d_nnG, b->d_nnC,
-                    b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr, b->d_obs, b->d_done, b->d_est};
+                    b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr, b->d_obs, b->d_done, b->d_est, b->d_eta};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -549,6 +551,7 @@ int mdt_batch_create(mdt_context *ctx, int ntraj, const int32_t *natoms, const i
     rc |= dalloc(&b->d_a, b->tot3pn);
     rc |= dalloc(&b->d_f, b->tot3pn);
     rc |= dalloc(&b->d_dens, b->totpn);
+    rc |= dalloc(&b->d_eta, b->tot3pn);
     rc |= dalloc(&b->d_epot, b->totp);
     rc |= dalloc(&b->d_cjk, cjk.size());
     rc |= dalloc(&b->d_status, ntraj);

@@ -84,6 +84,7 @@ struct BatchView {
     const uint32_t *traj_id;
     double *x, *v, *a, *f;  // 3PN blocks
     double *dens;           // PN blocks
+    double *eta;            // 3PN: langevin_1's deviates of the current step, reused by langevin_2
     double *epot;           // per replica (P per traj, at traj_soff/(3N))
     const long long *traj_eoff;  // offset of a trajectory's epot(1..P)
     int *status;            // per traj

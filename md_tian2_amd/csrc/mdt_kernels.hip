@@ -753,8 +753,9 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
                 const LanCoef L = lan_coef(bv.dens[PI_(b, i)], dt, temp, exact);
                 const double w = L.sv * sqrt(1 - L.crv * L.crv);
                 for (int c = 0; c < 3; ++c) {
-                    const double eta =
-                        mdt_normal(pc.seed, tid, step_prev, MDT_STREAM_LANGEVIN, (uint32_t)(i * P + b), (uint32_t)c);
+                    // the deviate of this step's langevin_1, kept by propagate_1
+                    // (mdt_normal(seed, tid, step_prev, MDT_STREAM_LANGEVIN, i P + b, c))
+                    const double eta = bv.eta[XI(c, b, i)];
                     const double v = bv.v[XI(c, b, i)];
                     bv.v[XI(c, b, i)] = (fb >> c & 1) ? 0.0 : v + L.c2 * bv.a[XI(c, b, i)] + w * eta;
                 }
@@ -936,6 +937,7 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
                 for (int c = 0; c < 3; ++c) {
                     const double eta =
                         mdt_normal(pc.seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)(i * P + b), (uint32_t)c);
+                    bv.eta[XI(c, b, i)] = eta;  // langevin_2 of this step takes the same deviate
                     const double r = bv.x[XI(c, b, i)], v = bv.v[XI(c, b, i)], a = bv.a[XI(c, b, i)];
                     if (!(fb >> c & 1)) {
                         bv.x[XI(c, b, i)] = r + L.c1 * v + L.c2 * dt * a + L.sr * eta;
