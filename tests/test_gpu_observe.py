@@ -229,3 +229,55 @@ def test_exit_on_the_network_potential(gpu_ctx, pes_hau):
     assert not np.array_equal(before[1]["r"], after[1]["r"])
     assert not (b.status() & ~16).any()
     b.close()
+
+
+def test_observation_in_the_mixed_bead_model(gpu_ctx, pes_hau):
+    """Mixed bead model (beads on the projectile only): the device's
+    characteristics equal a host recomputation from the state after every
+    step (centroids of the projectile's beads, classical slab atoms)."""
+    rng = np.random.default_rng(16)
+    systems = _systems(rng, 3, nbeads=8)
+    cols, pesm = pesfile.emt_tables(pes_hau, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    b = mdt.Batch(gpu_ctx, systems, dt=DT, tsurf=300.0, tproj=300.0, seed=SEED)
+    b.set_bead_mode(1)
+    b.compute_forces()
+    b.set_observe(proj_ul=PUL, adsorption=ADS)
+    big = np.finfo(np.float64).max
+    ref = [dict(cl=big, cls=0, rmin=np.full(3, big), tint=0.0, tp=0, down=True, ads=False, exit=0)
+           for _ in systems]
+    for istep in range(1, 121):
+        b.step(1)
+        st = b.get_state()
+        for t, s in enumerate(systems):
+            o = ref[t]
+            if o["exit"]:
+                continue
+            c = st[t]["r"].mean(axis=1)                      # centroids (classical beads are copies)
+            ic = np.linalg.inv(s.simbox)
+            d = (c[0] - c[1:]) @ ic.T
+            d -= np.round(d)
+            dist = np.sqrt(((d @ s.simbox.T) ** 2).sum(-1)).min()
+            if dist < o["cl"]:
+                o["cl"], o["cls"] = dist, istep
+            if not o["ads"] and dist < ADS[0]:
+                o["ads"] = True
+            if o["ads"] and dist > ADS[1]:
+                o["ads"] = False
+            if c[0, 2] < o["rmin"][2]:
+                o["rmin"] = c[0].copy()
+            if np.any(st[t]["f"][0] != 0.0):
+                o["tint"] += DT
+            down = st[t]["v"][0, :, 2].mean() < 0
+            if down != o["down"]:
+                o["tp"] += 1
+                o["down"] = down
+            if (st[t]["r"][0, :, 2] > PUL).all():
+                o["exit"] = istep
+    obs = b.observables()
+    for t, o in enumerate(ref):
+        g = obs[t]
+        assert g["exit_step"] == o["exit"] and g["cl_appr_step"] == o["cls"] and g["turning_points"] == o["tp"]
+        assert abs(g["cl_appr"] - o["cl"]) <= 1e-12 and np.abs(g["r_min"] - o["rmin"]).max() <= 1e-12
+        assert abs(g["t_int"] - o["tint"]) <= 1e-12 and g["adsorbed"] == o["ads"]
+    b.close()
