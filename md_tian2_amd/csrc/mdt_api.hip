@@ -917,6 +917,7 @@ static int configure(mdt_batch *b, int tpa, int nbcap) {
 }
 
 static int launch_mixed(mdt_batch *b, const BatchView &base, int tpa, int add_fin, hipStream_t st) {
+    if (!b->ctx->d_ec) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
     BatchView v = base;
     v.rep_list = b->d_mix;
     v.nbr_global = b->d_mix_nbr;
@@ -929,6 +930,7 @@ static int launch_mixed(mdt_batch *b, const BatchView &base, int tpa, int add_fi
 // calc_force on every replica, one launch per size class
 static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hipStream_t st) {
     mdt_context *c = b->ctx;
+    if (!c->d_ec) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
     for (const SizeClass &k : b->cls) {
         BatchView v = base;
         v.rep_list = k.d_reps;
@@ -1038,6 +1040,10 @@ static int autotune(mdt_batch *b) {
     hipStream_t st = c->stream;
     int rc = configure(b, 0, b->nbcap_req);
     if (rc) return rc;
+    // the EMT kernels read the context's EMT constants: without them (a
+    // network-only potential) there is nothing to time, and the heuristic
+    // launch shape of configure() stands
+    if (!c->have_params || !c->d_ec) return 0;
     double *sf = nullptr, *sa = nullptr, *se = nullptr, *sd = nullptr;
     rc = dalloc(&sf, b->tot3pn) | dalloc(&sa, b->tot3pn) | dalloc(&se, b->totp) | dalloc(&sd, b->totpn);
     if (rc) return rc;

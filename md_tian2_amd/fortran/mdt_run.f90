@@ -61,6 +61,7 @@ program mdt_run
     character(len=512) :: conf_path = "", proj_path = ""
     integer :: conf_n = 0, proj_n = 0
     logical :: einc_set = .false., pip_set = .false., check_input = .false.
+    logical :: pes_has_emt = .false.
     ! output and trajectory limits (src/run_config.f90:508-556, 595-650)
     logical :: out_scatter = .false.
     integer :: out_energy = 0, out_ads = 0, out_runner = 0
@@ -96,6 +97,14 @@ program mdt_run
         end if
     end do
     call read_deck(trim(deck))
+    if (.not. check_input) then
+        call scan_pes(trim(pes_file))
+        if (.not. pes_has_emt .and. len_trim(nene_dir) == 0) then
+            print '(a)', "Error: the pes file has neither emt nor nene blocks"
+            stop 1
+        end if
+        if (len_trim(nene_dir) > 0) print '(a,a)', "mdt_run: network potential from ", trim(nene_dir)
+    end if
 
     if (len_trim(conf_kind) > 0) then
         call conf_universe(tmpl, start)
@@ -122,9 +131,11 @@ program mdt_run
     end if
     call emt_gpu_init(0)
     ctx = emt_gpu_context()
-    call emt_params_from_file(tmpl, trim(pes_file), eta2, n0, e0, lambda, v0p, kappa, s0)
-    call mdt_check(mdt_emt_params_set(ctx, 2_c_int, eta2, n0, e0, lambda, v0p, kappa, s0, &
-                                      int(tmpl%pes, c_int32_t)), "mdt_emt_params_set")
+    if (pes_has_emt) then
+        call emt_params_from_file(tmpl, trim(pes_file), eta2, n0, e0, lambda, v0p, kappa, s0)
+        call mdt_check(mdt_emt_params_set(ctx, 2_c_int, eta2, n0, e0, lambda, v0p, kappa, s0, &
+                                          int(tmpl%pes, c_int32_t)), "mdt_emt_params_set")
+    end if
 
     ntot = 3 * nbeads * natoms
     allocate(nat(ntrajs), nbd(ntrajs), tid(ntrajs), ts(ntrajs), tp(ntrajs))
@@ -432,6 +443,47 @@ contains
         close(unit)
     end subroutine read_deck
 
+    ! The blocks of the pes file (src/md_init.f90:99-144): `pes emt` blocks
+    ! give the EMT parameters; `pes nene` blocks name the network's RuNNer
+    ! directory (inp_dir, relative to the working directory as in
+    ! read_nene), used unless the deck's `nene` keyword names one; `pes non`
+    ! pairs do not interact; any other PES stops the run.
+    subroutine scan_pes(path)
+        character(len=*), intent(in) :: path
+        character(len=1000) :: line
+        character(len=512) :: w(8)
+        integer :: unit, ios, nw
+        character(len=16) :: kind
+        open(newunit=unit, file=path, status="old", action="read", iostat=ios)
+        if (ios /= 0) then
+            print '(a,a)', "Error: cannot open pes file ", path
+            stop 1
+        end if
+        kind = ""
+        do
+            read(unit, '(a)', iostat=ios) line
+            if (ios /= 0) exit
+            call words(line, w, nw)
+            if (nw == 0) then
+                kind = ""
+                cycle
+            end if
+            if (trim(w(1)) == "pes" .and. nw >= 2) then
+                kind = w(2)
+                select case (trim(kind))
+                case ("emt"); pes_has_emt = .true.
+                case ("nene", "non")
+                case default
+                    print '(a,a,a)', "Error: pes ", trim(kind), " is not supported on the GPU path (emt, nene, non)"
+                    stop 1
+                end select
+            else if (trim(kind) == "nene" .and. nw == 2 .and. trim(w(1)) == "inp_dir") then
+                if (len_trim(nene_dir) == 0) nene_dir = unquote(w(2))
+            end if
+        end do
+        close(unit)
+    end subroutine scan_pes
+
     ! NUL-terminated copy of a Fortran string for the C-ABI
     function cstr(s) result(c)
         character(len=*), intent(in) :: s
@@ -555,6 +607,7 @@ contains
         type(universe) :: p1
         integer(c_int32_t), allocatable :: fx(:)
         integer :: k_
+        real(dp) :: cen(3)
         if (len_trim(conf_kind) > 0) then
             call conf_universe(u, traj)
             return
@@ -564,7 +617,12 @@ contains
         p1%r(:, :, 1) = u%r(:, :, 1)
         call mdt_check(mdt_apply_pip(pip_struct(), int(rng_seed, c_int32_t), int(traj, c_int32_t), 1_c_int, &
                                      int(nbeads, c_int), u%simbox, p1%r), "mdt_apply_pip")
-        u%r(:, :, 1) = p1%r(:, :, 1)
+        ! the centroid folded into the cell by int() of its direct coordinates,
+        ! as merge_universes does after apply_pip (src/md_init.f90:612-621)
+        cen = matmul(u%isimbox, sum(p1%r(:, :, 1), dim=2) / nbeads)
+        do k_ = 1, nbeads
+            u%r(:, k_, 1) = p1%r(:, k_, 1) - matmul(u%simbox, real(int(cen), dp))
+        end do
         u%v = 0.0_dp
         call incidence(u, traj)
         allocate(fx(3 * u%nbeads * u%natoms))

@@ -1,4 +1,4 @@
-"""Reader for md_tian2 ``.pes`` files (EMT blocks).
+"""Reader for md_tian2 ``.pes`` files (EMT, NENE and NON blocks).
 
 Format (src/md_init.f90:99-144, src/pes_emt_mod.f90:86-188): a block starts
 with ``pes emt``, followed by ``A B proj|latt proj|latt`` and key/value lines
@@ -6,6 +6,13 @@ with ``pes emt``, followed by ``A B proj|latt proj|latt`` and key/value lines
 first seven values belong to element A, the next seven to element B
 (pes_emt_mod.f90:133-179).  A blank line ends the block.  Each block marks
 the pair (A, B) as interacting through EMT (pes_emt_mod.f90:111-112).
+
+``pes nene`` blocks (read_nene, src/pes_nene_mod.f90) mark the pair as
+interacting through the network; their key/value lines name the RuNNer
+directory (``inp_dir``) and the extrapolation-warning limit
+(``maxnum_extrapolation_warnings``).  ``pes non`` blocks mark a pair that
+does not interact (src/pes_non_mod.f90).  Any other PES (LJ, Morse, HO,
+REBO) is refused: the GPU path has no kernel for it.
 """
 from __future__ import annotations
 
@@ -21,6 +28,10 @@ class EmtPes:
 
     params: dict = field(default_factory=dict)  # (name, is_proj) -> {key: val}
     pairs: set = field(default_factory=set)  # {((nameA, projA), (nameB, projB))}
+    nene_pairs: set = field(default_factory=set)  # pairs of `pes nene` blocks
+    non_pairs: set = field(default_factory=set)   # pairs of `pes non` blocks
+    nene_dir: str = None                          # inp_dir of the nene blocks
+    nene_max_warnings: int = None                 # maxnum_extrapolation_warnings
 
     def species(self):
         return list(self.params.keys())
@@ -41,8 +52,9 @@ def parse_pes(text: str) -> EmtPes:
         i += 1
         if len(words) < 2 or words[0].lower() != "pes":
             continue
-        if words[1].lower() != "emt":
-            raise ValueError(f"unsupported pes block '{words[1]}' (only emt)")
+        kind = words[1].lower()
+        if kind not in ("emt", "nene", "non"):
+            raise ValueError(f"unsupported pes block '{words[1]}' (emt, nene and non are supported)")
         if i >= len(lines):
             raise ValueError("pes block without interaction line")
         hdr = lines[i].split()
@@ -54,8 +66,22 @@ def parse_pes(text: str) -> EmtPes:
         a = (hdr[0], hdr[2] == "proj")
         b = (hdr[1], hdr[3] == "proj")
         key = tuple(sorted([a, b]))
-        if key in out.pairs:
+        if key in out.pairs or key in out.nene_pairs or key in out.non_pairs:
             raise ValueError(f"pes already defined for atoms {hdr}")
+        if kind != "emt":
+            (out.nene_pairs if kind == "nene" else out.non_pairs).add(key)
+            while i < len(lines):
+                kv = lines[i].split()
+                i += 1
+                if len(kv) == 0:
+                    break
+                if kind == "nene" and len(kv) == 2 and kv[0].lower() == "inp_dir":
+                    out.nene_dir = kv[1].strip("'\"")
+                elif kind == "nene" and len(kv) == 2 and kv[0].lower() == "maxnum_extrapolation_warnings":
+                    out.nene_max_warnings = int(kv[1])
+                else:
+                    raise ValueError(f"unknown {kind} parameter line '{' '.join(kv)}'")
+            continue
         out.pairs.add(key)
         counter = 1
         while i < len(lines):

@@ -136,3 +136,20 @@ def test_to_system():
     assert s.natoms == 3 and s.type_names == [("H", True), ("Au", False)]
     assert np.allclose(s.mass[1] / geometry.AMU2MASS, geometry.MASS_AMU["Au"])
     assert list(s.algo) == [geometry.PROP_VERLET, geometry.PROP_LANGEVIN]
+
+
+def test_pes_blocks_of_the_reference_files():
+    """The reference's .pes files (tests/golden/pes): nene blocks give the
+    network's directory and warning limit, non blocks non-interacting
+    pairs; REBO is refused (no GPU kernel)."""
+    from md_tian2_amd import pesfile
+    d = os.path.join(os.path.dirname(__file__), "golden", "pes")
+    p = pesfile.load_pes(os.path.join(d, "NENE-HC.pes"))
+    assert p.nene_dir == "RuNNer_input/" and p.nene_max_warnings == 100
+    assert (("C", False), ("H", True)) in p.nene_pairs and not p.pairs and not p.params
+    q = pesfile.load_pes(os.path.join(d, "NON-H.pes"))
+    assert q.non_pairs == {(("H", True), ("H", True))} and not q.pairs
+    cols, pesm = pesfile.emt_tables(q, [("H", True)])
+    assert pesm == [[0]]
+    with pytest.raises(ValueError, match="rebo"):
+        pesfile.load_pes(os.path.join(d, "REBO-EMFT_EMT-HCPt.pes"))

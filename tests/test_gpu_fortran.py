@@ -237,3 +237,31 @@ def test_fortran_driver_scatter_output(tmp_path, pes_hau):
     from md_tian2_amd import nene
     frames = nene.read_input_data(str(tmp_path / "runner" / "input.data"))
     assert len(frames) == nframes and not list((tmp_path / "runner").glob("*.tmp"))
+
+
+def test_fortran_driver_network_from_the_pes_file(tmp_path):
+    """A pes file of `pes nene` blocks (inp_dir naming the trained H/Au
+    network, as the reference's NENE decks do) runs the network; with no
+    replica extrapolating, the dump equals the run with the `nene` keyword
+    and the EMT fallback loaded."""
+    pes = tmp_path / "NENE-HAu.pes"
+    pes.write_text("pes nene\nH\tAu\tproj\tlatt\n\npes nene\nAu\tAu\tlatt\tlatt\n\npes nene\nH\tH\tproj\tproj\n"
+                   f" inp_dir '{ROOT}/md_tian2_amd/nn/hau'\n maxnum_extrapolation_warnings 100\n")
+    base = open(os.path.join(ROOT, "tests", "decks", "h_au_3x3x4.inp")).read()
+    dumps = []
+    for name, edit in (("keyword", lambda d: d + "\nnene 'md_tian2_amd/nn/hau'\n"),
+                       ("pesfile", lambda d: d.replace("pes 'md_tian2_amd/pes/EMT-HAu.pes'", f"pes '{pes}'"))):
+        dump = tmp_path / f"{name}.bin"
+        deck = edit(base.replace("dump 'gpurun_out/mdt_run_h_au.bin'", f"dump '{dump}'"))
+        p = tmp_path / f"{name}.inp"
+        p.write_text(deck)
+        exe = os.path.join(ROOT, "md_tian2_amd", "fortran", "mdt_run")
+        res = subprocess.run([exe, str(p)], cwd=ROOT, capture_output=True, text=True, timeout=120)
+        print(res.stdout)
+        assert res.returncode == 0, res.stdout + res.stderr
+        assert "network potential from" in res.stdout
+        if name == "keyword":
+            assert "trajectories on the EMT fallback at the last step: 0" in res.stdout
+        dumps.append(read_dump(str(dump)))
+    for k in ("r1", "v1", "epot"):
+        assert np.array_equal(dumps[0][k], dumps[1][k]), k
