@@ -232,6 +232,16 @@ int mdt_batch_set_bead_mode(mdt_batch *b, int mode);
  */
 int mdt_batch_set_com_removal(mdt_batch *b, int on);
 
+/* Verlet skin (A) of the classical atoms' neighbour lists in mixed bead
+ * mode (default 0.3; 0 rebuilds them at every force call).  The lists are
+ * built with cut-off + skin and kept while no classical atom of the
+ * trajectory moved more than skin / 2 since the build; a cell narrower than
+ * 2 (cut-off + skin) rebuilds every call.  Every pair is re-checked against
+ * the cut-off, so the skin changes only the summation order.  Extension: the reference
+ * visits every pair under the minimum image at every force call
+ * (src/pes_emt_mod.f90:1204-1206) and keeps no lists. */
+int mdt_batch_set_list_skin(mdt_batch *b, double skin);
+
 /*
  * Trajectory observation: collect_trajectory_characteristics
  * (src/trajectory_info.f90:47-263) after each step's propagate_2, and the

@@ -167,6 +167,12 @@ struct mdt_batch {
     int *d_mix = nullptr;
     uint16_t *d_mix_nbr = nullptr;  // classical atoms' owned half lists, one block per listed trajectory
     long long mix_stride = 0;
+    // their Verlet skin: build positions and fold, counts, valid flags
+    double *d_mix_ref = nullptr, *d_mix_fold = nullptr;
+    uint16_t *d_mix_cnt = nullptr;
+    int *d_mix_valid = nullptr;
+    long long mix_ns = 0;
+    double mix_skin = 0.3;  // A
     int mix_tpa = 0, mix_nbcap = 0, mix_nmax = 0, mix_pmax = 0;
     size_t mix_smem = 0;
     float mix_tuned_ms = 0.f;
@@ -210,6 +216,12 @@ struct mdt_batch {
         v.status = d_status;
         v.nbr_global = nullptr;  // per size class (launch_replicas)
         v.nbr_stride = 0;
+        v.mix_ref = d_mix_ref;
+        v.mix_fold = d_mix_fold;
+        v.mix_cnt = d_mix_cnt;
+        v.mix_valid = d_mix_valid;
+        v.mix_skin = mix_skin;
+        v.mix_ns = mix_ns;
         v.obs = d_obs;
         v.done = d_obs ? d_done : nullptr;
         v.obs_ul = obs_ul;
@@ -401,7 +413,7 @@ int mdt_batch_destroy(mdt_batch *b) {
                     b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
                     b->d_cjk, b->d_status, b->d_pc, b->d_step, b->d_nnG, b->d_nnC,
-                    b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr, b->d_obs, b->d_done, b->d_est, b->d_eta};
+                    b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr, b->d_mix_ref, b->d_mix_fold, b->d_mix_cnt, b->d_mix_valid, b->d_obs, b->d_done, b->d_est, b->d_eta};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -718,6 +730,8 @@ int mdt_batch_set_state(mdt_batch *b, const double *r, const double *v) {
         HIPCHK(hipMemcpyAsync(b->d_v, tmp.data(), sizeof(double) * b->tot3pn, hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
     }
+    if (b->d_mix_valid)  // new positions: the kept classical lists are stale
+        HIPCHK(hipMemset(b->d_mix_valid, 0, sizeof(int) * b->mix_traj.size()));
     b->have_state = true;
     return 0;
 }
@@ -875,8 +889,15 @@ static void note_tpa(mdt_batch *b) {
 static int configure_mixed(mdt_batch *b, int tpa, int nbcap) {
     if (b->d_mix) (void)hipFree(b->d_mix);
     if (b->d_mix_nbr) (void)hipFree(b->d_mix_nbr);
+    if (b->d_mix_ref) (void)hipFree(b->d_mix_ref);
+    if (b->d_mix_fold) (void)hipFree(b->d_mix_fold);
+    if (b->d_mix_cnt) (void)hipFree(b->d_mix_cnt);
+    if (b->d_mix_valid) (void)hipFree(b->d_mix_valid);
     b->d_mix = nullptr;
     b->d_mix_nbr = nullptr;
+    b->d_mix_ref = b->d_mix_fold = nullptr;
+    b->d_mix_cnt = nullptr;
+    b->d_mix_valid = nullptr;
     b->mix_traj.clear();
     b->mix_nmax = b->mix_pmax = 0;
     if (b->bead_mode != 1) return 0;
@@ -896,8 +917,12 @@ static int configure_mixed(mdt_batch *b, int tpa, int nbcap) {
                     b->mix_nmax, b->mix_pmax, smem + 4096, b->lds_limit);
     b->mix_stride = (long long)ns * cap;
     const size_t nm = b->mix_traj.size();
-    int rc = dalloc(&b->d_mix, nm) | dalloc(&b->d_mix_nbr, (size_t)b->mix_stride * nm);
+    b->mix_ns = ns;
+    int rc = dalloc(&b->d_mix, nm) | dalloc(&b->d_mix_nbr, (size_t)b->mix_stride * nm) |
+             dalloc(&b->d_mix_ref, 3 * (size_t)ns * nm) | dalloc(&b->d_mix_fold, 3 * (size_t)ns * nm) |
+             dalloc(&b->d_mix_cnt, (size_t)ns * nm) | dalloc(&b->d_mix_valid, nm);
     if (rc) return rc;
+    HIPCHK(hipMemset(b->d_mix_valid, 0, sizeof(int) * nm));
     HIPCHK(hipMemcpy(b->d_mix, b->mix_traj.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
     b->mix_tpa = (tpa == 2 || tpa == 4 || tpa == 8) ? tpa : (tpa == 1 ? 2 : 4);
     b->mix_nbcap = cap;
@@ -1266,6 +1291,16 @@ int mdt_batch_set_com_removal(mdt_batch *b, int on) {
     const int off = on ? 0 : 1;
     if (off != b->no_com) drop_graph(b);
     b->no_com = off;
+    return 0;
+}
+
+int mdt_batch_set_list_skin(mdt_batch *b, double skin) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (!(skin >= 0.0 && skin < 5.0)) return fail(MDT_EINVAL, "list skin %g A (0 <= skin < 5)", skin);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    if (skin != b->mix_skin) drop_graph(b);
+    b->mix_skin = skin;
+    if (b->d_mix_valid) HIPCHK(hipMemset(b->d_mix_valid, 0, sizeof(int) * b->mix_traj.size()));
     return 0;
 }
 

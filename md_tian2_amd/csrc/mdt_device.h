@@ -106,6 +106,16 @@ struct BatchView {
     // obs[t], and a trajectory whose projectile beads are all above
     // obs_ul or all below obs_ll is frozen (done[t] set: k_traj and the
     // force kernels skip it from then on)
+    // mixed bead mode, per listed trajectory (slot = position in the list):
+    // the classical atoms' lists are kept across steps (Verlet skin) when
+    // mix_skin > 0 and the cell's narrowest width exceeds 2 (cut-off + skin):
+    // positions and fold at the list build (3 per classical atom each,
+    // mix_ns atoms per slot), list counts, and a valid flag
+    double *mix_ref, *mix_fold;
+    uint16_t *mix_cnt;
+    int *mix_valid;
+    double mix_skin;
+    long long mix_ns;
     struct TrajObs *obs;
     int *done;
     double obs_ul, obs_ll, ads_start, ads_end;

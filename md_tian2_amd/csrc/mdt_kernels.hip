@@ -1158,7 +1158,38 @@ __global__ void __launch_bounds__(512, 2)
     uint8_t *typ = (uint8_t *)(hlist + (size_t)ngrp * MIX_CAPQ);
     uint16_t *nbr = bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
     int bad_local = 0;
+    // Verlet skin for the classical lists: built with cut-off + skin, reused
+    // while no classical atom moved more than skin / 2 since the build.  The
+    // positions keep the fold of the build, so the lists' image codes stay
+    // valid; a cell narrower than 2 (cut-off + skin) rebuilds every launch.
+    const long long slot = blockIdx.x;
+    double wmin = 1e300;
+    for (int q = 0; q < 3; ++q) {
+        const double w = 1.0 / sqrt(cell.ih[q] * cell.ih[q] + cell.ih[q + 3] * cell.ih[q + 3] +
+                                    cell.ih[q + 6] * cell.ih[q + 6]);
+        wmin = w < wmin ? w : wmin;
+    }
+    const double skin = bv.mix_skin;
+    const bool keep = skin > 0.0 && bv.mix_valid && 2.0 * (ecg->cutr + skin) < wmin;
+    double *ref = keep ? bv.mix_ref + slot * 3 * bv.mix_ns : nullptr;
+    double *fold = keep ? bv.mix_fold + slot * 3 * bv.mix_ns : nullptr;
+    __shared__ int s_reuse;
+    if (lane == 0) s_reuse = keep && bv.mix_valid[slot];
     __syncthreads();
+    if (s_reuse) {  // every thread reads the flag before any clears it
+        double dmax = 0.0;
+        for (int a = lane; a < Ns; a += nthr) {
+            const int i = a + 1;
+            const double d0 = bv.x[soff + i] - ref[3 * a], d1 = bv.x[soff + N + i] - ref[3 * a + 1],
+                         d2 = bv.x[soff + 2 * N + i] - ref[3 * a + 2];
+            const double d2s = d0 * d0 + d1 * d1 + d2 * d2;
+            dmax = d2s > dmax ? d2s : dmax;
+        }
+        __syncthreads();
+        if (dmax > 0.25 * skin * skin) s_reuse = 0;  // benign race: every writer stores 0
+    }
+    __syncthreads();
+    const bool reuse = s_reuse != 0;
 
     // ---- stage: classical atoms (bead 0) and the projectile's beads, folded ----
     for (int i = lane; i < N; i += nthr) typ[i] = (uint8_t)bv.type[aoff + i];
@@ -1168,9 +1199,24 @@ __global__ void __launch_bounds__(512, 2)
         double f0 = cell.ih[0] * r[0] + cell.ih[3] * r[1] + cell.ih[6] * r[2];
         double f1 = cell.ih[1] * r[0] + cell.ih[4] * r[1] + cell.ih[7] * r[2];
         double f2 = cell.ih[2] * r[0] + cell.ih[5] * r[1] + cell.ih[8] * r[2];
-        f0 -= floor(f0);
-        f1 -= floor(f1);
-        f2 -= floor(f2);
+        if (reuse) {  // the fold of the list build
+            f0 -= fold[3 * a];
+            f1 -= fold[3 * a + 1];
+            f2 -= fold[3 * a + 2];
+        } else {
+            const double g0 = floor(f0), g1 = floor(f1), g2 = floor(f2);
+            f0 -= g0;
+            f1 -= g1;
+            f2 -= g2;
+            if (keep) {
+                fold[3 * a] = g0;
+                fold[3 * a + 1] = g1;
+                fold[3 * a + 2] = g2;
+                ref[3 * a] = r[0];
+                ref[3 * a + 1] = r[1];
+                ref[3 * a + 2] = r[2];
+            }
+        }
         sx[a] = f0;
         sy[a] = f1;
         sz[a] = f2;
@@ -1205,6 +1251,9 @@ __global__ void __launch_bounds__(512, 2)
     // both passes re-check r <= cutr in FP64, and half-cell ties take the
     // reference's image (tie_image)
     float *fx32 = (float *)G0, *fy32 = (float *)E0, *fz32 = (float *)D0;
+    uint16_t *kcnt = keep ? bv.mix_cnt + slot * bv.mix_ns : nullptr;
+    if (reuse)
+        for (int a = lane; a < Ns; a += nthr) cnt[a] = kcnt[a];
     for (int a = lane; a < Ns; a += nthr) {
         fx32[a] = (float)sx[a];
         fy32[a] = (float)sy[a];
@@ -1214,9 +1263,10 @@ __global__ void __launch_bounds__(512, 2)
     float hf[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) hf[q] = (float)cell.h[q];
-    const float cut2f = (float)(cut2 * (1.0 + 1e-4));
+    const double cutl = keep ? (ecg->cutr + skin) * (ecg->cutr + skin) : cut2;
+    const float cut2f = (float)(cutl * (1.0 + 1e-4));
     const int mmax = Ns / 2 + 1;  // owned candidates only, in increasing index (as k_replica)
-    for (int k0 = 0; k0 < Ns; k0 += apb) {
+    for (int k0 = 0; k0 < (reuse ? 0 : Ns); k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < Ns;
@@ -1264,9 +1314,13 @@ __global__ void __launch_bounds__(512, 2)
             }
             c += __popcll(m);
         }
-        if (act && sub == 0) cnt[k] = (uint16_t)(c < cap ? c : cap);
+        if (act && sub == 0) {
+            cnt[k] = (uint16_t)(c < cap ? c : cap);
+            if (keep) kcnt[k] = cnt[k];
+        }
     }
     __syncthreads();
+    if (keep && !reuse && lane == 0) bv.mix_valid[slot] = 1;  // read by the next launch
     for (int a = lane; a < Ns; a += nthr) shi[a] = smid[a] = slo[a] = 0ull;
     __syncthreads();
 

@@ -29,6 +29,7 @@ SYMBOLS = (
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
     "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
+    "mdt_batch_set_list_skin",
     "mdt_batch_set_observe", "mdt_batch_get_observables", "mdt_batch_estimators",
     "mdt_sample_uniform", "mdt_sample_conf_index", "mdt_apply_pip", "mdt_set_proj_incidence", "mdt_sample_maxwell",
     "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats", "mdt_batch_nn_fallback",
@@ -87,6 +88,7 @@ def lib():
         L.mdt_batch_set_precision.argtypes = [vp, C.c_int]
         L.mdt_batch_set_bead_mode.argtypes = [vp, C.c_int]
         L.mdt_batch_set_com_removal.argtypes = [vp, C.c_int]
+        L.mdt_batch_set_list_skin.argtypes = [vp, C.c_double]
         L.mdt_batch_set_observe.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
         L.mdt_batch_get_observables.argtypes = [vp, C.c_void_p]
         L.mdt_batch_estimators.argtypes = [vp, dp]
@@ -266,6 +268,19 @@ class Batch:
         v = np.ascontiguousarray(np.concatenate([s.v.reshape(-1) for s in systems]))
         _check(lib().mdt_batch_set_state(self.h, _dp(r), _dp(v)))
 
+    def set_state(self, rs, vs=None):
+        """New positions (and velocities) per trajectory, (natoms, nbeads, 3)
+        arrays in the batch's order; forces are stale until compute_forces."""
+        r = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in rs]))
+        if r.size != self.n3:
+            raise ValueError("set_state: %d coordinates for a batch of %d" % (r.size, self.n3))
+        v = None
+        if vs is not None:
+            v = np.ascontiguousarray(np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in vs]))
+            if v.size != self.n3:
+                raise ValueError("set_state: %d velocities for a batch of %d" % (v.size, self.n3))
+        _check(lib().mdt_batch_set_state(self.h, _dp(r), None if v is None else _dp(v)))
+
     def close(self):
         if self.h:
             lib().mdt_batch_destroy(self.h)
@@ -298,6 +313,11 @@ class Batch:
         """remove_com_velocity after propagate_1 with Andersen / PILE atoms:
         on (the reference) or off (thermostatted bare slabs)."""
         _check(lib().mdt_batch_set_com_removal(self.h, 1 if on else 0))
+
+    def set_list_skin(self, skin):
+        """Verlet skin (A) of the classical atoms' lists in mixed bead mode;
+        0 rebuilds them at every force call."""
+        _check(lib().mdt_batch_set_list_skin(self.h, float(skin)))
 
     def set_observe(self, proj_ul=None, proj_ll=None, adsorption=None, exit_at_limits=True):
         """Trajectory observation and the exit at the projectile limits

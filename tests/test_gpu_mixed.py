@@ -180,3 +180,56 @@ def test_mixed_layout_checks(gpu_ctx, pes_hau):
         b.set_bead_mode(1)
     b.compute_forces()  # the batch stays usable in the all-beads model
     b.close()
+
+
+def test_mixed_list_skin_reuse_equals_rebuild(gpu_ctx, pes_hpt):
+    """Classical lists kept across steps (Verlet skin) give the rebuilt
+    lists' dynamics: an 8x8 Pt(111) cell (19.2 A wide, above 2 (cut-off +
+    skin)) under Langevin at 300 K for 200 steps of 0.5 fs, the slab atoms
+    moving beyond half the default skin; skin 0 (rebuild every call), 0.3 A (default) and
+    0.02 A (rebuild every few steps) agree to rounding, the forces of the
+    final state equal the oracle's per-bead EMT, and new positions from
+    set_state drop the kept lists."""
+    rng = np.random.default_rng(36)
+    systems = [_spread(geometry.h_on_slab("Pt", n=8, nlayers=4, nbeads=8, rng=rng, z0=z, einc=0.5, tsurf=300.0,
+                                          proj_algo=geometry.PROP_PILE), rng, 0.1, 0.05) for z in (1.8, 3.0)]
+    cols, pesm = pesfile.emt_tables(pes_hpt, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    res = {}
+    for skin in (0.0, 0.3, 0.02):
+        b = mdt.Batch(gpu_ctx, systems, dt=0.5, tsurf=300.0, tproj=300.0, pile_tau=100.0, seed=9)
+        b.set_bead_mode(1)
+        b.set_list_skin(skin)
+        b.compute_forces()
+        b.step(200)
+        assert not b.status().any()
+        res[skin] = b.get_state()
+        if skin == 0.3:
+            for s, o in zip(systems, res[skin]):
+                sr = s.copy()
+                sr.r = o["r"].copy()
+                e, f, d = _expected(sr, cols, pesm)
+                assert np.abs(o["epot"] - e).max() <= 1e-9 * np.abs(e).max()
+                assert np.abs(o["f"] - f).max() <= 1e-9
+            b.set_state([s.r for s in systems], [s.v for s in systems])
+            b.compute_forces()
+            for s, o in zip(systems, b.get_state()):
+                sr = s.copy()
+                sr.r = o["r"].copy()  # the free ring-polymer step moved the beads
+                assert np.abs(o["r"][1:] - s.r[1:]).max() == 0.0
+                e, f, d = _expected(sr, cols, pesm)
+                assert np.abs(o["f"] - f).max() <= 1e-9
+                assert np.abs(o["epot"] - e).max() <= 1e-9 * np.abs(e).max()
+        b.close()
+    moved = max(np.abs(o["r"][1:, 0] - s.r[1:, 0]).max() for s, o in zip(systems, res[0.0]))
+    print("slab atoms moved up to %.3f A" % moved)
+    assert moved > 0.15  # the default skin's lists were rebuilt on the way
+    for skin in (0.3, 0.02):
+        for ref, o in zip(res[0.0], res[skin]):
+            assert np.abs(o["r"] - ref["r"]).max() <= 1e-9, skin
+            assert np.abs(o["v"] - ref["v"]).max() <= 1e-9 * np.abs(ref["v"]).max(), skin
+            assert np.abs(o["epot"] - ref["epot"]).max() <= 1e-9 * np.abs(ref["epot"]).max(), skin
+    b = mdt.Batch(gpu_ctx, systems, dt=0.5, tsurf=300.0, tproj=300.0)
+    with pytest.raises(mdt.MdtError, match="skin"):
+        b.set_list_skin(-1.0)
+    b.close()
