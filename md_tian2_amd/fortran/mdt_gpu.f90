@@ -207,6 +207,20 @@ module mdt_gpu
             integer(c_int) :: rc
         end function
 
+        function mdt_batch_set_com_removal(batch, on) bind(C, name="mdt_batch_set_com_removal") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: on
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_set_list_skin(batch, skin) bind(C, name="mdt_batch_set_list_skin") result(rc)
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: batch
+            real(c_double), value :: skin
+            integer(c_int) :: rc
+        end function
+
         function mdt_batch_set_observe(batch, proj_ul, proj_ll, ads_start, ads_end, exit_at_limits) &
                 bind(C, name="mdt_batch_set_observe") result(rc)
             import :: c_int, c_ptr, c_double

@@ -50,6 +50,7 @@ program mdt_run
     real(dp) :: dt = 0.1_dp, einc = 1.0_dp, polar = 0.0_dp, tsurf = 0.0_dp, tproj = 0.0_dp
     real(dp) :: pile_tau = 200.0_dp, andersen_time = 30.0_dp, alat = 4.078_dp
     real(dp) :: sigma_einc = 0.0_dp, sigma_azimuth = 0.0_dp
+    real(dp) :: list_skin = 0.3_dp   ! Verlet skin of the mixed model's classical lists (A)
     real(dp) :: mproj = 1.00794_dp, mlatt = 196.966569_dp
     character(len=8) :: algo_proj = "ver", algo_latt = "lan", azimuth = "0"
     character(len=32) :: pip_w(3) = ["r  ", "r  ", "6.0"]
@@ -173,6 +174,7 @@ program mdt_run
     call mdt_check(mdt_batch_set_precision(batch, int(prec, c_int)), "mdt_batch_set_precision")
     call mdt_check(mdt_batch_set_graph(batch, int(graph_chunk, c_int)), "mdt_batch_set_graph")
     call mdt_check(mdt_batch_set_bead_mode(batch, int(bead_mode, c_int)), "mdt_batch_set_bead_mode")
+    call mdt_check(mdt_batch_set_list_skin(batch, real(list_skin, c_double)), "mdt_batch_set_list_skin")
     if (len_trim(nene_dir) > 0) then
         call mdt_check(mdt_nn_load(ctx, cstr(nene_dir), 2_c_int, cstr(name2(tmpl%name(1)) // name2(tmpl%name(2)))), &
                        "mdt_nn_load")
@@ -375,6 +377,7 @@ contains
             case ("nene"); nene_dir = unquote(w(2))
             case ("judge"); read(w(2), *) judge_interval
             case ("graph"); read(w(2), *) graph_chunk
+            case ("list_skin"); read(w(2), *) list_skin
             case ("beads")
                 if (trim(w(2)) == "mixed") then
                     bead_mode = 1

@@ -102,7 +102,7 @@ def test_fortran_driver_network_with_judge(tmp_path):
 
 
 def test_fortran_driver_mixed_beads_matches_library(tmp_path):
-    """`beads mixed` with force_beads 8 and a PILE projectile: the driver's
+    """`beads mixed` (list_skin 0) with force_beads 8 and a PILE projectile: the driver's
     final state equals the Python binding's mixed-model run from the driver's
     dumped initial state with the same seed and trajectory ids."""
     from md_tian2_amd import mdt
@@ -112,7 +112,7 @@ def test_fortran_driver_mixed_beads_matches_library(tmp_path):
     deck = deck.replace("dump 'gpurun_out/mdt_run_h_au.bin'", f"dump '{dump}'")
     deck = deck.replace("projectile 1 H 1.00794 ver", "projectile 1 H 1.00794 pil")
     deck = deck.replace("Tproj 0", "Tproj 100\npile_tau 200")
-    deck += "\nforce_beads 8\nbeads mixed\n"
+    deck += "\nforce_beads 8\nbeads mixed\nlist_skin 0.0\n"
     p = tmp_path / "mixed.inp"
     p.write_text(deck)
     exe = os.path.join(ROOT, "md_tian2_amd", "fortran", "mdt_run")
@@ -136,6 +136,7 @@ def test_fortran_driver_mixed_beads_matches_library(tmp_path):
     b = mdt.Batch(ctx, systems, dt=0.1, tsurf=300.0, tproj=100.0, pile_tau=200.0, seed=777,
                   traj_ids=[int(x) for x in d["tid"]])
     b.set_bead_mode(1)
+    b.set_list_skin(0.0)
     b.compute_forces()
     b.step(int(d["nsteps"]))
     out = b.get_state()
