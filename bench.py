@@ -220,6 +220,79 @@ def rpmd_variant(ctx, args):
     return out
 
 
+def uneven_variant(ctx, pes, args):
+    """The mixed uneven batch: trajectories drawn uniformly from 3x3, 6x6,
+    9x9 and 12x12 Au(111) slabs x 6 layers (bottom 2 fixed, 55 to 865 atoms)
+    and H bead counts {1, 4, 16}, Einc U(0.2, 6) eV, polar U(0, 60), azimuth
+    U(0, 360), Tsurf U(100, 500) K per trajectory (slab Langevin+LDFA, H
+    Verlet, ring polymer at Tsurf), dt 0.1 fs; one launch per size class with
+    autotuned threads per atom.  Three models of the same batch: FP64 EMT with
+    the slab classical (mixed bead model), FP64 EMT with every atom carrying
+    the beads (the reference's force_beads), and the NENE surrogate (every
+    atom carrying the beads) with the EMT fallback and the EMT judge on a
+    rotating 1 % of the replicas every 10 steps."""
+    from md_tian2_amd import geometry, mdt, pesfile, surrogate
+
+    g = np.random.default_rng(5150)
+    systems, tsurf = [], []
+    for _ in range(args.uneven_traj):
+        n = int(g.choice([3, 6, 9, 12]))
+        p = int(g.choice([1, 4, 16]))
+        ts = float(g.uniform(100.0, 500.0))
+        systems.append(geometry.h_on_slab("Au", n=n, nlayers=6, nfixed=2, rng=g, nbeads=p,
+                                          einc=g.uniform(0.2, 6.0), polar_deg=g.uniform(0.0, 60.0),
+                                          azimuth_deg=g.uniform(0.0, 360.0), z0=6.0, tsurf=ts))
+        tsurf.append(ts)
+    tsurf = np.array(tsurf)
+    cols, pesm = pesfile.emt_tables(pes, systems[0].type_names)
+    ctx.set_emt(cols, pesm)
+    nrep = int(sum(s.nbeads for s in systems))
+    out = {"trajectories": len(systems), "replicas_all_beads": nrep,
+           "atoms_per_traj": sorted({s.natoms for s in systems}), "beads": [1, 4, 16],
+           "workload": "mixed uneven batch: Au(111) 3x3..12x12 x 6 layers + H, beads {1, 4, 16}, Einc U(0.2,6) eV, "
+                       "Tsurf U(100,500) K, slab Langevin+LDFA, H Verlet, dt 0.1 fs"}
+
+    def timed(b, steps, key, extra=None):
+        b.tune(tpa=0)
+        b.compute_forces()
+        b.step(1)
+        b.step(steps)
+        ms, _ = b.timing()
+        res = {"value": len(systems) * steps / (ms / 1e3), "unit": "trajectory-steps/s",
+               "ms_per_step": ms / steps, "steps": steps}
+        if extra:
+            res.update(extra(b))
+        st = b.status()
+        ignore = 16 if key == "nn_surrogate" else 0  # replicas on the EMT fallback
+        assert not (st & ~ignore).any(), "trajectory failure flags set"
+        b.close()
+        print(f"[bench] uneven {key}: {ms / steps:.3f} ms/step", file=sys.stderr, flush=True)
+        out[key] = res
+
+    b = mdt.Batch(ctx, systems, dt=0.1, tsurf=tsurf, tproj=tsurf, seed=31)
+    b.set_bead_mode(1)
+    timed(b, args.uneven_steps, "emt_fp64_mixed_beads")
+    out["emt_fp64_mixed_beads"]["model"] = "H beads, slab classical (1 bead)"
+    b = mdt.Batch(ctx, systems, dt=0.1, tsurf=tsurf, tproj=tsurf, seed=31)
+    timed(b, args.uneven_all_steps, "emt_fp64_all_beads")
+    model = surrogate.load_model(surrogate.MODEL_DIR_THICK)
+    if model is None:
+        out["nn_surrogate"] = {"value": None, "reason": "no trained model in md_tian2_amd/nn/hau_thick"}
+        return out
+    ctx.set_nn(model, systems[0].type_names)
+    b = mdt.Batch(ctx, systems, dt=0.1, tsurf=tsurf, tproj=tsurf, seed=31)
+    b.set_pes(1, judge_interval=10)
+
+    def nn_stats(b):
+        rmse, ncomp, mx, nfb = b.nn_stats()
+        return {"judge_force_rmse_eV_per_A": rmse, "judge_components": ncomp, "judge_max_abs_eV_per_A": mx,
+                "fallback_replicas_last_step": nfb, "model": "md_tian2_amd/nn/hau_thick, every atom carries the beads"}
+
+    timed(b, args.uneven_nn_steps, "nn_surrogate", nn_stats)
+    out["nn_surrogate"]["vs_emt_fp64_all_beads"] = out["nn_surrogate"]["value"] / out["emt_fp64_all_beads"]["value"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--copy-calibration", action="store_true",
@@ -237,6 +310,11 @@ def main():
     ap.add_argument("--rpmd-steps", type=int, default=20)
     ap.add_argument("--rpmd-mixed-steps", type=int, default=200)
     ap.add_argument("--no-rpmd", action="store_true")
+    ap.add_argument("--uneven-traj", type=int, default=4096)
+    ap.add_argument("--uneven-steps", type=int, default=50)
+    ap.add_argument("--uneven-all-steps", type=int, default=3)
+    ap.add_argument("--uneven-nn-steps", type=int, default=10)
+    ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -301,6 +379,7 @@ def main():
     f32_launch_s = f32_prof["force_ms"] / 1e3 / max(1, f32_prof["nlaunch"])
     nn = None if args.no_nn else nn_variant(ctx, pes, args)
     rpmd = None if args.no_rpmd else rpmd_variant(ctx, args)
+    uneven = None if args.no_uneven else uneven_variant(ctx, pes, args)
 
     out = {
         "metric": METRIC,
@@ -343,6 +422,7 @@ def main():
             },
             "nn_surrogate": nn,
             "rpmd_h_pt": rpmd,
+            "uneven_h_au": uneven,
         },
     }
     if nn and nn.get("value"):

@@ -794,6 +794,8 @@ static int check_mixed(mdt_batch *b) {
 static int ensure_ready(mdt_batch *b, bool need_prop) {
     if (b->pes == 1 && !b->ctx->have_nn) return fail(MDT_ESTATE, "no network (call mdt_nn_params_set)");
     if (b->pes == 0 && !b->ctx->have_params) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
+    if (b->pes == 1 && b->bead_mode == MDT_BEADS_MIXED)
+        return fail(MDT_EUNSUP, "mixed bead mode runs with the EMT potential only");
     if (!b->have_atoms) return fail(MDT_ESTATE, "no atoms (call mdt_batch_set_atoms)");
     if (!b->have_cells) return fail(MDT_ESTATE, "no cells (call mdt_batch_set_cells)");
     if (!b->have_state) return fail(MDT_ESTATE, "no positions (call mdt_batch_set_state)");
@@ -995,8 +997,14 @@ static int ensure_nn(mdt_batch *b) {
 static int launch_pes(mdt_batch *b, const BatchView &bv, int add_fin, hipStream_t st) {
     mdt_context *c = b->ctx;
     if (b->pes == 1) {
-        HIPCHK(mdt_launch_nene(b->nrep, mdt_nene_smem(b->maxN), c->d_nn, bv, b->d_nnG, b->d_nnC, b->d_nnflag,
-                               b->d_nnF, b->maxN, add_fin, 1, st));
+        // one launch per size class: each sizes its LDS for its largest
+        // replica, so small slabs keep the occupancy of a uniform batch
+        for (const SizeClass &k : b->cls) {
+            BatchView v = bv;
+            v.rep_list = k.d_reps;
+            HIPCHK(mdt_launch_nene(k.nrep, mdt_nene_smem(k.nmax), c->d_nn, v, b->d_nnG, b->d_nnC, b->d_nnflag,
+                                   b->d_nnF, b->maxN, add_fin, 1, st));
+        }
         if (c->have_params) {
             BatchView fb = bv;
             fb.only = b->d_nnflag;
@@ -1611,6 +1619,8 @@ int mdt_batch_set_pes(mdt_batch *b, int pes, int judge_interval) {
     if (pes != 0 && pes != 1) return fail(MDT_EINVAL, "pes %d (0: EMT, 1: network)", pes);
     if (judge_interval < 0) return fail(MDT_EINVAL, "judge_interval %d", judge_interval);
     if (pes == 1 && !b->ctx->have_nn) return fail(MDT_ESTATE, "no network (call mdt_nn_params_set)");
+    if (pes == 1 && b->bead_mode == MDT_BEADS_MIXED)
+        return fail(MDT_EUNSUP, "mixed bead mode runs with the EMT potential only");
     HIPCHK(hipSetDevice(b->ctx->device));
     if (pes != b->pes) drop_graph(b);
     b->pes = pes;

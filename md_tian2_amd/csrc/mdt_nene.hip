@@ -911,7 +911,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__
     __shared__ int rowsb[NN_BLOCK / 64][16];
     __shared__ int s_ne[NN_MAXE];
     const NnConst &M = *Mg;
-    const int rep = blockIdx.x;
+    const int rep = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;  // size-class launches
     const int t = bv.rep_traj[rep];
     if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
     const int b = bv.rep_bead[rep];
@@ -1041,7 +1041,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_f(const NnConst *__restrict__
     __shared__ float cb[NN_CA][NN_K1];
     __shared__ float Wsh[NN_CA][NN_FB * NN_FB];
     const NnConst &M = *Mg;
-    const int rep = blockIdx.x;
+    const int rep = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;  // size-class launches
     if (skip_extrap && nv.rep_flag[rep]) return;  // whole block: uniform
     const int t = bv.rep_traj[rep];
     if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)

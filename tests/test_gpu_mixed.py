@@ -180,6 +180,19 @@ def test_mixed_layout_checks(gpu_ctx, pes_hau):
         b.set_bead_mode(1)
     b.compute_forces()  # the batch stays usable in the all-beads model
     b.close()
+    # the mixed model has no network path: refused in either order
+    from md_tian2_amd import surrogate
+    gpu_ctx.set_nn(surrogate.load_model(surrogate.MODEL_DIR), s.type_names)
+    b = mdt.Batch(gpu_ctx, [s], dt=0.1, tsurf=300.0, tproj=300.0)
+    b.set_bead_mode(1)
+    with pytest.raises(mdt.MdtError, match="EMT potential only"):
+        b.set_pes(1)
+    b.set_bead_mode(0)
+    b.set_pes(1)
+    with pytest.raises(mdt.MdtError, match="EMT potential only"):
+        b.set_bead_mode(1)
+    b.compute_forces()
+    b.close()
 
 
 def test_mixed_list_skin_reuse_equals_rebuild(gpu_ctx, pes_hpt):
