@@ -144,12 +144,20 @@ __device__ __forceinline__ void nbr_rir(const ChunkNbr &L, int e, float *r, floa
 
 }  // namespace
 
+// A fractional coordinate in [0, 1] as 2^-32 fixed point (1 wraps to 0,
+// the same point of the cell): 8e-9 A on a 35 A cell, below the FP32
+// rounding of the neighbour vectors built from it, in half the LDS of FP64.
+__device__ __forceinline__ uint32_t frac_fix(double f) {
+    return (uint32_t)(unsigned long long)(f * 4294967296.0);
+}
+
 // Builds the neighbour lists of centre atoms k0 .. k0+NN_CA-1 (16 lanes per
 // centre; lane `sub` tests atoms n = sub, sub+16, ...; hits are counted,
 // prefix-summed over the 16 lanes and written in (lane, n, image) order).
-__device__ void build_chunk(float rc2, float pi_rc, ChunkNbr &L, const double *fx, const double *fy, const double *fz,
-                            const uint8_t *elem, const double h[9], const int *imax, float wx, float wy, float wz,
-                            int N, int k0, int lane, int *bad) {
+// ux, uy, uz: the atoms' fractional coordinates (frac_fix).
+__device__ void build_chunk(float rc2, float pi_rc, ChunkNbr &L, const uint32_t *ux, const uint32_t *uy,
+                            const uint32_t *uz, const uint8_t *elem, const double h[9], const int *imax, float wx,
+                            float wy, float wz, int N, int k0, int lane, int *bad) {
     const int c = lane / NN_TPA, sub = lane % NN_TPA;
     const int i = k0 + c;
     int cnt = 0;
@@ -170,9 +178,11 @@ __device__ void build_chunk(float rc2, float pi_rc, ChunkNbr &L, const double *f
         }
         int pos = base;
         if (i < N) {
-            const double fxi = fx[i], fyi = fy[i], fzi = fz[i];
+            const long long xi = ux[i], yi = uy[i], zi = uz[i];
             for (int n = sub; n < N; n += NN_TPA) {
-                const double f0 = fx[n] - fxi, f1 = fy[n] - fyi, f2 = fz[n] - fzi;  // in (-1, 1)
+                const double f0 = (double)((long long)ux[n] - xi) * 0x1p-32;  // in (-1, 1)
+                const double f1 = (double)((long long)uy[n] - yi) * 0x1p-32;
+                const double f2 = (double)((long long)uz[n] - zi) * 0x1p-32;
                 // integer shifts s with |f + s| <= w on each axis (w: fractional half-width of the sphere)
                 const int a0 = (int)ceil(-wx - f0), b0 = (int)floor(wx - f0);
                 const int a1 = (int)ceil(-wy - f1), b1 = (int)floor(wy - f1);
@@ -925,9 +935,10 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__
     float *Crep = nv.C + (size_t)rep * nv.maxN * NN_K1;
     int bad = 0, extrap = 0;
 
-    // dynamic LDS: fx fy fz Ea (double N each) | elist (u16 N) | elem (u8 N)
-    double *fx = (double *)smem_raw, *fy = fx + N, *fz = fy + N, *Ea = fz + N;
-    uint16_t *elist = (uint16_t *)(Ea + N);
+    // dynamic LDS: Ea (double N) | ux uy uz (u32 N each) | elist (u16 N) | elem (u8 N)
+    double *Ea = (double *)smem_raw;
+    uint32_t *ux = (uint32_t *)(Ea + N), *uy = ux + N, *uz = uy + N;
+    uint16_t *elist = (uint16_t *)(uz + N);
     uint8_t *elem = (uint8_t *)(elist + N);
 
     // ---- tables ----
@@ -948,9 +959,9 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__
         double f0 = ih[0] * r0 + ih[3] * r1 + ih[6] * r2;
         double f1 = ih[1] * r0 + ih[4] * r1 + ih[7] * r2;
         double f2 = ih[2] * r0 + ih[5] * r1 + ih[8] * r2;
-        fx[i] = f0 - floor(f0);
-        fy[i] = f1 - floor(f1);
-        fz[i] = f2 - floor(f2);
+        ux[i] = frac_fix(f0 - floor(f0));
+        uy[i] = frac_fix(f1 - floor(f1));
+        uz[i] = frac_fix(f2 - floor(f2));
         Ea[i] = 0.0;
     }
     __syncthreads();
@@ -973,7 +984,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__
 
     // ---- pass G: symmetry functions ----
     for (int k0 = 0; k0 < N; k0 += NN_CA) {
-        build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
+        build_chunk(M.rc2, M.pi_rc, L, ux, uy, uz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
         __syncthreads();
         g_chunk(M, S, L, elem, N, k0, lane, M.rc2, M.pi_rc, Grep, &extrap);
         __syncthreads();
@@ -1055,10 +1066,10 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_f(const NnConst *__restrict__
     const float *Crep = nv.C + (size_t)rep * nv.maxN * NN_K1;
     int bad = 0;
 
-    // dynamic LDS: fx fy fz (double N each) | ax ay az (u64 N) | elem (u8 N)
-    double *fx = (double *)smem_raw, *fy = fx + N, *fz = fy + N;
-    unsigned long long *ax = (unsigned long long *)(fz + N), *ay = ax + N, *az = ay + N;
-    uint8_t *elem = (uint8_t *)(az + N);
+    // dynamic LDS: ax ay az (u64 N each) | ux uy uz (u32 N) | elem (u8 N)
+    unsigned long long *ax = (unsigned long long *)smem_raw, *ay = ax + N, *az = ay + N;
+    uint32_t *ux = (uint32_t *)(az + N), *uy = ux + N, *uz = uy + N;
+    uint8_t *elem = (uint8_t *)(uz + N);
 
     const int ne_model = M.nelem;
     load_sftab(S, M, lane);
@@ -1073,16 +1084,16 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_f(const NnConst *__restrict__
         double f0 = ih[0] * r0 + ih[3] * r1 + ih[6] * r2;
         double f1 = ih[1] * r0 + ih[4] * r1 + ih[7] * r2;
         double f2 = ih[2] * r0 + ih[5] * r1 + ih[8] * r2;
-        fx[i] = f0 - floor(f0);
-        fy[i] = f1 - floor(f1);
-        fz[i] = f2 - floor(f2);
+        ux[i] = frac_fix(f0 - floor(f0));
+        uy[i] = frac_fix(f1 - floor(f1));
+        uz[i] = frac_fix(f2 - floor(f2));
         ax[i] = ay[i] = az[i] = 0ull;
     }
     __syncthreads();
 
     // ---- pass F: forces ----
     for (int k0 = 0; k0 < N; k0 += NN_CA) {
-        build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
+        build_chunk(M.rc2, M.pi_rc, L, ux, uy, uz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
         for (int q = lane; q < NN_CA * NN_K1; q += NN_BLOCK) {
             const int ci = q / NN_K1, k = q % NN_K1;
             const int i = k0 + ci;
@@ -1149,10 +1160,10 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nn_frame(const NnConst *__restrict
     const long long ao = tv.fr_aoff[f];
     const int lane = threadIdx.x;
     int bad = 0, extrap = 0;
-    // dynamic LDS: fx fy fz (double N) | ax ay az (u64 N) | gx gy gz (float N) | elem (u8 N)
-    double *fx = (double *)smem_raw, *fy = fx + N, *fz = fy + N;
-    unsigned long long *ax = (unsigned long long *)(fz + N), *ay = ax + N, *az = ay + N;
-    float *gx = (float *)(az + N), *gy = gx + N, *gz = gy + N;
+    // dynamic LDS: ax ay az (u64 N) | ux uy uz (u32 N) | gx gy gz (float N) | elem (u8 N)
+    unsigned long long *ax = (unsigned long long *)smem_raw, *ay = ax + N, *az = ay + N;
+    uint32_t *ux = (uint32_t *)(az + N), *uy = ux + N, *uz = uy + N;
+    float *gx = (float *)(uz + N), *gy = gx + N, *gz = gy + N;
     uint8_t *elem = (uint8_t *)(gz + N);
     load_sftab(S, M, lane);
     for (int k = lane; k < NN_MAXE * NN_K1; k += NN_BLOCK) sdfac[k / NN_K1][k % NN_K1] = M.dfac[k / NN_K1][k % NN_K1];
@@ -1167,16 +1178,16 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nn_frame(const NnConst *__restrict
         const double f0 = ih[0] * r[0] + ih[3] * r[1] + ih[6] * r[2];
         const double f1 = ih[1] * r[0] + ih[4] * r[1] + ih[7] * r[2];
         const double f2 = ih[2] * r[0] + ih[5] * r[1] + ih[8] * r[2];
-        fx[i] = f0 - floor(f0);
-        fy[i] = f1 - floor(f1);
-        fz[i] = f2 - floor(f2);
+        ux[i] = frac_fix(f0 - floor(f0));
+        uy[i] = frac_fix(f1 - floor(f1));
+        uz[i] = frac_fix(f2 - floor(f2));
         ax[i] = ay[i] = az[i] = 0ull;
     }
     __syncthreads();
     float *Grow = tv.G + (size_t)ao * NN_K1;
     if (mode == 0) {
         for (int k0 = 0; k0 < N; k0 += NN_CA) {
-            build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
+            build_chunk(M.rc2, M.pi_rc, L, ux, uy, uz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
             __syncthreads();
             g_chunk(M, S, L, elem, N, k0, lane, M.rc2, M.pi_rc, Grow, &extrap);
             __syncthreads();
@@ -1186,7 +1197,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nn_frame(const NnConst *__restrict
     // forces with the current network (C rows from k_nn_mlp)
     const float *Crow = tv.C + (size_t)ao * NN_K1;
     for (int k0 = 0; k0 < N; k0 += NN_CA) {
-        build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
+        build_chunk(M.rc2, M.pi_rc, L, ux, uy, uz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
         for (int q = lane; q < NN_CA * NN_K1; q += NN_BLOCK) {
             const int ci = q / NN_K1, k = q % NN_K1;
             const int i = k0 + ci;
@@ -1225,7 +1236,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nn_frame(const NnConst *__restrict
     // Jacobian-vector product
     float *Urow = tv.U + (size_t)ao * NN_K1;
     for (int k0 = 0; k0 < N; k0 += NN_CA) {
-        build_chunk(M.rc2, M.pi_rc, L, fx, fy, fz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
+        build_chunk(M.rc2, M.pi_rc, L, ux, uy, uz, elem, h, imax, w[0], w[1], w[2], N, k0, lane, &bad);
         __syncthreads();
         jvp_chunk(S, L, elem, N, k0, lane, M.rc2, M.pi_rc, gx, gy, gz, sdfac, Urow);
         __syncthreads();
@@ -1569,7 +1580,7 @@ __global__ void k_nn_adam(NnConst *Mg, const float *partials, int W, float *grad
     *par -= lr * mh / (sqrtf(vh) + eps);
 }
 
-size_t mdt_nn_frame_smem(int N) { return (size_t)N * (3 * 8 + 3 * 8 + 3 * 4 + 1) + 16; }
+size_t mdt_nn_frame_smem(int N) { return (size_t)N * (3 * 8 + 3 * 4 + 3 * 4 + 1) + 16; }
 
 hipError_t mdt_launch_nn_frame(int nblocks, size_t smem, const NnConst *M, const NnTrainView &tv, const int *frames,
                                int mode, double cEw, double cFw, hipStream_t st) {
@@ -1600,7 +1611,8 @@ hipError_t mdt_launch_nn_adam(NnConst *M, const float *partials, int W, float *g
 
 // dynamic LDS of the larger of k_nene_g (4 N doubles, u16 + u8 per atom)
 // and k_nene_f (3 N doubles, 3 N u64, u8 per atom)
-size_t mdt_nene_smem(int N) { return (size_t)N * (6 * 8 + 1) + 16; }
+// the larger of k_nene_g's (23 B per atom) and k_nene_f's (37 B per atom)
+size_t mdt_nene_smem(int N) { return (size_t)N * (3 * 8 + 3 * 4 + 1) + 16; }
 
 // static LDS: the larger of the two kernels'
 hipError_t mdt_nene_attributes(hipFuncAttributes *fa) {
