@@ -232,14 +232,18 @@ int mdt_batch_set_bead_mode(mdt_batch *b, int mode);
  */
 int mdt_batch_set_com_removal(mdt_batch *b, int on);
 
-/* Verlet skin (A) of the classical atoms' neighbour lists in mixed bead
- * mode (default 0.3; 0 rebuilds them at every force call).  The lists are
- * built with cut-off + skin and kept while no classical atom of the
- * trajectory moved more than skin / 2 since the build; a cell narrower than
- * 2 (cut-off + skin) rebuilds every call.  Every pair is re-checked against
- * the cut-off, so the skin changes only the summation order.  Extension: the reference
- * visits every pair under the minimum image at every force call
- * (src/pes_emt_mod.f90:1204-1206) and keeps no lists. */
+/* Verlet skin (A) of the neighbour lists kept across force calls (default
+ * 0.3; 0 rebuilds every list at every call): the classical atoms' lists in
+ * mixed bead mode, and the lists of size classes of more than 256 atoms,
+ * which then live in HBM.  A list is built with cut-off + skin and kept while
+ * no atom of its replica (or trajectory) moved more than skin / 2 since the
+ * build, in cells whose narrowest width exceeds 2 (cut-off + skin); narrower
+ * cells rebuild at every call.  Every pair is re-checked against the
+ * cut-off under the reference's minimum image, so the skin changes at most
+ * the summation order.  Extension: the reference visits every pair under
+ * the minimum image at every call (src/pes_emt_mod.f90:1204-1206) and keeps
+ * no lists.  Switching between 0 and a positive skin re-lays the size
+ * classes' lists (keeping their tuned threads per atom). */
 int mdt_batch_set_list_skin(mdt_batch *b, double skin);
 
 /*

@@ -97,6 +97,9 @@ struct mdt_context {
 // 2048 form a size class with its own launch shape (threads per atom, block,
 // LDS, neighbour buffer), so that small slabs in a mixed batch do not run
 // with the largest slab's shape.
+// size classes whose replicas have more atoms keep their lists across launches
+#define SKIN_MIN_ATOMS 256
+
 struct SizeClass {
     int nmax = 0;              // largest replica of the class
     int nrep = 0;
@@ -105,6 +108,11 @@ struct SizeClass {
     size_t smem = 0;
     uint16_t *d_nbr = nullptr; // neighbour lists in HBM when they do not fit in LDS
     long long nbr_stride = 0;
+    // Verlet skin of the HBM lists (classes above SKIN_MIN_ATOMS): positions
+    // at the build (3 nmax per block slot), counts (nmax per slot), valid flags
+    double *d_ref = nullptr;
+    uint16_t *d_kcnt = nullptr;
+    int *d_valid = nullptr;
     float tuned_ms = 0.f;
 };
 
@@ -172,7 +180,7 @@ struct mdt_batch {
     uint16_t *d_mix_cnt = nullptr;
     int *d_mix_valid = nullptr;
     long long mix_ns = 0;
-    double mix_skin = 0.3;  // A
+    double list_skin = 0.3;  // A, mdt_batch_set_list_skin
     int mix_tpa = 0, mix_nbcap = 0, mix_nmax = 0, mix_pmax = 0;
     size_t mix_smem = 0;
     float mix_tuned_ms = 0.f;
@@ -220,7 +228,7 @@ struct mdt_batch {
         v.mix_fold = d_mix_fold;
         v.mix_cnt = d_mix_cnt;
         v.mix_valid = d_mix_valid;
-        v.mix_skin = mix_skin;
+        v.mix_skin = list_skin;
         v.mix_ns = mix_ns;
         v.obs = d_obs;
         v.done = d_obs ? d_done : nullptr;
@@ -441,12 +449,31 @@ static void build_cjk_host(int P, double *cjk) {  // rpmd.f90:36-59
         }
 }
 
+static void free_class_lists(SizeClass &c) {
+    void *ptrs[] = {c.d_nbr, c.d_ref, c.d_kcnt, c.d_valid};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    c.d_nbr = nullptr;
+    c.d_ref = nullptr;
+    c.d_kcnt = nullptr;
+    c.d_valid = nullptr;
+    c.nbr_stride = 0;
+}
+
 static void free_classes(mdt_batch *b) {
     for (SizeClass &c : b->cls) {
         if (c.d_reps) (void)hipFree(c.d_reps);
-        if (c.d_nbr) (void)hipFree(c.d_nbr);
+        free_class_lists(c);
     }
     b->cls.clear();
+}
+
+// new positions or skin: every kept neighbour list is rebuilt at the next launch
+static int invalidate_lists(mdt_batch *b) {
+    for (SizeClass &c : b->cls)
+        if (c.d_valid) HIPCHK(hipMemset(c.d_valid, 0, sizeof(int) * c.nrep));
+    if (b->d_mix_valid) HIPCHK(hipMemset(b->d_mix_valid, 0, sizeof(int) * b->mix_traj.size()));
+    return 0;
 }
 
 // Size classes of the replicas that k_replica computes: every replica, or in
@@ -730,8 +757,8 @@ int mdt_batch_set_state(mdt_batch *b, const double *r, const double *v) {
         HIPCHK(hipMemcpyAsync(b->d_v, tmp.data(), sizeof(double) * b->tot3pn, hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
     }
-    if (b->d_mix_valid)  // new positions: the kept classical lists are stale
-        HIPCHK(hipMemset(b->d_mix_valid, 0, sizeof(int) * b->mix_traj.size()));
+    int rc = invalidate_lists(b);  // new positions: the kept lists are stale
+    if (rc) return rc;
     b->have_state = true;
     return 0;
 }
@@ -833,21 +860,27 @@ static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap) {
     int cap = nbcap > 0 ? nbcap : 96;
     if (cap > c.nmax - 1) cap = std::max(1, c.nmax - 1);
     while (tpa > 1 && c.nmax * tpa > 512) tpa >>= 1;
+    // replicas above SKIN_MIN_ATOMS keep their lists in HBM across launches
+    // (Verlet skin) unless the skin is 0; smaller ones rebuild every launch,
+    // their scan visiting few more candidates than a kept list holds
+    const bool skin = b->list_skin > 0.0 && c.nmax > SKIN_MIN_ATOMS;
     const size_t smem_lds = mdt_replica_smem(c.nmax, cap, true);
-    const bool in_lds = smem_lds <= b->lds_limit;
+    const bool in_lds = !skin && smem_lds <= b->lds_limit;
     const size_t smem = in_lds ? smem_lds : mdt_replica_smem(c.nmax, cap, false);
     if (smem > b->lds_limit)
         return fail(MDT_EUNSUP, "replica of %d atoms needs %zu bytes of LDS (limit %zu)", c.nmax, smem,
                     b->lds_limit);
-    if (c.d_nbr) {
-        HIPCHK(hipFree(c.d_nbr));
-        c.d_nbr = nullptr;
-        c.nbr_stride = 0;
-    }
+    free_class_lists(c);
     if (!in_lds) {
         c.nbr_stride = (long long)c.nmax * cap;
         int rc = dalloc(&c.d_nbr, (size_t)c.nbr_stride * c.nrep);
         if (rc) return rc;
+    }
+    if (skin) {
+        int rc = dalloc(&c.d_ref, 3 * (size_t)c.nmax * c.nrep) | dalloc(&c.d_kcnt, (size_t)c.nmax * c.nrep) |
+                 dalloc(&c.d_valid, (size_t)c.nrep);
+        if (rc) return rc;
+        HIPCHK(hipMemset(c.d_valid, 0, sizeof(int) * c.nrep));
     }
     const int rounded = ((c.nmax * tpa + 63) / 64) * 64;
     c.block = std::min(512, std::max(64, rounded));
@@ -964,6 +997,11 @@ static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hip
         v.nbr_global = k.d_nbr;
         v.nbr_stride = k.nbr_stride;
         v.nbcap = k.nbcap;
+        v.rl_ref = k.d_ref;
+        v.rl_cnt = k.d_kcnt;
+        v.rl_valid = k.d_valid;
+        v.rl_skin = b->list_skin;
+        v.rl_ns = k.nmax;
         HIPCHK(mdt_launch_replica(k.tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, add_fin, st));
     }
     if (!b->mix_traj.empty()) return launch_mixed(b, base, b->mix_tpa, add_fin, st);
@@ -1306,10 +1344,17 @@ int mdt_batch_set_list_skin(mdt_batch *b, double skin) {
     if (!b) return fail(MDT_EINVAL, "NULL batch");
     if (!(skin >= 0.0 && skin < 5.0)) return fail(MDT_EINVAL, "list skin %g A (0 <= skin < 5)", skin);
     HIPCHK(hipSetDevice(b->ctx->device));
-    if (skin != b->mix_skin) drop_graph(b);
-    b->mix_skin = skin;
-    if (b->d_mix_valid) HIPCHK(hipMemset(b->d_mix_valid, 0, sizeof(int) * b->mix_traj.size()));
-    return 0;
+    if (skin != b->list_skin) drop_graph(b);
+    const bool relayout = (skin > 0.0) != (b->list_skin > 0.0);  // classes' lists move between LDS and HBM
+    b->list_skin = skin;
+    if (relayout && !b->cls.empty()) {
+        int rc = lds_limit(b);
+        if (rc) return rc;
+        for (SizeClass &c : b->cls)
+            if ((rc = configure_class(b, c, c.tpa > 0 ? c.tpa : heuristic_tpa(c.nmax), b->nbcap_req))) return rc;
+        note_tpa(b);
+    }
+    return invalidate_lists(b);
 }
 
 int mdt_batch_set_bead_mode(mdt_batch *b, int mode) {

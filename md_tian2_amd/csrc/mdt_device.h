@@ -116,6 +116,15 @@ struct BatchView {
     int *mix_valid;
     double mix_skin;
     long long mix_ns;
+    // k_replica's lists kept across launches (Verlet skin), per block slot of
+    // a size-class launch with lists in HBM: positions at the list build (SoA,
+    // rl_ns per coordinate), list counts and a valid flag; rl_valid NULL: the
+    // lists are rebuilt at every launch
+    double *rl_ref;
+    uint16_t *rl_cnt;
+    int *rl_valid;
+    double rl_skin;
+    long long rl_ns;
     struct TrajObs *obs;
     int *done;
     double obs_ul, obs_ll, ads_start, ads_end;

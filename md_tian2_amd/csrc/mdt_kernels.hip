@@ -298,13 +298,52 @@ __global__ void __launch_bounds__(512, 4)
     uint16_t *nbr = NBR_LDS ? cnt + N : bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
     uint8_t *typ = (uint8_t *)(cnt + N + (NBR_LDS ? (size_t)N * cap : 0));
     int bad_local = 0;
+    // Verlet skin (lists in HBM): the lists of this block slot are built with
+    // cut-off + skin and kept while no atom moved more than skin / 2 since
+    // the build; a kept list gets every entry's image recomputed from the
+    // current positions (the same rounding and tie rule as the scan).  Only
+    // in cells whose narrowest width exceeds 2 (cut-off + skin): there every
+    // image within cut-off + skin has fractional components in (-1/2, 1/2),
+    // so it is the rounded image the scan tests, and a pair within the
+    // cut-off now was listed at the build in that same image.  In narrower
+    // cells the rounded image need not be the nearest, and the lists are
+    // rebuilt at every launch.
+    const double skin = bv.rl_skin;
+    double wmin = 1e300;
+    for (int q = 0; q < 3; ++q) {
+        const double w = 1.0 / sqrt(cell.ih[q] * cell.ih[q] + cell.ih[q + 3] * cell.ih[q + 3] +
+                                    cell.ih[q + 6] * cell.ih[q + 6]);
+        wmin = w < wmin ? w : wmin;
+    }
+    const bool keep = !NBR_LDS && bv.rl_valid != nullptr && 2.0 * (ecg->cutr + skin) < wmin;
+    double *ref = keep ? bv.rl_ref + (long long)blockIdx.x * 3 * bv.rl_ns : nullptr;
+    __shared__ int s_reuse;
+    if (lane == 0) s_reuse = keep && bv.rl_valid[blockIdx.x];
     __syncthreads();
+    if (s_reuse) {  // every thread reads the flag before any clears it
+        double dmax = 0.0;
+        for (int i = lane; i < N; i += nthr) {
+            const double d0 = bv.x[roff + i] - ref[i], d1 = bv.x[roff + N + i] - ref[bv.rl_ns + i],
+                         d2 = bv.x[roff + 2 * N + i] - ref[2 * bv.rl_ns + i];
+            const double d2s = d0 * d0 + d1 * d1 + d2 * d2;
+            dmax = d2s > dmax ? d2s : dmax;
+        }
+        __syncthreads();
+        if (!(dmax <= 0.25 * skin * skin)) s_reuse = 0;  // benign race: every writer stores 0
+    }
+    __syncthreads();
+    const bool reuse = s_reuse != 0;
 
     // ---- phase 0: stage ----
     for (int i = lane; i < N; i += nthr) {
         const int ty = bv.type[aoff + i];
         typ[i] = (uint8_t)ty;
         const double r[3] = {bv.x[roff + i], bv.x[roff + N + i], bv.x[roff + 2 * N + i]};
+        if (keep && !reuse) {
+            ref[i] = r[0];
+            ref[bv.rl_ns + i] = r[1];
+            ref[2 * bv.rl_ns + i] = r[2];
+        }
         // fold into the cell: fractional in [0, 1)
         double f0 = cell.ih[0] * r[0] + cell.ih[3] * r[1] + cell.ih[6] * r[2];
         double f1 = cell.ih[1] * r[0] + cell.ih[4] * r[1] + cell.ih[7] * r[2];
@@ -352,13 +391,44 @@ __global__ void __launch_bounds__(512, 4)
         g02 = (float)(2 * (h[0] * h[6] + h[1] * h[7] + h[2] * h[8]));
         g12 = (float)(2 * (h[3] * h[6] + h[4] * h[7] + h[5] * h[8]));
     }
-    const float cut2f = (float)(cut2 * (1.0 + 1e-4));
+    const double cutl = keep ? (cutr + skin) * (cutr + skin) : cut2;
+    const float cut2f = (float)(cutl * (1.0 + 1e-4));
+    uint16_t *kcnt = keep ? bv.rl_cnt + (long long)blockIdx.x * bv.rl_ns : nullptr;
+    if (reuse) {  // kept list: the image of every entry from the current positions
+        for (int k0 = 0; k0 < N; k0 += apb) {
+            const int k = k0 + lane / TPA;
+            const int sub = lane % TPA;
+            if (k >= N) continue;
+            const int c = kcnt[k];
+            const float4 pk = f4[k];
+            for (int e = sub; e < c; e += TPA) {
+                const unsigned q = nbr[(size_t)k * cap + e];
+                const int n = q & MDT_NBR_IDX_MASK;
+                const float4 pn = f4[n];
+                const float e0 = pk.x - pn.x, e1 = pk.y - pn.y, e2 = pk.z - pn.z;
+                float m0f = rintf(e0), m1f = rintf(e1), m2f = rintf(e2);
+                const bool t0 = fabsf(e0 - m0f) > 0.4999f, t1 = fabsf(e1 - m1f) > 0.4999f,
+                           t2 = fabsf(e2 - m2f) > 0.4999f;
+                if (t0 | t1 | t2) {
+                    const double *xr = bv.x + roff;
+                    const double a0 = xr[k], a1 = xr[N + k], a2 = xr[2 * N + k];
+                    const double b0 = xr[n], b1 = xr[N + n], b2 = xr[2 * N + n];
+                    if (t0) m0f = (float)tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, sx[k] - sx[n]);
+                    if (t1) m1f = (float)tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, sy[k] - sy[n]);
+                    if (t2) m2f = (float)tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, sz[k] - sz[n]);
+                }
+                const int code = ((int)m0f + 1) + 3 * ((int)m1f + 1) + 9 * ((int)m2f + 1);
+                nbr[(size_t)k * cap + e] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
+            }
+            if (sub == 0) cnt[k] = (uint16_t)c;
+        }
+    }
     // only the owned candidates are visited, in increasing index: below k
     // those of the other parity (a = (k + 1) / 2 of them), above k those of
     // the same parity -- owns(k, n) of every n, without the half of the
     // lanes that would test and drop the others
     const int mmax = N / 2 + 1;
-    for (int k0 = 0; k0 < N; k0 += apb) {
+    for (int k0 = 0; k0 < (reuse ? 0 : N); k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
         const bool act = k < N;
@@ -405,9 +475,13 @@ __global__ void __launch_bounds__(512, 4)
             }
             c += __popcll(m);
         }
-        if (act && sub == 0) cnt[k] = (uint16_t)(c < cap ? c : cap);
+        if (act && sub == 0) {
+            cnt[k] = (uint16_t)(c < cap ? c : cap);
+            if (keep) kcnt[k] = cnt[k];
+        }
     }
     __syncthreads();
+    if (keep && !reuse && lane == 0) bv.rl_valid[blockIdx.x] = 1;  // read by the next launch
     for (int i = lane; i < N; i += nthr) {
         shi[i] = 0ull;
         smid[i] = 0ull;

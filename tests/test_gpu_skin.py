@@ -1,0 +1,103 @@
+"""Neighbour lists kept across force calls (Verlet skin) in k_replica.
+
+Size classes of more than 256 atoms keep their owned half lists in HBM,
+built with cut-off + skin, while no atom of the replica moved more than
+skin / 2 since the build; a kept list gets the image of every entry
+recomputed from the current positions.  The reference visits every pair
+under the minimum image at every call (src/pes_emt_mod.f90:1204-1206), so
+parity is what a rebuild at every call gives:
+
+* skins 0 (rebuild every call), 0.3 A (default) and 0.02 A (a rebuild every
+  few steps) give the same trajectories to rounding (only the summation
+  order of sigma in registers can differ), over 200 steps in which the
+  atoms move beyond half the default skin;
+* the forces of the final state equal the oracle's;
+* a 4x4 cell (10 A wide, below 2 (cut-off + skin), half-cell ties inside the
+  cut-off) rebuilds at every call: there the rounded image of a pair need not
+  be its nearest, so a list built for one image could miss the pair after it
+  rounds to another; the skins agree to rounding there too;
+* new positions from set_state drop the kept lists."""
+import numpy as np
+import pytest
+
+import oracle
+from md_tian2_amd import geometry, mdt, pesfile
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(ctx, systems, skin, steps, dt=0.5, seed=4):
+    b = mdt.Batch(ctx, systems, dt=dt, tsurf=300.0, tproj=300.0, seed=seed)
+    b.set_list_skin(skin)
+    b.compute_forces()
+    b.step(steps)
+    assert not b.status().any()
+    return b
+
+
+def _check_oracle(systems, out, cols, pesm, tol=1e-9):
+    """Energies and forces against the oracle.  With beads, the
+    reported f of a Langevin atom also holds its ring-polymer spring force
+    (the propagator's own use of f), so forces are compared on the other
+    atoms (the projectile and the fixed atoms) there."""
+    for s, o in zip(systems, out):
+        sr = s.copy()
+        sr.r = o["r"].copy()
+        e, f, d = oracle.emt_energy_forces(sr, cols, pesm)
+        assert np.abs(o["epot"] - e).max() <= tol * np.abs(e).max()
+        rows = np.ones(s.natoms, dtype=bool)
+        if s.nbeads > 1:
+            rows = (np.asarray(s.algo)[s.idx] != geometry.PROP_LANGEVIN) | s.fixed[:, 0, :].all(axis=1)
+        assert rows.sum() >= 1
+        assert np.abs(o["f"][rows] - f[rows]).max() <= tol
+
+
+@pytest.mark.parametrize("metal,n,nlayers,nbeads", [("Pt", 8, 4, 2), ("Au", 4, 17, 1)])
+def test_kept_lists_equal_rebuilt_lists(gpu_ctx, pes_hpt, pes_hau, metal, n, nlayers, nbeads):
+    rng = np.random.default_rng(60 + n)
+    pes = pes_hpt if metal == "Pt" else pes_hau
+    systems = []
+    for z in (1.8, 3.0):
+        s = geometry.h_on_slab(metal, n=n, nlayers=nlayers, nbeads=nbeads, rng=rng, z0=z, einc=0.5, tsurf=300.0)
+        s.r[0] += rng.normal(0, 0.05, s.r[0].shape)
+        systems.append(s)
+    assert systems[0].natoms > 256
+    cols, pesm = pesfile.emt_tables(pes, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    res = {}
+    for skin in (0.0, 0.3, 0.02):
+        b = _run(gpu_ctx, systems, skin, 200)
+        res[skin] = b.get_state()
+        if skin == 0.3:
+            _check_oracle(systems, res[skin], cols, pesm)
+            # new positions: the kept lists are dropped
+            b.set_state([s.r for s in systems], [s.v for s in systems])
+            b.compute_forces()
+            _check_oracle(systems, b.get_state(), cols, pesm)
+        b.close()
+    moved = max(np.abs(o["r"][1:] - s.r[1:]).max() for s, o in zip(systems, res[0.0]))
+    print(f"{metal} {n}x{n}x{nlayers}, {nbeads} beads: atoms moved up to {moved:.3f} A")
+    assert moved > 0.15  # the default skin's lists were rebuilt on the way
+    for skin in (0.3, 0.02):
+        for ref, o in zip(res[0.0], res[skin]):
+            assert np.abs(o["r"] - ref["r"]).max() <= 1e-9, skin
+            assert np.abs(o["v"] - ref["v"]).max() <= 1e-9 * np.abs(ref["v"]).max(), skin
+            assert np.abs(o["epot"] - ref["epot"]).max() <= 1e-9 * np.abs(ref["epot"]).max(), skin
+
+
+def test_skin_switch_keeps_batch_usable(gpu_ctx, pes_hpt):
+    """Switching the skin off and on moves the lists between LDS and HBM;
+    the forces stay the oracle's."""
+    rng = np.random.default_rng(70)
+    systems = [geometry.h_on_slab("Pt", n=8, nlayers=4, rng=rng, z0=2.0, tsurf=300.0) for _ in range(3)]
+    cols, pesm = pesfile.emt_tables(pes_hpt, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    b = mdt.Batch(gpu_ctx, systems, dt=0.5, tsurf=300.0, tproj=300.0)
+    b.tune(tpa=0)
+    for skin in (0.0, 0.3, 0.0, 0.25):
+        b.set_list_skin(skin)
+        b.compute_forces()
+        b.step(20)
+        assert not b.status().any()
+        _check_oracle(systems, b.get_state(), cols, pesm)
+    b.close()
