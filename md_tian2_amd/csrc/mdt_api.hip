@@ -78,9 +78,13 @@ const double kNneighs[3] = {12.0, 6.0, 24.0};
 
 }  // namespace
 
+#define MDT_NSIDE 3  // side streams for a step's independent force launches
+
 struct mdt_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side[MDT_NSIDE] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[MDT_NSIDE] = {};
     bool have_params = false;
     int ntypes = 0;
     EmtConst ec{};
@@ -371,8 +375,13 @@ int mdt_create(int device, mdt_context **out) {
     mdt_context *c = new mdt_context();
     c->device = device;
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    for (int i = 0; i < MDT_NSIDE && e == hipSuccess; ++i) {
+        e = hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&c->join_ev[i], hipEventDisableTiming);
+    }
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
     if (e != hipSuccess) {
-        delete c;
+        mdt_destroy(c);
         return fail(MDT_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
     }
     *out = c;
@@ -386,6 +395,11 @@ int mdt_destroy(mdt_context *ctx) {
     if (ctx->d_ec) (void)hipFree(ctx->d_ec);
     if (ctx->d_nn) (void)hipFree(ctx->d_nn);
     delete ctx->nn;
+    for (int i = 0; i < MDT_NSIDE; ++i) {
+        if (ctx->side[i]) (void)hipStreamDestroy(ctx->side[i]);
+        if (ctx->join_ev[i]) (void)hipEventDestroy(ctx->join_ev[i]);
+    }
+    if (ctx->fork_ev) (void)hipEventDestroy(ctx->fork_ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -976,6 +990,55 @@ static int configure(mdt_batch *b, int tpa, int nbcap) {
     return 0;
 }
 
+// A step's force launches over disjoint replicas (the size classes, the
+// mixed-model kernel) go round-robin to the context's side streams, forked
+// from the batch's stream by an event and joined back to it, so that a small
+// or latency-bound launch overlaps the others.  Inside graph capture, and
+// for a single launch, every launch stays on the batch's stream.
+struct Fork {
+    mdt_context *c = nullptr;
+    hipStream_t st = nullptr;
+    bool on = false;
+    unsigned used = 0;
+    int next = 0;
+};
+
+static int fork_open(Fork &f, mdt_context *c, hipStream_t st, int nlaunch) {
+    f = Fork{};
+    f.c = c;
+    f.st = st;
+    if (nlaunch < 2) return 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return 0;
+    HIPCHK(hipEventRecord(c->fork_ev, st));
+    f.on = true;
+    return 0;
+}
+
+static int fork_stream(Fork &f, hipStream_t *out) {
+    *out = f.st;
+    if (!f.on) return 0;
+    const int i = f.next++ % MDT_NSIDE;
+    if (!(f.used & (1u << i))) {
+        HIPCHK(hipStreamWaitEvent(f.c->side[i], f.c->fork_ev, 0));
+        f.used |= 1u << i;
+    }
+    *out = f.c->side[i];
+    return 0;
+}
+
+static int fork_close(Fork &f) {
+    if (!f.on) return 0;
+    for (int i = 0; i < MDT_NSIDE; ++i)
+        if (f.used & (1u << i)) {
+            HIPCHK(hipEventRecord(f.c->join_ev[i], f.c->side[i]));
+            HIPCHK(hipStreamWaitEvent(f.st, f.c->join_ev[i], 0));
+        }
+    f.on = false;
+    return 0;
+}
+
 static int launch_mixed(mdt_batch *b, const BatchView &base, int tpa, int add_fin, hipStream_t st) {
     if (!b->ctx->d_ec) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
     BatchView v = base;
@@ -988,10 +1051,15 @@ static int launch_mixed(mdt_batch *b, const BatchView &base, int tpa, int add_fi
 }
 
 // calc_force on every replica, one launch per size class
-static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hipStream_t st) {
+static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hipStream_t main_st) {
     mdt_context *c = b->ctx;
     if (!c->d_ec) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
+    Fork fk;
+    int rc = fork_open(fk, c, main_st, (int)b->cls.size() + (b->mix_traj.empty() ? 0 : 1));
+    if (rc) return rc;
+    hipStream_t st = main_st;
     for (const SizeClass &k : b->cls) {
+        if ((rc = fork_stream(fk, &st))) return rc;
         BatchView v = base;
         v.rep_list = k.d_reps;
         v.nbr_global = k.d_nbr;
@@ -1004,8 +1072,11 @@ static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hip
         v.rl_ns = k.nmax;
         HIPCHK(mdt_launch_replica(k.tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, add_fin, st));
     }
-    if (!b->mix_traj.empty()) return launch_mixed(b, base, b->mix_tpa, add_fin, st);
-    return 0;
+    if (!b->mix_traj.empty()) {
+        if ((rc = fork_stream(fk, &st))) return rc;
+        if ((rc = launch_mixed(b, base, b->mix_tpa, add_fin, st))) return rc;
+    }
+    return fork_close(fk);
 }
 
 static int ensure_nn(mdt_batch *b) {
@@ -1037,12 +1108,18 @@ static int launch_pes(mdt_batch *b, const BatchView &bv, int add_fin, hipStream_
     if (b->pes == 1) {
         // one launch per size class: each sizes its LDS for its largest
         // replica, so small slabs keep the occupancy of a uniform batch
+        Fork fk;
+        int rc = fork_open(fk, c, st, (int)b->cls.size());
+        if (rc) return rc;
         for (const SizeClass &k : b->cls) {
             BatchView v = bv;
             v.rep_list = k.d_reps;
+            hipStream_t s2 = st;
+            if ((rc = fork_stream(fk, &s2))) return rc;
             HIPCHK(mdt_launch_nene(k.nrep, mdt_nene_smem(k.nmax), c->d_nn, v, b->d_nnG, b->d_nnC, b->d_nnflag,
-                                   b->d_nnF, b->maxN, add_fin, 1, st));
+                                   b->d_nnF, b->maxN, add_fin, 1, s2));
         }
+        if ((rc = fork_close(fk))) return rc;
         if (c->have_params) {
             BatchView fb = bv;
             fb.only = b->d_nnflag;
