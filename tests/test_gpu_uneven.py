@@ -62,3 +62,33 @@ def test_uneven_batch_matches_separate_batches(gpu_ctx, pes_hau):
         b1.close()
         for k in ("r", "v", "f", "epot"):
             assert np.array_equal(together[t][k], alone[k]), (t, k)
+
+
+@pytest.mark.parametrize("bead_mode", [0, 1])
+def test_uneven_graph_replay_equals_forked_launches(gpu_ctx, pes_hau, bead_mode):
+    """Eager steps fork the size classes (and the mixed-model kernel) onto
+    side streams; graph capture keeps them on the batch's stream.  Both give
+    bitwise the same state, replay after replay with eager work (state
+    downloads) between the replays."""
+    rng = np.random.default_rng(23)
+    systems = _mixed(rng)
+    cols, pesm = pesfile.emt_tables(pes_hau, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    batches = []
+    for chunk in (0, 5):
+        b = mdt.Batch(gpu_ctx, systems, dt=0.1, tsurf=300.0, tproj=300.0, seed=8)
+        b.set_bead_mode(bead_mode)
+        b.set_graph(chunk)
+        b.compute_forces()
+        batches.append(b)
+    for rnd in range(3):
+        outs = []
+        for b in batches:
+            b.step(10)  # two replays of the captured chunk per round
+            assert not b.status().any()
+            outs.append(b.get_state())
+        for o0, o1 in zip(*outs):
+            for k in ("r", "v", "f", "a", "epot", "dens"):
+                assert np.array_equal(o0[k], o1[k]), (rnd, k)
+    for b in batches:
+        b.close()
