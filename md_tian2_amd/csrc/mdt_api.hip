@@ -482,11 +482,15 @@ static void free_classes(mdt_batch *b) {
     b->cls.clear();
 }
 
-// new positions or skin: every kept neighbour list is rebuilt at the next launch
+// new positions or skin: every kept neighbour list is rebuilt at the next
+// launch (after the batch's queued launches, which may still set the flags)
 static int invalidate_lists(mdt_batch *b) {
+    hipStream_t st = b->ctx->stream;
+    HIPCHK(hipStreamSynchronize(st));
     for (SizeClass &c : b->cls)
-        if (c.d_valid) HIPCHK(hipMemset(c.d_valid, 0, sizeof(int) * c.nrep));
-    if (b->d_mix_valid) HIPCHK(hipMemset(b->d_mix_valid, 0, sizeof(int) * b->mix_traj.size()));
+        if (c.d_valid) HIPCHK(hipMemsetAsync(c.d_valid, 0, sizeof(int) * c.nrep, st));
+    if (b->d_mix_valid) HIPCHK(hipMemsetAsync(b->d_mix_valid, 0, sizeof(int) * b->mix_traj.size(), st));
+    HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
 
