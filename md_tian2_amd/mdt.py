@@ -315,8 +315,9 @@ class Batch:
         _check(lib().mdt_batch_set_com_removal(self.h, 1 if on else 0))
 
     def set_list_skin(self, skin):
-        """Verlet skin (A) of the classical atoms' lists in mixed bead mode;
-        0 rebuilds them at every force call."""
+        """Verlet skin (A) of the neighbour lists kept across force calls
+        (mixed-model classical atoms; size classes above 256 atoms) in cells
+        wider than 2 (cut-off + skin); 0 rebuilds every list at every call."""
         _check(lib().mdt_batch_set_list_skin(self.h, float(skin)))
 
     def set_observe(self, proj_ul=None, proj_ll=None, adsorption=None, exit_at_limits=True):
