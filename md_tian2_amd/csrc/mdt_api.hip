@@ -175,7 +175,13 @@ struct mdt_batch {
     // those trajectories run k_mixed (one workgroup each), the rest k_replica
     int bead_mode = 0;
     std::vector<uint8_t> h_type;    // atom types (0-based), host copy for validation
-    std::vector<int> mix_traj;      // trajectories on k_mixed
+    std::vector<int> mix_traj;      // trajectories on k_mixed, by atom count
+    // their size classes: a run of mix_traj (offset, count) and its LDS
+    struct MixClass {
+        int off = 0, n = 0;
+        size_t smem = 0;
+    };
+    std::vector<MixClass> mix_cls;
     int *d_mix = nullptr;
     uint16_t *d_mix_nbr = nullptr;  // classical atoms' owned half lists, one block per listed trajectory
     long long mix_stride = 0;
@@ -952,6 +958,7 @@ static int configure_mixed(mdt_batch *b, int tpa, int nbcap) {
     b->d_mix_cnt = nullptr;
     b->d_mix_valid = nullptr;
     b->mix_traj.clear();
+    b->mix_cls.clear();
     b->mix_nmax = b->mix_pmax = 0;
     if (b->bead_mode != 1) return 0;
     for (int t = 0; t < b->ntraj; ++t)
@@ -961,13 +968,34 @@ static int configure_mixed(mdt_batch *b, int tpa, int nbcap) {
             b->mix_pmax = std::max(b->mix_pmax, b->P[t]);
         }
     if (b->mix_traj.empty()) return 0;
+    // size classes (bounds 64, 128, ..., 2048 atoms), each launched with
+    // LDS for its own largest trajectory, so small slabs are not held to
+    // the occupancy of the largest
+    std::stable_sort(b->mix_traj.begin(), b->mix_traj.end(), [b](int x, int y) { return b->N[x] < b->N[y]; });
+    const int bounds[6] = {64, 128, 256, 512, 1024, 2048};
+    size_t smem = 0;
+    for (int i = 0; i < (int)b->mix_traj.size();) {
+        int k = 0;
+        while (b->N[b->mix_traj[i]] > bounds[k]) ++k;
+        mdt_batch::MixClass mc;
+        mc.off = i;
+        int nmax = 0, pmax = 0;
+        while (i < (int)b->mix_traj.size() && b->N[b->mix_traj[i]] <= bounds[k]) {
+            nmax = std::max(nmax, b->N[b->mix_traj[i]]);
+            pmax = std::max(pmax, b->P[b->mix_traj[i]]);
+            ++i;
+        }
+        mc.n = i - mc.off;
+        mc.smem = mdt_mixed_smem(nmax, pmax);
+        if (mc.smem + 4096 > b->lds_limit)  // 4 KiB: the kernel's static tables
+            return fail(MDT_EUNSUP, "mixed bead mode: %d atoms and %d beads need %zu bytes of LDS (limit %zu)",
+                        nmax, pmax, mc.smem + 4096, b->lds_limit);
+        smem = std::max(smem, mc.smem);
+        b->mix_cls.push_back(mc);
+    }
     const int ns = b->mix_nmax - 1;
     int cap = nbcap > 0 ? nbcap : 96;
     if (cap > ns - 1) cap = std::max(1, ns - 1);
-    const size_t smem = mdt_mixed_smem(b->mix_nmax, b->mix_pmax);
-    if (smem + 4096 > b->lds_limit)  // 4 KiB: the kernel's static tables
-        return fail(MDT_EUNSUP, "mixed bead mode: %d atoms and %d beads need %zu bytes of LDS (limit %zu)",
-                    b->mix_nmax, b->mix_pmax, smem + 4096, b->lds_limit);
     b->mix_stride = (long long)ns * cap;
     const size_t nm = b->mix_traj.size();
     b->mix_ns = ns;
@@ -1043,14 +1071,30 @@ static int fork_close(Fork &f) {
     return 0;
 }
 
-static int launch_mixed(mdt_batch *b, const BatchView &base, int tpa, int add_fin, hipStream_t st) {
+// one launch per size class of the mixed-model trajectories; a class's
+// block slots start at its offset in the trajectory list and in every
+// per-slot buffer (lists, kept-list state).  With fk, each class takes the
+// next side stream; without, all go to st.
+static int launch_mixed(mdt_batch *b, const BatchView &base, int tpa, int add_fin, hipStream_t st, Fork *fk = nullptr) {
     if (!b->ctx->d_ec) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
-    BatchView v = base;
-    v.rep_list = b->d_mix;
-    v.nbr_global = b->d_mix_nbr;
-    v.nbr_stride = b->mix_stride;
-    v.nbcap = b->mix_nbcap;
-    HIPCHK(mdt_launch_mixed(tpa, (int)b->mix_traj.size(), b->mix_smem, b->ctx->d_ec, b->d_pc, v, add_fin, st));
+    for (const mdt_batch::MixClass &mc : b->mix_cls) {
+        BatchView v = base;
+        const long long off = mc.off;
+        v.rep_list = b->d_mix + off;
+        v.nbr_global = b->d_mix_nbr + off * b->mix_stride;
+        v.nbr_stride = b->mix_stride;
+        v.nbcap = b->mix_nbcap;
+        v.mix_ref = b->d_mix_ref + off * 3 * b->mix_ns;
+        v.mix_fold = b->d_mix_fold + off * 3 * b->mix_ns;
+        v.mix_cnt = b->d_mix_cnt + off * b->mix_ns;
+        v.mix_valid = b->d_mix_valid + off;
+        hipStream_t s2 = st;
+        if (fk) {
+            int rc = fork_stream(*fk, &s2);
+            if (rc) return rc;
+        }
+        HIPCHK(mdt_launch_mixed(tpa, mc.n, mc.smem, b->ctx->d_ec, b->d_pc, v, add_fin, s2));
+    }
     return 0;
 }
 
@@ -1059,7 +1103,7 @@ static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hip
     mdt_context *c = b->ctx;
     if (!c->d_ec) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
     Fork fk;
-    int rc = fork_open(fk, c, main_st, (int)b->cls.size() + (b->mix_traj.empty() ? 0 : 1));
+    int rc = fork_open(fk, c, main_st, (int)(b->cls.size() + b->mix_cls.size()));
     if (rc) return rc;
     hipStream_t st = main_st;
     for (const SizeClass &k : b->cls) {
@@ -1076,10 +1120,7 @@ static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hip
         v.rl_ns = k.nmax;
         HIPCHK(mdt_launch_replica(k.tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, add_fin, st));
     }
-    if (!b->mix_traj.empty()) {
-        if ((rc = fork_stream(fk, &st))) return rc;
-        if ((rc = launch_mixed(b, base, b->mix_tpa, add_fin, st))) return rc;
-    }
+    if (!b->mix_traj.empty() && (rc = launch_mixed(b, base, b->mix_tpa, add_fin, main_st, &fk))) return rc;
     return fork_close(fk);
 }
 
