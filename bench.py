@@ -86,35 +86,164 @@ def read_traffic(ntraj):
     return kib * 1024.0
 
 
+def read_counters(ntraj):
+    """Counter figures of one k_replica launch on this workload from the
+    committed rocprofv3 --pmc summary (None when absent or for another batch
+    size): VALU instructions, executed FP64 FLOP (64 lanes x (add + mul +
+    2 fma + trans)), VALU busy, average duration."""
+    path = os.path.join(ROOT, "profiles", "counters_k_replica_h_au_6x6x4_4096traj.txt")
+    if ntraj != 4096 or not os.path.exists(path):
+        return None
+    kv = {}
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 2 and not line.startswith("#"):
+            try:
+                kv[parts[0]] = float(parts[1])
+            except ValueError:
+                pass
+    return kv or None
+
+
+def host_info():
+    """The host the CPU baseline runs on: CPUs this process may use
+    (sched_getaffinity, what nproc prints), all CPUs of the machine, the
+    model name from lscpu."""
+    import subprocess
+
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(), "model": model}
+
+
+# The CPU share of one GPU job on the GPU hosts: worker pools are capped
+# there at 16 processes whatever the machine's CPU count.
+CPU_SHARE = 16
+NATIVE_FLAGS = ["-O3", "-march=native", "-std=c11", "-fPIC", "-shared", "-D_DEFAULT_SOURCE"]
+
+
+def build_native_oracle():
+    """The oracle's sources built for this host's CPU (gcc -O3 -march=native)
+    in a temporary folder: the timed CPU baseline.  The checker build
+    (oracle/liboracle_mdt.so) keeps -ffp-contract=off."""
+    import subprocess
+    import tempfile
+
+    d = tempfile.mkdtemp(prefix="mdt_oracle_native_")
+    so = os.path.join(d, "liboracle_native.so")
+    src = [os.path.join(ROOT, "oracle", f) for f in ("mdt_oracle.c", "nene_oracle.c")]
+    subprocess.run(["gcc"] + NATIVE_FLAGS + ["-o", so] + src + ["-lm"], check=True, capture_output=True,
+                   timeout=300)
+    ver = subprocess.run(["gcc", "--version"], capture_output=True, text=True).stdout.splitlines()[0]
+    return so, ver
+
+
+def _c1_system(traj):
+    """Config 1: H/Au(111) 3x3x4 (36 Au, bottom 2 layers fixed, Langevin+LDFA
+    at 300 K) + H (Verlet) at z = 6 A, random x,y, Einc 1.92 eV, polar 0."""
+    from md_tian2_amd import geometry
+
+    g = np.random.default_rng(1000 + traj)
+    return geometry.h_on_slab("Au", n=3, nlayers=4, nfixed=2, rng=g, einc=1.92, polar_deg=0.0,
+                              azimuth_deg=0.0, z0=6.0, tsurf=300.0)
+
+
 def _cpu_worker(args):
-    n_steps, seed = args
+    """Runs trajectories on the oracle; returns (steps, seconds)."""
+    kind, items, n_steps = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     from md_tian2_amd import pesfile
 
     pes = pesfile.load_pes(os.path.join(ROOT, "md_tian2_amd", "pes", "EMT-HAu.pes"))
-    s = make_workload(1, seed=seed)[0]
-    cols, pesm = pesfile.emt_tables(pes, s.type_names)
-    st = oracle.OracleState(s, cols, pesm, step=0.1, tsurf=300.0, tproj=0.0)
-    st.calc_force()
-    t0 = time.perf_counter()
-    st.run(12345, seed, 1, n_steps)
-    return n_steps, time.perf_counter() - t0
+    steps, secs = 0, 0.0
+    for item in items:
+        s = make_workload(1, seed=item)[0] if kind == "c2" else _c1_system(item)
+        cols, pesm = pesfile.emt_tables(pes, s.type_names)
+        st = oracle.OracleState(s, cols, pesm, step=0.1, tsurf=300.0, tproj=0.0)
+        st.calc_force()  # setup, not timed (src/md_tian2.f90:69)
+        t0 = time.perf_counter()
+        st.run(12345, item, 1, n_steps)
+        secs += time.perf_counter() - t0
+        steps += n_steps
+    return steps, secs
 
 
-def cpu_baseline(seconds=12.0, procs=None):
-    """The oracle on this host: one process per core over disjoint
-    trajectories (the reference's job-array model)."""
-    procs = procs or min(16, os.cpu_count() or 1)
-    n1, t1 = _cpu_worker((20, 0))  # calibrate
-    per = max(20, int(seconds * n1 / t1))
+def _validate_native(native_so):
+    """The native build against the checker build: 20 steps of one config-2
+    trajectory agree to 1e-9 A (they differ only by FMA contraction)."""
+    import subprocess
+
+    code = ("import os,sys,json;sys.path.insert(0,%r);sys.path.insert(0,%r);"
+            "import bench,oracle;from md_tian2_amd import pesfile;"
+            "pes=pesfile.load_pes(os.path.join(%r,'md_tian2_amd','pes','EMT-HAu.pes'));"
+            "s=bench.make_workload(1,seed=3)[0];c,p=pesfile.emt_tables(pes,s.type_names);"
+            "st=oracle.OracleState(s,c,p,step=0.1,tsurf=300.0);st.calc_force();st.run(12345,3,1,20);"
+            "print(json.dumps(st.r.ravel().tolist()))") % (ROOT, os.path.join(ROOT, "oracle"), ROOT)
+    outs = []
+    for lib_path in (None, native_so):
+        env = dict(os.environ)
+        env.pop("MDT_ORACLE_LIB", None)
+        if lib_path:
+            env["MDT_ORACLE_LIB"] = lib_path
+        res = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+        if res.returncode != 0:
+            raise RuntimeError(res.stderr[-2000:])
+        outs.append(np.array(json.loads(res.stdout.strip().splitlines()[-1])))
+    return float(np.abs(outs[0] - outs[1]).max())
+
+
+def _pool_run(procs, jobs):
     with mp.get_context("spawn").Pool(procs) as pool:
         t0 = time.perf_counter()
-        res = pool.map(_cpu_worker, [(per, k + 1) for k in range(procs)])
+        res = pool.map(_cpu_worker, jobs)
         wall = time.perf_counter() - t0
     steps = sum(r[0] for r in res)
-    single = per / np.mean([r[1] for r in res])
-    return steps / wall, procs, per, single
+    per_proc = [r[0] / r[1] for r in res if r[1] > 0]
+    return steps / wall, float(np.mean(per_proc)), steps, wall
+
+
+def cpu_baseline(seconds=12.0, c1_steps=10000, c1_traj=100):
+    """The oracle -- the serial FP64 restatement of the reference algorithm,
+    with its dense N^2 arrays -- built with gcc -O3 -march=native on this
+    host and validated against the checker build, one process per core of
+    this job's CPU share over disjoint trajectories (the reference's job-array
+    model, scripts/submit_scripts/local/ajob_MDT2.sh): config 2's H/Au 6x6x4
+    on a bounded sample (about `seconds` per process), and config 1 in full
+    (c1_traj trajectories x c1_steps steps)."""
+    info = host_info()
+    procs = max(1, min(info["nproc"], CPU_SHARE))
+    native_so, gcc_ver = build_native_oracle()
+    dr = _validate_native(native_so)
+    if not dr <= 1e-9:
+        raise RuntimeError(f"native oracle build differs from the checker build by {dr} A")
+    os.environ["MDT_ORACLE_LIB"] = native_so  # inherited by the spawned workers
+    try:
+        _, cal = _cpu_worker(("c2", [0], 20))
+        per = max(20, int(seconds * 20 / cal))
+        c2_rate, c2_single, c2_steps, c2_wall = _pool_run(procs, [("c2", [k + 1], per) for k in range(procs)])
+        jobs = [("c1", list(range(k, c1_traj, procs)), c1_steps) for k in range(procs)]
+        c1_rate, c1_single, c1_total, c1_wall = _pool_run(procs, jobs)
+    finally:
+        os.environ.pop("MDT_ORACLE_LIB", None)
+    return {
+        "value": c2_rate, "unit": "trajectory-steps/s", "cores": procs, "kind": "port",
+        "sample": f"config 2 (H/Au 6x6x4, 145 atoms): {procs} processes x 1 trajectory x {per} steps",
+        "single_core": c2_single,
+        "host": dict(info, cpu_share=CPU_SHARE),
+        "build": {"compiler": gcc_ver, "flags": " ".join(NATIVE_FLAGS), "max_dr_vs_checker_A": dr},
+        "c1": {"value": c1_rate, "unit": "trajectory-steps/s", "single_core": c1_single, "trajectories": c1_traj,
+               "steps": c1_steps, "wall_s": c1_wall,
+               "workload": "config 1: H/Au(111) 3x3x4 + H, Einc 1.92 eV, polar 0, Tsurf 300 K, dt 0.1 fs, "
+                           f"{c1_traj} trajectories x {c1_steps} steps in full"},
+    }
 
 
 def copy_calibration():
@@ -140,7 +269,7 @@ def copy_calibration():
 def nn_variant(ctx, pes, args):
     """The NENE surrogate on its workload: 6x6x4 Au + H trajectories, slab
     Langevin at 300 K, H Verlet, dt 0.1 fs; EMT fallback on extrapolation
-    and the EMT judge on a rotating 1 % of the replicas every 10 steps."""
+    and the EMT judge on a rotating 1 % of the replicas every step (1 % of frames)."""
     from md_tian2_amd import mdt, surrogate
 
     model = surrogate.load_model()
@@ -149,7 +278,7 @@ def nn_variant(ctx, pes, args):
     systems = make_workload(args.nn_traj, seed=4242)
     ctx.set_nn(model, systems[0].type_names)
     b = mdt.Batch(ctx, systems, dt=0.1, tsurf=300.0, tproj=0.0, seed=777)
-    b.set_pes(1, judge_interval=10)
+    b.set_pes(1, judge_interval=1)
     b.compute_forces()
     if args.warmup:
         b.step(min(args.warmup, 10))
@@ -165,11 +294,68 @@ def nn_variant(ctx, pes, args):
         "ms_per_step": ms / args.nn_steps, "steps": args.nn_steps, "trajectories": len(systems),
         "workload": "NENE surrogate H/Au(111) 6x6x4 + H (24 G2 + 16 G4 per element, 6.5 A, 40-32-32-1 tanh), "
                     "slab Langevin 300 K, H Verlet, dt 0.1 fs, EMT fallback on extrapolation, EMT judge on a "
-                    "rotating 1 % of replicas every 10 steps",
+                    "rotating 1 % of replicas every step (1 % of frames)",
         "judge_force_rmse_eV_per_A": rmse, "judge_components": ncomp, "judge_max_abs_eV_per_A": mx,
         "fallback_replicas_last_step": nfb, "trajectories_flagged": int((st != 0).sum()),
         "precision": "FP32 symmetry functions, BF16 hi/lo-split MFMA network (FP32 accumulate), FP64 integration",
     }
+
+
+def c1_variant(ctx, pes, args):
+    """Config 1 on the GPU: 100 trajectories of H/Au(111) 3x3x4 + H (Einc
+    1.92 eV, polar 0, Tsurf 300 K), FP64, 10000 steps each, hipGraph chunks
+    of 50 steps -- the workload the CPU baseline times in full."""
+    from md_tian2_amd import mdt
+
+    systems = [_c1_system(t) for t in range(100)]
+    cols, pesm = pesfile_tables(pes, systems[0].type_names)
+    ctx.set_emt(cols, pesm)
+    b = mdt.Batch(ctx, systems, dt=0.1, tsurf=300.0, tproj=0.0, seed=12345)
+    b.tune(tpa=0)
+    b.set_graph(50)
+    b.compute_forces()
+    b.step(100)
+    b.step(args.c1_steps)
+    ms, _ = b.timing()
+    assert not b.status().any(), "trajectory failure flags set"
+    b.close()
+    print(f"[bench] c1: {args.c1_steps} steps of {len(systems)} trajectories, events {ms:.3f} ms",
+          file=sys.stderr, flush=True)
+    return {"value": len(systems) * args.c1_steps / (ms / 1e3), "unit": "trajectory-steps/s",
+            "ms_per_step": ms / args.c1_steps, "steps": args.c1_steps, "trajectories": len(systems),
+            "workload": "config 1: H/Au(111) 3x3x4 (36 Au, 18 mobile Langevin+LDFA at 300 K) + H Verlet, "
+                        "Einc 1.92 eV, polar 0, dt 0.1 fs, FP64, hipGraph chunks of 50 steps"}
+
+
+def pesfile_tables(pes, type_names):
+    from md_tian2_amd import pesfile
+
+    return pesfile.emt_tables(pes, type_names)
+
+
+def near_radius(pes):
+    """The near radius of k_replica's lists (pairs beyond it are evaluated only
+    next to an atom of vanishing density): theta(r) max_t exp(-min(eta2_t,
+    kappa_t / beta) (r - beta s0_t)) = 2^-80, by bisection as the library does."""
+    beta = (16 * math.pi / 3) ** (1 / 3) / math.sqrt(2)
+    par = list(pes.params.values())
+    rcut = max(beta * p["s0"] for p in par) * math.sqrt(3)
+    acut = 9.21034037197618 / (4 * rcut / (math.sqrt(3) + 2) - rcut)
+
+    def w(r):
+        return max(math.exp(-min(p["eta2"], p["kappa"] / beta) * (r - beta * p["s0"])) for p in par) / \
+            (1 + math.exp(acut * (r - rcut)))
+
+    lo, hi = rcut, 1.5 * rcut
+    if w(hi) > 2.0 ** -80:
+        return hi
+    for _ in range(100):
+        mid = 0.5 * (lo + hi)
+        if w(mid) > 2.0 ** -80:
+            lo = mid
+        else:
+            hi = mid
+    return hi
 
 
 def rpmd_variant(ctx, args):
@@ -230,7 +416,7 @@ def uneven_variant(ctx, pes, args):
     the slab classical (mixed bead model), FP64 EMT with every atom carrying
     the beads (the reference's force_beads), and the NENE surrogate (every
     atom carrying the beads) with the EMT fallback and the EMT judge on a
-    rotating 1 % of the replicas every 10 steps."""
+    rotating 1 % of the replicas every step (1 % of frames)."""
     from md_tian2_amd import geometry, mdt, pesfile, surrogate
 
     g = np.random.default_rng(5150)
@@ -281,7 +467,7 @@ def uneven_variant(ctx, pes, args):
         return out
     ctx.set_nn(model, systems[0].type_names)
     b = mdt.Batch(ctx, systems, dt=0.1, tsurf=tsurf, tproj=tsurf, seed=31)
-    b.set_pes(1, judge_interval=10)
+    b.set_pes(1, judge_interval=1)
 
     def nn_stats(b):
         rmse, ncomp, mx, nfb = b.nn_stats()
@@ -316,6 +502,8 @@ def main():
     ap.add_argument("--uneven-nn-steps", type=int, default=10)
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c1-steps", type=int, default=10000)
+    ap.add_argument("--no-c1", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
@@ -365,18 +553,24 @@ def main():
 
     # algorithmic work from the kernel's own geometry at the end of timing
     cut = emt_cutoff(pes)
+    rnear = near_radius(pes)
     sample = range(0, len(systems), max(1, len(systems) // 256))
     pairs = np.mean([count_pairs(state[t]["r"][:, 0, :], systems[t].simbox, cut) for t in sample])
+    pairs_near = np.mean([count_pairs(state[t]["r"][:, 0, :], systems[t].simbox, rnear) for t in sample])
     s0 = systems[0]
     n_atoms = s0.natoms
     mobile = ~s0.fixed[:, 0, 0]
     n_lan = int(((s0.idx == 1) & mobile).sum())
     n_ver = int(((s0.idx == 0) & mobile).sum())
-    flop_traj = FLOP_PAIR * pairs + FLOP_ATOM * n_atoms + FLOP_LAN * n_lan + FLOP_VER * n_ver
+    # k_replica's own work (calc_force); the integrator's runs in k_traj
+    flop_force = FLOP_PAIR * pairs + FLOP_ATOM * n_atoms
+    flop_traj = flop_force + FLOP_LAN * n_lan + FLOP_VER * n_ver
     launch_s = force_ms / 1e3 / max(1, nlaunch)
-    achieved = flop_traj * len(systems) / launch_s / 1e12
+    achieved = flop_force * len(systems) / launch_s / 1e12
     value = len(systems) * args.steps / (ms_total / 1e3)
     f32_launch_s = f32_prof["force_ms"] / 1e3 / max(1, f32_prof["nlaunch"])
+    c1 = None if args.no_c1 else c1_variant(ctx, pes, args)
+    ctx.set_emt(cols, pesm)
     nn = None if args.no_nn else nn_variant(ctx, pes, args)
     rpmd = None if args.no_rpmd else rpmd_variant(ctx, args)
     uneven = None if args.no_uneven else uneven_variant(ctx, pes, args)
@@ -399,13 +593,15 @@ def main():
                         "+ H Verlet, Einc U(0.5,5) eV, polar U(0,60), azimuth U(0,360), dt 0.1 fs, FP64",
             "trajectories": len(systems), "natoms": n_atoms, "nbeads": 1, "precision": "fp64",
             "threads_per_atom": tpa, "pairs_per_traj": float(pairs),
+            "pairs_evaluated_per_traj": float(pairs_near), "near_radius_A": rnear, "cutoff_A": cut,
         },
         "roofline": {
-            "bound": "mfma", "pipe": "fp64 vector ALU (MI355X FP64 vector peak = FP64 matrix peak)",
+            "bound": "valu", "pipe": "FP64 vector ALU (no MFMA in this kernel; MI355X FP64 vector peak 78.6 TFLOP/s)",
             "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS, "traffic": read_traffic(len(systems)),
             "kernel": "k_replica (calc_force: min-image lists + EMT density + forces)",
-            "flop_per_traj_step": flop_traj, "avg_launch_ms": launch_s * 1e3,
+            "flop_per_traj_launch": flop_force, "avg_launch_ms": launch_s * 1e3,
+            "counters": read_counters(len(systems)),
         },
         "variants": {
             "fp64_hipgraph": {
@@ -420,6 +616,7 @@ def main():
                 "achieved_tflops": flop_traj * len(systems) / f32_launch_s / 1e12,
                 "error_bound": "<= 1e-6 eV/atom, <= 1e-5 eV/A vs the FP64 oracle (tests/test_gpu_precision.py)",
             },
+            "c1_h_au_3x3x4": c1,
             "nn_surrogate": nn,
             "rpmd_h_pt": rpmd,
             "uneven_h_au": uneven,
@@ -428,14 +625,13 @@ def main():
     if nn and nn.get("value"):
         out["variants"]["nn_surrogate"]["vs_fp64_emt_per_traj_step"] = nn["value"] / value
     if not args.no_cpu:
-        v, procs, per, single = cpu_baseline(args.cpu_seconds)
-        out["cpu_baseline"] = {
-            "value": v, "unit": "trajectory-steps/s", "cores": procs, "kind": "port",
-            "sample": f"{procs} processes x 1 trajectory x {per} steps of the same H/Au 6x6x4 "
-                      f"workload (oracle: serial reference-algorithm restatement, dense N^2 arrays)",
-            "single_core": single,
-        }
-        out["gpu_over_cpu"] = value / v
+        cb = cpu_baseline(args.cpu_seconds, c1_steps=args.c1_steps)
+        out["cpu_baseline"] = cb
+        out["gpu_over_cpu"] = value / cb["value"]
+        out["gpu_over_cpu_single_core"] = value / cb["single_core"]
+        if c1 is not None:
+            c1["gpu_over_cpu"] = c1["value"] / cb["c1"]["value"]
+            c1["gpu_over_cpu_single_core"] = c1["value"] / cb["c1"]["single_core"]
     ctx.close()
     print(json.dumps(out), flush=True)
 

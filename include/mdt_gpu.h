@@ -189,13 +189,24 @@ int mdt_batch_tune(mdt_batch *b, int tpa, int nbcap, int *tpa_chosen);
  * (src/pes_emt_mod.f90:646-763, 793-842, 871-972), which the reference
  * evaluates in FP64:
  *   mode 0: FP64 throughout (the reference's arithmetic; default);
- *   mode 1: the cut-off function, the sigma / pair-potential exponentials
- *           and their products in FP32, with r, the exponent arguments, the
- *           per-atom embedding terms and all accumulations (sigma, energy,
- *           forces) in FP64 or exact fixed point.  Integration, LDFA and the
- *           normal-mode transform are FP64 in both modes.
+ *   mode 1: FP32 pair terms where the error stays within 1e-6 eV/atom and
+ *           1e-5 eV/A of FP64, with r, the exponent arguments, the per-atom
+ *           embedding terms and all accumulations (sigma, energy, forces) in
+ *           FP64 or exact fixed point.  When every EMT type's embedding
+ *           stiffness 6 V0 kappa / (beta eta2) is at most 20 eV (H 1.25, Au
+ *           13.2 eV) the cut-off function, the sigma / pair-potential
+ *           exponentials and their products are FP32 (arithmetic 1); otherwise
+ *           (Pt: 129 eV, where FP32 sigma / V terms err by 6e-6 eV/atom and
+ *           8e-5 eV/A) only the cut-off exponential and theta are FP32
+ *           (arithmetic 2).  Integration, LDFA and the normal-mode transform
+ *           are FP64 in both modes.
  */
 int mdt_batch_set_precision(mdt_batch *b, int mode);
+
+/* The pair-term arithmetic the next force launch uses with the context's
+ * current EMT parameters: 0 FP64, 1 FP32 pair terms, 2 FP32 cut-off
+ * exponential and theta only (see mdt_batch_set_precision). */
+int mdt_batch_precision_arith(mdt_batch *b, int *arith);
 
 /*
  * Bead model of the trajectories with nbeads > 1.  The reference gives every

@@ -155,7 +155,8 @@ struct mdt_batch {
     int profile = 0;                // record events around every force launch
     std::vector<hipEvent_t> evf;    // start/stop pairs, one per step
     double last_force_ms = 0.0;     // summed force-kernel time of the last call
-    int precision = 0;              // 0: FP64; 1: FP32 pair terms, FP64 accumulation
+    int precision = 0;              // mode 0: FP64; 1: FP32 pair terms where the EMT types allow, FP64 accumulation
+    int graph_arith = -1;           // pair-term arithmetic the captured graph launches
     // hipGraph replay of the step loop: a chunk of graph_chunk steps is
     // captured once per launch configuration; the step counter lives on the
     // device (d_step) and the chunk's last node advances it
@@ -203,12 +204,17 @@ struct mdt_batch {
     int no_com = 0;  // mdt_batch_set_com_removal(b, 0)
     double *d_est = nullptr;  // MDT_NEST per trajectory
 
+    // k_replica's pair-term arithmetic: mode 1 is full FP32 pair terms when
+    // the EMT types' stiffness allows (EmtConst::fp32_ok), else only the
+    // cut-off exponential in FP32
+    int arith() const { return precision == 1 ? (ctx->ec.fp32_ok ? 1 : 2) : 0; }
+
     BatchView view() const {
         BatchView v{};
         v.ntraj = ntraj;
         v.nrep = nrep;
         v.nbcap = 0;  // per size class (launch_replicas)
-        v.precision = precision;
+        v.precision = arith();
         v.rep_traj = d_rep_traj;
         v.rep_bead = d_rep_bead;
         v.traj_N = d_N;
@@ -304,6 +310,9 @@ static int build_emt_const(int ntypes, const double *eta2, const double *n0, con
         ec.densc[t] = 0.5 / MDT_BOHR2ANG - ec.betaeta2[t];
         ec.owner[t] = std::find(emt.begin(), emt.end(), t) != emt.end() ? 1 : 0;
     }
+    ec.fp32_ok = 1;
+    for (int t : emt)
+        if (6.0 * v0[t] * kappa[t] / (kBeta * eta2[t]) > MDT_FP32_STIFF_MAX) ec.fp32_ok = 0;
     double rcut;
     if (emt.size() == 1) {
         rcut = ec.betas0[emt[0]] * sqrt3;  // :295
@@ -316,6 +325,24 @@ static int build_emt_const(int ntypes, const double *eta2, const double *n0, con
     ec.acut = acut;
     ec.cutr = 1.5 * rcut;
     ec.cut2hi = ec.cutr * ec.cutr * (1.0 + 1e-12);
+    // near radius: theta(r) max_t exp(-min(eta2_t, kappa_t / beta) (r - beta s0_t)) <= 2^-80
+    auto weight = [&](double r) {
+        double w = 0.0;
+        for (int t : emt)
+            w = std::max(w, std::exp(-std::min(eta2[t], kappa[t] / kBeta) * (r - kBeta * s0[t])));
+        return w / (1.0 + std::exp(acut * (r - rcut)));
+    };
+    const double wmax = std::ldexp(1.0, -80);
+    if (weight(ec.cutr) > wmax) {
+        ec.rnear = ec.cutr;  // no far pairs
+    } else {
+        double lo = rcut, hi = ec.cutr;
+        for (int it = 0; it < 100; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            (weight(mid) > wmax ? lo : hi) = mid;
+        }
+        ec.rnear = hi;
+    }
     if (emt.size() == 1) {
         const int e = emt[0];
         double ig1, ig2;
@@ -901,7 +928,7 @@ static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap) {
         if (rc) return rc;
     }
     if (skin) {
-        int rc = dalloc(&c.d_ref, 3 * (size_t)c.nmax * c.nrep) | dalloc(&c.d_kcnt, (size_t)c.nmax * c.nrep) |
+        int rc = dalloc(&c.d_ref, 3 * (size_t)c.nmax * c.nrep) | dalloc(&c.d_kcnt, 2 * (size_t)c.nmax * c.nrep) |
                  dalloc(&c.d_valid, (size_t)c.nrep);
         if (rc) return rc;
         HIPCHK(hipMemset(c.d_valid, 0, sizeof(int) * c.nrep));
@@ -1001,7 +1028,7 @@ static int configure_mixed(mdt_batch *b, int tpa, int nbcap) {
     b->mix_ns = ns;
     int rc = dalloc(&b->d_mix, nm) | dalloc(&b->d_mix_nbr, (size_t)b->mix_stride * nm) |
              dalloc(&b->d_mix_ref, 3 * (size_t)ns * nm) | dalloc(&b->d_mix_fold, 3 * (size_t)ns * nm) |
-             dalloc(&b->d_mix_cnt, (size_t)ns * nm) | dalloc(&b->d_mix_valid, nm);
+             dalloc(&b->d_mix_cnt, 2 * (size_t)ns * nm) | dalloc(&b->d_mix_valid, nm);
     if (rc) return rc;
     HIPCHK(hipMemset(b->d_mix_valid, 0, sizeof(int) * nm));
     HIPCHK(hipMemcpy(b->d_mix, b->mix_traj.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
@@ -1086,7 +1113,7 @@ static int launch_mixed(mdt_batch *b, const BatchView &base, int tpa, int add_fi
         v.nbcap = b->mix_nbcap;
         v.mix_ref = b->d_mix_ref + off * 3 * b->mix_ns;
         v.mix_fold = b->d_mix_fold + off * 3 * b->mix_ns;
-        v.mix_cnt = b->d_mix_cnt + off * b->mix_ns;
+        v.mix_cnt = b->d_mix_cnt + off * 2 * b->mix_ns;  // near and far counts per slot
         v.mix_valid = b->d_mix_valid + off;
         hipStream_t s2 = st;
         if (fk) {
@@ -1409,7 +1436,11 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
     const int S = (b->profile || (b->pes == 1 && b->judge_interval > 0)) ? 0 : b->graph_chunk;
     int replays = 0;
     if (S > 0 && nsteps >= S) {
-        if (!b->gexec && (rc = capture_chunk(b))) return rc;
+        if (b->gexec && b->graph_arith != b->arith()) drop_graph(b);  // EMT parameters changed the arithmetic
+        if (!b->gexec) {
+            if ((rc = capture_chunk(b))) return rc;
+            b->graph_arith = b->arith();
+        }
         HIPCHK(mdt_launch_advance(b->d_step, b->step, 1, c->stream));
         replays = nsteps / S;
         for (int k = 0; k < replays; ++k) HIPCHK(hipGraphLaunch(b->gexec, c->stream));
@@ -1450,6 +1481,13 @@ int mdt_batch_set_precision(mdt_batch *b, int mode) {
     HIPCHK(hipSetDevice(b->ctx->device));
     if (mode != b->precision) drop_graph(b);
     b->precision = mode;
+    return 0;
+}
+
+int mdt_batch_precision_arith(mdt_batch *b, int *arith) {
+    if (!b || !arith) return fail(MDT_EINVAL, "NULL argument");
+    if (!b->ctx->have_params) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
+    *arith = b->arith();
     return 0;
 }
 

@@ -39,14 +39,39 @@
 #define MDT_ST_BAD_PROP 4
 #define MDT_ST_FIXED_OVERFLOW 8
 
+// Precision mode 1 evaluates the sigma and pair-potential exponentials in
+// FP32 only when every EMT type's embedding stiffness B = 6 V0 kappa /
+// (beta eta2) [eV] -- the scale of dE/d(ln sigma) at s = 0, which turns a
+// relative rounding of sigma or of the pair sum into an energy error -- is
+// at most this bound: H 1.25, Au 13.2 eV (FP32 terms give <= 3.3e-7
+// eV/atom, 5.5e-6 eV/A); Pt has 129 eV (6e-6 eV/atom, 7.6e-5 eV/A, outside
+// 1e-6 eV/atom and 1e-5 eV/A), so there only the cut-off exponential and
+// theta are FP32.
+#define MDT_FP32_STIFF_MAX 20.0
+
+// An atom whose sigma~ from its near pairs is below this bound is "tiny":
+// its far pairs (and those of its partners with it) are evaluated too.  For
+// sigma~ >= 1e-3, |G| = |dE/dsigma~| <= 1.4e5 eV (the embedding terms give
+// |G sigma~| <= 140 eV), so a far pair's force on either atom is at most
+// |G| csig (eta + acut) 2^-80 < 1e-16 eV/A.  The reference's far-H force
+// (ds/dr divided by the 1e-30-clamped sigma~) depends on the relative size of
+// those tiny terms, which this keeps exact.
+#define MDT_SIG_TINY 1.0e-3
+
 // EMT constants derived on the host exactly as src/pes_emt_mod.f90:533-592
 // (2 species) or :267-307 (1 species), flattened into per-type and
 // per-(type a, neighbour type n) tables for a gather formulation.
 struct EmtConst {
     int ntypes;
-    int pad_;
+    int fp32_ok;  // every EMT type's embedding stiffness 6 V0 kappa / (beta eta2) <= MDT_FP32_STIFF_MAX
     double rcut, acut, cutr;  // cutr = 1.5 * rcut (the reference's hard cut)
     double cut2hi;            // neighbour-list bound on r^2 (cutr^2 widened by 1e-12 rel)
+    // Near radius: beyond it every pair weight theta(r) exp(-a (r - beta s0)),
+    // a in {eta2, kappa/beta} of either type, is below 2^-80.  Pairs between
+    // rnear and cutr are listed apart and evaluated only when one of the two
+    // atoms has sigma~ < MDT_SIG_TINY; elsewhere they change no force by more
+    // than 1e-16 eV/A and no sigma or energy by more than 1e-18 relative.
+    double rnear;
     double eta2[MDT_MAXT], betas0[MDT_MAXT], kappadbeta[MDT_MAXT], betaeta2[MDT_MAXT];
     double e0[MDT_MAXT], lambda[MDT_MAXT], v0[MDT_MAXT], kappa[MDT_MAXT], n0[MDT_MAXT];
     double densc[MDT_MAXT];            // 0.5/bohr2ang - beta*eta2
@@ -71,7 +96,7 @@ struct PropConst {
 struct BatchView {
     int ntraj, nrep;
     int nbcap;      // neighbour-list capacity per atom
-    int precision;  // 0: FP64 pair terms; 1: FP32 pair terms, FP64 accumulation
+    int precision;  // pair-term arithmetic of k_replica: 0 FP64; 1 FP32 pair terms; 2 FP32 cut-off exponential only
     const int *rep_traj, *rep_bead;
     const int *traj_N, *traj_P;
     const long long *traj_soff, *traj_aoff;

@@ -174,6 +174,26 @@ __device__ void load_tables(LdsTables &T, const EmtConst *__restrict__ ec, const
 #define MDT_F_SCALE 4398046511104.0          // 2^42: force up to 2.1e6 eV/A, step 2.3e-13
 #define MDT_F_ISCALE (1.0 / 4398046511104.0)
 
+// Round-to-nearest-even of y to int64, returned with the rounded value as a
+// double in *r.  For |y| < 2^51 adding 1.5 * 2^52 leaves rint(y) in the low
+// mantissa bits (the sum lies in [2^52, 2^53), where the spacing is 1 and the
+// default rounding is to nearest even), so the integer is the difference of
+// two bit patterns: three instructions instead of the f64 -> i64 expansion.
+// fix_round_small requires |y| < 2^51; fix_round votes across the wave and
+// takes rint and the conversion when any lane's |y| (up to the callers' 9e18
+// bound) is larger, so the common case issues no second path.
+#define MDT_RND_MAGIC 6755399441055744.0  // 1.5 * 2^52
+__device__ __forceinline__ long long fix_round_small(double y, double *r) {
+    const double t = y + MDT_RND_MAGIC;
+    *r = t - MDT_RND_MAGIC;
+    return __double_as_longlong(t) - __double_as_longlong(MDT_RND_MAGIC);
+}
+__device__ __forceinline__ long long fix_round(double y, double *r) {
+    if (__all(fabs(y) < 2251799813685248.0)) return fix_round_small(y, r);  // 2^51, wave-uniform branch
+    *r = rint(y);
+    return (long long)*r;
+}
+
 // sigma contributions span ~1e3 (close contact) down to ~1e-35 (pairs at the
 // cut-off) and an isolated atom's sigma sits near the reference's 1e-30
 // clamp, where G = dE/dsigma ~ 1/sigma: a sum with relative precision at
@@ -187,13 +207,14 @@ __device__ __forceinline__ void sig_add(unsigned long long *hi, unsigned long lo
         *bad |= MDT_ST_FIXED_OVERFLOW;
         return;
     }
-    const double h = rint(y);
-    const double y1 = (y - h) * MDT_W48;
-    const double m = rint(y1);
-    const double l = rint((y1 - m) * MDT_W48);
-    atomicAdd(hi, (unsigned long long)(long long)h);
-    if (m != 0.0) atomicAdd(mid, (unsigned long long)(long long)m);
-    if (l != 0.0) atomicAdd(lo, (unsigned long long)(long long)l);
+    double h, m, l;
+    const long long ih = fix_round(y, &h);
+    const double y1 = (y - h) * MDT_W48;  // |y1| <= 2^47
+    const long long im = fix_round_small(y1, &m);
+    const long long il = fix_round_small((y1 - m) * MDT_W48, &l);  // |.| <= 2^47
+    atomicAdd(hi, (unsigned long long)ih);
+    if (im != 0) atomicAdd(mid, (unsigned long long)im);
+    if (il != 0) atomicAdd(lo, (unsigned long long)il);
 }
 
 __device__ __forceinline__ double sig_value(unsigned long long hi, unsigned long long mid, unsigned long long lo) {
@@ -207,7 +228,8 @@ __device__ __forceinline__ void fx_add(unsigned long long *acc, double x, double
         *bad |= MDT_ST_FIXED_OVERFLOW;
         return;
     }
-    atomicAdd(acc, (unsigned long long)(long long)rint(y));
+    double r;
+    atomicAdd(acc, (unsigned long long)fix_round(y, &r));
 }
 
 // s, embedded density, E_coh + vref and G = dE/dsigma of an atom of type tk
@@ -249,14 +271,20 @@ __device__ __forceinline__ double tie_image(double ia, double ib, double ic, dou
 // ~1e-10 eV/A, since that atom's force scales with sigma itself.
 __device__ __forceinline__ bool owns(int k, int n) { return (k < n) != (((k + n) & 1) != 0); }
 
-// F32 (precision mode 1): the pair terms -- cut-off function, the sigma and
-// pair-potential exponentials and their products -- are evaluated in FP32;
-// the geometry (differences, r, the exponent arguments r - rcut and
-// r - beta s0, which carry factors up to acut ~ 26/A), the per-atom
-// embedding terms, G and every accumulation stay FP64 / fixed point.
+// PREC (arithmetic of the pair terms):
+//   0: FP64 throughout;
+//   1: the cut-off function, the sigma and pair-potential exponentials and
+//      their products in FP32;
+//   2: only the cut-off exponential exp(acut (r - rcut)) and theta in FP32,
+//      the sigma and pair-potential exponentials in FP64 (for metals whose
+//      embedding stiffness makes FP32 sigma / V terms miss 1e-6 eV/atom and
+//      1e-5 eV/A, e.g. Pt with V0 = 21.7 eV).
+// In every mode the geometry (differences, r, the exponent arguments
+// r - rcut and r - beta s0, which carry factors up to acut ~ 26/A), the
+// per-atom embedding terms, G and every accumulation stay FP64 / fixed point.
 // JUDGE: FP64 EMT forces of the selected replicas compared with
 // bv.judge_f (the network's judge); nothing else is written.
-template <int TPA, bool NBR_LDS, bool F32, bool JUDGE>
+template <int TPA, bool NBR_LDS, int PREC, bool JUDGE>
 __global__ void __launch_bounds__(512, 4)
     k_replica(const EmtConst *__restrict__ ecg, const PropConst *__restrict__ pcg, BatchView bv, int add_fin) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -283,10 +311,12 @@ __global__ void __launch_bounds__(512, 4)
     load_cell(bv, t, cell);
     load_tables(T, ecg, pcg, cell, lane, nthr);
 
-    // LDS: px py pz | three words A0 A1 A2 | G | E | cnt | nbr | typ, where the
-    // words hold the fractional coordinates in the scan, the sigma fixed
-    // point (hi, mid, lo) in pass A and the force fixed point (x, y, z) in
-    // pass B.
+    // LDS: px py pz | three words A0 A1 A2 | G | E | cnt | cntf | nbr | typ |
+    // tiny, where the words hold the fractional coordinates in the scan, the
+    // sigma fixed point (hi, mid, lo) in pass A and the force fixed point
+    // (x, y, z) in pass B.  An atom's list holds its near entries (r <=
+    // rnear) from the front, cnt of them, and its far entries (rnear < r <=
+    // cutr) from the back, cntf of them.
     double *px = (double *)smem_raw;
     double *py = px + N, *pz = py + N;
     double *sx = pz + N, *sy = sx + N, *sz = sy + N;
@@ -295,8 +325,11 @@ __global__ void __launch_bounds__(512, 4)
     unsigned long long *fxa = shi, *fya = smid, *fza = slo;
     double *G = sz + N, *E = G + N;
     uint16_t *cnt = (uint16_t *)(E + N);
-    uint16_t *nbr = NBR_LDS ? cnt + N : bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
-    uint8_t *typ = (uint8_t *)(cnt + N + (NBR_LDS ? (size_t)N * cap : 0));
+    uint16_t *cntf = cnt + N;
+    uint16_t *nbr = NBR_LDS ? cntf + N : bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
+    uint8_t *typ = (uint8_t *)(cntf + N + (NBR_LDS ? (size_t)N * cap : 0));
+    uint8_t *tiny = typ + N;
+    __shared__ int s_anytiny;
     int bad_local = 0;
     // Verlet skin (lists in HBM): the lists of this block slot are built with
     // cut-off + skin and kept while no atom moved more than skin / 2 since
@@ -393,15 +426,21 @@ __global__ void __launch_bounds__(512, 4)
     }
     const double cutl = keep ? (cutr + skin) * (cutr + skin) : cut2;
     const float cut2f = (float)(cutl * (1.0 + 1e-4));
-    uint16_t *kcnt = keep ? bv.rl_cnt + (long long)blockIdx.x * bv.rl_ns : nullptr;
+    // near / far split: a kept list classifies with rnear + skin, so a pair
+    // now within rnear was a near entry at the build
+    const double rn = keep ? ecg->rnear + skin : ecg->rnear;
+    const float near2f = (float)(rn * rn * (1.0 + 1e-4));
+    uint16_t *kcnt = keep ? bv.rl_cnt + (long long)blockIdx.x * 2 * bv.rl_ns : nullptr;
+    uint16_t *kcntf = keep ? kcnt + bv.rl_ns : nullptr;
     if (reuse) {  // kept list: the image of every entry from the current positions
         for (int k0 = 0; k0 < N; k0 += apb) {
             const int k = k0 + lane / TPA;
             const int sub = lane % TPA;
             if (k >= N) continue;
-            const int c = kcnt[k];
+            const int c = kcnt[k], cf = kcntf[k];
             const float4 pk = f4[k];
-            for (int e = sub; e < c; e += TPA) {
+            for (int ee = sub; ee < c + cf; ee += TPA) {
+                const int e = ee < c ? ee : cap - cf + (ee - c);
                 const unsigned q = nbr[(size_t)k * cap + e];
                 const int n = q & MDT_NBR_IDX_MASK;
                 const float4 pn = f4[n];
@@ -420,7 +459,10 @@ __global__ void __launch_bounds__(512, 4)
                 const int code = ((int)m0f + 1) + 3 * ((int)m1f + 1) + 9 * ((int)m2f + 1);
                 nbr[(size_t)k * cap + e] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
             }
-            if (sub == 0) cnt[k] = (uint16_t)c;
+            if (sub == 0) {
+                cnt[k] = (uint16_t)c;
+                cntf[k] = (uint16_t)cf;
+            }
         }
     }
     // only the owned candidates are visited, in increasing index: below k
@@ -435,11 +477,11 @@ __global__ void __launch_bounds__(512, 4)
         const int tk = act ? typ[k] : 0;
         const float4 pk = act ? f4[k] : make_float4(0.f, 0.f, 0.f, 0.f);
         const int a_lo = (k + 1) >> 1;
-        int c = 0;
+        int c = 0, cf = 0;
         for (int m0 = 0; m0 < mmax; m0 += TPA) {
             const int mi = m0 + sub;
             const int n = mi < a_lo ? ((k + 1) & 1) + 2 * mi : k + 2 + 2 * (mi - a_lo);
-            bool in = false;
+            bool in = false, nr = false;
             int code = 0;
             if (act && n < N) {
                 const float4 pn = f4[n];
@@ -462,22 +504,29 @@ __global__ void __launch_bounds__(512, 4)
                     }
                     const float r2 = d0 * (g00 * d0 + g01 * d1 + g02 * d2) + d1 * (g11 * d1 + g12 * d2) + g22 * d2 * d2;
                     in = r2 <= cut2f;
+                    nr = r2 <= near2f;
                     code = ((int)m0f + 1) + 3 * ((int)m1f + 1) + 9 * ((int)m2f + 1);
                 }
             }
-            const unsigned long long m = __ballot(in) & gmask;
-            if (in) {
-                const int pos = c + __popcll(m & below);
+            const unsigned long long mall = __ballot(in) & gmask;
+            const unsigned long long mn = __ballot(in && nr) & gmask;
+            const unsigned long long mf = mall & ~mn;
+            if (in) {  // near entries from the front, far entries from the back
+                const int pos = nr ? c + __popcll(mn & below) : cf + __popcll(mf & below);
                 if (pos < cap)
-                    nbr[(size_t)k * cap + pos] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
-                else
-                    bad_local |= MDT_ST_NBR_OVERFLOW;
+                    nbr[(size_t)k * cap + (nr ? pos : cap - 1 - pos)] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
             }
-            c += __popcll(m);
+            c += __popcll(mn);
+            cf += __popcll(mf);
         }
         if (act && sub == 0) {
+            if (c + cf > cap) bad_local |= MDT_ST_NBR_OVERFLOW;
             cnt[k] = (uint16_t)(c < cap ? c : cap);
-            if (keep) kcnt[k] = cnt[k];
+            cntf[k] = (uint16_t)(cf < cap - cnt[k] ? cf : cap - cnt[k]);
+            if (keep) {
+                kcnt[k] = cnt[k];
+                kcntf[k] = cntf[k];
+            }
         }
     }
     __syncthreads();
@@ -490,6 +539,42 @@ __global__ void __launch_bounds__(512, 4)
     __syncthreads();
 
     // ---- phase 2: pass A, sigma ----
+    // one listed pair (entry e of atom k's list): sigma_k += own in registers,
+    // sigma_n += other into the fixed point; far entries only with a tiny atom
+    auto pair_a = [&](int k, int tk, double xk, double yk, double zk, int e, bool far, double &sig) {
+        const unsigned q = nbr[(size_t)k * cap + e];
+        const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+        if (far && !(tiny[k] | tiny[n])) return;
+        const int tn = typ[n];
+        const double dx = (xk - px[n]) - T.shift[3 * code];
+        const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+        const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r = r2 * mdt_rsqrt(r2);
+        if (r > cutr) return;
+        double own, other;
+        if constexpr (PREC == 1) {
+            const float e1 = mdt_exp2f(acutl2f * (float)(r - rcut));
+            const float th = mdt_rcpf(1.0f + e1);
+            const float fsn = th * mdt_exp2f(-T.f_eta2l2[tn] * (float)(r - T.bs0[tn]));
+            const float fsk = (tn == tk) ? fsn : th * mdt_exp2f(-T.f_eta2l2[tk] * (float)(r - T.bs0[tk]));
+            own = (double)(T.f_csig[tk * MDT_MAXT + tn] * fsn);
+            other = (double)(T.f_csig[tn * MDT_MAXT + tk] * fsk);
+        } else {
+            double th;
+            if constexpr (PREC == 2) {
+                th = (double)mdt_rcpf(1.0f + mdt_exp2f(acutl2f * (float)(r - rcut)));
+            } else {
+                th = mdt_rcp(1 + mdt_exp(acut * (r - rcut), T.exptab));
+            }
+            const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+            const double fsk = (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+            own = T.csig[tk * MDT_MAXT + tn] * fsn;
+            other = T.csig[tn * MDT_MAXT + tk] * fsk;
+        }
+        sig += own;
+        sig_add(&shi[n], &smid[n], &slo[n], other, &bad_local);
+    };
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
@@ -499,42 +584,39 @@ __global__ void __launch_bounds__(512, 4)
             const int tk = typ[k];
             const double xk = px[k], yk = py[k], zk = pz[k];
             const int ck = cnt[k];
-            for (int e = sub; e < ck; e += TPA) {
-                const unsigned q = nbr[(size_t)k * cap + e];
-                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
-                const int tn = typ[n];
-                const double dx = (xk - px[n]) - T.shift[3 * code];
-                const double dy = (yk - py[n]) - T.shift[3 * code + 1];
-                const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                const double r = r2 * mdt_rsqrt(r2);
-                if (r > cutr) continue;
-                double own, other;
-                if constexpr (F32) {
-                    const float e1 = mdt_exp2f(acutl2f * (float)(r - rcut));
-                    const float th = mdt_rcpf(1.0f + e1);
-                    const float fsn = th * mdt_exp2f(-T.f_eta2l2[tn] * (float)(r - T.bs0[tn]));
-                    const float fsk =
-                        (tn == tk) ? fsn : th * mdt_exp2f(-T.f_eta2l2[tk] * (float)(r - T.bs0[tk]));
-                    own = (double)(T.f_csig[tk * MDT_MAXT + tn] * fsn);
-                    other = (double)(T.f_csig[tn * MDT_MAXT + tk] * fsk);
-                } else {
-                    const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
-                    const double th = mdt_rcp(1 + e1);
-                    const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
-                    const double fsk =
-                        (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
-                    own = T.csig[tk * MDT_MAXT + tn] * fsn;
-                    other = T.csig[tn * MDT_MAXT + tk] * fsk;
-                }
-                sig += own;
-                sig_add(&shi[n], &smid[n], &slo[n], other, &bad_local);
-            }
+            for (int e = sub; e < ck; e += TPA) pair_a(k, tk, xk, yk, zk, e, false, sig);
         }
         sig = group_sum<TPA>(sig);
         if (act && sub == 0) G[k] = sig;  // own part; the partners' part is in shi/smid/slo
     }
+    if (lane == 0) s_anytiny = 0;
     __syncthreads();
+
+    // ---- tiny atoms: sigma~ of the near pairs below MDT_SIG_TINY; then the
+    // far pairs of every tiny atom (both ends) ----
+    for (int k = lane; k < N; k += nthr) {
+        const uint8_t ty = (T.owner[typ[k]] && G[k] + sig_value(shi[k], smid[k], slo[k]) < MDT_SIG_TINY) ? 1 : 0;
+        tiny[k] = ty;
+        if (ty) s_anytiny = 1;  // benign race: every writer stores 1
+    }
+    __syncthreads();
+    const bool anytiny = s_anytiny != 0;
+    if (anytiny) {
+        for (int k0 = 0; k0 < N; k0 += apb) {
+            const int k = k0 + lane / TPA;
+            const int sub = lane % TPA;
+            const bool act = k < N;
+            double sig = 0.0;
+            if (act) {
+                const int tk = typ[k];
+                const double xk = px[k], yk = py[k], zk = pz[k];
+                for (int e = cap - cntf[k] + sub; e < cap; e += TPA) pair_a(k, tk, xk, yk, zk, e, true, sig);
+            }
+            sig = group_sum<TPA>(sig);
+            if (act && sub == 0) G[k] += sig;
+        }
+        __syncthreads();
+    }
 
     // ---- per atom: s, dens, E_coh + vref, G = dE/dsigma ----
     for (int k = lane; k < N; k += nthr) {
@@ -557,6 +639,73 @@ __global__ void __launch_bounds__(512, 4)
     __syncthreads();
 
     // ---- phase 3: pass B, pair forces and pair energies ----
+    // one listed pair: the pair's dE/dr gives +F on the owner (registers) and
+    // -F on the partner (fixed point); the owner takes the pair energy
+    auto pair_b = [&](int k, int tk, double xk, double yk, double zk, double Gk, int e, bool far, double &fx,
+                      double &fy, double &fz, double &ep) {
+        const unsigned q = nbr[(size_t)k * cap + e];
+        const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+        if (far && !(tiny[k] | tiny[n])) return;
+        const int tn = typ[n];
+        const double dx = (xk - px[n]) - T.shift[3 * code];
+        const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+        const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double ri = mdt_rsqrt(r2);
+        const double r = r2 * ri;
+        if (r > cutr) return;
+        const int kn = tk * MDT_MAXT + tn, nk = tn * MDT_MAXT + tk;
+        double c;
+        if constexpr (PREC == 1) {
+            const float e1 = mdt_exp2f(acutl2f * (float)(r - rcut));
+            const float th = mdt_rcpf(1.0f + e1);
+            const float rt1 = acutf * e1 * th;
+            const float drn = (float)(r - T.bs0[tn]);
+            const float fsn = th * mdt_exp2f(-T.f_eta2l2[tn] * drn);
+            const float fvn = th * mdt_exp2f(-T.f_kdbl2[tn] * drn);
+            float fsk = fsn, fvk = fvn;
+            if (tn != tk) {
+                const float drk = (float)(r - T.bs0[tk]);
+                fsk = th * mdt_exp2f(-T.f_eta2l2[tk] * drk);
+                fvk = th * mdt_exp2f(-T.f_kdbl2[tk] * drk);
+            }
+            const float ta = T.f_csig[kn] * ((T.f_etad[kn] + rt1) * fsn);
+            const float tb = T.f_csig[nk] * ((T.f_etad[nk] + rt1) * fsk);
+            const float tv = 0.5f * (T.f_wv[kn] * ((T.f_kdb[tn] + rt1) * fvn) + T.f_wv[nk] * ((T.f_kdb[tk] + rt1) * fvk));
+            c = Gk * (double)ta + G[n] * (double)tb + (double)tv;
+            ep += (double)(T.f_wv[kn] * fvn + T.f_wv[nk] * fvk);
+        } else {
+            double th, rt1;
+            if constexpr (PREC == 2) {
+                const float e1 = mdt_exp2f(acutl2f * (float)(r - rcut));
+                const float thf = mdt_rcpf(1.0f + e1);
+                th = (double)thf;
+                rt1 = (double)(acutf * e1 * thf);
+            } else {
+                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
+                th = mdt_rcp(1 + e1);
+                rt1 = acut * e1 * th;
+            }
+            const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+            const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
+            double fsk = fsn, fvk = fvn;
+            if (tn != tk) {
+                fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+                fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
+            }
+            // dE/dr of this pair: density part through G, pair (V) part directly
+            c = Gk * T.csig[kn] * ((T.etad[kn] + rt1) * fsn) + G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsk) +
+                0.5 * (T.wv[kn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[nk] * ((T.kdb[tk] + rt1) * fvk));
+            ep += T.wv[kn] * fvn + T.wv[nk] * fvk;
+        }
+        const double cr = c * ri;
+        fx -= cr * dx;
+        fy -= cr * dy;
+        fz -= cr * dz;
+        fx_add(&fxa[n], cr * dx, MDT_F_SCALE, &bad_local);
+        fx_add(&fya[n], cr * dy, MDT_F_SCALE, &bad_local);
+        fx_add(&fza[n], cr * dz, MDT_F_SCALE, &bad_local);
+    };
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
@@ -567,62 +716,10 @@ __global__ void __launch_bounds__(512, 4)
             const double xk = px[k], yk = py[k], zk = pz[k];
             const double Gk = G[k];
             const int ck = cnt[k];
-            for (int e = sub; e < ck; e += TPA) {
-                const unsigned q = nbr[(size_t)k * cap + e];
-                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
-                const int tn = typ[n];
-                const double dx = (xk - px[n]) - T.shift[3 * code];
-                const double dy = (yk - py[n]) - T.shift[3 * code + 1];
-                const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                const double ri = mdt_rsqrt(r2);
-                const double r = r2 * ri;
-                if (r > cutr) continue;
-                const int kn = tk * MDT_MAXT + tn, nk = tn * MDT_MAXT + tk;
-                double c;
-                if constexpr (F32) {
-                    const float e1 = mdt_exp2f(acutl2f * (float)(r - rcut));
-                    const float th = mdt_rcpf(1.0f + e1);
-                    const float rt1 = acutf * e1 * th;
-                    const float drn = (float)(r - T.bs0[tn]);
-                    const float fsn = th * mdt_exp2f(-T.f_eta2l2[tn] * drn);
-                    const float fvn = th * mdt_exp2f(-T.f_kdbl2[tn] * drn);
-                    float fsk = fsn, fvk = fvn;
-                    if (tn != tk) {
-                        const float drk = (float)(r - T.bs0[tk]);
-                        fsk = th * mdt_exp2f(-T.f_eta2l2[tk] * drk);
-                        fvk = th * mdt_exp2f(-T.f_kdbl2[tk] * drk);
-                    }
-                    const float ta = T.f_csig[kn] * ((T.f_etad[kn] + rt1) * fsn);
-                    const float tb = T.f_csig[nk] * ((T.f_etad[nk] + rt1) * fsk);
-                    const float tv =
-                        0.5f * (T.f_wv[kn] * ((T.f_kdb[tn] + rt1) * fvn) + T.f_wv[nk] * ((T.f_kdb[tk] + rt1) * fvk));
-                    c = Gk * (double)ta + G[n] * (double)tb + (double)tv;
-                    ep += (double)(T.f_wv[kn] * fvn + T.f_wv[nk] * fvk);
-                } else {
-                    const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
-                    const double th = mdt_rcp(1 + e1);
-                    const double rt1 = acut * e1 * th;
-                    const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
-                    const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
-                    double fsk = fsn, fvk = fvn;
-                    if (tn != tk) {
-                        fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
-                        fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
-                    }
-                    // dE/dr of this pair: density part through G, pair (V) part directly
-                    c = Gk * T.csig[kn] * ((T.etad[kn] + rt1) * fsn) + G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsk) +
-                        0.5 * (T.wv[kn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[nk] * ((T.kdb[tk] + rt1) * fvk));
-                    ep += T.wv[kn] * fvn + T.wv[nk] * fvk;
-                }
-                const double cr = c * ri;
-                fx -= cr * dx;
-                fy -= cr * dy;
-                fz -= cr * dz;
-                fx_add(&fxa[n], cr * dx, MDT_F_SCALE, &bad_local);
-                fx_add(&fya[n], cr * dy, MDT_F_SCALE, &bad_local);
-                fx_add(&fza[n], cr * dz, MDT_F_SCALE, &bad_local);
-            }
+            for (int e = sub; e < ck; e += TPA) pair_b(k, tk, xk, yk, zk, Gk, e, false, fx, fy, fz, ep);
+            if (anytiny)
+                for (int e = cap - cntf[k] + sub; e < cap; e += TPA)
+                    pair_b(k, tk, xk, yk, zk, Gk, e, true, fx, fy, fz, ep);
         }
         fx = group_sum<TPA>(fx);
         fy = group_sum<TPA>(fy);
@@ -723,7 +820,11 @@ __global__ void __launch_bounds__(512, 4)
 // mode, positions advance in propagate_1 and the ring-polymer step skips
 // them; in the COM sum a classical atom counts as P identical beads.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg, BatchView bv,
+// Four waves per SIMD: the register allocation is held to 128 VGPRs (a
+// 16-byte spill); measured 66 -> 58 us per launch on 4096 H/Au 6x6x4
+// trajectories against the unbounded 133-VGPR build at three waves.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_traj(const PropConst *__restrict__ pcg, BatchView bv,
                                               const double *cjk_all, const long long *cjk_off,
                                               const uint32_t *step_base, uint32_t step_prev, uint32_t step,
                                               int flags) {
@@ -793,6 +894,13 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
         }
     __syncthreads();
     const int n_pile = s_cnt[0], n_tr = s_cnt[1], n_lan = s_cnt[2];
+    // langevin_2 of the previous step and langevin_1 of this one read the
+    // same friction (dens after ldfa), temperature and branch, so their
+    // coefficients are equal: with one atom-bead slot per lane (nab <= nthr)
+    // the lane keeps them in LDS from the first for the second
+    const bool one_slot = nab <= nthr;
+    __shared__ LanCoef s_lan[256];
+    bool have_l = false;
 
     if (flags & 1) {
         // ldfa for Langevin atoms, per bead (md_algo.f90:97)
@@ -825,6 +933,10 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
                 const double temp = MDT_KB * PT(ty) * (s_proj[ty] ? tproj : tsurf) / mass;
                 const bool exact = big[i] && temp > MDT_TOLERANCE;
                 const LanCoef L = lan_coef(bv.dens[PI_(b, i)], dt, temp, exact);
+                if (one_slot) {
+                    s_lan[lane] = L;
+                    have_l = true;
+                }
                 const double w = L.sv * sqrt(1 - L.crv * L.crv);
                 for (int c = 0; c < 3; ++c) {
                     // the deviate of this step's langevin_1, kept by propagate_1
@@ -1005,12 +1117,18 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
             const uint8_t fb = bv.fixed[PI_(b, i)];
             const double mass = bv.mass[aoff + i];
             if (algo == MDT_PROP_LANGEVIN) {
-                const double temp = MDT_KB * pb * (s_proj[ty] ? tproj : tsurf) / mass;
-                const bool exact = big[i] && temp > MDT_TOLERANCE;
-                const LanCoef L = lan_coef(bv.dens[PI_(b, i)], dt, temp, exact);
+                LanCoef L;
+                if (have_l) {
+                    L = s_lan[lane];
+                } else {
+                    const double temp = MDT_KB * pb * (s_proj[ty] ? tproj : tsurf) / mass;
+                    const bool exact = big[i] && temp > MDT_TOLERANCE;
+                    L = lan_coef(bv.dens[PI_(b, i)], dt, temp, exact);
+                }
+                double z[3];
+                mdt_normal3(pc.seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)(i * P + b), z);
                 for (int c = 0; c < 3; ++c) {
-                    const double eta =
-                        mdt_normal(pc.seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)(i * P + b), (uint32_t)c);
+                    const double eta = z[c];
                     bv.eta[XI(c, b, i)] = eta;  // langevin_2 of this step takes the same deviate
                     const double r = bv.x[XI(c, b, i)], v = bv.v[XI(c, b, i)], a = bv.a[XI(c, b, i)];
                     if (!(fb >> c & 1)) {
@@ -1031,12 +1149,12 @@ __global__ void __launch_bounds__(256) k_traj(const PropConst *__restrict__ pcg,
                     const double ibetaN = pb * MDT_KB * (s_proj[ty] ? tproj : tsurf);
                     const double sigma = sqrt(ibetaN / mass);
                     const double u = mdt_uniform(pc.seed, tid, step, MDT_STREAM_ANDERSEN_U, (uint32_t)(i * P + b), 0u);
-                    if (u < dt / pc.andersen_time)
-                        for (int c = 0; c < 3; ++c) {
-                            const double z = mdt_normal(pc.seed, tid, step, MDT_STREAM_ANDERSEN_V,
-                                                        (uint32_t)(i * P + b), (uint32_t)c);
-                            if (!(fb >> c & 1)) bv.v[XI(c, b, i)] = z * sigma + 0.0;
-                        }
+                    if (u < dt / pc.andersen_time) {
+                        double z[3];
+                        mdt_normal3(pc.seed, tid, step, MDT_STREAM_ANDERSEN_V, (uint32_t)(i * P + b), z);
+                        for (int c = 0; c < 3; ++c)
+                            if (!(fb >> c & 1)) bv.v[XI(c, b, i)] = z[c] * sigma + 0.0;
+                    }
                 }
             } else {
                 bad_local |= MDT_ST_BAD_PROP;
@@ -1213,9 +1331,12 @@ __global__ void __launch_bounds__(512, 2)
 
     // LDS: px py pz | A0 A1 A2 | G0 E0 D0 S0 | Gacc Dacc (classical atoms,
     // local index a = atom - 1) | per bead hx hy hz Eb | per
-    // bead group GH EH DH | cnt | group hcnt, hlist | typ (all atoms).  The
-    // words A0-A2 hold the fractional coordinates in the scan, the sigma
-    // fixed point in pass A and the force fixed point afterwards.
+    // bead group GH EH DH | cnt cntf | group hcnt, hlist | typ (all atoms) |
+    // tiny (classical atoms).  The words A0-A2 hold the fractional
+    // coordinates in the scan, the sigma fixed point in pass A and the force
+    // fixed point afterwards.  A classical atom's list holds its near entries
+    // from the front (cnt) and its far entries from the back (cntf), as in
+    // k_replica.
     const int ngrp = MIX_BLOCK / MIX_G;
     double *px = (double *)smem_raw, *py = px + Ns, *pz = py + Ns;
     double *sx = pz + Ns, *sy = sx + Ns, *sz = sy + Ns;
@@ -1227,9 +1348,12 @@ __global__ void __launch_bounds__(512, 2)
     double *hx = (double *)(Dacc + Ns), *hy = hx + P, *hz = hy + P, *Eb = hz + P;
     double *GH = Eb + P, *EH = GH + ngrp, *DH = EH + ngrp;
     uint16_t *cnt = (uint16_t *)(DH + ngrp);
-    uint16_t *hcnt = cnt + Ns;
+    uint16_t *cntf = cnt + Ns;
+    uint16_t *hcnt = cntf + Ns;
     uint16_t *hlist = hcnt + ngrp;
     uint8_t *typ = (uint8_t *)(hlist + (size_t)ngrp * MIX_CAPQ);
+    uint8_t *tiny = typ + N;
+    __shared__ int s_anytiny;
     uint16_t *nbr = bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
     int bad_local = 0;
     // Verlet skin for the classical lists: built with cut-off + skin, reused
@@ -1325,9 +1449,13 @@ __global__ void __launch_bounds__(512, 2)
     // both passes re-check r <= cutr in FP64, and half-cell ties take the
     // reference's image (tie_image)
     float *fx32 = (float *)G0, *fy32 = (float *)E0, *fz32 = (float *)D0;
-    uint16_t *kcnt = keep ? bv.mix_cnt + slot * bv.mix_ns : nullptr;
+    uint16_t *kcnt = keep ? bv.mix_cnt + slot * 2 * bv.mix_ns : nullptr;
+    uint16_t *kcntf = keep ? kcnt + bv.mix_ns : nullptr;
     if (reuse)
-        for (int a = lane; a < Ns; a += nthr) cnt[a] = kcnt[a];
+        for (int a = lane; a < Ns; a += nthr) {
+            cnt[a] = kcnt[a];
+            cntf[a] = kcntf[a];
+        }
     for (int a = lane; a < Ns; a += nthr) {
         fx32[a] = (float)sx[a];
         fy32[a] = (float)sy[a];
@@ -1339,6 +1467,8 @@ __global__ void __launch_bounds__(512, 2)
     for (int q = 0; q < 9; ++q) hf[q] = (float)cell.h[q];
     const double cutl = keep ? (ecg->cutr + skin) * (ecg->cutr + skin) : cut2;
     const float cut2f = (float)(cutl * (1.0 + 1e-4));
+    const double rn = keep ? ecg->rnear + skin : ecg->rnear;  // near / far split (k_replica)
+    const float near2f = (float)(rn * rn * (1.0 + 1e-4));
     const int mmax = Ns / 2 + 1;  // owned candidates only, in increasing index (as k_replica)
     for (int k0 = 0; k0 < (reuse ? 0 : Ns); k0 += apb) {
         const int k = k0 + lane / TPA;
@@ -1347,11 +1477,11 @@ __global__ void __launch_bounds__(512, 2)
         const int tk = act ? typ[k + 1] : 0;
         const float fxk = act ? fx32[k] : 0.f, fyk = act ? fy32[k] : 0.f, fzk = act ? fz32[k] : 0.f;
         const int a_lo = (k + 1) >> 1;
-        int c = 0;
+        int c = 0, cf = 0;
         for (int m0 = 0; m0 < mmax; m0 += TPA) {
             const int mi = m0 + sub;
             const int n = mi < a_lo ? ((k + 1) & 1) + 2 * mi : k + 2 + 2 * (mi - a_lo);
-            bool in = false;
+            bool in = false, nr = false;
             int code = 0;
             if (act && n < Ns) {
                 const int tn = typ[n + 1];
@@ -1374,23 +1504,31 @@ __global__ void __launch_bounds__(512, 2)
                     const float ex = hf[0] * d0 + hf[3] * d1 + hf[6] * d2;
                     const float ey = hf[1] * d0 + hf[4] * d1 + hf[7] * d2;
                     const float ez = hf[2] * d0 + hf[5] * d1 + hf[8] * d2;
-                    in = (ex * ex + ey * ey + ez * ez) <= cut2f;
+                    const float rr2 = ex * ex + ey * ey + ez * ez;
+                    in = rr2 <= cut2f;
+                    nr = rr2 <= near2f;
                     code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
                 }
             }
-            const unsigned long long m = __ballot(in) & gmask;
-            if (in) {
-                const int pos = c + __popcll(m & below);
+            const unsigned long long mall = __ballot(in) & gmask;
+            const unsigned long long mn = __ballot(in && nr) & gmask;
+            const unsigned long long mf = mall & ~mn;
+            if (in) {  // near entries from the front, far entries from the back
+                const int pos = nr ? c + __popcll(mn & below) : cf + __popcll(mf & below);
                 if (pos < cap)
-                    nbr[(size_t)k * cap + pos] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
-                else
-                    bad_local |= MDT_ST_NBR_OVERFLOW;
+                    nbr[(size_t)k * cap + (nr ? pos : cap - 1 - pos)] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
             }
-            c += __popcll(m);
+            c += __popcll(mn);
+            cf += __popcll(mf);
         }
         if (act && sub == 0) {
+            if (c + cf > cap) bad_local |= MDT_ST_NBR_OVERFLOW;
             cnt[k] = (uint16_t)(c < cap ? c : cap);
-            if (keep) kcnt[k] = cnt[k];
+            cntf[k] = (uint16_t)(cf < cap - cnt[k] ? cf : cap - cnt[k]);
+            if (keep) {
+                kcnt[k] = cnt[k];
+                kcntf[k] = cntf[k];
+            }
         }
     }
     __syncthreads();
@@ -1399,6 +1537,26 @@ __global__ void __launch_bounds__(512, 2)
     __syncthreads();
 
     // ---- 1b: sigma^0 of the classical atoms (pass A over their pairs) ----
+    // near entries; then, for the pairs with a tiny atom (sigma^0 from the
+    // near pairs below MDT_SIG_TINY), the far entries too
+    auto pair_a = [&](int k, int tk, double xk, double yk, double zk, int e, bool far, double &sig) {
+        const unsigned q = nbr[(size_t)k * cap + e];
+        const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+        if (far && !(tiny[k] | tiny[n])) return;
+        const int tn = typ[n + 1];
+        const double dx = (xk - px[n]) - T.shift[3 * code];
+        const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+        const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r = r2 * mdt_rsqrt(r2);
+        if (r > cutr) return;
+        const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
+        const double th = mdt_rcp(1 + e1);
+        const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+        const double fsk = (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+        sig += T.csig[tk * MDT_MAXT + tn] * fsn;
+        sig_add(&shi[n], &smid[n], &slo[n], T.csig[tn * MDT_MAXT + tk] * fsk, &bad_local);
+    };
     for (int k0 = 0; k0 < Ns; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
@@ -1408,28 +1566,36 @@ __global__ void __launch_bounds__(512, 2)
             const int tk = typ[k + 1];
             const double xk = px[k], yk = py[k], zk = pz[k];
             const int ck = cnt[k];
-            for (int e = sub; e < ck; e += TPA) {
-                const unsigned q = nbr[(size_t)k * cap + e];
-                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
-                const int tn = typ[n + 1];
-                const double dx = (xk - px[n]) - T.shift[3 * code];
-                const double dy = (yk - py[n]) - T.shift[3 * code + 1];
-                const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                const double r = r2 * mdt_rsqrt(r2);
-                if (r > cutr) continue;
-                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
-                const double th = mdt_rcp(1 + e1);
-                const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
-                const double fsk = (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
-                sig += T.csig[tk * MDT_MAXT + tn] * fsn;
-                sig_add(&shi[n], &smid[n], &slo[n], T.csig[tn * MDT_MAXT + tk] * fsk, &bad_local);
-            }
+            for (int e = sub; e < ck; e += TPA) pair_a(k, tk, xk, yk, zk, e, false, sig);
         }
         sig = group_sum<TPA>(sig);
         if (act && sub == 0) G0[k] = sig;
     }
+    if (lane == 0) s_anytiny = 0;
     __syncthreads();
+    for (int a = lane; a < Ns; a += nthr) {
+        const uint8_t ty = (T.owner[typ[a + 1]] && G0[a] + sig_value(shi[a], smid[a], slo[a]) < MDT_SIG_TINY) ? 1 : 0;
+        tiny[a] = ty;
+        if (ty) s_anytiny = 1;  // benign race: every writer stores 1
+    }
+    __syncthreads();
+    const bool anytiny = s_anytiny != 0;
+    if (anytiny) {
+        for (int k0 = 0; k0 < Ns; k0 += apb) {
+            const int k = k0 + lane / TPA;
+            const int sub = lane % TPA;
+            const bool act = k < Ns;
+            double sig = 0.0;
+            if (act) {
+                const int tk = typ[k + 1];
+                const double xk = px[k], yk = py[k], zk = pz[k];
+                for (int e = cap - cntf[k] + sub; e < cap; e += TPA) pair_a(k, tk, xk, yk, zk, e, true, sig);
+            }
+            sig = group_sum<TPA>(sig);
+            if (act && sub == 0) G0[k] += sig;
+        }
+        __syncthreads();
+    }
     for (int a = lane; a < Ns; a += nthr) {
         const int ta = typ[a + 1];
         double g = 0.0, e = 0.0, dn = 0.0;
@@ -1606,6 +1772,42 @@ __global__ void __launch_bounds__(512, 2)
     for (int a = lane; a < Ns; a += nthr)
         G0[a] += (double)(long long)Gacc[a] * MDT_F_ISCALE;  // G0 now holds the bead average
     __syncthreads();
+    auto pair_b = [&](int k, int tk, double xk, double yk, double zk, double Gk, int e, bool far, double &fx,
+                      double &fy, double &fz, double &ep) {
+        const unsigned q = nbr[(size_t)k * cap + e];
+        const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+        if (far && !(tiny[k] | tiny[n])) return;
+        const int tn = typ[n + 1];
+        const double dx = (xk - px[n]) - T.shift[3 * code];
+        const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+        const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double ri = mdt_rsqrt(r2);
+        const double r = r2 * ri;
+        if (r > cutr) return;
+        const int ikn = tk * MDT_MAXT + tn, ink = tn * MDT_MAXT + tk;
+        const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
+        const double th = mdt_rcp(1 + e1);
+        const double rt1 = acut * e1 * th;
+        const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
+        const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
+        double fsk = fsn, fvk = fvn;
+        if (tn != tk) {
+            fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+            fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
+        }
+        const double c = Gk * T.csig[ikn] * ((T.etad[ikn] + rt1) * fsn) +
+                         G0[n] * T.csig[ink] * ((T.etad[ink] + rt1) * fsk) +
+                         0.5 * (T.wv[ikn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[ink] * ((T.kdb[tk] + rt1) * fvk));
+        ep += T.wv[ikn] * fvn + T.wv[ink] * fvk;
+        const double cr = c * ri;
+        fx -= cr * dx;
+        fy -= cr * dy;
+        fz -= cr * dz;
+        fx_add(&fxa[n], cr * dx, MDT_F_SCALE, &bad_local);
+        fx_add(&fya[n], cr * dy, MDT_F_SCALE, &bad_local);
+        fx_add(&fza[n], cr * dz, MDT_F_SCALE, &bad_local);
+    };
     for (int k0 = 0; k0 < Ns; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
@@ -1616,40 +1818,10 @@ __global__ void __launch_bounds__(512, 2)
             const double xk = px[k], yk = py[k], zk = pz[k];
             const double Gk = G0[k];
             const int ck = cnt[k];
-            for (int e = sub; e < ck; e += TPA) {
-                const unsigned q = nbr[(size_t)k * cap + e];
-                const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
-                const int tn = typ[n + 1];
-                const double dx = (xk - px[n]) - T.shift[3 * code];
-                const double dy = (yk - py[n]) - T.shift[3 * code + 1];
-                const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
-                const double r2 = dx * dx + dy * dy + dz * dz;
-                const double ri = mdt_rsqrt(r2);
-                const double r = r2 * ri;
-                if (r > cutr) continue;
-                const int ikn = tk * MDT_MAXT + tn, ink = tn * MDT_MAXT + tk;
-                const double e1 = mdt_exp(acut * (r - rcut), T.exptab);
-                const double th = mdt_rcp(1 + e1);
-                const double rt1 = acut * e1 * th;
-                const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
-                const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
-                double fsk = fsn, fvk = fvn;
-                if (tn != tk) {
-                    fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
-                    fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
-                }
-                const double c = Gk * T.csig[ikn] * ((T.etad[ikn] + rt1) * fsn) +
-                                 G0[n] * T.csig[ink] * ((T.etad[ink] + rt1) * fsk) +
-                                 0.5 * (T.wv[ikn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[ink] * ((T.kdb[tk] + rt1) * fvk));
-                ep += T.wv[ikn] * fvn + T.wv[ink] * fvk;
-                const double cr = c * ri;
-                fx -= cr * dx;
-                fy -= cr * dy;
-                fz -= cr * dz;
-                fx_add(&fxa[n], cr * dx, MDT_F_SCALE, &bad_local);
-                fx_add(&fya[n], cr * dy, MDT_F_SCALE, &bad_local);
-                fx_add(&fza[n], cr * dz, MDT_F_SCALE, &bad_local);
-            }
+            for (int e = sub; e < ck; e += TPA) pair_b(k, tk, xk, yk, zk, Gk, e, false, fx, fy, fz, ep);
+            if (anytiny)
+                for (int e = cap - cntf[k] + sub; e < cap; e += TPA)
+                    pair_b(k, tk, xk, yk, zk, Gk, e, true, fx, fy, fz, ep);
         }
         fx = group_sum<TPA>(fx);
         fy = group_sum<TPA>(fy);
@@ -1702,12 +1874,12 @@ __global__ void __launch_bounds__(512, 2)
 // ---------------------------------------------------------------------------
 size_t mdt_replica_smem(int N, int cap, bool nbr_lds) {
     const int c = cap < N - 1 ? cap : (N > 1 ? N - 1 : 1);
-    // px py pz | A0 A1 A2 | G | E: 8 N doubles
-    return sizeof(double) * (8 * (size_t)N) + sizeof(uint16_t) * ((size_t)N + (nbr_lds ? (size_t)N * c : 0)) + N +
-           16;
+    // px py pz | A0 A1 A2 | G | E: 8 N doubles; cnt, cntf, lists; typ, tiny
+    return sizeof(double) * (8 * (size_t)N) + sizeof(uint16_t) * (2 * (size_t)N + (nbr_lds ? (size_t)N * c : 0)) +
+           2 * (size_t)N + 16;
 }
 
-template <bool L, bool F, bool J>
+template <bool L, int F, bool J>
 static hipError_t launch_replica_t(int tpa, dim3 g, dim3 blk, size_t smem, const EmtConst *ec, const PropConst *pc,
                                    const BatchView &bv, int add_fin, hipStream_t st) {
     switch (tpa) {
@@ -1727,13 +1899,16 @@ hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const E
     dim3 g(nrep), blk(block);
     const bool lds = bv.nbr_global == nullptr;
     if (bv.judge_acc)
-        return lds ? launch_replica_t<true, false, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
-                   : launch_replica_t<false, false, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+        return lds ? launch_replica_t<true, 0, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
+                   : launch_replica_t<false, 0, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
     if (bv.precision == 1)
-        return lds ? launch_replica_t<true, true, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
-                   : launch_replica_t<false, true, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
-    return lds ? launch_replica_t<true, false, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
-               : launch_replica_t<false, false, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+        return lds ? launch_replica_t<true, 1, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
+                   : launch_replica_t<false, 1, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+    if (bv.precision == 2)
+        return lds ? launch_replica_t<true, 2, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
+                   : launch_replica_t<false, 2, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
+    return lds ? launch_replica_t<true, 0, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
+               : launch_replica_t<false, 0, false>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);
 }
 
 // LDS of k_mixed for trajectories of up to nmax atoms (one projectile) and
@@ -1741,7 +1916,7 @@ hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const E
 size_t mdt_mixed_smem(int nmax, int pmax) {
     const size_t ns = (size_t)(nmax - 1), ngrp = MIX_BLOCK / MIX_G;
     return sizeof(double) * (12 * ns + 4 * (size_t)pmax + 3 * ngrp) +
-           sizeof(uint16_t) * (ns + ngrp + ngrp * MIX_CAPQ) + (size_t)nmax + 16;
+           sizeof(uint16_t) * (2 * ns + ngrp + ngrp * MIX_CAPQ) + (size_t)nmax + ns + 16;
 }
 
 int mdt_mixed_block(void) { return MIX_BLOCK; }

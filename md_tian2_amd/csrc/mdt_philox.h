@@ -7,8 +7,12 @@
 // atom, bead, component) therefore owns one draw, independent of launch
 // shape and batch composition.
 //
-// Deviates use the Box-Muller cosine branch in the reference's form
-// sqrt(-2 ln u1) cos(2 pi u2) (src/useful_things.f90:121-173); the stream
+// Deviates are Box-Muller pairs: the draw (step, slot, stream, k) gives
+// u1, u2 and the two independent standard normals sqrt(-2 ln u1) cos(2 pi
+// u2) (component 2k) and sqrt(-2 ln u1) sin(2 pi u2) (component 2k+1), so
+// an atom-bead's three components take two draws.  The reference draws a
+// fresh (u1, u2) per deviate and keeps the cosine (src/useful_things.f90:
+// 121-173); the two are equal in distribution, and the reference's stream
 // itself (Fortran random_number) is compiler-runtime specific and is not
 // reproduced.
 #pragma once
@@ -45,14 +49,40 @@ MDT_HD double mdt_u53(uint32_t a, uint32_t b) {
     return (double)m * (1.0 / 9007199254740992.0);
 }
 
-MDT_HD double mdt_normal(uint32_t seed, uint32_t traj, uint32_t step, uint32_t stream,
-                         uint32_t slot, uint32_t comp) {
-    uint32_t c[4] = {step, slot, stream, comp};
+#ifdef __HIPCC__
+// the Box-Muller pair k of (step, slot, stream): *zc = component 2k, *zs = 2k+1
+__device__ __forceinline__ void mdt_normal_pair(uint32_t seed, uint32_t traj, uint32_t step, uint32_t stream,
+                                                uint32_t slot, uint32_t k, double *zc, double *zs) {
+    uint32_t c[4] = {step, slot, stream, k};
     mdt_philox4x32_10(c, seed, traj);
     const double rnd1 = 1.0 - mdt_u53(c[0], c[1]);
     const double rnd2 = mdt_u53(c[2], c[3]);
-    return sqrt(-2 * log(rnd1)) * cos(2 * 3.14159265358979323846 * rnd2);
+    const double rho = sqrt(-2 * log(rnd1));
+    double sn, cs;
+    sincospi(2 * rnd2, &sn, &cs);
+    *zc = rho * cs;
+    *zs = rho * sn;
 }
+
+// component comp (0, 1, 2) of (step, slot, stream)
+__device__ __forceinline__ double mdt_normal(uint32_t seed, uint32_t traj, uint32_t step, uint32_t stream,
+                                            uint32_t slot, uint32_t comp) {
+    double zc, zs;
+    mdt_normal_pair(seed, traj, step, stream, slot, comp >> 1, &zc, &zs);
+    return (comp & 1) ? zs : zc;
+}
+
+// the three components of (step, slot, stream): two draws
+__device__ __forceinline__ void mdt_normal3(uint32_t seed, uint32_t traj, uint32_t step, uint32_t stream,
+                                           uint32_t slot, double z[3]) {
+    mdt_normal_pair(seed, traj, step, stream, slot, 0u, &z[0], &z[1]);
+    uint32_t c[4] = {step, slot, stream, 1u};
+    mdt_philox4x32_10(c, seed, traj);
+    const double rnd1 = 1.0 - mdt_u53(c[0], c[1]);
+    const double rnd2 = mdt_u53(c[2], c[3]);
+    z[2] = sqrt(-2 * log(rnd1)) * cospi(2 * rnd2);
+}
+#endif
 
 MDT_HD double mdt_uniform(uint32_t seed, uint32_t traj, uint32_t step, uint32_t stream,
                           uint32_t slot, uint32_t comp) {

@@ -28,7 +28,7 @@ SYMBOLS = (
     "mdt_batch_set_atoms", "mdt_batch_set_cells", "mdt_batch_set_propagation", "mdt_batch_seed",
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
-    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
+    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_precision_arith", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
     "mdt_batch_set_list_skin",
     "mdt_batch_set_observe", "mdt_batch_get_observables", "mdt_batch_estimators",
     "mdt_sample_uniform", "mdt_sample_conf_index", "mdt_apply_pip", "mdt_set_proj_incidence", "mdt_sample_maxwell",
@@ -86,6 +86,7 @@ def lib():
         L.mdt_batch_profile.argtypes = [vp, C.c_int]
         L.mdt_batch_force_timing.argtypes = [vp, dp]
         L.mdt_batch_set_precision.argtypes = [vp, C.c_int]
+        L.mdt_batch_precision_arith.argtypes = [vp, C.POINTER(C.c_int)]
         L.mdt_batch_set_bead_mode.argtypes = [vp, C.c_int]
         L.mdt_batch_set_com_removal.argtypes = [vp, C.c_int]
         L.mdt_batch_set_list_skin.argtypes = [vp, C.c_double]
@@ -301,8 +302,17 @@ class Batch:
         _check(lib().mdt_batch_compute_forces(self.h))
 
     def set_precision(self, mode):
-        """0: FP64 pair terms (reference arithmetic); 1: FP32 pair terms with FP64 accumulation."""
+        """0: FP64 pair terms (reference arithmetic); 1: FP32 pair terms where
+        the EMT types' embedding stiffness keeps them within 1e-6 eV/atom and
+        1e-5 eV/A (else only the cut-off exponential), FP64 accumulation."""
         _check(lib().mdt_batch_set_precision(self.h, int(mode)))
+
+    def precision_arith(self):
+        """Pair-term arithmetic of the next force launch: 0 FP64, 1 FP32 pair
+        terms, 2 FP32 cut-off exponential and theta only."""
+        out = C.c_int(0)
+        _check(lib().mdt_batch_precision_arith(self.h, C.byref(out)))
+        return out.value
 
     def set_bead_mode(self, mode):
         """0: every atom carries the beads; 1 (mixed): the projectile (atom 0)

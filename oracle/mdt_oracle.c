@@ -1128,14 +1128,18 @@ double orc_philox_uniform(uint64_t seed, uint32_t traj, uint32_t step,
     return u53(o[0], o[1]);
 }
 
-/* Box-Muller, cosine branch, in the form of useful_things.f90:121-131 */
+/* Box-Muller in the form of useful_things.f90:121-131, both branches: the
+ * draw (step, slot, stream, comp / 2) gives component comp's deviate from its
+ * cosine (even comp) or sine (odd comp), as the GPU path draws them; the
+ * reference keeps the cosine of a fresh pair per deviate (same distribution) */
 double orc_philox_normal(uint64_t seed, uint32_t traj, uint32_t step,
                          uint32_t stream, uint32_t slot, uint32_t comp) {
     uint32_t o[4];
-    philox_draw(seed, traj, step, stream, slot, comp, o);
+    philox_draw(seed, traj, step, stream, slot, comp >> 1, o);
     const double rnd1 = 1.0 - u53(o[0], o[1]);
     const double rnd2 = u53(o[2], o[3]);
-    return sqrt(-2 * log(rnd1)) * cos(2 * ORC_PI * rnd2) * 1.0 + 0.0;
+    const double branch = (comp & 1) ? sin(2 * ORC_PI * rnd2) : cos(2 * ORC_PI * rnd2);
+    return sqrt(-2 * log(rnd1)) * branch * 1.0 + 0.0;
 }
 
 void orc_philox_fill_normals(uint64_t seed, uint32_t traj, uint32_t step,

@@ -50,9 +50,12 @@ class TrajObs(C.Structure):
 
 
 def lib():
+    """The oracle library: oracle/liboracle_mdt.so (the checker build,
+    -ffp-contract=off), or the build named by MDT_ORACLE_LIB (bench.py's
+    CPU-baseline leg times an -O3 -march=native build of the same sources)."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle_mdt.so")
+        path = os.environ.get("MDT_ORACLE_LIB") or os.path.join(_HERE, "liboracle_mdt.so")
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run make -C oracle)")
         L = C.CDLL(path)

@@ -266,3 +266,63 @@ def test_fortran_driver_network_from_the_pes_file(tmp_path):
         dumps.append(read_dump(str(dump)))
     for k in ("r1", "v1", "epot"):
         assert np.array_equal(dumps[0][k], dumps[1][k]), k
+
+
+def test_fortran_emt_dropin_read_emt_compute_emt(tmp_path, pes_hau):
+    """The Fortran drop-in driven as md_tian2 drives its EMT plugin
+    (emt_dropin: read_pes's loop calling read_emt(atoms, unit) per block,
+    calc_force's zeroing, compute_emt(atoms, 3001) twice) on universes of
+    config 1's and config 2's geometry (H/Au(111) 3x3x4 and 6x6x4, slab
+    displaced, H in range) and a 4-bead 3x3x4 one, in one run: energies,
+    forces and densities against the oracle's compute_emt
+    (src/pes_emt_mod.f90:483-1027), the second call adding exactly the same
+    again (epot and f accumulate, src/pes_emt_mod.f90:976, 1001-1011), dens
+    set, not accumulated, and reallocated for each universe's shape."""
+    from md_tian2_amd import conf
+
+    g = np.random.default_rng(404)
+    systems = []
+    for n, p, z in ((3, 1, 1.6), (6, 1, 2.3), (3, 4, 1.1)):
+        s = geometry.h_on_slab("Au", n=n, nlayers=4, nfixed=2, rng=g, nbeads=p, z0=z)
+        s.r += g.normal(0, 0.05, s.r.shape)
+        systems.append(s)
+    paths = []
+    for k, s in enumerate(systems):
+        c = conf.Conf(s.r.copy(), s.v.copy(), True, s.fixed.copy(), (s.idx + 1).astype(np.int64),
+                      [n for n, _ in s.type_names], [bool(p) for _, p in s.type_names], s.simbox.copy(),
+                      np.linalg.inv(s.simbox), 1)
+        path = str(tmp_path / ("u%d.dat" % k))
+        conf.write_mxt(path, c)
+        paths.append(path)
+    exe = os.path.join(ROOT, "md_tian2_amd", "fortran", "emt_dropin")
+    pes_path = os.path.join(ROOT, "md_tian2_amd", "pes", "EMT-HAu.pes")
+    res = subprocess.run([exe, pes_path] + paths, cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    out = {}
+    cur = None
+    for line in res.stdout.splitlines():
+        w = line.split()
+        if not w:
+            continue
+        if w[0] == "universe":
+            cur = int(w[1])
+            N, P = int(w[2]), int(w[3])
+            out[cur] = {c: {"e": np.zeros(P), "f": np.zeros((N, P, 3)), "d": np.zeros((N, P))} for c in (1, 2)}
+        elif w[0] == "epot":
+            out[cur][int(w[1])]["e"][int(w[2]) - 1] = float(w[3])
+        elif w[0] == "force":
+            out[cur][int(w[1])]["f"][int(w[2]) - 1, int(w[3]) - 1] = [float(x) for x in w[4:7]]
+        elif w[0] == "dens":
+            out[cur][int(w[1])]["d"][int(w[2]) - 1, int(w[3]) - 1] = float(w[4])
+    assert sorted(out) == [0, 1, 2]
+    for k, s in enumerate(systems):
+        cols, pesm = pesfile.emt_tables(pes_hau, s.type_names)
+        e_r, f_r, d_r = oracle.emt_energy_forces(s, cols, pesm)
+        o1, o2 = out[k][1], out[k][2]
+        assert o1["f"].shape == f_r.shape
+        assert np.abs(o1["e"] - e_r).max() <= 1e-9 * max(1.0, np.abs(e_r).max()), k
+        assert np.abs(o1["f"] - f_r).max() <= 1e-9, k
+        assert np.abs(o1["d"] - d_r).max() <= 1e-11, k
+        assert np.array_equal(o2["e"], 2 * o1["e"]), k
+        assert np.array_equal(o2["f"], 2 * o1["f"]), k
+        assert np.array_equal(o2["d"], o1["d"]), k
