@@ -100,6 +100,27 @@ int mdt_compute_emt(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx
                     double *f, double *dens, int32_t flag);
 
 /*
+ * mdt_compute_emt for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference: a variant of
+ * mdt_compute_emt, same arithmetic and semantics (epot(b) += E_b,
+ * f(:,b,i) += F, dens(b,i) = embedded density of every EMT atom), with
+ * every array a DEVICE pointer in the same layout (idx(N), simbox(3,3),
+ * isimbox(3,3), r(3,P,N), epot(P), f(3,P,N), dens(P,N)); status (device,
+ * may be NULL) receives the call's MDT_ST_* bits.  The universe is staged
+ * in a fixed scratch array of about 4.7 MB per GPU, held until the process
+ * ends: natoms <= 2048, nbeads <= 64, natoms * nbeads <= 16384 (larger
+ * universes return MDT_EINVAL).  Launches go to `stream` (a hipStream_t;
+ * NULL: the context's stream) and are not waited for; the call allocates
+ * nothing and records no memset, so it can be captured in a graph.  Calls
+ * on one GPU share the scratch array: calls that overlap (other streams or
+ * threads) race.
+ */
+int mdt_compute_emt_noalloc(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx,
+                            const double *simbox, const double *isimbox, const double *r, double *epot,
+                            double *f, double *dens, int32_t *status, int32_t flag, void *stream);
+
+/*
  * Batch of `ntraj` independent trajectories resident on the GPU.
  * natoms[t], nbeads[t] may differ between trajectories.
  * Replaces the trajectory loop of src/md_tian2.f90:66-115.

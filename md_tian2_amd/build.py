@@ -20,7 +20,7 @@ CSRC = os.path.join(PKG, "csrc")
 FSRC = os.path.join(PKG, "fortran")
 LIB = os.path.join(PKG, "libmdt_gpu.so")
 
-HIP_SOURCES = ["mdt_kernels.hip", "mdt_nene.hip", "mdt_observe.hip", "mdt_sample.hip", "mdt_api.hip"]
+HIP_SOURCES = ["mdt_kernels.hip", "mdt_nene.hip", "mdt_observe.hip", "mdt_sample.hip", "mdt_noalloc.hip", "mdt_api.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions stay disabled on this pool

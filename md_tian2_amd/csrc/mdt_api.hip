@@ -56,6 +56,18 @@ int fail(int code, const char *fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+// error reporting of mdt_noalloc.hip
+int mdt_noalloc_fail(int code, const char *msg) { return fail(code, "%s", msg); }
+int mdt_noalloc_init(int device, void **ptr);
+int mdt_noalloc_check_sizes(int natoms, int nbeads);
+int mdt_noalloc_compute(void *ptr, size_t lds_limit, const EmtConst *d_ec, int ntypes, int N, int P, const int32_t *idx,
+                        const double *simbox, const double *isimbox, const double *r, double *epot, double *f,
+                        double *dens, int32_t *status, hipStream_t st);
+
+namespace {
+
 #define HIPCHK(expr)                                                                        \
     do {                                                                                    \
         hipError_t e_ = (expr);                                                             \
@@ -91,6 +103,8 @@ struct mdt_context {
     EmtConst *d_ec = nullptr;  // device copy read by the kernels
     std::vector<int32_t> pes;
     mdt_batch *scratch = nullptr;  // cached single-universe batch for mdt_compute_emt / mdt_compute_nene
+    void *noalloc = nullptr;       // this device's scratch array of mdt_compute_emt_noalloc
+    size_t lds_limit = 0;          // LDS per workgroup of the device
     int scratch_N = -1, scratch_P = -1;
     NnConst *nn = nullptr;    // NENE surrogate (host copy)
     NnConst *d_nn = nullptr;  // device copy read by the kernels
@@ -416,6 +430,20 @@ int mdt_create(int device, mdt_context **out) {
     if (e != hipSuccess) {
         mdt_destroy(c);
         return fail(MDT_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        mdt_destroy(c);
+        return fail(MDT_EHIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+    }
+    c->lds_limit = (size_t)prop.sharedMemPerBlock;
+    // the device's scratch array of mdt_compute_emt_noalloc (its address,
+    // taken once per device with the device current)
+    int rc = mdt_noalloc_init(device, &c->noalloc);
+    if (rc) {
+        mdt_destroy(c);
+        return rc;
     }
     *out = c;
     return 0;
@@ -1684,6 +1712,25 @@ int mdt_compute_emt(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx
         if (ctx->ec.owner[idx[i] - 1])
             for (int k = 0; k < nbeads; ++k) dens[k + (long long)nbeads * i] = dd[k + (long long)nbeads * i];
     return 0;
+}
+
+// mdt_compute_emt without allocation: device arrays in and out, the
+// universe staged in the device's scratch array (mdt_noalloc.hip), every
+// launch on `stream` (NULL: the context's stream), nothing waited for.
+int mdt_compute_emt_noalloc(mdt_context *ctx, int natoms, int nbeads, const int32_t *idx, const double *simbox,
+                            const double *isimbox, const double *r, double *epot, double *f, double *dens,
+                            int32_t *status, int32_t flag, void *stream) {
+    (void)flag;  // compute_emt ignores it (forces are always computed)
+    if (!ctx || !idx || !simbox || !isimbox || !r || !epot || !f || !dens)
+        return fail(MDT_EINVAL, "mdt_compute_emt_noalloc: NULL argument");
+    if (!ctx->have_params) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
+    if (!ctx->noalloc) return fail(MDT_ESTATE, "mdt_compute_emt_noalloc: no scratch array on this device");
+    int rc = mdt_noalloc_check_sizes(natoms, nbeads);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(ctx->device));
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    return mdt_noalloc_compute(ctx->noalloc, ctx->lds_limit, ctx->d_ec, ctx->ntypes, natoms, nbeads, idx, simbox,
+                               isimbox, r, epot, f, dens, status, st);
 }
 
 // ---------------------------------------------------------------------------

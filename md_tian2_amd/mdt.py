@@ -24,7 +24,7 @@ ENERGY_AND_FORCE = 3001
 # exported symbols of include/mdt_gpu.h
 SYMBOLS = (
     "mdt_last_error", "mdt_version", "mdt_device_count", "mdt_create", "mdt_destroy",
-    "mdt_emt_params_set", "mdt_compute_emt", "mdt_batch_create", "mdt_batch_destroy",
+    "mdt_emt_params_set", "mdt_compute_emt", "mdt_compute_emt_noalloc", "mdt_batch_create", "mdt_batch_destroy",
     "mdt_batch_set_atoms", "mdt_batch_set_cells", "mdt_batch_set_propagation", "mdt_batch_seed",
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
@@ -69,6 +69,7 @@ def lib():
         L.mdt_destroy.argtypes = [vp]
         L.mdt_emt_params_set.argtypes = [vp, C.c_int, dp, dp, dp, dp, dp, dp, dp, ip]
         L.mdt_compute_emt.argtypes = [vp, C.c_int, C.c_int, ip, dp, dp, dp, dp, dp, dp, C.c_int32]
+        L.mdt_compute_emt_noalloc.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp, C.c_int32, vp]
         L.mdt_batch_create.argtypes = [vp, C.c_int, ip, ip, C.POINTER(vp)]
         L.mdt_batch_destroy.argtypes = [vp]
         L.mdt_batch_set_atoms.argtypes = [vp, ip, dp, ip]
