@@ -206,6 +206,16 @@ int mdt_batch_force_timing(mdt_batch *b, double *force_ms_total);
 int mdt_batch_tune(mdt_batch *b, int tpa, int nbcap, int *tpa_chosen);
 
 /*
+ * Launch shape of the k_replica size classes (replicas grouped by atom
+ * count; no counterpart in the reference, which has no GPU launch): up to
+ * `max` classes, 6 int32 each -- largest atom count, replicas, threads per
+ * atom, lists kept across launches with the Verlet skin (1) or rebuilt
+ * every launch (0), threads per workgroup, list capacity per atom.
+ * *nclass receives the number of classes (0 in mixed bead mode).
+ */
+int mdt_batch_class_shapes(mdt_batch *b, int max, int32_t *out, int *nclass);
+
+/*
  * Arithmetic of the EMT pair terms in compute_emt_2species
  * (src/pes_emt_mod.f90:646-763, 793-842, 871-972), which the reference
  * evaluates in FP64:

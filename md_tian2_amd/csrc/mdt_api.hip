@@ -115,7 +115,8 @@ struct mdt_context {
 // 2048 form a size class with its own launch shape (threads per atom, block,
 // LDS, neighbour buffer), so that small slabs in a mixed batch do not run
 // with the largest slab's shape.
-// size classes whose replicas have more atoms keep their lists across launches
+// Before autotuning, size classes whose replicas have more atoms keep their
+// lists across launches (the autotuner then times both ways per class)
 #define SKIN_MIN_ATOMS 256
 
 struct SizeClass {
@@ -123,11 +124,12 @@ struct SizeClass {
     int nrep = 0;
     int *d_reps = nullptr;     // replica ids; NULL when the class holds every replica in order
     int tpa = 0, nbcap = 0, block = 0;
+    int keep = 0;              // 1: lists in HBM kept across launches (Verlet skin)
     size_t smem = 0;
-    uint16_t *d_nbr = nullptr; // neighbour lists in HBM when they do not fit in LDS
+    uint16_t *d_nbr = nullptr; // neighbour lists in HBM when kept or when they do not fit in LDS
     long long nbr_stride = 0;
-    // Verlet skin of the HBM lists (classes above SKIN_MIN_ATOMS): positions
-    // at the build (3 nmax per block slot), counts (nmax per slot), valid flags
+    // Verlet skin of the kept lists: positions at the build (3 nmax per block
+    // slot), counts (nmax per slot), valid flags
     double *d_ref = nullptr;
     uint16_t *d_kcnt = nullptr;
     int *d_valid = nullptr;
@@ -932,17 +934,19 @@ static int heuristic_tpa(int maxN) {
     return tpa;
 }
 
-static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap) {
+// keep: 1 lists kept in HBM across launches (when the skin is positive), 0
+// rebuilt every launch (LDS when they fit), -1 the default before tuning
+// (kept above SKIN_MIN_ATOMS atoms)
+static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap, int keep = -1) {
     // capacity of the owned half lists: every other atom for small replicas,
     // else 96 (fcc within 1.5 rcut holds 134 neighbours, of which an atom
     // owns about half; overflow is detected and reported)
     int cap = nbcap > 0 ? nbcap : 96;
     if (cap > c.nmax - 1) cap = std::max(1, c.nmax - 1);
     while (tpa > 1 && c.nmax * tpa > 512) tpa >>= 1;
-    // replicas above SKIN_MIN_ATOMS keep their lists in HBM across launches
-    // (Verlet skin) unless the skin is 0; smaller ones rebuild every launch,
-    // their scan visiting few more candidates than a kept list holds
-    const bool skin = b->list_skin > 0.0 && c.nmax > SKIN_MIN_ATOMS;
+    // lists kept in HBM across launches (Verlet skin) unless the skin is 0
+    if (keep < 0) keep = c.nmax > SKIN_MIN_ATOMS ? 1 : 0;
+    const bool skin = b->list_skin > 0.0 && keep == 1;
     const size_t smem_lds = mdt_replica_smem(c.nmax, cap, true);
     const bool in_lds = !skin && smem_lds <= b->lds_limit;
     const size_t smem = in_lds ? smem_lds : mdt_replica_smem(c.nmax, cap, false);
@@ -956,13 +960,14 @@ static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap) {
         if (rc) return rc;
     }
     if (skin) {
-        int rc = dalloc(&c.d_ref, 3 * (size_t)c.nmax * c.nrep) | dalloc(&c.d_kcnt, 2 * (size_t)c.nmax * c.nrep) |
+        int rc = dalloc(&c.d_ref, 3 * (size_t)c.nmax * c.nrep) | dalloc(&c.d_kcnt, (size_t)c.nmax * c.nrep) |
                  dalloc(&c.d_valid, (size_t)c.nrep);
         if (rc) return rc;
         HIPCHK(hipMemset(c.d_valid, 0, sizeof(int) * c.nrep));
     }
     const int rounded = ((c.nmax * tpa + 63) / 64) * 64;
     c.block = std::min(512, std::max(64, rounded));
+    c.keep = skin ? 1 : 0;
     c.tpa = tpa;
     c.nbcap = cap;
     c.smem = smem;
@@ -1299,34 +1304,48 @@ static int autotune(mdt_batch *b) {
     HIPCHK(hipMemcpyAsync(sa, b->d_a, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(se, b->d_epot, sizeof(double) * b->totp, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(sd, b->d_dens, sizeof(double) * b->totpn, hipMemcpyDeviceToDevice, st));
+    // per class: threads per atom x (lists rebuilt every launch, or kept in
+    // HBM with the Verlet skin).  A kept configuration is timed over one
+    // build and four reuses -- about the rebuild rate of a 5 eV H at dt 0.1
+    // fs with the default skin of 0.3 A -- so its figure is not a reuse-only
+    // best case.
     const int cands[4] = {1, 2, 4, 8};
+    const int ntime = 5;
     float total = 0.f;
     for (SizeClass &k : b->cls) {
-        int best = k.tpa;
+        int best = k.tpa, best_keep = k.keep;
         float best_ms = 1e30f;
-        for (int tpa : cands) {
-            if (k.nmax * tpa > 512 && tpa != 1) continue;
-            if ((rc = configure_class(b, k, tpa, b->nbcap_req))) return rc;
-            BatchView v = b->view();
-            v.rep_list = k.d_reps;
-            v.nbr_global = k.d_nbr;
-            v.nbr_stride = k.nbr_stride;
-            v.nbcap = k.nbcap;
-            HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
-            HIPCHK(hipEventRecord(b->ev0, st));
-            for (int q = 0; q < 3; ++q)
+        for (int keep = 0; keep <= (b->list_skin > 0.0 ? 1 : 0); ++keep)
+            for (int tpa : cands) {
+                if (k.nmax * tpa > 512 && tpa != 1) continue;
+                if ((rc = configure_class(b, k, tpa, b->nbcap_req, keep))) return rc;
+                BatchView v = b->view();
+                v.rep_list = k.d_reps;
+                v.nbr_global = k.d_nbr;
+                v.nbr_stride = k.nbr_stride;
+                v.nbcap = k.nbcap;
+                v.rl_ref = k.d_ref;
+                v.rl_cnt = k.d_kcnt;
+                v.rl_valid = k.d_valid;
+                v.rl_skin = b->list_skin;
+                v.rl_ns = k.nmax;
                 HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
-            HIPCHK(hipEventRecord(b->ev1, st));
-            HIPCHK(hipEventSynchronize(b->ev1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = tpa;
+                if (k.keep) HIPCHK(hipMemsetAsync(k.d_valid, 0, sizeof(int) * k.nrep, st));
+                HIPCHK(hipEventRecord(b->ev0, st));
+                for (int q = 0; q < ntime; ++q)
+                    HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
+                HIPCHK(hipEventRecord(b->ev1, st));
+                HIPCHK(hipEventSynchronize(b->ev1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best = tpa;
+                    best_keep = k.keep;
+                }
             }
-        }
-        if ((rc = configure_class(b, k, best, b->nbcap_req))) return rc;
-        k.tuned_ms = best_ms / 3;
+        if ((rc = configure_class(b, k, best, b->nbcap_req, best_keep))) return rc;
+        k.tuned_ms = best_ms / ntime;
         total += k.tuned_ms;
     }
     if (!b->mix_traj.empty()) {
@@ -1512,6 +1531,17 @@ int mdt_batch_set_precision(mdt_batch *b, int mode) {
     return 0;
 }
 
+int mdt_batch_class_shapes(mdt_batch *b, int max, int32_t *out, int *nclass) {
+    if (!b || !nclass || (max > 0 && !out)) return fail(MDT_EINVAL, "NULL argument");
+    *nclass = (int)b->cls.size();
+    for (int k = 0; k < max && k < (int)b->cls.size(); ++k) {
+        const SizeClass &c = b->cls[k];
+        const int32_t v[6] = {c.nmax, c.nrep, c.tpa, c.keep, c.block, c.nbcap};
+        for (int q = 0; q < 6; ++q) out[6 * k + q] = v[q];
+    }
+    return 0;
+}
+
 int mdt_batch_precision_arith(mdt_batch *b, int *arith) {
     if (!b || !arith) return fail(MDT_EINVAL, "NULL argument");
     if (!b->ctx->have_params) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
@@ -1539,7 +1569,9 @@ int mdt_batch_set_list_skin(mdt_batch *b, double skin) {
         int rc = lds_limit(b);
         if (rc) return rc;
         for (SizeClass &c : b->cls)
-            if ((rc = configure_class(b, c, c.tpa > 0 ? c.tpa : heuristic_tpa(c.nmax), b->nbcap_req))) return rc;
+            if ((rc = configure_class(b, c, c.tpa > 0 ? c.tpa : heuristic_tpa(c.nmax), b->nbcap_req,
+                                      c.tpa > 0 ? c.keep : -1)))
+                return rc;
         note_tpa(b);
     }
     return invalidate_lists(b);

@@ -264,6 +264,79 @@ __device__ __forceinline__ double tie_image(double ia, double ib, double ic, dou
     return rint(x - (s - round(s)));
 }
 
+// Minimum image of the pair (k, n) of a replica staged folded in LDS:
+// d = p_k - p_n - shift[code], code the rounding m of the folded fractional
+// difference (in {-1, 0, 1}^3, the list scan's image code).  The rounding
+// is decided in FP32 from ihf (H^-1, 9 floats, column major; error ~1e-7
+// against the 1e-4 tie window); a component within 1e-4 of +-1/2 takes the
+// reference's image from the FP64 H^-1 ihg and the unfolded positions xr
+// (SoA, N per component), both in global memory, as tie_image does.  Only
+// 9 floats of the cell are read: this runs after both passes, where FP64
+// cell registers kept live would cost the hot loops registers.
+// Returns |d|^2.
+__device__ __forceinline__ double pair_image(const float *ihf, const double *ihg, const double *shift,
+                                             const double *px, const double *py, const double *pz,
+                                             const double *xr, int N, int k, int n, double d[3]) {
+    const double cx = px[k] - px[n], cy = py[k] - py[n], cz = pz[k] - pz[n];
+    const float ex = (float)cx, ey = (float)cy, ez = (float)cz;
+    const float f0 = ihf[0] * ex + ihf[3] * ey + ihf[6] * ez;
+    const float f1 = ihf[1] * ex + ihf[4] * ey + ihf[7] * ez;
+    const float f2 = ihf[2] * ex + ihf[5] * ey + ihf[8] * ez;
+    float m0 = rintf(f0), m1 = rintf(f1), m2 = rintf(f2);
+    const bool t0 = fabsf(fabsf(f0) - 0.5f) < 1e-4f, t1 = fabsf(fabsf(f1) - 0.5f) < 1e-4f,
+               t2 = fabsf(fabsf(f2) - 0.5f) < 1e-4f;
+    if (t0 | t1 | t2) {  // rare: the exact fractional difference and the reference's tie rule
+        const double a0 = xr[k], a1 = xr[N + k], a2 = xr[2 * N + k];
+        const double b0 = xr[n], b1 = xr[N + n], b2 = xr[2 * N + n];
+        if (t0) m0 = (float)tie_image(ihg[0], ihg[3], ihg[6], a0, a1, a2, b0, b1, b2, ihg[0] * cx + ihg[3] * cy + ihg[6] * cz);
+        if (t1) m1 = (float)tie_image(ihg[1], ihg[4], ihg[7], a0, a1, a2, b0, b1, b2, ihg[1] * cx + ihg[4] * cy + ihg[7] * cz);
+        if (t2) m2 = (float)tie_image(ihg[2], ihg[5], ihg[8], a0, a1, a2, b0, b1, b2, ihg[2] * cx + ihg[5] * cy + ihg[8] * cz);
+    }
+    const int code = ((int)m0 + 1) + 3 * ((int)m1 + 1) + 9 * ((int)m2 + 1);
+    d[0] = cx - shift[3 * code];
+    d[1] = cy - shift[3 * code + 1];
+    d[2] = cz - shift[3 * code + 2];
+    return d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+}
+
+// Hides v's derivation from the optimiser (an empty asm with a register
+// operand; no instruction is emitted): in a loop that usually runs once,
+// addresses derived from v are then formed per iteration instead of being
+// strength-reduced into one induction register per array.
+__device__ __forceinline__ void opaque(int &v) { asm volatile("" : "+v"(v)); }
+
+// Sum of v over the block in a fixed order (butterfly within each wave, then
+// the waves in order through wsum[nthr / 64]); the result is valid in lane 0.
+// Every lane of the block must call it (it synchronises the block).
+__device__ __forceinline__ double block_sum_fixed(double v, double *wsum, int lane, int nthr) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((lane & 63) == 0) wsum[lane >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (lane == 0)
+        for (int w = 0; w < (nthr + 63) / 64; ++w) s += wsum[w];
+    __syncthreads();
+    return s;
+}
+
+// The same for four values at once (one pair of barriers); wsum holds 64.
+__device__ __forceinline__ void block_sum4_fixed(double v[4], double *wsum, int lane, int nthr) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o, 64);
+    if ((lane & 63) == 0)
+        for (int q = 0; q < 4; ++q) wsum[4 * (lane >> 6) + q] = v[q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double s = 0.0;
+        if (lane == 0)
+            for (int w = 0; w < (nthr + 63) / 64; ++w) s += wsum[4 * w + q];
+        v[q] = s;
+    }
+    __syncthreads();
+}
+
 // Pair ownership: each unordered pair is evaluated once, by its owner; the
 // owner alternates with index parity, which gives every atom about half of
 // its neighbours.  The 2^48 fixed point of the partner sigma sums bounds the
@@ -295,6 +368,7 @@ __global__ void __launch_bounds__(512, 4)
     if (bv.skip && bv.skip[rep]) return;                                           // whole block: uniform
     if (JUDGE && (rep % bv.judge_mod) != bv.judge_phase) return;                  // whole block: uniform
     const int t = bv.rep_traj[rep];
+    const int traj = t;
     if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
     const int b = bv.rep_bead[rep];
     const int N = bv.traj_N[t];
@@ -311,12 +385,12 @@ __global__ void __launch_bounds__(512, 4)
     load_cell(bv, t, cell);
     load_tables(T, ecg, pcg, cell, lane, nthr);
 
-    // LDS: px py pz | three words A0 A1 A2 | G | E | cnt | cntf | nbr | typ |
-    // tiny, where the words hold the fractional coordinates in the scan, the
-    // sigma fixed point (hi, mid, lo) in pass A and the force fixed point
-    // (x, y, z) in pass B.  An atom's list holds its near entries (r <=
-    // rnear) from the front, cnt of them, and its far entries (rnear < r <=
-    // cutr) from the back, cntf of them.
+    // LDS: px py pz | three words A0 A1 A2 | G | E | cnt | nbr | typ | tiny,
+    // where the words hold the fractional coordinates in the scan, the sigma
+    // fixed point (hi, mid, lo) in pass A and the force fixed point (x, y, z)
+    // in pass B.  The lists hold the near pairs (r <= rnear; rnear + skin for
+    // kept lists); the pairs beyond rnear with a tiny atom come from a direct
+    // minimum-image scan of that atom's partners (pair_image).
     double *px = (double *)smem_raw;
     double *py = px + N, *pz = py + N;
     double *sx = pz + N, *sy = sx + N, *sz = sy + N;
@@ -325,22 +399,24 @@ __global__ void __launch_bounds__(512, 4)
     unsigned long long *fxa = shi, *fya = smid, *fza = slo;
     double *G = sz + N, *E = G + N;
     uint16_t *cnt = (uint16_t *)(E + N);
-    uint16_t *cntf = cnt + N;
-    uint16_t *nbr = NBR_LDS ? cntf + N : bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
-    uint8_t *typ = (uint8_t *)(cntf + N + (NBR_LDS ? (size_t)N * cap : 0));
+    uint16_t *nbr = NBR_LDS ? cnt + N : bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
+    uint8_t *typ = (uint8_t *)(cnt + N + (NBR_LDS ? (size_t)N * cap : 0));
     uint8_t *tiny = typ + N;
     __shared__ int s_anytiny;
+    __shared__ double s_wsum[64];
+    __shared__ float s_ihf[9];  // H^-1 in FP32 for the scan of the tiny atoms' partners
+    if (lane < 9) s_ihf[lane] = (float)bv.icell[9 * t + lane];
     int bad_local = 0;
     // Verlet skin (lists in HBM): the lists of this block slot are built with
-    // cut-off + skin and kept while no atom moved more than skin / 2 since
-    // the build; a kept list gets every entry's image recomputed from the
-    // current positions (the same rounding and tie rule as the scan).  Only
-    // in cells whose narrowest width exceeds 2 (cut-off + skin): there every
-    // image within cut-off + skin has fractional components in (-1/2, 1/2),
-    // so it is the rounded image the scan tests, and a pair within the
-    // cut-off now was listed at the build in that same image.  In narrower
-    // cells the rounded image need not be the nearest, and the lists are
-    // rebuilt at every launch.
+    // rnear + skin and kept while no atom moved more than skin / 2 since the
+    // build; a kept list gets every entry's image recomputed from the current
+    // positions (the same rounding and tie rule as the scan).  Only in cells
+    // whose narrowest width exceeds 2 (rnear + skin): there every image
+    // within rnear + skin has fractional components in (-1/2, 1/2), so it is
+    // the rounded image the scan tests, and a pair within rnear now was
+    // listed at the build in that same image.  In narrower cells the rounded
+    // image need not be the nearest, and the lists are rebuilt at every
+    // launch.  (The pairs beyond rnear never come from a list.)
     const double skin = bv.rl_skin;
     double wmin = 1e300;
     for (int q = 0; q < 3; ++q) {
@@ -348,7 +424,8 @@ __global__ void __launch_bounds__(512, 4)
                                     cell.ih[q + 6] * cell.ih[q + 6]);
         wmin = w < wmin ? w : wmin;
     }
-    const bool keep = !NBR_LDS && bv.rl_valid != nullptr && 2.0 * (ecg->cutr + skin) < wmin;
+    const double rnear = ecg->rnear;
+    const bool keep = !NBR_LDS && bv.rl_valid != nullptr && 2.0 * (rnear + skin) < wmin;
     double *ref = keep ? bv.rl_ref + (long long)blockIdx.x * 3 * bv.rl_ns : nullptr;
     __shared__ int s_reuse;
     if (lane == 0) s_reuse = keep && bv.rl_valid[blockIdx.x];
@@ -424,23 +501,19 @@ __global__ void __launch_bounds__(512, 4)
         g02 = (float)(2 * (h[0] * h[6] + h[1] * h[7] + h[2] * h[8]));
         g12 = (float)(2 * (h[3] * h[6] + h[4] * h[7] + h[5] * h[8]));
     }
-    const double cutl = keep ? (cutr + skin) * (cutr + skin) : cut2;
-    const float cut2f = (float)(cutl * (1.0 + 1e-4));
-    // near / far split: a kept list classifies with rnear + skin, so a pair
-    // now within rnear was a near entry at the build
-    const double rn = keep ? ecg->rnear + skin : ecg->rnear;
-    const float near2f = (float)(rn * rn * (1.0 + 1e-4));
-    uint16_t *kcnt = keep ? bv.rl_cnt + (long long)blockIdx.x * 2 * bv.rl_ns : nullptr;
-    uint16_t *kcntf = keep ? kcnt + bv.rl_ns : nullptr;
+    // the lists hold the near pairs; a kept list is built with rnear + skin,
+    // so a pair now within rnear was listed at the build
+    const double rn = keep ? rnear + skin : rnear;
+    const float cut2f = (float)(rn * rn * (1.0 + 1e-4));
+    uint16_t *kcnt = keep ? bv.rl_cnt + (long long)blockIdx.x * bv.rl_ns : nullptr;
     if (reuse) {  // kept list: the image of every entry from the current positions
         for (int k0 = 0; k0 < N; k0 += apb) {
             const int k = k0 + lane / TPA;
             const int sub = lane % TPA;
             if (k >= N) continue;
-            const int c = kcnt[k], cf = kcntf[k];
+            const int c = kcnt[k];
             const float4 pk = f4[k];
-            for (int ee = sub; ee < c + cf; ee += TPA) {
-                const int e = ee < c ? ee : cap - cf + (ee - c);
+            for (int e = sub; e < c; e += TPA) {
                 const unsigned q = nbr[(size_t)k * cap + e];
                 const int n = q & MDT_NBR_IDX_MASK;
                 const float4 pn = f4[n];
@@ -459,10 +532,7 @@ __global__ void __launch_bounds__(512, 4)
                 const int code = ((int)m0f + 1) + 3 * ((int)m1f + 1) + 9 * ((int)m2f + 1);
                 nbr[(size_t)k * cap + e] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
             }
-            if (sub == 0) {
-                cnt[k] = (uint16_t)c;
-                cntf[k] = (uint16_t)cf;
-            }
+            if (sub == 0) cnt[k] = (uint16_t)c;
         }
     }
     // only the owned candidates are visited, in increasing index: below k
@@ -477,11 +547,11 @@ __global__ void __launch_bounds__(512, 4)
         const int tk = act ? typ[k] : 0;
         const float4 pk = act ? f4[k] : make_float4(0.f, 0.f, 0.f, 0.f);
         const int a_lo = (k + 1) >> 1;
-        int c = 0, cf = 0;
+        int c = 0;
         for (int m0 = 0; m0 < mmax; m0 += TPA) {
             const int mi = m0 + sub;
             const int n = mi < a_lo ? ((k + 1) & 1) + 2 * mi : k + 2 + 2 * (mi - a_lo);
-            bool in = false, nr = false;
+            bool in = false;
             int code = 0;
             if (act && n < N) {
                 const float4 pn = f4[n];
@@ -504,29 +574,22 @@ __global__ void __launch_bounds__(512, 4)
                     }
                     const float r2 = d0 * (g00 * d0 + g01 * d1 + g02 * d2) + d1 * (g11 * d1 + g12 * d2) + g22 * d2 * d2;
                     in = r2 <= cut2f;
-                    nr = r2 <= near2f;
                     code = ((int)m0f + 1) + 3 * ((int)m1f + 1) + 9 * ((int)m2f + 1);
                 }
             }
-            const unsigned long long mall = __ballot(in) & gmask;
-            const unsigned long long mn = __ballot(in && nr) & gmask;
-            const unsigned long long mf = mall & ~mn;
-            if (in) {  // near entries from the front, far entries from the back
-                const int pos = nr ? c + __popcll(mn & below) : cf + __popcll(mf & below);
+            const unsigned long long m = __ballot(in) & gmask;
+            if (in) {
+                const int pos = c + __popcll(m & below);
                 if (pos < cap)
-                    nbr[(size_t)k * cap + (nr ? pos : cap - 1 - pos)] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
+                    nbr[(size_t)k * cap + pos] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
+                else
+                    bad_local |= MDT_ST_NBR_OVERFLOW;
             }
-            c += __popcll(mn);
-            cf += __popcll(mf);
+            c += __popcll(m);
         }
         if (act && sub == 0) {
-            if (c + cf > cap) bad_local |= MDT_ST_NBR_OVERFLOW;
             cnt[k] = (uint16_t)(c < cap ? c : cap);
-            cntf[k] = (uint16_t)(cf < cap - cnt[k] ? cf : cap - cnt[k]);
-            if (keep) {
-                kcnt[k] = cnt[k];
-                kcntf[k] = cntf[k];
-            }
+            if (keep) kcnt[k] = cnt[k];
         }
     }
     __syncthreads();
@@ -539,19 +602,10 @@ __global__ void __launch_bounds__(512, 4)
     __syncthreads();
 
     // ---- phase 2: pass A, sigma ----
-    // one listed pair (entry e of atom k's list): sigma_k += own in registers,
-    // sigma_n += other into the fixed point; far entries only with a tiny atom
-    auto pair_a = [&](int k, int tk, double xk, double yk, double zk, int e, bool far, double &sig) {
-        const unsigned q = nbr[(size_t)k * cap + e];
-        const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
-        if (far && !(tiny[k] | tiny[n])) return;
+    // the sigma terms of one pair (k, n) at distance r: sigma_k += own in
+    // registers, sigma_n += other into the fixed point
+    auto sigma_pair = [&](int tk, int n, double r, double &sig) {
         const int tn = typ[n];
-        const double dx = (xk - px[n]) - T.shift[3 * code];
-        const double dy = (yk - py[n]) - T.shift[3 * code + 1];
-        const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const double r = r2 * mdt_rsqrt(r2);
-        if (r > cutr) return;
         double own, other;
         if constexpr (PREC == 1) {
             const float e1 = mdt_exp2f(acutl2f * (float)(r - rcut));
@@ -575,6 +629,18 @@ __global__ void __launch_bounds__(512, 4)
         sig += own;
         sig_add(&shi[n], &smid[n], &slo[n], other, &bad_local);
     };
+    // one listed pair (entry e of atom k's list), within rnear
+    auto pair_a = [&](int k, int tk, double xk, double yk, double zk, int e, double &sig) {
+        const unsigned q = nbr[(size_t)k * cap + e];
+        const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+        const double dx = (xk - px[n]) - T.shift[3 * code];
+        const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+        const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double r = r2 * mdt_rsqrt(r2);
+        if (r > rnear) return;
+        sigma_pair(tk, n, r, sig);
+    };
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
@@ -584,7 +650,7 @@ __global__ void __launch_bounds__(512, 4)
             const int tk = typ[k];
             const double xk = px[k], yk = py[k], zk = pz[k];
             const int ck = cnt[k];
-            for (int e = sub; e < ck; e += TPA) pair_a(k, tk, xk, yk, zk, e, false, sig);
+            for (int e = sub; e < ck; e += TPA) pair_a(k, tk, xk, yk, zk, e, sig);
         }
         sig = group_sum<TPA>(sig);
         if (act && sub == 0) G[k] = sig;  // own part; the partners' part is in shi/smid/slo
@@ -593,7 +659,8 @@ __global__ void __launch_bounds__(512, 4)
     __syncthreads();
 
     // ---- tiny atoms: sigma~ of the near pairs below MDT_SIG_TINY; then the
-    // far pairs of every tiny atom (both ends) ----
+    // pairs beyond rnear of every tiny atom, from a minimum-image scan of
+    // all atoms (a pair of two tiny atoms by its owner) ----
     for (int k = lane; k < N; k += nthr) {
         const uint8_t ty = (T.owner[typ[k]] && G[k] + sig_value(shi[k], smid[k], slo[k]) < MDT_SIG_TINY) ? 1 : 0;
         tiny[k] = ty;
@@ -601,19 +668,33 @@ __global__ void __launch_bounds__(512, 4)
     }
     __syncthreads();
     const bool anytiny = s_anytiny != 0;
+    // the far partners n of tiny atom t: beyond rnear, within the cut-off
+    auto far_partner = [&](int t, int n, double d[3], double &r, double &ri) {
+        if (n == t || (tiny[n] && !owns(t, n)) || !T.pair_on[typ[t] * MDT_MAXT + typ[n]]) return false;
+        const double r2 = pair_image(s_ihf, bv.icell + 9 * traj, T.shift, px, py, pz, bv.x + roff, N, t, n, d);
+        ri = mdt_rsqrt(r2);
+        r = r2 * ri;
+        return r > rnear && r <= cutr;
+    };
+    // the tiny atoms in index order, 64 flags per ballot (the same mask in
+    // every wave, so the loop and its barriers are uniform over the block)
     if (anytiny) {
-        for (int k0 = 0; k0 < N; k0 += apb) {
-            const int k = k0 + lane / TPA;
-            const int sub = lane % TPA;
-            const bool act = k < N;
+        for (int t0 = 0; t0 < N; t0 += 64) {
+          unsigned long long tm = __ballot(t0 + wl < N && tiny[t0 + wl] != 0);
+          while (tm) {
+            const int t = t0 + __builtin_ctzll(tm);
+            tm &= tm - 1;
+            const int tt = typ[t];
             double sig = 0.0;
-            if (act) {
-                const int tk = typ[k];
-                const double xk = px[k], yk = py[k], zk = pz[k];
-                for (int e = cap - cntf[k] + sub; e < cap; e += TPA) pair_a(k, tk, xk, yk, zk, e, true, sig);
+            for (int n0 = lane; n0 < N; n0 += nthr) {
+                int n = n0;
+                opaque(n);
+                double d[3], r, ri;
+                if (far_partner(t, n, d, r, ri)) sigma_pair(tt, n, r, sig);
             }
-            sig = group_sum<TPA>(sig);
-            if (act && sub == 0) G[k] += sig;
+            sig = block_sum_fixed(sig, s_wsum, lane, nthr);
+            if (lane == 0) G[t] += sig;
+          }
         }
         __syncthreads();
     }
@@ -641,19 +722,11 @@ __global__ void __launch_bounds__(512, 4)
     // ---- phase 3: pass B, pair forces and pair energies ----
     // one listed pair: the pair's dE/dr gives +F on the owner (registers) and
     // -F on the partner (fixed point); the owner takes the pair energy
-    auto pair_b = [&](int k, int tk, double xk, double yk, double zk, double Gk, int e, bool far, double &fx,
-                      double &fy, double &fz, double &ep) {
-        const unsigned q = nbr[(size_t)k * cap + e];
-        const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
-        if (far && !(tiny[k] | tiny[n])) return;
+    // the force and energy terms of one pair (k, n), d = r_k - r_n (minimum
+    // image), r = |d|, ri = 1 / r
+    auto force_pair = [&](int tk, int n, double Gk, double dx, double dy, double dz, double r, double ri,
+                          double &fx, double &fy, double &fz, double &ep) {
         const int tn = typ[n];
-        const double dx = (xk - px[n]) - T.shift[3 * code];
-        const double dy = (yk - py[n]) - T.shift[3 * code + 1];
-        const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        const double ri = mdt_rsqrt(r2);
-        const double r = r2 * ri;
-        if (r > cutr) return;
         const int kn = tk * MDT_MAXT + tn, nk = tn * MDT_MAXT + tk;
         double c;
         if constexpr (PREC == 1) {
@@ -706,6 +779,20 @@ __global__ void __launch_bounds__(512, 4)
         fx_add(&fya[n], cr * dy, MDT_F_SCALE, &bad_local);
         fx_add(&fza[n], cr * dz, MDT_F_SCALE, &bad_local);
     };
+    // one listed pair (entry e of atom k's list), within rnear
+    auto pair_b = [&](int k, int tk, double xk, double yk, double zk, double Gk, int e, double &fx, double &fy,
+                      double &fz, double &ep) {
+        const unsigned q = nbr[(size_t)k * cap + e];
+        const int n = q & MDT_NBR_IDX_MASK, code = q >> MDT_NBR_IDX_BITS;
+        const double dx = (xk - px[n]) - T.shift[3 * code];
+        const double dy = (yk - py[n]) - T.shift[3 * code + 1];
+        const double dz = (zk - pz[n]) - T.shift[3 * code + 2];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        const double ri = mdt_rsqrt(r2);
+        const double r = r2 * ri;
+        if (r > rnear) return;
+        force_pair(tk, n, Gk, dx, dy, dz, r, ri, fx, fy, fz, ep);
+    };
     for (int k0 = 0; k0 < N; k0 += apb) {
         const int k = k0 + lane / TPA;
         const int sub = lane % TPA;
@@ -716,10 +803,7 @@ __global__ void __launch_bounds__(512, 4)
             const double xk = px[k], yk = py[k], zk = pz[k];
             const double Gk = G[k];
             const int ck = cnt[k];
-            for (int e = sub; e < ck; e += TPA) pair_b(k, tk, xk, yk, zk, Gk, e, false, fx, fy, fz, ep);
-            if (anytiny)
-                for (int e = cap - cntf[k] + sub; e < cap; e += TPA)
-                    pair_b(k, tk, xk, yk, zk, Gk, e, true, fx, fy, fz, ep);
+            for (int e = sub; e < ck; e += TPA) pair_b(k, tk, xk, yk, zk, Gk, e, fx, fy, fz, ep);
         }
         fx = group_sum<TPA>(fx);
         fy = group_sum<TPA>(fy);
@@ -733,6 +817,34 @@ __global__ void __launch_bounds__(512, 4)
         }
     }
     __syncthreads();
+
+    // ---- the pairs beyond rnear of the tiny atoms (as in pass A) ----
+    if (anytiny) {
+        for (int t0 = 0; t0 < N; t0 += 64) {
+          unsigned long long tm = __ballot(t0 + wl < N && tiny[t0 + wl] != 0);
+          while (tm) {
+            const int t = t0 + __builtin_ctzll(tm);
+            tm &= tm - 1;
+            const int tt = typ[t];
+            const double Gt = G[t];
+            double v[4] = {0.0, 0.0, 0.0, 0.0};  // fx fy fz ep
+            for (int n0 = lane; n0 < N; n0 += nthr) {
+                int n = n0;
+                opaque(n);
+                double d[3], r, ri;
+                if (far_partner(t, n, d, r, ri)) force_pair(tt, n, Gt, d[0], d[1], d[2], r, ri, v[0], v[1], v[2], v[3]);
+            }
+            block_sum4_fixed(v, s_wsum, lane, nthr);
+            if (lane == 0) {
+                fx_add(&fxa[t], v[0], MDT_F_SCALE, &bad_local);
+                fx_add(&fya[t], v[1], MDT_F_SCALE, &bad_local);
+                fx_add(&fza[t], v[2], MDT_F_SCALE, &bad_local);
+                E[t] -= 0.5 * v[3];
+            }
+          }
+        }
+        __syncthreads();
+    }
 
     // ---- judge mode: compare with the given forces, write nothing else ----
     if constexpr (JUDGE) {
@@ -1874,8 +1986,8 @@ __global__ void __launch_bounds__(512, 2)
 // ---------------------------------------------------------------------------
 size_t mdt_replica_smem(int N, int cap, bool nbr_lds) {
     const int c = cap < N - 1 ? cap : (N > 1 ? N - 1 : 1);
-    // px py pz | A0 A1 A2 | G | E: 8 N doubles; cnt, cntf, lists; typ, tiny
-    return sizeof(double) * (8 * (size_t)N) + sizeof(uint16_t) * (2 * (size_t)N + (nbr_lds ? (size_t)N * c : 0)) +
+    // px py pz | A0 A1 A2 | G | E: 8 N doubles; cnt, lists; typ, tiny
+    return sizeof(double) * (8 * (size_t)N) + sizeof(uint16_t) * ((size_t)N + (nbr_lds ? (size_t)N * c : 0)) +
            2 * (size_t)N + 16;
 }
 

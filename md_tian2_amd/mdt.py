@@ -28,7 +28,7 @@ SYMBOLS = (
     "mdt_batch_set_atoms", "mdt_batch_set_cells", "mdt_batch_set_propagation", "mdt_batch_seed",
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
-    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_precision_arith", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
+    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_precision_arith", "mdt_batch_class_shapes", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
     "mdt_batch_set_list_skin",
     "mdt_batch_set_observe", "mdt_batch_get_observables", "mdt_batch_estimators",
     "mdt_sample_uniform", "mdt_sample_conf_index", "mdt_apply_pip", "mdt_set_proj_incidence", "mdt_sample_maxwell",
@@ -88,6 +88,7 @@ def lib():
         L.mdt_batch_force_timing.argtypes = [vp, dp]
         L.mdt_batch_set_precision.argtypes = [vp, C.c_int]
         L.mdt_batch_precision_arith.argtypes = [vp, C.POINTER(C.c_int)]
+        L.mdt_batch_class_shapes.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int)]
         L.mdt_batch_set_bead_mode.argtypes = [vp, C.c_int]
         L.mdt_batch_set_com_removal.argtypes = [vp, C.c_int]
         L.mdt_batch_set_list_skin.argtypes = [vp, C.c_double]
@@ -314,6 +315,16 @@ class Batch:
         out = C.c_int(0)
         _check(lib().mdt_batch_precision_arith(self.h, C.byref(out)))
         return out.value
+
+    def class_shapes(self):
+        """Launch shape per size class: dicts with nmax, nrep, tpa, keep
+        (lists kept across launches), block, nbcap."""
+        n = C.c_int(0)
+        _check(lib().mdt_batch_class_shapes(self.h, 0, None, C.byref(n)))
+        buf = (C.c_int32 * (6 * max(n.value, 1)))()
+        _check(lib().mdt_batch_class_shapes(self.h, n.value, buf, C.byref(n)))
+        keys = ("nmax", "nrep", "tpa", "keep", "block", "nbcap")
+        return [dict(zip(keys, buf[6 * k:6 * k + 6])) for k in range(n.value)]
 
     def set_bead_mode(self, mode):
         """0: every atom carries the beads; 1 (mixed): the projectile (atom 0)

@@ -1,9 +1,12 @@
 """Neighbour lists kept across force calls (Verlet skin) in k_replica.
 
-Size classes of more than 256 atoms keep their owned half lists in HBM,
-built with cut-off + skin, while no atom of the replica moved more than
-skin / 2 since the build; a kept list gets the image of every entry
-recomputed from the current positions.  The reference visits every pair
+Size classes of more than 256 atoms (and smaller ones where the autotuner
+finds it faster) keep their owned half lists in HBM, built with the near
+radius + skin (the lists hold the pairs within the near radius r_near,
+beyond which a pair weighs less than 2^-80; the pairs beyond it of atoms
+whose near sigma is tiny come from a direct scan), while no atom of the
+replica moved more than skin / 2 since the build; a kept list gets the
+image of every entry recomputed from the current positions.  The reference visits every pair
 under the minimum image at every call (src/pes_emt_mod.f90:1204-1206), so
 parity is what a rebuild at every call gives:
 
@@ -12,7 +15,10 @@ parity is what a rebuild at every call gives:
   order of sigma in registers can differ), over 200 steps in which the
   atoms move beyond half the default skin;
 * the forces of the final state equal the oracle's;
-* a 4x4 cell (10 A wide, below 2 (cut-off + skin), half-cell ties inside the
+* a projectile starting 6.5 A above the surface (tiny sigma: its pairs
+  beyond r_near come from the direct scan) moves into the near region on
+  kept lists;
+* a 4x4 cell (10 A wide, below 2 (r_near + skin), half-cell ties inside the
   cut-off) rebuilds at every call: there the rounded image of a pair need not
   be its nearest, so a list built for one image could miss the pair after it
   rounds to another; the skins agree to rounding there too;
@@ -57,7 +63,7 @@ def test_kept_lists_equal_rebuilt_lists(gpu_ctx, pes_hpt, pes_hau, metal, n, nla
     rng = np.random.default_rng(60 + n)
     pes = pes_hpt if metal == "Pt" else pes_hau
     systems = []
-    for z in (1.8, 3.0):
+    for z in (1.8, 3.0, 6.5):
         s = geometry.h_on_slab(metal, n=n, nlayers=nlayers, nbeads=nbeads, rng=rng, z0=z, einc=0.5, tsurf=300.0)
         s.r[0] += rng.normal(0, 0.05, s.r[0].shape)
         systems.append(s)
@@ -83,6 +89,38 @@ def test_kept_lists_equal_rebuilt_lists(gpu_ctx, pes_hpt, pes_hau, metal, n, nla
             assert np.abs(o["r"] - ref["r"]).max() <= 1e-9, skin
             assert np.abs(o["v"] - ref["v"]).max() <= 1e-9 * np.abs(ref["v"]).max(), skin
             assert np.abs(o["epot"] - ref["epot"]).max() <= 1e-9 * np.abs(ref["epot"]).max(), skin
+
+
+def test_autotuned_keep_equals_rebuilt_lists(gpu_ctx, pes_hau):
+    """H/Au(111) 6x6x4 (145 atoms, 15 A cell: kept lists allowed): the
+    autotuner times rebuilt and kept lists per size class; whichever it
+    picks, the trajectories equal those with lists rebuilt at every call."""
+    rng = np.random.default_rng(80)
+    systems = [geometry.h_on_slab("Au", n=6, nlayers=4, rng=rng, z0=z, einc=1.0, tsurf=300.0)
+               for z in (2.0, 4.0, 6.0, 7.0)]
+    cols, pesm = pesfile.emt_tables(pes_hau, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    res = {}
+    for skin in (0.0, 0.3):
+        b = mdt.Batch(gpu_ctx, systems, dt=0.5, tsurf=300.0, tproj=300.0, seed=9)
+        b.set_list_skin(skin)
+        b.tune(tpa=0)
+        shapes = b.class_shapes()
+        print(f"skin {skin}: {shapes}")
+        assert len(shapes) == 1 and shapes[0]["nmax"] == systems[0].natoms
+        if skin == 0.0:
+            assert shapes[0]["keep"] == 0
+        b.compute_forces()
+        b.step(150)
+        assert not b.status().any()
+        res[skin] = b.get_state()
+        if skin == 0.3:
+            _check_oracle(systems, res[skin], cols, pesm)
+        b.close()
+    for ref, o in zip(res[0.0], res[0.3]):
+        assert np.abs(o["r"] - ref["r"]).max() <= 1e-9
+        assert np.abs(o["v"] - ref["v"]).max() <= 1e-9 * np.abs(ref["v"]).max()
+        assert np.abs(o["epot"] - ref["epot"]).max() <= 1e-9 * np.abs(ref["epot"]).max()
 
 
 def test_skin_switch_keeps_batch_usable(gpu_ctx, pes_hpt):
