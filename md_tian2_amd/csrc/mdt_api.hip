@@ -23,7 +23,7 @@ hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const E
 hipError_t mdt_launch_estimators(int ntraj, const PropConst *pc, const BatchView &bv, double *out, hipStream_t st);
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
                            const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
-                           int flags, int block, hipStream_t st);
+                           int flags, int block, bool classical, hipStream_t st);
 hipError_t mdt_launch_advance(uint32_t *step_base, uint32_t n, int set, hipStream_t st);
 size_t mdt_mixed_smem(int nmax, int pmax);
 hipError_t mdt_launch_mixed(int tpa, int ntraj, size_t smem, const EmtConst *ec, const PropConst *pc,
@@ -1256,6 +1256,12 @@ static int traj_block(const mdt_batch *b) {
     return w < 64 ? 64 : (w > 256 ? 256 : w);
 }
 
+// every trajectory classical (P = 1) with one atom per lane and no PILE or
+// Andersen atoms: the propagators run in k_traj_classical
+static bool traj_classical(const mdt_batch *b) {
+    return b->allP1 && b->maxN <= traj_block(b) && !b->pc.any_pile && !b->pc.any_andersen;
+}
+
 static size_t traj_smem(const mdt_batch *b) {
     return sizeof(double) * b->maxP * b->maxP + 7 * (size_t)b->maxN + 16;
 }
@@ -1266,7 +1272,7 @@ static int launch_force(mdt_batch *b) {
     if (!b->allP1) {
         const size_t ts = traj_smem(b);
         HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, nullptr, b->step, b->step, 4,
-                               traj_block(b), c->stream));
+                               traj_block(b), false, c->stream));
     }
     return launch_pes(b, bv, b->allP1 ? 0 : 1, c->stream);
 }
@@ -1413,12 +1419,13 @@ static int enqueue_steps(mdt_batch *b, int nsteps, const uint32_t *step_base, ui
     mdt_context *c = b->ctx;
     const BatchView bv = b->view();
     const size_t ts = traj_smem(b);
+    const bool classical = traj_classical(b);
     const int rp = b->allP1 ? 0 : 4;
     for (int s = 0; s < nsteps; ++s) {
         const uint32_t prev = first + s, st = first + s + 1;
         const int flags = (s > 0 ? 1 : 0) | 2 | rp;
         HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, step_base, prev, st, flags,
-                               traj_block(b), c->stream));
+                               traj_block(b), classical, c->stream));
         if (events) HIPCHK(hipEventRecord(b->evf[2 * s], c->stream));
         int rc = launch_pes(b, bv, b->allP1 ? 0 : 1, c->stream);
         if (rc) return rc;
@@ -1429,7 +1436,7 @@ static int enqueue_steps(mdt_batch *b, int nsteps, const uint32_t *step_base, ui
     }
     if (nsteps > 0)
         HIPCHK(mdt_launch_traj(b->ntraj, ts, b->d_pc, bv, b->d_cjk, b->d_cjkoff, step_base, first + nsteps,
-                               first + nsteps, 1, traj_block(b), c->stream));
+                               first + nsteps, 1, traj_block(b), classical, c->stream));
     return 0;
 }
 

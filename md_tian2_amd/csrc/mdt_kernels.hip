@@ -39,6 +39,7 @@ struct LdsTables {
     double e0[MDT_MAXT], lambda[MDT_MAXT], v0[MDT_MAXT], kappa[MDT_MAXT], n0[MDT_MAXT];
     double densc[MDT_MAXT];
     double csig[MDT_MAXT * MDT_MAXT], etad[MDT_MAXT * MDT_MAXT], wv[MDT_MAXT * MDT_MAXT];
+    double csig48[MDT_MAXT * MDT_MAXT];  // csig * 2^48: partner sigma terms in fixed-point units
     double shift[27 * 3];  // cartesian lattice translation of each image code
     double exptab[64];
     int owner[MDT_MAXT], pair_on[MDT_MAXT * MDT_MAXT], algo[MDT_MAXT], is_proj[MDT_MAXT];
@@ -75,6 +76,7 @@ struct LanCoef {
 };
 
 __device__ __forceinline__ LanCoef lan_coef(double fric, double dt, double temp, bool exact) {
+#pragma clang fp contract(off)
     LanCoef L;
     const double xidt = fric * dt;
     const double xidt2 = xidt * xidt;
@@ -103,6 +105,7 @@ __device__ __forceinline__ bool lan_big(double fric, double dt) { return fric * 
 
 // ldfa, src/md_algo.f90:506-570: friction [1/fs] from the EMT density
 __device__ __forceinline__ double ldfa(double dens, double mass, int *bad) {
+#pragma clang fp contract(off)
     const double convert = 1.00794 * MDT_AMU2MASS;
     const double lo = (double)(-1e-12f), hi = (double)0.36f;
     double fric;
@@ -147,6 +150,7 @@ __device__ void load_tables(LdsTables &T, const EmtConst *__restrict__ ec, const
     }
     for (int k = lane; k < MDT_MAXT * MDT_MAXT; k += nthr) {
         T.csig[k] = ec->csig[k];
+        T.csig48[k] = ec->csig[k] * 281474976710656.0;  // exact (power of two)
         T.etad[k] = ec->etad[k];
         T.wv[k] = ec->wv[k];
         T.pair_on[k] = ec->pair_on[k];
@@ -179,19 +183,14 @@ __device__ void load_tables(LdsTables &T, const EmtConst *__restrict__ ec, const
 // mantissa bits (the sum lies in [2^52, 2^53), where the spacing is 1 and the
 // default rounding is to nearest even), so the integer is the difference of
 // two bit patterns: three instructions instead of the f64 -> i64 expansion.
-// fix_round_small requires |y| < 2^51; fix_round votes across the wave and
-// takes rint and the conversion when any lane's |y| (up to the callers' 9e18
-// bound) is larger, so the common case issues no second path.
+// fix_round_small requires |y| < 2^51; its callers vote across the wave and
+// take rint, the conversion and the overflow test (the callers' 9e18 bound)
+// only when some lane's |y| is larger, so the common case issues neither.
 #define MDT_RND_MAGIC 6755399441055744.0  // 1.5 * 2^52
 __device__ __forceinline__ long long fix_round_small(double y, double *r) {
     const double t = y + MDT_RND_MAGIC;
     *r = t - MDT_RND_MAGIC;
     return __double_as_longlong(t) - __double_as_longlong(MDT_RND_MAGIC);
-}
-__device__ __forceinline__ long long fix_round(double y, double *r) {
-    if (__all(fabs(y) < 2251799813685248.0)) return fix_round_small(y, r);  // 2^51, wave-uniform branch
-    *r = rint(y);
-    return (long long)*r;
 }
 
 // sigma contributions span ~1e3 (close contact) down to ~1e-35 (pairs at the
@@ -200,15 +199,23 @@ __device__ __forceinline__ long long fix_round(double y, double *r) {
 // any magnitude is needed.  Three int64 words at 2^-48, 2^-96 and 2^-144
 // hold each contribution exactly down to 4.5e-44 (each conversion step is an
 // exact power-of-two scaling and an exact subtraction of the rounded part).
-__device__ __forceinline__ void sig_add(unsigned long long *hi, unsigned long long *mid, unsigned long long *lo,
-                                        double x, int *bad) {
-    const double y = x * MDT_W48;
-    if (!(fabs(y) < 9.0e18)) {
-        *bad |= MDT_ST_FIXED_OVERFLOW;
-        return;
-    }
+__device__ __forceinline__ void sig_add_scaled(unsigned long long *hi, unsigned long long *mid,
+                                               unsigned long long *lo, double y, int *bad) {
+    // y: the contribution times 2^48 (callers fold the power of two into a
+    // constant, an exact scaling).  The overflow test sits in the slow path
+    // of the wave-uniform vote.
     double h, m, l;
-    const long long ih = fix_round(y, &h);
+    long long ih;
+    if (__all(fabs(y) < 2251799813685248.0)) {  // 2^51
+        ih = fix_round_small(y, &h);
+    } else {
+        if (!(fabs(y) < 9.0e18)) {
+            *bad |= MDT_ST_FIXED_OVERFLOW;
+            return;
+        }
+        h = rint(y);
+        ih = (long long)h;
+    }
     const double y1 = (y - h) * MDT_W48;  // |y1| <= 2^47
     const long long im = fix_round_small(y1, &m);
     const long long il = fix_round_small((y1 - m) * MDT_W48, &l);  // |.| <= 2^47
@@ -222,14 +229,19 @@ __device__ __forceinline__ double sig_value(unsigned long long hi, unsigned long
            MDT_IW48;
 }
 
-__device__ __forceinline__ void fx_add(unsigned long long *acc, double x, double scale, int *bad) {
-    const double y = x * scale;
-    if (!(fabs(y) < 9.0e18)) {
-        *bad |= MDT_ST_FIXED_OVERFLOW;
-        return;
-    }
+// y: the contribution in fixed-point units (times the accumulator's scale)
+__device__ __forceinline__ void fx_add_scaled(unsigned long long *acc, double y, int *bad) {
     double r;
-    atomicAdd(acc, (unsigned long long)fix_round(y, &r));
+    if (__all(fabs(y) < 2251799813685248.0)) {  // 2^51, wave-uniform branch
+        atomicAdd(acc, (unsigned long long)fix_round_small(y, &r));
+    } else if (fabs(y) < 9.0e18) {
+        atomicAdd(acc, (unsigned long long)(long long)rint(y));
+    } else {
+        *bad |= MDT_ST_FIXED_OVERFLOW;
+    }
+}
+__device__ __forceinline__ void fx_add(unsigned long long *acc, double x, double scale, int *bad) {
+    fx_add_scaled(acc, x * scale, bad);
 }
 
 // s, embedded density, E_coh + vref and G = dE/dsigma of an atom of type tk
@@ -606,14 +618,14 @@ __global__ void __launch_bounds__(512, 4)
     // registers, sigma_n += other into the fixed point
     auto sigma_pair = [&](int tk, int n, double r, double &sig) {
         const int tn = typ[n];
-        double own, other;
+        double own, other48;  // other48: the partner's term times 2^48
         if constexpr (PREC == 1) {
             const float e1 = mdt_exp2f(acutl2f * (float)(r - rcut));
             const float th = mdt_rcpf(1.0f + e1);
             const float fsn = th * mdt_exp2f(-T.f_eta2l2[tn] * (float)(r - T.bs0[tn]));
             const float fsk = (tn == tk) ? fsn : th * mdt_exp2f(-T.f_eta2l2[tk] * (float)(r - T.bs0[tk]));
             own = (double)(T.f_csig[tk * MDT_MAXT + tn] * fsn);
-            other = (double)(T.f_csig[tn * MDT_MAXT + tk] * fsk);
+            other48 = (double)(T.f_csig[tn * MDT_MAXT + tk] * fsk) * MDT_W48;
         } else {
             double th;
             if constexpr (PREC == 2) {
@@ -624,10 +636,10 @@ __global__ void __launch_bounds__(512, 4)
             const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
             const double fsk = (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
             own = T.csig[tk * MDT_MAXT + tn] * fsn;
-            other = T.csig[tn * MDT_MAXT + tk] * fsk;
+            other48 = T.csig48[tn * MDT_MAXT + tk] * fsk;
         }
         sig += own;
-        sig_add(&shi[n], &smid[n], &slo[n], other, &bad_local);
+        sig_add_scaled(&shi[n], &smid[n], &slo[n], other48, &bad_local);
     };
     // one listed pair (entry e of atom k's list), within rnear
     auto pair_a = [&](int k, int tk, double xk, double yk, double zk, int e, double &sig) {
@@ -761,23 +773,28 @@ __global__ void __launch_bounds__(512, 4)
             }
             const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
             const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
-            double fsk = fsn, fvk = fvn;
-            if (tn != tk) {
-                fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
-                fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
-            }
             // dE/dr of this pair: density part through G, pair (V) part directly
-            c = Gk * T.csig[kn] * ((T.etad[kn] + rt1) * fsn) + G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsk) +
-                0.5 * (T.wv[kn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[nk] * ((T.kdb[tk] + rt1) * fvk));
-            ep += T.wv[kn] * fvn + T.wv[nk] * fvk;
+            if (tn == tk) {  // one element: one set of pair constants (kn == nk)
+                c = T.csig[kn] * ((T.etad[kn] + rt1) * fsn) * (Gk + G[n]) + T.wv[kn] * ((T.kdb[tn] + rt1) * fvn);
+                ep += 2.0 * T.wv[kn] * fvn;
+            } else {
+                const double fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+                const double fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
+                c = Gk * T.csig[kn] * ((T.etad[kn] + rt1) * fsn) + G[n] * T.csig[nk] * ((T.etad[nk] + rt1) * fsk) +
+                    0.5 * (T.wv[kn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[nk] * ((T.kdb[tk] + rt1) * fvk));
+                ep += T.wv[kn] * fvn + T.wv[nk] * fvk;
+            }
         }
-        const double cr = c * ri;
-        fx -= cr * dx;
-        fy -= cr * dy;
-        fz -= cr * dz;
-        fx_add(&fxa[n], cr * dx, MDT_F_SCALE, &bad_local);
-        fx_add(&fya[n], cr * dy, MDT_F_SCALE, &bad_local);
-        fx_add(&fza[n], cr * dz, MDT_F_SCALE, &bad_local);
+        // the pair force in fixed-point units (2^42, an exact scaling): the
+        // owner's sums fx, fy, fz are kept in these units too
+        const double crs = c * ri * MDT_F_SCALE;
+        const double gx = crs * dx, gy = crs * dy, gz = crs * dz;
+        fx = fma(-crs, dx, fx);
+        fy = fma(-crs, dy, fy);
+        fz = fma(-crs, dz, fz);
+        fx_add_scaled(&fxa[n], gx, &bad_local);
+        fx_add_scaled(&fya[n], gy, &bad_local);
+        fx_add_scaled(&fza[n], gz, &bad_local);
     };
     // one listed pair (entry e of atom k's list), within rnear
     auto pair_b = [&](int k, int tk, double xk, double yk, double zk, double Gk, int e, double &fx, double &fy,
@@ -810,9 +827,9 @@ __global__ void __launch_bounds__(512, 4)
         fz = group_sum<TPA>(fz);
         ep = group_sum<TPA>(ep);
         if (act && sub == 0) {
-            fx_add(&fxa[k], fx, MDT_F_SCALE, &bad_local);
-            fx_add(&fya[k], fy, MDT_F_SCALE, &bad_local);
-            fx_add(&fza[k], fz, MDT_F_SCALE, &bad_local);
+            fx_add_scaled(&fxa[k], fx, &bad_local);
+            fx_add_scaled(&fya[k], fy, &bad_local);
+            fx_add_scaled(&fza[k], fz, &bad_local);
             E[k] -= 0.5 * ep;
         }
     }
@@ -827,7 +844,7 @@ __global__ void __launch_bounds__(512, 4)
             tm &= tm - 1;
             const int tt = typ[t];
             const double Gt = G[t];
-            double v[4] = {0.0, 0.0, 0.0, 0.0};  // fx fy fz ep
+            double v[4] = {0.0, 0.0, 0.0, 0.0};  // fx fy fz (fixed-point units), ep
             for (int n0 = lane; n0 < N; n0 += nthr) {
                 int n = n0;
                 opaque(n);
@@ -836,9 +853,9 @@ __global__ void __launch_bounds__(512, 4)
             }
             block_sum4_fixed(v, s_wsum, lane, nthr);
             if (lane == 0) {
-                fx_add(&fxa[t], v[0], MDT_F_SCALE, &bad_local);
-                fx_add(&fya[t], v[1], MDT_F_SCALE, &bad_local);
-                fx_add(&fza[t], v[2], MDT_F_SCALE, &bad_local);
+                fx_add_scaled(&fxa[t], v[0], &bad_local);
+                fx_add_scaled(&fya[t], v[1], &bad_local);
+                fx_add_scaled(&fza[t], v[2], &bad_local);
                 E[t] -= 0.5 * v[3];
             }
           }
@@ -922,6 +939,135 @@ __global__ void __launch_bounds__(512, 4)
 // P > 1 trajectory kernel: [propagate_2 of the previous step] then
 // [propagate_1 + COM removal] then [ring-polymer free step] for one
 // trajectory per workgroup.  cjk (P x P, column-major cjk(j,k)) in global.
+// collect_trajectory_characteristics after propagate_2 of step step_prev
+// (src/trajectory_info.f90:47-263) for trajectory t: centroids, the
+// projectile (atom 0) against every other atom (min image).  Reads x, v and
+// f of the projectile and x of every atom from global memory (written
+// before the caller's barrier); every lane of the block calls it.  Returns
+// (block-uniform) whether the trajectory left the box and is frozen from
+// this step.
+__device__ bool traj_observe(const PropConst &pc, const int *s_proj, const BatchView &bv, int t, int N, int P,
+                             long long soff, long long aoff, int lane, int nthr, uint32_t step_prev,
+                             double *s_rmin, int *s_done) {
+#pragma clang fp contract(off)
+#define OX(c, b, i) (soff + ((long long)(b) * 3 + (c)) * N + (i))
+        // centroids, the projectile (atom 0) against every other atom (min image)
+        double cx = 0.0, cy = 0.0, cz = 0.0, vz = 0.0;
+        int fnz = 0, above = 1, below = 1;
+        for (int b = 0; b < P; ++b) {
+            cx += bv.x[OX(0, b, 0)];
+            cy += bv.x[OX(1, b, 0)];
+            cz += bv.x[OX(2, b, 0)];
+            vz += bv.v[OX(2, b, 0)];
+            if (bv.f[OX(0, b, 0)] != 0.0 || bv.f[OX(1, b, 0)] != 0.0 || bv.f[OX(2, b, 0)] != 0.0) fnz = 1;
+            if (!(bv.x[OX(2, b, 0)] > bv.obs_ul)) above = 0;
+            if (!(bv.x[OX(2, b, 0)] < bv.obs_ll)) below = 0;
+        }
+        cx /= P;
+        cy /= P;
+        cz /= P;
+        vz /= P;
+        double h[9], ih[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            h[k] = bv.cell[9 * t + k];
+            ih[k] = bv.icell[9 * t + k];
+        }
+        double rmin = 1.0e300;
+        for (int i = 1 + lane; i < N; i += nthr) {
+            const int ty = bv.type[aoff + i];
+        const int pb = (pc.mixed && !s_proj[ty]) ? 1 : P;
+            double ax = 0.0, ay = 0.0, az = 0.0;
+            for (int b = 0; b < pb; ++b) {
+                ax += bv.x[OX(0, b, i)];
+                ay += bv.x[OX(1, b, i)];
+                az += bv.x[OX(2, b, i)];
+            }
+            const double d0 = cx - ax / pb, d1 = cy - ay / pb, d2 = cz - az / pb;
+            double s0 = ih[0] * d0 + ih[3] * d1 + ih[6] * d2;
+            double s1 = ih[1] * d0 + ih[4] * d1 + ih[7] * d2;
+            double s2 = ih[2] * d0 + ih[5] * d1 + ih[8] * d2;
+            s0 -= round(s0);
+            s1 -= round(s1);
+            s2 -= round(s2);
+            const double e0 = h[0] * s0 + h[3] * s1 + h[6] * s2;
+            const double e1 = h[1] * s0 + h[4] * s1 + h[7] * s2;
+            const double e2 = h[2] * s0 + h[5] * s1 + h[8] * s2;
+            const double r = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+            rmin = r < rmin ? r : rmin;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const double y = __shfl_xor(rmin, o, 64);
+            rmin = y < rmin ? y : rmin;
+        }
+        if ((lane & 63) == 0) s_rmin[lane >> 6] = rmin;
+        __syncthreads();
+        if (lane == 0) {
+            for (int w = 1; w < (nthr + 63) / 64; ++w) rmin = s_rmin[w] < rmin ? s_rmin[w] : rmin;
+            TrajObs &o = bv.obs[t];
+            const int istep = (int)(step_prev - bv.obs_step0);
+            if (rmin < o.cl_appr) {
+                o.cl_appr = rmin;
+                o.cl_appr_step = istep;
+            }
+            o.cl_last = rmin;
+            o.last_step = istep;
+            o.adsorbed_prev = o.adsorbed;
+            if (!o.adsorbed && rmin < bv.ads_start) o.adsorbed = 1;
+            if (o.adsorbed && rmin > bv.ads_end) o.adsorbed = 0;
+            if (cz < o.r_min[2]) {
+                o.r_min[0] = cx;
+                o.r_min[1] = cy;
+                o.r_min[2] = cz;
+            }
+            if (fnz) o.t_int += pc.dt;
+            const int down = vz < 0.0;
+            if (down != o.moving_down) {
+                ++o.turning_points;
+                o.moving_down = down;
+            }
+            *s_done = bv.obs_exit && (above || below);
+            if (*s_done) {
+                o.exit_step = istep;
+                bv.done[t] = 1;
+            }
+        }
+        __syncthreads();
+    return *s_done != 0;
+#undef OX
+}
+
+// The propagator updates shared by k_traj and k_traj_classical, in the
+// reference's operation order with FMA contraction off (the pragma's flag
+// stays on the instructions after inlining): both kernels then give
+// bitwise the same states, whatever code surrounds the updates.
+// langevin_2 (md_algo.f90:253-285) and langevin_1 (:165-198):
+__device__ __forceinline__ double lan_w(const LanCoef &L) {
+#pragma clang fp contract(off)
+    return L.sv * sqrt(1 - L.crv * L.crv);
+}
+__device__ __forceinline__ double lan_v2(const LanCoef &L, double w, double v, double a, double eta) {
+#pragma clang fp contract(off)
+    return v + L.c2 * a + w * eta;
+}
+__device__ __forceinline__ double lan_x1(const LanCoef &L, double dt, double r, double v, double a, double eta) {
+#pragma clang fp contract(off)
+    return r + L.c1 * v + L.c2 * dt * a + L.sr * eta;
+}
+__device__ __forceinline__ double lan_v1(const LanCoef &L, double v, double a, double eta) {
+#pragma clang fp contract(off)
+    return L.c0 * v + (L.c1 - L.c2) * a + L.sv * L.crv * eta;
+}
+// velocity Verlet half step and drift (md_algo.f90:108-131, 226-245)
+__device__ __forceinline__ double verlet_v(double v, double dt, double a) {
+#pragma clang fp contract(off)
+    return v + 0.5 * dt * a;
+}
+__device__ __forceinline__ double verlet_x(double x, double dt, double vn) {
+#pragma clang fp contract(off)
+    return x + dt * vn;
+}
+
 // flags bit0: propagate_2, bit1: propagate_1, bit2: ring-polymer step
 // (trajectories with P == 1 only get f = 0 from bit2, as calc_force zeroes f).
 // step_base (graph replays): when non-null, *step_base is added to both step
@@ -991,6 +1137,8 @@ k_traj(const PropConst *__restrict__ pcg, BatchView bv,
         i = (q) % N;                      \
         b = (q) / N;                      \
     }
+    __shared__ double s_rmin[4];
+    __shared__ int s_done;
     // ring-polymer atoms (P > 1 beads) by propagator; list order is free
     // (every task below is independent of the others)
     if (P > 1)
@@ -1049,18 +1197,18 @@ k_traj(const PropConst *__restrict__ pcg, BatchView bv,
                     s_lan[lane] = L;
                     have_l = true;
                 }
-                const double w = L.sv * sqrt(1 - L.crv * L.crv);
+                const double w = lan_w(L);
                 for (int c = 0; c < 3; ++c) {
                     // the deviate of this step's langevin_1, kept by propagate_1
                     // (mdt_normal(seed, tid, step_prev, MDT_STREAM_LANGEVIN, i P + b, c))
                     const double eta = bv.eta[XI(c, b, i)];
                     const double v = bv.v[XI(c, b, i)];
-                    bv.v[XI(c, b, i)] = (fb >> c & 1) ? 0.0 : v + L.c2 * bv.a[XI(c, b, i)] + w * eta;
+                    bv.v[XI(c, b, i)] = (fb >> c & 1) ? 0.0 : lan_v2(L, w, v, bv.a[XI(c, b, i)], eta);
                 }
             } else {
                 for (int c = 0; c < 3; ++c) {
                     const double v = bv.v[XI(c, b, i)];
-                    bv.v[XI(c, b, i)] = (fb >> c & 1) ? 0.0 : v + 0.5 * dt * bv.a[XI(c, b, i)];
+                    bv.v[XI(c, b, i)] = (fb >> c & 1) ? 0.0 : verlet_v(v, dt, bv.a[XI(c, b, i)]);
                 }
             }
         }
@@ -1068,92 +1216,8 @@ k_traj(const PropConst *__restrict__ pcg, BatchView bv,
     }
 
     if ((flags & 1) && bv.obs) {
-        // collect_trajectory_characteristics after propagate_2 of step
-        // step_prev (src/trajectory_info.f90:47-263): centroids, the
-        // projectile (atom 0) against every other atom (min image)
-        double cx = 0.0, cy = 0.0, cz = 0.0, vz = 0.0;
-        int fnz = 0, above = 1, below = 1;
-        for (int b = 0; b < P; ++b) {
-            cx += bv.x[XI(0, b, 0)];
-            cy += bv.x[XI(1, b, 0)];
-            cz += bv.x[XI(2, b, 0)];
-            vz += bv.v[XI(2, b, 0)];
-            if (bv.f[XI(0, b, 0)] != 0.0 || bv.f[XI(1, b, 0)] != 0.0 || bv.f[XI(2, b, 0)] != 0.0) fnz = 1;
-            if (!(bv.x[XI(2, b, 0)] > bv.obs_ul)) above = 0;
-            if (!(bv.x[XI(2, b, 0)] < bv.obs_ll)) below = 0;
-        }
-        cx /= P;
-        cy /= P;
-        cz /= P;
-        vz /= P;
-        double h[9], ih[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            h[k] = bv.cell[9 * t + k];
-            ih[k] = bv.icell[9 * t + k];
-        }
-        double rmin = 1.0e300;
-        for (int i = 1 + lane; i < N; i += nthr) {
-            const int pb = PT(bv.type[aoff + i]);
-            double ax = 0.0, ay = 0.0, az = 0.0;
-            for (int b = 0; b < pb; ++b) {
-                ax += bv.x[XI(0, b, i)];
-                ay += bv.x[XI(1, b, i)];
-                az += bv.x[XI(2, b, i)];
-            }
-            const double d0 = cx - ax / pb, d1 = cy - ay / pb, d2 = cz - az / pb;
-            double s0 = ih[0] * d0 + ih[3] * d1 + ih[6] * d2;
-            double s1 = ih[1] * d0 + ih[4] * d1 + ih[7] * d2;
-            double s2 = ih[2] * d0 + ih[5] * d1 + ih[8] * d2;
-            s0 -= round(s0);
-            s1 -= round(s1);
-            s2 -= round(s2);
-            const double e0 = h[0] * s0 + h[3] * s1 + h[6] * s2;
-            const double e1 = h[1] * s0 + h[4] * s1 + h[7] * s2;
-            const double e2 = h[2] * s0 + h[5] * s1 + h[8] * s2;
-            const double r = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-            rmin = r < rmin ? r : rmin;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const double y = __shfl_xor(rmin, o, 64);
-            rmin = y < rmin ? y : rmin;
-        }
-        __shared__ double s_rmin[4];
-        __shared__ int s_done;
-        if ((lane & 63) == 0) s_rmin[lane >> 6] = rmin;
-        __syncthreads();
-        if (lane == 0) {
-            for (int w = 1; w < (nthr + 63) / 64; ++w) rmin = s_rmin[w] < rmin ? s_rmin[w] : rmin;
-            TrajObs &o = bv.obs[t];
-            const int istep = (int)(step_prev - bv.obs_step0);
-            if (rmin < o.cl_appr) {
-                o.cl_appr = rmin;
-                o.cl_appr_step = istep;
-            }
-            o.cl_last = rmin;
-            o.last_step = istep;
-            o.adsorbed_prev = o.adsorbed;
-            if (!o.adsorbed && rmin < bv.ads_start) o.adsorbed = 1;
-            if (o.adsorbed && rmin > bv.ads_end) o.adsorbed = 0;
-            if (cz < o.r_min[2]) {
-                o.r_min[0] = cx;
-                o.r_min[1] = cy;
-                o.r_min[2] = cz;
-            }
-            if (fnz) o.t_int += dt;
-            const int down = vz < 0.0;
-            if (down != o.moving_down) {
-                ++o.turning_points;
-                o.moving_down = down;
-            }
-            s_done = bv.obs_exit && (above || below);
-            if (s_done) {
-                o.exit_step = istep;
-                bv.done[t] = 1;
-            }
-        }
-        __syncthreads();
-        if (s_done) {  // frozen after this step: no propagate_1 / ring-polymer step
+        // frozen after this step: no propagate_1 / ring-polymer step
+        if (traj_observe(pc, s_proj, bv, t, N, P, soff, aoff, lane, nthr, step_prev, s_rmin, &s_done)) {
             if (bad_local) atomicOr(&bv.status[t], bad_local);
             return;
         }
@@ -1244,8 +1308,8 @@ k_traj(const PropConst *__restrict__ pcg, BatchView bv,
                     bv.eta[XI(c, b, i)] = eta;  // langevin_2 of this step takes the same deviate
                     const double r = bv.x[XI(c, b, i)], v = bv.v[XI(c, b, i)], a = bv.a[XI(c, b, i)];
                     if (!(fb >> c & 1)) {
-                        bv.x[XI(c, b, i)] = r + L.c1 * v + L.c2 * dt * a + L.sr * eta;
-                        bv.v[XI(c, b, i)] = L.c0 * v + (L.c1 - L.c2) * a + L.sv * L.crv * eta;
+                        bv.x[XI(c, b, i)] = lan_x1(L, dt, r, v, a, eta);
+                        bv.v[XI(c, b, i)] = lan_v1(L, v, a, eta);
                     } else {
                         bv.v[XI(c, b, i)] = 0.0;
                     }
@@ -1253,9 +1317,9 @@ k_traj(const PropConst *__restrict__ pcg, BatchView bv,
             } else if (algo == MDT_PROP_VERLET || algo == MDT_PROP_ANDERSEN || algo == MDT_PROP_PILE) {
                 for (int c = 0; c < 3; ++c) {
                     const double v = bv.v[XI(c, b, i)];
-                    const double vn = (fb >> c & 1) ? 0.0 : v + 0.5 * dt * bv.a[XI(c, b, i)];
+                    const double vn = (fb >> c & 1) ? 0.0 : verlet_v(v, dt, bv.a[XI(c, b, i)]);
                     bv.v[XI(c, b, i)] = vn;
-                    if (pb == 1 && !(fb >> c & 1)) bv.x[XI(c, b, i)] += dt * vn;
+                    if (pb == 1 && !(fb >> c & 1)) bv.x[XI(c, b, i)] = verlet_x(bv.x[XI(c, b, i)], dt, vn);
                 }
                 if (algo == MDT_PROP_ANDERSEN) {
                     const double ibetaN = pb * MDT_KB * (s_proj[ty] ? tproj : tsurf);
@@ -1395,6 +1459,137 @@ k_traj(const PropConst *__restrict__ pcg, BatchView bv,
 #undef PI_
 #undef PT
 #undef AB
+    if (bad_local) atomicOr(&bv.status[t], bad_local);
+}
+
+// ---------------------------------------------------------------------------
+// k_traj_classical: k_traj for batches of classical trajectories (P = 1)
+// with every atom on its own lane and no PILE or Andersen atoms (no COM
+// removal): each lane keeps its atom's state in registers from propagate_2
+// through propagate_1 and the force reset, reading and writing each array
+// once; the only barriers are the observer's.  Same arithmetic, in the same
+// order, as k_traj; a kernel of its own so that k_traj's ring-polymer code
+// does not set its register budget.
+__global__ void __launch_bounds__(256)
+k_traj_classical(const PropConst *__restrict__ pcg, BatchView bv, const uint32_t *step_base, uint32_t step_prev,
+                 uint32_t step, int flags) {
+    __shared__ int s_algo[MDT_MAXT], s_proj[MDT_MAXT];
+    __shared__ double s_rmin[4];
+    __shared__ int s_done;
+    const PropConst &pc = *pcg;
+    const int t = blockIdx.x;
+    if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
+    const int N = bv.traj_N[t];
+    const long long soff = bv.traj_soff[t];
+    const long long poff = soff / 3;
+    const long long aoff = bv.traj_aoff[t];
+    const uint32_t tid = bv.traj_id[t];
+    const double tsurf = bv.tsurf[t], tproj = bv.tproj[t];
+    const double dt = pc.dt;
+    const int nthr = blockDim.x, lane = threadIdx.x;
+    if (step_base) {  // graph replays: step numbers are offsets from the device counter
+        const uint32_t s0 = *step_base;
+        step_prev += s0;
+        step += s0;
+    }
+    for (int k = lane; k < MDT_MAXT; k += nthr) {
+        s_algo[k] = pc.algo[k];
+        s_proj[k] = pc.is_proj[k];
+    }
+    int bad_local = 0;
+#define XI(c, b, i) (soff + ((long long)(b) * 3 + (c)) * N + (i))
+#define PI_(b, i) (poff + (long long)(b) * N + (i))
+    __syncthreads();
+    {
+        const int i = lane;
+        const bool act = i < N;
+        double x[3] = {0.0, 0.0, 0.0}, v[3] = {0.0, 0.0, 0.0}, a[3] = {0.0, 0.0, 0.0};
+        double dn = 0.0, mass = 1.0;
+        int ty = 0, algo = MDT_PROP_VERLET;
+        uint8_t fb = 0;
+        if (act) {
+            ty = bv.type[aoff + i];
+            algo = s_algo[ty];
+            fb = bv.fixed[PI_(0, i)];
+            mass = bv.mass[aoff + i];
+            dn = bv.dens[PI_(0, i)];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                x[c] = bv.x[XI(c, 0, i)];
+                v[c] = bv.v[XI(c, 0, i)];
+                a[c] = bv.a[XI(c, 0, i)];
+            }
+        }
+        const double temp = MDT_KB * (s_proj[ty] ? tproj : tsurf) / mass;
+        LanCoef L;
+        bool have = false;
+        if ((flags & 1) && act) {  // propagate_2 of step step_prev
+            if (algo == MDT_PROP_LANGEVIN) {
+                int bad = 0;
+                dn = ldfa(dn, mass, &bad);  // md_algo.f90:97
+                if (bad) bad_local |= MDT_ST_NEG_DENSITY;
+                bv.dens[PI_(0, i)] = dn;
+                L = lan_coef(dn, dt, temp, lan_big(dn, dt) && temp > MDT_TOLERANCE);
+                have = true;
+                const double w = lan_w(L);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double eta = bv.eta[XI(c, 0, i)];
+                    v[c] = (fb >> c & 1) ? 0.0 : lan_v2(L, w, v[c], a[c], eta);
+                }
+            } else {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) v[c] = (fb >> c & 1) ? 0.0 : verlet_v(v[c], dt, a[c]);
+            }
+        }
+        if ((flags & 1) && bv.obs) {
+            if (act)
+                for (int c = 0; c < 3; ++c) bv.v[XI(c, 0, i)] = v[c];  // the observer reads the projectile's
+            __syncthreads();
+            if (traj_observe(pc, s_proj, bv, t, N, 1, soff, aoff, lane, nthr, step_prev, s_rmin, &s_done)) {
+                if (bad_local) atomicOr(&bv.status[t], bad_local);
+                return;
+            }
+        }
+        if ((flags & 2) && act) {  // propagate_1 of step step
+            if (algo == MDT_PROP_LANGEVIN) {
+                if (!have) L = lan_coef(dn, dt, temp, lan_big(dn, dt) && temp > MDT_TOLERANCE);
+                double z[3];
+                mdt_normal3(pc.seed, tid, step, MDT_STREAM_LANGEVIN, (uint32_t)i, z);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double eta = z[c];
+                    bv.eta[XI(c, 0, i)] = eta;  // langevin_2 of this step takes the same deviate
+                    if (!(fb >> c & 1)) {
+                        const double r = x[c], vo = v[c];
+                        x[c] = lan_x1(L, dt, r, vo, a[c], eta);
+                        v[c] = lan_v1(L, vo, a[c], eta);
+                    } else {
+                        v[c] = 0.0;
+                    }
+                }
+            } else if (algo == MDT_PROP_VERLET) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double vn = (fb >> c & 1) ? 0.0 : verlet_v(v[c], dt, a[c]);
+                    v[c] = vn;
+                    if (!(fb >> c & 1)) x[c] = verlet_x(x[c], dt, vn);
+                }
+            } else {
+                bad_local |= MDT_ST_BAD_PROP;
+            }
+        }
+        if (act) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (flags & 3) bv.v[XI(c, 0, i)] = v[c];
+                if (flags & 2) bv.x[XI(c, 0, i)] = x[c];
+                if (flags & 4) bv.f[XI(c, 0, i)] = 0.0;  // calc_force zeroes f (P == 1)
+            }
+        }
+    }
+#undef XI
+#undef PI_
     if (bad_local) atomicOr(&bv.status[t], bad_local);
 }
 
@@ -1667,7 +1862,7 @@ __global__ void __launch_bounds__(512, 2)
         const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
         const double fsk = (tn == tk) ? fsn : th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
         sig += T.csig[tk * MDT_MAXT + tn] * fsn;
-        sig_add(&shi[n], &smid[n], &slo[n], T.csig[tn * MDT_MAXT + tk] * fsk, &bad_local);
+        sig_add_scaled(&shi[n], &smid[n], &slo[n], T.csig48[tn * MDT_MAXT + tk] * fsk, &bad_local);
     };
     for (int k0 = 0; k0 < Ns; k0 += apb) {
         const int k = k0 + lane / TPA;
@@ -1903,22 +2098,25 @@ __global__ void __launch_bounds__(512, 2)
         const double rt1 = acut * e1 * th;
         const double fsn = th * mdt_exp(-T.eta2[tn] * (r - T.bs0[tn]), T.exptab);
         const double fvn = th * mdt_exp(-T.kdb[tn] * (r - T.bs0[tn]), T.exptab);
-        double fsk = fsn, fvk = fvn;
-        if (tn != tk) {
-            fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
-            fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
+        double c;
+        if (tn == tk) {  // one element: one set of pair constants (as in k_replica)
+            c = T.csig[ikn] * ((T.etad[ikn] + rt1) * fsn) * (Gk + G0[n]) + T.wv[ikn] * ((T.kdb[tn] + rt1) * fvn);
+            ep += 2.0 * T.wv[ikn] * fvn;
+        } else {
+            const double fsk = th * mdt_exp(-T.eta2[tk] * (r - T.bs0[tk]), T.exptab);
+            const double fvk = th * mdt_exp(-T.kdb[tk] * (r - T.bs0[tk]), T.exptab);
+            c = Gk * T.csig[ikn] * ((T.etad[ikn] + rt1) * fsn) + G0[n] * T.csig[ink] * ((T.etad[ink] + rt1) * fsk) +
+                0.5 * (T.wv[ikn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[ink] * ((T.kdb[tk] + rt1) * fvk));
+            ep += T.wv[ikn] * fvn + T.wv[ink] * fvk;
         }
-        const double c = Gk * T.csig[ikn] * ((T.etad[ikn] + rt1) * fsn) +
-                         G0[n] * T.csig[ink] * ((T.etad[ink] + rt1) * fsk) +
-                         0.5 * (T.wv[ikn] * ((T.kdb[tn] + rt1) * fvn) + T.wv[ink] * ((T.kdb[tk] + rt1) * fvk));
-        ep += T.wv[ikn] * fvn + T.wv[ink] * fvk;
-        const double cr = c * ri;
-        fx -= cr * dx;
-        fy -= cr * dy;
-        fz -= cr * dz;
-        fx_add(&fxa[n], cr * dx, MDT_F_SCALE, &bad_local);
-        fx_add(&fya[n], cr * dy, MDT_F_SCALE, &bad_local);
-        fx_add(&fza[n], cr * dz, MDT_F_SCALE, &bad_local);
+        const double crs = c * ri * MDT_F_SCALE;  // fixed-point units, as in k_replica
+        const double gx = crs * dx, gy = crs * dy, gz = crs * dz;
+        fx = fma(-crs, dx, fx);
+        fy = fma(-crs, dy, fy);
+        fz = fma(-crs, dz, fz);
+        fx_add_scaled(&fxa[n], gx, &bad_local);
+        fx_add_scaled(&fya[n], gy, &bad_local);
+        fx_add_scaled(&fza[n], gz, &bad_local);
     };
     for (int k0 = 0; k0 < Ns; k0 += apb) {
         const int k = k0 + lane / TPA;
@@ -1940,9 +2138,9 @@ __global__ void __launch_bounds__(512, 2)
         fz = group_sum<TPA>(fz);
         ep = group_sum<TPA>(ep);
         if (act && sub == 0) {
-            fx_add(&fxa[k], fx, MDT_F_SCALE, &bad_local);
-            fx_add(&fya[k], fy, MDT_F_SCALE, &bad_local);
-            fx_add(&fza[k], fz, MDT_F_SCALE, &bad_local);
+            fx_add_scaled(&fxa[k], fx, &bad_local);
+            fx_add_scaled(&fya[k], fy, &bad_local);
+            fx_add_scaled(&fza[k], fz, &bad_local);
             E0[k] -= 0.5 * ep;
         }
     }
@@ -2048,11 +2246,17 @@ hipError_t mdt_launch_mixed(int tpa, int ntraj, size_t smem, const EmtConst *ec,
     return hipGetLastError();
 }
 
+// classical: every trajectory has P = 1 and at most `block` atoms, and no
+// atom type uses PILE or Andersen (k_traj_classical)
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
                            const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
-                           int flags, int block, hipStream_t st) {
-    hipLaunchKernelGGL(k_traj, dim3(ntraj), dim3(block), smem, st, pc, bv, cjk, cjk_off, step_base, step_prev, step,
-                       flags);
+                           int flags, int block, bool classical, hipStream_t st) {
+    if (classical)
+        hipLaunchKernelGGL(k_traj_classical, dim3(ntraj), dim3(block), 0, st, pc, bv, step_base, step_prev, step,
+                           flags);
+    else
+        hipLaunchKernelGGL(k_traj, dim3(ntraj), dim3(block), smem, st, pc, bv, cjk, cjk_off, step_base, step_prev,
+                           step, flags);
     return hipGetLastError();
 }
 
