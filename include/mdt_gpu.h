@@ -216,6 +216,18 @@ int mdt_batch_tune(mdt_batch *b, int tpa, int nbcap, int *tpa_chosen);
 int mdt_batch_class_shapes(mdt_batch *b, int max, int32_t *out, int *nclass);
 
 /*
+ * Workgroup width of the large size classes (no counterpart in the
+ * reference): classes whose atoms x threads per atom exceed 512 (k_replica,
+ * one or two threads per atom) or whose classical atoms x threads per atom
+ * exceed 512 (k_mixed) may run one 1024-lane workgroup per replica instead
+ * of 512 lanes looping over the atoms.  mode -1: the autotuner times both
+ * (default); 0: always 512; 1: always 1024 where the class is that large.
+ * Results are bitwise the same either way.  The next launch autotunes
+ * again.
+ */
+int mdt_batch_set_wide(mdt_batch *b, int mode);
+
+/*
  * Arithmetic of the EMT pair terms in compute_emt_2species
  * (src/pes_emt_mod.f90:646-763, 793-842, 871-972), which the reference
  * evaluates in FP64:

@@ -26,7 +26,7 @@ hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const Ba
                            int flags, int block, bool classical, hipStream_t st);
 hipError_t mdt_launch_advance(uint32_t *step_base, uint32_t n, int set, hipStream_t st);
 size_t mdt_mixed_smem(int nmax, int pmax);
-hipError_t mdt_launch_mixed(int tpa, int ntraj, size_t smem, const EmtConst *ec, const PropConst *pc,
+hipError_t mdt_launch_mixed(int tpa, int block, int ntraj, size_t smem, const EmtConst *ec, const PropConst *pc,
                             const BatchView &bv, int add_fin, hipStream_t st);
 size_t mdt_nene_smem(int N);
 hipError_t mdt_nene_attributes(hipFuncAttributes *fa);
@@ -125,6 +125,7 @@ struct SizeClass {
     int *d_reps = nullptr;     // replica ids; NULL when the class holds every replica in order
     int tpa = 0, nbcap = 0, block = 0;
     int keep = 0;              // 1: lists in HBM kept across launches (Verlet skin)
+    int wide = 0;              // 1: workgroups of up to 1024 lanes (k_replica's 1024-lane instantiation)
     size_t smem = 0;
     uint16_t *d_nbr = nullptr; // neighbour lists in HBM when kept or when they do not fit in LDS
     long long nbr_stride = 0;
@@ -195,8 +196,9 @@ struct mdt_batch {
     std::vector<int> mix_traj;      // trajectories on k_mixed, by atom count
     // their size classes: a run of mix_traj (offset, count) and its LDS
     struct MixClass {
-        int off = 0, n = 0;
+        int off = 0, n = 0, nmax = 0;
         size_t smem = 0;
+        int tpa = 0, block = 512;  // launch shape (tpa 0: the batch's mix_tpa)
     };
     std::vector<MixClass> mix_cls;
     int *d_mix = nullptr;
@@ -208,6 +210,7 @@ struct mdt_batch {
     int *d_mix_valid = nullptr;
     long long mix_ns = 0;
     double list_skin = 0.3;  // A, mdt_batch_set_list_skin
+    int wide_mode = -1;      // mdt_batch_set_wide: -1 autotuned, 0 512-lane workgroups, 1 up to 1024
     int mix_tpa = 0, mix_nbcap = 0, mix_nmax = 0, mix_pmax = 0;
     size_t mix_smem = 0;
     float mix_tuned_ms = 0.f;
@@ -936,14 +939,21 @@ static int heuristic_tpa(int maxN) {
 
 // keep: 1 lists kept in HBM across launches (when the skin is positive), 0
 // rebuilt every launch (LDS when they fit), -1 the default before tuning
-// (kept above SKIN_MIN_ATOMS atoms)
-static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap, int keep = -1) {
+// (kept above SKIN_MIN_ATOMS atoms).  wide: 1 lets a class of more than 512
+// atoms x threads per atom (one or two threads per atom) run on one
+// workgroup of up to 1024 lanes instead of looping 512 lanes over its atoms;
+// -1 follows mdt_batch_set_wide (1024 lanes only for mode 1).
+static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap, int keep = -1, int wide = -1) {
+    if (wide < 0) wide = b->wide_mode == 1 ? 1 : 0;  // untuned: 1024 lanes only when asked for
     // capacity of the owned half lists: every other atom for small replicas,
     // else 96 (fcc within 1.5 rcut holds 134 neighbours, of which an atom
     // owns about half; overflow is detected and reported)
     int cap = nbcap > 0 ? nbcap : 96;
     if (cap > c.nmax - 1) cap = std::max(1, c.nmax - 1);
-    while (tpa > 1 && c.nmax * tpa > 512) tpa >>= 1;
+    if (wide && tpa > 2) tpa = 2;
+    const int lanes = wide ? 1024 : 512;
+    while (tpa > 1 && c.nmax * tpa > lanes) tpa >>= 1;
+    if (c.nmax * tpa <= 512) wide = 0;  // fits 512 lanes: the 512-lane instantiation
     // lists kept in HBM across launches (Verlet skin) unless the skin is 0
     if (keep < 0) keep = c.nmax > SKIN_MIN_ATOMS ? 1 : 0;
     const bool skin = b->list_skin > 0.0 && keep == 1;
@@ -966,8 +976,9 @@ static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap, int k
         HIPCHK(hipMemset(c.d_valid, 0, sizeof(int) * c.nrep));
     }
     const int rounded = ((c.nmax * tpa + 63) / 64) * 64;
-    c.block = std::min(512, std::max(64, rounded));
+    c.block = std::min(wide ? 1024 : 512, std::max(64, rounded));
     c.keep = skin ? 1 : 0;
+    c.wide = wide;
     c.tpa = tpa;
     c.nbcap = cap;
     c.smem = smem;
@@ -1046,6 +1057,7 @@ static int configure_mixed(mdt_batch *b, int tpa, int nbcap) {
             ++i;
         }
         mc.n = i - mc.off;
+        mc.nmax = nmax;
         mc.smem = mdt_mixed_smem(nmax, pmax);
         if (mc.smem + 4096 > b->lds_limit)  // 4 KiB: the kernel's static tables
             return fail(MDT_EUNSUP, "mixed bead mode: %d atoms and %d beads need %zu bytes of LDS (limit %zu)",
@@ -1135,25 +1147,38 @@ static int fork_close(Fork &f) {
 // block slots start at its offset in the trajectory list and in every
 // per-slot buffer (lists, kept-list state).  With fk, each class takes the
 // next side stream; without, all go to st.
+static int launch_mixed_class(mdt_batch *b, const BatchView &base, const mdt_batch::MixClass &mc, int tpa,
+                              int block, int add_fin, hipStream_t st) {
+    BatchView v = base;
+    const long long off = mc.off;
+    v.rep_list = b->d_mix + off;
+    v.nbr_global = b->d_mix_nbr + off * b->mix_stride;
+    v.nbr_stride = b->mix_stride;
+    v.nbcap = b->mix_nbcap;
+    v.mix_ref = b->d_mix_ref + off * 3 * b->mix_ns;
+    v.mix_fold = b->d_mix_fold + off * 3 * b->mix_ns;
+    v.mix_cnt = b->d_mix_cnt + off * 2 * b->mix_ns;  // near and far counts per slot
+    v.mix_valid = b->d_mix_valid + off;
+    HIPCHK(mdt_launch_mixed(tpa, block, mc.n, mc.smem, b->ctx->d_ec, b->d_pc, v, add_fin, st));
+    return 0;
+}
+
+// tpa > 0 overrides every class's tuned threads per atom
 static int launch_mixed(mdt_batch *b, const BatchView &base, int tpa, int add_fin, hipStream_t st, Fork *fk = nullptr) {
     if (!b->ctx->d_ec) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
     for (const mdt_batch::MixClass &mc : b->mix_cls) {
-        BatchView v = base;
-        const long long off = mc.off;
-        v.rep_list = b->d_mix + off;
-        v.nbr_global = b->d_mix_nbr + off * b->mix_stride;
-        v.nbr_stride = b->mix_stride;
-        v.nbcap = b->mix_nbcap;
-        v.mix_ref = b->d_mix_ref + off * 3 * b->mix_ns;
-        v.mix_fold = b->d_mix_fold + off * 3 * b->mix_ns;
-        v.mix_cnt = b->d_mix_cnt + off * 2 * b->mix_ns;  // near and far counts per slot
-        v.mix_valid = b->d_mix_valid + off;
         hipStream_t s2 = st;
         if (fk) {
             int rc = fork_stream(*fk, &s2);
             if (rc) return rc;
         }
-        HIPCHK(mdt_launch_mixed(tpa, mc.n, mc.smem, b->ctx->d_ec, b->d_pc, v, add_fin, s2));
+        // the class's tuned shape, else the batch's threads per atom on 512
+        // lanes (1024 where mdt_batch_set_wide asks for it and the class is large)
+        const bool own = tpa <= 0 && mc.tpa > 0;
+        const int ct = own ? mc.tpa : (tpa > 0 ? tpa : b->mix_tpa);
+        const int cb = own ? mc.block : (b->wide_mode == 1 && (mc.nmax - 1) * ct > 512 ? 1024 : 512);
+        int rc = launch_mixed_class(b, base, mc, ct, cb, add_fin, s2);
+        if (rc) return rc;
     }
     return 0;
 }
@@ -1180,7 +1205,7 @@ static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hip
         v.rl_ns = k.nmax;
         HIPCHK(mdt_launch_replica(k.tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, add_fin, st));
     }
-    if (!b->mix_traj.empty() && (rc = launch_mixed(b, base, b->mix_tpa, add_fin, main_st, &fk))) return rc;
+    if (!b->mix_traj.empty() && (rc = launch_mixed(b, base, 0, add_fin, main_st, &fk))) return rc;
     return fork_close(fk);
 }
 
@@ -1311,7 +1336,8 @@ static int autotune(mdt_batch *b) {
     HIPCHK(hipMemcpyAsync(se, b->d_epot, sizeof(double) * b->totp, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(sd, b->d_dens, sizeof(double) * b->totpn, hipMemcpyDeviceToDevice, st));
     // per class: threads per atom x (lists rebuilt every launch, or kept in
-    // HBM with the Verlet skin).  A kept configuration is timed over one
+    // HBM with the Verlet skin) x (for more than 512 atoms x threads per
+    // atom: 512 lanes looping over the atoms, or up to 1024 lanes).  A kept configuration is timed over one
     // build and four reuses -- about the rebuild rate of a 5 eV H at dt 0.1
     // fs with the default skin of 0.3 A -- so its figure is not a reuse-only
     // best case.
@@ -1319,60 +1345,79 @@ static int autotune(mdt_batch *b) {
     const int ntime = 5;
     float total = 0.f;
     for (SizeClass &k : b->cls) {
-        int best = k.tpa, best_keep = k.keep;
+        int best = k.tpa, best_keep = k.keep, best_wide = k.wide;
         float best_ms = 1e30f;
         for (int keep = 0; keep <= (b->list_skin > 0.0 ? 1 : 0); ++keep)
-            for (int tpa : cands) {
-                if (k.nmax * tpa > 512 && tpa != 1) continue;
-                if ((rc = configure_class(b, k, tpa, b->nbcap_req, keep))) return rc;
-                BatchView v = b->view();
-                v.rep_list = k.d_reps;
-                v.nbr_global = k.d_nbr;
-                v.nbr_stride = k.nbr_stride;
-                v.nbcap = k.nbcap;
-                v.rl_ref = k.d_ref;
-                v.rl_cnt = k.d_kcnt;
-                v.rl_valid = k.d_valid;
-                v.rl_skin = b->list_skin;
-                v.rl_ns = k.nmax;
-                HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
-                if (k.keep) HIPCHK(hipMemsetAsync(k.d_valid, 0, sizeof(int) * k.nrep, st));
-                HIPCHK(hipEventRecord(b->ev0, st));
-                for (int q = 0; q < ntime; ++q)
+            for (int tpa : cands)
+                for (int wide = b->wide_mode == 1 ? 1 : 0; wide <= (b->wide_mode == 0 ? 0 : 1); ++wide) {
+                    if (!wide && k.nmax * tpa > 512 && tpa != 1) continue;
+                    if (wide && (tpa > 2 || k.nmax * tpa <= 512 || k.nmax * tpa > 1024)) continue;
+                    if (!wide && b->wide_mode == 1 && tpa <= 2 && k.nmax * tpa > 512 && k.nmax * tpa <= 1024)
+                        continue;
+                    if ((rc = configure_class(b, k, tpa, b->nbcap_req, keep, wide))) return rc;
+                    BatchView v = b->view();
+                    v.rep_list = k.d_reps;
+                    v.nbr_global = k.d_nbr;
+                    v.nbr_stride = k.nbr_stride;
+                    v.nbcap = k.nbcap;
+                    v.rl_ref = k.d_ref;
+                    v.rl_cnt = k.d_kcnt;
+                    v.rl_valid = k.d_valid;
+                    v.rl_skin = b->list_skin;
+                    v.rl_ns = k.nmax;
                     HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
-                HIPCHK(hipEventRecord(b->ev1, st));
-                HIPCHK(hipEventSynchronize(b->ev1));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = tpa;
-                    best_keep = k.keep;
+                    if (k.keep) HIPCHK(hipMemsetAsync(k.d_valid, 0, sizeof(int) * k.nrep, st));
+                    HIPCHK(hipEventRecord(b->ev0, st));
+                    for (int q = 0; q < ntime; ++q)
+                        HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
+                    HIPCHK(hipEventRecord(b->ev1, st));
+                    HIPCHK(hipEventSynchronize(b->ev1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = tpa;
+                        best_keep = k.keep;
+                        best_wide = k.wide;
+                    }
                 }
-            }
-        if ((rc = configure_class(b, k, best, b->nbcap_req, best_keep))) return rc;
+        if ((rc = configure_class(b, k, best, b->nbcap_req, best_keep, best_wide))) return rc;
         k.tuned_ms = best_ms / ntime;
         total += k.tuned_ms;
     }
     if (!b->mix_traj.empty()) {
-        int best = b->mix_tpa;
-        float best_ms = 1e30f;
-        for (int tpa : {2, 4, 8}) {
-            if ((rc = launch_mixed(b, b->view(), tpa, 0, st))) return rc;
-            HIPCHK(hipEventRecord(b->ev0, st));
-            for (int q = 0; q < 3; ++q)
-                if ((rc = launch_mixed(b, b->view(), tpa, 0, st))) return rc;
-            HIPCHK(hipEventRecord(b->ev1, st));
-            HIPCHK(hipEventSynchronize(b->ev1));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
-            if (ms < best_ms) {
-                best_ms = ms;
-                best = tpa;
-            }
+        // per mixed-model class: threads per atom x (512 lanes, or 1024 when
+        // the classical atoms x threads per atom exceed 512)
+        b->mix_tuned_ms = 0.f;
+        int most = -1;
+        for (mdt_batch::MixClass &mc : b->mix_cls) {
+            int best = b->mix_tpa, best_block = 512;
+            float best_ms = 1e30f;
+            for (int tpa : {2, 4, 8})
+                for (int block : {512, 1024}) {
+                    const bool big = (mc.nmax - 1) * tpa > 512;
+                    if (block == 1024 && (!big || b->wide_mode == 0)) continue;
+                    if (block == 512 && big && b->wide_mode == 1) continue;
+                    if ((rc = launch_mixed_class(b, b->view(), mc, tpa, block, 0, st))) return rc;
+                    HIPCHK(hipEventRecord(b->ev0, st));
+                    for (int q = 0; q < 3; ++q)
+                        if ((rc = launch_mixed_class(b, b->view(), mc, tpa, block, 0, st))) return rc;
+                    HIPCHK(hipEventRecord(b->ev1, st));
+                    HIPCHK(hipEventSynchronize(b->ev1));
+                    float ms = 0.f;
+                    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best = tpa;
+                        best_block = block;
+                    }
+                }
+            mc.tpa = best;
+            mc.block = best_block;
+            b->mix_tuned_ms += best_ms / 3;
+            if (most < 0 || mc.n > b->mix_cls[most].n) most = (int)(&mc - b->mix_cls.data());
         }
-        b->mix_tpa = best;
-        b->mix_tuned_ms = best_ms / 3;
+        b->mix_tpa = b->mix_cls[most].tpa;  // the reported choice: the class with most trajectories
         total += b->mix_tuned_ms;
     }
     HIPCHK(hipMemcpyAsync(b->d_f, sf, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
@@ -1538,6 +1583,15 @@ int mdt_batch_set_precision(mdt_batch *b, int mode) {
     return 0;
 }
 
+int mdt_batch_set_wide(mdt_batch *b, int mode) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (mode < -1 || mode > 1) return fail(MDT_EINVAL, "mdt_batch_set_wide: mode %d (want -1, 0 or 1)", mode);
+    b->wide_mode = mode;
+    b->tpa = 0;  // the next launch autotunes again
+    drop_graph(b);
+    return 0;
+}
+
 int mdt_batch_class_shapes(mdt_batch *b, int max, int32_t *out, int *nclass) {
     if (!b || !nclass || (max > 0 && !out)) return fail(MDT_EINVAL, "NULL argument");
     *nclass = (int)b->cls.size();
@@ -1577,7 +1631,7 @@ int mdt_batch_set_list_skin(mdt_batch *b, double skin) {
         if (rc) return rc;
         for (SizeClass &c : b->cls)
             if ((rc = configure_class(b, c, c.tpa > 0 ? c.tpa : heuristic_tpa(c.nmax), b->nbcap_req,
-                                      c.tpa > 0 ? c.keep : -1)))
+                                      c.tpa > 0 ? c.keep : -1, c.tpa > 0 ? c.wide : 0)))
                 return rc;
         note_tpa(b);
     }

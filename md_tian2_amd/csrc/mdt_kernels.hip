@@ -369,12 +369,14 @@ __device__ __forceinline__ bool owns(int k, int n) { return (k < n) != (((k + n)
 // per-atom embedding terms, G and every accumulation stay FP64 / fixed point.
 // JUDGE: FP64 EMT forces of the selected replicas compared with
 // bv.judge_f (the network's judge); nothing else is written.
-template <int TPA, bool NBR_LDS, int PREC, bool JUDGE>
-__global__ void __launch_bounds__(512, 4)
+// MAXB: the largest workgroup of the instantiation, 512, or 1024 for size
+// classes of more than 512 atoms x threads per atom, whose few replicas
+// would otherwise leave each CU two waves per SIMD.
+template <int TPA, bool NBR_LDS, int PREC, bool JUDGE, int MAXB = 512>
+__global__ void __launch_bounds__(MAXB, 4)
     k_replica(const EmtConst *__restrict__ ecg, const PropConst *__restrict__ pcg, BatchView bv, int add_fin) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ LdsTables T;
-    __shared__ double red[4];
     const int rep = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;
     if (bv.only && !bv.only[rep]) return;                                          // whole block: uniform
     if (bv.skip && bv.skip[rep]) return;                                           // whole block: uniform
@@ -882,7 +884,7 @@ __global__ void __launch_bounds__(512, 4)
             const double m2 = __shfl_xor(mx, o, 64);
             mx = m2 > mx ? m2 : mx;
         }
-        __shared__ double jr[16];
+        __shared__ double jr[32];
         if ((lane & 63) == 0) {
             jr[2 * (lane >> 6)] = sq;
             jr[2 * (lane >> 6) + 1] = mx;
@@ -1617,8 +1619,11 @@ k_traj_classical(const PropConst *__restrict__ pcg, BatchView bv, const uint32_t
 #define MIX_G 16      // lanes per bead
 #define MIX_BLOCK 512
 
-template <int TPA>
-__global__ void __launch_bounds__(512, 2)
+// MAXB: 512, or 1024 for trajectories large enough that one 512-lane
+// workgroup per CU leaves the classical passes short of waves (the extra
+// lanes work the classical atoms; the bead groups stay in the first 512).
+template <int TPA, int MAXB = 512>
+__global__ void __launch_bounds__(MAXB, 2)
     k_mixed(const EmtConst *__restrict__ ecg, const PropConst *__restrict__ pcg, BatchView bv, int add_fin) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ LdsTables T;
@@ -1925,7 +1930,7 @@ __global__ void __launch_bounds__(512, 2)
     const int kn = tq * MDT_MAXT, kq = tq;
     for (int b0 = 0; b0 < P; b0 += ngrp) {
         const int b = b0 + grp;
-        const bool bact = b < P;
+        const bool bact = grp < ngrp && b < P;
         // 2a: classical atoms within the cut-off of bead b
         {
             int c = 0;
@@ -1971,7 +1976,7 @@ __global__ void __launch_bounds__(512, 2)
                 }
                 c += __popcll(m);
             }
-            if (gsub == 0) hcnt[grp] = (uint16_t)(c < MIX_CAPQ ? c : MIX_CAPQ);
+            if (gsub == 0 && grp < ngrp) hcnt[grp] = (uint16_t)(c < MIX_CAPQ ? c : MIX_CAPQ);
         }
         __syncthreads();
         // 2b: sigma of the projectile's bead b
@@ -2192,6 +2197,14 @@ size_t mdt_replica_smem(int N, int cap, bool nbr_lds) {
 template <bool L, int F, bool J>
 static hipError_t launch_replica_t(int tpa, dim3 g, dim3 blk, size_t smem, const EmtConst *ec, const PropConst *pc,
                                    const BatchView &bv, int add_fin, hipStream_t st) {
+    if (blk.x > 512) {  // 1024-lane instantiations: one or two threads per atom
+        switch (tpa) {
+        case 1: hipLaunchKernelGGL((k_replica<1, L, F, J, 1024>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+        case 2: hipLaunchKernelGGL((k_replica<2, L, F, J, 1024>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (tpa) {
     case 1: hipLaunchKernelGGL((k_replica<1, L, F, J>), g, blk, smem, st, ec, pc, bv, add_fin); break;
     case 2: hipLaunchKernelGGL((k_replica<2, L, F, J>), g, blk, smem, st, ec, pc, bv, add_fin); break;
@@ -2234,9 +2247,20 @@ int mdt_mixed_block(void) { return MIX_BLOCK; }
 // calc_force of the mixed-bead trajectories listed in bv.rep_list (ntraj of
 // them); the classical atoms' owned half lists go to bv.nbr_global +
 // list position x bv.nbr_stride
-hipError_t mdt_launch_mixed(int tpa, int ntraj, size_t smem, const EmtConst *ec, const PropConst *pc,
+// block: MIX_BLOCK (512) or 1024 lanes
+hipError_t mdt_launch_mixed(int tpa, int block, int ntraj, size_t smem, const EmtConst *ec, const PropConst *pc,
                             const BatchView &bv, int add_fin, hipStream_t st) {
-    dim3 g(ntraj), blk(MIX_BLOCK);
+    dim3 g(ntraj), blk(block);
+    if (block == 1024) {
+        switch (tpa) {
+        case 2: hipLaunchKernelGGL((k_mixed<2, 1024>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+        case 4: hipLaunchKernelGGL((k_mixed<4, 1024>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+        case 8: hipLaunchKernelGGL((k_mixed<8, 1024>), g, blk, smem, st, ec, pc, bv, add_fin); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    if (block != MIX_BLOCK) return hipErrorInvalidValue;
     switch (tpa) {
     case 2: hipLaunchKernelGGL((k_mixed<2>), g, blk, smem, st, ec, pc, bv, add_fin); break;
     case 4: hipLaunchKernelGGL((k_mixed<4>), g, blk, smem, st, ec, pc, bv, add_fin); break;
