@@ -33,7 +33,9 @@ METRIC = "trajectory-steps/s: FP64 EMT vs NN surrogate on 1 GPU at stated force 
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 
 # algorithmic work per trajectory-step (counting convention: +,*,FMA = 2;
-# exp, log, sqrt, div = 1): 100 per pair-bead inside 1.5 rcut, 25 per EMT
+# exp, log, sqrt, div = 1): 100 per evaluated pair-bead (the pairs within the
+# near radius, beyond which every pair term weighs less than 2^-80; a tiny
+# atom's far pairs, at most N - 1 per replica, are not counted), 25 per EMT
 # atom-bead, 110 per mobile Langevin atom-bead, 21 per mobile Verlet atom-bead
 FLOP_PAIR, FLOP_ATOM, FLOP_LAN, FLOP_VER = 100, 25, 110, 21
 
@@ -562,9 +564,9 @@ def main():
     mobile = ~s0.fixed[:, 0, 0]
     n_lan = int(((s0.idx == 1) & mobile).sum())
     n_ver = int(((s0.idx == 0) & mobile).sum())
-    # k_replica's own work (calc_force); the integrator's runs in k_traj
-    flop_force = FLOP_PAIR * pairs + FLOP_ATOM * n_atoms
-    flop_traj = flop_force + FLOP_LAN * n_lan + FLOP_VER * n_ver
+    # k_replica's own work (calc_force) over the pairs it evaluates; the
+    # integrator's runs in k_traj
+    flop_force = FLOP_PAIR * pairs_near + FLOP_ATOM * n_atoms
     launch_s = force_ms / 1e3 / max(1, nlaunch)
     achieved = flop_force * len(systems) / launch_s / 1e12
     value = len(systems) * args.steps / (ms_total / 1e3)
@@ -613,7 +615,7 @@ def main():
                 "ms_per_step": f32_run["ms"] / args.steps, "graph_chunk": 50,
                 "precision": "FP32 pair terms, FP64 accumulation and integration",
                 "avg_force_launch_ms": f32_launch_s * 1e3,
-                "achieved_tflops": flop_traj * len(systems) / f32_launch_s / 1e12,
+                "achieved_tflops": flop_force * len(systems) / f32_launch_s / 1e12,
                 "error_bound": "<= 1e-6 eV/atom, <= 1e-5 eV/A vs the FP64 oracle (tests/test_gpu_precision.py)",
             },
             "c1_h_au_3x3x4": c1,

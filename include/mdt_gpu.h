@@ -209,8 +209,9 @@ int mdt_batch_tune(mdt_batch *b, int tpa, int nbcap, int *tpa_chosen);
  * Launch shape of the k_replica size classes (replicas grouped by atom
  * count; no counterpart in the reference, which has no GPU launch): up to
  * `max` classes, 6 int32 each -- largest atom count, replicas, threads per
- * atom, lists kept across launches with the Verlet skin (1) or rebuilt
- * every launch (0), threads per workgroup, list capacity per atom.
+ * atom, lists kept across launches with the Verlet skin (1: staged in LDS
+ * during a launch, 2: read from HBM) or rebuilt every launch (0), threads
+ * per workgroup, list capacity per atom.
  * *nclass receives the number of classes (0 in mixed bead mode).
  */
 int mdt_batch_class_shapes(mdt_batch *b, int max, int32_t *out, int *nclass);
@@ -226,6 +227,17 @@ int mdt_batch_class_shapes(mdt_batch *b, int max, int32_t *out, int *nclass);
  * again.
  */
 int mdt_batch_set_wide(mdt_batch *b, int mode);
+
+/*
+ * Neighbour lists of k_replica's size classes (an extension; the reference
+ * visits every pair at every call): mode -1 autotuned (default); 0 rebuilt
+ * at every launch (in LDS where they fit); 1 kept across launches with the
+ * Verlet skin (mdt_batch_set_list_skin) in HBM and staged in LDS during a
+ * launch (read from HBM where LDS is too small); 2 kept and read from HBM.
+ * Forces agree to rounding whichever mode runs.  The next launch
+ * configures again.
+ */
+int mdt_batch_set_keep(mdt_batch *b, int mode);
 
 /*
  * Arithmetic of the EMT pair terms in compute_emt_2species

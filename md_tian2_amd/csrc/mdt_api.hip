@@ -124,7 +124,8 @@ struct SizeClass {
     int nrep = 0;
     int *d_reps = nullptr;     // replica ids; NULL when the class holds every replica in order
     int tpa = 0, nbcap = 0, block = 0;
-    int keep = 0;              // 1: lists in HBM kept across launches (Verlet skin)
+    int keep = 0;              // lists kept across launches (Verlet skin): 1 staged in LDS, 2 read from HBM
+    int stage = 0;             // kept lists staged in LDS during a launch
     int wide = 0;              // 1: workgroups of up to 1024 lanes (k_replica's 1024-lane instantiation)
     size_t smem = 0;
     uint16_t *d_nbr = nullptr; // neighbour lists in HBM when kept or when they do not fit in LDS
@@ -211,6 +212,7 @@ struct mdt_batch {
     long long mix_ns = 0;
     double list_skin = 0.3;  // A, mdt_batch_set_list_skin
     int wide_mode = -1;      // mdt_batch_set_wide: -1 autotuned, 0 512-lane workgroups, 1 up to 1024
+    int keep_mode = -1;      // mdt_batch_set_keep: -1 autotuned, else the SizeClass::keep of every class
     int mix_tpa = 0, mix_nbcap = 0, mix_nmax = 0, mix_pmax = 0;
     size_t mix_smem = 0;
     float mix_tuned_ms = 0.f;
@@ -937,9 +939,11 @@ static int heuristic_tpa(int maxN) {
     return tpa;
 }
 
-// keep: 1 lists kept in HBM across launches (when the skin is positive), 0
-// rebuilt every launch (LDS when they fit), -1 the default before tuning
-// (kept above SKIN_MIN_ATOMS atoms).  wide: 1 lets a class of more than 512
+// keep: lists kept across launches when the skin is positive -- 1 kept in
+// HBM between launches and staged in LDS during one (where they fit, else
+// as 2), 2 kept in HBM and read from there -- or 0 rebuilt every launch (in
+// LDS where they fit); -1 the default before tuning (1 above SKIN_MIN_ATOMS
+// atoms).  wide: 1 lets a class of more than 512
 // atoms x threads per atom (one or two threads per atom) run on one
 // workgroup of up to 1024 lanes instead of looping 512 lanes over its atoms;
 // -1 follows mdt_batch_set_wide (1024 lanes only for mode 1).
@@ -955,16 +959,17 @@ static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap, int k
     while (tpa > 1 && c.nmax * tpa > lanes) tpa >>= 1;
     if (c.nmax * tpa <= 512) wide = 0;  // fits 512 lanes: the 512-lane instantiation
     // lists kept in HBM across launches (Verlet skin) unless the skin is 0
+    if (b->keep_mode >= 0) keep = b->keep_mode;
     if (keep < 0) keep = c.nmax > SKIN_MIN_ATOMS ? 1 : 0;
-    const bool skin = b->list_skin > 0.0 && keep == 1;
+    const bool skin = b->list_skin > 0.0 && keep >= 1;
     const size_t smem_lds = mdt_replica_smem(c.nmax, cap, true);
-    const bool in_lds = !skin && smem_lds <= b->lds_limit;
+    const bool in_lds = (!skin || keep == 1) && smem_lds <= b->lds_limit;
     const size_t smem = in_lds ? smem_lds : mdt_replica_smem(c.nmax, cap, false);
     if (smem > b->lds_limit)
         return fail(MDT_EUNSUP, "replica of %d atoms needs %zu bytes of LDS (limit %zu)", c.nmax, smem,
                     b->lds_limit);
     free_class_lists(c);
-    if (!in_lds) {
+    if (!in_lds || skin) {
         c.nbr_stride = (long long)c.nmax * cap;
         int rc = dalloc(&c.d_nbr, (size_t)c.nbr_stride * c.nrep);
         if (rc) return rc;
@@ -977,7 +982,8 @@ static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap, int k
     }
     const int rounded = ((c.nmax * tpa + 63) / 64) * 64;
     c.block = std::min(wide ? 1024 : 512, std::max(64, rounded));
-    c.keep = skin ? 1 : 0;
+    c.stage = skin && in_lds ? 1 : 0;
+    c.keep = skin ? (c.stage ? 1 : 2) : 0;
     c.wide = wide;
     c.tpa = tpa;
     c.nbcap = cap;
@@ -1203,6 +1209,7 @@ static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hip
         v.rl_valid = k.d_valid;
         v.rl_skin = b->list_skin;
         v.rl_ns = k.nmax;
+        v.rl_stage = k.stage;
         HIPCHK(mdt_launch_replica(k.tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, add_fin, st));
     }
     if (!b->mix_traj.empty() && (rc = launch_mixed(b, base, 0, add_fin, main_st, &fk))) return rc;
@@ -1335,8 +1342,8 @@ static int autotune(mdt_batch *b) {
     HIPCHK(hipMemcpyAsync(sa, b->d_a, sizeof(double) * b->tot3pn, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(se, b->d_epot, sizeof(double) * b->totp, hipMemcpyDeviceToDevice, st));
     HIPCHK(hipMemcpyAsync(sd, b->d_dens, sizeof(double) * b->totpn, hipMemcpyDeviceToDevice, st));
-    // per class: threads per atom x (lists rebuilt every launch, or kept in
-    // HBM with the Verlet skin) x (for more than 512 atoms x threads per
+    // per class: threads per atom x (lists rebuilt every launch, kept with
+    // the Verlet skin and staged in LDS, or kept and read from HBM) x (for more than 512 atoms x threads per
     // atom: 512 lanes looping over the atoms, or up to 1024 lanes).  A kept configuration is timed over one
     // build and four reuses -- about the rebuild rate of a 5 eV H at dt 0.1
     // fs with the default skin of 0.3 A -- so its figure is not a reuse-only
@@ -1347,7 +1354,9 @@ static int autotune(mdt_batch *b) {
     for (SizeClass &k : b->cls) {
         int best = k.tpa, best_keep = k.keep, best_wide = k.wide;
         float best_ms = 1e30f;
-        for (int keep = 0; keep <= (b->list_skin > 0.0 ? 1 : 0); ++keep)
+        const int kfirst = b->keep_mode >= 0 ? b->keep_mode : 0;
+        const int klast = b->keep_mode >= 0 ? b->keep_mode : (b->list_skin > 0.0 ? 2 : 0);
+        for (int keep = kfirst; keep <= klast; ++keep)
             for (int tpa : cands)
                 for (int wide = b->wide_mode == 1 ? 1 : 0; wide <= (b->wide_mode == 0 ? 0 : 1); ++wide) {
                     if (!wide && k.nmax * tpa > 512 && tpa != 1) continue;
@@ -1365,6 +1374,7 @@ static int autotune(mdt_batch *b) {
                     v.rl_valid = k.d_valid;
                     v.rl_skin = b->list_skin;
                     v.rl_ns = k.nmax;
+                    v.rl_stage = k.stage;
                     HIPCHK(mdt_launch_replica(tpa, k.nrep, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
                     if (k.keep) HIPCHK(hipMemsetAsync(k.d_valid, 0, sizeof(int) * k.nrep, st));
                     HIPCHK(hipEventRecord(b->ev0, st));
@@ -1580,6 +1590,15 @@ int mdt_batch_set_precision(mdt_batch *b, int mode) {
     HIPCHK(hipSetDevice(b->ctx->device));
     if (mode != b->precision) drop_graph(b);
     b->precision = mode;
+    return 0;
+}
+
+int mdt_batch_set_keep(mdt_batch *b, int mode) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (mode < -1 || mode > 2) return fail(MDT_EINVAL, "mdt_batch_set_keep: mode %d (want -1, 0, 1 or 2)", mode);
+    b->keep_mode = mode;
+    b->tpa = 0;  // the next launch configures (and autotunes) again
+    drop_graph(b);
     return 0;
 }
 

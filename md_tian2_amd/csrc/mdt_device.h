@@ -145,11 +145,14 @@ struct BatchView {
     // a size-class launch with lists in HBM: positions at the list build (SoA,
     // rl_ns per coordinate), list counts and a valid flag; rl_valid NULL: the
     // lists are rebuilt at every launch
+    // rl_stage != 0: the kept lists live in nbr_global between launches and
+    // are staged in LDS during one (the kernel's LDS-list instantiation)
     double *rl_ref;
     uint16_t *rl_cnt;
     int *rl_valid;
     double rl_skin;
     long long rl_ns;
+    int rl_stage;
     struct TrajObs *obs;
     int *done;
     double obs_ul, obs_ll, ads_start, ads_end;

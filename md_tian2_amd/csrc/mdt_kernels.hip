@@ -414,6 +414,8 @@ __global__ void __launch_bounds__(MAXB, 4)
     double *G = sz + N, *E = G + N;
     uint16_t *cnt = (uint16_t *)(E + N);
     uint16_t *nbr = NBR_LDS ? cnt + N : bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride;
+    // kept lists staged in LDS: their copy between launches
+    uint16_t *nbr_kept = (NBR_LDS && bv.rl_stage) ? bv.nbr_global + (size_t)blockIdx.x * bv.nbr_stride : nbr;
     uint8_t *typ = (uint8_t *)(cnt + N + (NBR_LDS ? (size_t)N * cap : 0));
     uint8_t *tiny = typ + N;
     __shared__ int s_anytiny;
@@ -439,7 +441,7 @@ __global__ void __launch_bounds__(MAXB, 4)
         wmin = w < wmin ? w : wmin;
     }
     const double rnear = ecg->rnear;
-    const bool keep = !NBR_LDS && bv.rl_valid != nullptr && 2.0 * (rnear + skin) < wmin;
+    const bool keep = (!NBR_LDS || bv.rl_stage) && bv.rl_valid != nullptr && 2.0 * (rnear + skin) < wmin;
     double *ref = keep ? bv.rl_ref + (long long)blockIdx.x * 3 * bv.rl_ns : nullptr;
     __shared__ int s_reuse;
     if (lane == 0) s_reuse = keep && bv.rl_valid[blockIdx.x];
@@ -528,7 +530,7 @@ __global__ void __launch_bounds__(MAXB, 4)
             const int c = kcnt[k];
             const float4 pk = f4[k];
             for (int e = sub; e < c; e += TPA) {
-                const unsigned q = nbr[(size_t)k * cap + e];
+                const unsigned q = nbr_kept[(size_t)k * cap + e];
                 const int n = q & MDT_NBR_IDX_MASK;
                 const float4 pn = f4[n];
                 const float e0 = pk.x - pn.x, e1 = pk.y - pn.y, e2 = pk.z - pn.z;
@@ -607,6 +609,16 @@ __global__ void __launch_bounds__(MAXB, 4)
         }
     }
     __syncthreads();
+    if constexpr (NBR_LDS) {
+        if (keep && !reuse) {  // a fresh build of staged lists: the rows to their copy for the next launches
+            for (int k0 = 0; k0 < N; k0 += apb) {
+                const int k = k0 + lane / TPA;
+                if (k >= N) continue;
+                const int c = cnt[k];
+                for (int e = lane % TPA; e < c; e += TPA) nbr_kept[(size_t)k * cap + e] = nbr[(size_t)k * cap + e];
+            }
+        }
+    }
     if (keep && !reuse && lane == 0) bv.rl_valid[blockIdx.x] = 1;  // read by the next launch
     for (int i = lane; i < N; i += nthr) {
         shi[i] = 0ull;
@@ -2220,7 +2232,7 @@ static hipError_t launch_replica_t(int tpa, dim3 g, dim3 blk, size_t smem, const
 hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const EmtConst *ec, const PropConst *pc,
                               const BatchView &bv, int add_fin, hipStream_t st) {
     dim3 g(nrep), blk(block);
-    const bool lds = bv.nbr_global == nullptr;
+    const bool lds = bv.nbr_global == nullptr || bv.rl_stage;  // kept lists staged in LDS: the LDS instantiation
     if (bv.judge_acc)
         return lds ? launch_replica_t<true, 0, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st)
                    : launch_replica_t<false, 0, true>(tpa, g, blk, smem, ec, pc, bv, add_fin, st);

@@ -28,7 +28,7 @@ SYMBOLS = (
     "mdt_batch_set_atoms", "mdt_batch_set_cells", "mdt_batch_set_propagation", "mdt_batch_seed",
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
-    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_precision_arith", "mdt_batch_class_shapes", "mdt_batch_set_wide", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
+    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_precision_arith", "mdt_batch_class_shapes", "mdt_batch_set_wide", "mdt_batch_set_keep", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
     "mdt_batch_set_list_skin",
     "mdt_batch_set_observe", "mdt_batch_get_observables", "mdt_batch_estimators",
     "mdt_sample_uniform", "mdt_sample_conf_index", "mdt_apply_pip", "mdt_set_proj_incidence", "mdt_sample_maxwell",
@@ -90,6 +90,7 @@ def lib():
         L.mdt_batch_precision_arith.argtypes = [vp, C.POINTER(C.c_int)]
         L.mdt_batch_class_shapes.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int)]
         L.mdt_batch_set_wide.argtypes = [vp, C.c_int]
+        L.mdt_batch_set_keep.argtypes = [vp, C.c_int]
         L.mdt_batch_set_bead_mode.argtypes = [vp, C.c_int]
         L.mdt_batch_set_com_removal.argtypes = [vp, C.c_int]
         L.mdt_batch_set_list_skin.argtypes = [vp, C.c_double]
@@ -321,6 +322,11 @@ class Batch:
         """Workgroup width of the large size classes: -1 autotuned (default),
         0 always 512 lanes, 1 always 1024 where a class is large enough."""
         _check(lib().mdt_batch_set_wide(self.h, int(mode)))
+
+    def set_keep(self, mode):
+        """k_replica's lists: -1 autotuned (default), 0 rebuilt every launch,
+        1 kept (Verlet skin) and staged in LDS, 2 kept and read from HBM."""
+        _check(lib().mdt_batch_set_keep(self.h, int(mode)))
 
     def class_shapes(self):
         """Launch shape per size class: dicts with nmax, nrep, tpa, keep

@@ -123,6 +123,40 @@ def test_autotuned_keep_equals_rebuilt_lists(gpu_ctx, pes_hau):
         assert np.abs(o["epot"] - ref["epot"]).max() <= 1e-9 * np.abs(ref["epot"]).max()
 
 
+@pytest.mark.parametrize("metal,n,nlayers", [("Au", 6, 4), ("Pt", 8, 4)])
+def test_list_modes_agree(gpu_ctx, pes_hau, pes_hpt, metal, n, nlayers):
+    """mdt_batch_set_keep: lists rebuilt every launch (0), kept and staged in
+    LDS during a launch (1), kept and read from HBM (2) give the same
+    trajectories to rounding over 150 steps, with the same threads per
+    atom; mode 1 runs staged where the lists fit in LDS."""
+    rng = np.random.default_rng(85 + n)
+    pes = pes_hau if metal == "Au" else pes_hpt
+    systems = [geometry.h_on_slab(metal, n=n, nlayers=nlayers, rng=rng, z0=z, einc=1.0, tsurf=300.0)
+               for z in (2.0, 6.5, 3.5)]
+    cols, pesm = pesfile.emt_tables(pes, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    res = {}
+    for mode in (0, 1, 2):
+        b = mdt.Batch(gpu_ctx, systems, dt=0.5, tsurf=300.0, tproj=300.0, seed=9)
+        b.set_keep(mode)
+        b.tune(tpa=2)
+        shapes = b.class_shapes()
+        print(f"{metal} mode {mode}: {shapes}")
+        assert shapes[0]["keep"] in ((1, 2) if mode == 1 else (mode,))
+        b.compute_forces()
+        b.step(150)
+        assert not b.status().any()
+        res[mode] = b.get_state()
+        if mode == 1:
+            _check_oracle(systems, res[mode], cols, pesm)
+        b.close()
+    for mode in (1, 2):
+        for ref, o in zip(res[0], res[mode]):
+            assert np.abs(o["r"] - ref["r"]).max() <= 1e-9, mode
+            assert np.abs(o["v"] - ref["v"]).max() <= 1e-9 * np.abs(ref["v"]).max(), mode
+            assert np.abs(o["epot"] - ref["epot"]).max() <= 1e-9 * np.abs(ref["epot"]).max(), mode
+
+
 def test_skin_switch_keeps_batch_usable(gpu_ctx, pes_hpt):
     """Switching the skin off and on moves the lists between LDS and HBM;
     the forces stay the oracle's."""
