@@ -1521,6 +1521,11 @@ k_traj_classical(const PropConst *__restrict__ pcg, BatchView bv, const uint32_t
         double dn = 0.0, mass = 1.0;
         int ty = 0, algo = MDT_PROP_VERLET;
         uint8_t fb = 0;
+        // A fixed component keeps its position, gets v = 0 and uses no
+        // deviate: its x, a and eta are not read, its x and eta not written,
+        // and its v written only when it was not already 0 (the fixed bottom
+        // layers are half the atoms of a slab).
+        double v0[3] = {0.0, 0.0, 0.0};
         if (act) {
             ty = bv.type[aoff + i];
             algo = s_algo[ty];
@@ -1529,9 +1534,11 @@ k_traj_classical(const PropConst *__restrict__ pcg, BatchView bv, const uint32_t
             dn = bv.dens[PI_(0, i)];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                x[c] = bv.x[XI(c, 0, i)];
-                v[c] = bv.v[XI(c, 0, i)];
-                a[c] = bv.a[XI(c, 0, i)];
+                v0[c] = v[c] = bv.v[XI(c, 0, i)];
+                if (!(fb >> c & 1)) {
+                    x[c] = bv.x[XI(c, 0, i)];
+                    a[c] = bv.a[XI(c, 0, i)];
+                }
             }
         }
         const double temp = MDT_KB * (s_proj[ty] ? tproj : tsurf) / mass;
@@ -1547,20 +1554,21 @@ k_traj_classical(const PropConst *__restrict__ pcg, BatchView bv, const uint32_t
                 have = true;
                 const double w = lan_w(L);
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const double eta = bv.eta[XI(c, 0, i)];
-                    v[c] = (fb >> c & 1) ? 0.0 : lan_v2(L, w, v[c], a[c], eta);
-                }
+                for (int c = 0; c < 3; ++c) v[c] = (fb >> c & 1) ? 0.0 : lan_v2(L, w, v[c], a[c], bv.eta[XI(c, 0, i)]);
             } else {
 #pragma unroll
                 for (int c = 0; c < 3; ++c) v[c] = (fb >> c & 1) ? 0.0 : verlet_v(v[c], dt, a[c]);
             }
         }
         if ((flags & 1) && bv.obs) {
-            if (act)
+            if (i == 0)
                 for (int c = 0; c < 3; ++c) bv.v[XI(c, 0, i)] = v[c];  // the observer reads the projectile's
             __syncthreads();
             if (traj_observe(pc, s_proj, bv, t, N, 1, soff, aoff, lane, nthr, step_prev, s_rmin, &s_done)) {
+                // frozen: the state after this propagate_2 is the trajectory's last
+                if (act)
+                    for (int c = 0; c < 3; ++c)
+                        if (!(fb >> c & 1) || v0[c] != 0.0) bv.v[XI(c, 0, i)] = v[c];
                 if (bad_local) atomicOr(&bv.status[t], bad_local);
                 return;
             }
@@ -1573,8 +1581,8 @@ k_traj_classical(const PropConst *__restrict__ pcg, BatchView bv, const uint32_t
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
                     const double eta = z[c];
-                    bv.eta[XI(c, 0, i)] = eta;  // langevin_2 of this step takes the same deviate
                     if (!(fb >> c & 1)) {
+                        bv.eta[XI(c, 0, i)] = eta;  // langevin_2 of this step takes the same deviate
                         const double r = x[c], vo = v[c];
                         x[c] = lan_x1(L, dt, r, vo, a[c], eta);
                         v[c] = lan_v1(L, vo, a[c], eta);
@@ -1596,8 +1604,9 @@ k_traj_classical(const PropConst *__restrict__ pcg, BatchView bv, const uint32_t
         if (act) {
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                if (flags & 3) bv.v[XI(c, 0, i)] = v[c];
-                if (flags & 2) bv.x[XI(c, 0, i)] = x[c];
+                const bool fixc = fb >> c & 1;
+                if ((flags & 3) && (!fixc || v0[c] != 0.0)) bv.v[XI(c, 0, i)] = v[c];
+                if ((flags & 2) && !fixc) bv.x[XI(c, 0, i)] = x[c];
                 if (flags & 4) bv.f[XI(c, 0, i)] = 0.0;  // calc_force zeroes f (P == 1)
             }
         }
