@@ -1668,8 +1668,10 @@ __global__ void __launch_bounds__(MAXB, 2)
     // tiny (classical atoms).  The words A0-A2 hold the fractional
     // coordinates in the scan, the sigma fixed point in pass A and the force
     // fixed point afterwards.  A classical atom's list holds its near entries
-    // from the front (cnt) and its far entries from the back (cntf), as in
-    // k_replica.
+    // (within the near radius) from the front (cnt) and its far entries
+    // (between the near radius and the cut-off) from the back (cntf); the far
+    // entries are evaluated only for pairs with a tiny atom, one whose sigma
+    // from its near pairs is below MDT_SIG_TINY.
     const int ngrp = MIX_BLOCK / MIX_G;
     double *px = (double *)smem_raw, *py = px + Ns, *pz = py + Ns;
     double *sx = pz + Ns, *sy = sx + Ns, *sz = sy + Ns;

@@ -194,3 +194,36 @@ def test_sizes_beyond_the_limits_are_refused(gpu_ctx, pes_hau):
         gpu_ctx.set_emt(cols, pesm)
         with pytest.raises(mdt.MdtError, match="supported|natoms|beads"):
             mdt.Batch(gpu_ctx, [s], dt=0.1).compute_forces()
+
+
+def test_tiny_atoms_far_pairs_at_half_cell_ties(gpu_ctx, pes_hau):
+    """Atoms with no pair within the near radius are tiny: their pairs
+    between r_near (6.9 A) and the cut-off (7.72 A) come from the direct
+    scan, whose minimum image is decided in FP32 and, within 1e-4 of a
+    half-cell tie, taken from the unfolded positions by the reference's
+    rounding of s = isimbox (r_j - r_i) (src/universe_mod.f90:411-428).  Two
+    Au atoms exactly half the 14.5 A x width apart (7.25 A) and two half the
+    15 A y width apart (7.5 A) make far pairs of two tiny atoms at ties:
+    the image picked sets the force's sign, and sigma ~ 1e-30 puts them in
+    the s-clamp regime.  An H atom far from everything is tiny with no
+    partner at all."""
+    tmpl = geometry.h_on_slab("Au", n=3, nlayers=4, rng=np.random.default_rng(1))
+    pos = np.array([[5.0, 3.0, 22.0],                       # H, alone
+                    [1.0, 2.0, 4.0], [1.0 + 7.25, 2.0, 4.0],  # x tie, 7.25 A
+                    [3.0, 4.0, 12.0], [3.0, 4.0 + 7.5, 12.0]])  # y tie, 7.5 A
+    n = len(pos)
+    idx = np.array([0, 1, 1, 1, 1], dtype=np.int32)
+    s = geometry.System(list(tmpl.type_names), idx, tmpl.mass[[0, 1, 1, 1, 1]].copy(),
+                        np.zeros((n, 1, 3), dtype=bool), tmpl.algo.copy(),
+                        np.diag([14.5, 15.0, 30.0]), pos.reshape(n, 1, 3).copy(), np.zeros((n, 1, 3)))
+    e_g, f_g = _cmp(gpu_ctx, s, pes_hau)
+    e_r, f_r, _ = oracle.emt_energy_forces(s, *pesfile.emt_tables(pes_hau, s.type_names))
+    # the tie pairs interact (nonzero forces along the tie axis) and the signs agree
+    assert abs(f_r[1, 0, 0]) > 0 and abs(f_r[3, 0, 1]) > 0
+    assert np.sign(f_g[1, 0, 0]) == np.sign(f_r[1, 0, 0]) and np.sign(f_g[3, 0, 1]) == np.sign(f_r[3, 0, 1])
+    # the same system shifted by whole cell vectors (unfolded positions change
+    # the rounded s, so the reference may pick the other image) still agrees
+    s2 = s.copy()
+    s2.r[2, 0, 0] -= 14.5
+    s2.r[4, 0, 1] += 15.0
+    _cmp(gpu_ctx, s2, pes_hau)
