@@ -221,6 +221,37 @@ module mdt_gpu
             integer(c_int) :: rc
         end function
 
+        function mdt_batch_set_keep(batch, mode) bind(C, name="mdt_batch_set_keep") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: mode
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_set_wide(batch, mode) bind(C, name="mdt_batch_set_wide") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: mode
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_class_shapes(batch, nmax, shapes, nclass) bind(C, name="mdt_batch_class_shapes") &
+                result(rc)
+            import :: c_int, c_int32_t, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: nmax
+            integer(c_int32_t), intent(out) :: shapes(6, *)
+            integer(c_int), intent(out) :: nclass
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_precision_arith(batch, arith) bind(C, name="mdt_batch_precision_arith") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), intent(out) :: arith
+            integer(c_int) :: rc
+        end function
+
         function mdt_batch_set_observe(batch, proj_ul, proj_ll, ads_start, ads_end, exit_at_limits) &
                 bind(C, name="mdt_batch_set_observe") result(rc)
             import :: c_int, c_ptr, c_double
