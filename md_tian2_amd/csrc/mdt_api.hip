@@ -29,7 +29,8 @@ size_t mdt_mixed_smem(int nmax, int pmax);
 hipError_t mdt_launch_mixed(int tpa, int block, int ntraj, size_t smem, const EmtConst *ec, const PropConst *pc,
                             const BatchView &bv, int add_fin, hipStream_t st);
 size_t mdt_nene_smem(int N);
-hipError_t mdt_nene_attributes(hipFuncAttributes *fa);
+size_t mdt_nene_radial_smem(int N);
+hipError_t mdt_nene_attributes(hipFuncAttributes *fa, int radial);
 size_t mdt_nn_frame_smem(int N);
 hipError_t mdt_launch_nn_frame(int nblocks, size_t smem, const NnConst *M, const NnTrainView &tv, const int *frames,
                                int mode, double cEw, double cFw, hipStream_t st);
@@ -40,7 +41,8 @@ hipError_t mdt_launch_nn_grad(int nblocks, const NnConst *M, const NnTrainView &
 hipError_t mdt_launch_nn_adam(NnConst *M, const float *partials, int W, float *grad, float *am, float *av, float lr,
                               float beta1, float beta2, float eps, int step, int mode, hipStream_t st);
 hipError_t mdt_launch_nene(int nrep, size_t smem, const NnConst *M, const BatchView &bv, float *G, float *C,
-                           int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, hipStream_t st);
+                           int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, int radial,
+                           hipStream_t st);
 
 namespace {
 
@@ -1216,18 +1218,27 @@ static int launch_replicas(mdt_batch *b, const BatchView &base, int add_fin, hip
     return fork_close(fk);
 }
 
+// radial-only networks run k_nene_rg / k_nene_rf
+static bool nn_radial(const mdt_context *c) { return c->nn && c->nn->radial_only; }
+
+static size_t nn_smem(const mdt_context *c, int N) {
+    return nn_radial(c) ? mdt_nene_radial_smem(N) : mdt_nene_smem(N);
+}
+
+// LDS check for the context's current network (every call: the network may
+// change between calls), buffers at the first call
 static int ensure_nn(mdt_batch *b) {
-    if (b->d_nnG) return 0;
     if (b->lds_limit == 0) {
         hipDeviceProp_t prop;
         HIPCHK(hipGetDeviceProperties(&prop, b->ctx->device));
         b->lds_limit = (size_t)prop.sharedMemPerBlock;
     }
     hipFuncAttributes fa;
-    HIPCHK(mdt_nene_attributes(&fa));
-    if (fa.sharedSizeBytes + mdt_nene_smem(b->maxN) > b->lds_limit)
+    HIPCHK(mdt_nene_attributes(&fa, nn_radial(b->ctx)));
+    if (fa.sharedSizeBytes + nn_smem(b->ctx, b->maxN) > b->lds_limit)
         return fail(MDT_EUNSUP, "network on %d atoms needs %zu bytes of LDS (limit %zu)", b->maxN,
-                    fa.sharedSizeBytes + mdt_nene_smem(b->maxN), b->lds_limit);
+                    fa.sharedSizeBytes + nn_smem(b->ctx, b->maxN), b->lds_limit);
+    if (b->d_nnG) return 0;
     const size_t rows = (size_t)b->nrep * b->maxN * NN_K1;
     int rc = dalloc(&b->d_nnG, rows) | dalloc(&b->d_nnC, rows) | dalloc(&b->d_nnflag, (size_t)b->nrep) |
              dalloc(&b->d_nnF, (size_t)b->tot3pn) | dalloc(&b->d_judge, 3 * (size_t)b->nrep);
@@ -1253,8 +1264,8 @@ static int launch_pes(mdt_batch *b, const BatchView &bv, int add_fin, hipStream_
             v.rep_list = k.d_reps;
             hipStream_t s2 = st;
             if ((rc = fork_stream(fk, &s2))) return rc;
-            HIPCHK(mdt_launch_nene(k.nrep, mdt_nene_smem(k.nmax), c->d_nn, v, b->d_nnG, b->d_nnC, b->d_nnflag,
-                                   b->d_nnF, b->maxN, add_fin, 1, s2));
+            HIPCHK(mdt_launch_nene(k.nrep, nn_smem(c, k.nmax), c->d_nn, v, b->d_nnG, b->d_nnC, b->d_nnflag,
+                                   b->d_nnF, b->maxN, add_fin, 1, nn_radial(c), s2));
         }
         if ((rc = fork_close(fk))) return rc;
         if (c->have_params) {
@@ -1959,6 +1970,25 @@ int mdt_nn_params_set(mdt_context *ctx, int nelem, const int32_t *elem_of_type, 
     }
     M->nh1 = nh1;
     M->nh2 = nh2;
+    M->radial_only = 1;
+    for (int e = 0; e < nelem; ++e) {
+        if (M->na[e] > 0) M->radial_only = 0;
+        // radial functions grouped by neighbour element (stable)
+        int q = 0;
+        for (int x = 0; x < NN_MAXE; ++x) {
+            M->rk_lo[e][x] = q;
+            for (int k = 0; k < M->nr[e]; ++k) {
+                if (M->r_el[e][k] != x) continue;
+                M->rk_perm[e][q] = k;
+                M->rk_rs[e][q] = M->r_rs[e][k];
+                M->rk_ce[e][q] = (float)(-(double)M->r_eta[e][k] * 1.4426950408889634);
+                M->rk_b[e][q] = -2.0f * M->r_eta[e][k];
+                ++q;
+            }
+            M->rk_hi[e][x] = q;
+        }
+        if (q != M->nr[e]) return bail(MDT_EINVAL, "element %d: radial function on an element outside the model", e);
+    }
     M->rc = (float)rc0;
     M->rc2 = (float)(rc0 * rc0);
     M->pi_rc = (float)(M_PI / rc0);
@@ -2061,8 +2091,8 @@ int mdt_compute_nene(mdt_context *ctx, int natoms, int nbeads, const int32_t *id
     if ((rc = ensure_nn(b))) return rc;
     if ((rc = ensure_ready(b, false))) return rc;
     HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(int) * b->ntraj, ctx->stream));
-    HIPCHK(mdt_launch_nene(b->nrep, mdt_nene_smem(b->maxN), ctx->d_nn, b->view(), b->d_nnG, b->d_nnC, b->d_nnflag,
-                           b->d_nnF, b->maxN, 0, 0, ctx->stream));
+    HIPCHK(mdt_launch_nene(b->nrep, nn_smem(ctx, b->maxN), ctx->d_nn, b->view(), b->d_nnG, b->d_nnC, b->d_nnflag,
+                           b->d_nnF, b->maxN, 0, 0, nn_radial(ctx), ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     std::vector<int> st(1);
     HIPCHK(hipMemcpy(st.data(), b->d_status, sizeof(int), hipMemcpyDeviceToHost));

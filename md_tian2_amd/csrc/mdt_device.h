@@ -217,6 +217,15 @@ struct NnConst {
     float W2[NN_MAXE][NN_H][NN_H], b2[NN_MAXE][NN_H];
     float w3[NN_MAXE][NN_H], b3[NN_MAXE];
     int nh1, nh2;                                       // real hidden widths (<= NN_H)
+    // radial-only networks (no element has angular functions) run
+    // k_nene_rg / k_nene_rf, which visit an element's radial functions
+    // grouped by neighbour element: functions rk_lo[e][x] .. rk_hi[e][x]-1
+    // of that order act on neighbours of element x; per function its input
+    // index, shift, exponent factor -eta log2(e) and derivative factor -2 eta
+    int radial_only;
+    int rk_lo[NN_MAXE][NN_MAXE], rk_hi[NN_MAXE][NN_MAXE];
+    int rk_perm[NN_MAXE][NN_MAXR];
+    float rk_rs[NN_MAXE][NN_MAXR], rk_ce[NN_MAXE][NN_MAXR], rk_b[NN_MAXE][NN_MAXR];
 };
 
 // padded gradient / Adam layout of one element's parameters

@@ -151,16 +151,19 @@ __device__ __forceinline__ uint32_t frac_fix(double f) {
     return (uint32_t)(unsigned long long)(f * 4294967296.0);
 }
 
-// Builds the neighbour lists of centre atoms k0 .. k0+NN_CA-1 (16 lanes per
-// centre; lane `sub` tests atoms n = sub, sub+16, ...; hits are counted,
-// prefix-summed over the 16 lanes and written in (lane, n, image) order).
-// ux, uy, uz: the atoms' fractional coordinates (frac_fix).
-__device__ void build_chunk(float rc2, float pi_rc, ChunkNbr &L, const uint32_t *ux, const uint32_t *uy,
-                            const uint32_t *uz, const uint8_t *elem, const double h[9], const int *imax, float wx,
-                            float wy, float wz, int N, int k0, int lane, int *bad) {
-    const int c = lane / NN_TPA, sub = lane % NN_TPA;
-    const int i = k0 + c;
-    int cnt = 0;
+// Neighbour list of centre i (16 lanes; i < 0: none) within rc over all
+// periodic images: lane `sub` tests atoms n = sub, sub+16, ...; hits are
+// counted, prefix-summed over the 16 lanes and passed to put(pos, n, ex, ey,
+// ez, r2) in (lane, n, image) order (vector in Bohr).  Returns the number of
+// hits (in every lane of the group); more than NN_CAP set the overflow flag
+// and only the first NN_CAP are passed.  ux, uy, uz: the atoms' fractional
+// coordinates (frac_fix).
+template <class Put>
+__device__ int build_images(float rc2, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz, const double h[9],
+                            const int *imax, float wx, float wy, float wz, int N, int i, int lane, int *bad,
+                            Put put) {
+    const int sub = lane % NN_TPA;
+    int cnt = 0, tot = 0;
     for (int pass = 0; pass < 2; ++pass) {
         int base = 0;
         if (pass == 1) {
@@ -172,12 +175,11 @@ __device__ void build_chunk(float rc2, float pi_rc, ChunkNbr &L, const uint32_t 
                 if (sub >= o) incl += y;
             }
             base = incl - cnt;
-            const int tot = __shfl(incl, NN_TPA - 1, NN_TPA);
-            if (sub == 0) L.cnt[c] = tot < NN_CAP ? tot : NN_CAP;
+            tot = __shfl(incl, NN_TPA - 1, NN_TPA);
             if (tot > NN_CAP) *bad |= MDT_ST_NBR_OVERFLOW;
         }
         int pos = base;
-        if (i < N) {
+        if (i >= 0) {
             const long long xi = ux[i], yi = uy[i], zi = uz[i];
             for (int n = sub; n < N; n += NN_TPA) {
                 const double f0 = (double)((long long)ux[n] - xi) * 0x1p-32;  // in (-1, 1)
@@ -200,26 +202,38 @@ __device__ void build_chunk(float rc2, float pi_rc, ChunkNbr &L, const uint32_t 
                             if (pass == 0) {
                                 ++cnt;
                             } else {
-                                if (pos < NN_CAP) {
-                                    const int e = c * NN_CAP + pos;
-                                    const float rr = (float)sqrt(r2);
-                                    float fcv, dfcv;
-                                    cutoff(pi_rc, rr, &fcv, &dfcv);
-                                    L.dx[e] = (float)ex;
-                                    L.dy[e] = (float)ey;
-                                    L.dz[e] = (float)ez;
-                                    L.fc[e] = fcv;
-                                    L.dfc[e] = dfcv;
-                                    L.j[e] = (uint16_t)n;
-                                    L.el[e] = elem[n];
-                                }
+                                if (pos < NN_CAP) put(pos, n, ex, ey, ez, r2);
                                 ++pos;
                             }
                         }
             }
         }
     }
-    if (i >= N && sub == 0) L.cnt[c] = 0;
+    return tot;
+}
+
+// Builds the neighbour lists of centre atoms k0 .. k0+NN_CA-1 (16 lanes per
+// centre) with d, fc, dfc, the neighbour and its element per entry.
+__device__ void build_chunk(float rc2, float pi_rc, ChunkNbr &L, const uint32_t *ux, const uint32_t *uy,
+                            const uint32_t *uz, const uint8_t *elem, const double h[9], const int *imax, float wx,
+                            float wy, float wz, int N, int k0, int lane, int *bad) {
+    const int c = lane / NN_TPA, sub = lane % NN_TPA;
+    const int i = k0 + c;
+    const int tot = build_images(rc2, ux, uy, uz, h, imax, wx, wy, wz, N, i < N ? i : -1, lane, bad,
+                                 [&](int pos, int n, double ex, double ey, double ez, double r2) {
+                                     const int e = c * NN_CAP + pos;
+                                     const float rr = (float)sqrt(r2);
+                                     float fcv, dfcv;
+                                     cutoff(pi_rc, rr, &fcv, &dfcv);
+                                     L.dx[e] = (float)ex;
+                                     L.dy[e] = (float)ey;
+                                     L.dz[e] = (float)ez;
+                                     L.fc[e] = fcv;
+                                     L.dfc[e] = dfcv;
+                                     L.j[e] = (uint16_t)n;
+                                     L.el[e] = elem[n];
+                                 });
+    if (sub == 0) L.cnt[c] = tot < NN_CAP ? tot : NN_CAP;
 }
 
 template <int WIDTH>
@@ -471,10 +485,11 @@ __device__ void g_chunk(const NnConst &Mc, const NnSfTab &S, const ChunkNbr &L, 
             }
         }
     } else {
-        // generic form: every function per triplet
+        // generic form: every function per triplet (no pair loop for an
+        // element without angular functions)
 #pragma unroll
     for (int k = 0; k < NN_MAXA; ++k) g[k] = 0.f;
-    for (int p = sub; p < cnt; p += NN_TPA) {
+    for (int p = sub; p < (na > 0 ? cnt : 0); p += NN_TPA) {
         const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
         float ra, ia;
         nbr_rir(L, base + p, &ra, &ia);
@@ -593,7 +608,7 @@ __device__ void force_chunk(const NnSfTab &S, const ChunkNbr &L, const float (*c
 #pragma unroll
         for (int q = 0; q < NN_FB * NN_FB; ++q) W[q] = Wsh[c][q];
     }
-    for (int p = sub; p < cnt; p += NN_TPA) {
+    for (int p = sub; p < (na > 0 ? cnt : 0); p += NN_TPA) {
         const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
         float ra, ia;
         nbr_rir(L, base + p, &ra, &ia);
@@ -753,7 +768,7 @@ __device__ void jvp_chunk(const NnSfTab &S, const ChunkNbr &L, const uint8_t *el
     }
 #pragma unroll
     for (int k = 0; k < NN_MAXA; ++k) P[k] = 0.f;
-    for (int p = sub; p < cnt; p += NN_TPA) {
+    for (int p = sub; p < (na > 0 ? cnt : 0); p += NN_TPA) {
         const float ax_ = L.dx[base + p], ay_ = L.dy[base + p], az_ = L.dz[base + p];
         float ra, ia;
         nbr_rir(L, base + p, &ra, &ia);
@@ -904,6 +919,99 @@ __device__ void cell_setup(const NnConst &M, const double *cellA, const double *
     }
 }
 
+// Atoms grouped by element, ascending index within each group, into elist
+// (wave 0; the caller synchronises); s_ne[e]: the group sizes.
+__device__ void group_elements(const uint8_t *elem, int N, int ne_model, uint16_t *elist, int *s_ne, int lane) {
+    if (lane >= 64) return;
+    int base = 0;
+    for (int e = 0; e < ne_model; ++e) {
+        const int start = base;
+        for (int c0 = 0; c0 < N; c0 += 64) {
+            const int i = c0 + lane;
+            const bool in = i < N && elem[i] == e;
+            const unsigned long long m = __ballot(in);
+            if (in) elist[base + __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = (uint16_t)i;
+            base += __popcll(m);
+        }
+        if (lane == 0) s_ne[e] = base - start;
+    }
+}
+
+// The network on every atom of the replica (per element, 16-atom tiles, one
+// per wavefront per round: atomic energies to Ea, c = dE/dG rows to Crep)
+// and the replica's energy [eV] to *epot (fixed-order sum).  Every lane of
+// the block calls this (barriers inside).
+__device__ void network_energy(const NnConst &M, const int *s_ne, const uint16_t *elist, int (*rowsb)[16],
+                               float (*T)[16 * 33], const float *Grep, float *Crep, double *Ea, int N, int lane,
+                               double *epot) {
+    const int wave = lane >> 6;
+    const int ne_model = M.nelem;
+    int ntiles = 0;
+    for (int e = 0; e < ne_model; ++e) ntiles += (s_ne[e] + 15) / 16;
+    const int nwaves = NN_BLOCK / 64;
+    for (int r0 = 0; r0 < ntiles; r0 += nwaves) {
+        const int tile = r0 + wave;
+        int e = -1, first = 0, cntE = 0, tl = tile;
+        {
+            int start = 0;
+            for (int q = 0; q < ne_model; ++q) {
+                const int nt = (s_ne[q] + 15) / 16;
+                if (e < 0 && tl < nt) {
+                    e = q;
+                    first = start + tl * 16;
+                    cntE = s_ne[q] - tl * 16;
+                } else if (e < 0) {
+                    tl -= nt;
+                }
+                start += s_ne[q];
+            }
+        }
+        if ((lane & 63) < 16) {
+            const int rr = lane & 15;
+            rowsb[wave][rr] = (e >= 0 && rr < cntE) ? (int)elist[first + rr] : -1;
+        }
+        __syncthreads();
+        mlp_tile(M, e, rowsb[wave], Grep, Crep, Ea, T[wave], lane);
+        __syncthreads();
+    }
+    if (lane < 64) {
+        double e = 0.0;
+        for (int i = lane; i < N; i += 64) e += Ea[i];
+        for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+        if (lane == 0) *epot = e * NN_HA2EV;
+    }
+}
+
+// Outputs of pass F from the fixed-point accumulators: f [eV/A] (+ the
+// forces already in f when add_fin; the network's alone to nv.fpes), a = f/m
+// (fixed components 0), dens = 0 (no EMT density).
+__device__ void write_forces(const BatchView &bv, const NnView &nv, const unsigned long long *ax,
+                             const unsigned long long *ay, const unsigned long long *az, int N, long long roff,
+                             long long doff, long long aoff, int add_fin, int lane) {
+    const double fconv = NN_HA2EV / NN_BOHR2ANG;
+    for (int i = lane; i < N; i += NN_BLOCK) {
+        double F[3] = {(double)(long long)ax[i] * NN_FISCALE * fconv, (double)(long long)ay[i] * NN_FISCALE * fconv,
+                       (double)(long long)az[i] * NN_FISCALE * fconv};
+        nv.fpes[roff + i] = F[0];
+        nv.fpes[roff + N + i] = F[1];
+        nv.fpes[roff + 2 * N + i] = F[2];
+        if (add_fin) {
+            F[0] += bv.f[roff + i];
+            F[1] += bv.f[roff + N + i];
+            F[2] += bv.f[roff + 2 * N + i];
+        }
+        const uint8_t fb = bv.fixed[doff + i];
+        const double mass = bv.mass[aoff + i];
+        bv.f[roff + i] = F[0];
+        bv.f[roff + N + i] = F[1];
+        bv.f[roff + 2 * N + i] = F[2];
+        bv.a[roff + i] = (fb & 1) ? 0.0 : F[0] / mass;
+        bv.a[roff + N + i] = (fb & 2) ? 0.0 : F[1] / mass;
+        bv.a[roff + 2 * N + i] = (fb & 4) ? 0.0 : F[2] / mass;
+        bv.dens[doff + i] = 0.0;
+    }
+}
+
 // Pass G and the network: raw G rows, c = dE/dG rows, the replica's energy
 // and extrapolation flag.  A replica outside the training range leaves
 // with its flag set when skip_extrap (k_nene_f skips it too).
@@ -965,21 +1073,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__
         Ea[i] = 0.0;
     }
     __syncthreads();
-    // atoms grouped by element, ascending index within each group (wave 0)
-    if (wave == 0) {
-        int base = 0;
-        for (int e = 0; e < ne_model; ++e) {
-            const int start = base;
-            for (int c0 = 0; c0 < N; c0 += 64) {
-                const int i = c0 + lane;
-                const bool in = i < N && elem[i] == e;
-                const unsigned long long m = __ballot(in);
-                if (in) elist[base + __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))))] = (uint16_t)i;
-                base += __popcll(m);
-            }
-            if (lane == 0) s_ne[e] = base - start;
-        }
-    }
+    group_elements(elem, N, ne_model, elist, s_ne, lane);
     __syncthreads();
 
     // ---- pass G: symmetry functions ----
@@ -1001,43 +1095,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_g(const NnConst *__restrict__
         return;
     }
 
-    // ---- network: per element, 16-atom tiles, one per wavefront per round ----
-    int ntiles = 0;
-    for (int e = 0; e < ne_model; ++e) ntiles += (s_ne[e] + 15) / 16;
-    const int nwaves = NN_BLOCK / 64;
-    for (int r0 = 0; r0 < ntiles; r0 += nwaves) {
-        const int tile = r0 + wave;
-        int e = -1, first = 0, cntE = 0, tl = tile;
-        {
-            int start = 0;
-            for (int q = 0; q < ne_model; ++q) {
-                const int nt = (s_ne[q] + 15) / 16;
-                if (e < 0 && tl < nt) {
-                    e = q;
-                    first = start + tl * 16;
-                    cntE = s_ne[q] - tl * 16;
-                } else if (e < 0) {
-                    tl -= nt;
-                }
-                start += s_ne[q];
-            }
-        }
-        if ((lane & 63) < 16) {
-            const int rr = lane & 15;
-            rowsb[wave][rr] = (e >= 0 && rr < cntE) ? (int)elist[first + rr] : -1;
-        }
-        __syncthreads();
-        mlp_tile(M, e, rowsb[wave], Grep, Crep, Ea, T[wave], lane);
-        __syncthreads();
-    }
-
-    // ---- energy [eV] (fixed-order sum) ----
-    if (lane < 64) {
-        double e = 0.0;
-        for (int i = lane; i < N; i += 64) e += Ea[i];
-        for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
-        if (lane == 0) bv.epot[bv.traj_eoff[t] + b] = e * NN_HA2EV;
-    }
+    network_energy(M, s_ne, elist, rowsb, T, Grep, Crep, Ea, N, lane, &bv.epot[bv.traj_eoff[t] + b]);
     if (extrap) bad |= MDT_ST_NN_EXTRAPOLATION;
     if (bad) atomicOr(&bv.status[t], bad);
 }
@@ -1104,29 +1162,472 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_f(const NnConst *__restrict__
         __syncthreads();
     }
 
-    // ---- outputs: f [eV/A], a, dens (no EMT density: 0) ----
-    const double fconv = NN_HA2EV / NN_BOHR2ANG;
-    for (int i = lane; i < N; i += NN_BLOCK) {
-        double F[3] = {(double)(long long)ax[i] * NN_FISCALE * fconv, (double)(long long)ay[i] * NN_FISCALE * fconv,
-                       (double)(long long)az[i] * NN_FISCALE * fconv};
-        nv.fpes[roff + i] = F[0];
-        nv.fpes[roff + N + i] = F[1];
-        nv.fpes[roff + 2 * N + i] = F[2];
-        if (add_fin) {
-            F[0] += bv.f[roff + i];
-            F[1] += bv.f[roff + N + i];
-            F[2] += bv.f[roff + 2 * N + i];
+    write_forces(bv, nv, ax, ay, az, N, roff, doff, aoff, add_fin, lane);
+    if (bad) atomicOr(&bv.status[t], bad);
+}
+
+
+// ===========================================================================
+// Radial-only networks (no element has angular functions): k_nene_rg /
+// k_nene_rf run the passes of k_nene_g / k_nene_f shaped for radial sums.
+//  * Centres go in element order, each element's atoms padded to a multiple
+//    of 4, so the 4 centres of a wavefront share one element and every
+//    function parameter is wavefront-uniform (scalar loads, no LDS tables).
+//  * A function term exp2(ce (r - rs)^2) fc(r) is branch-free; the
+//    neighbour-element test is a select, and functions on neighbour
+//    elements that none of the wavefront's current entries has are skipped
+//    (ballot).
+//  * List entries hold r, the neighbour element and, for pass F, the unit
+//    vector and the neighbour; the cut-off is evaluated per entry.
+//  * When every perpendicular width of the cell exceeds 2 rc, the only
+//    image of an atom within rc is the minimum image of the fractional
+//    difference: one pass over the candidates, hits compacted in ascending
+//    atom order with a wavefront ballot (no count pass, no image loops).
+//    Other cells take the image search of build_images.
+//  * Sums over a centre's 16 lanes use DPP row permutations (no LDS).
+// ===========================================================================
+
+struct RadNbrG {  // pass G entries
+    float r[NN_CA * NN_CAP];
+    uint8_t el[NN_CA * NN_CAP];
+    int cnt[NN_CA];
+};
+
+struct RadNbrF {  // pass F entries
+    float r[NN_CA * NN_CAP], ux[NN_CA * NN_CAP], uy[NN_CA * NN_CAP], uz[NN_CA * NN_CAP];
+    uint16_t j[NN_CA * NN_CAP];
+    uint8_t el[NN_CA * NN_CAP];
+    int cnt[NN_CA];
+};
+
+__device__ __forceinline__ void rad_put(RadNbrG &L, int e, int n, float ex, float ey, float ez, float r2,
+                                        const uint8_t *elem) {
+    (void)ex;
+    (void)ey;
+    (void)ez;
+    L.r[e] = r2 * __builtin_amdgcn_rsqf(r2);
+    L.el[e] = elem[n];
+}
+
+__device__ __forceinline__ void rad_put(RadNbrF &L, int e, int n, float ex, float ey, float ez, float r2,
+                                        const uint8_t *elem) {
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    L.r[e] = r2 * ir;
+    L.ux[e] = ex * ir;
+    L.uy[e] = ey * ir;
+    L.uz[e] = ez * ir;
+    L.j[e] = (uint16_t)n;
+    L.el[e] = elem[n];
+}
+
+// sum over the 16 lanes of a DPP row (row mirror, half-row mirror, quad
+// swaps 2 and 1); every lane of the row receives the same sum
+__device__ __forceinline__ float row_sum16(float v) {
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x140, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x141, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x4e, 0xf, 0xf, false));
+    v += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xb1, 0xf, 0xf, false));
+    return v;
+}
+
+// Neighbour list of centre i (16 lanes; i < 0: none) within rc when the cell
+// admits only the minimum image: candidates n = it*16 + sub, hits compacted
+// in ascending n with a wavefront ballot.  hf: the cell in Bohr (FP32,
+// columns = lattice vectors).  Returns the (clamped) count in every lane of
+// the group; more than NN_CAP hits set the overflow flag.
+template <class Lt>
+__device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz,
+                                   const uint8_t *elem, const float hf[9], float rc2, int N, int i, int lane,
+                                   int *bad) {
+    const int c = lane >> 4, sub = lane & 15, sh = lane & 48;
+    const bool act = i >= 0;
+    const uint32_t xi = ux[act ? i : 0], yi = uy[act ? i : 0], zi = uz[act ? i : 0];
+    int cnt = 0;
+    for (int n0 = 0; n0 < N; n0 += NN_TPA) {
+        const int n = n0 + sub;
+        bool hit = false;
+        float ex = 0.f, ey = 0.f, ez = 0.f, r2 = 0.f;
+        if (act && n < N && n != i) {
+            // the wrapped 32-bit difference is the fractional difference in [-1/2, 1/2)
+            const float f0 = (float)(int)(ux[n] - xi) * 0x1p-32f;
+            const float f1 = (float)(int)(uy[n] - yi) * 0x1p-32f;
+            const float f2 = (float)(int)(uz[n] - zi) * 0x1p-32f;
+            ex = hf[0] * f0 + hf[3] * f1 + hf[6] * f2;
+            ey = hf[1] * f0 + hf[4] * f1 + hf[7] * f2;
+            ez = hf[2] * f0 + hf[5] * f1 + hf[8] * f2;
+            r2 = ex * ex + ey * ey + ez * ez;
+            hit = r2 <= rc2;
         }
-        const uint8_t fb = bv.fixed[doff + i];
-        const double mass = bv.mass[aoff + i];
-        bv.f[roff + i] = F[0];
-        bv.f[roff + N + i] = F[1];
-        bv.f[roff + 2 * N + i] = F[2];
-        bv.a[roff + i] = (fb & 1) ? 0.0 : F[0] / mass;
-        bv.a[roff + N + i] = (fb & 2) ? 0.0 : F[1] / mass;
-        bv.a[roff + 2 * N + i] = (fb & 4) ? 0.0 : F[2] / mass;
-        bv.dens[doff + i] = 0.0;
+        const unsigned int row = (unsigned int)(__ballot(hit) >> sh) & 0xffffu;
+        const int pos = cnt + __popc(row & ((1u << sub) - 1u));
+        if (hit && pos < NN_CAP) rad_put(L, c * NN_CAP + pos, n, ex, ey, ez, r2, elem);
+        cnt += __popc(row);
     }
+    if (cnt > NN_CAP) *bad |= MDT_ST_NBR_OVERFLOW;
+    return cnt < NN_CAP ? cnt : NN_CAP;
+}
+
+// The chunk's lists: the minimum-image pass or the image search.
+template <class Lt>
+__device__ void rad_build(Lt &L, bool min_image, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz,
+                          const uint8_t *elem, const float hf[9], const double h[9], const int *imax, const float w[3],
+                          float rc2, int N, int i, int lane, int *bad) {
+    const int c = lane >> 4;
+    int cnt;
+    if (min_image) {
+        cnt = rad_build_min_image(L, ux, uy, uz, elem, hf, rc2, N, i, lane, bad);
+    } else {
+        cnt = build_images(rc2, ux, uy, uz, h, imax, w[0], w[1], w[2], N, i, lane, bad,
+                           [&](int pos, int n, double ex, double ey, double ez, double r2) {
+                               rad_put(L, c * NN_CAP + pos, n, (float)ex, (float)ey, (float)ez, (float)r2, elem);
+                           });
+        cnt = cnt < NN_CAP ? cnt : NN_CAP;
+    }
+    if ((lane & 15) == 0) L.cnt[c] = cnt;
+}
+
+// the largest list count of the wavefront's 4 centres (uniform)
+__device__ __forceinline__ int wave_max_cnt(int cnt) {
+    const int a = max(__builtin_amdgcn_readlane(cnt, 0), __builtin_amdgcn_readlane(cnt, 16));
+    const int b = max(__builtin_amdgcn_readlane(cnt, 32), __builtin_amdgcn_readlane(cnt, 48));
+    return max(a, b);
+}
+
+// Centre schedule of the radial kernels: each element's atoms (elist
+// order) padded to a multiple of 4, entries (e << 16) | atom (atom 0xffff
+// for padding).  All lanes of the block; returns the slot count.
+__device__ int make_slots(const uint16_t *elist, const int *s_ne, int ne_model, uint32_t *slot, int lane) {
+    int so = 0, start = 0;
+    for (int e = 0; e < ne_model; ++e) {
+        const int ne = s_ne[e], padded = (ne + 3) & ~3;
+        for (int q = lane; q < padded; q += NN_BLOCK)
+            slot[so + q] = ((uint32_t)e << 16) | (q < ne ? (uint32_t)elist[start + q] : 0xffffu);
+        so += padded;
+        start += ne;
+    }
+    return so;
+}
+
+#define NN_RQ (NN_CAP / NN_TPA)  // list entries per lane (6)
+
+// The lane's list entries p = sub + 16 q into registers: r (rc past the
+// count), the neighbour element (-1 past the count); *nq: the wavefront's
+// largest number of entries per lane (uniform), *present: bit x set when
+// some entry of the wavefront has element x (uniform).
+template <class Lt>
+__device__ __forceinline__ void rad_entries(const Lt &L, int lane, float rc, int nel, float rr[NN_RQ],
+                                            int eq[NN_RQ], int *nq, unsigned int *present) {
+    const int c = lane >> 4, sub = lane & 15;
+    const int cnt = L.cnt[c], base = c * NN_CAP;
+    *nq = (wave_max_cnt(cnt) + NN_TPA - 1) / NN_TPA;
+#pragma unroll
+    for (int q = 0; q < NN_RQ; ++q) {
+        const int p = sub + NN_TPA * q;
+        const bool v = p < cnt;
+        rr[q] = v ? L.r[base + p] : rc;
+        eq[q] = v ? (int)L.el[base + p] : -1;
+    }
+    unsigned int pr = 0;
+    for (int x = 0; x < nel; ++x) {
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < NN_RQ; ++q) any |= eq[q] == x;
+        if (__ballot(any)) pr |= 1u << x;
+    }
+    *present = pr;
+}
+
+// Radial symmetry functions of the chunk's centres (atom i of the lane's
+// group, element e uniform over the wavefront) into Grep rows; *extrapp set
+// when a value leaves [min - 1e-4, max + 1e-4].  Functions outer (their
+// parameters loaded once, scalar), the lane's entries inner; each function's
+// 16-lane sum goes to the centre's row.
+__device__ void rg_chunk(const NnConst &M, const RadNbrG &L, int e, int i, int lane, float *Grep, int *extrapp) {
+    const int sub = lane & 15;
+    const int nel = M.nelem;
+    const float pi_rc = M.pi_rc;
+    float rr[NN_RQ], fcq[NN_RQ];
+    int eq[NN_RQ], nq;
+    unsigned int present;
+    rad_entries(L, lane, M.rc, nel, rr, eq, &nq, &present);
+#pragma unroll
+    for (int q = 0; q < NN_RQ; ++q) {
+        float fc = 0.f, dfc;
+        if (q < nq) cutoff(pi_rc, rr[q], &fc, &dfc);
+        fcq[q] = eq[q] >= 0 ? fc : 0.f;
+    }
+    int extrap = 0;
+    for (int x = 0; x < nel; ++x) {
+        const int lo = M.rk_lo[e][x], hi = M.rk_hi[e][x];
+        if (lo == hi || !((present >> x) & 1u)) {
+            // no neighbour of element x: those functions are 0
+            for (int k = lo; k < hi; ++k) {
+                const int ko = M.rk_perm[e][k];
+                if (i >= 0 && sub == (k & (NN_TPA - 1))) {
+                    Grep[(size_t)i * NN_K1 + ko] = 0.f;
+                    if (0.f < M.glo[e][ko] || 0.f > M.ghi[e][ko]) extrap = 1;
+                }
+            }
+            continue;
+        }
+        float fx[NN_RQ];
+#pragma unroll
+        for (int q = 0; q < NN_RQ; ++q) fx[q] = eq[q] == x ? fcq[q] : 0.f;
+        for (int k = lo; k < hi; ++k) {
+            const float rs = M.rk_rs[e][k], ce = M.rk_ce[e][k];
+            float acc = 0.f;
+#pragma unroll
+            for (int q = 0; q < NN_RQ; ++q) {
+                if (q < nq) {
+                    const float dr = rr[q] - rs;
+                    acc = fmaf(__builtin_amdgcn_exp2f(ce * dr * dr), fx[q], acc);
+                }
+            }
+            const float v = row_sum16(acc);
+            const int ko = M.rk_perm[e][k];
+            if (i >= 0 && sub == (k & (NN_TPA - 1))) {
+                Grep[(size_t)i * NN_K1 + ko] = v;
+                if (v < M.glo[e][ko] || v > M.ghi[e][ko]) extrap = 1;
+            }
+        }
+    }
+    if (extrap) *extrapp = 1;
+}
+
+// dE/dr of the chunk's centres' radial functions with coefficients crow =
+// dE/dG of the lane's centre (input order): +F on the centre (row sum), -F
+// on each neighbour into the fixed-point accumulators (Hartree/Bohr).
+__device__ void rf_chunk(const NnConst &M, const RadNbrF &L, const float *crow, int e, int i, int lane,
+                         unsigned long long *ax, unsigned long long *ay, unsigned long long *az, int *badp) {
+    const int c = lane >> 4, sub = lane & 15;
+    const int base = c * NN_CAP;
+    const int nel = M.nelem;
+    const float pi_rc = M.pi_rc;
+    float rr[NN_RQ], fcq[NN_RQ], dfcq[NN_RQ], sq[NN_RQ];
+    int eq[NN_RQ], nq;
+    unsigned int present;
+    rad_entries(L, lane, M.rc, nel, rr, eq, &nq, &present);
+#pragma unroll
+    for (int q = 0; q < NN_RQ; ++q) {
+        float fc = 0.f, dfc = 0.f;
+        if (q < nq) cutoff(pi_rc, rr[q], &fc, &dfc);
+        fcq[q] = eq[q] >= 0 ? fc : 0.f;
+        dfcq[q] = eq[q] >= 0 ? dfc : 0.f;
+        sq[q] = 0.f;
+    }
+    // s_q = sum_k c_k dG_k/dr, dG_k/dr = exp(-eta (r - rs)^2) (dfc - 2 eta (r - rs) fc)
+    for (int x = 0; x < nel; ++x) {
+        const int lo = M.rk_lo[e][x], hi = M.rk_hi[e][x];
+        if (lo == hi || !((present >> x) & 1u)) continue;
+        float fx[NN_RQ], dx[NN_RQ];
+#pragma unroll
+        for (int q = 0; q < NN_RQ; ++q) {
+            fx[q] = eq[q] == x ? fcq[q] : 0.f;
+            dx[q] = eq[q] == x ? dfcq[q] : 0.f;
+        }
+        for (int k = lo; k < hi; ++k) {
+            const float rs = M.rk_rs[e][k], ce = M.rk_ce[e][k], bk = M.rk_b[e][k];
+            const float ck = crow[M.rk_perm[e][k]];
+#pragma unroll
+            for (int q = 0; q < NN_RQ; ++q) {
+                if (q < nq) {
+                    const float dr = rr[q] - rs;
+                    const float t = __builtin_amdgcn_exp2f(ce * dr * dr) * ck;
+                    sq[q] = fmaf(t, fmaf(bk * dr, fx[q], dx[q]), sq[q]);
+                }
+            }
+        }
+    }
+    float F0 = 0.f, F1 = 0.f, F2 = 0.f;
+    int bad = 0;
+#pragma unroll
+    for (int q = 0; q < NN_RQ; ++q) {
+        if (q < nq && eq[q] >= 0) {
+            const int e2 = base + sub + NN_TPA * q;
+            const float v0 = sq[q] * L.ux[e2], v1 = sq[q] * L.uy[e2], v2 = sq[q] * L.uz[e2];
+            F0 += v0;
+            F1 += v1;
+            F2 += v2;
+            const int j = L.j[e2];
+            fxadd(&ax[j], -(double)v0, &bad);
+            fxadd(&ay[j], -(double)v1, &bad);
+            fxadd(&az[j], -(double)v2, &bad);
+        }
+    }
+    const float S0 = row_sum16(F0), S1 = row_sum16(F1), S2 = row_sum16(F2);
+    if (i >= 0 && sub == 0) {
+        fxadd(&ax[i], (double)S0, &bad);
+        fxadd(&ay[i], (double)S1, &bad);
+        fxadd(&az[i], (double)S2, &bad);
+    }
+    if (bad) *badp |= bad;
+}
+
+// Pass G and the network of a radial-only model (k_nene_g's contract).
+__global__ void __launch_bounds__(NN_BLOCK) k_nene_rg(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
+                                                      int skip_extrap) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ union {
+        RadNbrG L;
+        float T[NN_BLOCK / 64][16 * 33];
+    } U;
+    RadNbrG &L = U.L;
+    float(*T)[16 * 33] = U.T;
+    __shared__ int rowsb[NN_BLOCK / 64][16];
+    __shared__ int s_ne[NN_MAXE];
+    const NnConst &M = *Mg;
+    const int rep = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;  // size-class launches
+    const int t = bv.rep_traj[rep];
+    if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
+    const int b = bv.rep_bead[rep];
+    const int N = bv.traj_N[t];
+    const long long soff = bv.traj_soff[t];
+    const long long roff = soff + (long long)b * 3 * N;
+    const long long aoff = bv.traj_aoff[t];
+    const int lane = threadIdx.x;
+    float *Grep = nv.G + (size_t)rep * nv.maxN * NN_K1;
+    float *Crep = nv.C + (size_t)rep * nv.maxN * NN_K1;
+    int bad = 0, extrap = 0;
+
+    // dynamic LDS: Ea (double N) | ux uy uz (u32 N each) | slot (u32 N + 4 NN_MAXE) | elist (u16 N) | elem (u8 N)
+    double *Ea = (double *)smem_raw;
+    uint32_t *ux = (uint32_t *)(Ea + N), *uy = ux + N, *uz = uy + N;
+    uint32_t *slot = uz + N;
+    uint16_t *elist = (uint16_t *)(slot + N + 4 * NN_MAXE);
+    uint8_t *elem = (uint8_t *)(elist + N);
+
+    const int ne_model = M.nelem;
+    double h[9], ih[9];
+    float w[3];
+    int imax[3];
+    cell_setup(M, bv.cell + 9 * t, bv.icell + 9 * t, h, ih, w, imax);
+    float hf[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) hf[k] = (float)h[k];
+    const bool min_image = w[0] < 0.4999f && w[1] < 0.4999f && w[2] < 0.4999f;
+
+    for (int i = lane; i < N; i += NN_BLOCK) {
+        const int ty = bv.type[aoff + i];
+        const int e = M.elem_of_type[ty];
+        if (e < 0 || e >= ne_model) bad |= MDT_ST_BAD_PROP;
+        elem[i] = (uint8_t)(e < 0 ? 0 : e);
+        const double r0 = bv.x[roff + i], r1 = bv.x[roff + N + i], r2 = bv.x[roff + 2 * N + i];
+        double f0 = ih[0] * r0 + ih[3] * r1 + ih[6] * r2;
+        double f1 = ih[1] * r0 + ih[4] * r1 + ih[7] * r2;
+        double f2 = ih[2] * r0 + ih[5] * r1 + ih[8] * r2;
+        ux[i] = frac_fix(f0 - floor(f0));
+        uy[i] = frac_fix(f1 - floor(f1));
+        uz[i] = frac_fix(f2 - floor(f2));
+        Ea[i] = 0.0;
+    }
+    __syncthreads();
+    group_elements(elem, N, ne_model, elist, s_ne, lane);
+    __syncthreads();
+    const int nslot = make_slots(elist, s_ne, ne_model, slot, lane);
+    __syncthreads();
+
+    // ---- pass G: symmetry functions ----
+    for (int k0 = 0; k0 < nslot; k0 += NN_CA) {
+        const int sl = k0 + (lane >> 4);
+        const uint32_t sv = sl < nslot ? slot[sl] : 0xffffu;
+        const int i = (sv & 0xffffu) == 0xffffu ? -1 : (int)(sv & 0xffffu);
+        const int e = __builtin_amdgcn_readfirstlane((int)(sv >> 16));
+        rad_build(L, min_image, ux, uy, uz, elem, hf, h, imax, w, M.rc2, N, i, lane, &bad);
+        __syncthreads();
+        rg_chunk(M, L, e, i, lane, Grep, &extrap);
+        __syncthreads();
+    }
+
+    // replicas outside the training range go to the EMT fallback untouched
+    extrap = __syncthreads_or(extrap);
+    if (lane == 0) nv.rep_flag[rep] = extrap;
+    if (extrap && skip_extrap) {
+        if (lane == 0)
+            atomicOr(&bv.status[t], bad | MDT_ST_NN_EXTRAPOLATION);
+        else if (bad)
+            atomicOr(&bv.status[t], bad);
+        return;
+    }
+    network_energy(M, s_ne, elist, rowsb, T, Grep, Crep, Ea, N, lane, &bv.epot[bv.traj_eoff[t] + b]);
+    if (extrap) bad |= MDT_ST_NN_EXTRAPOLATION;
+    if (bad) atomicOr(&bv.status[t], bad);
+}
+
+// Pass F and the outputs of a radial-only model (k_nene_f's contract).
+__global__ void __launch_bounds__(NN_BLOCK) k_nene_rf(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
+                                                      int add_fin, int skip_extrap) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ RadNbrF L;
+    __shared__ float cb[NN_CA][NN_K1];
+    __shared__ int s_ne[NN_MAXE];
+    const NnConst &M = *Mg;
+    const int rep = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;  // size-class launches
+    if (skip_extrap && nv.rep_flag[rep]) return;  // whole block: uniform
+    const int t = bv.rep_traj[rep];
+    if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
+    const int b = bv.rep_bead[rep];
+    const int N = bv.traj_N[t];
+    const long long soff = bv.traj_soff[t];
+    const long long roff = soff + (long long)b * 3 * N;
+    const long long doff = soff / 3 + (long long)b * N;
+    const long long aoff = bv.traj_aoff[t];
+    const int lane = threadIdx.x;
+    const float *Crep = nv.C + (size_t)rep * nv.maxN * NN_K1;
+    int bad = 0;
+
+    // dynamic LDS: ax ay az (u64 N each) | ux uy uz (u32 N) | slot (u32 N + 4 NN_MAXE) | elist (u16 N) | elem (u8 N)
+    unsigned long long *ax = (unsigned long long *)smem_raw, *ay = ax + N, *az = ay + N;
+    uint32_t *ux = (uint32_t *)(az + N), *uy = ux + N, *uz = uy + N;
+    uint32_t *slot = uz + N;
+    uint16_t *elist = (uint16_t *)(slot + N + 4 * NN_MAXE);
+    uint8_t *elem = (uint8_t *)(elist + N);
+
+    const int ne_model = M.nelem;
+    double h[9], ih[9];
+    float w[3];
+    int imax[3];
+    cell_setup(M, bv.cell + 9 * t, bv.icell + 9 * t, h, ih, w, imax);
+    float hf[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) hf[k] = (float)h[k];
+    const bool min_image = w[0] < 0.4999f && w[1] < 0.4999f && w[2] < 0.4999f;
+    for (int i = lane; i < N; i += NN_BLOCK) {
+        const int e = M.elem_of_type[bv.type[aoff + i]];
+        elem[i] = (uint8_t)(e < 0 || e >= ne_model ? 0 : e);
+        const double r0 = bv.x[roff + i], r1 = bv.x[roff + N + i], r2 = bv.x[roff + 2 * N + i];
+        double f0 = ih[0] * r0 + ih[3] * r1 + ih[6] * r2;
+        double f1 = ih[1] * r0 + ih[4] * r1 + ih[7] * r2;
+        double f2 = ih[2] * r0 + ih[5] * r1 + ih[8] * r2;
+        ux[i] = frac_fix(f0 - floor(f0));
+        uy[i] = frac_fix(f1 - floor(f1));
+        uz[i] = frac_fix(f2 - floor(f2));
+        ax[i] = ay[i] = az[i] = 0ull;
+    }
+    __syncthreads();
+    group_elements(elem, N, ne_model, elist, s_ne, lane);
+    __syncthreads();
+    const int nslot = make_slots(elist, s_ne, ne_model, slot, lane);
+    __syncthreads();
+
+    // ---- pass F: forces ----
+    for (int k0 = 0; k0 < nslot; k0 += NN_CA) {
+        const int sl = k0 + (lane >> 4);
+        const uint32_t sv = sl < nslot ? slot[sl] : 0xffffu;
+        const int i = (sv & 0xffffu) == 0xffffu ? -1 : (int)(sv & 0xffffu);
+        const int e = __builtin_amdgcn_readfirstlane((int)(sv >> 16));
+        rad_build(L, min_image, ux, uy, uz, elem, hf, h, imax, w, M.rc2, N, i, lane, &bad);
+        for (int q = lane; q < NN_CA * NN_K1; q += NN_BLOCK) {
+            const int ci = q / NN_K1, k = q % NN_K1;
+            const int s2 = k0 + ci;
+            const uint32_t v2 = s2 < nslot ? slot[s2] : 0xffffu;
+            const int ii = (int)(v2 & 0xffffu);
+            cb[ci][k] = (ii != 0xffff && k < M.nr[v2 >> 16]) ? Crep[(size_t)ii * NN_K1 + k] : 0.f;
+        }
+        __syncthreads();
+        rf_chunk(M, L, cb[lane >> 4], e, i, lane, ax, ay, az, &bad);
+        __syncthreads();
+    }
+    write_forces(bv, nv, ax, ay, az, N, roff, doff, aoff, add_fin, lane);
     if (bad) atomicOr(&bv.status[t], bad);
 }
 
@@ -1614,23 +2115,33 @@ hipError_t mdt_launch_nn_adam(NnConst *M, const float *partials, int W, float *g
 // the larger of k_nene_g's (23 B per atom) and k_nene_f's (37 B per atom)
 size_t mdt_nene_smem(int N) { return (size_t)N * (3 * 8 + 3 * 4 + 1) + 16; }
 
+// k_nene_rg: 27 B per atom, k_nene_rf: 43 B per atom (+ the slot padding)
+size_t mdt_nene_radial_smem(int N) { return (size_t)N * (3 * 8 + 3 * 4 + 4 + 2 + 1) + 4 * 4 * NN_MAXE + 16; }
+
 // static LDS: the larger of the two kernels'
-hipError_t mdt_nene_attributes(hipFuncAttributes *fa) {
+hipError_t mdt_nene_attributes(hipFuncAttributes *fa, int radial) {
     hipFuncAttributes g, f;
-    hipError_t e = hipFuncGetAttributes(&g, (const void *)k_nene_g);
+    hipError_t e = hipFuncGetAttributes(&g, radial ? (const void *)k_nene_rg : (const void *)k_nene_g);
     if (e != hipSuccess) return e;
-    e = hipFuncGetAttributes(&f, (const void *)k_nene_f);
+    e = hipFuncGetAttributes(&f, radial ? (const void *)k_nene_rf : (const void *)k_nene_f);
     if (e != hipSuccess) return e;
     *fa = g.sharedSizeBytes >= f.sharedSizeBytes ? g : f;
     return hipSuccess;
 }
 
 hipError_t mdt_launch_nene(int nrep, size_t smem, const NnConst *M, const BatchView &bv, float *G, float *C,
-                           int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, hipStream_t st) {
+                           int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, int radial,
+                           hipStream_t st) {
     NnView nv{G, C, rep_flag, fpes, maxN};
-    hipLaunchKernelGGL(k_nene_g, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, skip_extrap);
+    if (radial)
+        hipLaunchKernelGGL(k_nene_rg, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, skip_extrap);
+    else
+        hipLaunchKernelGGL(k_nene_g, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, skip_extrap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_nene_f, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, add_fin, skip_extrap);
+    if (radial)
+        hipLaunchKernelGGL(k_nene_rf, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, add_fin, skip_extrap);
+    else
+        hipLaunchKernelGGL(k_nene_f, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, add_fin, skip_extrap);
     return hipGetLastError();
 }

@@ -281,6 +281,46 @@ def hau_design(rc_ang=6.5, hidden=(32, 32)):
     return m
 
 
+def hau_radial_design(rc=6.5, hidden=(32, 32), nh=32, nau=24, nauh=8):
+    """A radial-only H/Au surrogate: nh radial symmetry functions on H and
+    nau + nauh on Au, cut-off rc (A), network n-32-32-1 tanh/tanh/linear.
+
+    EMT's atomic energy is a function of distance sums only: the neighbour
+    density sigma_1 and the pair sum sigma_2 are sums over neighbours of
+    chi exp(-eta (r - beta s0)) theta(r) and chi exp(-kappa/beta (r - beta
+    s0)) theta(r) per neighbour element (src/pes_emt_mod.f90:646-763), and
+    E_i is built from them alone (:793-976).  A radial fingerprint of each
+    neighbour element therefore holds everything the EMT energy depends on,
+    and angular functions add nothing for an EMT surrogate.
+
+    H:  nh on Au neighbours, a quarter centred (eta 0.002-0.5 Bohr^-2), the
+        rest shifted Gaussians over 2 to 11.5 rc / 6.5 Bohr;
+    Au: nau on Au, a quarter centred (eta 0.002-0.2), the rest shifted over
+        4 to 11.5 rc / 6.5 Bohr; nauh centred on H (eta 0.003-1).
+    The defaults (32, 24 + 8, 6.5 A) are md_tian2_amd/nn/hau_radial.
+    """
+    H, AU = ATOMIC_NUMBER["H"], ATOMIC_NUMBER["Au"]
+    rcb = rc / BOHR2ANG
+    top = 11.5 * rc / 6.5  # Bohr
+
+    def shifted(lo, n):
+        sh = np.linspace(lo, top, n)
+        return [SymFunc(2, AU, 0, 1.0 / (2 * (sh[1] - sh[0]) ** 2), rs, 0, 0, rcb) for rs in sh]
+
+    sf_h = [SymFunc(2, AU, 0, eta, 0.0, 0, 0, rcb) for eta in np.geomspace(0.002, 0.5, nh // 4)]
+    sf_h += shifted(2.0, nh - nh // 4)
+    sf_au = [SymFunc(2, AU, 0, eta, 0.0, 0, 0, rcb) for eta in np.geomspace(0.002, 0.2, nau // 4)]
+    sf_au += shifted(4.0, nau - nau // 4)
+    sf_au += [SymFunc(2, H, 0, eta, 0.0, 0, 0, rcb) for eta in np.geomspace(0.003, 1.0, nauh)]
+    m = NeneModel([H, AU], {H: sf_h, AU: sf_au}, list(hidden), "ttl", scale=True, center=False,
+                  scmin=0.0, scmax=1.0)
+    m.sort()
+    return m
+
+
+DESIGNS = {"hau": hau_design, "radial": hau_radial_design}
+
+
 def init_weights(m, rng):
     """Uniform +-sqrt(6/(fan_in + fan_out)) weights, zero biases."""
     for z in m.elements:

@@ -13,10 +13,16 @@ so that the H visits the whole approach / bounce / penetration range.
         (two layers above and below within the cut-off):
         python -m md_tian2_amd.surrogate train --frames 200000 --layers 4,6 --tsurf 100,500 \
             --out md_tian2_amd/nn/hau_thick
+    md_tian2_amd/nn/hau_radial the same frames, radial symmetry functions
+        only (nene.hau_radial_design: EMT energies depend on distance sums
+        alone):
+        python -m md_tian2_amd.surrogate train --design radial --frames 200000 --layers 4,6 \
+            --tsurf 100,500 --out md_tian2_amd/nn/hau_radial
 
 Energies and forces are the GPU EMT path's (FP64).  The network is
 nene.hau_design (24 radial + 16 angular symmetry functions per element,
-6.5 A cut-off, 40-32-32-1), trained by the library's own GPU kernels (BF16
+6.5 A cut-off, 40-32-32-1) or nene.hau_radial_design (32 radial, 6.5 A,
+32-32-32-1), trained by the library's own GPU kernels (BF16
 MFMA, Adam) on energy + force loss.
 """
 from __future__ import annotations
@@ -34,6 +40,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PES_HAU = os.path.join(HERE, "pes", "EMT-HAu.pes")
 MODEL_DIR = os.path.join(HERE, "nn", "hau")
 MODEL_DIR_THICK = os.path.join(HERE, "nn", "hau_thick")
+MODEL_DIR_RADIAL = os.path.join(HERE, "nn", "hau_radial")
 
 
 def generate_frames(ctx, ntraj=2000, nsteps=1000, every=10, seed=7, n=3, nlayers=4, tsurf=(100.0, 500.0), log=None):
@@ -73,12 +80,12 @@ def write_input_data(path, template, R, E, F):
                                                   np.array([E[k]])))
 
 
-def train(ctx, sets, epochs=60, batch_frames=64, lr=2e-3, wE=1.0, wF=10.0, seed=11, log=None):
+def train(ctx, sets, epochs=60, batch_frames=64, lr=2e-3, wE=1.0, wF=10.0, seed=11, log=None, design="hau"):
     """Designs, scales and trains the H/Au network on the GPU from frame
     sets [(template System, R, E, F), ...] (generate_frames's output, one
     per slab); returns (model, history) with the trained weights in
-    model.weights."""
-    m = nene.hau_design()
+    model.weights.  design: a key of nene.DESIGNS."""
+    m = nene.DESIGNS[design]()
     for z in m.elements:
         m.gmin[z], m.gmax[z], m.gavg[z] = np.zeros(m.nsf(z)), np.ones(m.nsf(z)), np.zeros(m.nsf(z))
     nene.init_weights(m, np.random.default_rng(seed))
@@ -132,6 +139,8 @@ def main(argv=None):
     t.add_argument("--layers", default="4", help="slab layer counts of the frame sets (3x3 cells)")
     t.add_argument("--tsurf", default="300", help="Tsurf [K], or a range lo,hi drawn per trajectory")
     t.add_argument("--out", default=MODEL_DIR)
+    t.add_argument("--design", default="hau", choices=sorted(nene.DESIGNS),
+                   help="symmetry functions: hau (radial + angular) or radial (radial only)")
     t.add_argument("--input-data", default=None,
                    help="also write the EMT frames as RuNNer input.data files (<path>.<layers>)")
     a = ap.parse_args(argv)
@@ -152,7 +161,7 @@ def main(argv=None):
     if a.input_data:
         for nl, st in zip(layers, sets):
             write_input_data(f"{a.input_data}.{nl}", *st)
-    m, hist = train(ctx, sets, epochs=a.epochs, batch_frames=a.batch, lr=a.lr, log=sys.stderr)
+    m, hist = train(ctx, sets, epochs=a.epochs, batch_frames=a.batch, lr=a.lr, log=sys.stderr, design=a.design)
     nene.write(m, a.out)
     for h in hist:
         print(f"[surrogate] {h[0]}: train {h[1] * 1e3:.2f} meV/atom {h[2] * 1e3:.1f} meV/A; "
