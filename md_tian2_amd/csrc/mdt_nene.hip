@@ -1267,17 +1267,55 @@ __device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy
     return cnt < NN_CAP ? cnt : NN_CAP;
 }
 
+// The replica's cell for the list builds, kept in LDS so that its registers
+// are live only during a build: h in Bohr (FP64 and FP32), the fractional
+// half-widths of the cut-off sphere, the image bounds, and whether the
+// minimum image alone suffices (every half-width below 1/2).
+struct RadCell {
+    double h[9];
+    float hf[9], w[3];
+    int imax[3];
+    int min_image;
+};
+
+__device__ void rad_cell(RadCell &C, const NnConst &M, const double *cellA, const double *icellA, double ih[9],
+                         int lane) {
+    double h[9];
+    float w[3];
+    int imax[3];
+    cell_setup(M, cellA, icellA, h, ih, w, imax);
+    if (lane == 0) {
+        for (int k = 0; k < 9; ++k) {
+            C.h[k] = h[k];
+            C.hf[k] = (float)h[k];
+        }
+        for (int k = 0; k < 3; ++k) {
+            C.w[k] = w[k];
+            C.imax[k] = imax[k];
+        }
+        C.min_image = w[0] < 0.4999f && w[1] < 0.4999f && w[2] < 0.4999f;
+    }
+}
+
 // The chunk's lists: the minimum-image pass or the image search.
 template <class Lt>
-__device__ void rad_build(Lt &L, bool min_image, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz,
-                          const uint8_t *elem, const float hf[9], const double h[9], const int *imax, const float w[3],
-                          float rc2, int N, int i, int lane, int *bad) {
+__device__ void rad_build(Lt &L, const RadCell &C, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz,
+                          const uint8_t *elem, float rc2, int N, int i, int lane, int *bad) {
     const int c = lane >> 4;
     int cnt;
-    if (min_image) {
+    if (__builtin_amdgcn_readfirstlane(C.min_image)) {
+        float hf[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) hf[k] = C.hf[k];
         cnt = rad_build_min_image(L, ux, uy, uz, elem, hf, rc2, N, i, lane, bad);
     } else {
-        cnt = build_images(rc2, ux, uy, uz, h, imax, w[0], w[1], w[2], N, i, lane, bad,
+        double h[9];
+        int imax[3];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) h[k] = C.h[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) imax[k] = C.imax[k];
+        cnt = build_images(rc2, ux, uy, uz, h, imax, C.w[0], C.w[1], C.w[2], N, i, lane, bad,
                            [&](int pos, int n, double ex, double ey, double ez, double r2) {
                                rad_put(L, c * NN_CAP + pos, n, (float)ex, (float)ey, (float)ez, (float)r2, elem);
                            });
@@ -1337,14 +1375,40 @@ __device__ __forceinline__ void rad_entries(const Lt &L, int lane, float rc, int
     *present = pr;
 }
 
+// Functions lo .. hi-1 (kernel order, one neighbour element) of element e
+// on the lane's first NQ entries: each function's 16-lane sum lands in the
+// lane that owns it (kernel index sub -> g0, sub + 16 -> g1).  The next
+// function's parameters load while the current one is evaluated.
+template <int NQ>
+__device__ __forceinline__ void rg_class(const NnConst &M, int e, int lo, int hi, const float rr[NN_RQ],
+                                         const float fx[NN_RQ], int sub, float &g0, float &g1) {
+    float rs = M.rk_rs[e][lo], ce = M.rk_ce[e][lo];
+    for (int k = lo; k < hi; ++k) {
+        const int kn = k + 1 < hi ? k + 1 : k;
+        const float rsn = M.rk_rs[e][kn], cen = M.rk_ce[e][kn];
+        float acc = 0.f;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const float dr = rr[q] - rs;
+            acc = fmaf(__builtin_amdgcn_exp2f(ce * dr * dr), fx[q], acc);
+        }
+        const float v = row_sum16(acc);
+        const bool mine = sub == (k & (NN_TPA - 1));
+        g0 = (mine && k < NN_TPA) ? v : g0;
+        g1 = (mine && k >= NN_TPA) ? v : g1;
+        rs = rsn;
+        ce = cen;
+    }
+}
+
 // Radial symmetry functions of the chunk's centres (atom i of the lane's
 // group, element e uniform over the wavefront) into Grep rows; *extrapp set
 // when a value leaves [min - 1e-4, max + 1e-4].  Functions outer (their
-// parameters loaded once, scalar), the lane's entries inner; each function's
-// 16-lane sum goes to the centre's row.
+// parameters scalar), the lane's entries inner, instantiated for the
+// wavefront's number of entries per lane.
 __device__ void rg_chunk(const NnConst &M, const RadNbrG &L, int e, int i, int lane, float *Grep, int *extrapp) {
     const int sub = lane & 15;
-    const int nel = M.nelem;
+    const int nel = M.nelem, nr = M.nr[e];
     const float pi_rc = M.pi_rc;
     float rr[NN_RQ], fcq[NN_RQ];
     int eq[NN_RQ], nq;
@@ -1352,46 +1416,67 @@ __device__ void rg_chunk(const NnConst &M, const RadNbrG &L, int e, int i, int l
     rad_entries(L, lane, M.rc, nel, rr, eq, &nq, &present);
 #pragma unroll
     for (int q = 0; q < NN_RQ; ++q) {
-        float fc = 0.f, dfc;
-        if (q < nq) cutoff(pi_rc, rr[q], &fc, &dfc);
+        float fc, dfc;
+        cutoff(pi_rc, rr[q], &fc, &dfc);
         fcq[q] = eq[q] >= 0 ? fc : 0.f;
     }
-    int extrap = 0;
+    float g0 = 0.f, g1 = 0.f;  // functions of no present neighbour element stay 0
     for (int x = 0; x < nel; ++x) {
         const int lo = M.rk_lo[e][x], hi = M.rk_hi[e][x];
-        if (lo == hi || !((present >> x) & 1u)) {
-            // no neighbour of element x: those functions are 0
-            for (int k = lo; k < hi; ++k) {
-                const int ko = M.rk_perm[e][k];
-                if (i >= 0 && sub == (k & (NN_TPA - 1))) {
-                    Grep[(size_t)i * NN_K1 + ko] = 0.f;
-                    if (0.f < M.glo[e][ko] || 0.f > M.ghi[e][ko]) extrap = 1;
-                }
-            }
-            continue;
-        }
+        if (lo == hi || !((present >> x) & 1u)) continue;
         float fx[NN_RQ];
 #pragma unroll
         for (int q = 0; q < NN_RQ; ++q) fx[q] = eq[q] == x ? fcq[q] : 0.f;
-        for (int k = lo; k < hi; ++k) {
-            const float rs = M.rk_rs[e][k], ce = M.rk_ce[e][k];
-            float acc = 0.f;
+        switch (nq) {
+            case 1: rg_class<1>(M, e, lo, hi, rr, fx, sub, g0, g1); break;
+            case 2: rg_class<2>(M, e, lo, hi, rr, fx, sub, g0, g1); break;
+            case 3: rg_class<3>(M, e, lo, hi, rr, fx, sub, g0, g1); break;
+            case 4: rg_class<4>(M, e, lo, hi, rr, fx, sub, g0, g1); break;
+            case 5: rg_class<5>(M, e, lo, hi, rr, fx, sub, g0, g1); break;
+            case 6: rg_class<6>(M, e, lo, hi, rr, fx, sub, g0, g1); break;
+            default: break;
+        }
+    }
+    int extrap = 0;
+    if (i >= 0) {
 #pragma unroll
-            for (int q = 0; q < NN_RQ; ++q) {
-                if (q < nq) {
-                    const float dr = rr[q] - rs;
-                    acc = fmaf(__builtin_amdgcn_exp2f(ce * dr * dr), fx[q], acc);
-                }
-            }
-            const float v = row_sum16(acc);
-            const int ko = M.rk_perm[e][k];
-            if (i >= 0 && sub == (k & (NN_TPA - 1))) {
+        for (int h = 0; h < 2; ++h) {
+            const int k = sub + NN_TPA * h;
+            if (k < nr) {
+                const int ko = M.rk_perm[e][k];
+                const float v = h ? g1 : g0;
                 Grep[(size_t)i * NN_K1 + ko] = v;
                 if (v < M.glo[e][ko] || v > M.ghi[e][ko]) extrap = 1;
             }
         }
     }
     if (extrap) *extrapp = 1;
+}
+
+// Functions lo .. hi-1 of element e on the lane's first NQ entries:
+// sq[q] += c_k dG_k/dr at entry q, dG_k/dr = exp(-eta (r - rs)^2) (dfc - 2
+// eta (r - rs) fc).  Next function's parameters prefetched.
+template <int NQ>
+__device__ __forceinline__ void rf_class(const NnConst &M, int e, int lo, int hi, const float *crow,
+                                         const float rr[NN_RQ], const float fx[NN_RQ], const float dx[NN_RQ],
+                                         float sq[NN_RQ]) {
+    float rs = M.rk_rs[e][lo], ce = M.rk_ce[e][lo], bk = M.rk_b[e][lo];
+    float ck = crow[M.rk_perm[e][lo]];
+    for (int k = lo; k < hi; ++k) {
+        const int kn = k + 1 < hi ? k + 1 : k;
+        const float rsn = M.rk_rs[e][kn], cen = M.rk_ce[e][kn], bkn = M.rk_b[e][kn];
+        const float ckn = crow[M.rk_perm[e][kn]];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const float dr = rr[q] - rs;
+            const float t = __builtin_amdgcn_exp2f(ce * dr * dr) * ck;
+            sq[q] = fmaf(t, fmaf(bk * dr, fx[q], dx[q]), sq[q]);
+        }
+        rs = rsn;
+        ce = cen;
+        bk = bkn;
+        ck = ckn;
+    }
 }
 
 // dE/dr of the chunk's centres' radial functions with coefficients crow =
@@ -1409,13 +1494,12 @@ __device__ void rf_chunk(const NnConst &M, const RadNbrF &L, const float *crow, 
     rad_entries(L, lane, M.rc, nel, rr, eq, &nq, &present);
 #pragma unroll
     for (int q = 0; q < NN_RQ; ++q) {
-        float fc = 0.f, dfc = 0.f;
-        if (q < nq) cutoff(pi_rc, rr[q], &fc, &dfc);
+        float fc, dfc;
+        cutoff(pi_rc, rr[q], &fc, &dfc);
         fcq[q] = eq[q] >= 0 ? fc : 0.f;
         dfcq[q] = eq[q] >= 0 ? dfc : 0.f;
         sq[q] = 0.f;
     }
-    // s_q = sum_k c_k dG_k/dr, dG_k/dr = exp(-eta (r - rs)^2) (dfc - 2 eta (r - rs) fc)
     for (int x = 0; x < nel; ++x) {
         const int lo = M.rk_lo[e][x], hi = M.rk_hi[e][x];
         if (lo == hi || !((present >> x) & 1u)) continue;
@@ -1425,17 +1509,14 @@ __device__ void rf_chunk(const NnConst &M, const RadNbrF &L, const float *crow, 
             fx[q] = eq[q] == x ? fcq[q] : 0.f;
             dx[q] = eq[q] == x ? dfcq[q] : 0.f;
         }
-        for (int k = lo; k < hi; ++k) {
-            const float rs = M.rk_rs[e][k], ce = M.rk_ce[e][k], bk = M.rk_b[e][k];
-            const float ck = crow[M.rk_perm[e][k]];
-#pragma unroll
-            for (int q = 0; q < NN_RQ; ++q) {
-                if (q < nq) {
-                    const float dr = rr[q] - rs;
-                    const float t = __builtin_amdgcn_exp2f(ce * dr * dr) * ck;
-                    sq[q] = fmaf(t, fmaf(bk * dr, fx[q], dx[q]), sq[q]);
-                }
-            }
+        switch (nq) {
+            case 1: rf_class<1>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
+            case 2: rf_class<2>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
+            case 3: rf_class<3>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
+            case 4: rf_class<4>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
+            case 5: rf_class<5>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
+            case 6: rf_class<6>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
+            default: break;
         }
     }
     float F0 = 0.f, F1 = 0.f, F2 = 0.f;
@@ -1464,7 +1545,7 @@ __device__ void rf_chunk(const NnConst &M, const RadNbrF &L, const float *crow, 
 }
 
 // Pass G and the network of a radial-only model (k_nene_g's contract).
-__global__ void __launch_bounds__(NN_BLOCK) k_nene_rg(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
+__global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rg(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
                                                       int skip_extrap) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ union {
@@ -1475,6 +1556,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_rg(const NnConst *__restrict_
     float(*T)[16 * 33] = U.T;
     __shared__ int rowsb[NN_BLOCK / 64][16];
     __shared__ int s_ne[NN_MAXE];
+    __shared__ RadCell Cl;
     const NnConst &M = *Mg;
     const int rep = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;  // size-class launches
     const int t = bv.rep_traj[rep];
@@ -1497,14 +1579,8 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_rg(const NnConst *__restrict_
     uint8_t *elem = (uint8_t *)(elist + N);
 
     const int ne_model = M.nelem;
-    double h[9], ih[9];
-    float w[3];
-    int imax[3];
-    cell_setup(M, bv.cell + 9 * t, bv.icell + 9 * t, h, ih, w, imax);
-    float hf[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) hf[k] = (float)h[k];
-    const bool min_image = w[0] < 0.4999f && w[1] < 0.4999f && w[2] < 0.4999f;
+    double ih[9];
+    rad_cell(Cl, M, bv.cell + 9 * t, bv.icell + 9 * t, ih, lane);
 
     for (int i = lane; i < N; i += NN_BLOCK) {
         const int ty = bv.type[aoff + i];
@@ -1532,7 +1608,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_rg(const NnConst *__restrict_
         const uint32_t sv = sl < nslot ? slot[sl] : 0xffffu;
         const int i = (sv & 0xffffu) == 0xffffu ? -1 : (int)(sv & 0xffffu);
         const int e = __builtin_amdgcn_readfirstlane((int)(sv >> 16));
-        rad_build(L, min_image, ux, uy, uz, elem, hf, h, imax, w, M.rc2, N, i, lane, &bad);
+        rad_build(L, Cl, ux, uy, uz, elem, M.rc2, N, i, lane, &bad);
         __syncthreads();
         rg_chunk(M, L, e, i, lane, Grep, &extrap);
         __syncthreads();
@@ -1554,12 +1630,13 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_rg(const NnConst *__restrict_
 }
 
 // Pass F and the outputs of a radial-only model (k_nene_f's contract).
-__global__ void __launch_bounds__(NN_BLOCK) k_nene_rf(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
+__global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rf(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
                                                       int add_fin, int skip_extrap) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ RadNbrF L;
     __shared__ float cb[NN_CA][NN_K1];
     __shared__ int s_ne[NN_MAXE];
+    __shared__ RadCell Cl;
     const NnConst &M = *Mg;
     const int rep = bv.rep_list ? bv.rep_list[blockIdx.x] : (int)blockIdx.x;  // size-class launches
     if (skip_extrap && nv.rep_flag[rep]) return;  // whole block: uniform
@@ -1583,14 +1660,8 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_rf(const NnConst *__restrict_
     uint8_t *elem = (uint8_t *)(elist + N);
 
     const int ne_model = M.nelem;
-    double h[9], ih[9];
-    float w[3];
-    int imax[3];
-    cell_setup(M, bv.cell + 9 * t, bv.icell + 9 * t, h, ih, w, imax);
-    float hf[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) hf[k] = (float)h[k];
-    const bool min_image = w[0] < 0.4999f && w[1] < 0.4999f && w[2] < 0.4999f;
+    double ih[9];
+    rad_cell(Cl, M, bv.cell + 9 * t, bv.icell + 9 * t, ih, lane);
     for (int i = lane; i < N; i += NN_BLOCK) {
         const int e = M.elem_of_type[bv.type[aoff + i]];
         elem[i] = (uint8_t)(e < 0 || e >= ne_model ? 0 : e);
@@ -1615,7 +1686,7 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_rf(const NnConst *__restrict_
         const uint32_t sv = sl < nslot ? slot[sl] : 0xffffu;
         const int i = (sv & 0xffffu) == 0xffffu ? -1 : (int)(sv & 0xffffu);
         const int e = __builtin_amdgcn_readfirstlane((int)(sv >> 16));
-        rad_build(L, min_image, ux, uy, uz, elem, hf, h, imax, w, M.rc2, N, i, lane, &bad);
+        rad_build(L, Cl, ux, uy, uz, elem, M.rc2, N, i, lane, &bad);
         for (int q = lane; q < NN_CA * NN_K1; q += NN_BLOCK) {
             const int ci = q / NN_K1, k = q % NN_K1;
             const int s2 = k0 + ci;
