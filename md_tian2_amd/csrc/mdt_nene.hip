@@ -125,6 +125,19 @@ __device__ __forceinline__ void fxadd(unsigned long long *acc, double x, int *ba
     atomicAdd(acc, (unsigned long long)(long long)rint(y));
 }
 
+// x * 2^32 from an FP32 x as a 64-bit fixed-point addend: floor(x) and the
+// 32-bit fraction (x - floor(x)) 2^32, both exact in FP32 for |x| < 2^31
+// (exact when x has no bits below 2^-32, truncated below that)
+__device__ __forceinline__ void fxadd32(unsigned long long *acc, float x, int *bad) {
+    if (!(fabsf(x) < 2.0e9f)) {
+        *bad |= MDT_ST_FIXED_OVERFLOW;
+        return;
+    }
+    const float fl = floorf(x);
+    const uint32_t lo = (uint32_t)((x - fl) * 4294967296.0f);
+    atomicAdd(acc, ((unsigned long long)(uint32_t)(int)fl << 32) + lo);
+}
+
 // per-chunk neighbour data (structure of arrays)
 struct ChunkNbr {
     float dx[NN_CA * NN_CAP], dy[NN_CA * NN_CAP], dz[NN_CA * NN_CAP];
@@ -1232,12 +1245,15 @@ __device__ __forceinline__ float row_sum16(float v) {
 
 // Neighbour list of centre i (16 lanes; i < 0: none) within rc when the cell
 // admits only the minimum image: candidates n = it*16 + sub, hits compacted
-// in ascending n with a wavefront ballot.  hf: the cell in Bohr (FP32,
-// columns = lattice vectors).  Returns the (clamped) count in every lane of
-// the group; more than NN_CAP hits set the overflow flag.
-template <class Lt>
+// in ascending n with a wavefront ballot.  hs: the cell in Bohr times 2^-32
+// (FP32, columns = lattice vectors), applied to the wrapped 32-bit
+// fractional differences; TRI: a1 along x, a2 in the xy plane, a3 along z
+// (the slab cells), whose zero entries are skipped.  Returns the (clamped)
+// count in every lane of the group; more than NN_CAP hits set the overflow
+// flag.
+template <bool TRI, class Lt>
 __device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz,
-                                   const uint8_t *elem, const float hf[9], float rc2, int N, int i, int lane,
+                                   const uint8_t *elem, const float hs[9], float rc2, int N, int i, int lane,
                                    int *bad) {
     const int c = lane >> 4, sub = lane & 15, sh = lane & 48;
     const bool act = i >= 0;
@@ -1248,13 +1264,19 @@ __device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy
         bool hit = false;
         float ex = 0.f, ey = 0.f, ez = 0.f, r2 = 0.f;
         if (act && n < N && n != i) {
-            // the wrapped 32-bit difference is the fractional difference in [-1/2, 1/2)
-            const float f0 = (float)(int)(ux[n] - xi) * 0x1p-32f;
-            const float f1 = (float)(int)(uy[n] - yi) * 0x1p-32f;
-            const float f2 = (float)(int)(uz[n] - zi) * 0x1p-32f;
-            ex = hf[0] * f0 + hf[3] * f1 + hf[6] * f2;
-            ey = hf[1] * f0 + hf[4] * f1 + hf[7] * f2;
-            ez = hf[2] * f0 + hf[5] * f1 + hf[8] * f2;
+            // the wrapped 32-bit difference is 2^32 times the fractional difference in [-1/2, 1/2)
+            const float f0 = (float)(int)(ux[n] - xi);
+            const float f1 = (float)(int)(uy[n] - yi);
+            const float f2 = (float)(int)(uz[n] - zi);
+            if (TRI) {
+                ex = hs[0] * f0 + hs[3] * f1;
+                ey = hs[4] * f1;
+                ez = hs[8] * f2;
+            } else {
+                ex = hs[0] * f0 + hs[3] * f1 + hs[6] * f2;
+                ey = hs[1] * f0 + hs[4] * f1 + hs[7] * f2;
+                ez = hs[2] * f0 + hs[5] * f1 + hs[8] * f2;
+            }
             r2 = ex * ex + ey * ey + ez * ez;
             hit = r2 <= rc2;
         }
@@ -1273,9 +1295,9 @@ __device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy
 // minimum image alone suffices (every half-width below 1/2).
 struct RadCell {
     double h[9];
-    float hf[9], w[3];
+    float hs[9], w[3];  // hs: h 2^-32 (FP32)
     int imax[3];
-    int min_image;
+    int min_image, tri;
 };
 
 __device__ void rad_cell(RadCell &C, const NnConst &M, const double *cellA, const double *icellA, double ih[9],
@@ -1287,13 +1309,14 @@ __device__ void rad_cell(RadCell &C, const NnConst &M, const double *cellA, cons
     if (lane == 0) {
         for (int k = 0; k < 9; ++k) {
             C.h[k] = h[k];
-            C.hf[k] = (float)h[k];
+            C.hs[k] = (float)h[k] * 0x1p-32f;
         }
         for (int k = 0; k < 3; ++k) {
             C.w[k] = w[k];
             C.imax[k] = imax[k];
         }
         C.min_image = w[0] < 0.4999f && w[1] < 0.4999f && w[2] < 0.4999f;
+        C.tri = h[1] == 0.0 && h[2] == 0.0 && h[5] == 0.0 && h[6] == 0.0 && h[7] == 0.0;
     }
 }
 
@@ -1304,10 +1327,13 @@ __device__ void rad_build(Lt &L, const RadCell &C, const uint32_t *ux, const uin
     const int c = lane >> 4;
     int cnt;
     if (__builtin_amdgcn_readfirstlane(C.min_image)) {
-        float hf[9];
+        float hs[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) hf[k] = C.hf[k];
-        cnt = rad_build_min_image(L, ux, uy, uz, elem, hf, rc2, N, i, lane, bad);
+        for (int k = 0; k < 9; ++k) hs[k] = C.hs[k];
+        if (__builtin_amdgcn_readfirstlane(C.tri))
+            cnt = rad_build_min_image<true>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, bad);
+        else
+            cnt = rad_build_min_image<false>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, bad);
     } else {
         double h[9];
         int imax[3];
@@ -1530,16 +1556,16 @@ __device__ void rf_chunk(const NnConst &M, const RadNbrF &L, const float *crow, 
             F1 += v1;
             F2 += v2;
             const int j = L.j[e2];
-            fxadd(&ax[j], -(double)v0, &bad);
-            fxadd(&ay[j], -(double)v1, &bad);
-            fxadd(&az[j], -(double)v2, &bad);
+            fxadd32(&ax[j], -v0, &bad);
+            fxadd32(&ay[j], -v1, &bad);
+            fxadd32(&az[j], -v2, &bad);
         }
     }
     const float S0 = row_sum16(F0), S1 = row_sum16(F1), S2 = row_sum16(F2);
     if (i >= 0 && sub == 0) {
-        fxadd(&ax[i], (double)S0, &bad);
-        fxadd(&ay[i], (double)S1, &bad);
-        fxadd(&az[i], (double)S2, &bad);
+        fxadd32(&ax[i], S0, &bad);
+        fxadd32(&ay[i], S1, &bad);
+        fxadd32(&az[i], S2, &bad);
     }
     if (bad) *badp |= bad;
 }
