@@ -1570,6 +1570,194 @@ __device__ void rf_chunk(const NnConst &M, const RadNbrF &L, const float *crow, 
     if (bad) *badp |= bad;
 }
 
+// Activation and its derivative with one exponential and one reciprocal:
+// tanh z = 1 - 2 / (1 + e^{2z}) (absolute error ~1e-7 near 0, exact
+// saturation), sigmoid z = 1 / (1 + e^{-z}).
+__device__ __forceinline__ float act_fast(int code, float z, float *d1) {
+    if (code == 1) {
+        const float h = 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(2.0f * NN_L2E * z));
+        *d1 = 1.0f - h * h;
+        return h;
+    }
+    if (code == 2) {
+        const float h = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-NN_L2E * z));
+        *d1 = h * (1.0f - h);
+        return h;
+    }
+    *d1 = 1.0f;
+    return z;
+}
+
+// orders a wavefront's LDS writes before its later LDS reads (the transpose
+// buffers of mlp_tile_w are private to one wavefront)
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// mlp_tile for one wavefront on its own (no block barriers): the same
+// products on v_mfma_f32_16x16x32_bf16 with hi/lo split operands; the
+// second 32-input block of layer 1 and the dE/dx tiles past the element's
+// inputs are skipped; act_fast for the activations.
+__device__ void mlp_tile_w(const NnConst &M, int e, const int *rows, const float *Grep, float *Crep, double *Ea,
+                           float *T, int lane) {
+    const int l = lane & 63;
+    const int rA = l & 15, kg = (l >> 4) * 8;
+    const int nB = l & 15;
+    const int rC = (l >> 4) * 4;
+    const int nin = M.nr[e] + M.na[e];
+    const int nks = nin > 32 ? 2 : 1;
+    const int atomA = rows[rA];
+
+    f32x4 z1[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    for (int ks = 0; ks < nks; ++ks) {
+        float av[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int k = ks * 32 + kg + j;
+            av[j] = (atomA >= 0 && k < nin) ? Grep[(size_t)atomA * NN_K1 + k] * M.smul[e][k] + M.sadd[e][k] : 0.f;
+        }
+        bf16x8 ah, al;
+        split8(av, ah, al);
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            float bv[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) bv[j] = M.W1[e][ks * 32 + kg + j][nt * 16 + nB];
+            bf16x8 bh, bl;
+            split8(bv, bh, bl);
+            z1[nt] = mfma3(ah, al, bh, bl, z1[nt]);
+        }
+    }
+    float h1[2][4], d1[2][4];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h1[nt][i] = act_fast(M.act1, z1[nt][i] + M.b1[e][nt * 16 + nB], &d1[nt][i]);
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) T[(rC + i) * 33 + nt * 16 + nB] = h1[nt][i];
+    wave_lds_sync();
+    float av2[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) av2[j] = T[rA * 33 + kg + j];
+    wave_lds_sync();
+    bf16x8 ah, al;
+    split8(av2, ah, al);
+    f32x4 z2[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        float bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bv[j] = M.W2[e][kg + j][nt * 16 + nB];
+        bf16x8 bh, bl;
+        split8(bv, bh, bl);
+        z2[nt] = mfma3(ah, al, bh, bl, z2[nt]);
+    }
+    float h2[2][4], d2[2][4];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h2[nt][i] = act_fast(M.act2, z2[nt][i] + M.b2[e][nt * 16 + nB], &d2[nt][i]);
+    float dz3[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        float p = h2[0][i] * M.w3[e][nB] + h2[1][i] * M.w3[e][16 + nB];
+        p = row_sum16(p);
+        float d3;
+        const float E = act_fast(M.act3, p + M.b3[e], &d3);
+        dz3[i] = d3;
+        const int row = rC + i;
+        if (nB == 0 && rows[row] >= 0) Ea[rows[row]] = (double)E;
+    }
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) T[(rC + i) * 33 + nt * 16 + nB] = dz3[i] * M.w3[e][nt * 16 + nB] * d2[nt][i];
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) av2[j] = T[rA * 33 + kg + j];
+    wave_lds_sync();
+    split8(av2, ah, al);
+    f32x4 g1[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        float bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bv[j] = M.W2[e][nt * 16 + nB][kg + j];
+        bf16x8 bh, bl;
+        split8(bv, bh, bl);
+        g1[nt] = mfma3(ah, al, bh, bl, g1[nt]);
+    }
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) T[(rC + i) * 33 + nt * 16 + nB] = g1[nt][i] * d1[nt][i];
+    wave_lds_sync();
+#pragma unroll
+    for (int j = 0; j < 8; ++j) av2[j] = T[rA * 33 + kg + j];
+    wave_lds_sync();
+    split8(av2, ah, al);
+    for (int nt = 0; nt < 4; ++nt) {
+        if (nt * 16 >= nin) break;
+        float bv[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) bv[j] = M.W1[e][nt * 16 + nB][kg + j];
+        bf16x8 bh, bl;
+        split8(bv, bh, bl);
+        f32x4 dx = {0.f, 0.f, 0.f, 0.f};
+        dx = mfma3(ah, al, bh, bl, dx);
+        const int k = nt * 16 + nB;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int row = rC + i;
+            if (rows[row] >= 0 && k < nin) Crep[(size_t)rows[row] * NN_K1 + k] = dx[i] * M.dfac[e][k];
+        }
+    }
+}
+
+// network_energy with wavefronts working through the tiles independently
+// (tile = wave, wave + 4, ...), one block barrier before the energy sum.
+__device__ void network_energy_w(const NnConst &M, const int *s_ne, const uint16_t *elist, int (*rowsb)[16],
+                                 float (*T)[16 * 33], const float *Grep, float *Crep, double *Ea, int N, int lane,
+                                 double *epot) {
+    const int wave = lane >> 6;
+    const int ne_model = M.nelem;
+    int ntiles = 0;
+    for (int e = 0; e < ne_model; ++e) ntiles += (s_ne[e] + 15) / 16;
+    for (int tile = wave; tile < ntiles; tile += NN_BLOCK / 64) {
+        int e = -1, first = 0, cntE = 0, tl = tile, start = 0;
+        for (int q = 0; q < ne_model; ++q) {
+            const int nt = (s_ne[q] + 15) / 16;
+            if (e < 0 && tl < nt) {
+                e = q;
+                first = start + tl * 16;
+                cntE = s_ne[q] - tl * 16;
+            } else if (e < 0) {
+                tl -= nt;
+            }
+            start += s_ne[q];
+        }
+        e = __builtin_amdgcn_readfirstlane(e);
+        if ((lane & 63) < 16) {
+            const int rr = lane & 15;
+            rowsb[wave][rr] = rr < cntE ? (int)elist[first + rr] : -1;
+        }
+        wave_lds_sync();
+        mlp_tile_w(M, e, rowsb[wave], Grep, Crep, Ea, T[wave], lane);
+        wave_lds_sync();
+    }
+    __syncthreads();
+    if (lane < 64) {
+        double e = 0.0;
+        for (int i = lane; i < N; i += 64) e += Ea[i];
+        for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+        if (lane == 0) *epot = e * NN_HA2EV;
+    }
+}
+
 // Pass G and the network of a radial-only model (k_nene_g's contract).
 __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rg(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
                                                       int skip_extrap) {
@@ -1650,7 +1838,7 @@ __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rg(const NnConst *__restri
             atomicOr(&bv.status[t], bad);
         return;
     }
-    network_energy(M, s_ne, elist, rowsb, T, Grep, Crep, Ea, N, lane, &bv.epot[bv.traj_eoff[t] + b]);
+    network_energy_w(M, s_ne, elist, rowsb, T, Grep, Crep, Ea, N, lane, &bv.epot[bv.traj_eoff[t] + b]);
     if (extrap) bad |= MDT_ST_NN_EXTRAPOLATION;
     if (bad) atomicOr(&bv.status[t], bad);
 }
