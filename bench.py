@@ -268,39 +268,59 @@ def copy_calibration():
     print(json.dumps({"calibration": "copy", "bytes_read": n * 8, "bytes_written": n * 8}), flush=True)
 
 
-def nn_variant(ctx, pes, args):
-    """The NENE surrogate on its workload: 6x6x4 Au + H trajectories, slab
-    Langevin at 300 K, H Verlet, dt 0.1 fs; EMT fallback on extrapolation
-    and the EMT judge on a rotating 1 % of the replicas every step (1 % of frames)."""
-    from md_tian2_amd import mdt, surrogate
+def _nn_run(ctx, model, systems, steps, warmup):
+    from md_tian2_amd import mdt
 
-    model = surrogate.load_model()
-    if model is None:
-        return {"value": None, "reason": "no trained model in md_tian2_amd/nn/hau (python -m md_tian2_amd.surrogate train)"}
-    systems = make_workload(args.nn_traj, seed=4242)
     ctx.set_nn(model, systems[0].type_names)
     b = mdt.Batch(ctx, systems, dt=0.1, tsurf=300.0, tproj=0.0, seed=777)
     b.set_pes(1, judge_interval=1)
     b.compute_forces()
-    if args.warmup:
-        b.step(min(args.warmup, 10))
-    b.step(args.nn_steps)
+    if warmup:
+        b.step(min(warmup, 10))
+    b.step(steps)
     ms, _ = b.timing()
     rmse, ncomp, mx, nfb = b.nn_stats()
     st = b.status()
     b.close()
-    print(f"[bench] nn: {args.nn_steps} steps of {len(systems)} trajectories, events {ms:.3f} ms, "
-          f"judge RMSE {rmse} eV/A over {ncomp:.0f} components", file=sys.stderr, flush=True)
     return {
-        "value": len(systems) * args.nn_steps / (ms / 1e3), "unit": "trajectory-steps/s",
-        "ms_per_step": ms / args.nn_steps, "steps": args.nn_steps, "trajectories": len(systems),
-        "workload": "NENE surrogate H/Au(111) 6x6x4 + H (24 G2 + 16 G4 per element, 6.5 A, 40-32-32-1 tanh), "
-                    "slab Langevin 300 K, H Verlet, dt 0.1 fs, EMT fallback on extrapolation, EMT judge on a "
-                    "rotating 1 % of replicas every step (1 % of frames)",
+        "value": len(systems) * steps / (ms / 1e3), "unit": "trajectory-steps/s",
+        "ms_per_step": ms / steps, "steps": steps, "trajectories": len(systems),
         "judge_force_rmse_eV_per_A": rmse, "judge_components": ncomp, "judge_max_abs_eV_per_A": mx,
         "fallback_replicas_last_step": nfb, "trajectories_flagged": int((st != 0).sum()),
-        "precision": "FP32 symmetry functions, BF16 hi/lo-split MFMA network (FP32 accumulate), FP64 integration",
     }
+
+
+def nn_variant(ctx, pes, args):
+    """The NENE surrogate on its workload: 6x6x4 Au + H trajectories, slab
+    Langevin at 300 K, H Verlet, dt 0.1 fs; EMT fallback on extrapolation
+    and the EMT judge on a rotating 1 % of the replicas every step (1 % of
+    frames).  The radial-only network (md_tian2_amd/nn/hau_radial: 16 radial
+    functions per element, 5.5 A) is the surrogate; the radial + angular
+    network (md_tian2_amd/nn/hau: 24 radial + 16 angular, 6.5 A) is timed
+    beside it on fewer steps."""
+    from md_tian2_amd import surrogate
+
+    model = surrogate.load_model(surrogate.MODEL_DIR_RADIAL)
+    if model is None:
+        return {"value": None, "reason": "no trained model in md_tian2_amd/nn/hau_radial "
+                                         "(python -m md_tian2_amd.surrogate train --design radial)"}
+    systems = make_workload(args.nn_traj, seed=4242)
+    out = _nn_run(ctx, model, systems, args.nn_steps, args.warmup)
+    out.update({
+        "workload": "NENE surrogate H/Au(111) 6x6x4 + H (16 radial symmetry functions per element, 5.5 A, "
+                    "16-32-32-1 tanh), slab Langevin 300 K, H Verlet, dt 0.1 fs, EMT fallback on extrapolation, "
+                    "EMT judge on a rotating 1 % of replicas every step (1 % of frames)",
+        "precision": "FP32 symmetry functions, BF16 hi/lo-split MFMA network (FP32 accumulate), FP64 integration",
+    })
+    print(f"[bench] nn: {args.nn_steps} steps of {len(systems)} trajectories, {out['ms_per_step']:.3f} ms/step, "
+          f"judge RMSE {out['judge_force_rmse_eV_per_A']} eV/A", file=sys.stderr, flush=True)
+    ang = surrogate.load_model()
+    if ang is not None and args.nn_angular_steps > 0:
+        a = _nn_run(ctx, ang, systems, args.nn_angular_steps, args.warmup)
+        a["workload"] = "the same with the radial + angular network md_tian2_amd/nn/hau (24 G2 + 16 G4, 6.5 A)"
+        out["angular_network"] = a
+        print(f"[bench] nn angular: {a['ms_per_step']:.3f} ms/step", file=sys.stderr, flush=True)
+    return out
 
 
 def c1_variant(ctx, pes, args):
@@ -463,9 +483,9 @@ def uneven_variant(ctx, pes, args):
     out["emt_fp64_mixed_beads"]["model"] = "H beads, slab classical (1 bead)"
     b = mdt.Batch(ctx, systems, dt=0.1, tsurf=tsurf, tproj=tsurf, seed=31)
     timed(b, args.uneven_all_steps, "emt_fp64_all_beads")
-    model = surrogate.load_model(surrogate.MODEL_DIR_THICK)
+    model = surrogate.load_model(surrogate.MODEL_DIR_RADIAL)
     if model is None:
-        out["nn_surrogate"] = {"value": None, "reason": "no trained model in md_tian2_amd/nn/hau_thick"}
+        out["nn_surrogate"] = {"value": None, "reason": "no trained model in md_tian2_amd/nn/hau_radial"}
         return out
     ctx.set_nn(model, systems[0].type_names)
     b = mdt.Batch(ctx, systems, dt=0.1, tsurf=tsurf, tproj=tsurf, seed=31)
@@ -474,7 +494,9 @@ def uneven_variant(ctx, pes, args):
     def nn_stats(b):
         rmse, ncomp, mx, nfb = b.nn_stats()
         return {"judge_force_rmse_eV_per_A": rmse, "judge_components": ncomp, "judge_max_abs_eV_per_A": mx,
-                "fallback_replicas_last_step": nfb, "model": "md_tian2_amd/nn/hau_thick, every atom carries the beads"}
+                "fallback_replicas_last_step": nfb,
+                "model": "md_tian2_amd/nn/hau_radial (trained on 4- and 6-layer slabs at 100-500 K), every atom "
+                         "carries the beads"}
 
     timed(b, args.uneven_nn_steps, "nn_surrogate", nn_stats)
     out["nn_surrogate"]["vs_emt_fp64_all_beads"] = out["nn_surrogate"]["value"] / out["emt_fp64_all_beads"]["value"]
@@ -493,6 +515,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--nn-traj", type=int, default=8192)
     ap.add_argument("--nn-steps", type=int, default=200)
+    ap.add_argument("--nn-angular-steps", type=int, default=20)
     ap.add_argument("--no-nn", action="store_true")
     ap.add_argument("--rpmd-traj", type=int, default=1024)
     ap.add_argument("--rpmd-steps", type=int, default=20)

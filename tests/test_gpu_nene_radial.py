@@ -64,7 +64,7 @@ def _check_against_oracle(m, frames, out):
     return worst_e, worst_f
 
 
-@pytest.mark.parametrize("design", [dict(), dict(nh=16, nau=12, nauh=4, rc=6.0)], ids=["r32", "r16-6A"])
+@pytest.mark.parametrize("design", [dict(nh=32, nau=24, nauh=8, rc=6.5), dict()], ids=["r32-6.5A", "r16-5.5A"])
 def test_radial_network_batch_forces(gpu_ctx, design):
     """6x6x4 and 9x9x6 cells (minimum-image lists) and 3x3x4 (image search),
     the H from inside the surface to 6 A above it."""
