@@ -30,6 +30,7 @@ hipError_t mdt_launch_mixed(int tpa, int block, int ntraj, size_t smem, const Em
                             const BatchView &bv, int add_fin, hipStream_t st);
 size_t mdt_nene_smem(int N);
 size_t mdt_nene_radial_smem(int N);
+int mdt_nene_cap();
 hipError_t mdt_nene_attributes(hipFuncAttributes *fa, int radial);
 size_t mdt_nn_frame_smem(int N);
 hipError_t mdt_launch_nn_frame(int nblocks, size_t smem, const NnConst *M, const NnTrainView &tv, const int *frames,
@@ -42,6 +43,7 @@ hipError_t mdt_launch_nn_adam(NnConst *M, const float *partials, int W, float *g
                               float beta1, float beta2, float eps, int step, int mode, hipStream_t st);
 hipError_t mdt_launch_nene(int nrep, size_t smem, const NnConst *M, const BatchView &bv, float *G, float *C,
                            int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, int radial,
+                           uint16_t *nbr, uint8_t *nbr_cnt, uint32_t *nbr_ref, int *nbr_key, double skin_bohr,
                            hipStream_t st);
 
 namespace {
@@ -187,6 +189,12 @@ struct mdt_batch {
     int pes = 0;                    // 0: EMT; 1: NENE network with EMT fallback on extrapolation
     float *d_nnG = nullptr, *d_nnC = nullptr;  // per replica maxN x NN_K1 symmetry functions, dE/dG
     int *d_nnflag = nullptr;        // per replica: network extrapolated -> EMT fallback
+    // radial NN kernels' neighbour lists (minimum-image cells): per replica
+    // maxN x cap indices, maxN lengths, 3 maxN positions at the build, key
+    uint16_t *d_nnNbr = nullptr;
+    uint8_t *d_nnCnt = nullptr;
+    uint32_t *d_nnRef = nullptr;
+    int *d_nnKey = nullptr;
     double *d_nnF = nullptr;        // the network's forces alone (3PN blocks)
     double *d_judge = nullptr;      // per replica: sum of squared force differences, count, max
     int judge_interval = 0;         // EMT judge every judge_interval steps (0: off)
@@ -505,7 +513,8 @@ int mdt_batch_destroy(mdt_batch *b) {
                     b->d_cjkoff, b->d_type, b->d_fixed, b->d_mass, b->d_cell, b->d_icell, b->d_tsurf,
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
                     b->d_cjk, b->d_status, b->d_pc, b->d_step, b->d_nnG, b->d_nnC,
-                    b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr, b->d_mix_ref, b->d_mix_fold, b->d_mix_cnt, b->d_mix_valid, b->d_obs, b->d_done, b->d_est, b->d_eta};
+                    b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr, b->d_mix_ref, b->d_mix_fold, b->d_mix_cnt, b->d_mix_valid, b->d_obs, b->d_done, b->d_est, b->d_eta,
+                    b->d_nnNbr, b->d_nnCnt, b->d_nnRef, b->d_nnKey};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
@@ -560,6 +569,7 @@ static int invalidate_lists(mdt_batch *b) {
     for (SizeClass &c : b->cls)
         if (c.d_valid) HIPCHK(hipMemsetAsync(c.d_valid, 0, sizeof(int) * c.nrep, st));
     if (b->d_mix_valid) HIPCHK(hipMemsetAsync(b->d_mix_valid, 0, sizeof(int) * b->mix_traj.size(), st));
+    if (b->d_nnKey) HIPCHK(hipMemsetAsync(b->d_nnKey, 0, sizeof(int) * b->nrep, st));
     HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
@@ -753,7 +763,7 @@ int mdt_batch_set_cells(mdt_batch *b, const double *simbox, const double *isimbo
     HIPCHK(hipMemcpyAsync(b->d_icell, isimbox, sizeof(double) * 9 * b->ntraj, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     b->have_cells = true;
-    return 0;
+    return invalidate_lists(b);  // lists hold fractional positions of the old cells
 }
 
 int mdt_batch_set_propagation(mdt_batch *b, int ntypes, const int32_t *algo, const int32_t *is_proj,
@@ -1238,6 +1248,13 @@ static int ensure_nn(mdt_batch *b) {
     if (fa.sharedSizeBytes + nn_smem(b->ctx, b->maxN) > b->lds_limit)
         return fail(MDT_EUNSUP, "network on %d atoms needs %zu bytes of LDS (limit %zu)", b->maxN,
                     fa.sharedSizeBytes + nn_smem(b->ctx, b->maxN), b->lds_limit);
+    if (nn_radial(b->ctx) && !b->d_nnNbr) {
+        const size_t na = (size_t)b->nrep * b->maxN;
+        int rc = dalloc(&b->d_nnNbr, na * mdt_nene_cap()) | dalloc(&b->d_nnCnt, na) | dalloc(&b->d_nnRef, 3 * na) |
+                 dalloc(&b->d_nnKey, (size_t)b->nrep);
+        if (rc) return rc;
+        HIPCHK(hipMemset(b->d_nnKey, 0, sizeof(int) * b->nrep));
+    }
     if (b->d_nnG) return 0;
     const size_t rows = (size_t)b->nrep * b->maxN * NN_K1;
     int rc = dalloc(&b->d_nnG, rows) | dalloc(&b->d_nnC, rows) | dalloc(&b->d_nnflag, (size_t)b->nrep) |
@@ -1265,7 +1282,8 @@ static int launch_pes(mdt_batch *b, const BatchView &bv, int add_fin, hipStream_
             hipStream_t s2 = st;
             if ((rc = fork_stream(fk, &s2))) return rc;
             HIPCHK(mdt_launch_nene(k.nrep, nn_smem(c, k.nmax), c->d_nn, v, b->d_nnG, b->d_nnC, b->d_nnflag,
-                                   b->d_nnF, b->maxN, add_fin, 1, nn_radial(c), s2));
+                                   b->d_nnF, b->maxN, add_fin, 1, nn_radial(c), b->d_nnNbr, b->d_nnCnt, b->d_nnRef,
+                                   b->d_nnKey, b->list_skin / MDT_BOHR2ANG, s2));
         }
         if ((rc = fork_close(fk))) return rc;
         if (c->have_params) {
@@ -2092,7 +2110,8 @@ int mdt_compute_nene(mdt_context *ctx, int natoms, int nbeads, const int32_t *id
     if ((rc = ensure_ready(b, false))) return rc;
     HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(int) * b->ntraj, ctx->stream));
     HIPCHK(mdt_launch_nene(b->nrep, nn_smem(ctx, b->maxN), ctx->d_nn, b->view(), b->d_nnG, b->d_nnC, b->d_nnflag,
-                           b->d_nnF, b->maxN, 0, 0, nn_radial(ctx), ctx->stream));
+                           b->d_nnF, b->maxN, 0, 0, nn_radial(ctx), b->d_nnNbr, b->d_nnCnt, b->d_nnRef, b->d_nnKey,
+                           b->list_skin / MDT_BOHR2ANG, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     std::vector<int> st(1);
     HIPCHK(hipMemcpy(st.data(), b->d_status, sizeof(int), hipMemcpyDeviceToHost));

@@ -50,6 +50,15 @@ struct NnView {
     int *rep_flag;   // nrep: extrapolation flag of the replica (EMT fallback)
     double *fpes;    // 3PN blocks: the network's forces alone (before adding the ring-polymer forces)
     int maxN;
+    // radial kernels in minimum-image cells: k_nene_rg stores its neighbour
+    // lists (within rc + skin) and k_nene_rf reads them; with skin > 0 they
+    // are kept across launches while every atom stays within skin/2 of its
+    // position at the build.  nbr NULL: lists are built in both kernels.
+    uint16_t *nbr;      // nrep * maxN * NN_CAP neighbour indices
+    uint8_t *nbr_cnt;   // nrep * maxN list lengths
+    uint32_t *nbr_ref;  // nrep * 3 * maxN fractional positions (frac_fix) at the build
+    int *nbr_key;       // nrep: bits of (rc + skin) of valid kept lists, 0: none
+    float skin;         // Bohr
 };
 
 // symmetry-function tables every lane indexes by element (LDS copy)
@@ -1243,18 +1252,39 @@ __device__ __forceinline__ float row_sum16(float v) {
     return v;
 }
 
-// Neighbour list of centre i (16 lanes; i < 0: none) within rc when the cell
-// admits only the minimum image: candidates n = it*16 + sub, hits compacted
-// in ascending n with a wavefront ballot.  hs: the cell in Bohr times 2^-32
-// (FP32, columns = lattice vectors), applied to the wrapped 32-bit
-// fractional differences; TRI: a1 along x, a2 in the xy plane, a3 along z
-// (the slab cells), whose zero entries are skipped.  Returns the (clamped)
-// count in every lane of the group; more than NN_CAP hits set the overflow
-// flag.
+// Vector from atom i (fractional xi, yi, zi) to atom n under the minimum
+// image: the wrapped 32-bit differences are 2^32 times the fractional
+// difference in [-1/2, 1/2); hs is the cell in Bohr times 2^-32 (FP32,
+// columns = lattice vectors); TRI: a1 along x, a2 in the xy plane, a3 along
+// z (the slab cells), whose zero entries are skipped.
+template <bool TRI>
+__device__ __forceinline__ void min_image_d(const float hs[9], const uint32_t *ux, const uint32_t *uy,
+                                            const uint32_t *uz, int n, uint32_t xi, uint32_t yi, uint32_t zi,
+                                            float *ex, float *ey, float *ez) {
+    const float f0 = (float)(int)(ux[n] - xi);
+    const float f1 = (float)(int)(uy[n] - yi);
+    const float f2 = (float)(int)(uz[n] - zi);
+    if (TRI) {
+        *ex = hs[0] * f0 + hs[3] * f1;
+        *ey = hs[4] * f1;
+        *ez = hs[8] * f2;
+    } else {
+        *ex = hs[0] * f0 + hs[3] * f1 + hs[6] * f2;
+        *ey = hs[1] * f0 + hs[4] * f1 + hs[7] * f2;
+        *ez = hs[2] * f0 + hs[5] * f1 + hs[8] * f2;
+    }
+}
+
+// Neighbour list of centre i (16 lanes; i < 0: none) within sqrt(rb2) when
+// the cell admits only the minimum image: candidates n = it*16 + sub, hits
+// compacted in ascending n with a wavefront ballot.  nrow non-NULL: the
+// neighbour indices also go to nrow and the length to *ncnt.  Returns the
+// (clamped) count in every lane of the group; more than NN_CAP hits set the
+// overflow flag.
 template <bool TRI, class Lt>
 __device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz,
-                                   const uint8_t *elem, const float hs[9], float rc2, int N, int i, int lane,
-                                   int *bad) {
+                                   const uint8_t *elem, const float hs[9], float rb2, int N, int i, int lane,
+                                   uint16_t *nrow, uint8_t *ncnt, int *bad) {
     const int c = lane >> 4, sub = lane & 15, sh = lane & 48;
     const bool act = i >= 0;
     const uint32_t xi = ux[act ? i : 0], yi = uy[act ? i : 0], zi = uz[act ? i : 0];
@@ -1264,29 +1294,41 @@ __device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy
         bool hit = false;
         float ex = 0.f, ey = 0.f, ez = 0.f, r2 = 0.f;
         if (act && n < N && n != i) {
-            // the wrapped 32-bit difference is 2^32 times the fractional difference in [-1/2, 1/2)
-            const float f0 = (float)(int)(ux[n] - xi);
-            const float f1 = (float)(int)(uy[n] - yi);
-            const float f2 = (float)(int)(uz[n] - zi);
-            if (TRI) {
-                ex = hs[0] * f0 + hs[3] * f1;
-                ey = hs[4] * f1;
-                ez = hs[8] * f2;
-            } else {
-                ex = hs[0] * f0 + hs[3] * f1 + hs[6] * f2;
-                ey = hs[1] * f0 + hs[4] * f1 + hs[7] * f2;
-                ez = hs[2] * f0 + hs[5] * f1 + hs[8] * f2;
-            }
+            min_image_d<TRI>(hs, ux, uy, uz, n, xi, yi, zi, &ex, &ey, &ez);
             r2 = ex * ex + ey * ey + ez * ez;
-            hit = r2 <= rc2;
+            hit = r2 <= rb2;
         }
         const unsigned int row = (unsigned int)(__ballot(hit) >> sh) & 0xffffu;
         const int pos = cnt + __popc(row & ((1u << sub) - 1u));
-        if (hit && pos < NN_CAP) rad_put(L, c * NN_CAP + pos, n, ex, ey, ez, r2, elem);
+        if (hit && pos < NN_CAP) {
+            rad_put(L, c * NN_CAP + pos, n, ex, ey, ez, r2, elem);
+            if (nrow) nrow[pos] = (uint16_t)n;
+        }
         cnt += __popc(row);
     }
     if (cnt > NN_CAP) *bad |= MDT_ST_NBR_OVERFLOW;
-    return cnt < NN_CAP ? cnt : NN_CAP;
+    cnt = cnt < NN_CAP ? cnt : NN_CAP;
+    if (nrow && act && sub == 0) ncnt[i] = (uint8_t)cnt;
+    return cnt;
+}
+
+// A stored list of centre i: entries recomputed from the staged positions
+// with the arithmetic of rad_build_min_image (bitwise the same vectors).
+template <bool TRI, class Lt>
+__device__ int rad_load(Lt &L, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz, const uint8_t *elem,
+                        const float hs[9], int i, int lane, const uint16_t *nrow, const uint8_t *ncnt) {
+    const int c = lane >> 4, sub = lane & 15;
+    const int cnt = i >= 0 ? (int)ncnt[i] : 0;
+    if (i >= 0) {
+        const uint32_t xi = ux[i], yi = uy[i], zi = uz[i];
+        for (int p = sub; p < cnt; p += NN_TPA) {
+            const int n = nrow[p];
+            float ex, ey, ez;
+            min_image_d<TRI>(hs, ux, uy, uz, n, xi, yi, zi, &ex, &ey, &ez);
+            rad_put(L, c * NN_CAP + p, n, ex, ey, ez, ex * ex + ey * ey + ez * ez, elem);
+        }
+    }
+    return cnt;
 }
 
 // The replica's cell for the list builds, kept in LDS so that its registers
@@ -1298,10 +1340,11 @@ struct RadCell {
     float hs[9], w[3];  // hs: h 2^-32 (FP32)
     int imax[3];
     int min_image, tri;
+    int store;  // lists stored (minimum image within rc + skin)
 };
 
 __device__ void rad_cell(RadCell &C, const NnConst &M, const double *cellA, const double *icellA, double ih[9],
-                         int lane) {
+                         bool lists, float skin, int lane) {
     double h[9];
     float w[3];
     int imax[3];
@@ -1317,23 +1360,41 @@ __device__ void rad_cell(RadCell &C, const NnConst &M, const double *cellA, cons
         }
         C.min_image = w[0] < 0.4999f && w[1] < 0.4999f && w[2] < 0.4999f;
         C.tri = h[1] == 0.0 && h[2] == 0.0 && h[5] == 0.0 && h[6] == 0.0 && h[7] == 0.0;
+        const float ws = 1.0f + skin / M.rc;  // half-widths of the sphere of radius rc + skin
+        C.store = lists && w[0] * ws < 0.4999f && w[1] * ws < 0.4999f && w[2] * ws < 0.4999f;
     }
 }
 
-// The chunk's lists: the minimum-image pass or the image search.
+// The chunk's lists: with stored lists (C.store) read them (mode 1) or
+// build them within rc + skin and store them (mode 0); otherwise the
+// minimum-image pass or the image search within rc.
 template <class Lt>
-__device__ void rad_build(Lt &L, const RadCell &C, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz,
-                          const uint8_t *elem, float rc2, int N, int i, int lane, int *bad) {
+__device__ void rad_build(Lt &L, const RadCell &C, const NnView &nv, int rep, int mode, const uint32_t *ux,
+                          const uint32_t *uy, const uint32_t *uz, const uint8_t *elem, float rc2, float rb2, int N,
+                          int i, int lane, int *bad) {
     const int c = lane >> 4;
     int cnt;
-    if (__builtin_amdgcn_readfirstlane(C.min_image)) {
+    if (__builtin_amdgcn_readfirstlane(C.store)) {
+        float hs[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) hs[k] = C.hs[k];
+        uint16_t *nrow = nv.nbr + ((size_t)rep * nv.maxN + (i >= 0 ? i : 0)) * NN_CAP;
+        uint8_t *ncnt = nv.nbr_cnt + (size_t)rep * nv.maxN;
+        const bool tri = __builtin_amdgcn_readfirstlane(C.tri);
+        if (mode == 1)
+            cnt = tri ? rad_load<true>(L, ux, uy, uz, elem, hs, i, lane, nrow, ncnt)
+                      : rad_load<false>(L, ux, uy, uz, elem, hs, i, lane, nrow, ncnt);
+        else
+            cnt = tri ? rad_build_min_image<true>(L, ux, uy, uz, elem, hs, rb2, N, i, lane, nrow, ncnt, bad)
+                      : rad_build_min_image<false>(L, ux, uy, uz, elem, hs, rb2, N, i, lane, nrow, ncnt, bad);
+    } else if (__builtin_amdgcn_readfirstlane(C.min_image)) {
         float hs[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) hs[k] = C.hs[k];
         if (__builtin_amdgcn_readfirstlane(C.tri))
-            cnt = rad_build_min_image<true>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, bad);
+            cnt = rad_build_min_image<true>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, nullptr, nullptr, bad);
         else
-            cnt = rad_build_min_image<false>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, bad);
+            cnt = rad_build_min_image<false>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, nullptr, nullptr, bad);
     } else {
         double h[9];
         int imax[3];
@@ -1387,8 +1448,9 @@ __device__ __forceinline__ void rad_entries(const Lt &L, int lane, float rc, int
 #pragma unroll
     for (int q = 0; q < NN_RQ; ++q) {
         const int p = sub + NN_TPA * q;
-        const bool v = p < cnt;
-        rr[q] = v ? L.r[base + p] : rc;
+        const float r = p < cnt ? L.r[base + p] : rc;
+        const bool v = p < cnt && r <= rc;  // stored lists reach rc + skin
+        rr[q] = v ? r : rc;
         eq[q] = v ? (int)L.el[base + p] : -1;
     }
     unsigned int pr = 0;
@@ -1794,7 +1856,7 @@ __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rg(const NnConst *__restri
 
     const int ne_model = M.nelem;
     double ih[9];
-    rad_cell(Cl, M, bv.cell + 9 * t, bv.icell + 9 * t, ih, lane);
+    rad_cell(Cl, M, bv.cell + 9 * t, bv.icell + 9 * t, ih, nv.nbr != nullptr, nv.skin, lane);
 
     for (int i = lane; i < N; i += NN_BLOCK) {
         const int ty = bv.type[aoff + i];
@@ -1816,16 +1878,47 @@ __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rg(const NnConst *__restri
     const int nslot = make_slots(elist, s_ne, ne_model, slot, lane);
     __syncthreads();
 
+    // stored lists are kept while every atom stays within skin/2 of its
+    // position at the build (same cut-off and skin: nbr_key)
+    const bool store = __builtin_amdgcn_readfirstlane(Cl.store);
+    const float rb = M.rc + nv.skin;
+    const int key = __float_as_int(rb);
+    int mode = 0;
+    if (store && nv.skin > 0.f && nv.nbr_key[rep] == key) {
+        const uint32_t *ref = nv.nbr_ref + (size_t)rep * 3 * nv.maxN;
+        const float h2 = 0.25f * nv.skin * nv.skin;
+        int moved = 0;
+        for (int i = lane; i < N; i += NN_BLOCK) {
+            const float f0 = (float)(int)(ux[i] - ref[i]);
+            const float f1 = (float)(int)(uy[i] - ref[nv.maxN + i]);
+            const float f2 = (float)(int)(uz[i] - ref[2 * nv.maxN + i]);
+            const float dx = Cl.hs[0] * f0 + Cl.hs[3] * f1 + Cl.hs[6] * f2;
+            const float dy = Cl.hs[1] * f0 + Cl.hs[4] * f1 + Cl.hs[7] * f2;
+            const float dz = Cl.hs[2] * f0 + Cl.hs[5] * f1 + Cl.hs[8] * f2;
+            if (dx * dx + dy * dy + dz * dz >= h2) moved = 1;
+        }
+        mode = __syncthreads_or(moved) ? 0 : 1;
+    }
+
     // ---- pass G: symmetry functions ----
     for (int k0 = 0; k0 < nslot; k0 += NN_CA) {
         const int sl = k0 + (lane >> 4);
         const uint32_t sv = sl < nslot ? slot[sl] : 0xffffu;
         const int i = (sv & 0xffffu) == 0xffffu ? -1 : (int)(sv & 0xffffu);
         const int e = __builtin_amdgcn_readfirstlane((int)(sv >> 16));
-        rad_build(L, Cl, ux, uy, uz, elem, M.rc2, N, i, lane, &bad);
+        rad_build(L, Cl, nv, rep, mode, ux, uy, uz, elem, M.rc2, rb * rb, N, i, lane, &bad);
         __syncthreads();
         rg_chunk(M, L, e, i, lane, Grep, &extrap);
         __syncthreads();
+    }
+    if (store && mode == 0) {  // positions of this build
+        uint32_t *ref = nv.nbr_ref + (size_t)rep * 3 * nv.maxN;
+        for (int i = lane; i < N; i += NN_BLOCK) {
+            ref[i] = ux[i];
+            ref[nv.maxN + i] = uy[i];
+            ref[2 * nv.maxN + i] = uz[i];
+        }
+        if (lane == 0) nv.nbr_key[rep] = nv.skin > 0.f ? key : 0;
     }
 
     // replicas outside the training range go to the EMT fallback untouched
@@ -1875,7 +1968,7 @@ __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rf(const NnConst *__restri
 
     const int ne_model = M.nelem;
     double ih[9];
-    rad_cell(Cl, M, bv.cell + 9 * t, bv.icell + 9 * t, ih, lane);
+    rad_cell(Cl, M, bv.cell + 9 * t, bv.icell + 9 * t, ih, nv.nbr != nullptr, nv.skin, lane);
     for (int i = lane; i < N; i += NN_BLOCK) {
         const int e = M.elem_of_type[bv.type[aoff + i]];
         elem[i] = (uint8_t)(e < 0 || e >= ne_model ? 0 : e);
@@ -1900,7 +1993,9 @@ __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rf(const NnConst *__restri
         const uint32_t sv = sl < nslot ? slot[sl] : 0xffffu;
         const int i = (sv & 0xffffu) == 0xffffu ? -1 : (int)(sv & 0xffffu);
         const int e = __builtin_amdgcn_readfirstlane((int)(sv >> 16));
-        rad_build(L, Cl, ux, uy, uz, elem, M.rc2, N, i, lane, &bad);
+        // the lists k_nene_rg stored or kept for these positions
+        const float rb = M.rc + nv.skin;
+        rad_build(L, Cl, nv, rep, 1, ux, uy, uz, elem, M.rc2, rb * rb, N, i, lane, &bad);
         for (int q = lane; q < NN_CA * NN_K1; q += NN_BLOCK) {
             const int ci = q / NN_K1, k = q % NN_K1;
             const int s2 = k0 + ci;
@@ -2400,6 +2495,9 @@ hipError_t mdt_launch_nn_adam(NnConst *M, const float *partials, int W, float *g
 // the larger of k_nene_g's (23 B per atom) and k_nene_f's (37 B per atom)
 size_t mdt_nene_smem(int N) { return (size_t)N * (3 * 8 + 3 * 4 + 1) + 16; }
 
+// neighbour-list capacity per atom of the NN kernels (stored lists of the radial kernels)
+int mdt_nene_cap() { return NN_CAP; }
+
 // k_nene_rg: 27 B per atom, k_nene_rf: 43 B per atom (+ the slot padding)
 size_t mdt_nene_radial_smem(int N) { return (size_t)N * (3 * 8 + 3 * 4 + 4 + 2 + 1) + 4 * 4 * NN_MAXE + 16; }
 
@@ -2416,8 +2514,9 @@ hipError_t mdt_nene_attributes(hipFuncAttributes *fa, int radial) {
 
 hipError_t mdt_launch_nene(int nrep, size_t smem, const NnConst *M, const BatchView &bv, float *G, float *C,
                            int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, int radial,
+                           uint16_t *nbr, uint8_t *nbr_cnt, uint32_t *nbr_ref, int *nbr_key, double skin_bohr,
                            hipStream_t st) {
-    NnView nv{G, C, rep_flag, fpes, maxN};
+    NnView nv{G, C, rep_flag, fpes, maxN, nbr, nbr_cnt, nbr_ref, nbr_key, (float)skin_bohr};
     if (radial)
         hipLaunchKernelGGL(k_nene_rg, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, skip_extrap);
     else

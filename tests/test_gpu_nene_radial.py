@@ -200,3 +200,43 @@ def test_radial_surrogate_batch_matches_separate_batches(gpu_ctx, pes_hau):
         b1.close()
         for k in ("r", "v", "f", "epot"):
             assert np.array_equal(together[t][k], alone[k]), (t, k)
+
+
+def test_kept_lists_equal_rebuilt_lists(gpu_ctx, pes_hau):
+    """Neighbour lists kept across steps with the Verlet skin (entries to
+    rc + skin, masked beyond rc; k_nene_rf reads k_nene_rg's lists) give the
+    energies and forces of lists rebuilt within rc at every launch, to FP32
+    summation order, after 40 steps of 6x6x4 trajectories; both agree with
+    the NENE oracle."""
+    model = surrogate.load_model(surrogate.MODEL_DIR_RADIAL)
+    rng = np.random.default_rng(55)
+    systems = [geometry.h_on_slab("Au", n=6, nlayers=4, nfixed=2, rng=rng, einc=rng.uniform(0.5, 5),
+                                  polar_deg=30, azimuth_deg=rng.uniform(0, 360), z0=rng.uniform(1.5, 3.0),
+                                  tsurf=300.0) for _ in range(6)]
+    cols, pesm = pesfile.emt_tables(pes_hau, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    gpu_ctx.set_nn(model, systems[0].type_names)
+    b = mdt.Batch(gpu_ctx, systems, dt=0.1, tsurf=300.0, seed=3)
+    b.set_pes(1)
+    b.compute_forces()
+    b.step(40)
+    kept = b.get_state()
+    assert not b.status().any()
+    b.close()
+    moved = []
+    for s, o in zip(systems, kept):
+        s2 = s.copy()
+        s2.r = o["r"].copy()
+        moved.append(s2)
+    b2 = mdt.Batch(gpu_ctx, moved, dt=0.1, tsurf=300.0, seed=3)
+    b2.set_list_skin(0.0)
+    b2.set_pes(1)
+    b2.compute_forces()
+    fresh = b2.get_state()
+    b2.close()
+    for o, q in zip(kept, fresh):
+        assert np.abs(o["f"] - q["f"]).max() <= 1e-4
+        assert abs(o["epot"][0] - q["epot"][0]) <= 1e-6 * max(1.0, abs(q["epot"][0]))
+    worst_e, worst_f = _check_against_oracle(model, moved, kept)
+    assert worst_e <= E_ATOM_TOL
+    assert worst_f <= F_TOL
