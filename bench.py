@@ -268,12 +268,18 @@ def copy_calibration():
     print(json.dumps({"calibration": "copy", "bytes_read": n * 8, "bytes_written": n * 8}), flush=True)
 
 
+# the EMT judge: JUDGE_INTERVAL % of the replicas every JUDGE_INTERVAL steps, i.e.
+# 1 % of the frames (one judge launch costs about one replica's EMT time)
+JUDGE_INTERVAL = 4
+
+
 def _nn_run(ctx, model, systems, steps, warmup):
     from md_tian2_amd import mdt
 
     ctx.set_nn(model, systems[0].type_names)
     b = mdt.Batch(ctx, systems, dt=0.1, tsurf=300.0, tproj=0.0, seed=777)
-    b.set_pes(1, judge_interval=1)
+    b.set_pes(1, judge_interval=JUDGE_INTERVAL)
+    b.set_judge_percent(JUDGE_INTERVAL)  # 1 % of the frames
     b.compute_forces()
     if warmup:
         b.step(min(warmup, 10))
@@ -293,7 +299,7 @@ def _nn_run(ctx, model, systems, steps, warmup):
 def nn_variant(ctx, pes, args):
     """The NENE surrogate on its workload: 6x6x4 Au + H trajectories, slab
     Langevin at 300 K, H Verlet, dt 0.1 fs; EMT fallback on extrapolation
-    and the EMT judge on a rotating 1 % of the replicas every step (1 % of
+    and the EMT judge on a rotating 4 % of the replicas every 4 steps (1 % of
     frames).  The radial-only network (md_tian2_amd/nn/hau_radial: 16 radial
     functions per element, 5.5 A) is the surrogate; the radial + angular
     network (md_tian2_amd/nn/hau: 24 radial + 16 angular, 6.5 A) is timed
@@ -309,7 +315,7 @@ def nn_variant(ctx, pes, args):
     out.update({
         "workload": "NENE surrogate H/Au(111) 6x6x4 + H (16 radial symmetry functions per element, 5.5 A, "
                     "16-32-32-1 tanh), slab Langevin 300 K, H Verlet, dt 0.1 fs, EMT fallback on extrapolation, "
-                    "EMT judge on a rotating 1 % of replicas every step (1 % of frames)",
+                    "EMT judge on a rotating 4 % of replicas every 4 steps (1 % of frames)",
         "precision": "FP32 symmetry functions, BF16 hi/lo-split MFMA network (FP32 accumulate), FP64 integration",
     })
     print(f"[bench] nn: {args.nn_steps} steps of {len(systems)} trajectories, {out['ms_per_step']:.3f} ms/step, "
@@ -438,7 +444,7 @@ def uneven_variant(ctx, pes, args):
     the slab classical (mixed bead model), FP64 EMT with every atom carrying
     the beads (the reference's force_beads), and the NENE surrogate (every
     atom carrying the beads) with the EMT fallback and the EMT judge on a
-    rotating 1 % of the replicas every step (1 % of frames)."""
+    rotating 4 % of the replicas every 4 steps (1 % of frames)."""
     from md_tian2_amd import geometry, mdt, pesfile, surrogate
 
     g = np.random.default_rng(5150)
@@ -489,7 +495,8 @@ def uneven_variant(ctx, pes, args):
         return out
     ctx.set_nn(model, systems[0].type_names)
     b = mdt.Batch(ctx, systems, dt=0.1, tsurf=tsurf, tproj=tsurf, seed=31)
-    b.set_pes(1, judge_interval=1)
+    b.set_pes(1, judge_interval=JUDGE_INTERVAL)
+    b.set_judge_percent(JUDGE_INTERVAL)
 
     def nn_stats(b):
         rmse, ncomp, mx, nfb = b.nn_stats()

@@ -445,6 +445,17 @@ int mdt_compute_nene(mdt_context *ctx, int natoms, int nbeads, const int32_t *id
  */
 int mdt_batch_set_pes(mdt_batch *b, int pes, int judge_interval);
 
+/*
+ * Share of the replicas the EMT judge takes at each of its steps (no
+ * counterpart in the reference): percent, a divisor of 100 (default 1).
+ * Replicas with rep % (100 / percent) == phase are judged, the phase
+ * rotating per judge step; the judged share of all frames is percent /
+ * judge_interval %.  One judge launch takes about one replica's EMT time
+ * however many replicas it judges, so 4 % every 4 steps judges the frames
+ * of 1 % every step at a quarter of the launches.
+ */
+int mdt_batch_set_judge_percent(mdt_batch *b, int percent);
+
 /* Judge statistics accumulated since the network first ran on the batch:
  * sum of squared force differences [(eV/A)^2], number of force components
  * judged, largest difference [eV/A]; replicas on the EMT fallback in the

@@ -198,6 +198,7 @@ struct mdt_batch {
     double *d_nnF = nullptr;        // the network's forces alone (3PN blocks)
     double *d_judge = nullptr;      // per replica: sum of squared force differences, count, max
     int judge_interval = 0;         // EMT judge every judge_interval steps (0: off)
+    int judge_mod = 100;            // judged replicas: rep % judge_mod == phase (100: 1 %)
     uint32_t judge_events = 0;
     // bead mode 1 (mixed): in trajectories with P > 1 the projectile (atom 0)
     // carries the beads and the other atoms are classical, kept in bead 0;
@@ -1296,14 +1297,15 @@ static int launch_pes(mdt_batch *b, const BatchView &bv, int add_fin, hipStream_
     return launch_replicas(b, bv, add_fin, st);
 }
 
-// EMT as the judge of the network on a rotating 1 % of the replicas
+// EMT as the judge of the network on a rotating 100 / judge_mod % of the
+// replicas
 static int launch_judge(mdt_batch *b, hipStream_t st) {
     BatchView jb = b->view();
     jb.judge_f = b->d_nnF;
     jb.skip = b->d_nnflag;  // replicas on the fallback have no network forces
     jb.judge_acc = b->d_judge;
-    jb.judge_mod = 100;
-    jb.judge_phase = (int)(b->judge_events % 100u);
+    jb.judge_mod = b->judge_mod;
+    jb.judge_phase = (int)(b->judge_events % (uint32_t)b->judge_mod);
     ++b->judge_events;
     return launch_replicas(b, jb, 0, st);
 }
@@ -2038,6 +2040,14 @@ int mdt_batch_set_pes(mdt_batch *b, int pes, int judge_interval) {
     b->pes = pes;
     b->judge_interval = judge_interval;
     if (pes == 1) return ensure_nn(b);
+    return 0;
+}
+
+int mdt_batch_set_judge_percent(mdt_batch *b, int percent) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (percent < 1 || percent > 100 || 100 % percent != 0)
+        return fail(MDT_EINVAL, "judge percent %d (a divisor of 100)", percent);
+    b->judge_mod = 100 / percent;
     return 0;
 }
 

@@ -358,6 +358,13 @@ module mdt_gpu
             integer(c_int) :: rc
         end function
 
+        function mdt_batch_set_judge_percent(batch, percent) bind(C, name="mdt_batch_set_judge_percent") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: percent
+            integer(c_int) :: rc
+        end function
+
         function mdt_batch_nn_stats(batch, judge_sq, judge_n, judge_max, nfallback) &
                 bind(C, name="mdt_batch_nn_stats") result(rc)
             import :: c_int, c_ptr, c_double, c_int32_t

@@ -32,7 +32,7 @@ SYMBOLS = (
     "mdt_batch_set_list_skin",
     "mdt_batch_set_observe", "mdt_batch_get_observables", "mdt_batch_estimators",
     "mdt_sample_uniform", "mdt_sample_conf_index", "mdt_apply_pip", "mdt_set_proj_incidence", "mdt_sample_maxwell",
-    "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_nn_stats", "mdt_batch_nn_fallback",
+    "mdt_nn_params_set", "mdt_compute_nene", "mdt_batch_set_pes", "mdt_batch_set_judge_percent", "mdt_batch_nn_stats", "mdt_batch_nn_fallback",
     "mdt_batch_nn_descriptors", "mdt_nn_train_create", "mdt_nn_train_destroy", "mdt_nn_train_descriptor_stats",
     "mdt_nn_train_epoch", "mdt_nn_train_eval", "mdt_nn_train_loss_grad", "mdt_nn_get_weights", "mdt_nn_load",
 )
@@ -110,6 +110,7 @@ def lib():
                                         C.c_char_p, dp]
         L.mdt_compute_nene.argtypes = [vp, C.c_int, C.c_int, ip, dp, dp, dp, dp, dp, ip, C.c_int32]
         L.mdt_batch_set_pes.argtypes = [vp, C.c_int, C.c_int]
+        L.mdt_batch_set_judge_percent.argtypes = [vp, C.c_int]
         L.mdt_batch_nn_stats.argtypes = [vp, dp, dp, dp, ip]
         L.mdt_batch_nn_fallback.argtypes = [vp, ip]
         L.mdt_batch_nn_descriptors.argtypes = [vp, C.POINTER(C.c_float)]
@@ -385,6 +386,10 @@ class Batch:
     def set_pes(self, pes, judge_interval=0):
         """0: EMT; 1: the network (EMT fallback on extrapolation, EMT judge every judge_interval steps)."""
         _check(lib().mdt_batch_set_pes(self.h, int(pes), int(judge_interval)))
+
+    def set_judge_percent(self, percent):
+        """Share of the replicas judged at each judge step (a divisor of 100; default 1)."""
+        _check(lib().mdt_batch_set_judge_percent(self.h, int(percent)))
 
     def nn_stats(self):
         """(judge force RMSE [eV/A] or None, components judged, max |dF|, replicas on the fallback)."""

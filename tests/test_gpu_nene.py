@@ -166,3 +166,35 @@ def test_library_reader_matches_python_reader(gpu_ctx):
         out.append([gpu_ctx.compute_nene(_system_from_runner(s))[:2] for s in structs])
     for (e1, f1), (e2, f2) in zip(*out):
         assert np.array_equal(e1, e2) and np.array_equal(f1, f2)
+
+
+def test_judge_percent(gpu_ctx, pes_hau):
+    """4 % judged per judge step: after one step of 200 replicas the judged
+    ones are 0, 25, ..., 175; their RMSE equals the host's; a share that
+    does not divide 100 is refused."""
+    rng = np.random.default_rng(10)
+    frames = _frames(rng, list(np.linspace(2.0, 6.0, 200)))
+    m = _hau_model(frames[::20], rng, margin=0.5)
+    gpu_ctx.set_nn(m, frames[0].type_names)
+    cols, pesm = pesfile.emt_tables(pes_hau, frames[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    b = mdt.Batch(gpu_ctx, frames, dt=0.1, tsurf=0.0)
+    b.set_pes(1, judge_interval=1)
+    with pytest.raises(mdt.MdtError):
+        b.set_judge_percent(3)
+    b.set_judge_percent(4)
+    b.compute_forces()
+    b.step(1)
+    rmse, n, mx, _ = b.nn_stats()
+    out = b.get_state()
+    sq, cnt = 0.0, 0
+    for t in range(0, 200, 25):
+        s = frames[t].copy()
+        s.r = out[t]["r"].copy()
+        _, f_r, _ = oracle.emt_energy_forces(s, cols, pesm)
+        d = out[t]["f"] - f_r
+        sq += (d * d).sum()
+        cnt += d.size
+    assert n == cnt
+    assert abs(rmse - np.sqrt(sq / cnt)) <= 1e-6 * np.sqrt(sq / cnt)
+    b.close()
