@@ -85,12 +85,14 @@ def test_fortran_nene_dropin_matches_oracle():
         assert np.abs(np.array(f_g) - F * nene.HABOHR2EVANG).max() <= 5e-3
 
 
-def test_fortran_driver_network_with_judge(tmp_path):
-    """The batched driver on the trained H/Au network (nene, judge keywords):
+@pytest.mark.parametrize("net", ["hau", "hau_radial"])
+def test_fortran_driver_network_with_judge(tmp_path, net):
+    """The batched driver on a trained H/Au network (nene, judge keywords; the
+    radial one on the radial kernels, whose 3x3 cells take the image search):
     the EMT judge's force RMSE stays within the surrogate's 5 meV/A class."""
     deck = open(os.path.join(ROOT, "tests", "decks", "h_au_3x3x4.inp")).read()
     deck = deck.replace("dump 'gpurun_out/mdt_run_h_au.bin'", "")
-    deck += "\nnene 'md_tian2_amd/nn/hau'\njudge 1\nntrajs 200\n"
+    deck += f"\nnene 'md_tian2_amd/nn/{net}'\njudge 1\nntrajs 200\n"
     p = tmp_path / "nn.inp"
     p.write_text(deck)
     exe = os.path.join(ROOT, "md_tian2_amd", "fortran", "mdt_run")
@@ -98,7 +100,7 @@ def test_fortran_driver_network_with_judge(tmp_path):
     print(res.stdout)
     assert res.returncode == 0, res.stdout + res.stderr
     rmse = [float(line.split()[4]) for line in res.stdout.splitlines() if line.startswith("network judge")]
-    assert rmse and rmse[0] < 0.02
+    assert rmse and rmse[0] < (0.02 if net == "hau" else 0.005)
 
 
 def test_fortran_driver_mixed_beads_matches_library(tmp_path):
