@@ -240,3 +240,30 @@ def test_kept_lists_equal_rebuilt_lists(gpu_ctx, pes_hau):
     worst_e, worst_f = _check_against_oracle(model, moved, kept)
     assert worst_e <= E_ATOM_TOL
     assert worst_f <= F_TOL
+
+
+def test_radial_surrogate_in_hip_graphs(gpu_ctx, pes_hau):
+    """The radial kernels with kept lists replayed from hipGraph chunks (the
+    list validity is decided on the device) give bitwise the trajectories of
+    eager launches."""
+    model = surrogate.load_model(surrogate.MODEL_DIR_RADIAL)
+    rng = np.random.default_rng(56)
+    systems = [geometry.h_on_slab("Au", n=6, nlayers=4, nfixed=2, rng=rng, einc=rng.uniform(0.5, 5),
+                                  polar_deg=30, azimuth_deg=rng.uniform(0, 360), z0=rng.uniform(1.5, 3.0),
+                                  tsurf=300.0) for _ in range(8)]
+    cols, pesm = pesfile.emt_tables(pes_hau, systems[0].type_names)
+    gpu_ctx.set_emt(cols, pesm)
+    gpu_ctx.set_nn(model, systems[0].type_names)
+    out = []
+    for graph in (0, 10):
+        b = mdt.Batch(gpu_ctx, systems, dt=0.1, tsurf=300.0, seed=21)
+        b.set_pes(1)
+        b.set_graph(graph)
+        b.compute_forces()
+        b.step(30)
+        out.append(b.get_state())
+        assert not b.status().any()
+        b.close()
+    for a, g in zip(*out):
+        for k in ("r", "v", "f", "epot"):
+            assert np.array_equal(a[k], g[k]), k
