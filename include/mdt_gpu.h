@@ -240,6 +240,20 @@ int mdt_batch_set_wide(mdt_batch *b, int mode);
 int mdt_batch_set_keep(mdt_batch *b, int mode);
 
 /*
+ * Two streams for one-bead batches (no counterpart in the reference): a
+ * batch of one size class, one bead per trajectory and the EMT potential
+ * steps as two independent halves of its trajectories, each on its own
+ * stream, so that one half's propagator launches overlap the other half's
+ * force launches.  mode -1: when the batch has at least 1024 trajectories
+ * (default); 0: never; 1: whenever the batch qualifies.  Not while
+ * profiling.  Results are bitwise the same either way.
+ */
+int mdt_batch_set_split(mdt_batch *b, int mode);
+
+/* *halves: 2 when the last mdt_batch_step ran as two halves, else 1. */
+int mdt_batch_last_split(mdt_batch *b, int *halves);
+
+/*
  * Arithmetic of the EMT pair terms in compute_emt_2species
  * (src/pes_emt_mod.f90:646-763, 793-842, 871-972), which the reference
  * evaluates in FP64:

@@ -23,7 +23,7 @@ hipError_t mdt_launch_replica(int tpa, int nrep, int block, size_t smem, const E
 hipError_t mdt_launch_estimators(int ntraj, const PropConst *pc, const BatchView &bv, double *out, hipStream_t st);
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
                            const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
-                           int flags, int block, bool classical, hipStream_t st);
+                           int flags, int block, bool classical, hipStream_t st, int t_base = 0);
 hipError_t mdt_launch_advance(uint32_t *step_base, uint32_t n, int set, hipStream_t st);
 size_t mdt_mixed_smem(int nmax, int pmax);
 hipError_t mdt_launch_mixed(int tpa, int block, int ntraj, size_t smem, const EmtConst *ec, const PropConst *pc,
@@ -185,6 +185,14 @@ struct mdt_batch {
     int graph_chunk = 0;
     hipGraphExec_t gexec = nullptr;
     uint32_t *d_step = nullptr;
+    // two independent halves of the trajectories on two streams (one size
+    // class, one bead, EMT): mode -1 auto, 0 off, 1 on; their graph chunks,
+    // step counters and the identity replica list their force launches index
+    int split_mode = -1;
+    int last_split = 0;  // the last mdt_batch_step ran as two halves
+    hipGraphExec_t gexec_h[2] = {nullptr, nullptr};
+    uint32_t *d_step_h = nullptr;
+    int *d_ident = nullptr;
     int graph_replays = 0;          // replays of the last call
     int pes = 0;                    // 0: EMT; 1: NENE network with EMT fallback on extrapolation
     float *d_nnG = nullptr, *d_nnC = nullptr;  // per replica maxN x NN_K1 symmetry functions, dE/dG
@@ -515,10 +523,12 @@ int mdt_batch_destroy(mdt_batch *b) {
                     b->d_tproj, b->d_tid, b->d_x, b->d_v, b->d_a, b->d_f, b->d_dens, b->d_epot,
                     b->d_cjk, b->d_status, b->d_pc, b->d_step, b->d_nnG, b->d_nnC,
                     b->d_nnflag, b->d_nnF, b->d_judge, b->d_mix, b->d_mix_nbr, b->d_mix_ref, b->d_mix_fold, b->d_mix_cnt, b->d_mix_valid, b->d_obs, b->d_done, b->d_est, b->d_eta,
-                    b->d_nnNbr, b->d_nnCnt, b->d_nnRef, b->d_nnKey};
+                    b->d_nnNbr, b->d_nnCnt, b->d_nnRef, b->d_nnKey, b->d_step_h, b->d_ident};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    for (hipGraphExec_t g : b->gexec_h)
+        if (g) (void)hipGraphExecDestroy(g);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     for (hipEvent_t e : b->evf) (void)hipEventDestroy(e);
@@ -944,6 +954,10 @@ static int ensure_ready(mdt_batch *b, bool need_prop) {
 static void drop_graph(mdt_batch *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     b->gexec = nullptr;
+    for (hipGraphExec_t &g : b->gexec_h) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
 }
 
 static int heuristic_tpa(int maxN) {
@@ -1526,6 +1540,120 @@ static int enqueue_steps(mdt_batch *b, int nsteps, const uint32_t *step_base, ui
     return 0;
 }
 
+// One-bead, one-class EMT batches step as two independent halves of the
+// trajectories, each on its own stream (the context's and a side stream):
+// one half's propagator launches overlap the other half's force launches,
+// which alone fill the GPU less well.  Every trajectory runs the same
+// kernels with the same step numbers, so the results are bitwise those of
+// the whole batch.  Not while profiling (events around every force launch).
+static bool use_split(const mdt_batch *b) {
+    if (b->split_mode == 0 || b->profile) return false;
+    if (!(b->allP1 && traj_classical(b) && b->pes == 0 && b->cls.size() == 1 && b->mix_traj.empty() &&
+          b->cls[0].d_reps == nullptr && b->ntraj >= 2))
+        return false;
+    return b->split_mode == 1 || b->ntraj >= 1024;
+}
+
+// The steps of half h (trajectories [t0, t1)) on stream st: enqueue_steps
+// for one size class with the replica list and the kept-list slots offset
+// to the half.
+static int enqueue_half(mdt_batch *b, int h, int nsteps, const uint32_t *step_base, uint32_t first, hipStream_t st) {
+    mdt_context *c = b->ctx;
+    if (!c->d_ec) return fail(MDT_ESTATE, "no EMT parameters (call mdt_emt_params_set)");
+    const int t0 = h ? b->ntraj / 2 : 0, t1 = h ? b->ntraj : b->ntraj / 2;
+    const BatchView bv = b->view();
+    const SizeClass &k = b->cls[0];
+    BatchView v = bv;
+    v.rep_list = b->d_ident + t0;
+    v.nbr_global = k.d_nbr ? k.d_nbr + (size_t)t0 * k.nbr_stride : nullptr;
+    v.nbr_stride = k.nbr_stride;
+    v.nbcap = k.nbcap;
+    v.rl_ref = k.d_ref ? k.d_ref + (size_t)t0 * 3 * k.nmax : nullptr;
+    v.rl_cnt = k.d_kcnt ? k.d_kcnt + (size_t)t0 * k.nmax : nullptr;
+    v.rl_valid = k.d_valid ? k.d_valid + t0 : nullptr;
+    v.rl_skin = b->list_skin;
+    v.rl_ns = k.nmax;
+    v.rl_stage = k.stage;
+    for (int s = 0; s < nsteps; ++s) {
+        const uint32_t prev = first + s, stp = first + s + 1;
+        const int flags = (s > 0 ? 1 : 0) | 2;
+        HIPCHK(mdt_launch_traj(t1 - t0, 0, b->d_pc, bv, b->d_cjk, b->d_cjkoff, step_base, prev, stp, flags,
+                               traj_block(b), true, st, t0));
+        HIPCHK(mdt_launch_replica(k.tpa, t1 - t0, k.block, k.smem, c->d_ec, b->d_pc, v, 0, st));
+    }
+    if (nsteps > 0)
+        HIPCHK(mdt_launch_traj(t1 - t0, 0, b->d_pc, bv, b->d_cjk, b->d_cjkoff, step_base, first + nsteps,
+                               first + nsteps, 1, traj_block(b), true, st, t0));
+    return 0;
+}
+
+// Captures half h's chunk of graph_chunk steps (step numbers relative to
+// d_step_h[h], which the chunk's last node advances) and instantiates it.
+static int capture_half(mdt_batch *b, int h) {
+    mdt_context *c = b->ctx;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_half(b, h, b->graph_chunk, b->d_step_h + h, 0u, c->stream);
+    hipError_t e = rc ? hipSuccess : mdt_launch_advance(b->d_step_h + h, (uint32_t)b->graph_chunk, 0, c->stream);
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(MDT_EHIP, "graph capture failed: %s / %s", hipGetErrorString(e), hipGetErrorString(e2));
+    }
+    e = hipGraphInstantiate(&b->gexec_h[h], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->gexec_h[h] = nullptr;
+        return fail(MDT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return 0;
+}
+
+// nsteps steps as two halves on the context's stream and side stream 0,
+// joined on the context's stream; graph chunks of S steps when S > 0
+static int step_split(mdt_batch *b, int nsteps, int S, int *replays_out) {
+    mdt_context *c = b->ctx;
+    int rc;
+    if (!b->d_ident) {
+        std::vector<int> id(b->nrep);
+        for (int i = 0; i < b->nrep; ++i) id[i] = i;
+        if ((rc = dalloc(&b->d_ident, (size_t)b->nrep))) return rc;
+        HIPCHK(hipMemcpy(b->d_ident, id.data(), sizeof(int) * b->nrep, hipMemcpyHostToDevice));
+    }
+    if (!b->d_step_h && (rc = dalloc(&b->d_step_h, 2))) return rc;
+    int replays = 0;
+    if (S > 0 && nsteps >= S) {
+        if ((b->gexec_h[0] || b->gexec_h[1]) && b->graph_arith != b->arith()) drop_graph(b);
+        if (!b->gexec_h[0] || !b->gexec_h[1]) {
+            drop_graph(b);
+            if ((rc = capture_half(b, 0)) || (rc = capture_half(b, 1))) return rc;
+            b->graph_arith = b->arith();
+        }
+    }
+    hipStream_t st[2] = {c->stream, c->side[0]};
+    HIPCHK(hipEventRecord(c->fork_ev, c->stream));
+    HIPCHK(hipStreamWaitEvent(st[1], c->fork_ev, 0));
+    if (S > 0 && nsteps >= S) {
+        for (int h = 0; h < 2; ++h) HIPCHK(mdt_launch_advance(b->d_step_h + h, b->step, 1, st[h]));
+        replays = nsteps / S;
+        for (int k = 0; k < replays; ++k)
+            for (int h = 0; h < 2; ++h) HIPCHK(hipGraphLaunch(b->gexec_h[h], st[h]));
+        b->step += (uint32_t)(replays * S);
+    }
+    const int rest = nsteps - replays * S;
+    for (int h = 0; h < 2; ++h)
+        if ((rc = enqueue_half(b, h, rest, nullptr, b->step, st[h]))) return rc;
+    b->step += (uint32_t)rest;
+    HIPCHK(hipEventRecord(c->join_ev[0], st[1]));
+    HIPCHK(hipStreamWaitEvent(st[0], c->join_ev[0], 0));
+    *replays_out = replays;
+    return 0;
+}
+
 // Captures one chunk of graph_chunk steps (step numbers relative to d_step,
 // which the chunk's last node advances) and instantiates it.
 static int capture_chunk(mdt_batch *b) {
@@ -1575,20 +1703,25 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
     // host's step count: both launch directly
     const int S = (b->profile || (b->pes == 1 && b->judge_interval > 0)) ? 0 : b->graph_chunk;
     int replays = 0;
-    if (S > 0 && nsteps >= S) {
-        if (b->gexec && b->graph_arith != b->arith()) drop_graph(b);  // EMT parameters changed the arithmetic
-        if (!b->gexec) {
-            if ((rc = capture_chunk(b))) return rc;
-            b->graph_arith = b->arith();
+    b->last_split = use_split(b) ? 1 : 0;
+    if (b->last_split) {
+        if ((rc = step_split(b, nsteps, S, &replays))) return rc;
+    } else {
+        if (S > 0 && nsteps >= S) {
+            if (b->gexec && b->graph_arith != b->arith()) drop_graph(b);  // EMT parameters changed the arithmetic
+            if (!b->gexec) {
+                if ((rc = capture_chunk(b))) return rc;
+                b->graph_arith = b->arith();
+            }
+            HIPCHK(mdt_launch_advance(b->d_step, b->step, 1, c->stream));
+            replays = nsteps / S;
+            for (int k = 0; k < replays; ++k) HIPCHK(hipGraphLaunch(b->gexec, c->stream));
+            b->step += (uint32_t)(replays * S);
         }
-        HIPCHK(mdt_launch_advance(b->d_step, b->step, 1, c->stream));
-        replays = nsteps / S;
-        for (int k = 0; k < replays; ++k) HIPCHK(hipGraphLaunch(b->gexec, c->stream));
-        b->step += (uint32_t)(replays * S);
+        const int rest = nsteps - replays * S;
+        if ((rc = enqueue_steps(b, rest, nullptr, b->step, b->profile != 0))) return rc;
+        b->step += (uint32_t)rest;
     }
-    const int rest = nsteps - replays * S;
-    if ((rc = enqueue_steps(b, rest, nullptr, b->step, b->profile != 0))) return rc;
-    b->step += (uint32_t)rest;
     HIPCHK(hipEventRecord(b->ev1, c->stream));
     HIPCHK(hipEventSynchronize(b->ev1));
     float ms = 0.f;
@@ -1604,6 +1737,21 @@ int mdt_batch_step(mdt_batch *b, int nsteps) {
             b->last_force_ms += fm;
         }
     return check_status(b);
+}
+
+int mdt_batch_set_split(mdt_batch *b, int mode) {
+    if (!b) return fail(MDT_EINVAL, "NULL batch");
+    if (mode < -1 || mode > 1) return fail(MDT_EINVAL, "mdt_batch_set_split: mode %d (want -1, 0 or 1)", mode);
+    HIPCHK(hipSetDevice(b->ctx->device));
+    if (mode != b->split_mode) drop_graph(b);
+    b->split_mode = mode;
+    return 0;
+}
+
+int mdt_batch_last_split(mdt_batch *b, int *halves) {
+    if (!b || !halves) return fail(MDT_EINVAL, "mdt_batch_last_split: NULL argument");
+    *halves = b->last_split ? 2 : 1;
+    return 0;
 }
 
 int mdt_batch_set_graph(mdt_batch *b, int chunk) {

@@ -1486,12 +1486,12 @@ k_traj(const PropConst *__restrict__ pcg, BatchView bv,
 // does not set its register budget.
 __global__ void __launch_bounds__(256)
 k_traj_classical(const PropConst *__restrict__ pcg, BatchView bv, const uint32_t *step_base, uint32_t step_prev,
-                 uint32_t step, int flags) {
+                 uint32_t step, int flags, int t_base) {
     __shared__ int s_algo[MDT_MAXT], s_proj[MDT_MAXT];
     __shared__ double s_rmin[4];
     __shared__ int s_done;
     const PropConst &pc = *pcg;
-    const int t = blockIdx.x;
+    const int t = t_base + (int)blockIdx.x;  // trajectories t_base .. t_base + gridDim.x - 1
     if (bv.done && bv.done[t]) return;  // frozen trajectory (whole block: uniform)
     const int N = bv.traj_N[t];
     const long long soff = bv.traj_soff[t];
@@ -2297,10 +2297,10 @@ hipError_t mdt_launch_mixed(int tpa, int block, int ntraj, size_t smem, const Em
 // atom type uses PILE or Andersen (k_traj_classical)
 hipError_t mdt_launch_traj(int ntraj, size_t smem, const PropConst *pc, const BatchView &bv, const double *cjk,
                            const long long *cjk_off, const uint32_t *step_base, uint32_t step_prev, uint32_t step,
-                           int flags, int block, bool classical, hipStream_t st) {
+                           int flags, int block, bool classical, hipStream_t st, int t_base) {
     if (classical)
         hipLaunchKernelGGL(k_traj_classical, dim3(ntraj), dim3(block), 0, st, pc, bv, step_base, step_prev, step,
-                           flags);
+                           flags, t_base);
     else
         hipLaunchKernelGGL(k_traj, dim3(ntraj), dim3(block), smem, st, pc, bv, cjk, cjk_off, step_base, step_prev,
                            step, flags);

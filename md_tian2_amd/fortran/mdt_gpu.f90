@@ -228,6 +228,20 @@ module mdt_gpu
             integer(c_int) :: rc
         end function
 
+        function mdt_batch_set_split(batch, mode) bind(C, name="mdt_batch_set_split") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), value :: mode
+            integer(c_int) :: rc
+        end function
+
+        function mdt_batch_last_split(batch, halves) bind(C, name="mdt_batch_last_split") result(rc)
+            import :: c_int, c_ptr
+            type(c_ptr), value :: batch
+            integer(c_int), intent(out) :: halves
+            integer(c_int) :: rc
+        end function
+
         function mdt_batch_set_wide(batch, mode) bind(C, name="mdt_batch_set_wide") result(rc)
             import :: c_int, c_ptr
             type(c_ptr), value :: batch

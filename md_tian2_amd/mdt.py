@@ -28,7 +28,7 @@ SYMBOLS = (
     "mdt_batch_set_atoms", "mdt_batch_set_cells", "mdt_batch_set_propagation", "mdt_batch_seed",
     "mdt_batch_set_state", "mdt_batch_get_state", "mdt_batch_compute_forces", "mdt_batch_step",
     "mdt_batch_status", "mdt_batch_last_timing", "mdt_batch_tune", "mdt_batch_profile",
-    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_precision_arith", "mdt_batch_class_shapes", "mdt_batch_set_wide", "mdt_batch_set_keep", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
+    "mdt_batch_force_timing", "mdt_batch_set_precision", "mdt_batch_precision_arith", "mdt_batch_class_shapes", "mdt_batch_set_wide", "mdt_batch_set_keep", "mdt_batch_set_split", "mdt_batch_last_split", "mdt_batch_set_graph", "mdt_batch_set_bead_mode", "mdt_batch_set_com_removal",
     "mdt_batch_set_list_skin",
     "mdt_batch_set_observe", "mdt_batch_get_observables", "mdt_batch_estimators",
     "mdt_sample_uniform", "mdt_sample_conf_index", "mdt_apply_pip", "mdt_set_proj_incidence", "mdt_sample_maxwell",
@@ -91,6 +91,8 @@ def lib():
         L.mdt_batch_class_shapes.argtypes = [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int)]
         L.mdt_batch_set_wide.argtypes = [vp, C.c_int]
         L.mdt_batch_set_keep.argtypes = [vp, C.c_int]
+        L.mdt_batch_set_split.argtypes = [vp, C.c_int]
+        L.mdt_batch_last_split.argtypes = [vp, C.POINTER(C.c_int)]
         L.mdt_batch_set_bead_mode.argtypes = [vp, C.c_int]
         L.mdt_batch_set_com_removal.argtypes = [vp, C.c_int]
         L.mdt_batch_set_list_skin.argtypes = [vp, C.c_double]
@@ -386,6 +388,16 @@ class Batch:
     def set_pes(self, pes, judge_interval=0):
         """0: EMT; 1: the network (EMT fallback on extrapolation, EMT judge every judge_interval steps)."""
         _check(lib().mdt_batch_set_pes(self.h, int(pes), int(judge_interval)))
+
+    def set_split(self, mode):
+        """Two streams for one-bead, one-class EMT batches: -1 auto (>= 1024 trajectories), 0 off, 1 on."""
+        _check(lib().mdt_batch_set_split(self.h, int(mode)))
+
+    def last_split(self):
+        """2 when the last step call ran as two halves on two streams, else 1."""
+        n = C.c_int(0)
+        _check(lib().mdt_batch_last_split(self.h, C.byref(n)))
+        return n.value
 
     def set_judge_percent(self, percent):
         """Share of the replicas judged at each judge step (a divisor of 100; default 1)."""
