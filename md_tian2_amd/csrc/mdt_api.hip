@@ -365,14 +365,14 @@ static int build_emt_const(int ntypes, const double *eta2, const double *n0, con
     ec.acut = acut;
     ec.cutr = 1.5 * rcut;
     ec.cut2hi = ec.cutr * ec.cutr * (1.0 + 1e-12);
-    // near radius: theta(r) max_t exp(-min(eta2_t, kappa_t / beta) (r - beta s0_t)) <= 2^-80
+    // near radius: theta(r) max_t exp(-min(eta2_t, kappa_t / beta) (r - beta s0_t)) <= 2^-56
     auto weight = [&](double r) {
         double w = 0.0;
         for (int t : emt)
             w = std::max(w, std::exp(-std::min(eta2[t], kappa[t] / kBeta) * (r - kBeta * s0[t])));
         return w / (1.0 + std::exp(acut * (r - rcut)));
     };
-    const double wmax = std::ldexp(1.0, -80);
+    const double wmax = std::ldexp(1.0, -56);
     if (weight(ec.cutr) > wmax) {
         ec.rnear = ec.cutr;  // no far pairs
     } else {

@@ -51,12 +51,14 @@
 
 // An atom whose sigma~ from its near pairs is below this bound is "tiny":
 // its far pairs (and those of its partners with it) are evaluated too.  For
-// sigma~ >= 1e-3, |G| = |dE/dsigma~| <= 1.4e5 eV (the embedding terms give
+// sigma~ >= 1e-2, |G| = |dE/dsigma~| <= 1.4e4 eV (the embedding terms give
 // |G sigma~| <= 140 eV), so a far pair's force on either atom is at most
-// |G| csig (eta + acut) 2^-80 < 1e-16 eV/A.  The reference's far-H force
-// (ds/dr divided by the 1e-30-clamped sigma~) depends on the relative size of
-// those tiny terms, which this keeps exact.
-#define MDT_SIG_TINY 1.0e-3
+// |G| csig (eta + acut) 2^-56 < 4e-11 eV/A (csig <= 6.5, eta + acut <= 31 /A
+// for H/Au and H/Pt), and its share of sigma~ at most csig 2^-56 / 1e-2 <
+// 1e-14 relative.  The reference's far-H force (ds/dr divided by the
+// 1e-30-clamped sigma~) depends on the relative size of those tiny terms,
+// which this keeps exact.
+#define MDT_SIG_TINY 1.0e-2
 
 // EMT constants derived on the host exactly as src/pes_emt_mod.f90:533-592
 // (2 species) or :267-307 (1 species), flattened into per-type and
@@ -67,10 +69,10 @@ struct EmtConst {
     double rcut, acut, cutr;  // cutr = 1.5 * rcut (the reference's hard cut)
     double cut2hi;            // neighbour-list bound on r^2 (cutr^2 widened by 1e-12 rel)
     // Near radius: beyond it every pair weight theta(r) exp(-a (r - beta s0)),
-    // a in {eta2, kappa/beta} of either type, is below 2^-80.  Pairs between
-    // rnear and cutr are listed apart and evaluated only when one of the two
-    // atoms has sigma~ < MDT_SIG_TINY; elsewhere they change no force by more
-    // than 1e-16 eV/A and no sigma or energy by more than 1e-18 relative.
+    // a in {eta2, kappa/beta} of either type, is below 2^-56.  Pairs between
+    // rnear and cutr are evaluated only when one of the two atoms has sigma~ <
+    // MDT_SIG_TINY; elsewhere each changes no force by more than 4e-11 eV/A
+    // and no sigma~ by more than 1e-14 relative.
     double rnear;
     double eta2[MDT_MAXT], betas0[MDT_MAXT], kappadbeta[MDT_MAXT], betaeta2[MDT_MAXT];
     double e0[MDT_MAXT], lambda[MDT_MAXT], v0[MDT_MAXT], kappa[MDT_MAXT], n0[MDT_MAXT];

@@ -518,9 +518,15 @@ __global__ void __launch_bounds__(MAXB, 4)
         g12 = (float)(2 * (h[3] * h[6] + h[4] * h[7] + h[5] * h[8]));
     }
     // the lists hold the near pairs; a kept list is built with rnear + skin,
-    // so a pair now within rnear was listed at the build
+    // so a pair now within rnear was listed at the build.  Staged kept lists
+    // (the rows in HBM, a copy in LDS during the launch) hold the whole
+    // rnear + skin rows in HBM and only the entries now within rnear (the
+    // FP32 bound of a build without skin) in LDS, in the same order: the
+    // passes then see the entries of a list built at this launch.
     const double rn = keep ? rnear + skin : rnear;
     const float cut2f = (float)(rn * rn * (1.0 + 1e-4));
+    const float near2f = (float)(rnear * rnear * (1.0 + 1e-4));
+    const bool split = NBR_LDS && keep;
     uint16_t *kcnt = keep ? bv.rl_cnt + (long long)blockIdx.x * bv.rl_ns : nullptr;
     if (reuse) {  // kept list: the image of every entry from the current positions
         for (int k0 = 0; k0 < N; k0 += apb) {
@@ -529,26 +535,45 @@ __global__ void __launch_bounds__(MAXB, 4)
             if (k >= N) continue;
             const int c = kcnt[k];
             const float4 pk = f4[k];
-            for (int e = sub; e < c; e += TPA) {
-                const unsigned q = nbr_kept[(size_t)k * cap + e];
-                const int n = q & MDT_NBR_IDX_MASK;
-                const float4 pn = f4[n];
-                const float e0 = pk.x - pn.x, e1 = pk.y - pn.y, e2 = pk.z - pn.z;
-                float m0f = rintf(e0), m1f = rintf(e1), m2f = rintf(e2);
-                const bool t0 = fabsf(e0 - m0f) > 0.4999f, t1 = fabsf(e1 - m1f) > 0.4999f,
-                           t2 = fabsf(e2 - m2f) > 0.4999f;
-                if (t0 | t1 | t2) {
-                    const double *xr = bv.x + roff;
-                    const double a0 = xr[k], a1 = xr[N + k], a2 = xr[2 * N + k];
-                    const double b0 = xr[n], b1 = xr[N + n], b2 = xr[2 * N + n];
-                    if (t0) m0f = (float)tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, sx[k] - sx[n]);
-                    if (t1) m1f = (float)tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, sy[k] - sy[n]);
-                    if (t2) m2f = (float)tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, sz[k] - sz[n]);
+            int cn = 0;  // staged: near entries so far
+            for (int e0 = 0; e0 < c; e0 += TPA) {  // the same trip count in the atom's lanes
+                const int e = e0 + sub;
+                bool near = false;
+                uint16_t ent = 0;
+                if (e < c) {
+                    const unsigned q = nbr_kept[(size_t)k * cap + e];
+                    const int n = q & MDT_NBR_IDX_MASK;
+                    const float4 pn = f4[n];
+                    const float e0f = pk.x - pn.x, e1f = pk.y - pn.y, e2f = pk.z - pn.z;
+                    float m0f = rintf(e0f), m1f = rintf(e1f), m2f = rintf(e2f);
+                    const bool t0 = fabsf(e0f - m0f) > 0.4999f, t1 = fabsf(e1f - m1f) > 0.4999f,
+                               t2 = fabsf(e2f - m2f) > 0.4999f;
+                    if (t0 | t1 | t2) {
+                        const double *xr = bv.x + roff;
+                        const double a0 = xr[k], a1 = xr[N + k], a2 = xr[2 * N + k];
+                        const double b0 = xr[n], b1 = xr[N + n], b2 = xr[2 * N + n];
+                        if (t0) m0f = (float)tie_image(cell.ih[0], cell.ih[3], cell.ih[6], a0, a1, a2, b0, b1, b2, sx[k] - sx[n]);
+                        if (t1) m1f = (float)tie_image(cell.ih[1], cell.ih[4], cell.ih[7], a0, a1, a2, b0, b1, b2, sy[k] - sy[n]);
+                        if (t2) m2f = (float)tie_image(cell.ih[2], cell.ih[5], cell.ih[8], a0, a1, a2, b0, b1, b2, sz[k] - sz[n]);
+                    }
+                    const int code = ((int)m0f + 1) + 3 * ((int)m1f + 1) + 9 * ((int)m2f + 1);
+                    ent = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
+                    if (split) {
+                        const float d0 = e0f - m0f, d1 = e1f - m1f, d2 = e2f - m2f;
+                        const float r2 =
+                            d0 * (g00 * d0 + g01 * d1 + g02 * d2) + d1 * (g11 * d1 + g12 * d2) + g22 * d2 * d2;
+                        near = r2 <= near2f;
+                    } else {
+                        nbr[(size_t)k * cap + e] = ent;  // the rows in HBM: the image in place
+                    }
                 }
-                const int code = ((int)m0f + 1) + 3 * ((int)m1f + 1) + 9 * ((int)m2f + 1);
-                nbr[(size_t)k * cap + e] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
+                if (split) {
+                    const unsigned long long m = __ballot(near) & gmask;
+                    if (near) nbr[(size_t)k * cap + cn + __popcll(m & below)] = ent;
+                    cn += __popcll(m);
+                }
             }
-            if (sub == 0) cnt[k] = (uint16_t)c;
+            if (sub == 0) cnt[k] = (uint16_t)(split ? cn : c);
         }
     }
     // only the owned candidates are visited, in increasing index: below k
@@ -563,11 +588,11 @@ __global__ void __launch_bounds__(MAXB, 4)
         const int tk = act ? typ[k] : 0;
         const float4 pk = act ? f4[k] : make_float4(0.f, 0.f, 0.f, 0.f);
         const int a_lo = (k + 1) >> 1;
-        int c = 0;
+        int c = 0, cn = 0;
         for (int m0 = 0; m0 < mmax; m0 += TPA) {
             const int mi = m0 + sub;
             const int n = mi < a_lo ? ((k + 1) & 1) + 2 * mi : k + 2 + 2 * (mi - a_lo);
-            bool in = false;
+            bool in = false, near = false;
             int code = 0;
             if (act && n < N) {
                 const float4 pn = f4[n];
@@ -590,35 +615,35 @@ __global__ void __launch_bounds__(MAXB, 4)
                     }
                     const float r2 = d0 * (g00 * d0 + g01 * d1 + g02 * d2) + d1 * (g11 * d1 + g12 * d2) + g22 * d2 * d2;
                     in = r2 <= cut2f;
+                    near = r2 <= near2f;
                     code = ((int)m0f + 1) + 3 * ((int)m1f + 1) + 9 * ((int)m2f + 1);
                 }
             }
             const unsigned long long m = __ballot(in) & gmask;
+            const uint16_t ent = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
             if (in) {
                 const int pos = c + __popcll(m & below);
                 if (pos < cap)
-                    nbr[(size_t)k * cap + pos] = (uint16_t)(n | (code << MDT_NBR_IDX_BITS));
+                    (split ? nbr_kept : nbr)[(size_t)k * cap + pos] = ent;  // staged: the whole row to HBM
                 else
                     bad_local |= MDT_ST_NBR_OVERFLOW;
             }
             c += __popcll(m);
+            if (split) {  // staged: the near entries to LDS
+                const bool nin = in && near;
+                const unsigned long long mn = __ballot(nin) & gmask;
+                const int pos = cn + __popcll(mn & below);
+                if (nin && pos < cap) nbr[(size_t)k * cap + pos] = ent;
+                cn += __popcll(mn);
+            }
         }
         if (act && sub == 0) {
-            cnt[k] = (uint16_t)(c < cap ? c : cap);
-            if (keep) kcnt[k] = cnt[k];
+            const int cc = c < cap ? c : cap;
+            cnt[k] = (uint16_t)(split ? (cn < cap ? cn : cap) : cc);
+            if (keep) kcnt[k] = (uint16_t)cc;
         }
     }
     __syncthreads();
-    if constexpr (NBR_LDS) {
-        if (keep && !reuse) {  // a fresh build of staged lists: the rows to their copy for the next launches
-            for (int k0 = 0; k0 < N; k0 += apb) {
-                const int k = k0 + lane / TPA;
-                if (k >= N) continue;
-                const int c = cnt[k];
-                for (int e = lane % TPA; e < c; e += TPA) nbr_kept[(size_t)k * cap + e] = nbr[(size_t)k * cap + e];
-            }
-        }
-    }
     if (keep && !reuse && lane == 0) bv.rl_valid[blockIdx.x] = 1;  // read by the next launch
     for (int i = lane; i < N; i += nthr) {
         shi[i] = 0ull;

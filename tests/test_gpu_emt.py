@@ -198,7 +198,7 @@ def test_sizes_beyond_the_limits_are_refused(gpu_ctx, pes_hau):
 
 def test_tiny_atoms_far_pairs_at_half_cell_ties(gpu_ctx, pes_hau):
     """Atoms with no pair within the near radius are tiny: their pairs
-    between r_near (6.9 A) and the cut-off (7.72 A) come from the direct
+    between r_near (6.3 A) and the cut-off (7.72 A) come from the direct
     scan, whose minimum image is decided in FP32 and, within 1e-4 of a
     half-cell tie, taken from the unfolded positions by the reference's
     rounding of s = isimbox (r_j - r_i) (src/universe_mod.f90:411-428).  Two

@@ -3,7 +3,7 @@
 Size classes of more than 256 atoms (and smaller ones where the autotuner
 finds it faster) keep their owned half lists in HBM, built with the near
 radius + skin (the lists hold the pairs within the near radius r_near,
-beyond which a pair weighs less than 2^-80; the pairs beyond it of atoms
+beyond which a pair weighs less than 2^-56; the pairs beyond it of atoms
 whose near sigma is tiny come from a direct scan), while no atom of the
 replica moved more than skin / 2 since the build; a kept list gets the
 image of every entry recomputed from the current positions.  The reference visits every pair
