@@ -977,9 +977,10 @@ static int heuristic_tpa(int maxN) {
 static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap, int keep = -1, int wide = -1) {
     if (wide < 0) wide = b->wide_mode == 1 ? 1 : 0;  // untuned: 1024 lanes only when asked for
     // capacity of the owned half lists: every other atom for small replicas,
-    // else 96 (fcc within 1.5 rcut holds 134 neighbours, of which an atom
-    // owns about half; overflow is detected and reported)
-    int cap = nbcap > 0 ? nbcap : 96;
+    // else 64.  The lists hold the pairs within the near radius (+ skin):
+    // fcc Au or Pt within it + 1 A has 86 neighbours, of which an atom owns
+    // about half; overflow is detected and reported.
+    int cap = nbcap > 0 ? nbcap : 64;
     if (cap > c.nmax - 1) cap = std::max(1, c.nmax - 1);
     if (wide && tpa > 2) tpa = 2;
     const int lanes = wide ? 1024 : 512;
@@ -989,12 +990,13 @@ static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap, int k
     if (b->keep_mode >= 0) keep = b->keep_mode;
     if (keep < 0) keep = c.nmax > SKIN_MIN_ATOMS ? 1 : 0;
     const bool skin = b->list_skin > 0.0 && keep >= 1;
+    // dynamic LDS beside the kernel's static tables (4.4 KB; 4.5 KiB allowed)
+    const size_t lds_dyn = b->lds_limit > 4608 ? b->lds_limit - 4608 : 0;
     const size_t smem_lds = mdt_replica_smem(c.nmax, cap, true);
-    const bool in_lds = (!skin || keep == 1) && smem_lds <= b->lds_limit;
+    const bool in_lds = (!skin || keep == 1) && smem_lds <= lds_dyn;
     const size_t smem = in_lds ? smem_lds : mdt_replica_smem(c.nmax, cap, false);
-    if (smem > b->lds_limit)
-        return fail(MDT_EUNSUP, "replica of %d atoms needs %zu bytes of LDS (limit %zu)", c.nmax, smem,
-                    b->lds_limit);
+    if (smem > lds_dyn)
+        return fail(MDT_EUNSUP, "replica of %d atoms needs %zu bytes of LDS (limit %zu)", c.nmax, smem, lds_dyn);
     free_class_lists(c);
     if (!in_lds || skin) {
         c.nbr_stride = (long long)c.nmax * cap;

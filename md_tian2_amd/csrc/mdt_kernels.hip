@@ -41,7 +41,7 @@ struct LdsTables {
     double csig[MDT_MAXT * MDT_MAXT], etad[MDT_MAXT * MDT_MAXT], wv[MDT_MAXT * MDT_MAXT];
     double csig48[MDT_MAXT * MDT_MAXT];  // csig * 2^48: partner sigma terms in fixed-point units
     double shift[27 * 3];  // cartesian lattice translation of each image code
-    double exptab[64];
+    double exptab[MDT_EXP_TAB_N];
     int owner[MDT_MAXT], pair_on[MDT_MAXT * MDT_MAXT], algo[MDT_MAXT], is_proj[MDT_MAXT];
     // FP32 copies for the single-precision pair terms; *_l2 carry a factor
     // log2(e) so that exp(-a x) = 2^(-a_l2 x)
@@ -164,8 +164,8 @@ __device__ void load_tables(LdsTables &T, const EmtConst *__restrict__ ec, const
         T.shift[3 * c + 1] = cell.h[1] * m0 + cell.h[4] * m1 + cell.h[7] * m2;
         T.shift[3 * c + 2] = cell.h[2] * m0 + cell.h[5] * m1 + cell.h[8] * m2;
     }
-    const double et[64] = MDT_EXP_TAB_INIT;
-    for (int k = lane; k < 64; k += nthr) T.exptab[k] = et[k];
+    const double et[MDT_EXP_TAB_N] = MDT_EXP_TAB_INIT;
+    for (int k = lane; k < MDT_EXP_TAB_N; k += nthr) T.exptab[k] = et[k];
 }
 
 }  // namespace

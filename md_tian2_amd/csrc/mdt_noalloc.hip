@@ -167,10 +167,11 @@ int mdt_noalloc_compute(void *ptr, size_t lds_limit, const EmtConst *d_ec, int n
     int cap = NOALLOC_NBCAP < N - 1 ? NOALLOC_NBCAP : (N > 1 ? N - 1 : 1);
     int tpa = 8;
     while (tpa > 1 && N * tpa > 512) tpa >>= 1;
+    const size_t lds_dyn = lds_limit > 4608 ? lds_limit - 4608 : 0;  // beside k_replica's static tables
     const size_t smem_lds = mdt_replica_smem(N, cap, true);
-    const bool lds = smem_lds <= lds_limit;
+    const bool lds = smem_lds <= lds_dyn;
     const size_t smem = lds ? smem_lds : mdt_replica_smem(N, cap, false);
-    if (smem > lds_limit) return mdt_noalloc_fail(MDT_EUNSUP, "universe too large for one workgroup's LDS");
+    if (smem > lds_dyn) return mdt_noalloc_fail(MDT_EUNSUP, "universe too large for one workgroup's LDS");
     const int block = (((N * tpa + 63) / 64) * 64) < 64 ? 64 : (((N * tpa + 63) / 64) * 64);
     const long long n3 = 3LL * P * N;
     const int grid = (int)((n3 + 255) / 256 < 1024 ? (n3 + 255) / 256 : 1024);

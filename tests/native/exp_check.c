@@ -13,7 +13,7 @@ static double ulp_err(double a, double b) {
 }
 
 int main(void) {
-    static const double tab[64] = MDT_EXP_TAB_INIT;
+    static const double tab[MDT_EXP_TAB_N] = MDT_EXP_TAB_INIT;
     double worst = 0.0, wx = 0.0;
     for (long i = 0; i <= 4000000; ++i) {
         const double x = -700.0 + 1400.0 * (double)i / 4000000.0;
