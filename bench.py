@@ -302,7 +302,7 @@ def nn_variant(ctx, pes, args):
     Langevin at 300 K, H Verlet, dt 0.1 fs; EMT fallback on extrapolation
     and the EMT judge on a rotating 4 % of the replicas every 4 steps (1 % of
     frames).  The radial-only network (md_tian2_amd/nn/hau_radial: 16 radial
-    functions per element, 5.0 A) is the surrogate; the radial + angular
+    functions per element, 4.5 A) is the surrogate; the radial + angular
     network (md_tian2_amd/nn/hau: 24 radial + 16 angular, 6.5 A) is timed
     beside it on fewer steps."""
     from md_tian2_amd import surrogate
@@ -314,7 +314,7 @@ def nn_variant(ctx, pes, args):
     systems = make_workload(args.nn_traj, seed=4242)
     out = _nn_run(ctx, model, systems, args.nn_steps, args.warmup)
     out.update({
-        "workload": "NENE surrogate H/Au(111) 6x6x4 + H (16 radial symmetry functions per element, 5.0 A, "
+        "workload": "NENE surrogate H/Au(111) 6x6x4 + H (16 radial symmetry functions per element, 4.5 A, "
                     "16-32-32-1 tanh), slab Langevin 300 K, H Verlet, dt 0.1 fs, EMT fallback on extrapolation, "
                     "EMT judge on a rotating 4 % of replicas every 4 steps (1 % of frames)",
         "precision": "FP32 symmetry functions, BF16 hi/lo-split MFMA network (FP32 accumulate), FP64 integration",

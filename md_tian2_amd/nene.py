@@ -281,7 +281,7 @@ def hau_design(rc_ang=6.5, hidden=(32, 32)):
     return m
 
 
-def hau_radial_design(rc=5.0, hidden=(32, 32), nh=16, nau=12, nauh=4):
+def hau_radial_design(rc=4.5, hidden=(32, 32), nh=16, nau=12, nauh=4):
     """A radial-only H/Au surrogate: nh radial symmetry functions on H and
     nau + nauh on Au, cut-off rc (A), network n-32-32-1 tanh/tanh/linear.
 
@@ -297,8 +297,8 @@ def hau_radial_design(rc=5.0, hidden=(32, 32), nh=16, nau=12, nauh=4):
         rest shifted Gaussians over 2 to 11.5 rc / 6.5 Bohr;
     Au: nau on Au, a quarter centred (eta 0.002-0.2), the rest shifted over
         4 to 11.5 rc / 6.5 Bohr; nauh centred on H (eta 0.003-1).
-    The defaults (16, 12 + 4, 5.0 A) are md_tian2_amd/nn/hau_radial: on
-    the 4- and 6-layer validation frames 0.05 meV/atom and 3.7 meV/A; 32 and
+    The defaults (16, 12 + 4, 4.5 A) are md_tian2_amd/nn/hau_radial: on
+    the 4- and 6-layer validation frames 0.44 meV/atom and 3.4 meV/A; 32 and
     24 + 8 functions at 6.5 A reach 1.0 meV/atom and 3.3 meV/A at 1.4x the
     cost (GPU training of each in about a minute).
     """

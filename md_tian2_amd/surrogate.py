@@ -21,7 +21,7 @@ so that the H visits the whole approach / bounce / penetration range.
 
 Energies and forces are the GPU EMT path's (FP64).  The network is
 nene.hau_design (24 radial + 16 angular symmetry functions per element,
-6.5 A cut-off, 40-32-32-1) or nene.hau_radial_design (16 radial, 5.0 A,
+6.5 A cut-off, 40-32-32-1) or nene.hau_radial_design (16 radial, 4.5 A,
 16-32-32-1), trained by the library's own GPU kernels (BF16
 MFMA, Adam) on energy + force loss.
 """
