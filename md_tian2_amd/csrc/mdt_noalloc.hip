@@ -167,7 +167,7 @@ int mdt_noalloc_compute(void *ptr, size_t lds_limit, const EmtConst *d_ec, int n
     int cap = NOALLOC_NBCAP < N - 1 ? NOALLOC_NBCAP : (N > 1 ? N - 1 : 1);
     int tpa = 8;
     while (tpa > 1 && N * tpa > 512) tpa >>= 1;
-    const size_t lds_dyn = lds_limit > 5120 ? lds_limit - 5120 : 0;  // beside k_replica's static tables (<= 4.7 KB)
+    const size_t lds_dyn = lds_limit > 5632 ? lds_limit - 5632 : 0;  // beside k_replica's static tables (<= 4.7 KB)
     const size_t smem_lds = mdt_replica_smem(N, cap, true);
     const bool lds = smem_lds <= lds_dyn;
     const size_t smem = lds ? smem_lds : mdt_replica_smem(N, cap, false);
