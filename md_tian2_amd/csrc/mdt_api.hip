@@ -44,7 +44,7 @@ hipError_t mdt_launch_nn_adam(NnConst *M, const float *partials, int W, float *g
 hipError_t mdt_launch_nene(int nrep, size_t smem, const NnConst *M, const BatchView &bv, float *G, float *C,
                            int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, int radial,
                            uint16_t *nbr, uint8_t *nbr_cnt, uint32_t *nbr_ref, int *nbr_key, double skin_bohr,
-                           hipStream_t st);
+                           hipStream_t st, int images);
 
 namespace {
 
@@ -130,6 +130,9 @@ struct SizeClass {
     int tpa = 0, nbcap = 0, block = 0;
     int keep = 0;              // lists kept across launches (Verlet skin): 1 staged in LDS, 2 read from HBM
     int stage = 0;             // kept lists staged in LDS during a launch
+    std::vector<int> h_mem;    // its replicas (host copy)
+    int nn_img = -1;           // radial network: some cell of the class needs the image search (-1: not known)
+    double nn_img_rc = 0.0;    // the cut-off (A) nn_img was decided for
     int wide = 0;              // 1: workgroups of up to 1024 lanes (k_replica's 1024-lane instantiation)
     size_t smem = 0;
     uint16_t *d_nbr = nullptr; // neighbour lists in HBM when kept or when they do not fit in LDS
@@ -213,6 +216,7 @@ struct mdt_batch {
     // those trajectories run k_mixed (one workgroup each), the rest k_replica
     int bead_mode = 0;
     std::vector<uint8_t> h_type;    // atom types (0-based), host copy for validation
+    std::vector<double> h_icell;    // inverse cells (9 per trajectory), host copy
     std::vector<int> mix_traj;      // trajectories on k_mixed, by atom count
     // their size classes: a run of mix_traj (offset, count) and its LDS
     struct MixClass {
@@ -612,6 +616,7 @@ static int build_classes(mdt_batch *b) {
         SizeClass c;
         c.nmax = nmax[k];
         c.nrep = (int)mem[k].size();
+        c.h_mem = mem[k];
         if (lists) {
             int rc = dalloc(&c.d_reps, mem[k].size());
             if (!rc && hipMemcpy(c.d_reps, mem[k].data(), sizeof(int) * mem[k].size(), hipMemcpyHostToDevice) !=
@@ -767,12 +772,17 @@ int mdt_batch_set_atoms(mdt_batch *b, const int32_t *idx, const double *mass, co
     return 0;
 }
 
+static void drop_graph(mdt_batch *b);
+
 int mdt_batch_set_cells(mdt_batch *b, const double *simbox, const double *isimbox) {
     if (!b || !simbox || !isimbox) return fail(MDT_EINVAL, "mdt_batch_set_cells: NULL argument");
     hipStream_t st = b->ctx->stream;
     HIPCHK(hipMemcpyAsync(b->d_cell, simbox, sizeof(double) * 9 * b->ntraj, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(b->d_icell, isimbox, sizeof(double) * 9 * b->ntraj, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
+    b->h_icell.assign(isimbox, isimbox + 9 * (size_t)b->ntraj);
+    for (SizeClass &k : b->cls) k.nn_img = -1;
+    drop_graph(b);  // captured network launches chose their kernels for the old cells
     b->have_cells = true;
     return invalidate_lists(b);  // lists hold fractional positions of the old cells
 }
@@ -1282,6 +1292,26 @@ static int ensure_nn(mdt_batch *b) {
     return 0;
 }
 
+// The radial network's kernels compile the image search in only for size
+// classes with a cell narrower than about 2 rc (where an atom can meet
+// another in more than one image): the kernels decide per replica with
+// w = rc |row of H^-1| < 0.4999; the host takes 0.499 as the bound.
+static int nn_images(mdt_batch *b, SizeClass &k) {
+    const mdt_context *c = b->ctx;
+    if (!nn_radial(c)) return 1;
+    const double rc = (double)c->nn->rc / (double)c->nn->ang2bohr;
+    if (k.nn_img >= 0 && k.nn_img_rc == rc) return k.nn_img;
+    int img = b->h_icell.size() == 9 * (size_t)b->ntraj ? 0 : 1;
+    for (int r = 0; r < (int)k.h_mem.size() && !img; ++r) {
+        const double *ih = b->h_icell.data() + 9 * (size_t)b->rep_t[k.h_mem[r]];
+        for (int q = 0; q < 3; ++q)
+            if (rc * std::sqrt(ih[q] * ih[q] + ih[q + 3] * ih[q + 3] + ih[q + 6] * ih[q + 6]) >= 0.499) img = 1;
+    }
+    k.nn_img = img;
+    k.nn_img_rc = rc;
+    return img;
+}
+
 // calc_force's PES dispatch (src/force.f90:77-82) for every replica: EMT, or
 // the network followed by EMT on the replicas whose symmetry functions left
 // the training range (when EMT parameters are set).
@@ -1293,14 +1323,14 @@ static int launch_pes(mdt_batch *b, const BatchView &bv, int add_fin, hipStream_
         Fork fk;
         int rc = fork_open(fk, c, st, (int)b->cls.size());
         if (rc) return rc;
-        for (const SizeClass &k : b->cls) {
+        for (SizeClass &k : b->cls) {
             BatchView v = bv;
             v.rep_list = k.d_reps;
             hipStream_t s2 = st;
             if ((rc = fork_stream(fk, &s2))) return rc;
             HIPCHK(mdt_launch_nene(k.nrep, nn_smem(c, k.nmax), c->d_nn, v, b->d_nnG, b->d_nnC, b->d_nnflag,
                                    b->d_nnF, b->maxN, add_fin, 1, nn_radial(c), b->d_nnNbr, b->d_nnCnt, b->d_nnRef,
-                                   b->d_nnKey, b->list_skin / MDT_BOHR2ANG, s2));
+                                   b->d_nnKey, b->list_skin / MDT_BOHR2ANG, s2, nn_images(b, k)));
         }
         if ((rc = fork_close(fk))) return rc;
         if (c->have_params) {
@@ -2271,7 +2301,7 @@ int mdt_compute_nene(mdt_context *ctx, int natoms, int nbeads, const int32_t *id
     HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(int) * b->ntraj, ctx->stream));
     HIPCHK(mdt_launch_nene(b->nrep, nn_smem(ctx, b->maxN), ctx->d_nn, b->view(), b->d_nnG, b->d_nnC, b->d_nnflag,
                            b->d_nnF, b->maxN, 0, 0, nn_radial(ctx), b->d_nnNbr, b->d_nnCnt, b->d_nnRef, b->d_nnKey,
-                           b->list_skin / MDT_BOHR2ANG, ctx->stream));
+                           b->list_skin / MDT_BOHR2ANG, ctx->stream, 1));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     std::vector<int> st(1);
     HIPCHK(hipMemcpy(st.data(), b->d_status, sizeof(int), hipMemcpyDeviceToHost));

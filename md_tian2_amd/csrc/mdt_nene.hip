@@ -1209,17 +1209,23 @@ __global__ void __launch_bounds__(NN_BLOCK) k_nene_f(const NnConst *__restrict__
 //  * Sums over a centre's 16 lanes use DPP row permutations (no LDS).
 // ===========================================================================
 
-struct RadNbrG {  // pass G entries
+#define NN_WF 32  // pass F entries per centre and window (a longer list takes several windows)
+
+// entries of one centre's list staged at a time: cnt those of this window,
+// tot the centre's whole list
+struct RadNbrG {  // pass G entries (the whole list)
+    static constexpr int W = NN_CAP;
     float r[NN_CA * NN_CAP];
     uint8_t el[NN_CA * NN_CAP];
-    int cnt[NN_CA];
+    int cnt[NN_CA], tot[NN_CA];
 };
 
-struct RadNbrF {  // pass F entries
-    float r[NN_CA * NN_CAP], ux[NN_CA * NN_CAP], uy[NN_CA * NN_CAP], uz[NN_CA * NN_CAP];
-    uint16_t j[NN_CA * NN_CAP];
-    uint8_t el[NN_CA * NN_CAP];
-    int cnt[NN_CA];
+struct RadNbrF {  // pass F entries (a window of NN_WF)
+    static constexpr int W = NN_WF;
+    float r[NN_CA * NN_WF], ux[NN_CA * NN_WF], uy[NN_CA * NN_WF], uz[NN_CA * NN_WF];
+    uint16_t j[NN_CA * NN_WF];
+    uint8_t el[NN_CA * NN_WF];
+    int cnt[NN_CA], tot[NN_CA];
 };
 
 __device__ __forceinline__ void rad_put(RadNbrG &L, int e, int n, float ex, float ey, float ez, float r2,
@@ -1284,7 +1290,8 @@ __device__ __forceinline__ void min_image_d(const float hs[9], const uint32_t *u
 template <bool TRI, class Lt>
 __device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz,
                                    const uint8_t *elem, const float hs[9], float rb2, int N, int i, int lane,
-                                   uint16_t *nrow, uint8_t *ncnt, int *bad) {
+                                   uint16_t *nrow, uint8_t *ncnt, int *bad, int w0) {
+    constexpr int W = Lt::W;
     const int c = lane >> 4, sub = lane & 15, sh = lane & 48;
     const bool act = i >= 0;
     const uint32_t xi = ux[act ? i : 0], yi = uy[act ? i : 0], zi = uz[act ? i : 0];
@@ -1301,7 +1308,7 @@ __device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy
         const unsigned int row = (unsigned int)(__ballot(hit) >> sh) & 0xffffu;
         const int pos = cnt + __popc(row & ((1u << sub) - 1u));
         if (hit && pos < NN_CAP) {
-            rad_put(L, c * NN_CAP + pos, n, ex, ey, ez, r2, elem);
+            if (pos >= w0 && pos < w0 + W) rad_put(L, c * W + pos - w0, n, ex, ey, ez, r2, elem);
             if (nrow) nrow[pos] = (uint16_t)n;
         }
         cnt += __popc(row);
@@ -1316,16 +1323,18 @@ __device__ int rad_build_min_image(Lt &L, const uint32_t *ux, const uint32_t *uy
 // with the arithmetic of rad_build_min_image (bitwise the same vectors).
 template <bool TRI, class Lt>
 __device__ int rad_load(Lt &L, const uint32_t *ux, const uint32_t *uy, const uint32_t *uz, const uint8_t *elem,
-                        const float hs[9], int i, int lane, const uint16_t *nrow, const uint8_t *ncnt) {
+                        const float hs[9], int i, int lane, const uint16_t *nrow, const uint8_t *ncnt, int w0) {
+    constexpr int W = Lt::W;
     const int c = lane >> 4, sub = lane & 15;
     const int cnt = i >= 0 ? (int)ncnt[i] : 0;
     if (i >= 0) {
         const uint32_t xi = ux[i], yi = uy[i], zi = uz[i];
-        for (int p = sub; p < cnt; p += NN_TPA) {
+        const int pe = cnt < w0 + W ? cnt : w0 + W;
+        for (int p = w0 + sub; p < pe; p += NN_TPA) {
             const int n = nrow[p];
             float ex, ey, ez;
             min_image_d<TRI>(hs, ux, uy, uz, n, xi, yi, zi, &ex, &ey, &ez);
-            rad_put(L, c * NN_CAP + p, n, ex, ey, ez, ex * ex + ey * ey + ez * ez, elem);
+            rad_put(L, c * W + p - w0, n, ex, ey, ez, ex * ex + ey * ey + ez * ez, elem);
         }
     }
     return cnt;
@@ -1367,11 +1376,15 @@ __device__ void rad_cell(RadCell &C, const NnConst &M, const double *cellA, cons
 
 // The chunk's lists: with stored lists (C.store) read them (mode 1) or
 // build them within rc + skin and store them (mode 0); otherwise the
-// minimum-image pass or the image search within rc.
-template <class Lt>
+// minimum-image pass or the image search within rc.  IMG false: the image
+// search is not compiled in (its FP64 registers are then not reserved); a
+// cell that needs it sets the overflow flag and gets no entries, which the
+// host rules out by launching IMG for every size class with such a cell.
+template <class Lt, bool IMG = true>
 __device__ void rad_build(Lt &L, const RadCell &C, const NnView &nv, int rep, int mode, const uint32_t *ux,
                           const uint32_t *uy, const uint32_t *uz, const uint8_t *elem, float rc2, float rb2, int N,
-                          int i, int lane, int *bad) {
+                          int i, int lane, int *bad, int w0 = 0) {
+    constexpr int W = Lt::W;
     const int c = lane >> 4;
     int cnt;
     if (__builtin_amdgcn_readfirstlane(C.store)) {
@@ -1382,20 +1395,20 @@ __device__ void rad_build(Lt &L, const RadCell &C, const NnView &nv, int rep, in
         uint8_t *ncnt = nv.nbr_cnt + (size_t)rep * nv.maxN;
         const bool tri = __builtin_amdgcn_readfirstlane(C.tri);
         if (mode == 1)
-            cnt = tri ? rad_load<true>(L, ux, uy, uz, elem, hs, i, lane, nrow, ncnt)
-                      : rad_load<false>(L, ux, uy, uz, elem, hs, i, lane, nrow, ncnt);
+            cnt = tri ? rad_load<true>(L, ux, uy, uz, elem, hs, i, lane, nrow, ncnt, w0)
+                      : rad_load<false>(L, ux, uy, uz, elem, hs, i, lane, nrow, ncnt, w0);
         else
-            cnt = tri ? rad_build_min_image<true>(L, ux, uy, uz, elem, hs, rb2, N, i, lane, nrow, ncnt, bad)
-                      : rad_build_min_image<false>(L, ux, uy, uz, elem, hs, rb2, N, i, lane, nrow, ncnt, bad);
+            cnt = tri ? rad_build_min_image<true>(L, ux, uy, uz, elem, hs, rb2, N, i, lane, nrow, ncnt, bad, w0)
+                      : rad_build_min_image<false>(L, ux, uy, uz, elem, hs, rb2, N, i, lane, nrow, ncnt, bad, w0);
     } else if (__builtin_amdgcn_readfirstlane(C.min_image)) {
         float hs[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) hs[k] = C.hs[k];
         if (__builtin_amdgcn_readfirstlane(C.tri))
-            cnt = rad_build_min_image<true>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, nullptr, nullptr, bad);
+            cnt = rad_build_min_image<true>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, nullptr, nullptr, bad, w0);
         else
-            cnt = rad_build_min_image<false>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, nullptr, nullptr, bad);
-    } else {
+            cnt = rad_build_min_image<false>(L, ux, uy, uz, elem, hs, rc2, N, i, lane, nullptr, nullptr, bad, w0);
+    } else if constexpr (IMG) {
         double h[9];
         int imax[3];
 #pragma unroll
@@ -1404,11 +1417,20 @@ __device__ void rad_build(Lt &L, const RadCell &C, const NnView &nv, int rep, in
         for (int k = 0; k < 3; ++k) imax[k] = C.imax[k];
         cnt = build_images(rc2, ux, uy, uz, h, imax, C.w[0], C.w[1], C.w[2], N, i, lane, bad,
                            [&](int pos, int n, double ex, double ey, double ez, double r2) {
-                               rad_put(L, c * NN_CAP + pos, n, (float)ex, (float)ey, (float)ez, (float)r2, elem);
+                               if (pos >= w0 && pos < w0 + W)
+                                   rad_put(L, c * W + pos - w0, n, (float)ex, (float)ey, (float)ez, (float)r2,
+                                           elem);
                            });
         cnt = cnt < NN_CAP ? cnt : NN_CAP;
+    } else {
+        *bad |= MDT_ST_NBR_OVERFLOW;
+        cnt = 0;
     }
-    if ((lane & 15) == 0) L.cnt[c] = cnt;
+    if ((lane & 15) == 0) {
+        const int wc = cnt - w0;
+        L.cnt[c] = wc < 0 ? 0 : (wc < W ? wc : W);  // this window's entries
+        L.tot[c] = cnt;
+    }
 }
 
 // the largest list count of the wavefront's 4 centres (uniform)
@@ -1440,13 +1462,14 @@ __device__ int make_slots(const uint16_t *elist, const int *s_ne, int ne_model, 
 // largest number of entries per lane (uniform), *present: bit x set when
 // some entry of the wavefront has element x (uniform).
 template <class Lt>
-__device__ __forceinline__ void rad_entries(const Lt &L, int lane, float rc, int nel, float rr[NN_RQ],
-                                            int eq[NN_RQ], int *nq, unsigned int *present) {
+__device__ __forceinline__ void rad_entries(const Lt &L, int lane, float rc, int nel, float rr[Lt::W / NN_TPA],
+                                            int eq[Lt::W / NN_TPA], int *nq, unsigned int *present) {
+    constexpr int RQ = Lt::W / NN_TPA;
     const int c = lane >> 4, sub = lane & 15;
-    const int cnt = L.cnt[c], base = c * NN_CAP;
+    const int cnt = L.cnt[c], base = c * Lt::W;
     *nq = (wave_max_cnt(cnt) + NN_TPA - 1) / NN_TPA;
 #pragma unroll
-    for (int q = 0; q < NN_RQ; ++q) {
+    for (int q = 0; q < RQ; ++q) {
         const int p = sub + NN_TPA * q;
         const float r = p < cnt ? L.r[base + p] : rc;
         const bool v = p < cnt && r <= rc;  // stored lists reach rc + skin
@@ -1457,7 +1480,7 @@ __device__ __forceinline__ void rad_entries(const Lt &L, int lane, float rc, int
     for (int x = 0; x < nel; ++x) {
         bool any = false;
 #pragma unroll
-        for (int q = 0; q < NN_RQ; ++q) any |= eq[q] == x;
+        for (int q = 0; q < RQ; ++q) any |= eq[q] == x;
         if (__ballot(any)) pr |= 1u << x;
     }
     *present = pr;
@@ -1573,15 +1596,16 @@ __device__ __forceinline__ void rf_class(const NnConst &M, int e, int lo, int hi
 __device__ void rf_chunk(const NnConst &M, const RadNbrF &L, const float *crow, int e, int i, int lane,
                          unsigned long long *ax, unsigned long long *ay, unsigned long long *az, int *badp) {
     const int c = lane >> 4, sub = lane & 15;
-    const int base = c * NN_CAP;
+    constexpr int RQ = RadNbrF::W / NN_TPA;  // entries per lane of a window (2)
+    const int base = c * RadNbrF::W;
     const int nel = M.nelem;
     const float pi_rc = M.pi_rc;
-    float rr[NN_RQ], fcq[NN_RQ], dfcq[NN_RQ], sq[NN_RQ];
-    int eq[NN_RQ], nq;
+    float rr[RQ], fcq[RQ], dfcq[RQ], sq[RQ];
+    int eq[RQ], nq;
     unsigned int present;
     rad_entries(L, lane, M.rc, nel, rr, eq, &nq, &present);
 #pragma unroll
-    for (int q = 0; q < NN_RQ; ++q) {
+    for (int q = 0; q < RQ; ++q) {
         float fc, dfc;
         cutoff(pi_rc, rr[q], &fc, &dfc);
         fcq[q] = eq[q] >= 0 ? fc : 0.f;
@@ -1591,26 +1615,22 @@ __device__ void rf_chunk(const NnConst &M, const RadNbrF &L, const float *crow, 
     for (int x = 0; x < nel; ++x) {
         const int lo = M.rk_lo[e][x], hi = M.rk_hi[e][x];
         if (lo == hi || !((present >> x) & 1u)) continue;
-        float fx[NN_RQ], dx[NN_RQ];
+        float fx[RQ], dx[RQ];
 #pragma unroll
-        for (int q = 0; q < NN_RQ; ++q) {
+        for (int q = 0; q < RQ; ++q) {
             fx[q] = eq[q] == x ? fcq[q] : 0.f;
             dx[q] = eq[q] == x ? dfcq[q] : 0.f;
         }
         switch (nq) {
             case 1: rf_class<1>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
             case 2: rf_class<2>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
-            case 3: rf_class<3>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
-            case 4: rf_class<4>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
-            case 5: rf_class<5>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
-            case 6: rf_class<6>(M, e, lo, hi, crow, rr, fx, dx, sq); break;
             default: break;
         }
     }
     float F0 = 0.f, F1 = 0.f, F2 = 0.f;
     int bad = 0;
 #pragma unroll
-    for (int q = 0; q < NN_RQ; ++q) {
+    for (int q = 0; q < RQ; ++q) {
         if (q < nq && eq[q] >= 0) {
             const int e2 = base + sub + NN_TPA * q;
             const float v0 = sq[q] * L.ux[e2], v1 = sq[q] * L.uy[e2], v2 = sq[q] * L.uz[e2];
@@ -1821,6 +1841,7 @@ __device__ void network_energy_w(const NnConst &M, const int *s_ne, const uint16
 }
 
 // Pass G and the network of a radial-only model (k_nene_g's contract).
+template <bool IMG>
 __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rg(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
                                                       int skip_extrap) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1906,7 +1927,7 @@ __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rg(const NnConst *__restri
         const uint32_t sv = sl < nslot ? slot[sl] : 0xffffu;
         const int i = (sv & 0xffffu) == 0xffffu ? -1 : (int)(sv & 0xffffu);
         const int e = __builtin_amdgcn_readfirstlane((int)(sv >> 16));
-        rad_build(L, Cl, nv, rep, mode, ux, uy, uz, elem, M.rc2, rb * rb, N, i, lane, &bad);
+        rad_build<RadNbrG, IMG>(L, Cl, nv, rep, mode, ux, uy, uz, elem, M.rc2, rb * rb, N, i, lane, &bad);
         __syncthreads();
         rg_chunk(M, L, e, i, lane, Grep, &extrap);
         __syncthreads();
@@ -1937,6 +1958,7 @@ __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rg(const NnConst *__restri
 }
 
 // Pass F and the outputs of a radial-only model (k_nene_f's contract).
+template <bool IMG>
 __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rf(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
                                                       int add_fin, int skip_extrap) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1987,7 +2009,8 @@ __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rf(const NnConst *__restri
     const int nslot = make_slots(elist, s_ne, ne_model, slot, lane);
     __syncthreads();
 
-    // ---- pass F: forces ----
+    // ---- pass F: forces, the chunk's lists in windows of NN_WF entries per
+    // centre (one window unless some centre of the chunk has more) ----
     for (int k0 = 0; k0 < nslot; k0 += NN_CA) {
         const int sl = k0 + (lane >> 4);
         const uint32_t sv = sl < nslot ? slot[sl] : 0xffffu;
@@ -1995,17 +2018,26 @@ __global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rf(const NnConst *__restri
         const int e = __builtin_amdgcn_readfirstlane((int)(sv >> 16));
         // the lists k_nene_rg stored or kept for these positions
         const float rb = M.rc + nv.skin;
-        rad_build(L, Cl, nv, rep, 1, ux, uy, uz, elem, M.rc2, rb * rb, N, i, lane, &bad);
-        for (int q = lane; q < NN_CA * NN_K1; q += NN_BLOCK) {
-            const int ci = q / NN_K1, k = q % NN_K1;
-            const int s2 = k0 + ci;
-            const uint32_t v2 = s2 < nslot ? slot[s2] : 0xffffu;
-            const int ii = (int)(v2 & 0xffffu);
-            cb[ci][k] = (ii != 0xffff && k < M.nr[v2 >> 16]) ? Crep[(size_t)ii * NN_K1 + k] : 0.f;
+        int nwin = 1;  // set after the first window's barrier, the same in every lane
+        for (int w = 0; w < nwin; ++w) {
+            rad_build<RadNbrF, IMG>(L, Cl, nv, rep, 1, ux, uy, uz, elem, M.rc2, rb * rb, N, i, lane, &bad, w * NN_WF);
+            if (w == 0)
+                for (int q = lane; q < NN_CA * NN_K1; q += NN_BLOCK) {
+                    const int ci = q / NN_K1, k = q % NN_K1;
+                    const int s2 = k0 + ci;
+                    const uint32_t v2 = s2 < nslot ? slot[s2] : 0xffffu;
+                    const int ii = (int)(v2 & 0xffffu);
+                    cb[ci][k] = (ii != 0xffff && k < M.nr[v2 >> 16]) ? Crep[(size_t)ii * NN_K1 + k] : 0.f;
+                }
+            __syncthreads();
+            if (w == 0)
+                for (int c2 = 0; c2 < NN_CA; ++c2) {
+                    const int nw = (L.tot[c2] + NN_WF - 1) / NN_WF;
+                    nwin = nw > nwin ? nw : nwin;
+                }
+            rf_chunk(M, L, cb[lane >> 4], e, i, lane, ax, ay, az, &bad);
+            __syncthreads();
         }
-        __syncthreads();
-        rf_chunk(M, L, cb[lane >> 4], e, i, lane, ax, ay, az, &bad);
-        __syncthreads();
     }
     write_forces(bv, nv, ax, ay, az, N, roff, doff, aoff, add_fin, lane);
     if (bad) atomicOr(&bv.status[t], bad);
@@ -2504,9 +2536,9 @@ size_t mdt_nene_radial_smem(int N) { return (size_t)N * (3 * 8 + 3 * 4 + 4 + 2 +
 // static LDS: the larger of the two kernels'
 hipError_t mdt_nene_attributes(hipFuncAttributes *fa, int radial) {
     hipFuncAttributes g, f;
-    hipError_t e = hipFuncGetAttributes(&g, radial ? (const void *)k_nene_rg : (const void *)k_nene_g);
+    hipError_t e = hipFuncGetAttributes(&g, radial ? (const void *)k_nene_rg<true> : (const void *)k_nene_g);
     if (e != hipSuccess) return e;
-    e = hipFuncGetAttributes(&f, radial ? (const void *)k_nene_rf : (const void *)k_nene_f);
+    e = hipFuncGetAttributes(&f, radial ? (const void *)k_nene_rf<true> : (const void *)k_nene_f);
     if (e != hipSuccess) return e;
     *fa = g.sharedSizeBytes >= f.sharedSizeBytes ? g : f;
     return hipSuccess;
@@ -2515,16 +2547,18 @@ hipError_t mdt_nene_attributes(hipFuncAttributes *fa, int radial) {
 hipError_t mdt_launch_nene(int nrep, size_t smem, const NnConst *M, const BatchView &bv, float *G, float *C,
                            int *rep_flag, double *fpes, int maxN, int add_fin, int skip_extrap, int radial,
                            uint16_t *nbr, uint8_t *nbr_cnt, uint32_t *nbr_ref, int *nbr_key, double skin_bohr,
-                           hipStream_t st) {
+                           hipStream_t st, int images) {
     NnView nv{G, C, rep_flag, fpes, maxN, nbr, nbr_cnt, nbr_ref, nbr_key, (float)skin_bohr};
-    if (radial)
-        hipLaunchKernelGGL(k_nene_rg, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, skip_extrap);
+    if (radial)  // the image search costs k_nene_rg no occupancy: one instantiation
+        hipLaunchKernelGGL(k_nene_rg<true>, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, skip_extrap);
     else
         hipLaunchKernelGGL(k_nene_g, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, skip_extrap);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (radial)
-        hipLaunchKernelGGL(k_nene_rf, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, add_fin, skip_extrap);
+    if (radial && images)
+        hipLaunchKernelGGL(k_nene_rf<true>, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, add_fin, skip_extrap);
+    else if (radial)
+        hipLaunchKernelGGL(k_nene_rf<false>, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, add_fin, skip_extrap);
     else
         hipLaunchKernelGGL(k_nene_f, dim3(nrep), dim3(NN_BLOCK), smem, st, M, bv, nv, add_fin, skip_extrap);
     return hipGetLastError();
