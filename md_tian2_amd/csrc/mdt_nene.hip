@@ -1842,7 +1842,7 @@ __device__ void network_energy_w(const NnConst &M, const int *s_ne, const uint16
 
 // Pass G and the network of a radial-only model (k_nene_g's contract).
 template <bool IMG>
-__global__ void __launch_bounds__(NN_BLOCK, 4) k_nene_rg(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
+__global__ void __launch_bounds__(NN_BLOCK, 5) k_nene_rg(const NnConst *__restrict__ Mg, BatchView bv, NnView nv,
                                                       int skip_extrap) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ union {
